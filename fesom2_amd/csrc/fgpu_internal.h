@@ -1,0 +1,142 @@
+// Internal declarations shared by the translation units of libfesom_gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fesom_gpu.h"
+
+#define FGPU_R_EARTH 6367500.0     // oce_modules.F90:29
+#define FGPU_EPS_AB 0.1            // oce_modules.F90:105
+#define FGPU_VCPW 4.2e6            // oce_modules.F90:31
+#define FGPU_MAX_NL 128            // columns deeper than this are rejected
+
+// tracer scheme codes
+enum { HOR_MFCT = 0, HOR_UPW1 = 1 };
+enum { VER_QR4C = 0, VER_UPW1 = 1 };
+enum { LIM_FCT = 0, LIM_NON = 1 };
+
+// Mesh as seen by the kernels: 0-based connectivity, level numbers 1-based.
+struct DevMesh {
+    int N2, F2, E2, L, NL, kmax;
+    const int *elem_nodes;   // F2*3
+    const int *edges;        // E2*2
+    const int *edge_tri;     // E2*2, -1 = none
+    const int *einfo;        // E2*4: nl1, nu1, nl2, nu2 (reference names, oce_adv_tra_hor.F90:104-128)
+    const int *erange;       // E2*2: nu12, nl12 of the max/min rule (oce_adv_tra_driver.F90:154-156)
+    const int *updn;         // E2*2 up/downwind triangles, -1 = none
+    const int *nie;          // N2*kmax nod_in_elem2D
+    const int *nie_num;      // N2
+    const int *nlev, *ulev;  // F2
+    const int *nlevn, *ulevn, *nlevn_min, *ulevn_max;  // N2
+    const int *ne_ptr;       // N2+1 node->edge gather lists
+    const int *ne_idx;       // 2*e + side (side 0: node is edges(1,e)), edges ascending
+    const double *elem_area, *elem_cos, *edge_dxdy, *ecd, *gsca;
+    const double *area, *areasvol;        // N2*NL
+    const double *hnode, *hnode_new;      // N2*L
+    const double *helem;                  // F2*L
+    const double *zbar3d;                 // N2*NL
+    const double *Z3d;                    // N2*L
+    const double *zbar_n_bot;             // N2
+    const double *zbar_e_bot, *zbar_e_srf;  // F2
+};
+
+struct DevDyn {
+    const double *uv, *w, *w_e, *w_i, *Kv;
+    const double *heat_flux, *water_flux, *virtual_salt, *relax_salt, *real_salt_flux;
+    double is_nonlinfs;
+};
+
+// Per-tracer state, tracer-major: tracer t occupies [t*stride, (t+1)*stride).
+struct DevTracers {
+    int ntr;
+    const int *id, *abo, *hor, *ver, *lim, *ivd;   // ntr each
+    const double *ph, *pv;                         // ntr each
+    size_t sN, sNI, sE, sF2, sOld;                 // strides: N2*L, N2*NL, E2*L, F2*L*2, N2*L*2
+    double *T, *Told, *TAB;
+    double *tr_xy;            // F2*L*2
+    double *lo_h, *a_h;       // E2*L
+    double *fct_lo;           // N2*L
+    double *a_v;              // N2*NL
+    double *aux;              // F2*L*2
+    double *rp, *rm;          // N2*L
+    double *dttf_h, *dttf_v;  // N2*L (per-tracer entry points only; one tracer's worth)
+    double *dbg_av;           // N2*NL limited vertical flux (per-tracer entry points only)
+    double *dbg_grad;         // E2*L*4 materialised edge_up_dn_grad (per-tracer entry points only)
+};
+
+struct ProfEntry {
+    std::string name;
+    double total_ms = 0.0;
+    int launches = 0;
+};
+
+struct fgpu_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool mesh_ready = false, vert_ready = false, vel_ready = false, kv_ready = false, forcing_ready = false;
+    bool tracers_ready = false;
+    DevMesh m{};
+    DevDyn d{};
+    DevTracers t{};
+    int N2 = 0, F2 = 0, E2 = 0, nl = 0, kmax = 0;
+    double cyclic_length = 0.0;
+    std::vector<void *> allocs;        // everything hipMalloc'ed by the context
+    std::vector<void *> tracer_allocs; // released on fgpu_tracers_init
+    // tracer configuration (host mirror)
+    std::vector<int> h_id, h_abo, h_hor, h_ver, h_lim, h_ivd;
+    std::vector<double> h_ph, h_pv;
+    std::vector<char> h_configured;
+    int *d_cfg_i = nullptr;
+    double *d_cfg_d = nullptr;
+    // host copies needed for setup
+    std::vector<int> h_updn, h_edges, h_edge_tri, h_ne_ptr, h_ne_idx, h_elem_nodes, h_ulevn;
+    // step graph
+    hipGraphExec_t graph = nullptr;
+    double graph_dt = 0.0;
+    bool graph_valid = false;
+    // profiling
+    bool prof = false;
+    std::vector<ProfEntry> prof_entries;
+    struct Pending { int idx; hipEvent_t e0, e1; };
+    std::vector<Pending> prof_pending;
+    std::vector<hipEvent_t> ev_free;
+    // reductions
+    double *d_red = nullptr;   // scratch for deterministic reductions
+    // SSH
+    int ssh_n = 0, ssh_nnz = 0, ssh_variant = -1;
+    bool ssh_ready = false, ssh_pr_ready = false;
+    int *ssh_rowptr = nullptr, *ssh_colind = nullptr;
+    double *ssh_val = nullptr, *ssh_pr = nullptr, *ssh_diag_inv = nullptr;
+    double *ssh_x = nullptr, *ssh_x0 = nullptr, *ssh_rhs = nullptr;
+    double *ssh_r = nullptr, *ssh_z = nullptr, *ssh_p = nullptr, *ssh_q = nullptr;
+    double *ssh_part = nullptr;   // per-block partial sums
+    double *ssh_out = nullptr;    // device stats
+    int ssh_blocks = 0;
+    std::vector<int> h_ssh_rowptr, h_ssh_colind;
+};
+
+// error helpers (fgpu_runtime.hip)
+int fgpu_fail(int code, const std::string &msg);
+#define FGPU_HIP(call)                                                               \
+    do {                                                                             \
+        hipError_t _e = (call);                                                      \
+        if (_e != hipSuccess)                                                        \
+            return fgpu_fail(FGPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+int fgpu_alloc(fgpu_context *c, void **p, size_t bytes, bool tracer_scope = false);
+
+// launch bracketing for the per-kernel profile (fgpu_runtime.hip): when the
+// context profiles, the constructor records an event before the launch and the
+// destructor one after it; fgpu_prof_flush() adds the elapsed times up.
+struct ProfScope {
+    fgpu_context *c;
+    int idx = -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProfScope(fgpu_context *c, const char *name);
+    ~ProfScope();
+};
+int fgpu_prof_flush(fgpu_context *c);

@@ -1,0 +1,632 @@
+// Context, mesh upload and data movement of libfesom_gpu.so (gfx950).
+//
+// Mesh preprocessing done once at upload, as the reference does in mesh_setup
+// and muscl_adv_init:
+//   * 1-based -> 0-based connectivity;
+//   * per-edge level numbers nl1/nu1/nl2/nu2 (oce_adv_tra_hor.F90:104-128) and
+//     the nu12/nl12 max/min rule of the edge scatters (oce_adv_tra_driver.F90:142-156);
+//   * node -> edge gather lists in ascending edge order, so that every
+//     edge->node sum is a deterministic gather in the serial reference order
+//     (no float atomics);
+//   * up/downwind triangles (oce_muscl_adv.F90:162-352);
+//   * SSH sparsity with the diagonal first (oce_ale.F90:1440-1496).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "fgpu_internal.h"
+
+static thread_local std::string g_last_error;
+
+int fgpu_fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+extern "C" const char *fgpu_last_error(void) { return g_last_error.c_str(); }
+extern "C" int fgpu_version(void) { return 1; }
+
+int fgpu_alloc(fgpu_context *c, void **p, size_t bytes, bool tracer_scope)
+{
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess)
+        return fgpu_fail(FGPU_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                             "): " + hipGetErrorString(e));
+    (tracer_scope ? c->tracer_allocs : c->allocs).push_back(*p);
+    return FGPU_OK;
+}
+
+template <typename T>
+static int upload(fgpu_context *c, T **dst, const T *src, size_t n)
+{
+    int rc = fgpu_alloc(c, (void **)dst, n * sizeof(T));
+    if (rc) return rc;
+    FGPU_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return FGPU_OK;
+}
+
+// ---------------------------------------------------------------- profiling
+ProfScope::ProfScope(fgpu_context *cc, const char *name) : c(cc)
+{
+    if (!c->prof) return;
+    for (size_t i = 0; i < c->prof_entries.size(); ++i)
+        if (c->prof_entries[i].name == name) { idx = (int)i; break; }
+    if (idx < 0) {
+        ProfEntry pe;
+        pe.name = name;
+        c->prof_entries.push_back(pe);
+        idx = (int)c->prof_entries.size() - 1;
+    }
+    for (hipEvent_t *e : {&e0, &e1}) {
+        if (!c->ev_free.empty()) { *e = c->ev_free.back(); c->ev_free.pop_back(); }
+        else if (hipEventCreate(e) != hipSuccess) { *e = nullptr; }
+    }
+    if (e0) (void)hipEventRecord(e0, c->stream);
+}
+
+ProfScope::~ProfScope()
+{
+    if (!c->prof || idx < 0 || !e0 || !e1) return;
+    (void)hipEventRecord(e1, c->stream);
+    c->prof_pending.push_back({idx, e0, e1});
+}
+
+int fgpu_prof_flush(fgpu_context *c)
+{
+    if (c->prof_pending.empty()) return FGPU_OK;
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    for (auto &p : c->prof_pending) {
+        float ms = 0.f;
+        FGPU_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
+        c->prof_entries[p.idx].total_ms += ms;
+        c->prof_entries[p.idx].launches += 1;
+        c->ev_free.push_back(p.e0);
+        c->ev_free.push_back(p.e1);
+    }
+    c->prof_pending.clear();
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_profile_enable(fgpu_context *c, int on)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    int rc = fgpu_prof_flush(c);
+    if (rc) return rc;
+    c->prof = on != 0;
+    c->prof_entries.clear();
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_profile_count(fgpu_context *c)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    int rc = fgpu_prof_flush(c);
+    if (rc) return rc;
+    return (int)c->prof_entries.size();
+}
+
+extern "C" int fgpu_profile_read(fgpu_context *c, int idx, char *name, int name_len, double *total_ms,
+                                 int *launches)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    int rc = fgpu_prof_flush(c);
+    if (rc) return rc;
+    if (idx < 0 || idx >= (int)c->prof_entries.size()) return fgpu_fail(FGPU_ERR_ARG, "profile index");
+    const ProfEntry &e = c->prof_entries[idx];
+    if (name && name_len > 0) {
+        std::strncpy(name, e.name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (total_ms) *total_ms = e.total_ms;
+    if (launches) *launches = e.launches;
+    return FGPU_OK;
+}
+
+// ---------------------------------------------------------------- context
+extern "C" int fgpu_create(fgpu_context **out, int device)
+{
+    if (!out) return fgpu_fail(FGPU_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    FGPU_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fgpu_fail(FGPU_ERR_ARG, "device " + std::to_string(device) + " out of range (" +
+                                           std::to_string(ndev) + " visible)");
+    FGPU_HIP(hipSetDevice(device));
+    fgpu_context *c = new fgpu_context();
+    c->device = device;
+    // default priority: stream priorities are left alone on purpose
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fgpu_fail(FGPU_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return FGPU_OK;
+}
+
+static void free_list(std::vector<void *> &v)
+{
+    for (void *p : v) (void)hipFree(p);
+    v.clear();
+}
+
+extern "C" int fgpu_destroy(fgpu_context *c)
+{
+    if (!c) return FGPU_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &p : c->prof_pending) { (void)hipEventDestroy(p.e0); (void)hipEventDestroy(p.e1); }
+    for (auto e : c->ev_free) (void)hipEventDestroy(e);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    free_list(c->tracer_allocs);
+    free_list(c->allocs);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_synchronize(fgpu_context *c)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    return FGPU_OK;
+}
+
+// ---------------------------------------------------------------- mesh
+static double trim_cyc(double b, double Lc)
+{   // gen_modules_rotate_grid.F90:204-211
+    if (b > Lc / 2.0) b = b - Lc;
+    if (b < -Lc / 2.0) b = b + Lc;
+    return b;
+}
+
+// find_up_downwind_triangles (oce_muscl_adv.F90:162-352), one search at a node;
+// every matching branch is a CYCLE in the reference, so the last match wins.
+static int updn_search(const double *coord, const int *elem_nodes, const int *nie, const int *nie_num,
+                       int kmax, double Lc, int node, const double x[2])
+{
+    int found = -1;
+    for (int k = 0; k < nie_num[node]; ++k) {
+        int elem = nie[(size_t)node * kmax + k];
+        const int *en = elem_nodes + 3 * (size_t)elem;
+        int a0, a1, a2;
+        if (en[0] == node) { a0 = 0; a1 = 1; a2 = 2; }
+        else if (en[1] == node) { a0 = 1; a1 = 0; a2 = 2; }
+        else { a0 = 2; a1 = 0; a2 = 1; }
+        double b[2], c[2];
+        b[0] = coord[2 * en[a1]] - coord[2 * en[a0]];
+        b[1] = coord[2 * en[a1] + 1] - coord[2 * en[a0] + 1];
+        c[0] = coord[2 * en[a2]] - coord[2 * en[a0]];
+        c[1] = coord[2 * en[a2] + 1] - coord[2 * en[a0] + 1];
+        b[0] = trim_cyc(b[0], Lc);
+        c[0] = trim_cyc(c[0], Lc);
+        double cr = c[0] * c[0] + c[1] * c[1];
+        double bx = (b[0] * c[0] + b[1] * c[1]) / cr;
+        double by = (-b[0] * c[1] + b[1] * c[0]) / cr;
+        double xx = (x[0] * c[0] + x[1] * c[1]) / cr;
+        double xy = (-x[0] * c[1] + x[1] * c[0]) / cr;
+        double ab = std::atan2(by, bx);
+        double ax = std::atan2(xy, xx);
+        if ((ab > 0.0) && (ax > 0.0) && (ax < ab)) { found = elem; continue; }
+        if ((ab < 0.0) && (ax < 0.0) && (ax > ab)) { found = elem; continue; }
+        if ((ab == ax) || (ax == 0.0)) { found = elem; continue; }
+    }
+    return found;
+}
+
+extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D, int nl, int kmax,
+                              double cyclic_length, const double *coord_nod2D, const int *elem2D_nodes,
+                              const int *edges, const int *edge_tri, const int *nod_in_elem2D,
+                              const int *nod_in_elem2D_num, const int *nlevels, const int *ulevels,
+                              const int *nlevels_nod2D, const int *ulevels_nod2D,
+                              const int *nlevels_nod2D_min, const int *ulevels_nod2D_max,
+                              const double *elem_area, const double *elem_cos, const double *edge_dxdy,
+                              const double *edge_cross_dxdy, const double *gradient_sca)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh already initialised for this context");
+    if (nod2D <= 0 || elem2D <= 0 || edge2D <= 0 || kmax <= 0)
+        return fgpu_fail(FGPU_ERR_ARG, "empty mesh");
+    if (nl < 3 || nl > FGPU_MAX_NL) return fgpu_fail(FGPU_ERR_ARG, "nl out of range [3, 128]");
+    if (!coord_nod2D || !elem2D_nodes || !edges || !edge_tri || !nod_in_elem2D || !nod_in_elem2D_num ||
+        !nlevels || !ulevels || !nlevels_nod2D || !ulevels_nod2D || !nlevels_nod2D_min ||
+        !ulevels_nod2D_max || !elem_area || !elem_cos || !edge_dxdy || !edge_cross_dxdy || !gradient_sca)
+        return fgpu_fail(FGPU_ERR_ARG, "null mesh array");
+    FGPU_HIP(hipSetDevice(c->device));
+    const int N2 = nod2D, F2 = elem2D, E2 = edge2D, L = nl - 1;
+    // ---- validate and convert connectivity to 0-based
+    std::vector<int> en(3 * (size_t)F2), ed(2 * (size_t)E2), et(2 * (size_t)E2), nie((size_t)N2 * kmax);
+    for (size_t i = 0; i < en.size(); ++i) {
+        en[i] = elem2D_nodes[i] - 1;
+        if (en[i] < 0 || en[i] >= N2) return fgpu_fail(FGPU_ERR_ARG, "elem2D_nodes out of range");
+    }
+    for (size_t i = 0; i < ed.size(); ++i) {
+        ed[i] = edges[i] - 1;
+        if (ed[i] < 0 || ed[i] >= N2) return fgpu_fail(FGPU_ERR_ARG, "edges out of range");
+    }
+    for (int e = 0; e < E2; ++e) {
+        int a = edge_tri[2 * e], b = edge_tri[2 * e + 1];
+        if (a < 1 || a > F2) return fgpu_fail(FGPU_ERR_ARG, "edge_tri(1,:) out of range");
+        if (b > F2) return fgpu_fail(FGPU_ERR_ARG, "edge_tri(2,:) out of range");
+        et[2 * e] = a - 1;
+        et[2 * e + 1] = b > 0 ? b - 1 : -1;
+    }
+    for (int n = 0; n < N2; ++n) {
+        int num = nod_in_elem2D_num[n];
+        if (num < 1 || num > kmax) return fgpu_fail(FGPU_ERR_ARG, "nod_in_elem2D_num out of range");
+        for (int k = 0; k < kmax; ++k) {
+            int v = nod_in_elem2D[(size_t)n * kmax + k];
+            if (k < num) {
+                if (v < 1 || v > F2) return fgpu_fail(FGPU_ERR_ARG, "nod_in_elem2D out of range");
+                nie[(size_t)n * kmax + k] = v - 1;
+            } else {
+                nie[(size_t)n * kmax + k] = -1;
+            }
+        }
+    }
+    for (int e = 0; e < F2; ++e)
+        if (ulevels[e] < 1 || nlevels[e] - ulevels[e] < 1 || nlevels[e] > nl)
+            return fgpu_fail(FGPU_ERR_ARG, "element level numbers out of range");
+    for (int n = 0; n < N2; ++n)
+        if (ulevels_nod2D[n] < 1 || nlevels_nod2D[n] - ulevels_nod2D[n] < 2 || nlevels_nod2D[n] > nl ||
+            nlevels_nod2D_min[n] > nlevels_nod2D[n] || ulevels_nod2D_max[n] < ulevels_nod2D[n])
+            return fgpu_fail(FGPU_ERR_ARG, "node level numbers out of range (columns need >= 2 layers)");
+    // ---- per-edge level numbers
+    std::vector<int> einfo(4 * (size_t)E2), erange(2 * (size_t)E2);
+    for (int e = 0; e < E2; ++e) {
+        int el1 = et[2 * e], el2 = et[2 * e + 1];
+        int nl1 = nlevels[el1] - 1, nu1 = ulevels[el1], nl2 = 0, nu2 = 0;
+        if (el2 >= 0) { nl2 = nlevels[el2] - 1; nu2 = ulevels[el2]; }
+        einfo[4 * e + 0] = nl1; einfo[4 * e + 1] = nu1; einfo[4 * e + 2] = nl2; einfo[4 * e + 3] = nu2;
+        int nl12 = std::max(nl1, nl2), nu12 = nu1;
+        if (nu2 > 0) nu12 = std::min(nu1, nu2);
+        erange[2 * e] = nu12;
+        erange[2 * e + 1] = nl12;
+        int n1 = ed[2 * e], n2 = ed[2 * e + 1];
+        if (nl12 > nlevels_nod2D[n1] - 1 || nl12 > nlevels_nod2D[n2] - 1)
+            return fgpu_fail(FGPU_ERR_ARG, "edge deeper than one of its nodes");
+    }
+    // ---- node -> edge gather lists, edges ascending
+    std::vector<int> ne_ptr(N2 + 1, 0), ne_idx(2 * (size_t)E2);
+    for (int e = 0; e < E2; ++e) { ne_ptr[ed[2 * e] + 1]++; ne_ptr[ed[2 * e + 1] + 1]++; }
+    for (int n = 0; n < N2; ++n) ne_ptr[n + 1] += ne_ptr[n];
+    {
+        std::vector<int> fill(ne_ptr.begin(), ne_ptr.end() - 1);
+        for (int e = 0; e < E2; ++e) {
+            ne_idx[fill[ed[2 * e]]++] = 2 * e;
+            ne_idx[fill[ed[2 * e + 1]]++] = 2 * e + 1;
+        }
+    }
+    // ---- up/downwind triangles
+    std::vector<int> updn(2 * (size_t)E2);
+    for (int e = 0; e < E2; ++e) {
+        int n1 = ed[2 * e], n2 = ed[2 * e + 1];
+        double x[2];
+        x[0] = coord_nod2D[2 * n2] - coord_nod2D[2 * n1];
+        x[1] = coord_nod2D[2 * n2 + 1] - coord_nod2D[2 * n1 + 1];
+        if (x[0] > cyclic_length / 2.0) x[0] = x[0] - cyclic_length;
+        if (x[0] < -cyclic_length / 2.0) x[0] = x[0] + cyclic_length;
+        double xm[2] = {-x[0], -x[1]};
+        updn[2 * e] = updn_search(coord_nod2D, en.data(), nie.data(), nod_in_elem2D_num, kmax,
+                                  cyclic_length, n1, xm);
+        updn[2 * e + 1] = updn_search(coord_nod2D, en.data(), nie.data(), nod_in_elem2D_num, kmax,
+                                      cyclic_length, n2, x);
+    }
+    // ---- upload
+    DevMesh &m = c->m;
+    m.N2 = N2; m.F2 = F2; m.E2 = E2; m.L = L; m.NL = nl; m.kmax = kmax;
+    int rc;
+#define UP(dst, src, n)                                                \
+    do {                                                               \
+        rc = upload(c, &dst, src, (size_t)(n));                        \
+        if (rc) return rc;                                             \
+    } while (0)
+    int *p_i;
+    double *p_d;
+    UP(p_i, en.data(), en.size()); m.elem_nodes = p_i;
+    UP(p_i, ed.data(), ed.size()); m.edges = p_i;
+    UP(p_i, et.data(), et.size()); m.edge_tri = p_i;
+    UP(p_i, einfo.data(), einfo.size()); m.einfo = p_i;
+    UP(p_i, erange.data(), erange.size()); m.erange = p_i;
+    UP(p_i, updn.data(), updn.size()); m.updn = p_i;
+    UP(p_i, nie.data(), nie.size()); m.nie = p_i;
+    UP(p_i, nod_in_elem2D_num, N2); m.nie_num = p_i;
+    UP(p_i, nlevels, F2); m.nlev = p_i;
+    UP(p_i, ulevels, F2); m.ulev = p_i;
+    UP(p_i, nlevels_nod2D, N2); m.nlevn = p_i;
+    UP(p_i, ulevels_nod2D, N2); m.ulevn = p_i;
+    UP(p_i, nlevels_nod2D_min, N2); m.nlevn_min = p_i;
+    UP(p_i, ulevels_nod2D_max, N2); m.ulevn_max = p_i;
+    UP(p_i, ne_ptr.data(), ne_ptr.size()); m.ne_ptr = p_i;
+    UP(p_i, ne_idx.data(), ne_idx.size()); m.ne_idx = p_i;
+    UP(p_d, elem_area, F2); m.elem_area = p_d;
+    UP(p_d, elem_cos, F2); m.elem_cos = p_d;
+    UP(p_d, edge_dxdy, 2 * (size_t)E2); m.edge_dxdy = p_d;
+    UP(p_d, edge_cross_dxdy, 4 * (size_t)E2); m.ecd = p_d;
+    UP(p_d, gradient_sca, 6 * (size_t)F2); m.gsca = p_d;
+#undef UP
+    c->N2 = N2; c->F2 = F2; c->E2 = E2; c->nl = nl; c->kmax = kmax;
+    c->cyclic_length = cyclic_length;
+    c->h_updn = updn;
+    c->h_edges = ed;
+    c->h_edge_tri = et;
+    c->h_ne_ptr = ne_ptr;
+    c->h_ne_idx = ne_idx;
+    c->h_elem_nodes = en;
+    c->h_ulevn.assign(ulevels_nod2D, ulevels_nod2D + N2);
+    rc = fgpu_alloc(c, (void **)&c->d_red, sizeof(double) * 4096);
+    if (rc) return rc;
+    c->mesh_ready = true;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_edge_up_dn_tri(fgpu_context *c, int *out)
+{
+    if (!c || !out) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    for (size_t i = 0; i < c->h_updn.size(); ++i) out[i] = c->h_updn[i] + 1;
+    return FGPU_OK;
+}
+
+template <typename T>
+static int upload_or_refresh(fgpu_context *c, const T **dst, const T *src, size_t n)
+{
+    if (!src) return fgpu_fail(FGPU_ERR_ARG, "null array");
+    if (*dst == nullptr) {
+        T *p;
+        int rc = fgpu_alloc(c, (void **)&p, n * sizeof(T));
+        if (rc) return rc;
+        *dst = p;
+    }
+    FGPU_HIP(hipMemcpy((void *)*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_set_vertical(fgpu_context *c, const double *area, const double *areasvol,
+                                 const double *hnode, const double *hnode_new, const double *helem,
+                                 const double *zbar_3d_n, const double *Z_3d_n, const double *zbar_n_bot,
+                                 const double *zbar_e_bot, const double *zbar_e_srf)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    FGPU_HIP(hipSetDevice(c->device));
+    const size_t N2 = c->N2, F2 = c->F2, L = c->nl - 1, NL = c->nl;
+    int rc;
+    if ((rc = upload_or_refresh(c, &c->m.area, area, N2 * NL))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.areasvol, areasvol, N2 * NL))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.hnode, hnode, N2 * L))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.hnode_new, hnode_new, N2 * L))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.helem, helem, F2 * L))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.zbar3d, zbar_3d_n, N2 * NL))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.Z3d, Z_3d_n, N2 * L))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.zbar_n_bot, zbar_n_bot, N2))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.zbar_e_bot, zbar_e_bot, F2))) return rc;
+    if ((rc = upload_or_refresh(c, &c->m.zbar_e_srf, zbar_e_srf, F2))) return rc;
+    c->vert_ready = true;
+    c->graph_valid = false;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_set_velocity(fgpu_context *c, const double *uv, const double *w, const double *w_e,
+                                 const double *w_i, int use_wsplit)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    if (use_wsplit) return fgpu_fail(FGPU_ERR_ARG, "use_wsplit=.true. (adv_tra_vert_impl) is not supported");
+    FGPU_HIP(hipSetDevice(c->device));
+    const size_t N2 = c->N2, F2 = c->F2, L = c->nl - 1, NL = c->nl;
+    int rc;
+    if ((rc = upload_or_refresh(c, &c->d.uv, uv, F2 * L * 2))) return rc;
+    if ((rc = upload_or_refresh(c, &c->d.w, w, N2 * NL))) return rc;
+    if ((rc = upload_or_refresh(c, &c->d.w_e, w_e, N2 * NL))) return rc;
+    if ((rc = upload_or_refresh(c, &c->d.w_i, w_i, N2 * NL))) return rc;
+    c->vel_ready = true;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_set_kv(fgpu_context *c, const double *Kv)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    FGPU_HIP(hipSetDevice(c->device));
+    int rc = upload_or_refresh(c, &c->d.Kv, Kv, (size_t)c->N2 * c->nl);
+    if (rc) return rc;
+    c->kv_ready = true;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_set_surface_forcing(fgpu_context *c, const double *heat_flux, const double *water_flux,
+                                        const double *virtual_salt, const double *relax_salt,
+                                        const double *real_salt_flux, double is_nonlinfs)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    FGPU_HIP(hipSetDevice(c->device));
+    const size_t N2 = c->N2;
+    int rc;
+    if ((rc = upload_or_refresh(c, &c->d.heat_flux, heat_flux, N2))) return rc;
+    if ((rc = upload_or_refresh(c, &c->d.water_flux, water_flux, N2))) return rc;
+    if ((rc = upload_or_refresh(c, &c->d.virtual_salt, virtual_salt, N2))) return rc;
+    if ((rc = upload_or_refresh(c, &c->d.relax_salt, relax_salt, N2))) return rc;
+    if ((rc = upload_or_refresh(c, &c->d.real_salt_flux, real_salt_flux, N2))) return rc;
+    c->d.is_nonlinfs = is_nonlinfs;
+    c->forcing_ready = true;
+    c->graph_valid = false;
+    return FGPU_OK;
+}
+
+// ---------------------------------------------------------------- tracers
+extern "C" int fgpu_tracers_init(fgpu_context *c, int num_tracers)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    if (num_tracers < 1 || num_tracers > 4096) return fgpu_fail(FGPU_ERR_ARG, "num_tracers out of range");
+    FGPU_HIP(hipSetDevice(c->device));
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    free_list(c->tracer_allocs);
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->graph_valid = false;
+    const int nt = num_tracers;
+    const size_t N2 = c->N2, F2 = c->F2, E2 = c->E2, L = c->nl - 1, NL = c->nl;
+    DevTracers &t = c->t;
+    t.ntr = nt;
+    t.sN = N2 * L;
+    t.sNI = N2 * NL;
+    t.sE = E2 * L;
+    t.sF2 = F2 * L * 2;
+    t.sOld = N2 * L * 2;
+    int rc;
+    struct A { double **p; size_t n; } arrs[] = {
+        {&t.T, t.sN * nt}, {&t.Told, t.sOld * nt}, {&t.TAB, t.sN * nt}, {&t.tr_xy, t.sF2 * nt},
+        {&t.lo_h, t.sE * nt}, {&t.a_h, t.sE * nt}, {&t.fct_lo, t.sN * nt}, {&t.a_v, t.sNI * nt},
+        {&t.aux, t.sF2 * nt}, {&t.rp, t.sN * nt}, {&t.rm, t.sN * nt}, {&t.dttf_h, t.sN},
+        {&t.dttf_v, t.sN}, {&t.dbg_av, t.sNI}, {&t.dbg_grad, t.sE * 4}};
+    for (auto &a : arrs) {
+        if ((rc = fgpu_alloc(c, (void **)a.p, a.n * sizeof(double), true))) return rc;
+        FGPU_HIP(hipMemset(*a.p, 0, a.n * sizeof(double)));
+    }
+    if ((rc = fgpu_alloc(c, (void **)&c->d_cfg_i, sizeof(int) * 6 * nt, true))) return rc;
+    if ((rc = fgpu_alloc(c, (void **)&c->d_cfg_d, sizeof(double) * 2 * nt, true))) return rc;
+    t.id = c->d_cfg_i;
+    t.abo = c->d_cfg_i + nt;
+    t.hor = c->d_cfg_i + 2 * nt;
+    t.ver = c->d_cfg_i + 3 * nt;
+    t.lim = c->d_cfg_i + 4 * nt;
+    t.ivd = c->d_cfg_i + 5 * nt;
+    t.ph = c->d_cfg_d;
+    t.pv = c->d_cfg_d + nt;
+    c->h_id.assign(nt, 1); c->h_abo.assign(nt, 2); c->h_hor.assign(nt, HOR_MFCT);
+    c->h_ver.assign(nt, VER_QR4C); c->h_lim.assign(nt, LIM_FCT); c->h_ivd.assign(nt, 1);
+    c->h_ph.assign(nt, 0.0); c->h_pv.assign(nt, 1.0);
+    c->h_configured.assign(nt, 0);
+    c->tracers_ready = true;
+    return FGPU_OK;
+}
+
+static int parse_scheme(const char *s, const char *const *names, int nnames, int *code)
+{
+    if (!s) return -1;
+    std::string v(s);
+    while (!v.empty() && (v.back() == ' ' || v.back() == '\0')) v.pop_back();
+    size_t b = 0;
+    while (b < v.size() && v[b] == ' ') ++b;
+    v = v.substr(b);
+    for (int i = 0; i < nnames; ++i)
+        if (v == names[i]) { *code = i; return 0; }
+    return -1;
+}
+
+static int push_config(fgpu_context *c)
+{
+    const int nt = c->t.ntr;
+    std::vector<int> ci(6 * nt);
+    std::vector<double> cd(2 * nt);
+    for (int i = 0; i < nt; ++i) {
+        ci[i] = c->h_id[i]; ci[nt + i] = c->h_abo[i]; ci[2 * nt + i] = c->h_hor[i];
+        ci[3 * nt + i] = c->h_ver[i]; ci[4 * nt + i] = c->h_lim[i]; ci[5 * nt + i] = c->h_ivd[i];
+        cd[i] = c->h_ph[i]; cd[nt + i] = c->h_pv[i];
+    }
+    FGPU_HIP(hipMemcpy(c->d_cfg_i, ci.data(), ci.size() * sizeof(int), hipMemcpyHostToDevice));
+    FGPU_HIP(hipMemcpy(c->d_cfg_d, cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice));
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_tracer_config(fgpu_context *c, int tr_num, int ID, int AB_order, const char *tra_adv_hor,
+                                  const char *tra_adv_ver, const char *tra_adv_lim, double tra_adv_ph,
+                                  double tra_adv_pv, int i_vert_diff)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
+    if (tr_num < 1 || tr_num > c->t.ntr) return fgpu_fail(FGPU_ERR_ARG, "tr_num out of range");
+    static const char *hor_names[] = {"MFCT", "UPW1"};
+    static const char *ver_names[] = {"QR4C", "UPW1"};
+    static const char *lim_names[] = {"FCT", "NON"};
+    int h, v, l;
+    if (parse_scheme(tra_adv_hor, hor_names, 2, &h))
+        return fgpu_fail(FGPU_ERR_SCHEME, std::string("Unknown horizontal advection type ") +
+                                              (tra_adv_hor ? tra_adv_hor : "(null)") + "! Check your namelists!");
+    if (parse_scheme(tra_adv_ver, ver_names, 2, &v))
+        return fgpu_fail(FGPU_ERR_SCHEME, std::string("Unknown vertical advection type ") +
+                                              (tra_adv_ver ? tra_adv_ver : "(null)") + "! Check your namelists!");
+    if (parse_scheme(tra_adv_lim, lim_names, 2, &l))
+        return fgpu_fail(FGPU_ERR_SCHEME, std::string("Unknown limiter ") +
+                                              (tra_adv_lim ? tra_adv_lim : "(null)"));
+    if (AB_order != 2 && AB_order != 3)
+        return fgpu_fail(FGPU_ERR_ARG, "Adams-Bashfort tracer order must be 2 or 3");
+    switch (ID) {   // bc_surface IDs of a plain build (oce_ale_tracer.F90:1862-2098)
+    case 1: case 2: case 100: case 301: case 302: case 303: break;
+    default:
+        return fgpu_fail(FGPU_ERR_BC, "invalid ID " + std::to_string(ID) +
+                                          " specified in boundary conditions (supported: 1, 2, 100, 301-303)");
+    }
+    const int i = tr_num - 1;
+    c->h_id[i] = ID; c->h_abo[i] = AB_order; c->h_hor[i] = h; c->h_ver[i] = v; c->h_lim[i] = l;
+    c->h_ph[i] = tra_adv_ph; c->h_pv[i] = tra_adv_pv; c->h_ivd[i] = i_vert_diff ? 1 : 0;
+    c->h_configured[i] = 1;
+    c->graph_valid = false;
+    FGPU_HIP(hipSetDevice(c->device));
+    return push_config(c);
+}
+
+extern "C" int fgpu_tracer_upload(fgpu_context *c, int tr_num, const double *values, const double *valuesold)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
+    if (tr_num < 1 || tr_num > c->t.ntr) return fgpu_fail(FGPU_ERR_ARG, "tr_num out of range");
+    if (!values || !valuesold) return fgpu_fail(FGPU_ERR_ARG, "null array");
+    FGPU_HIP(hipSetDevice(c->device));
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    const int i = tr_num - 1;
+    const size_t nold = c->t.sN * (size_t)(c->h_abo[i] - 1);
+    FGPU_HIP(hipMemcpy(c->t.T + c->t.sN * i, values, c->t.sN * sizeof(double), hipMemcpyHostToDevice));
+    FGPU_HIP(hipMemcpy(c->t.Told + c->t.sOld * i, valuesold, nold * sizeof(double), hipMemcpyHostToDevice));
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_tracer_download(fgpu_context *c, int tr_num, double *values, double *valuesold)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
+    if (tr_num < 1 || tr_num > c->t.ntr) return fgpu_fail(FGPU_ERR_ARG, "tr_num out of range");
+    FGPU_HIP(hipSetDevice(c->device));
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    const int i = tr_num - 1;
+    const size_t nold = c->t.sN * (size_t)(c->h_abo[i] - 1);
+    if (values)
+        FGPU_HIP(hipMemcpy(values, c->t.T + c->t.sN * i, c->t.sN * sizeof(double), hipMemcpyDeviceToHost));
+    if (valuesold)
+        FGPU_HIP(hipMemcpy(valuesold, c->t.Told + c->t.sOld * i, nold * sizeof(double), hipMemcpyDeviceToHost));
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_tracer_work_download(fgpu_context *c, int tr_num, const char *name, double *out, long n)
+{
+    if (!c || !name || !out) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
+    if (tr_num < 1 || tr_num > c->t.ntr) return fgpu_fail(FGPU_ERR_ARG, "tr_num out of range");
+    FGPU_HIP(hipSetDevice(c->device));
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    const size_t i = tr_num - 1;
+    const DevTracers &t = c->t;
+    const double *src = nullptr;
+    size_t cnt = 0;
+    std::string s(name);
+    if (s == "valuesAB") { src = t.TAB + t.sN * i; cnt = t.sN; }
+    else if (s == "tr_xy") { src = t.tr_xy + t.sF2 * i; cnt = t.sF2; }
+    else if (s == "edge_up_dn_grad") { src = t.dbg_grad; cnt = t.sE * 4; }
+    else if (s == "fct_LO") { src = t.fct_lo + t.sN * i; cnt = t.sN; }
+    else if (s == "adv_flux_hor") { src = t.a_h + t.sE * i; cnt = t.sE; }
+    else if (s == "adv_flux_ver") { src = t.dbg_av; cnt = t.sNI; }
+    else if (s == "del_ttf_advhoriz") { src = t.dttf_h; cnt = t.sN; }
+    else if (s == "del_ttf_advvert") { src = t.dttf_v; cnt = t.sN; }
+    else if (s == "fct_plus") { src = t.rp + t.sN * i; cnt = t.sN; }
+    else if (s == "fct_minus") { src = t.rm + t.sN * i; cnt = t.sN; }
+    else return fgpu_fail(FGPU_ERR_ARG, "unknown work array " + s);
+    if ((size_t)n != cnt) return fgpu_fail(FGPU_ERR_ARG, "work array " + s + " has " + std::to_string(cnt) + " entries");
+    FGPU_HIP(hipMemcpy(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    return FGPU_OK;
+}

@@ -1,0 +1,325 @@
+"""ctypes binding of libfesom_gpu.so (the C ABI declared in include/fesom_gpu.h).
+
+There is no fallback: if the library or a GPU is missing, loading or creating a
+context raises. Arrays are numpy, C-contiguous, with the memory image of the
+reference's Fortran arrays (level index fastest); connectivity handed to the
+library is 1-based, exactly as the Fortran derived types hold it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfesom_gpu.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+
+class FesomGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fesom_gpu error {code}: {msg}")
+        self.code = code
+
+
+class CgStats(C.Structure):
+    _fields_ = [("iters_last", C.c_int), ("converged", C.c_int), ("nbreakdown", C.c_int),
+                ("nnegrz", C.c_int), ("resid_last", C.c_double), ("rtol_last", C.c_double),
+                ("ms", C.c_double)]
+
+    def as_dict(self):
+        return dict(iters=self.iters_last, converged=bool(self.converged), nbreakdown=self.nbreakdown,
+                    nnegrz=self.nnegrz, resid=self.resid_last, rtol=self.rtol_last, ms=self.ms)
+
+
+# (name, argtypes); every function returns int status
+_SIGS = {
+    "fgpu_create": [C.POINTER(vp), C.c_int],
+    "fgpu_destroy": [vp],
+    "fgpu_synchronize": [vp],
+    "fgpu_mesh_init": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, ip, ip, ip, ip, ip,
+                       ip, ip, ip, ip, ip, ip, dp, dp, dp, dp, dp],
+    "fgpu_set_vertical": [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp],
+    "fgpu_set_velocity": [vp, dp, dp, dp, dp, C.c_int],
+    "fgpu_set_kv": [vp, dp],
+    "fgpu_set_surface_forcing": [vp, dp, dp, dp, dp, dp, C.c_double],
+    "fgpu_tracers_init": [vp, C.c_int],
+    "fgpu_tracer_config": [vp, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
+                           C.c_double, C.c_int],
+    "fgpu_tracer_upload": [vp, C.c_int, dp, dp],
+    "fgpu_tracer_download": [vp, C.c_int, dp, dp],
+    "fgpu_tracer_work_download": [vp, C.c_int, C.c_char_p, dp, C.c_long],
+    "fgpu_edge_up_dn_tri": [vp, ip],
+    "fgpu_init_tracers_AB": [vp, C.c_int],
+    "fgpu_do_oce_adv_tra": [vp, C.c_double, C.c_int],
+    "fgpu_diff_tracers_ale": [vp, C.c_double, C.c_int],
+    "fgpu_diff_ver_part_impl_ale": [vp, C.c_double, C.c_int],
+    "fgpu_solve_tracers_ale": [vp, C.c_double, C.c_int],
+    "fgpu_solve_tracers_ale_timed": [vp, C.c_double, C.c_int, dp],
+    "fgpu_profile_enable": [vp, C.c_int],
+    "fgpu_profile_count": [vp],
+    "fgpu_profile_read": [vp, C.c_int, C.c_char_p, C.c_int, dp, ip],
+    "fgpu_tracer_content": [vp, C.c_int, dp],
+    "fgpu_init_stiff_mat_ale": [vp, C.c_double, C.c_double, C.c_double, C.c_double],
+    "fgpu_ssh_set_matrix": [vp, C.c_int, C.c_int, ip, ip, dp],
+    "fgpu_ssh_get_matrix": [vp, ip, ip, dp, dp, ip],
+    "fgpu_ssh_solve_preconditioner": [vp, C.c_int],
+    "fgpu_ssh_solve_cg": [vp, dp, dp, C.c_double, C.c_int, C.POINTER(CgStats)],
+    "fgpu_ssh_upload": [vp, dp, dp],
+    "fgpu_ssh_solve_resident": [vp, C.c_double, C.c_int, C.c_int, C.POINTER(CgStats)],
+    "fgpu_ssh_download": [vp, dp],
+}
+EXPORTED = sorted(list(_SIGS) + ["fgpu_last_error", "fgpu_version"])
+
+_lib = None
+
+
+def load(path: Optional[str] = None):
+    """Load the library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise FesomGpuError(-100, f"{path} is not built; run fesom2_amd/build.py (or __graft_entry__.build())")
+    L = C.CDLL(path)
+    L.fgpu_last_error.restype = C.c_char_p
+    L.fgpu_last_error.argtypes = []
+    L.fgpu_version.restype = C.c_int
+    for name, args in _SIGS.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise FesomGpuError(rc, load().fgpu_last_error().decode())
+    return rc
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, "float64 C-contiguous array expected"
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous, "int32 C-contiguous array expected"
+    return a.ctypes.data_as(ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """One FESOM2 partition resident on one GPU."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = vp()
+        _check(L.fgpu_create(C.byref(h), int(device)))
+        self.h = h
+        self.L = L
+        self.ntr = 0
+        self.nl = 0
+        self.N2 = 0
+
+    def close(self):
+        if self.h:
+            self.L.fgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ mesh & fields
+    def mesh_init(self, m):
+        """Upload a meshgen.Mesh (0-based numpy connectivity -> 1-based)."""
+        self._mesh_keep = [
+            _f64(m.coord_nod2D), _i32(m.elem2D_nodes + 1), _i32(m.edges + 1),
+            _i32(np.where(m.edge_tri >= 0, m.edge_tri + 1, -999)),
+            _i32(np.where(m.nod_in_elem2D >= 0, m.nod_in_elem2D + 1, 0)), _i32(m.nod_in_elem2D_num),
+            _i32(m.nlevels), _i32(m.ulevels), _i32(m.nlevels_nod2D), _i32(m.ulevels_nod2D),
+            _i32(m.nlevels_nod2D_min), _i32(m.ulevels_nod2D_max), _f64(m.elem_area), _f64(m.elem_cos),
+            _f64(m.edge_dxdy), _f64(m.edge_cross_dxdy), _f64(m.gradient_sca)]
+        k = self._mesh_keep
+        _check(self.L.fgpu_mesh_init(self.h, m.nod2D, m.elem2D, m.edge2D, m.nl, m.kmax, m.cyclic_length,
+                                     _d(k[0]), _i(k[1]), _i(k[2]), _i(k[3]), _i(k[4]), _i(k[5]), _i(k[6]),
+                                     _i(k[7]), _i(k[8]), _i(k[9]), _i(k[10]), _i(k[11]), _d(k[12]),
+                                     _d(k[13]), _d(k[14]), _d(k[15]), _d(k[16])))
+        self.nl = m.nl
+        self.N2 = m.nod2D
+        self.set_vertical(m)
+
+    def set_vertical(self, m):
+        arrs = [_f64(a) for a in (m.area, m.areasvol, m.hnode, m.hnode_new, m.helem, m.zbar_3d_n, m.Z_3d_n,
+                                  m.zbar_n_bot, m.zbar_e_bot, m.zbar_e_srf)]
+        _check(self.L.fgpu_set_vertical(self.h, *[_d(a) for a in arrs]))
+
+    def set_velocity(self, uv, w, w_e, w_i, use_wsplit=False):
+        a = [_f64(x) for x in (uv, w, w_e, w_i)]
+        _check(self.L.fgpu_set_velocity(self.h, *[_d(x) for x in a], int(bool(use_wsplit))))
+
+    def set_kv(self, Kv):
+        _check(self.L.fgpu_set_kv(self.h, _d(_f64(Kv))))
+
+    def set_surface_forcing(self, heat_flux, water_flux, virtual_salt, relax_salt, real_salt_flux, is_nonlinfs=0.0):
+        a = [_f64(x) for x in (heat_flux, water_flux, virtual_salt, relax_salt, real_salt_flux)]
+        _check(self.L.fgpu_set_surface_forcing(self.h, *[_d(x) for x in a], float(is_nonlinfs)))
+
+    def set_state(self, st):
+        """Velocities, Kv and surface forcing of a meshgen.State."""
+        self.set_velocity(st.uv, st.w, st.w_e, st.w_i)
+        self.set_kv(st.Kv)
+        self.set_surface_forcing(st.heat_flux, st.water_flux, st.virtual_salt, st.relax_salt, st.real_salt_flux)
+
+    # ------------------------------------------------------------ tracers
+    def tracers_init(self, ntr):
+        _check(self.L.fgpu_tracers_init(self.h, int(ntr)))
+        self.ntr = ntr
+        self._abo = [2] * ntr
+
+    def tracer_config(self, tr_num, ID, AB_order=2, hor="MFCT", ver="QR4C", lim="FCT", ph=0.0, pv=1.0,
+                      i_vert_diff=True):
+        _check(self.L.fgpu_tracer_config(self.h, int(tr_num), int(ID), int(AB_order), hor.encode(), ver.encode(),
+                                         lim.encode(), float(ph), float(pv), int(bool(i_vert_diff))))
+        self._abo[tr_num - 1] = AB_order
+
+    def tracer_upload(self, tr_num, values, valuesold):
+        _check(self.L.fgpu_tracer_upload(self.h, int(tr_num), _d(_f64(values)), _d(_f64(valuesold))))
+
+    def tracer_download(self, tr_num):
+        L = self.nl - 1
+        v = np.zeros((self.N2, L))
+        abo = self._abo[tr_num - 1]
+        vo = np.zeros((self.N2, L) if abo == 2 else (self.N2, L, abo - 1))
+        _check(self.L.fgpu_tracer_download(self.h, int(tr_num), _d(v), _d(vo)))
+        return v, vo
+
+    def work(self, tr_num, name, shape):
+        out = np.zeros(shape)
+        _check(self.L.fgpu_tracer_work_download(self.h, int(tr_num), name.encode(), _d(out), int(out.size)))
+        return out
+
+    def edge_up_dn_tri(self, E2):
+        out = np.zeros((E2, 2), np.int32)
+        _check(self.L.fgpu_edge_up_dn_tri(self.h, _i(out)))
+        return out
+
+    # ------------------------------------------------------------ entry points
+    def init_tracers_AB(self, tr_num):
+        _check(self.L.fgpu_init_tracers_AB(self.h, int(tr_num)))
+
+    def do_oce_adv_tra(self, dt, tr_num):
+        _check(self.L.fgpu_do_oce_adv_tra(self.h, float(dt), int(tr_num)))
+
+    def diff_tracers_ale(self, dt, tr_num):
+        _check(self.L.fgpu_diff_tracers_ale(self.h, float(dt), int(tr_num)))
+
+    def diff_ver_part_impl_ale(self, dt, tr_num):
+        _check(self.L.fgpu_diff_ver_part_impl_ale(self.h, float(dt), int(tr_num)))
+
+    def solve_tracers_ale(self, dt, nsteps=1):
+        _check(self.L.fgpu_solve_tracers_ale(self.h, float(dt), int(nsteps)))
+
+    def solve_tracers_ale_timed(self, dt, nsteps):
+        ms = C.c_double()
+        _check(self.L.fgpu_solve_tracers_ale_timed(self.h, float(dt), int(nsteps), C.byref(ms)))
+        return ms.value
+
+    def profile_enable(self, on=True):
+        _check(self.L.fgpu_profile_enable(self.h, int(bool(on))))
+
+    def profile(self):
+        n = _check(self.L.fgpu_profile_count(self.h))
+        out = {}
+        for i in range(n):
+            name = C.create_string_buffer(64)
+            ms = C.c_double()
+            cnt = C.c_int()
+            _check(self.L.fgpu_profile_read(self.h, i, name, 64, C.byref(ms), C.byref(cnt)))
+            out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def tracer_content(self, tr_num):
+        v = C.c_double()
+        _check(self.L.fgpu_tracer_content(self.h, int(tr_num), C.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        _check(self.L.fgpu_synchronize(self.h))
+
+    # ------------------------------------------------------------ SSH
+    def init_stiff_mat_ale(self, dt, g=9.81, alpha=1.0, theta=1.0):
+        _check(self.L.fgpu_init_stiff_mat_ale(self.h, float(dt), float(g), float(alpha), float(theta)))
+
+    def ssh_set_matrix(self, rowptr1, colind1, values):
+        rowptr1, colind1 = _i32(rowptr1), _i32(colind1)
+        _check(self.L.fgpu_ssh_set_matrix(self.h, len(rowptr1) - 1, len(colind1), _i(rowptr1), _i(colind1),
+                                          _d(_f64(values))))
+
+    def ssh_get_matrix(self, with_pr=False):
+        nza = C.c_int()
+        _check(self.L.fgpu_ssh_get_matrix(self.h, None, None, None, None, C.byref(nza)))
+        rp = np.zeros(self.N2 + 1, np.int32)
+        ci = np.zeros(nza.value, np.int32)
+        v = np.zeros(nza.value)
+        pr = np.zeros(nza.value) if with_pr else None
+        _check(self.L.fgpu_ssh_get_matrix(self.h, _i(rp), _i(ci), _d(v), _d(pr) if with_pr else None,
+                                          C.byref(nza)))
+        return rp, ci, v, pr
+
+    def ssh_solve_preconditioner(self, variant):
+        _check(self.L.fgpu_ssh_solve_preconditioner(self.h, int(variant)))
+
+    def ssh_solve_cg(self, x, rhs, soltol=1e-10, maxiter=2000):
+        x = _f64(x).copy()
+        st = CgStats()
+        _check(self.L.fgpu_ssh_solve_cg(self.h, _d(x), _d(_f64(rhs)), float(soltol), int(maxiter), C.byref(st)))
+        return x, st.as_dict()
+
+    def ssh_upload(self, x0, rhs):
+        _check(self.L.fgpu_ssh_upload(self.h, _d(_f64(x0)), _d(_f64(rhs))))
+
+    def ssh_solve_resident(self, soltol=1e-10, maxiter=2000, nrep=1):
+        st = CgStats()
+        _check(self.L.fgpu_ssh_solve_resident(self.h, float(soltol), int(maxiter), int(nrep), C.byref(st)))
+        return st.as_dict()
+
+    def ssh_download(self):
+        x = np.zeros(self.N2)
+        _check(self.L.fgpu_ssh_download(self.h, _d(x)))
+        return x
+
+
+def context_for(mesh, state=None, device=0, schemes=None):
+    """Context with mesh, state and tracers uploaded (helper for tests/bench)."""
+    ctx = Context(device)
+    ctx.mesh_init(mesh)
+    if state is not None:
+        ctx.set_state(state)
+        ctx.tracers_init(state.ntr)
+        for i in range(state.ntr):
+            sch = dict(hor="MFCT", ver="QR4C", lim="FCT", ph=0.0, pv=1.0)
+            if schemes:
+                sch.update(schemes[i] if isinstance(schemes, (list, tuple)) else schemes)
+            ctx.tracer_config(i + 1, int(state.ids[i]), **sch)
+            ctx.tracer_upload(i + 1, state.values[i], state.valuesold[i])
+    return ctx

@@ -1,0 +1,163 @@
+/*
+ * fesom_gpu.h -- C-ABI of the MI355X (gfx950) FESOM2 tracer-transport / SSH library.
+ *
+ * One context = one FESOM2 partition resident on one GPU. The mesh is uploaded
+ * once; fields stay in HBM between calls. Arrays are passed exactly as the
+ * Fortran derived types hold them: column-major, level index fastest, FP64,
+ * 1-based connectivity (edge_tri(2,e) <= 0 marks a boundary edge).
+ *
+ * Every function returns FGPU_OK (0) or a negative status; fgpu_last_error()
+ * gives the message of the last failure in the calling thread. A status of
+ * FGPU_ERR_SCHEME / FGPU_ERR_BC corresponds to the reference's par_ex abort
+ * (unknown advection scheme, unknown boundary-condition tracer ID); the caller
+ * decides whether to abort.
+ *
+ * The reference interfaces each entry point replaces are named beside it
+ * (paths relative to the FESOM2 source tree).
+ */
+#ifndef FESOM_GPU_H
+#define FESOM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGPU_OK 0
+#define FGPU_ERR_ARG (-1)
+#define FGPU_ERR_HIP (-2)
+#define FGPU_ERR_SCHEME (-3)
+#define FGPU_ERR_BC (-4)
+#define FGPU_ERR_STATE (-5)
+#define FGPU_ERR_NOMEM (-6)
+
+typedef struct fgpu_context fgpu_context;
+
+typedef struct {
+    int iters_last;      /* solverinfo%iters_last   (MOD_DYN.F90:13-46) */
+    int converged;       /* 1 if ||r||/||b|| < soltol was reached */
+    int nbreakdown;      /* solverinfo%nbreakdown   (solver.F90:259-266, 327-333) */
+    int nnegrz;          /* solverinfo%nnegrz       (solver.F90:319-325) */
+    double resid_last;   /* sqrt(r.r / nod2D)       (solver.F90:362) */
+    double rtol_last;    /* soltol*sqrt(b.b/nod2D)  (solver.F90:180) */
+    double ms;           /* device time of the solve (hipEvent), milliseconds */
+} fgpu_cg_stats;
+
+const char *fgpu_last_error(void);
+int fgpu_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+int fgpu_create(fgpu_context **ctx, int device);
+int fgpu_destroy(fgpu_context *ctx);
+int fgpu_synchronize(fgpu_context *ctx);
+
+/* ---- mesh: replaces the device side of mesh_setup (oce_mesh.F90:153) -----
+ * Upload-once 2-D connectivity, geometry and level numbers of t_mesh
+ * (MOD_MESH.F90:22-175). coord_nod2D(2,nod2D) radians; elem2D_nodes(3,elem2D),
+ * edges(2,edge2D), edge_tri(2,edge2D), nod_in_elem2D(kmax,nod2D) 1-based;
+ * nlevels/ulevels(elem2D), nlevels_nod2D/ulevels_nod2D/nlevels_nod2D_min/
+ * ulevels_nod2D_max(nod2D); elem_area, elem_cos(elem2D); edge_dxdy(2,edge2D);
+ * edge_cross_dxdy(4,edge2D); gradient_sca(6,elem2D).
+ * Also runs find_up_downwind_triangles (oce_muscl_adv.F90:162-352) and builds
+ * the node->edge gather lists and the SSH sparsity (oce_ale.F90:1440-1496). */
+int fgpu_mesh_init(fgpu_context *ctx, int nod2D, int elem2D, int edge2D, int nl, int kmax,
+                   double cyclic_length, const double *coord_nod2D, const int *elem2D_nodes,
+                   const int *edges, const int *edge_tri, const int *nod_in_elem2D,
+                   const int *nod_in_elem2D_num, const int *nlevels, const int *ulevels,
+                   const int *nlevels_nod2D, const int *ulevels_nod2D,
+                   const int *nlevels_nod2D_min, const int *ulevels_nod2D_max,
+                   const double *elem_area, const double *elem_cos, const double *edge_dxdy,
+                   const double *edge_cross_dxdy, const double *gradient_sca);
+
+/* 3-D geometry of the current ALE state (linfs: once; zstar: per step).
+ * area, areasvol, zbar_3d_n (nl,nod2D); hnode, hnode_new, Z_3d_n (nl-1,nod2D);
+ * helem (nl-1,elem2D); zbar_n_bot(nod2D); zbar_e_bot, zbar_e_srf(elem2D). */
+int fgpu_set_vertical(fgpu_context *ctx, const double *area, const double *areasvol,
+                      const double *hnode, const double *hnode_new, const double *helem,
+                      const double *zbar_3d_n, const double *Z_3d_n, const double *zbar_n_bot,
+                      const double *zbar_e_bot, const double *zbar_e_srf);
+
+/* ---- dynamics inputs: the vel/w/wi/we arguments of do_oce_adv_tra --------
+ * (oce_adv_tra_driver.F90:46-66); uv(2,nl-1,elem2D); w, w_e, w_i (nl,nod2D).
+ * use_wsplit must be 0 (adv_tra_vert_impl is not provided). */
+int fgpu_set_velocity(fgpu_context *ctx, const double *uv, const double *w, const double *w_e,
+                      const double *w_i, int use_wsplit);
+/* o_ARRAYS%Kv(nl,nod2D), read by diff_ver_part_impl_ale (oce_ale_tracer.F90:914) */
+int fgpu_set_kv(fgpu_context *ctx, const double *Kv);
+/* surface fluxes read by bc_surface (oce_ale_tracer.F90:1873-1881), (nod2D) each */
+int fgpu_set_surface_forcing(fgpu_context *ctx, const double *heat_flux, const double *water_flux,
+                             const double *virtual_salt, const double *relax_salt,
+                             const double *real_salt_flux, double is_nonlinfs);
+
+/* ---- tracers: t_tracer (MOD_TRACER.F90:10-109) ---------------------------
+ * num_tracers tracers; each configured with its namelist entry
+ * (config/namelist.tra:33-38): hor in {"MFCT","UPW1"}, ver in {"QR4C","UPW1"},
+ * lim in {"FCT","NON"} (trailing blanks ignored), tra_adv_ph/pv, AB_order 2|3,
+ * i_vert_diff. tr_num is 1-based like the reference. */
+int fgpu_tracers_init(fgpu_context *ctx, int num_tracers);
+int fgpu_tracer_config(fgpu_context *ctx, int tr_num, int ID, int AB_order, const char *tra_adv_hor,
+                       const char *tra_adv_ver, const char *tra_adv_lim, double tra_adv_ph,
+                       double tra_adv_pv, int i_vert_diff);
+/* values(nl-1,nod2D); valuesold(AB_order-1,nl-1,nod2D) */
+int fgpu_tracer_upload(fgpu_context *ctx, int tr_num, const double *values, const double *valuesold);
+int fgpu_tracer_download(fgpu_context *ctx, int tr_num, double *values, double *valuesold);
+/* Download a work array of the last per-tracer call: name is one of
+ * "valuesAB", "tr_xy", "edge_up_dn_grad", "fct_LO", "adv_flux_hor",
+ * "adv_flux_ver", "del_ttf_advhoriz", "del_ttf_advvert", "fct_plus", "fct_minus".
+ * Shapes follow t_tracer_work (MOD_TRACER.F90:33-67). */
+int fgpu_tracer_work_download(fgpu_context *ctx, int tr_num, const char *name, double *out, long n);
+/* edge_up_dn_tri(2,edge2D), 1-based, 0 = none */
+int fgpu_edge_up_dn_tri(fgpu_context *ctx, int *out);
+
+/* ---- per-tracer entry points (reference call surface) --------------------- */
+/* init_tracers_AB (oce_tracer_mod.F90:13-143) */
+int fgpu_init_tracers_AB(fgpu_context *ctx, int tr_num);
+/* do_oce_adv_tra (oce_adv_tra_driver.F90:46-490): leaves del_ttf_advhoriz and
+ * del_ttf_advvert in device memory, like tracers%work. */
+int fgpu_do_oce_adv_tra(fgpu_context *ctx, double dt, int tr_num);
+/* diff_tracers_ale (oce_ale_tracer.F90:533-793) incl. the del_ttf sum of
+ * solve_tracers_ale (:318-322): T* update and, if i_vert_diff, implicit diffusion */
+int fgpu_diff_tracers_ale(fgpu_context *ctx, double dt, int tr_num);
+/* diff_ver_part_impl_ale (oce_ale_tracer.F90:907-1435) alone */
+int fgpu_diff_ver_part_impl_ale(fgpu_context *ctx, double dt, int tr_num);
+
+/* ---- batched step: replaces the tracer loop of solve_tracers_ale
+ * (oce_ale_tracer.F90:280-385) and the salinity clamp (:502-514). All tracers
+ * advance together; nsteps > 1 replays the captured step graph. */
+int fgpu_solve_tracers_ale(fgpu_context *ctx, double dt, int nsteps);
+/* Same, timed with hipEvents on the library stream; *ms = elapsed device time. */
+int fgpu_solve_tracers_ale_timed(fgpu_context *ctx, double dt, int nsteps, double *ms);
+/* Per-kernel timing of the batched step: when enabled, every kernel launch of
+ * fgpu_solve_tracers_ale is bracketed by hipEvents on the library stream. */
+int fgpu_profile_enable(fgpu_context *ctx, int on);
+int fgpu_profile_count(fgpu_context *ctx);
+int fgpu_profile_read(fgpu_context *ctx, int idx, char *name, int name_len, double *total_ms, int *launches);
+/* Tracer content sum_n sum_k areasvol*hnode_new*T (deterministic device sum). */
+int fgpu_tracer_content(fgpu_context *ctx, int tr_num, double *content);
+
+/* ---- SSH solve ------------------------------------------------------------ */
+/* init_stiff_mat_ale (oce_ale.F90:1393-1582): assemble on the device from the
+ * uploaded mesh (factor g*dt*alpha*theta). */
+int fgpu_init_stiff_mat_ale(fgpu_context *ctx, double dt, double g, double alpha, double theta);
+/* Alternatively hand over an assembled ssh_stiff (MOD_MESH.F90:11-20), 1-based,
+ * diagonal first in every row. */
+int fgpu_ssh_set_matrix(fgpu_context *ctx, int dim, int nza, const int *rowptr, const int *colind,
+                        const double *values);
+int fgpu_ssh_get_matrix(fgpu_context *ctx, int *rowptr, int *colind, double *values,
+                        double *pr_values, int *nza);
+/* ssh_solve_preconditioner (solver.F90:31-117): variant 0 and 1 as in the
+ * reference, 2 = diagonal (Jacobi) only. */
+int fgpu_ssh_solve_preconditioner(fgpu_context *ctx, int precond_variant);
+/* ssh_solve_cg (solver.F90:120-385): x (nod2D) in/out (warm start), rhs (nod2D). */
+int fgpu_ssh_solve_cg(fgpu_context *ctx, double *x, const double *rhs, double soltol, int maxiter,
+                      fgpu_cg_stats *stats);
+/* Device-resident variant for repeated solves: upload rhs/x0 once with
+ * fgpu_ssh_upload, solve nrep times from the same x0, download with fgpu_ssh_download. */
+int fgpu_ssh_upload(fgpu_context *ctx, const double *x0, const double *rhs);
+int fgpu_ssh_solve_resident(fgpu_context *ctx, double soltol, int maxiter, int nrep,
+                            fgpu_cg_stats *stats);
+int fgpu_ssh_download(fgpu_context *ctx, double *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

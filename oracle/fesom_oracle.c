@@ -1,0 +1,946 @@
+/*
+ * fesom_oracle.c -- CPU oracle (TEST INFRASTRUCTURE, never the product path).
+ *
+ * A single-threaded C restatement of the reference's tracer-transport step and
+ * SSH solve. Each routine names the reference file:line it follows. Loops run
+ * in the reference's serial order (edges ascending, elements in nod_in_elem2D
+ * order), so the edge->node scatters sum in the order a single-threaded
+ * reference run uses. Floating-point expressions keep the reference's
+ * evaluation order; build with -ffp-contract=off.
+ *
+ * fo_set_mutation(m) plants one deliberate mistake (m != 0) so the pinning
+ * tests can show that they catch it:
+ *   1  AB2 coefficient drops epsilon                      (init_tracers_AB)
+ *   2  y-gradient uses the x coefficients                 (tracer_gradient_elements)
+ *   3  upwind-triangle search does not negate the edge    (find_up_downwind_triangles)
+ *   4  shared levels take the upwind gradient for both    (fill_up_dn_grad)
+ *   5  second-element volume flux with the wrong sign     (adv_tra_hor_upw1)
+ *   6  MFCT reconstruction weight 3 instead of 2          (adv_tra_hor_mfct)
+ *   7  QR4C swaps the upper and lower slopes              (adv_tra_ver_qr4c)
+ *   8  FCT vertical limiter uses R+ of the upper cell for an upward flux
+ *   10 flux2dtracer divides node 2 by node 1's volume     (oce_tra_adv_flux2dtracer)
+ *   12 implicit diffusion uses area(nz) in c(nz)          (diff_ver_part_impl_ale)
+ *   13 Thomas back-substitution adds instead of subtracts
+ *   14 SSH diagonal adds areasvol*dt instead of /dt       (init_stiff_mat_ale)
+ *   15 CG beta inverted                                   (ssh_solve_cg)
+ *   16 preconditioner variant 1 off-diagonal sign flipped (ssh_solve_preconditioner)
+ *   17 upw1 vertical flux uses T(k) on both sides         (adv_tra_ver_upw1)
+ *   18 FCT b1 drops the vertical contribution to P+/P-    (oce_tra_adv_fct)
+ */
+#include "fesom_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static int g_mut = 0;
+void fo_set_mutation(int m) { g_mut = m; }
+int fo_get_mutation(void) { return g_mut; }
+
+#define R_EARTH 6367500.0
+#define G_ACC 9.81
+#define EPS_AB 0.1
+#define VCPW 4.2e6
+
+#define LAY(n, k) ((size_t)(n) * (size_t)L + (size_t)((k) - 1))
+#define INT(n, k) ((size_t)(n) * (size_t)NL + (size_t)((k) - 1))
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* ------------------------------------------------------------------------ */
+/* find_up_downwind_triangles: oce_muscl_adv.F90:162-352                     */
+/* ------------------------------------------------------------------------ */
+static double trim_cyc(double b, double Lc)
+{   /* gen_modules_rotate_grid.F90:204-211 */
+    if (b > Lc / 2.0) b = b - Lc;
+    if (b < -Lc / 2.0) b = b + Lc;
+    return b;
+}
+
+static int updn_search(const fo_mesh *m, int node, const double x[2])
+{
+    const double Lc = m->cyclic_length;
+    int found = -1;
+    for (int k = 0; k < m->nod_in_elem2D_num[node]; ++k) {
+        int elem = m->nod_in_elem2D[(size_t)node * m->kmax + k];
+        const int *en = m->elem2D_nodes + 3 * (size_t)elem;
+        int a0, a1, a2;
+        if (en[0] == node) { a0 = 0; a1 = 1; a2 = 2; }
+        else if (en[1] == node) { a0 = 1; a1 = 0; a2 = 2; }
+        else { a0 = 2; a1 = 0; a2 = 1; }
+        double b[2], c[2];
+        b[0] = m->coord_nod2D[2 * en[a1]] - m->coord_nod2D[2 * en[a0]];
+        b[1] = m->coord_nod2D[2 * en[a1] + 1] - m->coord_nod2D[2 * en[a0] + 1];
+        c[0] = m->coord_nod2D[2 * en[a2]] - m->coord_nod2D[2 * en[a0]];
+        c[1] = m->coord_nod2D[2 * en[a2] + 1] - m->coord_nod2D[2 * en[a0] + 1];
+        b[0] = trim_cyc(b[0], Lc);
+        c[0] = trim_cyc(c[0], Lc);
+        double cr = c[0] * c[0] + c[1] * c[1];
+        double bx = (b[0] * c[0] + b[1] * c[1]) / cr;
+        double by = (-b[0] * c[1] + b[1] * c[0]) / cr;
+        double xx = (x[0] * c[0] + x[1] * c[1]) / cr;
+        double xy = (-x[0] * c[1] + x[1] * c[0]) / cr;
+        double ab = atan2(by, bx);
+        double ax = atan2(xy, xx);
+        /* every branch is a Fortran CYCLE: the last matching element wins */
+        if ((ab > 0.0) && (ax > 0.0) && (ax < ab)) { found = elem; continue; }
+        if ((ab < 0.0) && (ax < 0.0) && (ax > ab)) { found = elem; continue; }
+        if ((ab == ax) || (ax == 0.0)) { found = elem; continue; }
+    }
+    return found;
+}
+
+int fo_find_up_downwind_triangles(const fo_mesh *m, int *up_dn)
+{
+    const double Lc = m->cyclic_length;
+    for (int e = 0; e < m->edge2D; ++e) {
+        int n1 = m->edges[2 * e], n2 = m->edges[2 * e + 1];
+        double x[2];
+        x[0] = m->coord_nod2D[2 * n2] - m->coord_nod2D[2 * n1];
+        x[1] = m->coord_nod2D[2 * n2 + 1] - m->coord_nod2D[2 * n1 + 1];
+        if (x[0] > Lc / 2.0) x[0] = x[0] - Lc;
+        if (x[0] < -Lc / 2.0) x[0] = x[0] + Lc;
+        double xm[2] = {-x[0], -x[1]};
+        if (g_mut == 3) { xm[0] = x[0]; xm[1] = x[1]; }
+        up_dn[2 * e] = updn_search(m, n1, xm);
+        up_dn[2 * e + 1] = updn_search(m, n2, x);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* tracer_gradient_elements: oce_tracer_mod.F90:147-188                      */
+/* ------------------------------------------------------------------------ */
+void fo_tracer_gradient_elements(const fo_mesh *m, const double *ttf, double *tr_xy)
+{
+    const int L = m->nl - 1;
+    for (int el = 0; el < m->elem2D; ++el) {
+        const int *en = m->elem2D_nodes + 3 * (size_t)el;
+        const double *gs = m->gradient_sca + 6 * (size_t)el;
+        for (int nz = m->ulevels[el]; nz <= m->nlevels[el] - 1; ++nz) {
+            double t0 = ttf[LAY(en[0], nz)], t1 = ttf[LAY(en[1], nz)], t2 = ttf[LAY(en[2], nz)];
+            double gx = gs[0] * t0 + gs[1] * t1 + gs[2] * t2;
+            double gy = gs[3] * t0 + gs[4] * t1 + gs[5] * t2;
+            if (g_mut == 2) gy = gs[0] * t0 + gs[1] * t1 + gs[2] * t2;
+            tr_xy[LAY(el, nz) * 2 + 0] = gx;
+            tr_xy[LAY(el, nz) * 2 + 1] = gy;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* fill_up_dn_grad: oce_muscl_adv.F90:356-525                                */
+/* ------------------------------------------------------------------------ */
+static void node_mean_grad(const fo_mesh *m, const double *tr_xy, int node, int nz,
+                           double *gx, double *gy)
+{
+    const int L = m->nl - 1;
+    double tvol = 0.0, tx = 0.0, ty = 0.0;
+    for (int k = 0; k < m->nod_in_elem2D_num[node]; ++k) {
+        int elem = m->nod_in_elem2D[(size_t)node * m->kmax + k];
+        if (m->nlevels[elem] - 1 < nz || nz < m->ulevels[elem]) continue;
+        tvol = tvol + m->elem_area[elem];
+        tx = tx + tr_xy[LAY(elem, nz) * 2 + 0] * m->elem_area[elem];
+        ty = ty + tr_xy[LAY(elem, nz) * 2 + 1] * m->elem_area[elem];
+    }
+    *gx = tx / tvol;
+    *gy = ty / tvol;
+}
+
+void fo_fill_up_dn_grad(const fo_mesh *m, const double *tr_xy, double *grad)
+{
+    const int L = m->nl - 1;
+    for (int e = 0; e < m->edge2D; ++e) {
+        int n1 = m->edges[2 * e], n2 = m->edges[2 * e + 1];
+        int up = m->edge_up_dn_tri[2 * e], dn = m->edge_up_dn_tri[2 * e + 1];
+        double gx, gy;
+        if (up >= 0 && dn >= 0) {
+            int nzmin = imax(m->ulevels_nod2D_max[n1], m->ulevels_nod2D_max[n2]);
+            int nzmax = imin(m->nlevels_nod2D_min[n1], m->nlevels_nod2D_min[n2]);
+            for (int nz = m->ulevels_nod2D[n1]; nz <= nzmin - 1; ++nz) {
+                node_mean_grad(m, tr_xy, n1, nz, &gx, &gy);
+                grad[LAY(e, nz) * 4 + 0] = gx;
+                grad[LAY(e, nz) * 4 + 2] = gy;
+            }
+            for (int nz = m->ulevels_nod2D[n2]; nz <= nzmin - 1; ++nz) {
+                node_mean_grad(m, tr_xy, n2, nz, &gx, &gy);
+                grad[LAY(e, nz) * 4 + 1] = gx;
+                grad[LAY(e, nz) * 4 + 3] = gy;
+            }
+            for (int nz = nzmin; nz <= nzmax - 1; ++nz) {
+                int dn_use = (g_mut == 4) ? up : dn;
+                grad[LAY(e, nz) * 4 + 0] = tr_xy[LAY(up, nz) * 2 + 0];
+                grad[LAY(e, nz) * 4 + 1] = tr_xy[LAY(dn_use, nz) * 2 + 0];
+                grad[LAY(e, nz) * 4 + 2] = tr_xy[LAY(up, nz) * 2 + 1];
+                grad[LAY(e, nz) * 4 + 3] = tr_xy[LAY(dn_use, nz) * 2 + 1];
+            }
+            for (int nz = nzmax; nz <= m->nlevels_nod2D[n1] - 1; ++nz) {
+                node_mean_grad(m, tr_xy, n1, nz, &gx, &gy);
+                grad[LAY(e, nz) * 4 + 0] = gx;
+                grad[LAY(e, nz) * 4 + 2] = gy;
+            }
+            for (int nz = nzmax; nz <= m->nlevels_nod2D[n2] - 1; ++nz) {
+                node_mean_grad(m, tr_xy, n2, nz, &gx, &gy);
+                grad[LAY(e, nz) * 4 + 1] = gx;
+                grad[LAY(e, nz) * 4 + 3] = gy;
+            }
+        } else {
+            for (int nz = m->ulevels_nod2D[n1]; nz <= m->nlevels_nod2D[n1] - 1; ++nz) {
+                node_mean_grad(m, tr_xy, n1, nz, &gx, &gy);
+                grad[LAY(e, nz) * 4 + 0] = gx;
+                grad[LAY(e, nz) * 4 + 2] = gy;
+            }
+            for (int nz = m->ulevels_nod2D[n2]; nz <= m->nlevels_nod2D[n2] - 1; ++nz) {
+                node_mean_grad(m, tr_xy, n2, nz, &gx, &gy);
+                grad[LAY(e, nz) * 4 + 1] = gx;
+                grad[LAY(e, nz) * 4 + 3] = gy;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* init_tracers_AB: oce_tracer_mod.F90:13-143                                */
+/* ------------------------------------------------------------------------ */
+int fo_init_tracers_AB(const fo_mesh *m, fo_tracer *t, fo_work *w)
+{
+    const int L = m->nl - 1;
+    const size_t n3 = (size_t)m->nod2D * L;
+    memset(w->del_ttf, 0, n3 * sizeof(double));
+    memset(w->del_ttf_advhoriz, 0, n3 * sizeof(double));
+    memset(w->del_ttf_advvert, 0, n3 * sizeof(double));
+    if (t->AB_order == 2) {
+        double c1 = (g_mut == 1) ? 1.5 : (1.5 + EPS_AB);
+        for (size_t i = 0; i < n3; ++i)
+            t->valuesAB[i] = -(0.5 + EPS_AB) * t->valuesold[i] + c1 * t->values[i];
+        for (size_t i = 0; i < n3; ++i) t->valuesold[i] = t->values[i];
+    } else if (t->AB_order == 3) {
+        for (size_t i = 0; i < n3; ++i) {
+            double v = 5.0 * t->valuesold[2 * i + 1] - 16.0 * t->valuesold[2 * i] + 23.0 * t->values[i];
+            t->valuesAB[i] = v / 12.0;
+        }
+        for (size_t i = 0; i < n3; ++i) {
+            t->valuesold[2 * i + 1] = t->valuesold[2 * i];
+            t->valuesold[2 * i] = t->values[i];
+        }
+    } else {
+        return -1;   /* par_ex: AB order must be 2 or 3 (oce_tracer_mod.F90:75-91) */
+    }
+    /* gradients from values, not valuesAB (oce_tracer_mod.F90:126-131) */
+    fo_tracer_gradient_elements(m, t->values, w->tr_xy);
+    fo_fill_up_dn_grad(m, w->tr_xy, w->edge_up_dn_grad);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* adv_tra_hor_upw1: oce_adv_tra_hor.F90:64-257                              */
+/* ------------------------------------------------------------------------ */
+typedef struct { int n1, n2, el1, el2, nl1, nu1, nl2, nu2; } edge_info;
+
+static edge_info get_edge(const fo_mesh *m, int e)
+{
+    edge_info q;
+    q.n1 = m->edges[2 * e];
+    q.n2 = m->edges[2 * e + 1];
+    q.el1 = m->edge_tri[2 * e];
+    q.el2 = m->edge_tri[2 * e + 1];
+    q.nl1 = m->nlevels[q.el1] - 1;
+    q.nu1 = m->ulevels[q.el1];
+    q.nl2 = 0;
+    q.nu2 = 0;
+    if (q.el2 >= 0) {
+        q.nl2 = m->nlevels[q.el2] - 1;
+        q.nu2 = m->ulevels[q.el2];
+    }
+    return q;
+}
+
+static inline double upw1_flux(double t1, double t2, double vflux, double fin)
+{
+    return -0.5 * (t1 * (vflux + fabs(vflux)) + t2 * (vflux - fabs(vflux))) - fin;
+}
+
+void fo_adv_tra_hor_upw1(const fo_mesh *m, const double *vel, const double *ttf, double *flux, int init_zero)
+{
+    const int L = m->nl - 1;
+    if (init_zero) memset(flux, 0, (size_t)m->edge2D * L * sizeof(double));
+    for (int e = 0; e < m->edge2D; ++e) {
+        edge_info q = get_edge(m, e);
+        double dX1 = m->edge_cross_dxdy[4 * e + 0], dY1 = m->edge_cross_dxdy[4 * e + 1];
+        double dX2 = 0.0, dY2 = 0.0;
+        if (q.el2 >= 0) { dX2 = m->edge_cross_dxdy[4 * e + 2]; dY2 = m->edge_cross_dxdy[4 * e + 3]; }
+        int nl12 = imin(q.nl1, q.nl2);
+        int nu12 = imax(q.nu1, q.nu2);
+        const double s2 = (g_mut == 5) ? -1.0 : 1.0;
+#define V1(c, nz) vel[(LAY(q.el1, nz)) * 2 + (c)]
+#define V2(c, nz) vel[(LAY(q.el2, nz)) * 2 + (c)]
+#define H1(nz) m->helem[LAY(q.el1, nz)]
+#define H2(nz) m->helem[LAY(q.el2, nz)]
+        for (int nz = q.nu1; nz <= nu12 - 1; ++nz) {               /* (A) */
+            double vflux = (-V1(1, nz) * dX1 + V1(0, nz) * dY1) * H1(nz);
+            flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
+        }
+        if (q.nu2 > 0) {
+            for (int nz = q.nu2; nz <= nu12 - 1; ++nz) {            /* (B) */
+                double vflux = s2 * (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
+                flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
+            }
+        }
+        for (int nz = nu12; nz <= nl12; ++nz) {                    /* (C) */
+            double vflux = (-V1(1, nz) * dX1 + V1(0, nz) * dY1) * H1(nz)
+                         + s2 * (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
+            flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
+        }
+        for (int nz = nl12 + 1; nz <= q.nl1; ++nz) {               /* (D) */
+            double vflux = (-V1(1, nz) * dX1 + V1(0, nz) * dY1) * H1(nz);
+            flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
+        }
+        for (int nz = nl12 + 1; nz <= q.nl2; ++nz) {               /* (E) */
+            double vflux = s2 * (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
+            flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* adv_tra_hor_mfct: oce_adv_tra_hor.F90:546-834                             */
+/* ------------------------------------------------------------------------ */
+void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf, double num_ord,
+                         double *flux, const double *grad, int init_zero)
+{
+    const int L = m->nl - 1;
+    const double w2 = (g_mut == 6) ? 3.0 : 2.0;
+    if (init_zero) memset(flux, 0, (size_t)m->edge2D * L * sizeof(double));
+    for (int e = 0; e < m->edge2D; ++e) {
+        edge_info q = get_edge(m, e);
+        double dX1 = m->edge_cross_dxdy[4 * e + 0], dY1 = m->edge_cross_dxdy[4 * e + 1];
+        double dX2 = 0.0, dY2 = 0.0;
+        double a = R_EARTH * m->elem_cos[q.el1];
+        if (q.el2 >= 0) {
+            dX2 = m->edge_cross_dxdy[4 * e + 2];
+            dY2 = m->edge_cross_dxdy[4 * e + 3];
+            a = 0.5 * (a + R_EARTH * m->elem_cos[q.el2]);
+        }
+        const double edx = m->edge_dxdy[2 * e], edy = m->edge_dxdy[2 * e + 1];
+        int nl12 = imin(q.nl1, q.nl2);
+        int nu12 = imax(q.nu1, q.nu2);
+        for (int pass = 0; pass < 5; ++pass) {
+            int lo, hi;
+            if (pass == 0) { lo = q.nu1; hi = nu12 - 1; }
+            else if (pass == 1) { if (q.nu2 <= 0) continue; lo = q.nu2; hi = nu12 - 1; }
+            else if (pass == 2) { lo = nu12; hi = nl12; }
+            else if (pass == 3) { lo = nl12 + 1; hi = q.nl1; }
+            else { lo = nl12 + 1; hi = q.nl2; }
+            for (int nz = lo; nz <= hi; ++nz) {
+                const double T1 = ttf[LAY(q.n1, nz)], T2 = ttf[LAY(q.n2, nz)];
+                const double *g = grad + LAY(e, nz) * 4;
+                double Tmean2 = T2 - (w2 * (T2 - T1) + edx * a * g[1] + edy * R_EARTH * g[3]) / 6.0;
+                double Tmean1 = T1 + (w2 * (T2 - T1) + edx * a * g[0] + edy * R_EARTH * g[2]) / 6.0;
+                double vflux;
+                if (pass == 0 || pass == 3)
+                    vflux = (-V1(1, nz) * dX1 + V1(0, nz) * dY1) * H1(nz);
+                else if (pass == 1 || pass == 4)
+                    vflux = (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
+                else
+                    vflux = (-V1(1, nz) * dX1 + V1(0, nz) * dY1) * H1(nz)
+                          + (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
+                double cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
+                flux[LAY(e, nz)] = -0.5 * (1.0 - num_ord) * cHO
+                                   - vflux * num_ord * 0.5 * (Tmean1 + Tmean2) - flux[LAY(e, nz)];
+            }
+        }
+    }
+#undef V1
+#undef V2
+#undef H1
+#undef H2
+}
+
+/* ------------------------------------------------------------------------ */
+/* adv_tra_ver_upw1: oce_adv_tra_ver.F90:244-328                             */
+/* ------------------------------------------------------------------------ */
+void fo_adv_tra_ver_upw1(const fo_mesh *m, const double *W, const double *ttf, double *flux, int init_zero)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    if (init_zero) memset(flux, 0, (size_t)m->nod2D * NL * sizeof(double));
+    for (int n = 0; n < m->nod2D; ++n) {
+        int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
+        int nz = nzmin;
+        flux[INT(n, nz)] = -W[INT(n, nz)] * ttf[LAY(n, nz)] * m->area[INT(n, nz)] - flux[INT(n, nz)];
+        nz = nzmax;
+        flux[INT(n, nz)] = 0.0 - flux[INT(n, nz)];
+        for (nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+            double w = W[INT(n, nz)];
+            double tup = (g_mut == 17) ? ttf[LAY(n, nz)] : ttf[LAY(n, nz - 1)];
+            flux[INT(n, nz)] = -0.5 * (ttf[LAY(n, nz)] * (w + fabs(w)) + tup * (w - fabs(w)))
+                               * m->area[INT(n, nz)] - flux[INT(n, nz)];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* adv_tra_ver_qr4c: oce_adv_tra_ver.F90:332-434                             */
+/* ------------------------------------------------------------------------ */
+void fo_adv_tra_ver_qr4c(const fo_mesh *m, const double *W, const double *ttf, double num_ord,
+                         double *flux, int init_zero)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    if (init_zero) memset(flux, 0, (size_t)m->nod2D * NL * sizeof(double));
+    for (int n = 0; n < m->nod2D; ++n) {
+        int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
+        const double *T = ttf + LAY(n, 1);
+        const double *Zn = m->Z_3d_n + LAY(n, 1);
+        const double *zb = m->zbar_3d_n + INT(n, 1);
+        const double *w = W + INT(n, 1);
+        const double *ar = m->area + INT(n, 1);
+        double *f = flux + INT(n, 1);
+        /* 1-based level k lives at index k-1 */
+#define I(k) ((k) - 1)
+        int nz = nzmin;
+        f[I(nz)] = -T[I(nz)] * w[I(nz)] * ar[I(nz)] - f[I(nz)];
+        nz = nzmin + 1;
+        f[I(nz)] = -0.5 * (T[I(nz - 1)] + T[I(nz)]) * w[I(nz)] * ar[I(nz)] - f[I(nz)];
+        nz = nzmax - 1;
+        f[I(nz)] = -0.5 * (T[I(nz - 1)] + T[I(nz)]) * w[I(nz)] * ar[I(nz)] - f[I(nz)];
+        nz = nzmax;
+        f[I(nz)] = 0.0 - f[I(nz)];
+        for (nz = nzmin + 2; nz <= nzmax - 2; ++nz) {
+            double qc = (T[I(nz - 1)] - T[I(nz)]) / (Zn[I(nz - 1)] - Zn[I(nz)]);
+            double qu = (T[I(nz)] - T[I(nz + 1)]) / (Zn[I(nz)] - Zn[I(nz + 1)]);
+            double qd = (T[I(nz - 2)] - T[I(nz - 1)]) / (Zn[I(nz - 2)] - Zn[I(nz - 1)]);
+            if (g_mut == 7) { double s = qu; qu = qd; qd = s; }
+            double Tmean1 = T[I(nz)] + (2 * qc + qu) * (zb[I(nz)] - Zn[I(nz)]) / 3.0;
+            double Tmean2 = T[I(nz - 1)] + (2 * qc + qd) * (zb[I(nz)] - Zn[I(nz - 1)]) / 3.0;
+            double ww = w[I(nz)];
+            double Tmean = (ww + fabs(ww)) * Tmean1 + (ww - fabs(ww)) * Tmean2;
+            f[I(nz)] = (-0.5 * (1.0 - num_ord) * Tmean - num_ord * (0.5 * (Tmean1 + Tmean2)) * ww)
+                       * ar[I(nz)] - f[I(nz)];
+        }
+#undef I
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* oce_tra_adv_fct: oce_adv_tra_fct.F90:72-512                               */
+/* ------------------------------------------------------------------------ */
+void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const double *lo,
+                        double *adf_h, double *adf_v, double *fct_ttf_min, double *fct_ttf_max,
+                        double *fct_plus, double *fct_minus)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    const double flux_eps = 1e-16, bignumber = 1e3;
+    double *aux = (double *)calloc((size_t)m->elem2D * L * 2, sizeof(double));
+    double *tvmax = (double *)calloc((size_t)m->nod2D * L, sizeof(double));
+    double *tvmin = (double *)calloc((size_t)m->nod2D * L, sizeof(double));
+    /* a1 */
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
+            fct_ttf_max[LAY(n, nz)] = dmax(lo[LAY(n, nz)], ttf[LAY(n, nz)]);
+            fct_ttf_min[LAY(n, nz)] = dmin(lo[LAY(n, nz)], ttf[LAY(n, nz)]);
+        }
+    /* a2 */
+    for (int el = 0; el < m->elem2D; ++el) {
+        const int *en = m->elem2D_nodes + 3 * (size_t)el;
+        int nu1 = m->ulevels[el], nl1 = m->nlevels[el];
+        for (int nz = 1; nz <= nu1 - 1; ++nz) {
+            aux[LAY(el, nz) * 2 + 0] = -bignumber;
+            aux[LAY(el, nz) * 2 + 1] = bignumber;
+        }
+        for (int nz = nu1; nz <= nl1 - 1; ++nz) {
+            aux[LAY(el, nz) * 2 + 0] = dmax(dmax(fct_ttf_max[LAY(en[0], nz)], fct_ttf_max[LAY(en[1], nz)]),
+                                            fct_ttf_max[LAY(en[2], nz)]);
+            aux[LAY(el, nz) * 2 + 1] = dmin(dmin(fct_ttf_min[LAY(en[0], nz)], fct_ttf_min[LAY(en[1], nz)]),
+                                            fct_ttf_min[LAY(en[2], nz)]);
+        }
+        for (int nz = nl1; nz <= L; ++nz) {
+            aux[LAY(el, nz) * 2 + 0] = -bignumber;
+            aux[LAY(el, nz) * 2 + 1] = bignumber;
+        }
+    }
+    /* a3 */
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int *ne = m->nod_in_elem2D + (size_t)n * m->kmax;
+        for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
+            double vmax = aux[LAY(ne[0], nz) * 2 + 0];
+            double vmin = aux[LAY(ne[0], nz) * 2 + 1];
+            for (int k = 1; k < m->nod_in_elem2D_num[n]; ++k) {
+                vmax = dmax(vmax, aux[LAY(ne[k], nz) * 2 + 0]);
+                vmin = dmin(vmin, aux[LAY(ne[k], nz) * 2 + 1]);
+            }
+            tvmax[LAY(n, nz)] = vmax;
+            tvmin[LAY(n, nz)] = vmin;
+        }
+    }
+    for (int n = 0; n < m->nod2D; ++n) {
+        int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
+        fct_ttf_max[LAY(n, nu1)] = tvmax[LAY(n, nu1)] - lo[LAY(n, nu1)];
+        fct_ttf_min[LAY(n, nu1)] = tvmin[LAY(n, nu1)] - lo[LAY(n, nu1)];
+        for (int nz = nu1 + 1; nz <= nl1 - 2; ++nz) {
+            fct_ttf_max[LAY(n, nz)] = dmax(dmax(tvmax[LAY(n, nz - 1)], tvmax[LAY(n, nz)]), tvmax[LAY(n, nz + 1)])
+                                      - lo[LAY(n, nz)];
+            fct_ttf_min[LAY(n, nz)] = dmin(dmin(tvmin[LAY(n, nz - 1)], tvmin[LAY(n, nz)]), tvmin[LAY(n, nz + 1)])
+                                      - lo[LAY(n, nz)];
+        }
+        int nz = nl1 - 1;
+        fct_ttf_max[LAY(n, nz)] = tvmax[LAY(n, nz)] - lo[LAY(n, nz)];
+        fct_ttf_min[LAY(n, nz)] = tvmin[LAY(n, nz)] - lo[LAY(n, nz)];
+    }
+    /* b1 */
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
+            fct_plus[LAY(n, nz)] = 0.0;
+            fct_minus[LAY(n, nz)] = 0.0;
+        }
+    if (g_mut != 18) {
+        for (int n = 0; n < m->nod2D; ++n)
+            for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
+                fct_plus[LAY(n, nz)] = fct_plus[LAY(n, nz)]
+                    + (dmax(0.0, adf_v[INT(n, nz)]) + dmax(0.0, -adf_v[INT(n, nz + 1)]));
+                fct_minus[LAY(n, nz)] = fct_minus[LAY(n, nz)]
+                    + (dmin(0.0, adf_v[INT(n, nz)]) + dmin(0.0, -adf_v[INT(n, nz + 1)]));
+            }
+    }
+    for (int e = 0; e < m->edge2D; ++e) {
+        edge_info q = get_edge(m, e);
+        int nl12 = imax(q.nl1, q.nl2);
+        int nu12 = q.nu1;
+        if (q.nu2 > 0) nu12 = imin(q.nu1, q.nu2);
+        for (int nz = nu12; nz <= nl12; ++nz) {
+            fct_plus[LAY(q.n1, nz)] = fct_plus[LAY(q.n1, nz)] + dmax(0.0, adf_h[LAY(e, nz)]);
+            fct_minus[LAY(q.n1, nz)] = fct_minus[LAY(q.n1, nz)] + dmin(0.0, adf_h[LAY(e, nz)]);
+        }
+        for (int nz = nu12; nz <= nl12; ++nz) {
+            fct_plus[LAY(q.n2, nz)] = fct_plus[LAY(q.n2, nz)] + dmax(0.0, -adf_h[LAY(e, nz)]);
+            fct_minus[LAY(q.n2, nz)] = fct_minus[LAY(q.n2, nz)] + dmin(0.0, -adf_h[LAY(e, nz)]);
+        }
+    }
+    /* b2 */
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
+            double flux = fct_plus[LAY(n, nz)] * dt / m->areasvol[INT(n, nz)] / m->hnode_new[LAY(n, nz)] + flux_eps;
+            fct_plus[LAY(n, nz)] = dmin(1.0, fct_ttf_max[LAY(n, nz)] / flux);
+            flux = fct_minus[LAY(n, nz)] * dt / m->areasvol[INT(n, nz)] / m->hnode_new[LAY(n, nz)] - flux_eps;
+            fct_minus[LAY(n, nz)] = dmin(1.0, fct_ttf_min[LAY(n, nz)] / flux);
+        }
+    /* b3 vertical */
+    for (int n = 0; n < m->nod2D; ++n) {
+        int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
+        int nz = nu1;
+        double ae = 1.0;
+        double flux = adf_v[INT(n, nz)];
+        if (flux >= 0.0) ae = dmin(ae, fct_plus[LAY(n, nz)]);
+        else ae = dmin(ae, fct_minus[LAY(n, nz)]);
+        adf_v[INT(n, nz)] = ae * adf_v[INT(n, nz)];
+        for (nz = nu1 + 1; nz <= nl1 - 1; ++nz) {
+            ae = 1.0;
+            flux = adf_v[INT(n, nz)];
+            if (flux >= 0.0) {
+                ae = dmin(ae, (g_mut == 8) ? fct_plus[LAY(n, nz - 1)] : fct_minus[LAY(n, nz - 1)]);
+                ae = dmin(ae, fct_plus[LAY(n, nz)]);
+            } else {
+                ae = dmin(ae, fct_plus[LAY(n, nz - 1)]);
+                ae = dmin(ae, fct_minus[LAY(n, nz)]);
+            }
+            adf_v[INT(n, nz)] = ae * adf_v[INT(n, nz)];
+        }
+    }
+    /* b3 horizontal */
+    for (int e = 0; e < m->edge2D; ++e) {
+        edge_info q = get_edge(m, e);
+        int nl12 = imax(q.nl1, q.nl2);
+        int nu12 = q.nu1;
+        if (q.nu2 > 0) nu12 = imin(q.nu1, q.nu2);
+        for (int nz = nu12; nz <= nl12; ++nz) {
+            double ae = 1.0;
+            double flux = adf_h[LAY(e, nz)];
+            if (flux >= 0.0) {
+                ae = dmin(ae, fct_plus[LAY(q.n1, nz)]);
+                ae = dmin(ae, fct_minus[LAY(q.n2, nz)]);
+            } else {
+                ae = dmin(ae, fct_minus[LAY(q.n1, nz)]);
+                ae = dmin(ae, fct_plus[LAY(q.n2, nz)]);
+            }
+            adf_h[LAY(e, nz)] = ae * adf_h[LAY(e, nz)];
+        }
+    }
+    free(aux);
+    free(tvmax);
+    free(tvmin);
+}
+
+/* ------------------------------------------------------------------------ */
+/* oce_tra_adv_flux2dtracer: oce_adv_tra_driver.F90:494-646                  */
+/* ------------------------------------------------------------------------ */
+void fo_oce_tra_adv_flux2dtracer(const fo_mesh *m, double dt, double *dttf_h, double *dttf_v,
+                                 const double *flux_h, const double *flux_v,
+                                 int use_lo, const double *ttf, const double *lo)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    if (use_lo) {
+        for (int n = 0; n < m->nod2D; ++n)
+            for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz)
+                dttf_v[LAY(n, nz)] = dttf_v[LAY(n, nz)] - ttf[LAY(n, nz)] * m->hnode[LAY(n, nz)]
+                                     + lo[LAY(n, nz)] * m->hnode_new[LAY(n, nz)];
+    }
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz)
+            dttf_v[LAY(n, nz)] = dttf_v[LAY(n, nz)]
+                + (flux_v[INT(n, nz)] - flux_v[INT(n, nz + 1)]) * dt / m->areasvol[INT(n, nz)];
+    for (int e = 0; e < m->edge2D; ++e) {
+        edge_info q = get_edge(m, e);
+        int nl12 = imax(q.nl1, q.nl2);
+        int nu12 = q.nu1;
+        if (q.nu2 > 0) nu12 = imin(q.nu1, q.nu2);
+        for (int nz = nu12; nz <= nl12; ++nz)
+            dttf_h[LAY(q.n1, nz)] = dttf_h[LAY(q.n1, nz)] + flux_h[LAY(e, nz)] * dt / m->areasvol[INT(q.n1, nz)];
+        int nb = (g_mut == 10) ? q.n1 : q.n2;
+        for (int nz = nu12; nz <= nl12; ++nz)
+            dttf_h[LAY(q.n2, nz)] = dttf_h[LAY(q.n2, nz)] - flux_h[LAY(e, nz)] * dt / m->areasvol[INT(nb, nz)];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* do_oce_adv_tra: oce_adv_tra_driver.F90:46-490 (ltra_diag and DVD off)     */
+/* ------------------------------------------------------------------------ */
+int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    if (d->use_wsplit) return -2;      /* adv_tra_vert_impl path is not restated */
+    if (t->hor != FO_HOR_MFCT && t->hor != FO_HOR_UPW1) return -3;
+    if (t->ver != FO_VER_QR4C && t->ver != FO_VER_UPW1) return -4;
+    int fct = (t->lim == FO_LIM_FCT);
+    if (fct) {
+        fo_adv_tra_hor_upw1(m, d->uv, t->values, w->adv_flux_hor, 1);
+        memset(w->fct_LO, 0, (size_t)m->nod2D * L * sizeof(double));
+        for (int e = 0; e < m->edge2D; ++e) {
+            edge_info q = get_edge(m, e);
+            int nl12 = imax(q.nl1, q.nl2);
+            int nu12 = q.nu1;
+            if (q.nu2 > 0) nu12 = imin(q.nu1, q.nu2);
+            for (int nz = nu12; nz <= nl12; ++nz) {
+                w->fct_LO[LAY(q.n1, nz)] = w->fct_LO[LAY(q.n1, nz)] + w->adv_flux_hor[LAY(e, nz)];
+                w->fct_LO[LAY(q.n2, nz)] = w->fct_LO[LAY(q.n2, nz)] - w->adv_flux_hor[LAY(e, nz)];
+            }
+        }
+        fo_adv_tra_ver_upw1(m, d->w_e, t->values, w->adv_flux_ver, 1);
+        for (int n = 0; n < m->nod2D; ++n)
+            for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz)
+                w->fct_LO[LAY(n, nz)] = (t->values[LAY(n, nz)] * m->hnode[LAY(n, nz)]
+                    + (w->fct_LO[LAY(n, nz)] + (w->adv_flux_ver[INT(n, nz)] - w->adv_flux_ver[INT(n, nz + 1)]))
+                      * dt / m->areasvol[INT(n, nz)]) / m->hnode_new[LAY(n, nz)];
+    }
+    int zero = !fct;
+    if (t->hor == FO_HOR_MFCT)
+        fo_adv_tra_hor_mfct(m, d->uv, t->valuesAB, t->ph, w->adv_flux_hor, w->edge_up_dn_grad, zero);
+    else
+        fo_adv_tra_hor_upw1(m, d->uv, t->valuesAB, w->adv_flux_hor, zero);
+    const double *pw = fct ? d->w : d->w_e;
+    if (t->ver == FO_VER_QR4C)
+        fo_adv_tra_ver_qr4c(m, pw, t->valuesAB, t->pv, w->adv_flux_ver, zero);
+    else
+        fo_adv_tra_ver_upw1(m, pw, t->valuesAB, w->adv_flux_ver, zero);
+    if (fct) {
+        fo_oce_tra_adv_fct(m, dt, t->values, w->fct_LO, w->adv_flux_hor, w->adv_flux_ver,
+                           w->fct_ttf_min, w->fct_ttf_max, w->fct_plus, w->fct_minus);
+        fo_oce_tra_adv_flux2dtracer(m, dt, w->del_ttf_advhoriz, w->del_ttf_advvert,
+                                    w->adv_flux_hor, w->adv_flux_ver, 1, t->values, w->fct_LO);
+    } else {
+        fo_oce_tra_adv_flux2dtracer(m, dt, w->del_ttf_advhoriz, w->del_ttf_advvert,
+                                    w->adv_flux_hor, w->adv_flux_ver, 0, NULL, NULL);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* T* update: oce_ale_tracer.F90:318-322 and 742-755                         */
+/* ------------------------------------------------------------------------ */
+void fo_diff_tracers_ale_tstar(const fo_mesh *m, fo_tracer *t, fo_work *w)
+{
+    const int L = m->nl - 1;
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int nz = 1; nz <= L; ++nz)
+            w->del_ttf[LAY(n, nz)] = w->del_ttf[LAY(n, nz)] + w->del_ttf_advhoriz[LAY(n, nz)]
+                                     + w->del_ttf_advvert[LAY(n, nz)];
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
+            w->del_ttf[LAY(n, nz)] = w->del_ttf[LAY(n, nz)]
+                + t->values[LAY(n, nz)] * (m->hnode[LAY(n, nz)] - m->hnode_new[LAY(n, nz)]);
+            t->values[LAY(n, nz)] = t->values[LAY(n, nz)] + w->del_ttf[LAY(n, nz)] / m->hnode_new[LAY(n, nz)];
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* bc_surface: oce_ale_tracer.F90:1828-2108 (non-RECOM IDs)                  */
+/* ------------------------------------------------------------------------ */
+int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, double sval, double *out)
+{
+    switch (id) {
+    case 1:
+        *out = -dt * (d->heat_flux[n] / VCPW + sval * d->water_flux[n] * d->is_nonlinfs);
+        return 0;
+    case 2:
+        *out = dt * (d->virtual_salt[n] + d->relax_salt[n] + d->real_salt_flux[n] * d->is_nonlinfs);
+        return 0;
+    case 100: case 301: case 302: case 303:
+        *out = 0.0;
+        return 0;
+    default:
+        return -1;   /* par_ex: invalid ID (oce_ale_tracer.F90:2091-2098) */
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* diff_ver_part_impl_ale: oce_ale_tracer.F90:907-1435                       */
+/* (Redi off -> isredi = 0; FCT or no w-split -> do_wimpl false; no KPP       */
+/*  non-local flux, no shortwave penetration, no icebergs, no hosing)        */
+/* ------------------------------------------------------------------------ */
+int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    double *a = (double *)calloc(NL, sizeof(double)), *b = (double *)calloc(NL, sizeof(double));
+    double *c = (double *)calloc(NL, sizeof(double)), *tr = (double *)calloc(NL, sizeof(double));
+    double *cp = (double *)calloc(NL, sizeof(double)), *tp = (double *)calloc(NL, sizeof(double));
+    double *zbar_n = (double *)calloc(NL, sizeof(double)), *Z_n = (double *)calloc(NL, sizeof(double));
+    int rc = 0;
+    for (int n = 0; n < m->nod2D && rc == 0; ++n) {
+        const double *hn = m->hnode_new + LAY(n, 1);
+        const double *ar = m->area + INT(n, 1);
+        const double *av = m->areasvol + INT(n, 1);
+        const double *kv = d->Kv + INT(n, 1);
+        double *T = t->values + LAY(n, 1);
+#define I(k) ((k) - 1)
+        for (int i = 0; i < NL; ++i) { a[i] = b[i] = c[i] = tr[i] = tp[i] = cp[i] = 0.0; zbar_n[i] = Z_n[i] = 0.0; }
+        int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
+        zbar_n[I(nzmax)] = m->zbar_n_bot[n];
+        Z_n[I(nzmax - 1)] = zbar_n[I(nzmax)] + hn[I(nzmax - 1)] / 2.0;
+        for (int nz = nzmax - 1; nz >= nzmin + 1; --nz) {
+            zbar_n[I(nz)] = zbar_n[I(nz + 1)] + hn[I(nz)];
+            Z_n[I(nz - 1)] = zbar_n[I(nz)] + hn[I(nz - 1)] / 2.0;
+        }
+        zbar_n[I(nzmin)] = zbar_n[I(nzmin + 1)] + hn[I(nzmin)];
+        int nz = nzmin;
+        double zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
+        double zinv = 1.0 * dt;
+        int ic = (g_mut == 12) ? nz : nz + 1;
+        a[I(nz)] = 0.0;
+        c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+        b[I(nz)] = -c[I(nz)] + hn[I(nz)];
+        double zinv1 = zinv2;
+        for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
+            zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
+            ic = (g_mut == 12) ? nz : nz + 1;
+            a[I(nz)] = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
+            c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+            b[I(nz)] = -a[I(nz)] - c[I(nz)] + hn[I(nz)];
+            zinv1 = zinv2;
+        }
+        nz = nzmax - 1;
+        zinv = 1.0 * dt;
+        a[I(nz)] = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
+        c[I(nz)] = 0.0;
+        b[I(nz)] = -a[I(nz)] + hn[I(nz)];
+
+        nz = nzmin;
+        double dz = hn[I(nz)];
+        tr[I(nz)] = -(b[I(nz)] - dz) * T[I(nz)] - c[I(nz)] * T[I(nz + 1)];
+        for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
+            dz = hn[I(nz)];
+            tr[I(nz)] = -a[I(nz)] * T[I(nz - 1)] - (b[I(nz)] - dz) * T[I(nz)] - c[I(nz)] * T[I(nz + 1)];
+        }
+        nz = nzmax - 1;
+        dz = hn[I(nz)];
+        tr[I(nz)] = -a[I(nz)] * T[I(nz - 1)] - (b[I(nz)] - dz) * T[I(nz)];
+
+        double bc;
+        if (fo_bc_surface(t->ID, dt, d, n, T[I(nzmin)], &bc) != 0) { rc = -1; break; }
+        tr[I(nzmin)] = tr[I(nzmin)] + bc;
+
+        cp[I(nzmin)] = c[I(nzmin)] / b[I(nzmin)];
+        tp[I(nzmin)] = tr[I(nzmin)] / b[I(nzmin)];
+        for (nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+            double mm = b[I(nz)] - cp[I(nz - 1)] * a[I(nz)];
+            cp[I(nz)] = c[I(nz)] / mm;
+            tp[I(nz)] = (tr[I(nz)] - tp[I(nz - 1)] * a[I(nz)]) / mm;
+        }
+        tr[I(nzmax - 1)] = tp[I(nzmax - 1)];
+        for (nz = nzmax - 2; nz >= nzmin; --nz) {
+            if (g_mut == 13) tr[I(nz)] = tp[I(nz)] + cp[I(nz)] * tr[I(nz + 1)];
+            else tr[I(nz)] = tp[I(nz)] - cp[I(nz)] * tr[I(nz + 1)];
+        }
+        for (nz = nzmin; nz <= nzmax - 1; ++nz) T[I(nz)] = T[I(nz)] + tr[I(nz)];
+#undef I
+    }
+    free(a); free(b); free(c); free(tr); free(cp); free(tp); free(zbar_n); free(Z_n);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* one tracer step for all tracers: solve_tracers_ale, oce_ale_tracer.F90:   */
+/* 280-385 (per tracer) and the salinity clamp at :502-514                   */
+/* ------------------------------------------------------------------------ */
+int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, int ntr, fo_work *w)
+{
+    const int L = m->nl - 1;
+    for (int i = 0; i < ntr; ++i) {
+        int rc = fo_init_tracers_AB(m, &t[i], w);
+        if (rc) return rc;
+        rc = fo_do_oce_adv_tra(m, d, dt, &t[i], w);
+        if (rc) return rc;
+        fo_diff_tracers_ale_tstar(m, &t[i], w);
+        if (t[i].i_vert_diff) {
+            rc = fo_diff_ver_part_impl_ale(m, d, dt, &t[i]);
+            if (rc) return rc;
+        }
+    }
+    if (ntr >= 2) {
+        double *s = t[1].values;
+        for (int n = 0; n < m->nod2D; ++n)
+            for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
+                if (s[LAY(n, nz)] > 45.0) s[LAY(n, nz)] = 45.0;
+                if (s[LAY(n, nz)] < 3.0) s[LAY(n, nz)] = 3.0;
+            }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* init_stiff_mat_ale: oce_ale.F90:1393-1582 (one partition, local numbering)*/
+/* ------------------------------------------------------------------------ */
+int fo_init_stiff_mat_ale(const fo_mesh *m, double dt, int *rowptr, int *colind, double *values, int max_nnz)
+{
+    const int N2 = m->nod2D;
+    int *n_num = (int *)calloc(N2, sizeof(int));
+    int *n_pos = (int *)calloc((size_t)N2 * 16, sizeof(int));
+    for (int n = 0; n < N2; ++n) { n_num[n] = 1; n_pos[(size_t)n * 16] = n; }
+    for (int e = 0; e < m->edge2D; ++e) {
+        int n1 = m->edges[2 * e], n2 = m->edges[2 * e + 1];
+        if (n_num[n1] >= 16 || n_num[n2] >= 16) { free(n_num); free(n_pos); return -1; }
+        n_pos[(size_t)n1 * 16 + n_num[n1]] = n2; n_num[n1]++;
+        n_pos[(size_t)n2 * 16 + n_num[n2]] = n1; n_num[n2]++;
+    }
+    rowptr[0] = 0;
+    for (int n = 0; n < N2; ++n) rowptr[n + 1] = rowptr[n] + n_num[n];
+    if (rowptr[N2] > max_nnz) { free(n_num); free(n_pos); return -2; }
+    for (int n = 0; n < N2; ++n)
+        for (int j = 0; j < n_num[n]; ++j) colind[rowptr[n] + j] = n_pos[(size_t)n * 16 + j];
+    for (int i = 0; i < rowptr[N2]; ++i) values[i] = 0.0;
+    /* position map: n_num(col) = position within the current row */
+    int *pos = (int *)malloc(sizeof(int) * (size_t)N2);
+    for (int n = 0; n < N2; ++n) pos[n] = -1;
+    const double factor = G_ACC * dt * 1.0 * 1.0;   /* g*dt*alpha*theta */
+    for (int ed = 0; ed < m->edge2D; ++ed) {
+        for (int i = 0; i < 2; ++i) {
+            int el = m->edge_tri[2 * ed + i];
+            if (el < 0) continue;
+            const int *en = m->elem2D_nodes + 3 * (size_t)el;
+            const double *gs = m->gradient_sca + 6 * (size_t)el;
+            double hh = m->zbar_e_bot[el] - m->zbar_e_srf[el];
+            double cx = m->edge_cross_dxdy[4 * ed + 2 * i];       /* edge_cross_dxdy(2*i-1) */
+            double cy = m->edge_cross_dxdy[4 * ed + 2 * i + 1];   /* edge_cross_dxdy(2*i)   */
+            double fy[3];
+            for (int j = 0; j < 3; ++j) fy[j] = hh * (gs[j] * cy - gs[3 + j] * cx);
+            if (i == 1) for (int j = 0; j < 3; ++j) fy[j] = -fy[j];
+            for (int side = 0; side < 2; ++side) {
+                int row = m->edges[2 * ed + side];
+                for (int p = rowptr[row]; p < rowptr[row + 1]; ++p) pos[colind[p]] = p;
+                for (int j = 0; j < 3; ++j) {
+                    int p = pos[en[j]];
+                    if (side == 0) values[p] = values[p] + fy[j] * factor;
+                    else values[p] = values[p] - fy[j] * factor;
+                }
+            }
+        }
+    }
+    for (int row = 0; row < N2; ++row) {
+        if (m->ulevels_nod2D[row] > 1) continue;
+        int off = rowptr[row];
+        double av = m->areasvol[(size_t)row * m->nl + (m->ulevels_nod2D[row] - 1)];
+        if (g_mut == 14) values[off] = values[off] + av * dt;
+        else values[off] = values[off] + av / dt;
+    }
+    free(pos); free(n_num); free(n_pos);
+    return rowptr[N2];
+}
+
+/* ------------------------------------------------------------------------ */
+/* ssh_solve_preconditioner: solver.F90:31-117                               */
+/* variant 0/1 as in the reference; variant 2 = diagonal (Jacobi) only       */
+/* ------------------------------------------------------------------------ */
+int fo_ssh_solve_preconditioner(int n, const int *rowptr, const int *colind, const double *values,
+                                int variant, double *pr)
+{
+    double *diag = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int r = 0; r < n; ++r) diag[r] = values[rowptr[r]];
+    for (int r = 0; r < n; ++r) {
+        int off = rowptr[r];
+        int nend = rowptr[r + 1] - rowptr[r];
+        pr[off] = 1.0 / values[off];
+        for (int j = 1; j < nend; ++j) {
+            int node = colind[off + j];
+            if (variant == 0)
+                pr[off + j] = -0.5 * (values[off + j] / values[off]) / (values[off] + diag[node]);
+            else if (variant == 1) {
+                double v = -values[off + j] / (values[off] * diag[node]);
+                pr[off + j] = (g_mut == 16) ? -v : v;
+            } else
+                pr[off + j] = 0.0;
+        }
+    }
+    free(diag);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ssh_solve_cg: solver.F90:120-385 (one partition; serial sums)             */
+/* ------------------------------------------------------------------------ */
+static double row_dot(const int *rowptr, const int *colind, const double *v, const double *x, int r)
+{
+    /* Fortran sum(values(a:b) * x(cind(a:b))): left-to-right */
+    double s = 0.0;
+    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) s = s + v[p] * x[colind[p]];
+    return s;
+}
+
+int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *values,
+                    const double *pr, double *x, const double *rhs, double soltol,
+                    int maxiter, fo_cg_stats *st)
+{
+    double *rr = (double *)calloc(n, sizeof(double)), *zz = (double *)calloc(n, sizeof(double));
+    double *pp = (double *)calloc(n, sizeof(double)), *App = (double *)calloc(n, sizeof(double));
+    double s_old = 0.0, s_aux, al, be, sprod[2] = {0.0, 0.0};
+    for (int r = 0; r < n; ++r) s_old = s_old + rhs[r] * rhs[r];
+    double rtol = soltol * sqrt(s_old / (double)n);
+    for (int r = 0; r < n; ++r) rr[r] = rhs[r] - row_dot(rowptr, colind, values, x, r);
+    for (int r = 0; r < n; ++r) { zz[r] = row_dot(rowptr, colind, pr, rr, r); pp[r] = zz[r]; }
+    s_old = 0.0;
+    for (int r = 0; r < n; ++r) s_old = s_old + rr[r] * zz[r];
+    int converged = 0, iter;
+    st->nbreakdown = 0;
+    st->nnegrz = 0;
+    for (iter = 1; iter <= maxiter; ++iter) {
+        for (int r = 0; r < n; ++r) App[r] = row_dot(rowptr, colind, values, pp, r);
+        s_aux = 0.0;
+        for (int r = 0; r < n; ++r) s_aux = s_aux + pp[r] * App[r];
+        if (s_aux <= 0.0 || s_aux != s_aux) { st->nbreakdown++; break; }
+        al = s_old / s_aux;
+        for (int r = 0; r < n; ++r) { x[r] = x[r] + al * pp[r]; rr[r] = rr[r] - al * App[r]; }
+        for (int r = 0; r < n; ++r) zz[r] = row_dot(rowptr, colind, pr, rr, r);
+        sprod[0] = 0.0; sprod[1] = 0.0;
+        for (int r = 0; r < n; ++r) { sprod[0] = sprod[0] + rr[r] * zz[r]; sprod[1] = sprod[1] + rr[r] * rr[r]; }
+        if (sqrt(sprod[1] / (double)n) < rtol) { converged = 1; break; }
+        if (sprod[0] < 0.0) st->nnegrz++;
+        if (s_old == 0.0) { st->nbreakdown++; break; }
+        be = (g_mut == 15) ? s_old / sprod[0] : sprod[0] / s_old;
+        s_old = sprod[0];
+        for (int r = 0; r < n; ++r) pp[r] = zz[r] + be * pp[r];
+    }
+    st->iters_last = iter < maxiter ? iter : maxiter;
+    st->converged = converged;
+    st->resid_last = sqrt(sprod[1] / (double)n);
+    st->rtol_last = rtol;
+    free(rr); free(zz); free(pp); free(App);
+    return converged ? 0 : 1;
+}

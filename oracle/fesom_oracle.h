@@ -1,0 +1,114 @@
+/*
+ * CPU oracle for the FESOM2 tracer-transport step and SSH solve.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference
+ * algorithm, used by tests/, by __graft_entry__.smoke() and by bench.py's CPU
+ * baseline leg as the checker. It is never part of the product path.
+ *
+ * Conventions (same memory image as the reference's Fortran arrays):
+ *   layer arrays      a[n*L + (k-1)],       L  = nl-1, k = 1..nl-1
+ *   interface arrays  a[n*NL + (k-1)],      NL = nl,   k = 1..nl
+ *   uv                uv[(e*L + k-1)*2 + c] (c = 0: u, 1: v)
+ *   tr_xy             [(e*L + k-1)*2 + c]
+ *   edge_up_dn_grad   [(e*L + k-1)*4 + c]
+ * Connectivity is 0-based, -1 marks a missing element. Level numbers keep the
+ * reference's 1-based meaning (a node's layers are ulevels..nlevels-1).
+ */
+#ifndef FESOM_ORACLE_H
+#define FESOM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nod2D, elem2D, edge2D, nl, kmax, cartesian;
+    double cyclic_length;
+    const double *coord_nod2D;        /* N2*2 */
+    const int *elem2D_nodes;          /* F2*3 */
+    const int *edges, *edge_tri;      /* E2*2 */
+    const int *nod_in_elem2D;         /* N2*kmax */
+    const int *nod_in_elem2D_num;     /* N2 */
+    const int *nlevels, *ulevels;     /* F2 */
+    const int *nlevels_nod2D, *ulevels_nod2D, *nlevels_nod2D_min, *ulevels_nod2D_max; /* N2 */
+    const double *elem_area, *elem_cos, *edge_dxdy, *edge_cross_dxdy, *gradient_sca;
+    const double *area, *areasvol;    /* N2*nl */
+    const double *hnode, *hnode_new;  /* N2*L */
+    const double *helem;              /* F2*L */
+    const double *zbar_3d_n;          /* N2*nl */
+    const double *Z_3d_n;             /* N2*L */
+    const double *zbar_n_bot;         /* N2 */
+    const double *zbar_e_bot, *zbar_e_srf; /* F2 */
+    const int *edge_up_dn_tri;        /* E2*2, from fo_find_up_downwind_triangles */
+} fo_mesh;
+
+typedef struct {
+    const double *uv, *w, *w_e, *w_i, *Kv;
+    const double *heat_flux, *water_flux, *virtual_salt, *relax_salt, *real_salt_flux;
+    double is_nonlinfs;
+    int use_wsplit;
+} fo_dyn;
+
+enum { FO_HOR_MFCT = 0, FO_HOR_UPW1 = 1 };
+enum { FO_VER_QR4C = 0, FO_VER_UPW1 = 1 };
+enum { FO_LIM_FCT = 0, FO_LIM_NON = 1 };
+
+typedef struct {
+    int ID, AB_order, hor, ver, lim, i_vert_diff;
+    double ph, pv;
+    double *values;      /* N2*L */
+    double *valuesold;   /* N2*L*(AB_order-1), AB index fastest */
+    double *valuesAB;    /* N2*L */
+} fo_tracer;
+
+typedef struct {
+    double *tr_xy;            /* F2*L*2 */
+    double *edge_up_dn_grad;  /* E2*L*4 */
+    double *fct_LO;           /* N2*L */
+    double *adv_flux_hor;     /* E2*L */
+    double *adv_flux_ver;     /* N2*NL */
+    double *fct_ttf_max, *fct_ttf_min, *fct_plus, *fct_minus; /* N2*L */
+    double *del_ttf, *del_ttf_advhoriz, *del_ttf_advvert;    /* N2*L */
+} fo_work;
+
+/* planted-mistake switch for the pinning tests (0 = faithful) */
+void fo_set_mutation(int m);
+int fo_get_mutation(void);
+
+int fo_find_up_downwind_triangles(const fo_mesh *m, int *edge_up_dn_tri);
+int fo_init_tracers_AB(const fo_mesh *m, fo_tracer *t, fo_work *w);
+void fo_tracer_gradient_elements(const fo_mesh *m, const double *ttf, double *tr_xy);
+void fo_fill_up_dn_grad(const fo_mesh *m, const double *tr_xy, double *grad);
+void fo_adv_tra_hor_upw1(const fo_mesh *m, const double *vel, const double *ttf, double *flux, int init_zero);
+void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf, double num_ord,
+                         double *flux, const double *grad, int init_zero);
+void fo_adv_tra_ver_upw1(const fo_mesh *m, const double *w, const double *ttf, double *flux, int init_zero);
+void fo_adv_tra_ver_qr4c(const fo_mesh *m, const double *w, const double *ttf, double num_ord,
+                         double *flux, int init_zero);
+void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const double *lo,
+                        double *adf_h, double *adf_v, double *ttf_min, double *ttf_max,
+                        double *fct_plus, double *fct_minus);
+void fo_oce_tra_adv_flux2dtracer(const fo_mesh *m, double dt, double *dttf_h, double *dttf_v,
+                                 const double *flux_h, const double *flux_v,
+                                 int use_lo, const double *ttf, const double *lo);
+int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w);
+void fo_diff_tracers_ale_tstar(const fo_mesh *m, fo_tracer *t, fo_work *w);
+int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t);
+int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, double sval, double *out);
+int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, int ntr, fo_work *w);
+
+int fo_init_stiff_mat_ale(const fo_mesh *m, double dt, int *rowptr, int *colind, double *values, int max_nnz);
+int fo_ssh_solve_preconditioner(int n, const int *rowptr, const int *colind, const double *values,
+                                int variant, double *pr_values);
+typedef struct {
+    int iters_last, converged, nbreakdown, nnegrz;
+    double resid_last, rtol_last;
+} fo_cg_stats;
+int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *values,
+                    const double *pr_values, double *x, const double *rhs, double soltol,
+                    int maxiter, fo_cg_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker. It wraps oracle/libfesom_oracle.so (built from
+fesom_oracle.c by oracle/Makefile) and works on the numpy containers of
+fesom2_amd.meshgen.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import List, Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libfesom_oracle.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+class FoMesh(C.Structure):
+    _fields_ = [("nod2D", C.c_int), ("elem2D", C.c_int), ("edge2D", C.c_int), ("nl", C.c_int),
+                ("kmax", C.c_int), ("cartesian", C.c_int), ("cyclic_length", C.c_double),
+                ("coord_nod2D", dp), ("elem2D_nodes", ip), ("edges", ip), ("edge_tri", ip),
+                ("nod_in_elem2D", ip), ("nod_in_elem2D_num", ip), ("nlevels", ip), ("ulevels", ip),
+                ("nlevels_nod2D", ip), ("ulevels_nod2D", ip), ("nlevels_nod2D_min", ip),
+                ("ulevels_nod2D_max", ip), ("elem_area", dp), ("elem_cos", dp), ("edge_dxdy", dp),
+                ("edge_cross_dxdy", dp), ("gradient_sca", dp), ("area", dp), ("areasvol", dp),
+                ("hnode", dp), ("hnode_new", dp), ("helem", dp), ("zbar_3d_n", dp), ("Z_3d_n", dp),
+                ("zbar_n_bot", dp), ("zbar_e_bot", dp), ("zbar_e_srf", dp), ("edge_up_dn_tri", ip)]
+
+
+class FoDyn(C.Structure):
+    _fields_ = [("uv", dp), ("w", dp), ("w_e", dp), ("w_i", dp), ("Kv", dp), ("heat_flux", dp),
+                ("water_flux", dp), ("virtual_salt", dp), ("relax_salt", dp), ("real_salt_flux", dp),
+                ("is_nonlinfs", C.c_double), ("use_wsplit", C.c_int)]
+
+
+class FoTracer(C.Structure):
+    _fields_ = [("ID", C.c_int), ("AB_order", C.c_int), ("hor", C.c_int), ("ver", C.c_int),
+                ("lim", C.c_int), ("i_vert_diff", C.c_int), ("ph", C.c_double), ("pv", C.c_double),
+                ("values", dp), ("valuesold", dp), ("valuesAB", dp)]
+
+
+class FoWork(C.Structure):
+    _fields_ = [("tr_xy", dp), ("edge_up_dn_grad", dp), ("fct_LO", dp), ("adv_flux_hor", dp),
+                ("adv_flux_ver", dp), ("fct_ttf_max", dp), ("fct_ttf_min", dp), ("fct_plus", dp),
+                ("fct_minus", dp), ("del_ttf", dp), ("del_ttf_advhoriz", dp), ("del_ttf_advvert", dp)]
+
+
+class FoCgStats(C.Structure):
+    _fields_ = [("iters_last", C.c_int), ("converged", C.c_int), ("nbreakdown", C.c_int),
+                ("nnegrz", C.c_int), ("resid_last", C.c_double), ("rtol_last", C.c_double)]
+
+
+HOR = {"MFCT": 0, "UPW1": 1}
+VER = {"QR4C": 0, "UPW1": 1}
+LIM = {"FCT": 0, "NON": 1}
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, serial, no FMA contraction)."""
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.fo_set_mutation.argtypes = [C.c_int]
+        L.fo_find_up_downwind_triangles.argtypes = [C.POINTER(FoMesh), ip]
+        L.fo_init_tracers_AB.argtypes = [C.POINTER(FoMesh), C.POINTER(FoTracer), C.POINTER(FoWork)]
+        L.fo_tracer_gradient_elements.argtypes = [C.POINTER(FoMesh), dp, dp]
+        L.fo_fill_up_dn_grad.argtypes = [C.POINTER(FoMesh), dp, dp]
+        L.fo_adv_tra_hor_upw1.argtypes = [C.POINTER(FoMesh), dp, dp, dp, C.c_int]
+        L.fo_adv_tra_hor_mfct.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, dp, C.c_int]
+        L.fo_adv_tra_ver_upw1.argtypes = [C.POINTER(FoMesh), dp, dp, dp, C.c_int]
+        L.fo_adv_tra_ver_qr4c.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, C.c_int]
+        L.fo_oce_tra_adv_fct.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.fo_oce_tra_adv_flux2dtracer.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, dp, C.c_int, dp, dp]
+        L.fo_do_oce_adv_tra.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
+                                        C.POINTER(FoTracer), C.POINTER(FoWork)]
+        L.fo_diff_tracers_ale_tstar.argtypes = [C.POINTER(FoMesh), C.POINTER(FoTracer), C.POINTER(FoWork)]
+        L.fo_diff_ver_part_impl_ale.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
+                                                C.POINTER(FoTracer)]
+        L.fo_solve_tracers_ale.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
+                                           C.POINTER(FoTracer), C.c_int, C.POINTER(FoWork)]
+        L.fo_init_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, ip, ip, dp, C.c_int]
+        L.fo_ssh_solve_preconditioner.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp]
+        L.fo_ssh_solve_cg.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_int,
+                                      C.POINTER(FoCgStats)]
+        _lib = L
+    return _lib
+
+
+def set_mutation(m: int) -> None:
+    lib().fo_set_mutation(int(m))
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ip)
+
+
+class Oracle:
+    """Oracle bound to one mesh; keeps the numpy buffers alive."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        m = mesh
+        c = lambda a, t=np.float64: np.ascontiguousarray(a, dtype=t)
+        self._keep = {
+            "coord": c(m.coord_nod2D), "elem": c(m.elem2D_nodes, np.int32), "edges": c(m.edges, np.int32),
+            "edge_tri": c(m.edge_tri, np.int32), "nie": c(m.nod_in_elem2D, np.int32),
+            "nien": c(m.nod_in_elem2D_num, np.int32), "nlev": c(m.nlevels, np.int32),
+            "ulev": c(m.ulevels, np.int32), "nlevn": c(m.nlevels_nod2D, np.int32),
+            "ulevn": c(m.ulevels_nod2D, np.int32), "nlmin": c(m.nlevels_nod2D_min, np.int32),
+            "ulmax": c(m.ulevels_nod2D_max, np.int32), "earea": c(m.elem_area), "ecos": c(m.elem_cos),
+            "edxdy": c(m.edge_dxdy), "ecd": c(m.edge_cross_dxdy), "gsca": c(m.gradient_sca),
+            "area": c(m.area), "areasvol": c(m.areasvol), "hnode": c(m.hnode), "hnode_new": c(m.hnode_new),
+            "helem": c(m.helem), "zbar3d": c(m.zbar_3d_n), "Z3d": c(m.Z_3d_n), "zbnb": c(m.zbar_n_bot),
+            "zbeb": c(m.zbar_e_bot), "zbes": c(m.zbar_e_srf),
+            "updn": np.zeros((m.edge2D, 2), np.int32),
+        }
+        k = self._keep
+        self.fm = FoMesh(m.nod2D, m.elem2D, m.edge2D, m.nl, m.kmax, int(m.cartesian), m.cyclic_length,
+                         _d(k["coord"]), _i(k["elem"]), _i(k["edges"]), _i(k["edge_tri"]), _i(k["nie"]),
+                         _i(k["nien"]), _i(k["nlev"]), _i(k["ulev"]), _i(k["nlevn"]), _i(k["ulevn"]),
+                         _i(k["nlmin"]), _i(k["ulmax"]), _d(k["earea"]), _d(k["ecos"]), _d(k["edxdy"]),
+                         _d(k["ecd"]), _d(k["gsca"]), _d(k["area"]), _d(k["areasvol"]), _d(k["hnode"]),
+                         _d(k["hnode_new"]), _d(k["helem"]), _d(k["zbar3d"]), _d(k["Z3d"]), _d(k["zbnb"]),
+                         _d(k["zbeb"]), _d(k["zbes"]), _i(k["updn"]))
+        lib().fo_find_up_downwind_triangles(C.byref(self.fm), _i(k["updn"]))
+        L = m.nl - 1
+        z = lambda *s: np.zeros(s)
+        self.work_arrays = dict(tr_xy=z(m.elem2D, L, 2), edge_up_dn_grad=z(m.edge2D, L, 4),
+                                fct_LO=z(m.nod2D, L), adv_flux_hor=z(m.edge2D, L),
+                                adv_flux_ver=z(m.nod2D, m.nl), fct_ttf_max=z(m.nod2D, L),
+                                fct_ttf_min=z(m.nod2D, L), fct_plus=z(m.nod2D, L),
+                                fct_minus=z(m.nod2D, L), del_ttf=z(m.nod2D, L),
+                                del_ttf_advhoriz=z(m.nod2D, L), del_ttf_advvert=z(m.nod2D, L))
+        wa = self.work_arrays
+        self.fw = FoWork(*[_d(wa[f[0]]) for f in FoWork._fields_])
+
+    @property
+    def edge_up_dn_tri(self):
+        return self._keep["updn"]
+
+    def refresh_updn(self):
+        lib().fo_find_up_downwind_triangles(C.byref(self.fm), _i(self._keep["updn"]))
+
+    # ---------------------------------------------------------------- dynamics
+    def dyn(self, st):
+        keep = dict(uv=np.ascontiguousarray(st.uv), w=np.ascontiguousarray(st.w),
+                    w_e=np.ascontiguousarray(st.w_e), w_i=np.ascontiguousarray(st.w_i),
+                    Kv=np.ascontiguousarray(st.Kv), hf=np.ascontiguousarray(st.heat_flux),
+                    wf=np.ascontiguousarray(st.water_flux), vs=np.ascontiguousarray(st.virtual_salt),
+                    rs=np.ascontiguousarray(st.relax_salt), rsf=np.ascontiguousarray(st.real_salt_flux))
+        d = FoDyn(_d(keep["uv"]), _d(keep["w"]), _d(keep["w_e"]), _d(keep["w_i"]), _d(keep["Kv"]),
+                  _d(keep["hf"]), _d(keep["wf"]), _d(keep["vs"]), _d(keep["rs"]), _d(keep["rsf"]),
+                  0.0, 0)
+        return d, keep
+
+    def tracer(self, values, valuesold, ID=1, AB_order=2, hor="MFCT", ver="QR4C", lim="FCT",
+               ph=0.0, pv=1.0, i_vert_diff=True):
+        vAB = np.zeros_like(values)
+        t = FoTracer(int(ID), int(AB_order), HOR[hor], VER[ver], LIM[lim], int(bool(i_vert_diff)),
+                     float(ph), float(pv), _d(values), _d(valuesold), _d(vAB))
+        return t, vAB
+
+    # ------------------------------------------------------------ full step
+    def step(self, st, values, valuesold, dt=None, schemes=None):
+        """One solve_tracers_ale step over all tracers, in place on values/valuesold."""
+        ntr = values.shape[0]
+        dt = st.dt if dt is None else dt
+        d, keep = self.dyn(st)
+        trs = (FoTracer * ntr)()
+        abs_ = []
+        for i in range(ntr):
+            sch = dict(hor="MFCT", ver="QR4C", lim="FCT", ph=0.0, pv=1.0)
+            if schemes:
+                sch.update(schemes[i] if isinstance(schemes, (list, tuple)) else schemes)
+            t, vab = self.tracer(values[i], valuesold[i], ID=int(st.ids[i]), **sch)
+            trs[i] = t
+            abs_.append(vab)
+        rc = lib().fo_solve_tracers_ale(C.byref(self.fm), C.byref(d), float(dt), trs, ntr, C.byref(self.fw))
+        if rc != 0:
+            raise RuntimeError(f"oracle solve_tracers_ale failed rc={rc}")
+        return rc
+
+    def adv_stage(self, st, values, valuesold, ID=1, dt=None, **sch):
+        """init_tracers_AB + do_oce_adv_tra for one tracer; returns work copies.
+
+        ``values``/``valuesold`` are updated in place (valuesold rotation)."""
+        dt = st.dt if dt is None else dt
+        d, keep = self.dyn(st)
+        t, vab = self.tracer(values, valuesold, ID=ID, **sch)
+        rc = lib().fo_init_tracers_AB(C.byref(self.fm), C.byref(t), C.byref(self.fw))
+        if rc:
+            raise RuntimeError("init_tracers_AB failed")
+        out = {"valuesAB": vab.copy(), "tr_xy": self.work_arrays["tr_xy"].copy(),
+               "edge_up_dn_grad": self.work_arrays["edge_up_dn_grad"].copy()}
+        rc = lib().fo_do_oce_adv_tra(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), C.byref(self.fw))
+        if rc:
+            raise RuntimeError(f"do_oce_adv_tra failed rc={rc}")
+        for k in ("fct_LO", "adv_flux_hor", "adv_flux_ver", "del_ttf_advhoriz", "del_ttf_advvert",
+                  "fct_plus", "fct_minus"):
+            out[k] = self.work_arrays[k].copy()
+        self._last = (t, d, keep, vab)
+        return out
+
+    def diff_stage(self, st, dt=None):
+        """T* update + implicit vertical diffusion after adv_stage (same tracer)."""
+        dt = st.dt if dt is None else dt
+        t, d, keep, vab = self._last
+        lib().fo_diff_tracers_ale_tstar(C.byref(self.fm), C.byref(t), C.byref(self.fw))
+        if t.i_vert_diff:
+            rc = lib().fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
+            if rc:
+                raise RuntimeError("diff_ver_part_impl_ale failed")
+
+    def vdiff(self, st, values, ID=1, dt=None):
+        """diff_ver_part_impl_ale alone on ``values`` (in place)."""
+        dt = st.dt if dt is None else dt
+        d, keep = self.dyn(st)
+        vo = np.zeros_like(values)
+        t, vab = self.tracer(values, vo, ID=ID)
+        rc = lib().fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
+        if rc:
+            raise RuntimeError("diff_ver_part_impl_ale failed")
+
+    # ------------------------------------------------------------ SSH
+    def stiff_matrix(self, dt):
+        m = self.mesh
+        max_nnz = m.nod2D * 16
+        rowptr = np.zeros(m.nod2D + 1, np.int32)
+        colind = np.zeros(max_nnz, np.int32)
+        vals = np.zeros(max_nnz)
+        nnz = lib().fo_init_stiff_mat_ale(C.byref(self.fm), float(dt), _i(rowptr), _i(colind), _d(vals), max_nnz)
+        if nnz < 0:
+            raise RuntimeError("stiffness assembly failed")
+        return rowptr, colind[:nnz].copy(), vals[:nnz].copy()
+
+    @staticmethod
+    def preconditioner(rowptr, colind, vals, variant):
+        pr = np.zeros_like(vals)
+        lib().fo_ssh_solve_preconditioner(len(rowptr) - 1, _i(rowptr), _i(colind), _d(vals), int(variant), _d(pr))
+        return pr
+
+    @staticmethod
+    def cg(rowptr, colind, vals, pr, rhs, x0=None, soltol=1e-10, maxiter=2000):
+        n = len(rowptr) - 1
+        x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        st = FoCgStats()
+        lib().fo_ssh_solve_cg(n, _i(rowptr), _i(colind), _d(vals), _d(pr), _d(x),
+                              _d(np.ascontiguousarray(rhs, dtype=np.float64)), float(soltol), int(maxiter),
+                              C.byref(st))
+        return x, dict(iters=st.iters_last, converged=bool(st.converged), nbreakdown=st.nbreakdown,
+                       nnegrz=st.nnegrz, resid=st.resid_last, rtol=st.rtol_last)
