@@ -1,0 +1,187 @@
+"""Benchmark of the MI355X tracer-transport step and SSH solve.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+
+One "step" is one solve_tracers_ale pass over every tracer of the workload
+(AB2, MFCT/QR4C/FCT advection, implicit vertical diffusion), replayed from a
+captured hipGraph with all fields resident in HBM. `value` is tracer-updates/s
+(wet 3-D nodes x tracers x steps / wall time of the K timed steps). The SSH
+solve (diagonally preconditioned CG to 1e-10 relative residual, x0 = 0) is
+timed separately and reported as ssh_solve_ms.
+
+Also reported: the per-kernel profile of the step (hipEvents on the library
+stream), the roofline of the dominant kernel, and the CPU oracle timed on a
+bounded sample of the same workload on this host (one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name, mesh, ntr):
+    """Algorithmic bytes of one launch of a step kernel (all tracers of the batch).
+
+    Each array a kernel must read or write is counted once over the entries the
+    kernel touches (wet levels; k_init_ab touches whole columns), FP64 = 8 B,
+    int32 = 4 B. Per-tracer terms scale with ntr, mesh/dynamics terms do not.
+    """
+    N2, E2, F2, L, NL = mesh.nod2D, mesh.edge2D, mesh.elem2D, mesh.nl - 1, mesh.nl
+    N3, E3, F3 = mesh.n3, mesh.e3, mesh.f3
+    NI = N3 + N2          # wet interfaces
+    if name == "k_init_ab":
+        return ntr * 32 * N2 * L                       # T, Told read; TAB, Told write
+    if name == "k_tr_xy":
+        return ntr * (8 * N3 + 16 * F3) + 48 * F2 + 12 * F2 + 8 * F2
+    if name == "k_edge_flux":
+        per_tr = 16 * N3 + 16 * F3 + 16 * E3           # T, TAB; tr_xy; LO_h, A_h written
+        shared = 24 * F3 + E2 * (8 + 16 + 16 + 16 + 8) + F2 * 8   # uv, helem; edge geometry
+        return ntr * per_tr + shared
+    if name == "k_node_lo":
+        per_tr = 8 * E3 + 16 * N3 + 16 * N3            # LO_h gather; T, TAB; fct_LO, A_v written
+        shared = 8 * (2 * NI) + 8 * (4 * N3) + 8 * (2 * NI) + N2 * 16 + 8 * E2
+        return ntr * per_tr + shared
+    if name == "k_elem_aux":
+        return ntr * (16 * N3 + 16 * F2 * L) + 12 * F2
+    if name == "k_node_limit":
+        per_tr = 16 * F3 + 8 * N3 + 8 * NI + 8 * E3 + 16 * N3   # aux; LO; A_v; A_h; R+/R- written
+        return ntr * per_tr + 16 * N3 + N2 * 16 + 8 * E2
+    if name == "k_edge_limit":
+        return ntr * (16 * E3 + 16 * N3) + 16 * E2
+    if name == "k_node_update":
+        per_tr = 8 * NI + 16 * N3 + 8 * E3 + 16 * N3   # A_v; R+/R-; A_h; T read+write... LO read
+        per_tr += 8 * N3
+        return ntr * per_tr + 24 * N3 + N2 * 16 + 8 * E2
+    if name == "k_vdiff":
+        return ntr * 16 * N3 + 8 * (N3 + 2 * NI) + 8 * NI + 8 * N2
+    return None
+
+
+def cpu_baseline(mesh, state, budget_s=12.0):
+    """Oracle (single-threaded C port) on whole steps of the same workload."""
+    from oracle.oracle import Oracle
+    orc = Oracle(mesh)
+    v = state.values.copy()
+    vo = state.valuesold.copy()
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        orc.step(state, v, vo)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 50:
+            break
+    upd = state.ntr * mesh.n3 * n / el
+    return {"value": upd, "unit": "tracer-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} full steps of the same workload (all {state.ntr} tracers), "
+                      f"{el:.1f} s, C oracle built with gcc -O2, one thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ssh-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is design-only "
+                                   "(halo exchange over RCCL is not built)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from fesom2_amd import gpu, meshgen as mg
+    t0 = time.perf_counter()
+    mesh, state, _ = mg.make_config(args.config)
+    t_gen = time.perf_counter() - t0
+    print(f"[bench] {args.config}: N2={mesh.nod2D} E2={mesh.edge2D} F2={mesh.elem2D} nl={mesh.nl} "
+          f"N3={mesh.n3} ntr={state.ntr} (generated in {t_gen:.1f}s)", file=sys.stderr, flush=True)
+    ctx = gpu.context_for(mesh, state)
+    dt = state.dt
+    ctx.solve_tracers_ale(dt, max(1, args.warmup))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ms_dev = ctx.solve_tracers_ale_timed(dt, args.steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    updates = state.ntr * mesh.n3 * args.steps
+    value = updates / wall
+    print(f"[bench] {args.steps} steps: wall {wall*1e3:.2f} ms, device {ms_dev:.2f} ms", file=sys.stderr, flush=True)
+
+    # per-kernel profile of a few more steps
+    ctx.profile_enable(True)
+    nprof = min(args.steps, 5)
+    ctx.solve_tracers_ale(dt, nprof)
+    prof = ctx.profile()
+    ctx.profile_enable(False)
+    kernels = {}
+    for name, (ms, cnt) in prof.items():
+        b = kernel_bytes(name, mesh, state.ntr)
+        avg = ms / max(cnt, 1)
+        kernels[name] = {"avg_ms": round(avg, 4), "launches": cnt,
+                         "alg_GBs": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+    dom = max(prof.items(), key=lambda kv: kv[1][0])[0]
+    dom_avg_ms = prof[dom][0] / prof[dom][1]
+    dom_bytes = kernel_bytes(dom, mesh, state.ntr)
+    achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.config}.txt")
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == dom:
+                traffic = float(parts[1])
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "alg_bytes_per_launch": int(dom_bytes)}
+    step_bytes = sum(kernel_bytes(n, mesh, state.ntr) or 0 for n in prof)
+    step_ms = sum(ms for ms, c in prof.values()) / nprof
+
+    # SSH solve
+    ctx.init_stiff_mat_ale(dt)
+    ctx.ssh_solve_preconditioner(2)
+    rhs = mg.ssh_rhs(mesh)
+    ctx.ssh_upload(np.zeros(mesh.nod2D), rhs)
+    ctx.ssh_solve_resident(1e-10, 5000, 1)
+    sst = ctx.ssh_solve_resident(1e-10, 5000, args.ssh_reps)
+    print(f"[bench] ssh: {sst}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mesh, state)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+    ctx.close()
+
+    out = {
+        "metric": "tracer-updates/s (3D node x tracer) + SSH solve ms at 1e-10 rel residual",
+        "value": value, "unit": "tracer-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh, T/S, velocity)",
+        "config": {"workload": f"{args.config}: {mesh.nod2D}-node jittered band mesh, {mesh.nl-1} levels, "
+                               f"{state.ntr} tracers, MFCT/QR4C/FCT + implicit vertical diffusion, dt={dt:g}s",
+                   "nod2D": mesh.nod2D, "elem2D": mesh.elem2D, "edge2D": mesh.edge2D, "levels": mesh.nl - 1,
+                   "n3": mesh.n3, "tracers": state.ntr, "parallelism": "single GPU"},
+        "ssh_solve_ms": round(sst["ms"], 3), "ssh_iters": sst["iters"], "ssh_converged": sst["converged"],
+        "ssh_precond": "diagonal (Jacobi)",
+        "device_ms_per_step": ms_dev / args.steps,
+        "step_alg_GBs": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
