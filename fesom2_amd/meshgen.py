@@ -419,6 +419,8 @@ def build_mesh(*, nx: int = 60, ny: int = 50, dx: float = 20e3,
             ddy = coord[:, 1] - cyb
             depth -= hgt * np.exp(-(ddx * ddx + ddy * ddy) / (2 * rad * rad))
         depth = np.maximum(depth, hmax - bump_height)
+    elif topo == "flat":
+        depth = np.full(N2, hmax)
     else:  # "shelf": wide range of depths down to a few layers
         f = _smooth_field(coord[:, 0], coord[:, 1] - y0, L, max(Ly, 1e-12), rng)
         f = (f - f.min()) / max(f.max() - f.min(), 1e-30)

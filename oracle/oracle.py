@@ -243,6 +243,41 @@ class Oracle:
         if rc:
             raise RuntimeError("diff_ver_part_impl_ale failed")
 
+    # ------------------------------------------------------- single routines
+    def gradient_elements(self, T):
+        out = np.zeros((self.mesh.elem2D, self.mesh.nl - 1, 2))
+        lib().fo_tracer_gradient_elements(C.byref(self.fm), _d(np.ascontiguousarray(T)), _d(out))
+        return out
+
+    def fill_up_dn_grad(self, tr_xy):
+        out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1, 4))
+        lib().fo_fill_up_dn_grad(C.byref(self.fm), _d(np.ascontiguousarray(tr_xy)), _d(out))
+        return out
+
+    def hor_upw1(self, uv, T):
+        out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1))
+        lib().fo_adv_tra_hor_upw1(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
+                                  _d(out), 1)
+        return out
+
+    def hor_mfct(self, uv, T, grad, num_ord=0.0):
+        out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1))
+        lib().fo_adv_tra_hor_mfct(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
+                                  float(num_ord), _d(out), _d(np.ascontiguousarray(grad)), 1)
+        return out
+
+    def ver_upw1(self, W, T):
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl))
+        lib().fo_adv_tra_ver_upw1(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
+                                  _d(out), 1)
+        return out
+
+    def ver_qr4c(self, W, T, num_ord=1.0):
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl))
+        lib().fo_adv_tra_ver_qr4c(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
+                                  float(num_ord), _d(out), 1)
+        return out
+
     # ------------------------------------------------------------ SSH
     def stiff_matrix(self, dt):
         m = self.mesh

@@ -1,0 +1,535 @@
+"""Pinning the CPU oracle (no GPU).
+
+The reference path cannot be built here (MOD_PARTIT.F90 needs an MPI module),
+and the reference's own tests pin nothing on this path, so every oracle routine
+is checked against what the mathematics fixes, computed independently of the
+oracle's code: plane fits, point-in-triangle geometry, exactness on
+polynomials, conservation, constant preservation, FCT bounds, a banded
+finite-volume solve, matrix identities and a direct sparse solve. The last test
+plants each mistake listed in fesom_oracle.c and checks that one of these pins
+fails.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from fesom2_amd import meshgen as mg
+from oracle import oracle as orc
+
+R = mg.R_EARTH
+
+
+@pytest.fixture(scope="module")
+def small():
+    mesh = mg.build_mesh(nx=20, ny=14, dx=20e3, jitter=0.2, seed=5, cartesian=True, nlayers=10,
+                         hmax=2000.0, min_layers=4, max_layers=10, topo="bumps", bump_height=900.0)
+    st = mg.build_state(mesh, seed=5, ntr=2, dt=900.0, dx=20e3)
+    return mesh, st
+
+
+@pytest.fixture(scope="module")
+def flat():
+    mesh = mg.build_mesh(nx=20, ny=14, dx=20e3, jitter=0.2, seed=6, cartesian=True, nlayers=10,
+                         hmax=2000.0, min_layers=4, max_layers=10, topo="flat")
+    st = mg.build_state(mesh, seed=6, ntr=2, dt=900.0, dx=20e3)
+    return mesh, st
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    mesh = mg.build_mesh(cartesian=False, target_nodes=2500, jitter=0.2, seed=7, nlayers=12, hmax=4000.0,
+                         dz0=20.0, min_layers=4, max_layers=12, topo="shelf")
+    st = mg.build_state(mesh, seed=7, ntr=2, dt=1800.0)
+    return mesh, st
+
+
+# ---------------------------------------------------------------- helpers
+def _metric_xy(mesh):
+    """Node coordinates in metres, unwrapped per element around node 1 (cyclic)."""
+    c = mesh.coord_nod2D
+    return c[:, 0] * R, c[:, 1] * R
+
+
+def _no_seam_elems(mesh):
+    x = mesh.coord_nod2D[mesh.elem2D_nodes, 0]
+    return (x.max(1) - x.min(1)) < mesh.cyclic_length / 4
+
+
+def _plane_grad(mesh, T, k):
+    """Least-squares-free exact plane through the 3 vertices (metric coordinates,
+    x scaled by cos(lat) of the element on the sphere), per element."""
+    en = mesh.elem2D_nodes
+    c = mesh.coord_nod2D
+    L = mesh.cyclic_length
+    x0 = c[en[:, 0], 0]
+    dx = np.stack([mg._trim_cyclic(c[en[:, j], 0] - x0, L) for j in range(3)], 1)
+    dy = np.stack([c[en[:, j], 1] - c[en[:, 0], 1] for j in range(3)], 1)
+    cosl = mesh.elem_cos[:, None]
+    X = dx * cosl * R
+    Y = dy * R
+    out = np.zeros((mesh.elem2D, 2))
+    for e in range(mesh.elem2D):
+        Amat = np.stack([np.ones(3), X[e], Y[e]], 1)
+        coef = np.linalg.solve(Amat, T[en[e], k])
+        out[e] = coef[1:]
+    return out
+
+
+def _wet(mesh):
+    k = np.arange(1, mesh.nl)
+    return (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+
+
+def _elem_wet(mesh):
+    k = np.arange(1, mesh.nl)
+    return (k[None, :] >= mesh.ulevels[:, None]) & (k[None, :] <= (mesh.nlevels - 1)[:, None])
+
+
+def _point_in_tri(p, a, b, c, L, tol=1e-9):
+    """Barycentric test; tol > 0 accepts points on the boundary, tol < 0 demands
+    strictly interior points."""
+    def v(q):
+        return np.array([mg._trim_cyclic(q[0] - a[0], L), q[1] - a[1]])
+    B, Cc, P = v(b), v(c), v(p)
+    M = np.stack([B, Cc], 1)
+    lam = np.linalg.solve(M, P)
+    return lam[0] >= -tol and lam[1] >= -tol and lam.sum() <= 1 + tol
+
+
+# ---------------------------------------------------------------- checks
+def check_ab2(mesh, st):
+    o = orc.Oracle(mesh)
+    v = st.values[0].copy()
+    vo = np.random.default_rng(1).normal(20, 1, v.shape)
+    vo0 = vo.copy()
+    out = o.adv_stage(st, v, vo, ID=1)
+    ab = out["valuesAB"]
+    exp = 1.6 * v - 0.6 * vo0
+    return np.max(np.abs(ab - exp)) <= 1e-13 * np.max(np.abs(exp)) and np.array_equal(vo, v)
+
+
+def check_gradient_plane(mesh):
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(2)
+    T = rng.normal(0, 1, (mesh.nod2D, mesh.nl - 1))
+    g = o.gradient_elements(T)
+    ok = _no_seam_elems(mesh) if mesh.cartesian else np.ones(mesh.elem2D, bool)
+    for k in (1, 3):
+        ref = _plane_grad(mesh, T, k - 1)
+        m = ok & (mesh.nlevels - 1 >= k)
+        if not np.allclose(g[m, k - 1], ref[m], rtol=1e-9, atol=1e-9 * np.abs(ref[m]).max()):
+            return False
+    return True
+
+
+def check_updown_geometry(mesh):
+    o = orc.Oracle(mesh)
+    ud = o.edge_up_dn_tri
+    c = mesh.coord_nod2D
+    L = mesh.cyclic_length
+    en = mesh.elem2D_nodes
+    bad = 0
+    checked = 0
+    for e in range(mesh.edge2D):
+        n1, n2 = mesh.edges[e]
+        d = np.array([mg._trim_cyclic(c[n2, 0] - c[n1, 0], L), c[n2, 1] - c[n1, 1]])
+        for side, (node, direc) in enumerate(((n1, -d), (n2, d))):
+            el = ud[e, side]
+            p = c[node] + 1e-4 * direc
+            # every element around the node that contains p must be a valid answer
+            cands = [q for q in mesh.nod_in_elem2D[node, :mesh.nod_in_elem2D_num[node]]
+                     if _point_in_tri(p, c[en[q, 0]], c[en[q, 1]], c[en[q, 2]], L)]
+            strict = [q for q in cands if _point_in_tri(p, c[en[q, 0]], c[en[q, 1]], c[en[q, 2]], L, -1e-9)]
+            if el >= 0:
+                checked += 1
+                if el not in cands:
+                    bad += 1
+            elif strict:
+                # a ray strictly inside an element around the node must be found
+                bad += 1
+    return bad == 0 and checked > 0
+
+
+def check_fill_up_dn_grad(mesh):
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(3)
+    T = rng.normal(0, 1, (mesh.nod2D, mesh.nl - 1))
+    trxy = o.gradient_elements(T)
+    g = o.fill_up_dn_grad(trxy)
+    ud = o.edge_up_dn_tri
+    ea = mesh.elem_area
+    bad = 0
+    for e in range(0, mesh.edge2D, 7):
+        n1, n2 = mesh.edges[e]
+        up, dn = ud[e]
+        nu, nl = mg.edge_level_range(mesh)
+        for k in range(nu[e], nl[e] + 1):
+            exp = np.zeros(4)
+            shared = up >= 0 and dn >= 0 and k <= min(mesh.nlevels_nod2D_min[n1], mesh.nlevels_nod2D_min[n2]) - 1
+            if shared:
+                exp[[0, 2]] = trxy[up, k - 1]
+                exp[[1, 3]] = trxy[dn, k - 1]
+            else:
+                for col, node in ((0, n1), (1, n2)):
+                    els = [q for q in mesh.nod_in_elem2D[node, :mesh.nod_in_elem2D_num[node]]
+                           if mesh.nlevels[q] - 1 >= k]
+                    w = ea[els]
+                    gm = (trxy[els, k - 1] * w[:, None]).sum(0) / w.sum()
+                    exp[col] = gm[0]
+                    exp[col + 2] = gm[1]
+            if not np.allclose(g[e, k - 1], exp, rtol=1e-12, atol=1e-20):
+                bad += 1
+    return bad == 0
+
+
+def check_mfct_linear(mesh, st):
+    """MFCT with ph=0 on a linear field: Tmean1 = Tmean2 = edge midpoint value, so
+    the flux equals T_mid times the upwind flux of T == 1."""
+    o = orc.Oracle(mesh)
+    x, y = _metric_xy(mesh)
+    if mesh.cartesian:
+        T = np.repeat((3.0 + 1e-5 * y)[:, None], mesh.nl - 1, 1) + 0.0
+        T = T + 2e-6 * np.repeat((x - x.mean())[:, None], mesh.nl - 1, 1)
+    else:
+        T = np.repeat((3.0 + 1e-5 * y)[:, None], mesh.nl - 1, 1)
+    trxy = o.gradient_elements(T)
+    grad = o.fill_up_dn_grad(trxy)
+    f = o.hor_mfct(st.uv, T, grad, 0.0)
+    one = o.hor_upw1(st.uv, np.ones_like(T))
+    c = mesh.coord_nod2D
+    seam = np.abs(c[mesh.edges[:, 0], 0] - c[mesh.edges[:, 1], 0]) > mesh.cyclic_length / 4
+    # avoid edges whose node means use elements across the seam
+    near = np.zeros(mesh.nod2D, bool)
+    en = mesh.elem2D_nodes
+    bad_el = ~_no_seam_elems(mesh)
+    near[en[bad_el].ravel()] = True
+    ok = ~seam & ~near[mesh.edges[:, 0]] & ~near[mesh.edges[:, 1]]
+    Tm = 0.5 * (T[mesh.edges[:, 0]] + T[mesh.edges[:, 1]])
+    exp = Tm * one
+    nu, nl = mg.edge_level_range(mesh)
+    k = np.arange(1, mesh.nl)
+    em = (k[None, :] >= nu[:, None]) & (k[None, :] <= nl[:, None]) & ok[:, None]
+    scale = np.abs(exp[em]).max()
+    return np.max(np.abs(f[em] - exp[em])) <= 1e-11 * scale
+
+
+def check_qr4c_cubic(mesh, st):
+    """Uniform dz, cell-mean tracer values of a polynomial profile f(z): pv=1 is
+    the 4th-order finite-volume interface value (exact for cubics) and pv=0 the
+    3rd-order upwind-biased one (exact for quadratics); interior interfaces."""
+    o = orc.Oracle(mesh)
+    zb = mesh.zbar_3d_n
+    W = st.w
+
+    def cell_mean(coef):
+        # coef: f(z) = sum_j coef[j] z^j ; mean over [zbar(k+1), zbar(k)]
+        F = lambda z: sum(c / (j + 1) * z ** (j + 1) for j, c in enumerate(coef))
+        top, bot = zb[:, :-1], zb[:, 1:]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            m = (F(top) - F(bot)) / (top - bot)
+        return np.where(_wet(mesh), m, 0.0)
+
+    ok = True
+    for coef, pv in (([10.0, 1e-3, 1e-6, 1e-9], 1.0), ([10.0, 1e-3, 1e-6], 0.0)):
+        f = o.ver_qr4c(W, cell_mean(coef), pv)
+        fi = sum(c * zb ** j for j, c in enumerate(coef))
+        for n in range(0, mesh.nod2D, 3):
+            nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+            for kk in range(nzmin + 2, nzmax - 1):
+                exp = -fi[n, kk - 1] * W[n, kk - 1] * mesh.area[n, kk - 1]
+                if abs(f[n, kk - 1] - exp) > 1e-11 * abs(exp) + 1e-30:
+                    ok = False
+    return ok
+
+
+def check_ver_upw1(mesh, st):
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(4)
+    T = rng.normal(5, 1, (mesh.nod2D, mesh.nl - 1))
+    W = st.w
+    f = o.ver_upw1(W, T)
+    for n in range(0, mesh.nod2D, 5):
+        nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+        for kk in range(nzmin + 1, nzmax):
+            w = W[n, kk - 1]
+            up = T[n, kk - 1] if w > 0 else T[n, kk - 2]
+            exp = -up * w * mesh.area[n, kk - 1]
+            if abs(f[n, kk - 1] - exp) > 1e-13 * max(abs(exp), 1e-300):
+                return False
+        if f[n, nzmax - 1] != 0.0:
+            return False
+    return True
+
+
+def _step(mesh, st, nsteps=1, values=None, Kv=None, schemes=None):
+    o = orc.Oracle(mesh)
+    st2 = st
+    if Kv is not None:
+        import dataclasses
+        st2 = dataclasses.replace(st, Kv=Kv)
+    v = (st.values if values is None else values).copy()
+    vo = v.copy()
+    for _ in range(nsteps):
+        o.step(st2, v, vo, schemes=schemes)
+    return v
+
+
+def check_constant(mesh, st):
+    v0 = np.where(_wet(mesh)[None], 5.0, 0.0) * np.ones((st.ntr, 1, 1))
+    v = _step(mesh, st, 3, values=v0)
+    w = _wet(mesh)
+    return max(np.max(np.abs(v[i][w] - 5.0)) for i in range(st.ntr)) < 1e-12 * 5.0
+
+
+def check_conservation(mesh, st, nsteps=5):
+    w = _wet(mesh)
+    vol = mesh.areasvol[:, :mesh.nl - 1] * mesh.hnode
+    c0 = [(vol * st.values[i]).sum() for i in range(st.ntr)]
+    v = _step(mesh, st, nsteps)
+    c1 = [(vol * v[i]).sum() for i in range(st.ntr)]
+    return max(abs(a - b) / abs(b) for a, b in zip(c1, c0)) < 1e-13
+
+
+def check_fct_bounds(mesh, st):
+    """Advection only (Kv = 0): T^{n+1} lies inside the a3 cluster bounds built
+    independently from LO and T^n (+-1 level, elements around the node).
+    Stressed so that the limiter acts everywhere: velocities x5 and unit noise
+    on the temperature field."""
+    o = orc.Oracle(mesh)
+    import dataclasses
+    rng = np.random.default_rng(11)
+    vals = st.values.copy()
+    vals[0] = np.where(_wet(mesh), vals[0] + rng.normal(0, 1, vals[0].shape), 0.0)
+    st0 = dataclasses.replace(st, Kv=np.zeros_like(st.Kv), uv=st.uv * 5, w=st.w * 5, w_e=st.w_e * 5,
+                              values=vals, valuesold=vals.copy())
+    st = st0
+    T = st.values[0].copy()
+    vo = T.copy()
+    out = o.adv_stage(st0, T.copy(), vo, ID=1)
+    LO = out["fct_LO"]
+    v = _step(mesh, st0, 1, values=st.values)[0]
+    w = _wet(mesh)
+    L = mesh.nl - 1
+    mx = np.maximum(LO, T)
+    mn = np.minimum(LO, T)
+    en = mesh.elem2D_nodes
+    ewet = _elem_wet(mesh)
+    emax = np.where(ewet, mx[en].max(1), -1e3)
+    emin = np.where(ewet, mn[en].min(1), 1e3)
+    bad = 0
+    for n in range(mesh.nod2D):
+        els = mesh.nod_in_elem2D[n, :mesh.nod_in_elem2D_num[n]]
+        tvmax = emax[els].max(0)
+        tvmin = emin[els].min(0)
+        nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+        for k in range(nzmin, nzmax):
+            lo_k = max(nzmin, k - 1) if nzmin < k < nzmax - 1 else k
+            hi_k = min(nzmax - 1, k + 1) if nzmin < k < nzmax - 1 else k
+            bmax = tvmax[lo_k - 1:hi_k].max()
+            bmin = tvmin[lo_k - 1:hi_k].min()
+            tol = 1e-11 * max(abs(bmax), 1.0)
+            if v[n, k - 1] > bmax + tol or v[n, k - 1] < bmin - tol:
+                bad += 1
+    return bad == 0
+
+
+def check_vdiff_fv(mesh, st):
+    """Implicit diffusion = banded finite-volume solve built from the physics:
+    areasvol_k h_k (T'_k - T_k) = F_{k-1/2} - F_{k+1/2},
+    F_{k+1/2} = Kv_{k+1} dt area_{k+1} (T'_k - T'_{k+1}) / (Z_k - Z_{k+1})."""
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(9)
+    T = rng.normal(10, 2, (mesh.nod2D, mesh.nl - 1))
+    import dataclasses
+    Kv = 10 ** rng.uniform(-5, -2, st.Kv.shape)
+    st2 = dataclasses.replace(st, Kv=Kv)
+    T1 = T.copy()
+    o.vdiff(st2, T1, ID=301)
+    dt = st.dt
+    for n in range(0, mesh.nod2D, 4):
+        nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+        ks = np.arange(nzmin, nzmax)       # layers
+        m = len(ks)
+        h = mesh.hnode_new[n, ks - 1]
+        zb = np.zeros(m + 1)
+        zb[m] = mesh.zbar_n_bot[n]
+        for j in range(m - 1, -1, -1):
+            zb[j] = zb[j + 1] + h[j]
+        Zc = zb[1:] + h / 2.0
+        av = mesh.areasvol[n, ks - 1]
+        ab = np.zeros((3, m))
+        ab[1, :] = av * h
+        for j in range(m):
+            if j + 1 < m:
+                g = Kv[n, ks[j]] * dt * mesh.area[n, ks[j]] / (Zc[j] - Zc[j + 1])
+                ab[1, j] += g
+                ab[1, j + 1] += g
+                ab[0, j + 1] -= g      # super-diagonal (row j, col j+1)
+                ab[2, j] -= g          # sub-diagonal (row j+1, col j)
+        rhs = av * h * T[n, ks - 1]
+        sol = sla.solve_banded((1, 1), ab, rhs)
+        if not np.allclose(T1[n, ks - 1], sol, rtol=1e-11, atol=1e-11):
+            return False
+    return True
+
+
+def check_stiffness(mesh, st):
+    o = orc.Oracle(mesh)
+    rp, ci, v = o.stiff_matrix(st.dt)
+    A = sp.csr_matrix((v.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    sym = abs(A - A.T).max() / abs(A).max()
+    rs = np.asarray(A.sum(1)).ravel()
+    mass = mesh.areasvol[:, 0] / st.dt
+    if sym > 1e-14 or np.max(np.abs(rs - mass)) > 1e-12 * mass.max():
+        return False
+    if not all(ci[rp[r]] == r for r in range(mesh.nod2D)):
+        return False
+    lmin = spla.eigsh(A.tocsc(), k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+    return lmin > 0
+
+
+def check_stiffness_linear_kernel(mesh, st):
+    """Flat bottom: the stiffness part annihilates a linear field at interior
+    nodes (closed median-dual cells away from the walls)."""
+    o = orc.Oracle(mesh)
+    rp, ci, v = o.stiff_matrix(st.dt)
+    A = sp.csr_matrix((v.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    K = A - sp.diags(mesh.areasvol[:, 0] / st.dt)
+    x, y = _metric_xy(mesh)
+    eta = 1e-6 * y
+    r = K @ eta
+    wall = np.zeros(mesh.nod2D, bool)
+    wall[mesh.edges[mesh.edge_tri[:, 1] < 0].ravel()] = True
+    near = wall.copy()
+    for n in np.nonzero(wall)[0]:
+        near[ci[rp[n]:rp[n + 1]]] = True
+    scale = np.abs(K.diagonal()).max() * np.abs(eta).max()
+    return np.max(np.abs(r[~near])) < 1e-10 * scale
+
+
+def check_precond_v1(mesh, st):
+    o = orc.Oracle(mesh)
+    rp, ci, v = o.stiff_matrix(st.dt)
+    pr = o.preconditioner(rp, ci, v, 1)
+    A = sp.csr_matrix((v.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    Dinv = sp.diags(1.0 / A.diagonal())
+    Mref = (2 * Dinv - Dinv @ A @ Dinv).tocsr()
+    P = sp.csr_matrix((pr.copy(), ci.copy(), rp.copy()), shape=A.shape)
+    return abs(P - Mref).max() <= 1e-14 * abs(Mref).max()
+
+
+def check_cg(mesh, st, variant=2):
+    o = orc.Oracle(mesh)
+    rp, ci, v = o.stiff_matrix(st.dt)
+    pr = o.preconditioner(rp, ci, v, variant)
+    rhs = mg.ssh_rhs(mesh, seed=1)
+    x, stats = o.cg(rp, ci, v, pr, rhs, soltol=1e-10, maxiter=3000)
+    A = sp.csr_matrix((v.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    xd = spla.spsolve(A.tocsc(), rhs)
+    res = np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs)
+    err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    return stats["converged"] and res < 1e-10 and err < 1e-6
+
+
+# ---------------------------------------------------------------- tests
+def test_ab2_closed_form(small):
+    assert check_ab2(*small)
+
+
+def test_gradient_plane_fit(small, sphere):
+    assert check_gradient_plane(small[0])
+    assert check_gradient_plane(sphere[0])
+
+
+def test_updown_triangles_geometry(small, sphere):
+    assert check_updown_geometry(small[0])
+    assert check_updown_geometry(sphere[0])
+
+
+def test_fill_up_dn_grad_definition(small):
+    assert check_fill_up_dn_grad(small[0])
+
+
+def test_mfct_linear_exact(small):
+    assert check_mfct_linear(*small)
+
+
+def test_qr4c_polynomial_exact(small):
+    assert check_qr4c_cubic(*small)
+
+
+def test_upw1_vertical_closed_form(small):
+    assert check_ver_upw1(*small)
+
+
+def test_constant_preserved(small, sphere):
+    assert check_constant(*small)
+    assert check_constant(*sphere)
+
+
+def test_conservation(small, sphere, c0):
+    assert check_conservation(*small)
+    assert check_conservation(*sphere)
+    assert check_conservation(c0[0], c0[1], nsteps=10)
+
+
+def test_fct_bounds(small):
+    assert check_fct_bounds(*small)
+
+
+def test_vdiff_finite_volume(small):
+    assert check_vdiff_fv(*small)
+
+
+def test_stiffness_properties(small, sphere):
+    assert check_stiffness(*small)
+    assert check_stiffness(*sphere)
+
+
+def test_stiffness_linear_kernel(flat):
+    assert check_stiffness_linear_kernel(*flat)
+
+
+def test_precond_variant1_identity(small):
+    assert check_precond_v1(*small)
+
+
+def test_cg_vs_direct(small):
+    assert check_cg(*small, variant=2)
+    assert check_cg(*small, variant=1)
+
+
+MUTATIONS = {
+    1: lambda s, f: check_ab2(*s),
+    2: lambda s, f: check_gradient_plane(s[0]),
+    3: lambda s, f: check_updown_geometry(s[0]),
+    4: lambda s, f: check_fill_up_dn_grad(s[0]),
+    5: lambda s, f: check_constant(*s),
+    6: lambda s, f: check_mfct_linear(*s),
+    7: lambda s, f: check_qr4c_cubic(*s),
+    8: lambda s, f: check_fct_bounds(*s),
+    10: lambda s, f: check_conservation(*s),
+    12: lambda s, f: check_vdiff_fv(*s),
+    13: lambda s, f: check_vdiff_fv(*s),
+    14: lambda s, f: check_stiffness(*s),
+    15: lambda s, f: check_cg(*s),
+    16: lambda s, f: check_precond_v1(*s),
+    17: lambda s, f: check_ver_upw1(*s),
+    18: lambda s, f: check_fct_bounds(*s),
+}
+
+
+@pytest.mark.parametrize("mut", sorted(MUTATIONS))
+def test_planted_mistake_is_caught(small, flat, mut):
+    """Each planted mistake in the oracle makes its pinning check fail."""
+    assert MUTATIONS[mut](small, flat), "check must pass on the faithful oracle"
+    orc.set_mutation(mut)
+    try:
+        caught = not MUTATIONS[mut](small, flat)
+    except Exception:
+        caught = True
+    finally:
+        orc.set_mutation(0)
+    assert caught, f"mutation {mut} not caught"
