@@ -32,39 +32,52 @@ HBM_PEAK_GBS = 8000.0
 def kernel_bytes(name, mesh, ntr):
     """Algorithmic bytes of one launch of a step kernel (all tracers of the batch).
 
-    Each array a kernel must read or write is counted once over the entries the
-    kernel touches (wet levels; k_init_ab touches whole columns), FP64 = 8 B,
-    int32 = 4 B. Per-tracer terms scale with ntr, mesh/dynamics terms do not.
+    Every array a kernel must read or write is counted once over the entries it
+    covers: wet layers N3, wet interfaces NI = N3 + N2, edge levels E3 (the
+    nu12..nl12 ranges), element levels F3; FP64 = 8 B, int32 = 4 B. Values of
+    neighbouring columns that a gather reads again are not counted twice.
+    Per-tracer terms scale with ntr; mesh and dynamics terms do not.
     """
-    N2, E2, F2, L, NL = mesh.nod2D, mesh.edge2D, mesh.elem2D, mesh.nl - 1, mesh.nl
+    N2, E2, F2, L = mesh.nod2D, mesh.edge2D, mesh.elem2D, mesh.nl - 1
     N3, E3, F3 = mesh.n3, mesh.e3, mesh.f3
-    NI = N3 + N2          # wet interfaces
+    NI = N3 + N2
+    lists = 4 * (N2 + 1) + 8 * E2           # node -> edge gather lists
+    if name == "kA_edge":                    # T^n, T^n-1 ; LO_h, A_h | uv, helem, edge/elem geometry
+        return ntr * (16 * N3 + 16 * E3) + 24 * F3 + 88 * E2 + 76 * F2 + 24 * N2
+    if name == "kB_node":                    # LO_h, T^n, T^n-1 ; LO, A_v | w, w_e, area, zbar, Z, h, h_new, areasvol
+        return ntr * (8 * E3 + 16 * N3 + 8 * N3 + 8 * NI) + 32 * NI + 32 * N3 + lists + 8 * E2 + 8 * N2
+    if name == "kC_node":                    # LO, T^n, A_v, A_h ; R+, R- | h_new, areasvol, edge info
+        return ntr * (16 * N3 + 8 * NI + 8 * E3 + 16 * N3) + 16 * N3 + lists + 32 * E2 + 8 * N2
+    if name == "kD_node":                    # R+-, A_v, A_h, LO, T^n ; T^n+1, T^n -> valuesold | h, h_new, areasvol, area, Kv
+        return ntr * (16 * N3 + 8 * NI + 8 * E3 + 16 * N3 + 16 * N3) + 32 * N3 + 8 * NI + lists + 16 * E2 + 16 * N2
+    # staged per-stage kernels (FGPU_STEP=staged)
     if name == "k_init_ab":
-        return ntr * 32 * N2 * L                       # T, Told read; TAB, Told write
+        return ntr * 32 * N2 * L
     if name == "k_tr_xy":
-        return ntr * (8 * N3 + 16 * F3) + 48 * F2 + 12 * F2 + 8 * F2
+        return ntr * (8 * N3 + 16 * F3) + 68 * F2
     if name == "k_edge_flux":
-        per_tr = 16 * N3 + 16 * F3 + 16 * E3           # T, TAB; tr_xy; LO_h, A_h written
-        shared = 24 * F3 + E2 * (8 + 16 + 16 + 16 + 8) + F2 * 8   # uv, helem; edge geometry
-        return ntr * per_tr + shared
+        return ntr * (16 * N3 + 16 * F3 + 16 * E3) + 24 * F3 + 88 * E2 + 8 * F2
     if name == "k_node_lo":
-        per_tr = 8 * E3 + 16 * N3 + 16 * N3            # LO_h gather; T, TAB; fct_LO, A_v written
-        shared = 8 * (2 * NI) + 8 * (4 * N3) + 8 * (2 * NI) + N2 * 16 + 8 * E2
-        return ntr * per_tr + shared
+        return ntr * (8 * E3 + 16 * N3 + 16 * N3) + 16 * NI + 32 * N3 + 16 * NI + lists + 8 * E2
     if name == "k_elem_aux":
         return ntr * (16 * N3 + 16 * F2 * L) + 12 * F2
     if name == "k_node_limit":
-        per_tr = 16 * F3 + 8 * N3 + 8 * NI + 8 * E3 + 16 * N3   # aux; LO; A_v; A_h; R+/R- written
-        return ntr * per_tr + 16 * N3 + N2 * 16 + 8 * E2
+        return ntr * (16 * F3 + 8 * N3 + 8 * NI + 8 * E3 + 16 * N3) + 16 * N3 + lists + 8 * E2
     if name == "k_edge_limit":
         return ntr * (16 * E3 + 16 * N3) + 16 * E2
     if name == "k_node_update":
-        per_tr = 8 * NI + 16 * N3 + 8 * E3 + 16 * N3   # A_v; R+/R-; A_h; T read+write... LO read
-        per_tr += 8 * N3
-        return ntr * per_tr + 24 * N3 + N2 * 16 + 8 * E2
+        return ntr * (8 * NI + 16 * N3 + 8 * E3 + 16 * N3 + 8 * N3) + 24 * N3 + lists + 8 * E2
     if name == "k_vdiff":
         return ntr * 16 * N3 + 8 * (N3 + 2 * NI) + 8 * NI + 8 * N2
     return None
+
+
+def step_model_bytes(mesh, ntr):
+    """Whole-step algorithmic bytes of the transport model: per tracer 88 N3 + 16 E3
+    (state in/out, LO, A_h, A_v, R+-), shared 24 F3 + 72 N3 + mesh terms."""
+    N2, E2, F2 = mesh.nod2D, mesh.edge2D, mesh.elem2D
+    N3, E3, F3 = mesh.n3, mesh.e3, mesh.f3
+    return ntr * (88 * N3 + 16 * E3) + 24 * F3 + 72 * N3 + 72 * E2 + 80 * F2 + 48 * N2
 
 
 def cpu_baseline(mesh, state, budget_s=12.0):
@@ -176,6 +189,8 @@ def main():
         "ssh_precond": "diagonal (Jacobi)",
         "device_ms_per_step": ms_dev / args.steps,
         "step_alg_GBs": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
+        "step_model_bytes": int(step_model_bytes(mesh, state.ntr)),
+        "step_model_frac_of_hbm": round(step_model_bytes(mesh, state.ntr) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -41,7 +41,17 @@ struct DevMesh {
     const double *Z3d;                    // N2*L
     const double *zbar_n_bot;             // N2
     const double *zbar_e_bot, *zbar_e_srf;  // F2
+    // packed per-edge records of the fused edge pass (one scalar batch per edge)
+    const int *erec_i;       // E2*ERI
+    const double *erec_d;    // E2*ERD
+    const double *zn_vd;     // N2*L: Z levels rebuilt from hnode_new for the implicit diffusion
 };
+// erec_i fields
+enum { ER_N1, ER_N2, ER_EL1, ER_EL2, ER_NL1, ER_NU1, ER_NL2, ER_NU2, ER_UA0, ER_UA1, ER_UA2, ER_DA0, ER_DA1,
+       ER_DA2, ER_RULE, ER_NZMIN, ER_NZMAX, ER_U1, ER_U2, ER_NL1N, ER_NL2N, ER_I_PAD1, ER_I_PAD2, ER_I_PAD3, ERI };
+// erec_d fields
+enum { ER_DX1, ER_DY1, ER_DX2, ER_DY2, ER_A, ER_EDX, ER_EDY, ER_GU0, ER_GU1, ER_GU2, ER_GU3, ER_GU4, ER_GU5,
+       ER_GD0, ER_GD1, ER_GD2, ER_GD3, ER_GD4, ER_GD5, ER_D_PAD, ERD };
 
 struct DevDyn {
     const double *uv, *w, *w_e, *w_i, *Kv;
@@ -54,7 +64,7 @@ struct DevTracers {
     int ntr;
     const int *id, *abo, *hor, *ver, *lim, *ivd;   // ntr each
     const double *ph, *pv;                         // ntr each
-    size_t sN, sNI, sE, sF2, sOld;                 // strides: N2*L, N2*NL, E2*L, F2*L*2, N2*L*2
+    size_t sN, sNI, sE, sF2, sOld, sG;             // strides: N2*L, N2*NL, E2*L, F2*L*2, N2*L*2, N2*L*2
     double *T, *Told, *TAB;
     double *tr_xy;            // F2*L*2
     double *lo_h, *a_h;       // E2*L
@@ -62,6 +72,7 @@ struct DevTracers {
     double *a_v;              // N2*NL
     double *aux;              // F2*L*2
     double *rp, *rm;          // N2*L
+    double *gmean;            // N2*L*2 node-mean gradients of the fused step
     double *dttf_h, *dttf_v;  // N2*L (per-tracer entry points only; one tracer's worth)
     double *dbg_av;           // N2*NL limited vertical flux (per-tracer entry points only)
     double *dbg_grad;         // E2*L*4 materialised edge_up_dn_grad (per-tracer entry points only)
@@ -89,6 +100,7 @@ struct fgpu_context {
     std::vector<int> h_id, h_abo, h_hor, h_ver, h_lim, h_ivd;
     std::vector<double> h_ph, h_pv;
     std::vector<char> h_configured;
+    int *d_kneed = nullptr;   // first level of a node whose mean gradient an edge may use
     int *d_cfg_i = nullptr;
     double *d_cfg_d = nullptr;
     // host copies needed for setup
@@ -98,6 +110,8 @@ struct fgpu_context {
     double graph_dt = 0.0;
     bool graph_valid = false;
     // profiling
+    bool fused = true;
+    int step_blocks = 8192;   // grid cap of the fused step kernels   // batched step: fused kernels (false: one kernel per reference stage)
     bool prof = false;
     std::vector<ProfEntry> prof_entries;
     struct Pending { int idx; hipEvent_t e0, e1; };
@@ -111,7 +125,10 @@ struct fgpu_context {
     int *ssh_rowptr = nullptr, *ssh_colind = nullptr;
     double *ssh_val = nullptr, *ssh_pr = nullptr, *ssh_diag_inv = nullptr;
     double *ssh_x = nullptr, *ssh_x0 = nullptr, *ssh_rhs = nullptr;
-    double *ssh_r = nullptr, *ssh_z = nullptr, *ssh_p = nullptr, *ssh_q = nullptr;
+    double *ssh_r = nullptr, *ssh_z = nullptr, *ssh_p = nullptr, *ssh_p1 = nullptr, *ssh_q = nullptr;
+    hipGraphExec_t ssh_graph = nullptr;
+    double ssh_graph_tol = 0.0;
+    int ssh_graph_maxiter = 0, ssh_graph_variant = -1, ssh_last_iters = 0;
     double *ssh_part = nullptr;   // per-block partial sums
     double *ssh_out = nullptr;    // device stats
     int ssh_blocks = 0;
@@ -140,3 +157,7 @@ struct ProfScope {
     ~ProfScope();
 };
 int fgpu_prof_flush(fgpu_context *c);
+
+// fused batched step (fgpu_step.hip): four launches per step for all tracers
+int fgpu_enqueue_fused_step(fgpu_context *c, double dt);
+int fgpu_compute_zn(fgpu_context *c);

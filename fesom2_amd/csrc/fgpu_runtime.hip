@@ -12,6 +12,7 @@
 //   * SSH sparsity with the diagonal first (oce_ale.F90:1440-1496).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -138,6 +139,12 @@ extern "C" int fgpu_create(fgpu_context **out, int device)
     FGPU_HIP(hipSetDevice(device));
     fgpu_context *c = new fgpu_context();
     c->device = device;
+    const char *sv = std::getenv("FGPU_STEP");
+    c->fused = !(sv && std::string(sv) == "staged");
+    if (const char *bv = std::getenv("FGPU_STEP_BLOCKS")) {
+        int v = std::atoi(bv);
+        if (v >= 8 && v <= (1 << 20)) c->step_blocks = v;
+    }
     // default priority: stream priorities are left alone on purpose
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -162,6 +169,7 @@ extern "C" int fgpu_destroy(fgpu_context *c)
     for (auto &p : c->prof_pending) { (void)hipEventDestroy(p.e0); (void)hipEventDestroy(p.e1); }
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->ssh_graph) (void)hipGraphExecDestroy(c->ssh_graph);
     free_list(c->tracer_allocs);
     free_list(c->allocs);
     (void)hipStreamDestroy(c->stream);
@@ -301,6 +309,11 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
             ne_idx[fill[ed[2 * e + 1]]++] = 2 * e + 1;
         }
     }
+    {
+        int maxdeg = 0;
+        for (int n = 0; n < N2; ++n) maxdeg = std::max(maxdeg, ne_ptr[n + 1] - ne_ptr[n]);
+        if (maxdeg > 64 || kmax > 64) return fgpu_fail(FGPU_ERR_ARG, "node degree above 64");
+    }
     // ---- up/downwind triangles
     std::vector<int> updn(2 * (size_t)E2);
     for (int e = 0; e < E2; ++e) {
@@ -315,6 +328,22 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
                                   cyclic_length, n1, xm);
         updn[2 * e + 1] = updn_search(coord_nod2D, en.data(), nie.data(), nod_in_elem2D_num, kmax,
                                       cyclic_length, n2, x);
+    }
+    // ---- lowest level at which an edge needs the mean gradient of a node
+    //      (the non-shared levels of fill_up_dn_grad, oce_muscl_adv.F90:383-520)
+    std::vector<int> kneed(N2);
+    for (int n = 0; n < N2; ++n) kneed[n] = nlevels_nod2D[n];
+    for (int e = 0; e < E2; ++e) {
+        int n1 = ed[2 * e], n2 = ed[2 * e + 1];
+        bool both = updn[2 * e] >= 0 && updn[2 * e + 1] >= 0;
+        int nzmin = std::max(ulevels_nod2D_max[n1], ulevels_nod2D_max[n2]);
+        int nzmax = std::min(nlevels_nod2D_min[n1], nlevels_nod2D_min[n2]);
+        for (int nn : {n1, n2}) {
+            int lo;
+            if (!both || ulevels_nod2D[nn] <= nzmin - 1) lo = ulevels_nod2D[nn];
+            else lo = nzmax;
+            kneed[nn] = std::min(kneed[nn], lo);
+        }
     }
     // ---- upload
     DevMesh &m = c->m;
@@ -343,6 +372,46 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
     UP(p_i, ulevels_nod2D_max, N2); m.ulevn_max = p_i;
     UP(p_i, ne_ptr.data(), ne_ptr.size()); m.ne_ptr = p_i;
     UP(p_i, ne_idx.data(), ne_idx.size()); m.ne_idx = p_i;
+    UP(p_i, kneed.data(), kneed.size()); c->d_kneed = p_i;
+    {
+        // packed edge records (mesh arrays and the fill_up_dn_grad level rule)
+        std::vector<int> ri((size_t)E2 * ERI, 0);
+        std::vector<double> rd((size_t)E2 * ERD, 0.0);
+        for (int e = 0; e < E2; ++e) {
+            int *r = ri.data() + (size_t)e * ERI;
+            double *q = rd.data() + (size_t)e * ERD;
+            const int n1 = ed[2 * e], n2 = ed[2 * e + 1], el1 = et[2 * e], el2 = et[2 * e + 1];
+            r[ER_N1] = n1; r[ER_N2] = n2; r[ER_EL1] = el1; r[ER_EL2] = el2;
+            for (int j = 0; j < 4; ++j) r[ER_NL1 + j] = einfo[4 * e + j];
+            const int up = updn[2 * e], dn = updn[2 * e + 1];
+            r[ER_RULE] = (up >= 0 && dn >= 0) ? 1 : 0;
+            if (r[ER_RULE]) {
+                for (int j = 0; j < 3; ++j) {
+                    r[ER_UA0 + j] = en[3 * up + j];
+                    r[ER_DA0 + j] = en[3 * dn + j];
+                    q[ER_GU0 + j] = gradient_sca[6 * up + j];
+                    q[ER_GU3 + j] = gradient_sca[6 * up + 3 + j];
+                    q[ER_GD0 + j] = gradient_sca[6 * dn + j];
+                    q[ER_GD3 + j] = gradient_sca[6 * dn + 3 + j];
+                }
+                r[ER_NZMIN] = std::max(ulevels_nod2D_max[n1], ulevels_nod2D_max[n2]);
+                r[ER_NZMAX] = std::min(nlevels_nod2D_min[n1], nlevels_nod2D_min[n2]);
+            }
+            r[ER_U1] = ulevels_nod2D[n1]; r[ER_U2] = ulevels_nod2D[n2];
+            r[ER_NL1N] = nlevels_nod2D[n1]; r[ER_NL2N] = nlevels_nod2D[n2];
+            q[ER_DX1] = edge_cross_dxdy[4 * e]; q[ER_DY1] = edge_cross_dxdy[4 * e + 1];
+            double a = 6367500.0 * elem_cos[el1];
+            if (el2 >= 0) {
+                q[ER_DX2] = edge_cross_dxdy[4 * e + 2];
+                q[ER_DY2] = edge_cross_dxdy[4 * e + 3];
+                a = 0.5 * (a + 6367500.0 * elem_cos[el2]);   // oce_adv_tra_hor.F90:598-612
+            }
+            q[ER_A] = a;
+            q[ER_EDX] = edge_dxdy[2 * e]; q[ER_EDY] = edge_dxdy[2 * e + 1];
+        }
+        UP(p_i, ri.data(), ri.size()); m.erec_i = p_i;
+        UP(p_d, rd.data(), rd.size()); m.erec_d = p_d;
+    }
     UP(p_d, elem_area, F2); m.elem_area = p_d;
     UP(p_d, elem_cos, F2); m.elem_cos = p_d;
     UP(p_d, edge_dxdy, 2 * (size_t)E2); m.edge_dxdy = p_d;
@@ -406,6 +475,7 @@ extern "C" int fgpu_set_vertical(fgpu_context *c, const double *area, const doub
     if ((rc = upload_or_refresh(c, &c->m.zbar_n_bot, zbar_n_bot, N2))) return rc;
     if ((rc = upload_or_refresh(c, &c->m.zbar_e_bot, zbar_e_bot, F2))) return rc;
     if ((rc = upload_or_refresh(c, &c->m.zbar_e_srf, zbar_e_srf, F2))) return rc;
+    if ((rc = fgpu_compute_zn(c))) return rc;
     c->vert_ready = true;
     c->graph_valid = false;
     return FGPU_OK;
@@ -479,11 +549,12 @@ extern "C" int fgpu_tracers_init(fgpu_context *c, int num_tracers)
     t.sE = E2 * L;
     t.sF2 = F2 * L * 2;
     t.sOld = N2 * L * 2;
+    t.sG = N2 * L * 2;
     int rc;
     struct A { double **p; size_t n; } arrs[] = {
         {&t.T, t.sN * nt}, {&t.Told, t.sOld * nt}, {&t.TAB, t.sN * nt}, {&t.tr_xy, t.sF2 * nt},
         {&t.lo_h, t.sE * nt}, {&t.a_h, t.sE * nt}, {&t.fct_lo, t.sN * nt}, {&t.a_v, t.sNI * nt},
-        {&t.aux, t.sF2 * nt}, {&t.rp, t.sN * nt}, {&t.rm, t.sN * nt}, {&t.dttf_h, t.sN},
+        {&t.aux, t.sF2 * nt}, {&t.rp, t.sN * nt}, {&t.rm, t.sN * nt}, {&t.gmean, t.sG * nt}, {&t.dttf_h, t.sN},
         {&t.dttf_v, t.sN}, {&t.dbg_av, t.sNI}, {&t.dbg_grad, t.sE * 4}};
     for (auto &a : arrs) {
         if ((rc = fgpu_alloc(c, (void **)a.p, a.n * sizeof(double), true))) return rc;

@@ -5,21 +5,14 @@
 //   order (the reference's serial order), so the assembly is deterministic
 //   and free of atomics.
 // * ssh_solve_preconditioner (solver.F90:31-117): one thread per row.
-// * ssh_solve_cg (solver.F90:120-385): one persistent cooperative launch holds
-//   the whole iteration on the device. Rows are owned by a fixed grid-stride
-//   assignment; each reduction writes one partial per block, the grid
-//   synchronises, and every block then adds the partials in the same fixed
-//   order, so all blocks see the same bit pattern and take the same branch
-//   (convergence, breakdown) without a host round trip. Iterates, residual
-//   and dot products are FP64.
-#include <hip/hip_cooperative_groups.h>
-
+// * ssh_solve_cg (solver.F90:120-385): the iteration runs on the device as a
+//   captured graph of small kernels; convergence, breakdown and the iteration
+//   cap are decided on the device and later kernels of a replay return at once.
+//   Iterates, residual and dot products are FP64.
 #include <cmath>
 #include <cstring>
 
 #include "fgpu_internal.h"
-
-namespace cg = cooperative_groups;
 
 #define SBLK 256
 
@@ -80,17 +73,35 @@ __global__ void __launch_bounds__(SBLK) k_precond(int n, const int *rowptr, cons
 }
 
 // ------------------------------------------------------------------------------
-// CG
+// CG as a hipGraph of small kernels (a grid-wide barrier costs far more than a
+// kernel boundary on this GPU). One iteration of the reference loop is split
+//   k_cg_a(i): convergence test of iteration i-1, beta, p_i = z + beta p_{i-1},
+//              q = A p_i, partial p.q
+//   k_cg_b(i): alpha, x and r update, z = M^-1 r, partial r.z and r.r
+// Every block re-adds the per-block partials of the previous kernel in the
+// same fixed order, so all blocks hold identical scalars and take identical
+// branches. p is double-buffered (iteration parity) because k_cg_a reads
+// p_{i-1} of neighbour rows while writing p_i; q = A p_i recomputes the
+// neighbours' p_i from z and p_{i-1} with the same expression.
 // ------------------------------------------------------------------------------
+struct CGState {
+    int iter;       // completed iterations (written by k_cg_b / init)
+    int par;        // parity of the iteration in flight (written by k_cg_a)
+    int done;       // 0 running, 1 converged, 2 breakdown, 3 maxiter
+    int nbreak, nneg, pad;
+    double s_old[2];
+    double rtol, rr_last;
+};
+
 struct CGArgs {
-    int n, variant, maxiter;
+    int n, variant, maxiter, nb;
     const int *rowptr, *colind;
     const double *val, *pr, *dinv;
     double *x;
     const double *rhs;
-    double *r, *z, *p, *q;
-    double *part;   // 6 slots x nblocks
-    double *out;    // iters, converged, nbreakdown, nnegrz, resid, rtol
+    double *r, *z, *p0, *p1, *q;
+    double *part;   // 4 slots x nb: [0] b.b / p.q  [1] r.z  [2] r.r
+    CGState *st;
     double soltol;
 };
 
@@ -101,120 +112,187 @@ __device__ __forceinline__ double row_dot(const int *rowptr, const int *colind, 
     return s;
 }
 
-// deterministic grid-wide sum of one or two values
-__device__ void grid_sum2(double a, double b, double *part, int slot, cg::grid_group &g, double &sa, double &sb)
+// block tree reduction of up to three values; result valid in every thread
+__device__ void block_sum3(double &a, double &b, double &c)
 {
-    __shared__ double sh[2][SBLK];
-    const int tid = threadIdx.x, nb = gridDim.x;
-    sh[0][tid] = a;
-    sh[1][tid] = b;
+    __shared__ double sh[3][SBLK];
+    const int t = threadIdx.x;
+    sh[0][t] = a; sh[1][t] = b; sh[2][t] = c;
     __syncthreads();
     for (int w = SBLK / 2; w > 0; w >>= 1) {
-        if (tid < w) {
-            sh[0][tid] += sh[0][tid + w];
-            sh[1][tid] += sh[1][tid + w];
-        }
+        if (t < w) { sh[0][t] += sh[0][t + w]; sh[1][t] += sh[1][t + w]; sh[2][t] += sh[2][t + w]; }
         __syncthreads();
     }
-    if (tid == 0) {
-        part[(size_t)slot * nb + blockIdx.x] = sh[0][0];
-        part[(size_t)(slot + 1) * nb + blockIdx.x] = sh[1][0];
-    }
-    __threadfence();
-    g.sync();
-    double s0 = 0.0, s1 = 0.0;
-    for (int j = tid; j < nb; j += SBLK) {
-        s0 += part[(size_t)slot * nb + j];
-        s1 += part[(size_t)(slot + 1) * nb + j];
-    }
-    __syncthreads();
-    sh[0][tid] = s0;
-    sh[1][tid] = s1;
-    __syncthreads();
-    for (int w = SBLK / 2; w > 0; w >>= 1) {
-        if (tid < w) {
-            sh[0][tid] += sh[0][tid + w];
-            sh[1][tid] += sh[1][tid + w];
-        }
-        __syncthreads();
-    }
-    sa = sh[0][0];
-    sb = sh[1][0];
+    a = sh[0][0]; b = sh[1][0]; c = sh[2][0];
     __syncthreads();
 }
 
-__device__ __forceinline__ void apply_prec(const CGArgs &A, int row)
+// every block adds the nb partials of slot s0 (and s0+1 if two) in one fixed order
+__device__ void sum_partials(const CGArgs &A, int s0, bool two, double &a, double &b)
 {
-    if (A.variant == 2) A.z[row] = A.dinv[row] * A.r[row];
-    else A.z[row] = row_dot(A.rowptr, A.colind, A.pr, A.r, row);
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (int j = threadIdx.x; j < A.nb; j += SBLK) {
+        x += A.part[(size_t)s0 * A.nb + j];
+        if (two) y += A.part[(size_t)(s0 + 1) * A.nb + j];
+    }
+    block_sum3(x, y, z);
+    a = x;
+    b = y;
 }
 
-__global__ void __launch_bounds__(SBLK) k_cg(CGArgs A)
+__device__ __forceinline__ double apply_m(const CGArgs &A, int row)
 {
-    cg::grid_group g = cg::this_grid();
-    const int gsize = gridDim.x * SBLK;
-    const int gtid = blockIdx.x * SBLK + threadIdx.x;
-    const int n = A.n;
-    const bool sparse_m = A.variant != 2;
-    double a0 = 0.0, dum;
-    for (int row = gtid; row < n; row += gsize) a0 += A.rhs[row] * A.rhs[row];
-    double s_old;
-    grid_sum2(a0, 0.0, A.part, 0, g, s_old, dum);
-    const double rtol = A.soltol * sqrt(s_old / (double)n);
-    for (int row = gtid; row < n; row += gsize)
-        A.r[row] = A.rhs[row] - row_dot(A.rowptr, A.colind, A.val, A.x, row);
-    if (sparse_m) { __threadfence(); g.sync(); }
-    a0 = 0.0;
-    for (int row = gtid; row < n; row += gsize) {
-        apply_prec(A, row);
-        A.p[row] = A.z[row];
-        a0 += A.r[row] * A.z[row];
-    }
-    grid_sum2(a0, 0.0, A.part, 2, g, s_old, dum);
-    int iter, converged = 0, nbreak = 0, nneg = 0;
-    double rr = 0.0;
-    for (iter = 1; iter <= A.maxiter; ++iter) {
-        __threadfence();
-        g.sync();   // p complete before the matrix-vector product reads it
-        a0 = 0.0;
-        for (int row = gtid; row < n; row += gsize) {
-            const double qq = row_dot(A.rowptr, A.colind, A.val, A.p, row);
-            A.q[row] = qq;
-            a0 += A.p[row] * qq;
+    if (A.variant == 2) return A.dinv[row] * A.r[row];
+    return row_dot(A.rowptr, A.colind, A.pr, A.r, row);
+}
+
+// r = b - A x; (Jacobi) z = D^-1 r; partials b.b and r.z; state reset
+__global__ void __launch_bounds__(SBLK) k_cg_init(CGArgs A)
+{
+    double bb = 0.0, rz = 0.0, dum = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        const double rr = A.rhs[row] - row_dot(A.rowptr, A.colind, A.val, A.x, row);
+        A.r[row] = rr;
+        bb += A.rhs[row] * A.rhs[row];
+        if (A.variant == 2) {
+            const double zz = A.dinv[row] * rr;
+            A.z[row] = zz;
+            rz += rr * zz;
         }
-        double s_aux;
-        grid_sum2(a0, 0.0, A.part, 2, g, s_aux, dum);
-        if (s_aux <= 0.0 || s_aux != s_aux) { nbreak++; break; }
-        const double al = s_old / s_aux;
-        for (int row = gtid; row < n; row += gsize) {
-            A.x[row] = A.x[row] + al * A.p[row];
-            A.r[row] = A.r[row] - al * A.q[row];
-        }
-        if (sparse_m) { __threadfence(); g.sync(); }
-        double b0 = 0.0;
-        a0 = 0.0;
-        for (int row = gtid; row < n; row += gsize) {
-            apply_prec(A, row);
-            a0 += A.r[row] * A.z[row];
-            b0 += A.r[row] * A.r[row];
-        }
-        double rz;
-        grid_sum2(a0, b0, A.part, 4, g, rz, rr);
-        if (sqrt(rr / (double)n) < rtol) { converged = 1; break; }
-        if (rz < 0.0) nneg++;
-        if (s_old == 0.0) { nbreak++; break; }
-        const double be = rz / s_old;
-        s_old = rz;
-        for (int row = gtid; row < n; row += gsize) A.p[row] = A.z[row] + be * A.p[row];
     }
-    if (gtid == 0) {
-        A.out[0] = (double)(iter < A.maxiter ? iter : A.maxiter);
-        A.out[1] = converged;
-        A.out[2] = nbreak;
-        A.out[3] = nneg;
-        A.out[4] = sqrt(rr / (double)n);
-        A.out[5] = rtol;
+    block_sum3(bb, rz, dum);
+    if (threadIdx.x == 0) {
+        A.part[blockIdx.x] = bb;
+        A.part[(size_t)A.nb + blockIdx.x] = rz;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CGState s{};
+        *A.st = s;
+    }
+}
+
+// sparse M^-1 only: z = M^-1 r (needs r of neighbour rows), partial r.z
+__global__ void __launch_bounds__(SBLK) k_cg_init_z(CGArgs A)
+{
+    double rz = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        const double zz = apply_m(A, row);
+        A.z[row] = zz;
+        rz += A.r[row] * zz;
+    }
+    block_sum3(rz, d1, d2);
+    if (threadIdx.x == 0) A.part[(size_t)A.nb + blockIdx.x] = rz;
+}
+
+__global__ void __launch_bounds__(SBLK) k_cg_a(CGArgs A)
+{
+    CGState *st = A.st;
+    if (st->done) return;
+    const int it = st->iter;          // completed iterations
+    const int par = (it + 1) & 1;     // parity of iteration it+1
+    double be = 0.0, s_new;
+    const bool first = (it == 0);
+    if (first) {
+        double bb, rz;
+        sum_partials(A, 0, true, bb, rz);
+        const double rtol = A.soltol * sqrt(bb / (double)A.n);
+        s_new = rz;
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->rtol = rtol; st->rr_last = 0.0; }
+    } else {
+        double rz, rr;
+        sum_partials(A, 1, true, rz, rr);
+        const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+        if (lead) st->rr_last = rr;
+        if (sqrt(rr / (double)A.n) < st->rtol) {
+            if (lead) st->done = 1;
+            return;
+        }
+        if (rz < 0.0 && lead) st->nneg += 1;
+        const double s_prev = st->s_old[par ^ 1];
+        if (s_prev == 0.0) {
+            if (lead) { st->nbreak += 1; st->done = 2; }
+            return;
+        }
+        be = rz / s_prev;
+        s_new = rz;
+    }
+    if (it >= A.maxiter) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st->s_old[par] = s_new; st->par = par; }
+    const double *pold = par ? A.p0 : A.p1;
+    double *pnew = par ? A.p1 : A.p0;
+    double pq = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        double s = 0.0;
+        for (int p = A.rowptr[row]; p < A.rowptr[row + 1]; ++p) {
+            const int j = A.colind[p];
+            const double pj = first ? A.z[j] : A.z[j] + be * pold[j];
+            s = s + A.val[p] * pj;
+        }
+        const double pr_ = first ? A.z[row] : A.z[row] + be * pold[row];
+        pnew[row] = pr_;
+        A.q[row] = s;
+        pq += pr_ * s;
+    }
+    block_sum3(pq, d1, d2);
+    if (threadIdx.x == 0) A.part[blockIdx.x] = pq;
+}
+
+__global__ void __launch_bounds__(SBLK) k_cg_b(CGArgs A, int fuse_z)
+{
+    CGState *st = A.st;
+    if (st->done) return;
+    const int par = st->par;
+    double s_aux, dum;
+    sum_partials(A, 0, false, s_aux, dum);
+    if (s_aux <= 0.0 || s_aux != s_aux) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->nbreak += 1; st->done = 2; st->iter += 1; }
+        return;
+    }
+    const double al = st->s_old[par] / s_aux;
+    const double *p = par ? A.p1 : A.p0;
+    double rz = 0.0, rr = 0.0, d2 = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        A.x[row] = A.x[row] + al * p[row];
+        const double r_ = A.r[row] - al * A.q[row];
+        A.r[row] = r_;
+        if (fuse_z) {
+            const double zz = A.dinv[row] * r_;
+            A.z[row] = zz;
+            rz += r_ * zz;
+            rr += r_ * r_;
+        }
+    }
+    if (fuse_z) {
+        block_sum3(rz, rr, d2);
+        if (threadIdx.x == 0) {
+            A.part[(size_t)A.nb + blockIdx.x] = rz;
+            A.part[(size_t)2 * A.nb + blockIdx.x] = rr;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->iter += 1;
+    }
+}
+
+// sparse M^-1: z = M^-1 r after the r update, partials r.z and r.r
+__global__ void __launch_bounds__(SBLK) k_cg_c(CGArgs A)
+{
+    CGState *st = A.st;
+    if (st->done) return;
+    double rz = 0.0, rr = 0.0, d2 = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        const double zz = apply_m(A, row);
+        A.z[row] = zz;
+        rz += A.r[row] * zz;
+        rr += A.r[row] * A.r[row];
+    }
+    block_sum3(rz, rr, d2);
+    if (threadIdx.x == 0) {
+        A.part[(size_t)A.nb + blockIdx.x] = rz;
+        A.part[(size_t)2 * A.nb + blockIdx.x] = rr;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->iter += 1;
 }
 
 __global__ void k_copy(double *dst, const double *src, int n)
@@ -230,20 +308,15 @@ static int ssh_alloc_vectors(fgpu_context *c)
 {
     const size_t n = c->ssh_n;
     int rc;
-    double **vs[] = {&c->ssh_x, &c->ssh_x0, &c->ssh_rhs, &c->ssh_r, &c->ssh_z, &c->ssh_p, &c->ssh_q, &c->ssh_diag_inv};
+    double **vs[] = {&c->ssh_x, &c->ssh_x0, &c->ssh_rhs, &c->ssh_r, &c->ssh_z, &c->ssh_p, &c->ssh_p1, &c->ssh_q,
+                     &c->ssh_diag_inv};
     for (double **v : vs)
         if (!*v && (rc = fgpu_alloc(c, (void **)v, n * sizeof(double)))) return rc;
-    int nb = 0, dev = c->device, ncu = 0;
-    FGPU_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    FGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cg, SBLK, 0));
-    if (nb < 1) return fgpu_fail(FGPU_ERR_HIP, "k_cg cannot be resident");
-    int blocks = ncu * std::min(nb, 2);
-    int need = (int)((n + SBLK - 1) / SBLK);
-    if (blocks > need) blocks = need;
-    if (blocks < 1) blocks = 1;
+    int blocks = (int)((n + SBLK - 1) / SBLK);
+    if (blocks > 512) blocks = 512;
     c->ssh_blocks = blocks;
-    if (!c->ssh_part && (rc = fgpu_alloc(c, (void **)&c->ssh_part, sizeof(double) * 6 * (size_t)blocks))) return rc;
-    if (!c->ssh_out && (rc = fgpu_alloc(c, (void **)&c->ssh_out, sizeof(double) * 8))) return rc;
+    if (!c->ssh_part && (rc = fgpu_alloc(c, (void **)&c->ssh_part, sizeof(double) * 4 * (size_t)blocks))) return rc;
+    if (!c->ssh_out && (rc = fgpu_alloc(c, (void **)&c->ssh_out, sizeof(CGState)))) return rc;
     return FGPU_OK;
 }
 
@@ -366,12 +439,13 @@ extern "C" int fgpu_ssh_solve_preconditioner(fgpu_context *c, int variant)
     return FGPU_OK;
 }
 
-static int launch_cg(fgpu_context *c, double soltol, int maxiter)
+static CGArgs cg_args(fgpu_context *c, double soltol, int maxiter)
 {
     CGArgs A;
     A.n = c->ssh_n;
     A.variant = c->ssh_variant;
     A.maxiter = maxiter;
+    A.nb = c->ssh_blocks;
     A.rowptr = c->ssh_rowptr;
     A.colind = c->ssh_colind;
     A.val = c->ssh_val;
@@ -381,27 +455,96 @@ static int launch_cg(fgpu_context *c, double soltol, int maxiter)
     A.rhs = c->ssh_rhs;
     A.r = c->ssh_r;
     A.z = c->ssh_z;
-    A.p = c->ssh_p;
+    A.p0 = c->ssh_p;
+    A.p1 = c->ssh_p1;
     A.q = c->ssh_q;
     A.part = c->ssh_part;
-    A.out = c->ssh_out;
+    A.st = (CGState *)c->ssh_out;
     A.soltol = soltol;
-    void *args[] = {&A};
-    FGPU_HIP(hipLaunchCooperativeKernel((const void *)k_cg, dim3(c->ssh_blocks), dim3(SBLK), args, 0, c->stream));
+    return A;
+}
+
+#define CG_GRAPH_ITERS 8
+
+static int enqueue_iters(fgpu_context *c, const CGArgs &A, int iters)
+{
+    const dim3 g(A.nb), b(SBLK);
+    for (int i = 0; i < iters; ++i) {
+        hipLaunchKernelGGL(k_cg_a, g, b, 0, c->stream, A);
+        if (A.variant == 2) {
+            hipLaunchKernelGGL(k_cg_b, g, b, 0, c->stream, A, 1);
+        } else {
+            hipLaunchKernelGGL(k_cg_b, g, b, 0, c->stream, A, 0);
+            hipLaunchKernelGGL(k_cg_c, g, b, 0, c->stream, A);
+        }
+    }
+    FGPU_HIP(hipGetLastError());
+    return FGPU_OK;
+}
+
+// graph of CG_GRAPH_ITERS iterations for the current (soltol, maxiter, variant)
+static int cg_graph(fgpu_context *c, const CGArgs &A)
+{
+    if (c->ssh_graph && c->ssh_graph_tol == A.soltol && c->ssh_graph_maxiter == A.maxiter &&
+        c->ssh_graph_variant == A.variant)
+        return FGPU_OK;
+    if (c->ssh_graph) { (void)hipGraphExecDestroy(c->ssh_graph); c->ssh_graph = nullptr; }
+    hipGraph_t g;
+    FGPU_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_iters(c, A, CG_GRAPH_ITERS);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->ssh_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    c->ssh_graph_tol = A.soltol;
+    c->ssh_graph_maxiter = A.maxiter;
+    c->ssh_graph_variant = A.variant;
+    return FGPU_OK;
+}
+
+// One solve from the x currently in ssh_x. Replays the iteration graph in
+// batches sized by the previous solve's iteration count and polls the device
+// flag between batches; kernels after convergence return immediately.
+static int launch_cg(fgpu_context *c, double soltol, int maxiter)
+{
+    CGArgs A = cg_args(c, soltol, maxiter);
+    int rc = cg_graph(c, A);
+    if (rc) return rc;
+    const dim3 g(A.nb), b(SBLK);
+    hipLaunchKernelGGL(k_cg_init, g, b, 0, c->stream, A);
+    if (A.variant != 2) hipLaunchKernelGGL(k_cg_init_z, g, b, 0, c->stream, A);
+    FGPU_HIP(hipGetLastError());
+    const int max_graphs = maxiter / CG_GRAPH_ITERS + 2;
+    int expect = c->ssh_last_iters > 0 ? c->ssh_last_iters : 4 * CG_GRAPH_ITERS;
+    int launched = 0;
+    int batch = (expect + CG_GRAPH_ITERS - 1) / CG_GRAPH_ITERS;
+    CGState hs;
+    while (true) {
+        for (int i = 0; i < batch && launched < max_graphs; ++i, ++launched)
+            FGPU_HIP(hipGraphLaunch(c->ssh_graph, c->stream));
+        FGPU_HIP(hipMemcpyAsync(&hs, c->ssh_out, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
+        FGPU_HIP(hipStreamSynchronize(c->stream));
+        if (hs.done || launched >= max_graphs) break;
+        batch = 1;
+    }
+    if (!hs.done) return fgpu_fail(FGPU_ERR_STATE, "CG iteration graph did not terminate");
+    c->ssh_last_iters = hs.iter;
     return FGPU_OK;
 }
 
 static int read_stats(fgpu_context *c, fgpu_cg_stats *st, double ms)
 {
-    double o[6];
-    FGPU_HIP(hipMemcpy(o, c->ssh_out, sizeof(o), hipMemcpyDeviceToHost));
+    CGState hs;
+    FGPU_HIP(hipMemcpy(&hs, c->ssh_out, sizeof(CGState), hipMemcpyDeviceToHost));
     if (st) {
-        st->iters_last = (int)o[0];
-        st->converged = (int)o[1];
-        st->nbreakdown = (int)o[2];
-        st->nnegrz = (int)o[3];
-        st->resid_last = o[4];
-        st->rtol_last = o[5];
+        st->iters_last = hs.iter < 1 ? 1 : hs.iter;
+        st->converged = hs.done == 1;
+        st->nbreakdown = hs.nbreak;
+        st->nnegrz = hs.nneg;
+        st->resid_last = std::sqrt(hs.rr_last / (double)c->ssh_n);
+        st->rtol_last = hs.rtol;
         st->ms = ms;
     }
     return FGPU_OK;

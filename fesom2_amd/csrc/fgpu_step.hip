@@ -1,0 +1,729 @@
+// Fused, batched tracer step for gfx950: five launches per step for all tracers.
+//
+//   K_G  node pass   area-weighted mean element gradient of T^n at the levels
+//                    where an edge needs it (fill_up_dn_grad's non-shared levels)
+//   K_A  edge pass   LO upwind flux (T^n) and antidiffusive flux HO(T^AB) - LO
+//                    (MFCT/UPW1); the up/downwind element gradients and T^AB are
+//                    evaluated on the fly
+//   K_B  node pass   LO solution and antidiffusive vertical flux (QR4C/UPW1)
+//   K_C  node pass   FCT bounds over the 1-ring (edge gather), P+/P-, R+/R-
+//   K_D  node pass   limited vertical and horizontal fluxes, flux2dtracer, T*,
+//                    implicit vertical diffusion (Thomas), salinity clamp,
+//                    valuesold rotation
+//
+// K_B..K_D are separated by the three global dependencies of FCT (LO of the
+// neighbours, R+/R- of the neighbours, limited edge fluxes). One block owns one
+// column (an edge or a node) and every tracer; lanes run over levels, so a
+// column's topology and geometry are wave-uniform and its field accesses are
+// contiguous column segments. The node kernels load the metadata of a node's
+// edges (or elements) once, one edge per lane, and hand it to the gather loops
+// with v_readlane, so the per-tracer loops carry no dependent scalar loads.
+// Blocks are mapped to columns XCD by XCD (workgroups are dispatched to the
+// XCDs round-robin), so neighbouring columns share an L2.
+//
+// Arithmetic is the same sequence of operations as the per-function kernels
+// and the reference (no FMA contraction); edge->node sums run over the node's
+// edges in ascending edge order. Results are bit-identical to a serial run of
+// the reference's loops.
+#include <algorithm>
+
+#include "fgpu_internal.h"
+
+#define NXCD 8
+#define TCH 4   // tracers per Thomas chunk in K_D (one lane per tracer)
+
+__device__ __forceinline__ double dmx(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmn(double a, double b) { return a < b ? a : b; }
+
+__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ double rld(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// column of this block, XCD-contiguous: blocks b, b+8, ... run on one XCD
+__device__ __forceinline__ int xcd_column(int ncol)
+{
+    const int per = (int)(gridDim.x / NXCD);
+    return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+}
+
+__device__ __forceinline__ double upw1_h(double t1, double t2, double vflux, double fin)
+{
+    return -0.5 * (t1 * (vflux + fabs(vflux)) + t2 * (vflux - fabs(vflux))) - fin;
+}
+
+// T^AB from values and valuesold (oce_tracer_mod.F90:44-59)
+__device__ __forceinline__ double t_ab(int abo, const double *T, const double *Told, size_t i)
+{
+    if (abo == 2) return -(0.5 + FGPU_EPS_AB) * Told[i] + (1.5 + FGPU_EPS_AB) * T[i];
+    const double a = 5.0 * Told[2 * i + 1] - 16.0 * Told[2 * i] + 23.0 * T[i];
+    return a / 12.0;
+}
+
+__device__ __forceinline__ double seg_vflux(int seg, double u1, double v1, double u2, double v2, double h1,
+                                            double h2, double dX1, double dY1, double dX2, double dY2)
+{
+    if (seg == 0 || seg == 3) return (-v1 * dX1 + u1 * dY1) * h1;
+    if (seg == 1 || seg == 4) return (v2 * dX2 - u2 * dY2) * h2;
+    return (-v1 * dX1 + u1 * dY1) * h1 + (v2 * dX2 - u2 * dY2) * h2;
+}
+
+// metadata of a node's edges, one edge per lane (lane < deg)
+struct NodeEdges {
+    int deg;
+    int q;          // 2*e + side
+    int other;      // the edge's other node
+    int nu12, nl12; // edge scatter range (max/min rule)
+    int nu1, nl1, nu2, nl2;
+};
+
+__device__ __forceinline__ NodeEdges load_node_edges(const DevMesh &m, int n)
+{
+    NodeEdges E;
+    const int p0 = m.ne_ptr[n];
+    E.deg = m.ne_ptr[n + 1] - p0;
+    const int j = threadIdx.x & 63;
+    E.q = 0; E.other = 0; E.nu12 = 1; E.nl12 = 0; E.nu1 = 1; E.nl1 = 0; E.nu2 = 0; E.nl2 = 0;
+    if (j < E.deg) {
+        const int q = m.ne_idx[p0 + j];
+        const int e = q >> 1;
+        E.q = q;
+        E.other = m.edges[2 * e + (1 - (q & 1))];
+        E.nu12 = m.erange[2 * e];
+        E.nl12 = m.erange[2 * e + 1];
+        E.nl1 = m.einfo[4 * e];
+        E.nu1 = m.einfo[4 * e + 1];
+        E.nl2 = m.einfo[4 * e + 2];
+        E.nu2 = m.einfo[4 * e + 3];
+    }
+    return E;
+}
+
+// ------------------------------------------------------------------------------
+// K_G: mean gradient around a node (oce_muscl_adv.F90:388-401, 443-520) for
+// levels kneed(n)..nlevels_nod2D-1
+// ------------------------------------------------------------------------------
+template <int LB>
+__global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int *kneed)
+{
+    const int n = xcd_column(m.N2);
+    if (n >= m.N2) return;
+    const int k0 = kneed[n], nzmax = m.nlevn[n];
+    if (k0 > nzmax - 1) return;
+    const int L = m.L;
+    const int k = threadIdx.x + 1;
+    const int num = m.nie_num[n];
+    const int j = threadIdx.x & 63;
+    int lo = 1, hi = 0, a0 = 0, a1 = 0, a2 = 0;
+    double ar = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0, g4 = 0.0, g5 = 0.0;
+    if (j < num) {
+        const int el = m.nie[(size_t)n * m.kmax + j];
+        lo = m.ulev[el];
+        hi = m.nlev[el] - 1;
+        ar = m.elem_area[el];
+        a0 = m.elem_nodes[3 * el]; a1 = m.elem_nodes[3 * el + 1]; a2 = m.elem_nodes[3 * el + 2];
+        const double *gs = m.gsca + 6 * (size_t)el;
+        g0 = gs[0]; g1 = gs[1]; g2 = gs[2]; g3 = gs[3]; g4 = gs[4]; g5 = gs[5];
+    }
+    const bool act = (k >= k0 && k <= nzmax - 1);
+    for (int tr = 0; tr < t.ntr; ++tr) {
+        if (t.hor[tr] != HOR_MFCT) continue;
+        const double *T = t.T + t.sN * tr;
+        double tvol = 0.0, tx = 0.0, ty = 0.0;
+        for (int i = 0; i < num; ++i) {
+            const int elo = rl(lo, i), ehi = rl(hi, i);
+            const int b0 = rl(a0, i), b1 = rl(a1, i), b2 = rl(a2, i);
+            const double a = rld(ar, i);
+            const double s0 = rld(g0, i), s1 = rld(g1, i), s2 = rld(g2, i);
+            const double s3 = rld(g3, i), s4 = rld(g4, i), s5 = rld(g5, i);
+            if (act && k >= elo && k <= ehi) {
+                const double t0 = T[(size_t)b0 * L + k - 1], t1 = T[(size_t)b1 * L + k - 1],
+                             t2 = T[(size_t)b2 * L + k - 1];
+                const double ex = s0 * t0 + s1 * t1 + s2 * t2;
+                const double ey = s3 * t0 + s4 * t1 + s5 * t2;
+                tvol = tvol + a;
+                tx = tx + ex * a;
+                ty = ty + ey * a;
+            }
+        }
+        if (act) {
+            double *o = t.gmean + t.sG * tr + ((size_t)n * L + k - 1) * 2;
+            o[0] = tx / tvol;
+            o[1] = ty / tvol;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
+// K_A: edge pass
+// ------------------------------------------------------------------------------
+template <int LB>
+__global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t)
+{
+    const int e = xcd_column(m.E2);
+    if (e >= m.E2) return;
+    const int k = threadIdx.x + 1;
+    const int L = m.L;
+    const int *ri = m.erec_i + (size_t)e * ERI;
+    const double *rd = m.erec_d + (size_t)e * ERD;
+    const int nl1 = ri[ER_NL1], nu1 = ri[ER_NU1], nl2 = ri[ER_NL2], nu2 = ri[ER_NU2];
+    const int nl12 = min(nl1, nl2), nu12 = max(nu1, nu2);
+    bool in[5];
+    in[0] = (k >= nu1 && k <= nu12 - 1);
+    in[1] = (nu2 > 0 && k >= nu2 && k <= nu12 - 1);
+    in[2] = (k >= nu12 && k <= nl12);
+    in[3] = (k >= nl12 + 1 && k <= nl1);
+    in[4] = (k >= nl12 + 1 && k <= nl2);
+    if (!(in[0] || in[1] || in[2] || in[3] || in[4])) return;
+    const int n1 = ri[ER_N1], n2 = ri[ER_N2], el1 = ri[ER_EL1], el2 = ri[ER_EL2];
+    const double dX1 = rd[ER_DX1], dY1 = rd[ER_DY1], dX2 = rd[ER_DX2], dY2 = rd[ER_DY2];
+    const double a = rd[ER_A], edx = rd[ER_EDX], edy = rd[ER_EDY];
+    const size_t q1 = (size_t)el1 * L + k - 1;
+    const double u1 = d.uv[2 * q1], v1 = d.uv[2 * q1 + 1], h1 = m.helem[q1];
+    double u2 = 0.0, v2 = 0.0, h2 = 0.0;
+    if (el2 >= 0 && (in[1] || in[2] || in[4])) {
+        const size_t q2 = (size_t)el2 * L + k - 1;
+        u2 = d.uv[2 * q2];
+        v2 = d.uv[2 * q2 + 1];
+        h2 = m.helem[q2];
+    }
+    double vf[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) vf[s] = in[s] ? seg_vflux(s, u1, v1, u2, v2, h1, h2, dX1, dY1, dX2, dY2) : 0.0;
+    // gradient rule of fill_up_dn_grad at this level (oce_muscl_adv.F90:375-520);
+    // writes in the reference's order, a later write wins
+    bool shared = false, mean1 = false, mean2 = false;
+    const int u1l = ri[ER_U1], u2l = ri[ER_U2], nl1n = ri[ER_NL1N], nl2n = ri[ER_NL2N];
+    if (ri[ER_RULE]) {
+        const int nzmin = ri[ER_NZMIN], nzmax = ri[ER_NZMAX];
+        mean1 = (k >= u1l && k <= nzmin - 1);
+        mean2 = (k >= u2l && k <= nzmin - 1);
+        if (k >= nzmin && k <= nzmax - 1) { shared = true; mean1 = false; mean2 = false; }
+        if (k >= nzmax && k <= nl1n - 1) { mean1 = true; shared = false; }
+        if (k >= nzmax && k <= nl2n - 1) { mean2 = true; shared = false; }
+    } else {
+        mean1 = (k >= u1l && k <= nl1n - 1);
+        mean2 = (k >= u2l && k <= nl2n - 1);
+    }
+    const int ua0 = ri[ER_UA0], ua1 = ri[ER_UA1], ua2 = ri[ER_UA2];
+    const int da0 = ri[ER_DA0], da1 = ri[ER_DA1], da2 = ri[ER_DA2];
+    const size_t o1 = (size_t)n1 * L + k - 1, o2 = (size_t)n2 * L + k - 1;
+    const size_t ei = (size_t)e * L + k - 1;
+    const size_t kk = (size_t)k - 1;
+    for (int tr = 0; tr < t.ntr; ++tr) {
+        const double *T = t.T + t.sN * tr;
+        const double *Told = t.Told + t.sOld * tr;
+        const int abo = t.abo[tr], lim = t.lim[tr], hor = t.hor[tr];
+        double flo = 0.0;
+        const double T1 = T[o1], T2 = T[o2];
+        if (lim == LIM_FCT) {
+#pragma unroll
+            for (int s = 0; s < 5; ++s)
+                if (in[s]) flo = upw1_h(T1, T2, vf[s], flo);
+            t.lo_h[t.sE * tr + ei] = flo;
+        }
+        double A1, A2;
+        if (abo == 2) {
+            A1 = -(0.5 + FGPU_EPS_AB) * Told[o1] + (1.5 + FGPU_EPS_AB) * T1;
+            A2 = -(0.5 + FGPU_EPS_AB) * Told[o2] + (1.5 + FGPU_EPS_AB) * T2;
+        } else {
+            A1 = t_ab(abo, T, Told, o1);
+            A2 = t_ab(abo, T, Told, o2);
+        }
+        double f = flo;
+        if (hor == HOR_MFCT) {
+            const double no = t.ph[tr];
+            double g[4] = {0.0, 0.0, 0.0, 0.0};
+            if (shared) {
+                const double u0 = T[(size_t)ua0 * L + kk], u1_ = T[(size_t)ua1 * L + kk], u2_ = T[(size_t)ua2 * L + kk];
+                const double w0 = T[(size_t)da0 * L + kk], w1 = T[(size_t)da1 * L + kk], w2 = T[(size_t)da2 * L + kk];
+                g[0] = rd[ER_GU0] * u0 + rd[ER_GU1] * u1_ + rd[ER_GU2] * u2_;
+                g[2] = rd[ER_GU3] * u0 + rd[ER_GU4] * u1_ + rd[ER_GU5] * u2_;
+                g[1] = rd[ER_GD0] * w0 + rd[ER_GD1] * w1 + rd[ER_GD2] * w2;
+                g[3] = rd[ER_GD3] * w0 + rd[ER_GD4] * w1 + rd[ER_GD5] * w2;
+            } else {
+                const double *gm = t.gmean + t.sG * tr;
+                if (mean1) { g[0] = gm[2 * o1]; g[2] = gm[2 * o1 + 1]; }
+                if (mean2) { g[1] = gm[2 * o2]; g[3] = gm[2 * o2 + 1]; }
+            }
+            const double Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
+            const double Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
+#pragma unroll
+            for (int s = 0; s < 5; ++s)
+                if (in[s]) {
+                    const double vflux = vf[s];
+                    const double cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
+                    f = -0.5 * (1.0 - no) * cHO - vflux * no * 0.5 * (Tmean1 + Tmean2) - f;
+                }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 5; ++s)
+                if (in[s]) f = upw1_h(A1, A2, vf[s], f);
+        }
+        t.a_h[t.sE * tr + ei] = f;
+    }
+}
+
+// ------------------------------------------------------------------------------
+// column helpers on LDS copies (index = level - 1)
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ double col_upw1(int j, int nzmin, int nzmax, const double *T, const double *W,
+                                           const double *ar, double f)
+{
+    if (j == nzmin) f = -W[j - 1] * T[j - 1] * ar[j - 1] - f;
+    if (j == nzmax) f = 0.0 - f;
+    if (j >= nzmin + 1 && j <= nzmax - 1) {
+        const double w = W[j - 1];
+        f = -0.5 * (T[j - 1] * (w + fabs(w)) + T[j - 2] * (w - fabs(w))) * ar[j - 1] - f;
+    }
+    return f;
+}
+
+__device__ __forceinline__ double col_qr4c(int j, int nzmin, int nzmax, const double *T, const double *W,
+                                           const double *ar, const double *Zn, const double *zb, double num_ord,
+                                           double f)
+{
+    if (j == nzmin) f = -T[j - 1] * W[j - 1] * ar[j - 1] - f;
+    if (j == nzmin + 1) f = -0.5 * (T[j - 2] + T[j - 1]) * W[j - 1] * ar[j - 1] - f;
+    if (j == nzmax - 1) f = -0.5 * (T[j - 2] + T[j - 1]) * W[j - 1] * ar[j - 1] - f;
+    if (j == nzmax) f = 0.0 - f;
+    if (j >= nzmin + 2 && j <= nzmax - 2) {
+        const int nz = j;
+        const double qc = (T[nz - 2] - T[nz - 1]) / (Zn[nz - 2] - Zn[nz - 1]);
+        const double qu = (T[nz - 1] - T[nz]) / (Zn[nz - 1] - Zn[nz]);
+        const double qd = (T[nz - 3] - T[nz - 2]) / (Zn[nz - 3] - Zn[nz - 2]);
+        const double Tmean1 = T[nz - 1] + (2 * qc + qu) * (zb[nz - 1] - Zn[nz - 1]) / 3.0;
+        const double Tmean2 = T[nz - 2] + (2 * qc + qd) * (zb[nz - 1] - Zn[nz - 2]) / 3.0;
+        const double ww = W[nz - 1];
+        const double Tmean = (ww + fabs(ww)) * Tmean1 + (ww - fabs(ww)) * Tmean2;
+        f = (-0.5 * (1.0 - num_ord) * Tmean - num_ord * (0.5 * (Tmean1 + Tmean2)) * ww) * ar[nz - 1] - f;
+    }
+    return f;
+}
+
+// ------------------------------------------------------------------------------
+// K_B: LO solution and antidiffusive vertical flux per node column
+// ------------------------------------------------------------------------------
+template <int LB>
+__global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t, double dt)
+{
+    const int n = xcd_column(m.N2);
+    if (n >= m.N2) return;
+    __shared__ double sT[LB + 2], sA[LB + 2], sW[LB + 2], sWe[LB + 2], sAr[LB + 2], sZ[LB + 2], sZb[LB + 2];
+    __shared__ double sLo[LB + 2];
+    const int tid = threadIdx.x, k = tid + 1;
+    const int L = m.L, NL = m.NL;
+    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    const bool act = (k >= nzmin && k <= nzmax - 1);
+    const size_t ci = (size_t)n * L + k - 1;
+    const NodeEdges E = load_node_edges(m, n);
+    if (k <= NL) {
+        sW[tid] = d.w[(size_t)n * NL + tid];
+        sWe[tid] = d.w_e[(size_t)n * NL + tid];
+        sAr[tid] = m.area[(size_t)n * NL + tid];
+        sZb[tid] = m.zbar3d[(size_t)n * NL + tid];
+    }
+    if (k <= L) sZ[tid] = m.Z3d[ci];
+    double hn = 1.0, hh = 0.0, av_ = 1.0;
+    if (act) {
+        hn = m.hnode_new[ci];
+        hh = m.hnode[ci];
+        av_ = m.areasvol[(size_t)n * NL + k - 1];
+    }
+    for (int tr = 0; tr < t.ntr; ++tr) {
+        const double *T = t.T + t.sN * tr;
+        const double *Told = t.Told + t.sOld * tr;
+        const int abo = t.abo[tr], lim = t.lim[tr], ver = t.ver[tr];
+        const double pv = t.pv[tr];
+        __syncthreads();
+        if (k <= L) {
+            sT[tid] = T[ci];
+            sA[tid] = t_ab(abo, T, Told, ci);
+        }
+        __syncthreads();
+        double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
+        const double *W = (lim == LIM_FCT) ? sW : sWe;
+        double lk = 0.0;
+        if (lim == LIM_FCT) {
+            if (k >= nzmin && k <= nzmax) lk = col_upw1(k, nzmin, nzmax, sT, sWe, sAr, 0.0);
+            sLo[tid] = lk;
+        }
+        __syncthreads();
+        if (k >= nzmin && k <= nzmax) {
+            const double ho = (ver == VER_QR4C) ? col_qr4c(k, nzmin, nzmax, sA, W, sAr, sZ, sZb, pv, lk)
+                                                : col_upw1(k, nzmin, nzmax, sA, W, sAr, lk);
+            av[k - 1] = ho;
+        }
+        if (lim == LIM_FCT) {
+            const double *lo_h = t.lo_h + t.sE * tr;
+            double acc = 0.0;
+            for (int j = 0; j < E.deg; ++j) {
+                const int q = rl(E.q, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                if (act && k >= a0 && k <= a1) {
+                    const double fl = lo_h[(size_t)(q >> 1) * L + k - 1];
+                    acc = (q & 1) ? acc - fl : acc + fl;
+                }
+            }
+            if (act) {
+                const double lk1 = sLo[tid + 1];
+                t.fct_lo[t.sN * tr + ci] = (sT[tid] * hh + (acc + (lk - lk1)) * dt / av_) / hn;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
+// K_C: FCT bounds, P+/P-, R+/R- per node column (oce_adv_tra_fct.F90:119-405)
+// The element bound (max over the 3 nodes of max(LO, T^n)), maximised over the
+// elements around the node, equals the maximum over the node and those edge
+// neighbours m for which an element adjacent to edge (n, m) has the level.
+// ------------------------------------------------------------------------------
+template <int LB>
+__global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt)
+{
+    const int n = xcd_column(m.N2);
+    if (n >= m.N2) return;
+    __shared__ double sMx[LB + 2], sMn[LB + 2];
+    const int tid = threadIdx.x, k = tid + 1;
+    const int L = m.L, NL = m.NL;
+    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    const bool act = (k >= nzmin && k <= nzmax - 1);
+    const size_t ci = (size_t)n * L + k - 1;
+    const NodeEdges E = load_node_edges(m, n);
+    double hn = 1.0, av_ = 1.0;
+    if (act) {
+        hn = m.hnode_new[ci];
+        av_ = m.areasvol[(size_t)n * NL + k - 1];
+    }
+    for (int tr = 0; tr < t.ntr; ++tr) {
+        if (t.lim[tr] != LIM_FCT) continue;   // uniform over the block
+        const double *T = t.T + t.sN * tr;
+        const double *LO = t.fct_lo + t.sN * tr;
+        const double *ah = t.a_h + t.sE * tr;
+        double vmax = -1e3, vmin = 1e3, lo = 0.0, p = 0.0, q = 0.0;
+        if (act) {
+            lo = LO[ci];
+            const double tk = T[ci];
+            vmax = dmx(lo, tk);
+            vmin = dmn(lo, tk);
+            const double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
+            p = p + (dmx(0.0, av[k - 1]) + dmx(0.0, -av[k]));
+            q = q + (dmn(0.0, av[k - 1]) + dmn(0.0, -av[k]));
+        }
+        for (int j = 0; j < E.deg; ++j) {
+            const int qq = rl(E.q, j), mo = rl(E.other, j);
+            const int a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+            const int b_nu1 = rl(E.nu1, j), b_nl1 = rl(E.nl1, j), b_nu2 = rl(E.nu2, j), b_nl2 = rl(E.nl2, j);
+            if (!act) continue;
+            const bool has = (k >= b_nu1 && k <= b_nl1) || (b_nu2 > 0 && k >= b_nu2 && k <= b_nl2);
+            if (has) {
+                const size_t mi = (size_t)mo * L + k - 1;
+                const double lm = LO[mi], tm = T[mi];
+                vmax = dmx(vmax, dmx(lm, tm));
+                vmin = dmn(vmin, dmn(lm, tm));
+            }
+            if (k >= a0 && k <= a1) {
+                const double fl = ah[(size_t)(qq >> 1) * L + k - 1];
+                if (qq & 1) {
+                    p = p + dmx(0.0, -fl);
+                    q = q + dmn(0.0, -fl);
+                } else {
+                    p = p + dmx(0.0, fl);
+                    q = q + dmn(0.0, fl);
+                }
+            }
+        }
+        __syncthreads();
+        sMx[tid] = vmax;
+        sMn[tid] = vmin;
+        __syncthreads();
+        if (act) {
+            double bmax, bmin;
+            if (k == nzmax - 1 || k == nzmin) {
+                bmax = vmax - lo;
+                bmin = vmin - lo;
+            } else {
+                bmax = dmx(dmx(sMx[tid - 1], vmax), sMx[tid + 1]) - lo;
+                bmin = dmn(dmn(sMn[tid - 1], vmin), sMn[tid + 1]) - lo;
+            }
+            double flux = p * dt / av_ / hn + 1e-16;
+            t.rp[t.sN * tr + ci] = dmn(1.0, bmax / flux);
+            flux = q * dt / av_ / hn - 1e-16;
+            t.rm[t.sN * tr + ci] = dmn(1.0, bmin / flux);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
+// K_D: limited fluxes, flux2dtracer, T*, implicit vertical diffusion
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &d, int n, double sval)
+{
+    switch (id) {
+    case 1: return -dt * (d.heat_flux[n] / FGPU_VCPW + sval * d.water_flux[n] * d.is_nonlinfs);
+    case 2: return dt * (d.virtual_salt[n] + d.relax_salt[n] + d.real_salt_flux[n] * d.is_nonlinfs);
+    default: return 0.0;
+    }
+}
+
+template <int LB>
+__global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer)
+{
+    const int n = xcd_column(m.N2);
+    if (n >= m.N2) return;
+    __shared__ double sRp[LB + 2], sRm[LB + 2], sAv[LB + 2];
+    __shared__ double sa[LB + 2], sb[LB + 2], sc[LB + 2], sZ[LB + 2];
+    __shared__ double sTs[TCH][LB + 2], stp[TCH][LB + 2], scp[TCH][LB + 2];
+    const int tid = threadIdx.x, k = tid + 1;
+    const int L = m.L, NL = m.NL;
+    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    const bool act = (k >= nzmin && k <= nzmax - 1);
+    const size_t ci = (size_t)n * L + k - 1;
+    const NodeEdges E = load_node_edges(m, n);
+    double hn = 1.0, hh = 0.0, av_ = 1.0, arv = 0.0;
+    if (act) {
+        hn = m.hnode_new[ci];
+        hh = m.hnode[ci];
+        av_ = m.areasvol[(size_t)n * NL + k - 1];
+        arv = m.area[(size_t)n * NL + k - 1];
+    }
+    // Z_n of the implicit diffusion, rebuilt from hnode_new when the geometry is set
+    if (act) sZ[tid] = m.zn_vd[ci];
+    __syncthreads();
+    // tridiagonal coefficients (oce_ale_tracer.F90:1098-1175): tracer independent
+    if (act) {
+        const double *kv = d.Kv + (size_t)n * NL;
+        const double zinv = 1.0 * dt;
+        double a = 0.0, b, c;
+        if (k == nzmin) {
+            const double zinv2 = 1.0 / (sZ[k - 1] - sZ[k]);
+            c = -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_;
+            b = -c + hn;
+        } else if (k <= nzmax - 2) {
+            const double zinv1 = 1.0 / (sZ[k - 2] - sZ[k - 1]);
+            const double zinv2 = 1.0 / (sZ[k - 1] - sZ[k]);
+            a = -(kv[k - 1]) * zinv1 * zinv * (arv / av_);
+            c = -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_;
+            b = -a - c + hn;
+        } else {
+            const double zinv1 = 1.0 / (sZ[k - 2] - sZ[k - 1]);
+            a = -(kv[k - 1]) * zinv1 * zinv * (arv / av_);
+            c = 0.0;
+            b = -a + hn;
+        }
+        sa[tid] = a;
+        sb[tid] = b;
+        sc[tid] = c;
+    }
+    for (int t0 = 0; t0 < t.ntr; t0 += TCH) {
+        const int nch = min(TCH, t.ntr - t0);
+        for (int c = 0; c < nch; ++c) {
+            const int tr = t0 + c;
+            double *T = t.T + t.sN * tr;
+            double *Told = t.Told + t.sOld * tr;
+            const int abo = t.abo[tr];
+            const bool fct = t.lim[tr] == LIM_FCT;
+            const double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
+            __syncthreads();
+            if (fct && k <= L) {
+                sRp[tid] = t.rp[t.sN * tr + ci];
+                sRm[tid] = t.rm[t.sN * tr + ci];
+            }
+            __syncthreads();
+            // limited vertical antidiffusive flux at interface k (oce_adv_tra_fct.F90:425-456)
+            if (k >= nzmin && k <= nzmax) {
+                const double flux = av[k - 1];
+                double v = flux;
+                if (fct && k <= nzmax - 1) {
+                    double ae = 1.0;
+                    if (k == nzmin) {
+                        if (flux >= 0.0) ae = dmn(ae, sRp[tid]);
+                        else ae = dmn(ae, sRm[tid]);
+                    } else {
+                        if (flux >= 0.0) {
+                            ae = dmn(ae, sRm[tid - 1]);
+                            ae = dmn(ae, sRp[tid]);
+                        } else {
+                            ae = dmn(ae, sRp[tid - 1]);
+                            ae = dmn(ae, sRm[tid]);
+                        }
+                    }
+                    v = ae * flux;
+                }
+                sAv[tid] = v;
+            }
+            __syncthreads();
+            double Tk = 0.0, v = 0.0, h = 0.0;
+            if (act) {
+                Tk = T[ci];
+                if (fct) v = v - Tk * hh + t.fct_lo[t.sN * tr + ci] * hn;
+                v = v + (sAv[tid] - sAv[tid + 1]) * dt / av_;
+            }
+            const double *ah = t.a_h + t.sE * tr;
+            const double rpn = (act && fct) ? sRp[tid] : 1.0, rmn = (act && fct) ? sRm[tid] : 1.0;
+            for (int j = 0; j < E.deg; ++j) {
+                const int q = rl(E.q, j), mo = rl(E.other, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                if (!(act && k >= a0 && k <= a1)) continue;
+                double fl = ah[(size_t)(q >> 1) * L + k - 1];
+                if (fct) {
+                    // b3 horizontal (oce_adv_tra_fct.F90:483-497) with R of both nodes
+                    const size_t oi = (size_t)mo * L + k - 1;
+                    const double rpo = t.rp[t.sN * tr + oi], rmo = t.rm[t.sN * tr + oi];
+                    const double rp1 = (q & 1) ? rpo : rpn, rm1 = (q & 1) ? rmo : rmn;
+                    const double rp2 = (q & 1) ? rpn : rpo, rm2 = (q & 1) ? rmn : rmo;
+                    double ae = 1.0;
+                    if (fl >= 0.0) {
+                        ae = dmn(ae, rp1);
+                        ae = dmn(ae, rm2);
+                    } else {
+                        ae = dmn(ae, rm1);
+                        ae = dmn(ae, rp2);
+                    }
+                    fl = ae * fl;
+                }
+                if (q & 1) h = h - fl * dt / av_;
+                else h = h + fl * dt / av_;
+            }
+            if (act) {
+                double del = 0.0 + h + v;
+                del = del + Tk * (hh - hn);
+                sTs[c][tid] = Tk + del / hn;
+                // valuesold rotation (oce_tracer_mod.F90:94-123)
+                if (abo == 2) {
+                    Told[ci] = Tk;
+                } else {
+                    Told[2 * ci + 1] = Told[2 * ci];
+                    Told[2 * ci] = Tk;
+                }
+            }
+        }
+        __syncthreads();
+        // right-hand sides (oce_ale_tracer.F90:1206-1229, 1377-1383), all tracers of the chunk
+        for (int c = 0; c < nch; ++c) {
+            const int tr = t0 + c;
+            if (!t.ivd[tr] || !act) continue;
+            const double dz = hn;
+            double r;
+            if (k == nzmin) r = -(sb[tid] - dz) * sTs[c][tid] - sc[tid] * sTs[c][tid + 1];
+            else if (k <= nzmax - 2) r = -sa[tid] * sTs[c][tid - 1] - (sb[tid] - dz) * sTs[c][tid] - sc[tid] * sTs[c][tid + 1];
+            else r = -sa[tid] * sTs[c][tid - 1] - (sb[tid] - dz) * sTs[c][tid];
+            if (k == nzmin) r = r + bc_surface_d(t.id[tr], dt, d, n, sTs[c][tid]);
+            stp[c][tid] = r;
+        }
+        __syncthreads();
+        // Thomas sweeps (oce_ale_tracer.F90:1406-1430), one lane per tracer; the
+        // pivot chain (cp, m) is evaluated in the same loop as tp so that the two
+        // recurrences overlap
+        if (tid < nch && t.ivd[t0 + tid]) {
+            const int c = tid;
+            double cp = sc[nzmin - 1] / sb[nzmin - 1];
+            double tp = stp[c][nzmin - 1] / sb[nzmin - 1];
+            scp[c][nzmin - 1] = cp;
+            stp[c][nzmin - 1] = tp;
+            for (int nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+                const double an = sa[nz - 1];
+                const double mm = sb[nz - 1] - cp * an;
+                cp = sc[nz - 1] / mm;
+                tp = (stp[c][nz - 1] - tp * an) / mm;
+                scp[c][nz - 1] = cp;
+                stp[c][nz - 1] = tp;
+            }
+            double x = stp[c][nzmax - 2];
+            sTs[c][nzmax - 2] = sTs[c][nzmax - 2] + x;
+            for (int nz = nzmax - 2; nz >= nzmin; --nz) {
+                x = stp[c][nz - 1] - scp[c][nz - 1] * x;
+                sTs[c][nz - 1] = sTs[c][nz - 1] + x;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            for (int c = 0; c < nch; ++c) {
+                const int tr = t0 + c;
+                double v = sTs[c][tid];
+                if (tr == clamp_tracer) {
+                    if (v > 45.0) v = 45.0;
+                    if (v < 3.0) v = 3.0;
+                }
+                t.T[t.sN * tr + ci] = v;
+            }
+        }
+    }
+}
+
+// Z levels of diff_ver_part_impl_ale rebuilt bottom-up from hnode_new and
+// zbar_n_bot (oce_ale_tracer.F90:1088-1096), one thread per column
+__global__ void k_zn_vdiff(DevMesh m, double *zn)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.N2) return;
+    const int L = m.L, nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    const double *hn = m.hnode_new + (size_t)n * L;
+    double *z = zn + (size_t)n * L;
+    double zb = m.zbar_n_bot[n];
+    z[nzmax - 2] = zb + hn[nzmax - 2] / 2.0;
+    for (int nz = nzmax - 1; nz >= nzmin + 1; --nz) {
+        zb = zb + hn[nz - 1];
+        z[nz - 2] = zb + hn[nz - 2] / 2.0;
+    }
+}
+
+// ==============================================================================
+// host side
+// ==============================================================================
+int fgpu_compute_zn(fgpu_context *c)
+{
+    if (!c->m.zn_vd) {
+        double *p;
+        int rc = fgpu_alloc(c, (void **)&p, sizeof(double) * (size_t)c->N2 * (c->nl - 1));
+        if (rc) return rc;
+        FGPU_HIP(hipMemset(p, 0, sizeof(double) * (size_t)c->N2 * (c->nl - 1)));
+        c->m.zn_vd = p;
+    }
+    hipLaunchKernelGGL(k_zn_vdiff, dim3((c->N2 + 255) / 256), dim3(256), 0, c->stream, c->m, (double *)c->m.zn_vd);
+    FGPU_HIP(hipGetLastError());
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    return FGPU_OK;
+}
+static inline unsigned xcd_grid(int ncol) { return (unsigned)(((ncol + NXCD - 1) / NXCD) * NXCD); }
+
+int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
+{
+    const DevMesh &m = c->m;
+    const int nt = c->t.ntr;
+    const int clamp = nt >= 2 ? 1 : -1;
+    if (m.NL > 128) return fgpu_fail(FGPU_ERR_ARG, "more than 127 layers");
+#define LAUNCH5(LBV)                                                                                                   \
+    {                                                                                                                  \
+        {                                                                                                              \
+            ProfScope ps(c, "kG_node");                                                                                \
+            hipLaunchKernelGGL(kG_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->t, c->d_kneed);      \
+        }                                                                                                              \
+        {                                                                                                              \
+            ProfScope ps(c, "kA_edge");                                                                                \
+            hipLaunchKernelGGL(kA_edge<LBV>, dim3(xcd_grid(m.E2)), dim3(LBV), 0, c->stream, m, c->d, c->t);            \
+        }                                                                                                              \
+        {                                                                                                              \
+            ProfScope ps(c, "kB_node");                                                                                \
+            hipLaunchKernelGGL(kB_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->d, c->t, dt);        \
+        }                                                                                                              \
+        {                                                                                                              \
+            ProfScope ps(c, "kC_node");                                                                                \
+            hipLaunchKernelGGL(kC_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->t, dt);              \
+        }                                                                                                              \
+        {                                                                                                              \
+            ProfScope ps(c, "kD_node");                                                                                \
+            hipLaunchKernelGGL(kD_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp); \
+        }                                                                                                              \
+    }
+    // one lane per level and per interface: LB >= NL
+    if (m.NL <= 64) LAUNCH5(64)
+    else LAUNCH5(128)
+#undef LAUNCH5
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("fused step launch: ") + hipGetErrorString(e));
+    return FGPU_OK;
+}

@@ -814,6 +814,7 @@ extern "C" int fgpu_diff_ver_part_impl_ale(fgpu_context *c, double dt, int tr_nu
 // one batched step of every tracer, enqueued on the context stream
 static int enqueue_step(fgpu_context *c, double dt)
 {
+    if (c->fused) return fgpu_enqueue_fused_step(c, dt);
     const int nt = c->t.ntr;
     int rc;
     if ((rc = launch_init_ab(c, 0, nt))) return rc;
