@@ -109,9 +109,9 @@ struct fgpu_context {
     hipGraphExec_t graph = nullptr;
     double graph_dt = 0.0;
     bool graph_valid = false;
+    bool fused = true;        // batched step: fused kernels (false: one kernel per reference stage)
+    int step_tpb = 4096;      // tracers per block of the fused step kernels (clamped to the tracer count)
     // profiling
-    bool fused = true;
-    int step_blocks = 8192;   // grid cap of the fused step kernels   // batched step: fused kernels (false: one kernel per reference stage)
     bool prof = false;
     std::vector<ProfEntry> prof_entries;
     struct Pending { int idx; hipEvent_t e0, e1; };

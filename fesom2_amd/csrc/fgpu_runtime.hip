@@ -141,9 +141,9 @@ extern "C" int fgpu_create(fgpu_context **out, int device)
     c->device = device;
     const char *sv = std::getenv("FGPU_STEP");
     c->fused = !(sv && std::string(sv) == "staged");
-    if (const char *bv = std::getenv("FGPU_STEP_BLOCKS")) {
+    if (const char *bv = std::getenv("FGPU_STEP_TPB")) {
         int v = std::atoi(bv);
-        if (v >= 8 && v <= (1 << 20)) c->step_blocks = v;
+        if (v >= 1 && v <= 4096) c->step_tpb = v;
     }
     // default priority: stream priorities are left alone on purpose
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -108,7 +108,7 @@ __device__ __forceinline__ NodeEdges load_node_edges(const DevMesh &m, int n)
 // levels kneed(n)..nlevels_nod2D-1
 // ------------------------------------------------------------------------------
 template <int LB>
-__global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int *kneed)
+__global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int *kneed, int tpb)
 {
     const int n = xcd_column(m.N2);
     if (n >= m.N2) return;
@@ -130,7 +130,8 @@ __global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int
         g0 = gs[0]; g1 = gs[1]; g2 = gs[2]; g3 = gs[3]; g4 = gs[4]; g5 = gs[5];
     }
     const bool act = (k >= k0 && k <= nzmax - 1);
-    for (int tr = 0; tr < t.ntr; ++tr) {
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    for (int tr = trb; tr < tre; ++tr) {
         if (t.hor[tr] != HOR_MFCT) continue;
         const double *T = t.T + t.sN * tr;
         double tvol = 0.0, tx = 0.0, ty = 0.0;
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int
 // K_A: edge pass
 // ------------------------------------------------------------------------------
 template <int LB>
-__global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t)
+__global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t, int tpb)
 {
     const int e = xcd_column(m.E2);
     if (e >= m.E2) return;
@@ -214,7 +215,8 @@ __global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t)
     const size_t o1 = (size_t)n1 * L + k - 1, o2 = (size_t)n2 * L + k - 1;
     const size_t ei = (size_t)e * L + k - 1;
     const size_t kk = (size_t)k - 1;
-    for (int tr = 0; tr < t.ntr; ++tr) {
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    for (int tr = trb; tr < tre; ++tr) {
         const double *T = t.T + t.sN * tr;
         const double *Told = t.Told + t.sOld * tr;
         const int abo = t.abo[tr], lim = t.lim[tr], hor = t.hor[tr];
@@ -309,7 +311,7 @@ __device__ __forceinline__ double col_qr4c(int j, int nzmin, int nzmax, const do
 // K_B: LO solution and antidiffusive vertical flux per node column
 // ------------------------------------------------------------------------------
 template <int LB>
-__global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t, double dt)
+__global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t, double dt, int tpb)
 {
     const int n = xcd_column(m.N2);
     if (n >= m.N2) return;
@@ -334,7 +336,8 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
         hh = m.hnode[ci];
         av_ = m.areasvol[(size_t)n * NL + k - 1];
     }
-    for (int tr = 0; tr < t.ntr; ++tr) {
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    for (int tr = trb; tr < tre; ++tr) {
         const double *T = t.T + t.sN * tr;
         const double *Told = t.Told + t.sOld * tr;
         const int abo = t.abo[tr], lim = t.lim[tr], ver = t.ver[tr];
@@ -383,7 +386,7 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
 // neighbours m for which an element adjacent to edge (n, m) has the level.
 // ------------------------------------------------------------------------------
 template <int LB>
-__global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt)
+__global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt, int tpb)
 {
     const int n = xcd_column(m.N2);
     if (n >= m.N2) return;
@@ -399,7 +402,8 @@ __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt
         hn = m.hnode_new[ci];
         av_ = m.areasvol[(size_t)n * NL + k - 1];
     }
-    for (int tr = 0; tr < t.ntr; ++tr) {
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    for (int tr = trb; tr < tre; ++tr) {
         if (t.lim[tr] != LIM_FCT) continue;   // uniform over the block
         const double *T = t.T + t.sN * tr;
         const double *LO = t.fct_lo + t.sN * tr;
@@ -471,7 +475,7 @@ __device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &
 }
 
 template <int LB>
-__global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer)
+__global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb)
 {
     const int n = xcd_column(m.N2);
     if (n >= m.N2) return;
@@ -519,8 +523,9 @@ __global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t,
         sb[tid] = b;
         sc[tid] = c;
     }
-    for (int t0 = 0; t0 < t.ntr; t0 += TCH) {
-        const int nch = min(TCH, t.ntr - t0);
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    for (int t0 = trb; t0 < tre; t0 += TCH) {
+        const int nch = min(TCH, tre - t0);
         for (int c = 0; c < nch; ++c) {
             const int tr = t0 + c;
             double *T = t.T + t.sN * tr;
@@ -696,27 +701,30 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     const int nt = c->t.ntr;
     const int clamp = nt >= 2 ? 1 : -1;
     if (m.NL > 128) return fgpu_fail(FGPU_ERR_ARG, "more than 127 layers");
+    // tracers per block: 1 runs every tracer's chain in its own waves
+    const int tpb = std::max(1, std::min(c->step_tpb, nt));
+    const unsigned ny = (unsigned)((nt + tpb - 1) / tpb);
 #define LAUNCH5(LBV)                                                                                                   \
     {                                                                                                                  \
         {                                                                                                              \
             ProfScope ps(c, "kG_node");                                                                                \
-            hipLaunchKernelGGL(kG_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->t, c->d_kneed);      \
+            hipLaunchKernelGGL(kG_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->t, c->d_kneed, tpb);      \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kA_edge");                                                                                \
-            hipLaunchKernelGGL(kA_edge<LBV>, dim3(xcd_grid(m.E2)), dim3(LBV), 0, c->stream, m, c->d, c->t);            \
+            hipLaunchKernelGGL(kA_edge<LBV>, dim3(xcd_grid(m.E2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, tpb);            \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kB_node");                                                                                \
-            hipLaunchKernelGGL(kB_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->d, c->t, dt);        \
+            hipLaunchKernelGGL(kB_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb);        \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kC_node");                                                                                \
-            hipLaunchKernelGGL(kC_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->t, dt);              \
+            hipLaunchKernelGGL(kC_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->t, dt, tpb);              \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
-            hipLaunchKernelGGL(kD_node<LBV>, dim3(xcd_grid(m.N2)), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp); \
+            hipLaunchKernelGGL(kD_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
         }                                                                                                              \
     }
     // one lane per level and per interface: LB >= NL
