@@ -25,6 +25,18 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_SOURCES = ["fesom_gpu_c.f90", "fesom_gpu_host.f90"]
+FORTRAN_DRIVER = os.path.join(FORTRAN_DIR, "fgpu_case_driver")
+
+
+def flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang", shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
 def flags():
     return ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -57,5 +69,35 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_fortran(verbose: bool = False):
+    """Compile the Fortran host modules and link the case driver against
+    libfesom_gpu.so (rpath to the package directory). Returns the driver path,
+    or None when no Fortran compiler is installed."""
+    fc = flang()
+    if fc is None:
+        return None
+    objdir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in FORTRAN_SOURCES + ["fgpu_case_driver.f90"]:
+        obj = os.path.join(objdir, src.replace(".f90", ".o"))
+        cmd = [fc, "-O2", "-fPIC", "-module-dir", objdir, "-I", objdir, "-c", os.path.join(FORTRAN_DIR, src),
+               "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed on {src}:\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    tmp = FORTRAN_DRIVER + ".tmp"
+    cmd = [fc, "-O2", "-o", tmp, *objs, "-L", HERE, "-lfesom_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, FORTRAN_DRIVER)
+    return FORTRAN_DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

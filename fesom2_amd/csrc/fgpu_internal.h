@@ -64,18 +64,20 @@ struct DevTracers {
     int ntr;
     const int *id, *abo, *hor, *ver, *lim, *ivd;   // ntr each
     const double *ph, *pv;                         // ntr each
-    size_t sN, sNI, sE, sF2, sOld, sG;             // strides: N2*L, N2*NL, E2*L, F2*L*2, N2*L*2, N2*L*2
-    double *T, *Told, *TAB;
-    double *tr_xy;            // F2*L*2
+    size_t sN, sNI, sE, sF2, sOld, sG;             // strides: N2*L, N2*NL, E2*L, F2*L*2, N2*L*(AB_order-1), N2*L*2
+    size_t sStg;              // tracer stride of tr_xy/aux: F2*L*2 with a slot per tracer, 0 with one shared slot
+    double *T, *Told;
     double *lo_h, *a_h;       // E2*L
     double *fct_lo;           // N2*L
     double *a_v;              // N2*NL
-    double *aux;              // F2*L*2
     double *rp, *rm;          // N2*L
     double *gmean;            // N2*L*2 node-mean gradients of the fused step
-    double *dttf_h, *dttf_v;  // N2*L (per-tracer entry points only; one tracer's worth)
-    double *dbg_av;           // N2*NL limited vertical flux (per-tracer entry points only)
-    double *dbg_grad;         // E2*L*4 materialised edge_up_dn_grad (per-tracer entry points only)
+    // buffers of the per-stage kernels only, allocated on first use (fgpu_ensure_staged)
+    double *TAB;              // N2*L per tracer (valuesAB)
+    double *tr_xy, *aux;      // F2*L*2 per slot (tracers%work: one slot shared by the per-tracer calls)
+    double *dttf_h, *dttf_v;  // N2*L, one tracer's worth
+    double *dbg_av;           // N2*NL limited vertical flux of the last per-tracer call
+    double *dbg_grad;         // E2*L*4 materialised edge_up_dn_grad of the last per-tracer call
 };
 
 struct ProfEntry {
@@ -111,6 +113,7 @@ struct fgpu_context {
     bool graph_valid = false;
     bool fused = true;        // batched step: fused kernels (false: one kernel per reference stage)
     int step_tpb = 4096;      // tracers per block of the fused step kernels (clamped to the tracer count)
+    int stage_slots = 0;      // tr_xy/aux slots allocated for the per-stage kernels (0: none yet)
     // profiling
     bool prof = false;
     std::vector<ProfEntry> prof_entries;
@@ -161,3 +164,4 @@ int fgpu_prof_flush(fgpu_context *c);
 // fused batched step (fgpu_step.hip): four launches per step for all tracers
 int fgpu_enqueue_fused_step(fgpu_context *c, double dt);
 int fgpu_compute_zn(fgpu_context *c);
+int fgpu_ensure_staged(fgpu_context *c, int slots);

@@ -161,6 +161,52 @@ static void free_list(std::vector<void *> &v)
     v.clear();
 }
 
+// free one tracer-scope allocation (the stream must be idle)
+static void free_tracer_buf(fgpu_context *c, double *&p)
+{
+    if (!p) return;
+    auto &v = c->tracer_allocs;
+    for (size_t i = 0; i < v.size(); ++i)
+        if (v[i] == (void *)p) { v.erase(v.begin() + (long)i); break; }
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+// Per-stage buffers: valuesAB for every tracer, the del_ttf/limited-flux/edge
+// gradient work arrays of one tracer, and `slots` copies of tr_xy and the FCT
+// element bounds (1 for the per-tracer entry points, which share them like
+// tracers%work; ntr for the batched per-stage step). Allocated on first use so
+// that the fused step's footprint stays at what it reads and writes.
+int fgpu_ensure_staged(fgpu_context *c, int slots)
+{
+    DevTracers &t = c->t;
+    if (t.TAB && c->stage_slots >= slots) return FGPU_OK;
+    FGPU_HIP(hipSetDevice(c->device));
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    int rc;
+    if (!t.TAB) {
+        struct A { double **p; size_t n; } arrs[] = {
+            {&t.TAB, t.sN * (size_t)t.ntr}, {&t.dttf_h, t.sN}, {&t.dttf_v, t.sN}, {&t.dbg_av, t.sNI},
+            {&t.dbg_grad, t.sE * 4}};
+        for (auto &a : arrs) {
+            if ((rc = fgpu_alloc(c, (void **)a.p, a.n * sizeof(double), true))) return rc;
+            FGPU_HIP(hipMemset(*a.p, 0, a.n * sizeof(double)));
+        }
+    }
+    if (c->stage_slots < slots) {
+        free_tracer_buf(c, t.tr_xy);
+        free_tracer_buf(c, t.aux);
+        for (double **p : {&t.tr_xy, &t.aux}) {
+            if ((rc = fgpu_alloc(c, (void **)p, t.sF2 * (size_t)slots * sizeof(double), true))) return rc;
+            FGPU_HIP(hipMemset(*p, 0, t.sF2 * (size_t)slots * sizeof(double)));
+        }
+        c->stage_slots = slots;
+        t.sStg = slots > 1 ? t.sF2 : 0;
+    }
+    c->graph_valid = false;
+    return FGPU_OK;
+}
+
 extern "C" int fgpu_destroy(fgpu_context *c)
 {
     if (!c) return FGPU_OK;
@@ -548,14 +594,18 @@ extern "C" int fgpu_tracers_init(fgpu_context *c, int num_tracers)
     t.sNI = N2 * NL;
     t.sE = E2 * L;
     t.sF2 = F2 * L * 2;
-    t.sOld = N2 * L * 2;
+    t.sOld = N2 * L;   // AB2; grows to 2*N2*L when a tracer is configured with AB_order 3
     t.sG = N2 * L * 2;
+    t.sStg = 0;
+    t.TAB = t.tr_xy = t.aux = t.dttf_h = t.dttf_v = t.dbg_av = t.dbg_grad = nullptr;
+    c->stage_slots = 0;
     int rc;
+    // the fused batched step needs only these; the per-stage buffers come with
+    // the first per-tracer call (fgpu_ensure_staged)
     struct A { double **p; size_t n; } arrs[] = {
-        {&t.T, t.sN * nt}, {&t.Told, t.sOld * nt}, {&t.TAB, t.sN * nt}, {&t.tr_xy, t.sF2 * nt},
-        {&t.lo_h, t.sE * nt}, {&t.a_h, t.sE * nt}, {&t.fct_lo, t.sN * nt}, {&t.a_v, t.sNI * nt},
-        {&t.aux, t.sF2 * nt}, {&t.rp, t.sN * nt}, {&t.rm, t.sN * nt}, {&t.gmean, t.sG * nt}, {&t.dttf_h, t.sN},
-        {&t.dttf_v, t.sN}, {&t.dbg_av, t.sNI}, {&t.dbg_grad, t.sE * 4}};
+        {&t.T, t.sN * nt}, {&t.Told, t.sOld * nt}, {&t.lo_h, t.sE * nt}, {&t.a_h, t.sE * nt},
+        {&t.fct_lo, t.sN * nt}, {&t.a_v, t.sNI * nt}, {&t.rp, t.sN * nt}, {&t.rm, t.sN * nt},
+        {&t.gmean, t.sG * nt}};
     for (auto &a : arrs) {
         if ((rc = fgpu_alloc(c, (void **)a.p, a.n * sizeof(double), true))) return rc;
         FGPU_HIP(hipMemset(*a.p, 0, a.n * sizeof(double)));
@@ -635,11 +685,28 @@ extern "C" int fgpu_tracer_config(fgpu_context *c, int tr_num, int ID, int AB_or
                                           " specified in boundary conditions (supported: 1, 2, 100, 301-303)");
     }
     const int i = tr_num - 1;
+    FGPU_HIP(hipSetDevice(c->device));
+    DevTracers &t = c->t;
+    if (AB_order == 3 && t.sOld < 2 * t.sN) {
+        // valuesold(2,nl-1,nod2D) for every tracer: the first time level of each
+        // tracer's old block is kept, the second starts at zero
+        FGPU_HIP(hipStreamSynchronize(c->stream));
+        double *grown = nullptr;
+        const size_t nt = (size_t)t.ntr;
+        int rc = fgpu_alloc(c, (void **)&grown, 2 * t.sN * nt * sizeof(double), true);
+        if (rc) return rc;
+        FGPU_HIP(hipMemset(grown, 0, 2 * t.sN * nt * sizeof(double)));
+        for (size_t k = 0; k < nt; ++k)
+            FGPU_HIP(hipMemcpy(grown + 2 * t.sN * k, t.Told + t.sOld * k, t.sOld * sizeof(double),
+                               hipMemcpyDeviceToDevice));
+        free_tracer_buf(c, t.Told);
+        t.Told = grown;
+        t.sOld = 2 * t.sN;
+    }
     c->h_id[i] = ID; c->h_abo[i] = AB_order; c->h_hor[i] = h; c->h_ver[i] = v; c->h_lim[i] = l;
     c->h_ph[i] = tra_adv_ph; c->h_pv[i] = tra_adv_pv; c->h_ivd[i] = i_vert_diff ? 1 : 0;
     c->h_configured[i] = 1;
     c->graph_valid = false;
-    FGPU_HIP(hipSetDevice(c->device));
     return push_config(c);
 }
 
@@ -686,8 +753,8 @@ extern "C" int fgpu_tracer_work_download(fgpu_context *c, int tr_num, const char
     const double *src = nullptr;
     size_t cnt = 0;
     std::string s(name);
-    if (s == "valuesAB") { src = t.TAB + t.sN * i; cnt = t.sN; }
-    else if (s == "tr_xy") { src = t.tr_xy + t.sF2 * i; cnt = t.sF2; }
+    if (s == "valuesAB") { src = t.TAB ? t.TAB + t.sN * i : nullptr; cnt = t.sN; }
+    else if (s == "tr_xy") { src = t.tr_xy ? t.tr_xy + t.sStg * i : nullptr; cnt = t.sF2; }
     else if (s == "edge_up_dn_grad") { src = t.dbg_grad; cnt = t.sE * 4; }
     else if (s == "fct_LO") { src = t.fct_lo + t.sN * i; cnt = t.sN; }
     else if (s == "adv_flux_hor") { src = t.a_h + t.sE * i; cnt = t.sE; }
@@ -697,6 +764,7 @@ extern "C" int fgpu_tracer_work_download(fgpu_context *c, int tr_num, const char
     else if (s == "fct_plus") { src = t.rp + t.sN * i; cnt = t.sN; }
     else if (s == "fct_minus") { src = t.rm + t.sN * i; cnt = t.sN; }
     else return fgpu_fail(FGPU_ERR_ARG, "unknown work array " + s);
+    if (!src) return fgpu_fail(FGPU_ERR_STATE, "work array " + s + " exists after a per-tracer call only");
     if ((size_t)n != cnt) return fgpu_fail(FGPU_ERR_ARG, "work array " + s + " has " + std::to_string(cnt) + " entries");
     FGPU_HIP(hipMemcpy(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
     return FGPU_OK;

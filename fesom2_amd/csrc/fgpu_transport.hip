@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(BLK) k_tr_xy(DevMesh m, DevTracers t, int tr0)
     const double t0 = T[(size_t)n0 * m.L + k - 1], t1 = T[(size_t)n1 * m.L + k - 1],
                  t2 = T[(size_t)n2 * m.L + k - 1];
     const double *gs = m.gsca + 6 * (size_t)el;
-    double *o = t.tr_xy + t.sF2 * tr + 2 * i;
+    double *o = t.tr_xy + t.sStg * tr + 2 * i;
     o[0] = gs[0] * t0 + gs[1] * t1 + gs[2] * t2;
     o[1] = gs[3] * t0 + gs[4] * t1 + gs[5] * t2;
 }
@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(BLK) k_fill_up_dn_grad(DevMesh m, DevTracers t
     const int n1 = m.edges[2 * e], n2 = m.edges[2 * e + 1];
     if (k > max(m.nlevn[n1], m.nlevn[n2]) - 1) return;
     double g[4];
-    up_dn_grad(m, t.tr_xy + t.sF2 * tr, e, n1, n2, k, g);
+    up_dn_grad(m, t.tr_xy + t.sStg * tr, e, n1, n2, k, g);
     double *o = t.dbg_grad + 4 * i;
     // levels a node does not reach keep zero in components it would write
     o[0] = g[0]; o[1] = g[1]; o[2] = g[2]; o[3] = g[3];
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(BLK) k_edge_flux(DevMesh m, DevDyn d, DevTrace
     if (hor == HOR_MFCT) {
         const double no = t.ph[tr];
         double g[4];
-        up_dn_grad(m, t.tr_xy + t.sF2 * tr, e, n1, n2, k, g);
+        up_dn_grad(m, t.tr_xy + t.sStg * tr, e, n1, n2, k, g);
         const double edx = m.edge_dxdy[2 * e], edy = m.edge_dxdy[2 * e + 1];
         const double Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
         const double Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(BLK) k_elem_aux(DevMesh m, DevTracers t, int t
     if (i >= (size_t)m.F2 * m.L) return;
     const int el = (int)(i / m.L);
     const int k = (int)(i - (size_t)el * m.L) + 1;
-    double *o = t.aux + t.sF2 * tr + 2 * i;
+    double *o = t.aux + t.sStg * tr + 2 * i;
     if (k <= m.ulev[el] - 1 || k >= m.nlev[el]) {
         o[0] = -1e3;
         o[1] = 1e3;
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(BLK) k_node_limit(DevMesh m, DevTracers t, int
     const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
     if (k < nzmin || k > nzmax - 1) return;
     const int L = m.L, NL = m.NL;
-    const double *aux = t.aux + t.sF2 * tr;
+    const double *aux = t.aux + t.sStg * tr;
     const double lo = t.fct_lo[t.sN * tr + i];
     double bmax, bmin, vmax, vmin;
     if (k == nzmax - 1 || k == nzmin) {
@@ -759,6 +759,7 @@ extern "C" int fgpu_init_tracers_AB(fgpu_context *c, int tr_num)
     int rc = check_ready(c);
     if (rc) return rc;
     if ((rc = check_tracer(c, tr_num))) return rc;
+    if ((rc = fgpu_ensure_staged(c, 1))) return rc;
     const int tr = tr_num - 1;
     if ((rc = launch_init_ab(c, tr, 1))) return rc;
     hipLaunchKernelGGL(k_fill_up_dn_grad, dim3(nblk((size_t)c->m.E2 * c->m.L)), dim3(BLK), 0, c->stream, c->m, c->t, tr);
@@ -772,6 +773,7 @@ extern "C" int fgpu_do_oce_adv_tra(fgpu_context *c, double dt, int tr_num)
     int rc = check_ready(c);
     if (rc) return rc;
     if ((rc = check_tracer(c, tr_num))) return rc;
+    if ((rc = fgpu_ensure_staged(c, 1))) return rc;
     const int tr = tr_num - 1;
     FGPU_HIP(hipMemsetAsync(c->t.dttf_h, 0, c->t.sN * sizeof(double), c->stream));
     FGPU_HIP(hipMemsetAsync(c->t.dttf_v, 0, c->t.sN * sizeof(double), c->stream));
@@ -786,6 +788,7 @@ extern "C" int fgpu_diff_tracers_ale(fgpu_context *c, double dt, int tr_num)
     int rc = check_ready(c);
     if (rc) return rc;
     if ((rc = check_tracer(c, tr_num))) return rc;
+    if ((rc = fgpu_ensure_staged(c, 1))) return rc;
     const int tr = tr_num - 1;
     if (c->h_ivd[tr] && !c->kv_ready) return fgpu_fail(FGPU_ERR_STATE, "Kv not set (fgpu_set_kv)");
     if (c->h_ivd[tr] && !c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "surface forcing not set");
@@ -802,6 +805,7 @@ extern "C" int fgpu_diff_ver_part_impl_ale(fgpu_context *c, double dt, int tr_nu
     int rc = check_ready(c);
     if (rc) return rc;
     if ((rc = check_tracer(c, tr_num))) return rc;
+    if ((rc = fgpu_ensure_staged(c, 1))) return rc;
     if (!c->kv_ready) return fgpu_fail(FGPU_ERR_STATE, "Kv not set (fgpu_set_kv)");
     if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "surface forcing not set");
     const int tr = tr_num - 1;
@@ -836,6 +840,7 @@ static int check_step(fgpu_context *c)
 static int run_steps(fgpu_context *c, double dt, int nsteps)
 {
     int rc;
+    if (!c->fused && (rc = fgpu_ensure_staged(c, c->t.ntr))) return rc;
     if (c->prof) {
         for (int s = 0; s < nsteps; ++s)
             if ((rc = enqueue_step(c, dt))) return rc;

@@ -1,0 +1,252 @@
+! fesom_gpu_host.f90 -- Fortran host side of the MI355X tracer-transport / SSH path.
+!
+! Mirrors the reference's call surface for one partition resident on one GPU:
+!   init_tracers_AB(tr_num)                   oce_tracer_mod.F90:13-143
+!   do_oce_adv_tra(dt, vel, w, wi, we, tr_num) oce_adv_tra_driver.F90:46-66
+!   diff_tracers_ale(dt, tr_num)              oce_ale_tracer.F90:533-793 (+ :318-322)
+!   diff_ver_part_impl_ale(dt, tr_num)        oce_ale_tracer.F90:907-1435
+!   solve_tracers_ale(dt, nsteps)             oce_ale_tracer.F90:135-529 (tracer loop, batched)
+!   init_stiff_mat_ale(dt, g, alpha, theta)   oce_ale.F90:1393-1582
+!   ssh_solve_preconditioner(solverinfo)      solver.F90:31-117
+!   ssh_solve_cg(x, rhs, solverinfo)          solver.F90:120-385
+!   solve_ssh_ale(d_eta, ssh_rhs, solverinfo) oce_ale.F90:3056-3099
+! The dynamics/tracers/mesh derived-type arguments of the reference are held by
+! the device context of this module (uploaded once by gpu_mesh_setup,
+! gpu_set_vertical, gpu_set_state and gpu_tracer_put); velocities passed to
+! do_oce_adv_tra are uploaded on each call like the reference's arguments.
+! A failing call prints the library's message and stops, as par_ex does for
+! an unknown scheme or boundary-condition ID.
+module fesom_gpu_host
+  use, intrinsic :: iso_c_binding
+  use fesom_gpu_c
+  implicit none
+  private
+
+  integer, parameter, public :: WP = c_double
+
+  ! mirrors T_SOLVERINFO (MOD_DYN.F90:13-46): knobs and per-run counters
+  type, public :: t_gpu_solverinfo
+     integer :: maxiter = 2000
+     real(WP) :: soltol = 1e-5_WP
+     integer :: precond_variant = 1
+     integer :: iters_last = 0, iters_max = 0, iters_sum = 0, nsolves = 0
+     integer :: nonconv = 0, nbreakdown = 0, nnegrz = 0
+     real(WP) :: resid_last = 0.0_WP, rtol_last = 0.0_WP
+     real(WP) :: ms_last = 0.0_WP
+     logical :: pr_ready = .false.
+  end type t_gpu_solverinfo
+
+  type(c_ptr), save :: ctx = c_null_ptr
+
+  public :: gpu_init, gpu_finalize, gpu_check, gpu_mesh_setup, gpu_set_vertical, gpu_set_state, &
+       gpu_tracers_init, gpu_tracer_config, gpu_tracer_put, gpu_tracer_get, gpu_tracer_content, &
+       init_tracers_AB, do_oce_adv_tra, diff_tracers_ale, diff_ver_part_impl_ale, solve_tracers_ale, &
+       init_stiff_mat_ale, ssh_solve_preconditioner, ssh_solve_cg, solve_ssh_ale, gpu_synchronize
+
+contains
+
+  subroutine gpu_check(status, where)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (status == FGPU_OK) return
+    call c_f_pointer(fgpu_last_error(), msg, [4096])
+    n = 0
+    do while (n < 4096)
+       if (msg(n + 1) == c_null_char) exit
+       n = n + 1
+    end do
+    write(*, '(a,a,a,i0,a,4096a)') 'fesom_gpu: ', where, ' failed with status ', status, ': ', msg(1:n)
+    error stop 1
+  end subroutine gpu_check
+
+  function cstr(s) result(c)
+    character(len=*), intent(in) :: s
+    character(kind=c_char, len=len_trim(s) + 1) :: c
+    c = trim(s) // c_null_char
+  end function cstr
+
+  subroutine gpu_init(device)
+    integer, intent(in) :: device
+    if (c_associated(ctx)) return
+    call gpu_check(fgpu_create(ctx, int(device, c_int)), 'fgpu_create')
+  end subroutine gpu_init
+
+  subroutine gpu_finalize()
+    if (.not. c_associated(ctx)) return
+    call gpu_check(fgpu_destroy(ctx), 'fgpu_destroy')
+    ctx = c_null_ptr
+  end subroutine gpu_finalize
+
+  subroutine gpu_synchronize()
+    call gpu_check(fgpu_synchronize(ctx), 'fgpu_synchronize')
+  end subroutine gpu_synchronize
+
+  ! t_mesh components (MOD_MESH.F90:22-175), 1-based as the reference holds them
+  subroutine gpu_mesh_setup(nod2D, elem2D, edge2D, nl, cyclic_length, coord_nod2D, elem2D_nodes, &
+       edges, edge_tri, nod_in_elem2D, nod_in_elem2D_num, nlevels, ulevels, nlevels_nod2D, &
+       ulevels_nod2D, nlevels_nod2D_min, ulevels_nod2D_max, elem_area, elem_cos, edge_dxdy, &
+       edge_cross_dxdy, gradient_sca)
+    integer, intent(in) :: nod2D, elem2D, edge2D, nl
+    real(WP), intent(in) :: cyclic_length
+    real(WP), intent(in) :: coord_nod2D(:, :), elem_area(:), elem_cos(:), edge_dxdy(:, :), &
+         edge_cross_dxdy(:, :), gradient_sca(:, :)
+    integer, intent(in) :: elem2D_nodes(:, :), edges(:, :), edge_tri(:, :), nod_in_elem2D(:, :), &
+         nod_in_elem2D_num(:), nlevels(:), ulevels(:), nlevels_nod2D(:), ulevels_nod2D(:), &
+         nlevels_nod2D_min(:), ulevels_nod2D_max(:)
+    call gpu_check(fgpu_mesh_init(ctx, int(nod2D, c_int), int(elem2D, c_int), int(edge2D, c_int), &
+         int(nl, c_int), int(size(nod_in_elem2D, 1), c_int), cyclic_length, coord_nod2D, elem2D_nodes, &
+         edges, edge_tri, nod_in_elem2D, nod_in_elem2D_num, nlevels, ulevels, nlevels_nod2D, &
+         ulevels_nod2D, nlevels_nod2D_min, ulevels_nod2D_max, elem_area, elem_cos, edge_dxdy, &
+         edge_cross_dxdy, gradient_sca), 'fgpu_mesh_init')
+  end subroutine gpu_mesh_setup
+
+  subroutine gpu_set_vertical(area, areasvol, hnode, hnode_new, helem, zbar_3d_n, Z_3d_n, zbar_n_bot, &
+       zbar_e_bot, zbar_e_srf)
+    real(WP), intent(in) :: area(:, :), areasvol(:, :), hnode(:, :), hnode_new(:, :), helem(:, :), &
+         zbar_3d_n(:, :), Z_3d_n(:, :), zbar_n_bot(:), zbar_e_bot(:), zbar_e_srf(:)
+    call gpu_check(fgpu_set_vertical(ctx, area, areasvol, hnode, hnode_new, helem, zbar_3d_n, Z_3d_n, &
+         zbar_n_bot, zbar_e_bot, zbar_e_srf), 'fgpu_set_vertical')
+  end subroutine gpu_set_vertical
+
+  ! velocities (t_dyn), o_ARRAYS%Kv and the bc_surface forcing arrays
+  subroutine gpu_set_state(uv, w, w_e, w_i, Kv, heat_flux, water_flux, virtual_salt, relax_salt, &
+       real_salt_flux, is_nonlinfs)
+    real(WP), intent(in) :: uv(:, :, :), w(:, :), w_e(:, :), w_i(:, :), Kv(:, :)
+    real(WP), intent(in) :: heat_flux(:), water_flux(:), virtual_salt(:), relax_salt(:), real_salt_flux(:)
+    real(WP), intent(in) :: is_nonlinfs
+    call gpu_check(fgpu_set_velocity(ctx, uv, w, w_e, w_i, 0_c_int), 'fgpu_set_velocity')
+    call gpu_check(fgpu_set_kv(ctx, Kv), 'fgpu_set_kv')
+    call gpu_check(fgpu_set_surface_forcing(ctx, heat_flux, water_flux, virtual_salt, relax_salt, &
+         real_salt_flux, is_nonlinfs), 'fgpu_set_surface_forcing')
+  end subroutine gpu_set_state
+
+  subroutine gpu_tracers_init(num_tracers)
+    integer, intent(in) :: num_tracers
+    call gpu_check(fgpu_tracers_init(ctx, int(num_tracers, c_int)), 'fgpu_tracers_init')
+  end subroutine gpu_tracers_init
+
+  ! one tracer's namelist entry (config/namelist.tra:33-38)
+  subroutine gpu_tracer_config(tr_num, ID, AB_order, tra_adv_hor, tra_adv_ver, tra_adv_lim, &
+       tra_adv_ph, tra_adv_pv, i_vert_diff)
+    integer, intent(in) :: tr_num, ID, AB_order
+    character(len=*), intent(in) :: tra_adv_hor, tra_adv_ver, tra_adv_lim
+    real(WP), intent(in) :: tra_adv_ph, tra_adv_pv
+    logical, intent(in) :: i_vert_diff
+    integer(c_int) :: ivd
+    ivd = merge(1_c_int, 0_c_int, i_vert_diff)
+    call gpu_check(fgpu_tracer_config(ctx, int(tr_num, c_int), int(ID, c_int), int(AB_order, c_int), &
+         cstr(tra_adv_hor), cstr(tra_adv_ver), cstr(tra_adv_lim), tra_adv_ph, tra_adv_pv, ivd), &
+         'fgpu_tracer_config')
+  end subroutine gpu_tracer_config
+
+  ! tracers%data(tr_num)%values(nl-1,nod2D) and %valuesold(AB_order-1,nl-1,nod2D)
+  subroutine gpu_tracer_put(tr_num, values, valuesold)
+    integer, intent(in) :: tr_num
+    real(WP), intent(in) :: values(:, :), valuesold(:, :)
+    call gpu_check(fgpu_tracer_upload(ctx, int(tr_num, c_int), values, valuesold), 'fgpu_tracer_upload')
+  end subroutine gpu_tracer_put
+
+  subroutine gpu_tracer_get(tr_num, values, valuesold)
+    integer, intent(in) :: tr_num
+    real(WP), intent(out) :: values(:, :), valuesold(:, :)
+    call gpu_check(fgpu_tracer_download(ctx, int(tr_num, c_int), values, valuesold), 'fgpu_tracer_download')
+  end subroutine gpu_tracer_get
+
+  real(WP) function gpu_tracer_content(tr_num)
+    integer, intent(in) :: tr_num
+    call gpu_check(fgpu_tracer_content(ctx, int(tr_num, c_int), gpu_tracer_content), 'fgpu_tracer_content')
+  end function gpu_tracer_content
+
+  ! ---- reference call surface ------------------------------------------------
+
+  subroutine init_tracers_AB(tr_num)
+    integer, intent(in) :: tr_num
+    call gpu_check(fgpu_init_tracers_AB(ctx, int(tr_num, c_int)), 'init_tracers_AB')
+  end subroutine init_tracers_AB
+
+  ! vel(2,nl-1,elem2D); W, WI, WE(nl,nod2D). Leaves del_ttf_advhoriz/advvert on
+  ! the device for diff_tracers_ale, like tracers%work in the reference.
+  subroutine do_oce_adv_tra(dt, vel, w, wi, we, tr_num)
+    real(WP), intent(in) :: dt
+    real(WP), intent(in) :: vel(:, :, :), w(:, :), wi(:, :), we(:, :)
+    integer, intent(in) :: tr_num
+    call gpu_check(fgpu_set_velocity(ctx, vel, w, we, wi, 0_c_int), 'do_oce_adv_tra (velocity)')
+    call gpu_check(fgpu_do_oce_adv_tra(ctx, dt, int(tr_num, c_int)), 'do_oce_adv_tra')
+  end subroutine do_oce_adv_tra
+
+  subroutine diff_tracers_ale(dt, tr_num)
+    real(WP), intent(in) :: dt
+    integer, intent(in) :: tr_num
+    call gpu_check(fgpu_diff_tracers_ale(ctx, dt, int(tr_num, c_int)), 'diff_tracers_ale')
+  end subroutine diff_tracers_ale
+
+  subroutine diff_ver_part_impl_ale(dt, tr_num)
+    real(WP), intent(in) :: dt
+    integer, intent(in) :: tr_num
+    call gpu_check(fgpu_diff_ver_part_impl_ale(ctx, dt, int(tr_num, c_int)), 'diff_ver_part_impl_ale')
+  end subroutine diff_ver_part_impl_ale
+
+  ! every tracer advances together (AB, advection, T*, vertical diffusion,
+  ! salinity clamp); nsteps > 1 replays the captured step
+  subroutine solve_tracers_ale(dt, nsteps)
+    real(WP), intent(in) :: dt
+    integer, intent(in) :: nsteps
+    call gpu_check(fgpu_solve_tracers_ale(ctx, dt, int(nsteps, c_int)), 'solve_tracers_ale')
+  end subroutine solve_tracers_ale
+
+  subroutine init_stiff_mat_ale(dt, g, alpha, theta)
+    real(WP), intent(in) :: dt, g, alpha, theta
+    call gpu_check(fgpu_init_stiff_mat_ale(ctx, dt, g, alpha, theta), 'init_stiff_mat_ale')
+  end subroutine init_stiff_mat_ale
+
+  subroutine ssh_solve_preconditioner(solverinfo)
+    type(t_gpu_solverinfo), intent(inout) :: solverinfo
+    call gpu_check(fgpu_ssh_solve_preconditioner(ctx, int(solverinfo%precond_variant, c_int)), &
+         'ssh_solve_preconditioner')
+    solverinfo%pr_ready = .true.
+  end subroutine ssh_solve_preconditioner
+
+  ! x(nod2D) in/out (warm start); counters and rate-limited warnings as in
+  ! solver.F90:319-380. CG never stops the run.
+  subroutine ssh_solve_cg(x, rhs, solverinfo)
+    real(WP), intent(inout) :: x(:)
+    real(WP), intent(in) :: rhs(:)
+    type(t_gpu_solverinfo), intent(inout) :: solverinfo
+    type(fgpu_cg_stats) :: st
+    call gpu_check(fgpu_ssh_solve_cg(ctx, x, rhs, solverinfo%soltol, int(solverinfo%maxiter, c_int), st), &
+         'ssh_solve_cg')
+    if (st%nnegrz > 0 .and. solverinfo%nnegrz == 0) &
+         write(*, *) 'WARNING: ssh CG r.z < 0 -- preconditioner is not symmetric positive definite'
+    solverinfo%nnegrz = solverinfo%nnegrz + st%nnegrz
+    if (st%nbreakdown > 0) then
+       solverinfo%nbreakdown = solverinfo%nbreakdown + st%nbreakdown
+       if (solverinfo%nbreakdown <= 5) write(*, *) 'WARNING: ssh CG breakdown: previous r.z is zero'
+    end if
+    solverinfo%iters_last = st%iters_last
+    solverinfo%iters_max = max(solverinfo%iters_max, st%iters_last)
+    solverinfo%iters_sum = solverinfo%iters_sum + st%iters_last
+    solverinfo%nsolves = solverinfo%nsolves + 1
+    solverinfo%resid_last = st%resid_last
+    solverinfo%rtol_last = st%rtol_last
+    solverinfo%ms_last = st%ms
+    if (st%converged == 0) then
+       solverinfo%nonconv = solverinfo%nonconv + 1
+       if (solverinfo%nonconv <= 5 .or. mod(solverinfo%nonconv, 100) == 0) &
+            write(*, *) 'WARNING: ssh CG did not converge in ', solverinfo%maxiter, ' iters; rms(resid)=', &
+            solverinfo%resid_last, ' target rtol=', solverinfo%rtol_last, ' (occurrence ', solverinfo%nonconv, ')'
+    end if
+  end subroutine ssh_solve_cg
+
+  ! lazy preconditioner build, then the CG solve of d_eta (oce_ale.F90:3094-3097);
+  ! the halo exchange is the one-partition no-op
+  subroutine solve_ssh_ale(d_eta, ssh_rhs, solverinfo)
+    real(WP), intent(inout) :: d_eta(:)
+    real(WP), intent(in) :: ssh_rhs(:)
+    type(t_gpu_solverinfo), intent(inout) :: solverinfo
+    if (.not. solverinfo%pr_ready) call ssh_solve_preconditioner(solverinfo)
+    call ssh_solve_cg(d_eta, ssh_rhs, solverinfo)
+  end subroutine solve_ssh_ale
+
+end module fesom_gpu_host

@@ -1,0 +1,84 @@
+"""Fortran host side (fesom2_amd/fortran): the ISO_C_BINDING modules and the case
+driver built with flang, linked against libfesom_gpu.so.
+
+CPU: the driver resolves the library and reads a case file written by
+tests/fortran_case.py with the sizes and sums numpy sees. GPU: the driver runs
+the reference-shaped per-tracer loop (init_tracers_AB, do_oce_adv_tra,
+diff_tracers_ale) for one step and the batched solve_tracers_ale for two more;
+the tracers must equal three oracle steps bit for bit, and solve_ssh_ale must
+reach the 1e-10 residual.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fesom2_amd import build as fbuild
+from fesom2_amd import meshgen as mg
+from tests import fortran_case as fc
+
+DRIVER = fbuild.FORTRAN_DRIVER
+
+
+def _need_driver():
+    if not os.path.exists(DRIVER):
+        if fbuild.flang() is None:
+            pytest.skip("no Fortran compiler in this image: the driver is not built")
+        pytest.fail("Fortran driver not built: run __graft_entry__.build()")
+
+
+def test_driver_links_library():
+    _need_driver()
+    r = subprocess.run([DRIVER, "--abi"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "fesom_gpu library version 1" in r.stdout
+
+
+def test_case_file_read_by_fortran(c0, tmp_path):
+    _need_driver()
+    mesh, state, _ = c0
+    rhs = mg.ssh_rhs(mesh)
+    path = str(tmp_path / "case.bin")
+    fc.write_case(path, mesh, state, rhs)
+    r = subprocess.run([DRIVER, "--read", path], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
+    si = [int(x) for x in out["sums_int"]]
+    assert si[0] == int((mesh.edges + 1).sum())
+    assert si[1] == int(np.where(mesh.nod_in_elem2D >= 0, mesh.nod_in_elem2D + 1, 0).sum())
+    assert si[2] == int(state.ids.sum())
+    sr = [float(x) for x in out["sums_real"]]
+    np.testing.assert_allclose(sr, [state.values.sum(), state.uv.sum(), rhs.sum()], rtol=1e-12)
+    last = [float(x) for x in out["last"]]
+    assert last == [float(mesh.gradient_sca[-1, 5]), float(state.Kv[-1, -1])]
+
+
+@pytest.mark.gpu
+def test_fortran_host_c0(c0, c0_oracle, tmp_path):
+    _need_driver()
+    import scipy.sparse as sp
+    mesh, state, _ = c0
+    rhs = mg.ssh_rhs(mesh)
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "result.bin")
+    fc.write_case(case, mesh, state, rhs, nsingle=1, nbatched=2, precond_variant=1)
+    print("[gpu] fortran driver c0: 1 per-tracer + 2 batched steps, solve_ssh_ale", flush=True)
+    r = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = fc.read_result(res, mesh, state.ntr)
+    v = state.values.copy()
+    vo = state.valuesold.copy()
+    for _ in range(3):
+        c0_oracle.step(state, v, vo)
+    L = mesh.nl - 1
+    k = np.arange(1, L + 1)
+    wet = (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+    for i in range(state.ntr):
+        np.testing.assert_array_equal(out["values"][i][wet], v[i][wet])
+        np.testing.assert_array_equal(out["valuesold"][i][wet], vo[i][wet])
+        assert abs(out["content_after"][i] - out["content_before"][i]) < 1e-13 * abs(out["content_before"][i])
+    assert out["converged"] and out["nbreakdown"] == 0 and out["nnegrz"] == 0
+    orp, oci, ov = c0_oracle.stiff_matrix(state.dt)
+    A = sp.csr_matrix((ov.copy(), oci.copy(), orp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    assert np.linalg.norm(rhs - A @ out["d_eta"]) / np.linalg.norm(rhs) < 1e-10 * 1.01
