@@ -81,11 +81,14 @@ def step_model_bytes(mesh, ntr):
 
 
 def cpu_baseline(mesh, state, budget_s=12.0):
-    """Oracle (single-threaded C port) on whole steps of the same workload."""
+    """Oracle (single-threaded C port) on whole steps of the same mesh and state.
+    The sample keeps as many of the tracers as about budget_s of one core allows
+    (at roughly 1e7 tracer-updates/s), at least one, and runs whole steps."""
     from oracle.oracle import Oracle
     orc = Oracle(mesh)
-    v = state.values.copy()
-    vo = state.valuesold.copy()
+    k = max(1, min(state.ntr, int(budget_s * 1.0e7 / max(mesh.n3, 1))))
+    v = state.values[:k].copy()
+    vo = state.valuesold[:k].copy()
     t0 = time.perf_counter()
     n = 0
     while True:
@@ -94,9 +97,10 @@ def cpu_baseline(mesh, state, budget_s=12.0):
         el = time.perf_counter() - t0
         if el > budget_s or n >= 50:
             break
-    upd = state.ntr * mesh.n3 * n / el
+    upd = k * mesh.n3 * n / el
+    which = f"all {state.ntr} tracers" if k == state.ntr else f"{k} of the {state.ntr} tracers"
     return {"value": upd, "unit": "tracer-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full steps of the same workload (all {state.ntr} tracers), "
+            "sample": f"{n} full steps of the same mesh and state ({which}), "
                       f"{el:.1f} s, C oracle built with gcc -O2, one thread"}
 
 

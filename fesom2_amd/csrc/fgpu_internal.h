@@ -136,6 +136,8 @@ struct fgpu_context {
     double *ssh_out = nullptr;    // device stats
     int ssh_blocks = 0;
     std::vector<int> h_ssh_rowptr, h_ssh_colind;
+    // learned preconditioner (fgpu_mpnn.hip)
+    struct NNDev *nn = nullptr;
 };
 
 // error helpers (fgpu_runtime.hip)
@@ -165,3 +167,11 @@ int fgpu_prof_flush(fgpu_context *c);
 int fgpu_enqueue_fused_step(fgpu_context *c, double dt);
 int fgpu_compute_zn(fgpu_context *c);
 int fgpu_ensure_staged(fgpu_context *c, int slots);
+
+// SSH solver pieces shared with the learned preconditioner
+struct FgpuCgSummary {
+    int iters, done, nbreak, nneg;   // done: 1 converged, 2 breakdown, 3 iteration cap
+    double rr_last, rtol;
+};
+int fgpu_ssh_cg_from_x(fgpu_context *c, double soltol, int maxiter, int variant, FgpuCgSummary *out);
+void fgpu_nn_release(fgpu_context *c);

@@ -216,6 +216,7 @@ extern "C" int fgpu_destroy(fgpu_context *c)
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->ssh_graph) (void)hipGraphExecDestroy(c->ssh_graph);
+    fgpu_nn_release(c);
     free_list(c->tracer_allocs);
     free_list(c->allocs);
     (void)hipStreamDestroy(c->stream);

@@ -507,9 +507,10 @@ static int cg_graph(fgpu_context *c, const CGArgs &A)
 // One solve from the x currently in ssh_x. Replays the iteration graph in
 // batches sized by the previous solve's iteration count and polls the device
 // flag between batches; kernels after convergence return immediately.
-static int launch_cg(fgpu_context *c, double soltol, int maxiter)
+static int launch_cg(fgpu_context *c, double soltol, int maxiter, int variant)
 {
     CGArgs A = cg_args(c, soltol, maxiter);
+    A.variant = variant;
     int rc = cg_graph(c, A);
     if (rc) return rc;
     const dim3 g(A.nb), b(SBLK);
@@ -531,6 +532,23 @@ static int launch_cg(fgpu_context *c, double soltol, int maxiter)
     }
     if (!hs.done) return fgpu_fail(FGPU_ERR_STATE, "CG iteration graph did not terminate");
     c->ssh_last_iters = hs.iter;
+    return FGPU_OK;
+}
+
+// plain CG from the current ssh_x with the given preconditioner variant (the
+// fallback of the learned solve); summary of the device state afterwards
+int fgpu_ssh_cg_from_x(fgpu_context *c, double soltol, int maxiter, int variant, FgpuCgSummary *out)
+{
+    int rc = launch_cg(c, soltol, maxiter, variant);
+    if (rc) return rc;
+    CGState hs;
+    FGPU_HIP(hipMemcpy(&hs, c->ssh_out, sizeof(CGState), hipMemcpyDeviceToHost));
+    out->iters = hs.iter;
+    out->done = hs.done;
+    out->nbreak = hs.nbreak;
+    out->nneg = hs.nneg;
+    out->rr_last = hs.rr_last;
+    out->rtol = hs.rtol;
     return FGPU_OK;
 }
 
@@ -573,7 +591,7 @@ extern "C" int fgpu_ssh_solve_cg(fgpu_context *c, double *x, const double *rhs, 
     FGPU_HIP(hipEventCreate(&e0));
     FGPU_HIP(hipEventCreate(&e1));
     FGPU_HIP(hipEventRecord(e0, c->stream));
-    rc = launch_cg(c, soltol, maxiter);
+    rc = launch_cg(c, soltol, maxiter, c->ssh_variant);
     FGPU_HIP(hipEventRecord(e1, c->stream));
     FGPU_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -610,7 +628,7 @@ extern "C" int fgpu_ssh_solve_resident(fgpu_context *c, double soltol, int maxit
                            c->ssh_n);
         FGPU_HIP(hipGetLastError());
         FGPU_HIP(hipEventRecord(e0, c->stream));
-        rc = launch_cg(c, soltol, maxiter);
+        rc = launch_cg(c, soltol, maxiter, c->ssh_variant);
         FGPU_HIP(hipEventRecord(e1, c->stream));
         FGPU_HIP(hipEventSynchronize(e1));
         float ms = 0.f;

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfesom_gpu.so")
 
 dp = C.POINTER(C.c_double)
+fp = C.POINTER(C.c_float)
 ip = C.POINTER(C.c_int)
 vp = C.c_void_p
 
@@ -73,6 +74,15 @@ _SIGS = {
     "fgpu_ssh_upload": [vp, dp, dp],
     "fgpu_ssh_solve_resident": [vp, C.c_double, C.c_int, C.c_int, C.POINTER(CgStats)],
     "fgpu_ssh_download": [vp, dp],
+    "fgpu_ssh_nn_param_count": [],
+    "fgpu_ssh_nn_init": [vp, C.c_ulonglong],
+    "fgpu_ssh_nn_set_weights": [vp, fp, C.c_int],
+    "fgpu_ssh_nn_get_weights": [vp, fp, C.c_int],
+    "fgpu_ssh_nn_apply": [vp, dp, dp],
+    "fgpu_ssh_nn_loss_grad": [vp, dp, dp, C.c_int, dp, fp],
+    "fgpu_ssh_nn_train": [vp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double, dp],
+    "fgpu_ssh_solve_learned": [vp, dp, dp, C.c_double, C.c_int, C.c_int, C.POINTER(CgStats), ip],
+    "fgpu_ssh_solve_learned_resident": [vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(CgStats), ip],
 }
 EXPORTED = sorted(list(_SIGS) + ["fgpu_last_error", "fgpu_version"])
 
@@ -307,6 +317,63 @@ class Context:
         x = np.zeros(self.N2)
         _check(self.L.fgpu_ssh_download(self.h, _d(x)))
         return x
+
+    # ------------------------------------------------------------ learned preconditioner
+    def nn_param_count(self):
+        return _check(self.L.fgpu_ssh_nn_param_count())
+
+    def nn_init(self, seed=0):
+        _check(self.L.fgpu_ssh_nn_init(self.h, int(seed)))
+
+    def nn_set_weights(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        _check(self.L.fgpu_ssh_nn_set_weights(self.h, th.ctypes.data_as(fp), int(th.size)))
+
+    def nn_get_weights(self):
+        th = np.zeros(self.nn_param_count(), np.float32)
+        _check(self.L.fgpu_ssh_nn_get_weights(self.h, th.ctypes.data_as(fp), int(th.size)))
+        return th
+
+    def nn_apply(self, r):
+        z = np.zeros(self.N2)
+        _check(self.L.fgpu_ssh_nn_apply(self.h, _d(_f64(r)), _d(z)))
+        return z
+
+    def nn_loss_grad(self, bs, xs):
+        b = _f64(np.asarray(bs))
+        x = _f64(np.asarray(xs))
+        loss = C.c_double()
+        g = np.zeros(self.nn_param_count(), np.float32)
+        _check(self.L.fgpu_ssh_nn_loss_grad(self.h, _d(b), _d(x), int(b.shape[0]), C.byref(loss),
+                                            g.ctypes.data_as(fp)))
+        return loss.value, g
+
+    def nn_train(self, bs, xs, epochs=10, batch=8, lr=1e-3):
+        b = _f64(np.asarray(bs))
+        x = _f64(np.asarray(xs))
+        hist = np.zeros(int(epochs))
+        _check(self.L.fgpu_ssh_nn_train(self.h, _d(b), _d(x), int(b.shape[0]), int(epochs), int(batch),
+                                        float(lr), _d(hist)))
+        return hist
+
+    def ssh_solve_learned(self, x, rhs, soltol=1e-10, maxiter=500, fallback_maxiter=2000):
+        x = _f64(x).copy()
+        st = CgStats()
+        fb = C.c_int()
+        _check(self.L.fgpu_ssh_solve_learned(self.h, _d(x), _d(_f64(rhs)), float(soltol), int(maxiter),
+                                             int(fallback_maxiter), C.byref(st), C.byref(fb)))
+        d = st.as_dict()
+        d["fallback_iters"] = fb.value
+        return x, d
+
+    def ssh_solve_learned_resident(self, soltol=1e-10, maxiter=500, fallback_maxiter=2000, nrep=1):
+        st = CgStats()
+        fb = C.c_int()
+        _check(self.L.fgpu_ssh_solve_learned_resident(self.h, float(soltol), int(maxiter), int(fallback_maxiter),
+                                                      int(nrep), C.byref(st), C.byref(fb)))
+        d = st.as_dict()
+        d["fallback_iters"] = fb.value
+        return d
 
 
 def context_for(mesh, state=None, device=0, schemes=None):

@@ -1,0 +1,53 @@
+"""Training data and training loop of the learned SSH preconditioner.
+
+The network, its kernels and the flexible CG live in the library
+(fgpu_ssh_nn_* / fgpu_ssh_solve_learned); this module only prepares right-hand
+sides, solves them with the plain diagonally preconditioned CG on the device
+(the pairs the network learns from) and calls the device training loop.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import meshgen as mg
+
+
+def rhs_samples(mesh, n, seed=0):
+    """Seeded SSH right-hand sides: area-weighted smooth fields of a few scales
+    plus white noise, the mix the divergence of a transport field produces."""
+    rng = np.random.default_rng(seed + 7000)
+    x = mesh.coord_nod2D[:, 0]
+    y = mesh.coord_nod2D[:, 1]
+    L = mesh.cyclic_length if mesh.cyclic_length > 0 else max(np.ptp(x), 1e-12)
+    Ly = max(np.ptp(y), 1e-12)
+    y0 = y.min()
+    area = mesh.areasvol[:, 0] / np.mean(mesh.areasvol[:, 0])
+    out = np.empty((n, mesh.nod2D))
+    for s in range(n):
+        kind = s % 3
+        if kind == 0:
+            f = rng.normal(0.0, 1.0, mesh.nod2D)
+        else:
+            kmax = 4 if kind == 1 else 16
+            f = mg._smooth_field(x, y - y0, L, Ly, rng, nmodes=24, kmax=kmax)
+            f = f / max(np.std(f), 1e-30) + 0.1 * rng.normal(0.0, 1.0, mesh.nod2D)
+        out[s] = f * area
+    return out
+
+
+def solve_pairs(ctx, rhs, soltol=1e-12, maxiter=20000):
+    """Solutions of A x = b with the plain diagonally preconditioned CG (device)."""
+    ctx.ssh_solve_preconditioner(2)
+    xs = np.empty_like(rhs)
+    for s in range(rhs.shape[0]):
+        x, st = ctx.ssh_solve_cg(np.zeros(rhs.shape[1]), rhs[s], soltol=soltol, maxiter=maxiter)
+        if not st["converged"]:
+            raise RuntimeError(f"plain CG did not converge on training sample {s}: {st}")
+        xs[s] = x
+    return xs
+
+
+def train(ctx, rhs, sol, epochs=20, batch=8, lr=2e-3, seed=0):
+    """Initialise the network and train it on the device; returns the loss per epoch."""
+    ctx.nn_init(seed)
+    return ctx.nn_train(rhs, sol, epochs=epochs, batch=batch, lr=lr)

@@ -157,6 +157,41 @@ int fgpu_ssh_solve_resident(fgpu_context *ctx, double soltol, int maxiter, int n
                             fgpu_cg_stats *stats);
 int fgpu_ssh_download(fgpu_context *ctx, double *x);
 
+/* ---- learned SSH preconditioner -------------------------------------------
+ * No counterpart in the reference: an optional replacement of the diagonal /
+ * sparse M^-1 of ssh_solve_cg (solver.F90:120-385) by a message-passing network
+ * on the SSH matrix graph (3 layers, width 64, bf16 weights and activations
+ * with FP32 accumulation) inside flexible CG (FCG(1)). Vectors, residual and
+ * dot products stay FP64; the stopping test is the reference's
+ * sqrt(r.r/nod2D) < soltol*sqrt(b.b/nod2D). A solve that breaks down or reaches
+ * maxiter continues with plain diagonally preconditioned CG from its x for up to
+ * fallback_maxiter iterations. All calls need the SSH matrix (and, for the
+ * solves, fgpu_ssh_solve_preconditioner) first.
+ * Parameters: FP32 vector of fgpu_ssh_nn_param_count() entries, in the order
+ * W0(2,64) | W_l(128,64) b_l(64), l = 1..3 | w_out(64) b_out(1), row-major. */
+int fgpu_ssh_nn_param_count(void);
+/* He-initialised hidden layers, zero output layer (= the diagonal preconditioner) */
+int fgpu_ssh_nn_init(fgpu_context *ctx, unsigned long long seed);
+int fgpu_ssh_nn_set_weights(fgpu_context *ctx, const float *theta, int np);
+int fgpu_ssh_nn_get_weights(fgpu_context *ctx, float *theta, int np);
+/* z = M(r), nod2D entries each */
+int fgpu_ssh_nn_apply(fgpu_context *ctx, const double *r, double *z);
+/* Training loss mean_s (M(b_s) - x_s)^T A (M(b_s) - x_s) / (x_s^T A x_s) over nsamp
+ * right-hand sides b (nsamp x nod2D) with solutions x, and its gradient (FP32, np entries). */
+int fgpu_ssh_nn_loss_grad(fgpu_context *ctx, const double *b, const double *x, int nsamp, double *loss,
+                          float *grad);
+/* Adam (lr, beta 0.9/0.999) over mini-batches of `batch` samples for `epochs`
+ * passes; loss_per_epoch[epochs] receives the mean loss of each pass. */
+int fgpu_ssh_nn_train(fgpu_context *ctx, const double *b, const double *x, int nsamp, int epochs, int batch,
+                      double lr, double *loss_per_epoch);
+/* learned FCG solve; stats->iters_last counts FCG plus fallback iterations,
+ * *fallback_iters the fallback's share */
+int fgpu_ssh_solve_learned(fgpu_context *ctx, double *x, const double *rhs, double soltol, int maxiter,
+                           int fallback_maxiter, fgpu_cg_stats *stats, int *fallback_iters);
+/* device-resident repeated solves from the x0/rhs of fgpu_ssh_upload (timing) */
+int fgpu_ssh_solve_learned_resident(fgpu_context *ctx, double soltol, int maxiter, int fallback_maxiter, int nrep,
+                                    fgpu_cg_stats *stats, int *fallback_iters);
+
 #ifdef __cplusplus
 }
 #endif
