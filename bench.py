@@ -126,6 +126,8 @@ def main():
           f"N3={mesh.n3} ntr={state.ntr} (generated in {t_gen:.1f}s)", file=sys.stderr, flush=True)
     ctx = gpu.context_for(mesh, state)
     dt = state.dt
+    tpb, tune_ms = ctx.autotune_step(dt, 3)
+    print(f"[bench] autotuner: {tpb} tracers per block ({tune_ms:.3f} ms/step)", file=sys.stderr, flush=True)
     ctx.solve_tracers_ale(dt, max(1, args.warmup))
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -192,6 +194,7 @@ def main():
         "ssh_solve_ms": round(sst["ms"], 3), "ssh_iters": sst["iters"], "ssh_converged": sst["converged"],
         "ssh_precond": "diagonal (Jacobi)",
         "device_ms_per_step": ms_dev / args.steps,
+        "autotuned_tracers_per_block": tpb,
         "step_alg_GBs": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
         "step_model_bytes": int(step_model_bytes(mesh, state.ntr)),
         "step_model_frac_of_hbm": round(step_model_bytes(mesh, state.ntr) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),

@@ -45,11 +45,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // partial-sum slots (double, nbd blocks each)
-enum { SL_BB = 0, SL_RR, SL_RTRT, SL_ZR, SL_ZQ, SL_PQ, SL_XAX, SL_EAE, NSLOT };
+enum { SL_BB = 0, SL_RR, SL_RTRT, SL_ZR, SL_ZQ, SL_PQ, SL_XAX, SL_EAE, SL_JR, SL_JQ, NSLOT };
 
 struct FCGState {
-    int iter, done, nbreak, nneg;
-    double rtol, rr_last, s;
+    int iter, done, nbreak, nneg;   // nneg: iterations that fell back to the diagonal direction
+    double rtol, rr_last, s, zr;
 };
 
 struct NNDev {
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(NNT) k_nn_out(NNArgs A, const double *rin, dou
     const bf16_t *H3 = A.H + (size_t)3 * A.n * NNC;
     const float *wo = A.theta + OFF_WOUT;
     const float bo = A.theta[OFF_BOUT];
-    double zr = 0.0, zq = 0.0, d2 = 0.0;
+    double zr = 0.0, zq = 0.0, jr = 0.0, jq = 0.0, d2 = 0.0;
     for (int i = blockIdx.x * NNT + threadIdx.x; i < A.n; i += A.nbd * NNT) {
         float yv = 0.f;
         for (int c8 = 0; c8 < NNC; c8 += 8) {
@@ -291,9 +291,17 @@ __global__ void __launch_bounds__(NNT) k_nn_out(NNArgs A, const double *rin, dou
         const double z = A.isq[i] * (rt + s * (double)yv);
         zout[i] = z;
         zr += z * rin[i];
-        if (q) zq += z * q[i];
+        const double jz = A.isq[i] * A.isq[i] * rin[i];   // the diagonal preconditioner's z
+        jr += jz * rin[i];
+        if (q) {
+            zq += z * q[i];
+            jq += jz * q[i];
+        }
     }
-    if (A.st) nn_put3(A, SL_ZR, zr, SL_ZQ, zq, -1, d2);
+    if (A.st) {
+        nn_put3(A, SL_ZR, zr, SL_ZQ, zq, SL_JR, jr);
+        nn_put3(A, SL_JQ, jq, -1, 0.0, -1, d2);
+    }
 }
 
 // ------------------------------------------------------------------------------
@@ -597,18 +605,29 @@ __global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F)
     }
 }
 
-// p = z + beta p, beta = -(z.q)/(p.q) of the previous direction (p = z first)
+// p = z + beta p, beta = -(z.q)/(p.q) of the previous direction (p = z first).
+// If the network's z is not a descent direction (z.r <= 0), this iteration
+// takes the diagonal preconditioner's z = D^-1 r instead; flexible CG admits a
+// preconditioner that changes from one iteration to the next.
 __global__ void __launch_bounds__(NNT) k_fcg_p(FCGArgs F)
 {
-    if (F.st->done) return;
-    const bool first = F.st->iter == 0;
-    double be = 0.0;
-    if (!first) {
-        const double zq = fcg_slot(F, SL_ZQ), pq = fcg_slot(F, SL_PQ);
-        be = -zq / pq;
+    FCGState *st = F.st;
+    if (st->done) return;
+    const bool first = st->iter == 0;
+    double zr = fcg_slot(F, SL_ZR), zq = fcg_slot(F, SL_ZQ);
+    const double jr = fcg_slot(F, SL_JR), jq = fcg_slot(F, SL_JQ);
+    const bool jac = !(zr > 0.0);
+    if (jac) { zr = jr; zq = jq; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->zr = zr;
+        if (jac) st->nneg += 1;
     }
-    for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT)
-        F.p[i] = first ? F.z[i] : F.z[i] + be * F.p[i];
+    double be = 0.0;
+    if (!first) be = -zq / fcg_slot(F, SL_PQ);
+    for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT) {
+        const double zi = jac ? F.isq[i] * F.isq[i] * F.r[i] : F.z[i];
+        F.p[i] = first ? zi : zi + be * F.p[i];
+    }
 }
 
 // q = A p, partial p.q
@@ -630,11 +649,10 @@ __global__ void __launch_bounds__(NNT) k_fcg_b(FCGArgs F)
 {
     FCGState *st = F.st;
     if (st->done) return;
-    const double zr = fcg_slot(F, SL_ZR), pq = fcg_slot(F, SL_PQ);
+    const double zr = st->zr, pq = fcg_slot(F, SL_PQ);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (!(pq > 0.0) || !(zr > 0.0)) {   // breakdown or a non-positive preconditioner value
+    if (!(pq > 0.0) || !(zr > 0.0)) {   // breakdown: continue with plain CG
         if (lead) {
-            if (!(zr > 0.0)) st->nneg += 1;
             st->nbreak += 1;
             st->done = 2;
         }

@@ -897,6 +897,70 @@ extern "C" int fgpu_solve_tracers_ale_timed(fgpu_context *c, double dt, int nste
     return FGPU_OK;
 }
 
+// Autotuner of the fused step's launch shape: each candidate number of tracers
+// per block (1, 2, 4, ... and all) is timed over `nsteps` replayed steps and the
+// fastest is kept. Every candidate computes the same arithmetic per column, so
+// results do not depend on the choice. The tracer state is saved before and
+// restored after the trials.
+extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *best_tpb, double *best_ms)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_step(c))) return rc;
+    if (nsteps < 1) return fgpu_fail(FGPU_ERR_ARG, "nsteps < 1");
+    if (!c->fused) return fgpu_fail(FGPU_ERR_STATE, "the autotuner tunes the fused step (FGPU_STEP=staged is set)");
+    const int nt = c->t.ntr;
+    const size_t nT = c->t.sN * (size_t)nt, nO = c->t.sOld * (size_t)nt;
+    double *save = nullptr;
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    FGPU_HIP(hipMalloc((void **)&save, sizeof(double) * (nT + nO)));
+    auto restore = [&]() -> int {
+        FGPU_HIP(hipMemcpyAsync(c->t.T, save, sizeof(double) * nT, hipMemcpyDeviceToDevice, c->stream));
+        FGPU_HIP(hipMemcpyAsync(c->t.Told, save + nT, sizeof(double) * nO, hipMemcpyDeviceToDevice, c->stream));
+        FGPU_HIP(hipStreamSynchronize(c->stream));
+        return FGPU_OK;
+    };
+    FGPU_HIP(hipMemcpyAsync(save, c->t.T, sizeof(double) * nT, hipMemcpyDeviceToDevice, c->stream));
+    FGPU_HIP(hipMemcpyAsync(save + nT, c->t.Told, sizeof(double) * nO, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<int> cand;
+    for (int v = 1; v < nt; v *= 2) cand.push_back(v);
+    cand.push_back(nt);
+    const bool was_prof = c->prof;
+    c->prof = false;
+    int best = c->step_tpb;
+    float best_t = -1.f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = FGPU_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fgpu_fail(FGPU_ERR_HIP, "hipEventCreate failed");
+    for (size_t i = 0; i < cand.size() && !rc; ++i) {
+        c->step_tpb = cand[i];
+        c->graph_valid = false;
+        if ((rc = run_steps(c, dt, 1))) break;   // capture and warm up
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord"); break; }
+        if ((rc = run_steps(c, dt, nsteps))) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fgpu_fail(FGPU_ERR_HIP, "timing the autotuner trial failed");
+            break;
+        }
+        if (best_t < 0.f || ms < best_t) { best_t = ms; best = cand[i]; }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const int rr = restore();
+    (void)hipFree(save);
+    c->prof = was_prof;
+    c->step_tpb = best;
+    c->graph_valid = false;
+    if (rc) return rc;
+    if (rr) return rr;
+    if (best_tpb) *best_tpb = best;
+    if (best_ms) *best_ms = best_t / nsteps;
+    return FGPU_OK;
+}
+
 extern "C" int fgpu_tracer_content(fgpu_context *c, int tr_num, double *content)
 {
     int rc = check_ready(c);

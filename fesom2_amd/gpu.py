@@ -66,6 +66,7 @@ _SIGS = {
     "fgpu_profile_count": [vp],
     "fgpu_profile_read": [vp, C.c_int, C.c_char_p, C.c_int, dp, ip],
     "fgpu_tracer_content": [vp, C.c_int, dp],
+    "fgpu_autotune_step": [vp, C.c_double, C.c_int, ip, dp],
     "fgpu_init_stiff_mat_ale": [vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fgpu_ssh_set_matrix": [vp, C.c_int, C.c_int, ip, ip, dp],
     "fgpu_ssh_get_matrix": [vp, ip, ip, dp, dp, ip],
@@ -267,6 +268,13 @@ class Context:
             _check(self.L.fgpu_profile_read(self.h, i, name, 64, C.byref(ms), C.byref(cnt)))
             out[name.value.decode()] = (ms.value, cnt.value)
         return out
+
+    def autotune_step(self, dt, nsteps=5):
+        """Pick the fastest launch shape of the fused step; returns (tracers per block, ms/step)."""
+        tpb = C.c_int()
+        ms = C.c_double()
+        _check(self.L.fgpu_autotune_step(self.h, float(dt), int(nsteps), C.byref(tpb), C.byref(ms)))
+        return tpb.value, ms.value
 
     def tracer_content(self, tr_num):
         v = C.c_double()

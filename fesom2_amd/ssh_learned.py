@@ -47,6 +47,29 @@ def solve_pairs(ctx, rhs, soltol=1e-12, maxiter=20000):
     return xs
 
 
+def residual_pairs(ctx, rhs, sol, ks=(0, 10, 30, 60, 100)):
+    """Residual / error pairs along plain CG runs: after k iterations from x0 = 0,
+    r_k = b - A x_k is a right-hand side whose solution is e_k = x - x_k. These
+    are the vectors a preconditioner meets inside CG."""
+    import scipy.sparse as sp
+    rp, ci, v, _ = ctx.ssh_get_matrix()
+    A = sp.csr_matrix((v, ci - 1, rp - 1), shape=(len(rp) - 1, len(rp) - 1))
+    ctx.ssh_solve_preconditioner(2)
+    rs, es = [], []
+    for s in range(rhs.shape[0]):
+        for k in ks:
+            if k == 0:
+                xk = np.zeros(rhs.shape[1])
+            else:
+                xk, _ = ctx.ssh_solve_cg(np.zeros(rhs.shape[1]), rhs[s], soltol=1e-14, maxiter=k)
+            e = sol[s] - xk
+            if not np.any(e):
+                continue
+            rs.append(rhs[s] - A @ xk)
+            es.append(e)
+    return np.stack(rs), np.stack(es)
+
+
 def train(ctx, rhs, sol, epochs=20, batch=8, lr=2e-3, seed=0):
     """Initialise the network and train it on the device; returns the loss per epoch."""
     ctx.nn_init(seed)

@@ -131,6 +131,11 @@ int fgpu_solve_tracers_ale_timed(fgpu_context *ctx, double dt, int nsteps, doubl
 int fgpu_profile_enable(fgpu_context *ctx, int on);
 int fgpu_profile_count(fgpu_context *ctx);
 int fgpu_profile_read(fgpu_context *ctx, int idx, char *name, int name_len, double *total_ms, int *launches);
+/* Autotuner of the fused step (no counterpart in the reference): times the
+ * candidate launch shapes (tracers per block 1, 2, 4, ..., all) over nsteps
+ * replayed steps, keeps the fastest for this context and reports it. The tracer
+ * state is restored afterwards; results do not depend on the choice. */
+int fgpu_autotune_step(fgpu_context *ctx, double dt, int nsteps, int *best_tpb, double *best_ms_per_step);
 /* Tracer content sum_n sum_k areasvol*hnode_new*T (deterministic device sum). */
 int fgpu_tracer_content(fgpu_context *ctx, int tr_num, double *content);
 
@@ -185,7 +190,9 @@ int fgpu_ssh_nn_loss_grad(fgpu_context *ctx, const double *b, const double *x, i
 int fgpu_ssh_nn_train(fgpu_context *ctx, const double *b, const double *x, int nsamp, int epochs, int batch,
                       double lr, double *loss_per_epoch);
 /* learned FCG solve; stats->iters_last counts FCG plus fallback iterations,
- * *fallback_iters the fallback's share */
+ * *fallback_iters the fallback's share; stats->nnegrz counts the iterations in
+ * which the network's z had z.r <= 0 and the diagonal preconditioner's z was
+ * used instead */
 int fgpu_ssh_solve_learned(fgpu_context *ctx, double *x, const double *rhs, double soltol, int maxiter,
                            int fallback_maxiter, fgpu_cg_stats *stats, int *fallback_iters);
 /* device-resident repeated solves from the x0/rhs of fgpu_ssh_upload (timing) */

@@ -123,6 +123,28 @@ def test_batched_step_c0(c0, c0_oracle, nsteps):
     ctx.close()
 
 
+def test_autotuned_step_c0(c0, c0_oracle):
+    """The autotuner restores the state it times on, and the tuned step stays
+    bit-identical to the oracle."""
+    mesh, state, _ = c0
+    ctx = _gpu().context_for(mesh, state)
+    print("[gpu] autotune c0", flush=True)
+    tpb, ms = ctx.autotune_step(state.dt, 3)
+    assert 1 <= tpb <= state.ntr and ms > 0.0
+    gv0, _ = ctx.tracer_download(1)
+    wet = _wet(mesh)
+    np.testing.assert_array_equal(gv0[wet], state.values[0][wet])
+    ctx.solve_tracers_ale(state.dt, 2)
+    v = state.values.copy()
+    vo = state.valuesold.copy()
+    for _ in range(2):
+        c0_oracle.step(state, v, vo)
+    for i in range(state.ntr):
+        gv, _ = ctx.tracer_download(i + 1)
+        np.testing.assert_array_equal(gv[wet], v[i][wet])
+    ctx.close()
+
+
 def test_scheme_variants_c0(c0, c0_oracle):
     """UPW1/UPW1/FCT, MFCT(ph=0.5)/QR4C(pv=0)/NON and AB3 against the oracle."""
     mesh, state, _ = c0
