@@ -33,6 +33,8 @@ struct DevMesh {
     const int *nlevn, *ulevn, *nlevn_min, *ulevn_max;  // N2
     const int *ne_ptr;       // N2+1 node->edge gather lists
     const int *ne_idx;       // 2*e + side (side 0: node is edges(1,e)), edges ascending
+    const int *ne_rec;       // 8 per ne_idx entry: q, other node, nu12, nl12, nl1, nu1, nl2, nu2
+    const int *node_rec;     // 4 per node: ne_ptr, degree, ulevels_nod2D, nlevels_nod2D
     const double *elem_area, *elem_cos, *edge_dxdy, *ecd, *gsca;
     const double *area, *areasvol;        // N2*NL
     const double *hnode, *hnode_new;      // N2*L

@@ -419,6 +419,28 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
     UP(p_i, ulevels_nod2D_max, N2); m.ulevn_max = p_i;
     UP(p_i, ne_ptr.data(), ne_ptr.size()); m.ne_ptr = p_i;
     UP(p_i, ne_idx.data(), ne_idx.size()); m.ne_idx = p_i;
+    {
+        // per-node gather records: the node's edge list with each edge's level
+        // ranges, so a node kernel reaches them with one dependent load
+        std::vector<int> rec(8 * ne_idx.size()), nrec(4 * (size_t)N2);
+        for (size_t p = 0; p < ne_idx.size(); ++p) {
+            const int q = ne_idx[p], e = q >> 1;
+            int *r = rec.data() + 8 * p;
+            r[0] = q;
+            r[1] = ed[2 * e + (1 - (q & 1))];
+            r[2] = erange[2 * e];
+            r[3] = erange[2 * e + 1];
+            for (int j = 0; j < 4; ++j) r[4 + j] = einfo[4 * e + j];
+        }
+        for (int n = 0; n < N2; ++n) {
+            nrec[4 * n] = ne_ptr[n];
+            nrec[4 * n + 1] = ne_ptr[n + 1] - ne_ptr[n];
+            nrec[4 * n + 2] = ulevels_nod2D[n];
+            nrec[4 * n + 3] = nlevels_nod2D[n];
+        }
+        UP(p_i, rec.data(), rec.size()); m.ne_rec = p_i;
+        UP(p_i, nrec.data(), nrec.size()); m.node_rec = p_i;
+    }
     UP(p_i, kneed.data(), kneed.size()); c->d_kneed = p_i;
     {
         // packed edge records (mesh arrays and the fill_up_dn_grad level rule)

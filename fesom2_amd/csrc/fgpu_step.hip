@@ -84,21 +84,22 @@ struct NodeEdges {
 __device__ __forceinline__ NodeEdges load_node_edges(const DevMesh &m, int n)
 {
     NodeEdges E;
-    const int p0 = m.ne_ptr[n];
-    E.deg = m.ne_ptr[n + 1] - p0;
+    const int4 nr = *(const int4 *)(m.node_rec + 4 * (size_t)n);
+    const int p0 = nr.x;
+    E.deg = nr.y;
     const int j = threadIdx.x & 63;
     E.q = 0; E.other = 0; E.nu12 = 1; E.nl12 = 0; E.nu1 = 1; E.nl1 = 0; E.nu2 = 0; E.nl2 = 0;
     if (j < E.deg) {
-        const int q = m.ne_idx[p0 + j];
-        const int e = q >> 1;
-        E.q = q;
-        E.other = m.edges[2 * e + (1 - (q & 1))];
-        E.nu12 = m.erange[2 * e];
-        E.nl12 = m.erange[2 * e + 1];
-        E.nl1 = m.einfo[4 * e];
-        E.nu1 = m.einfo[4 * e + 1];
-        E.nl2 = m.einfo[4 * e + 2];
-        E.nu2 = m.einfo[4 * e + 3];
+        const int4 a = *(const int4 *)(m.ne_rec + 8 * (size_t)(p0 + j));
+        const int4 b = *(const int4 *)(m.ne_rec + 8 * (size_t)(p0 + j) + 4);
+        E.q = a.x;
+        E.other = a.y;
+        E.nu12 = a.z;
+        E.nl12 = a.w;
+        E.nl1 = b.x;
+        E.nu1 = b.y;
+        E.nl2 = b.z;
+        E.nu2 = b.w;
     }
     return E;
 }

@@ -31,7 +31,9 @@ module fesom_gpu_c
        fgpu_solve_tracers_ale_timed, fgpu_profile_enable, fgpu_profile_count, fgpu_profile_read, &
        fgpu_tracer_content, fgpu_init_stiff_mat_ale, fgpu_ssh_set_matrix, fgpu_ssh_get_matrix, &
        fgpu_ssh_solve_preconditioner, fgpu_ssh_solve_cg, fgpu_ssh_upload, fgpu_ssh_solve_resident, &
-       fgpu_ssh_download
+       fgpu_ssh_download, fgpu_autotune_step, fgpu_ssh_nn_param_count, fgpu_ssh_nn_init, &
+       fgpu_ssh_nn_set_weights, fgpu_ssh_nn_get_weights, fgpu_ssh_nn_apply, fgpu_ssh_nn_loss_grad, &
+       fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident
 
   interface
      type(c_ptr) function fgpu_last_error() bind(C, name="fgpu_last_error")
@@ -287,5 +289,87 @@ module fesom_gpu_c
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: x(*)
      end function fgpu_ssh_download
+
+     integer(c_int) function fgpu_autotune_step(ctx, dt, nsteps, best_tpb, best_ms) &
+          bind(C, name="fgpu_autotune_step")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dt
+       integer(c_int), value :: nsteps
+       integer(c_int), intent(out) :: best_tpb
+       real(c_double), intent(out) :: best_ms
+     end function fgpu_autotune_step
+
+     integer(c_int) function fgpu_ssh_nn_param_count() bind(C, name="fgpu_ssh_nn_param_count")
+       import :: c_int
+     end function fgpu_ssh_nn_param_count
+
+     integer(c_int) function fgpu_ssh_nn_init(ctx, seed) bind(C, name="fgpu_ssh_nn_init")
+       import :: c_int, c_ptr, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_long_long), value :: seed
+     end function fgpu_ssh_nn_init
+
+     integer(c_int) function fgpu_ssh_nn_set_weights(ctx, theta, np) bind(C, name="fgpu_ssh_nn_set_weights")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: theta(*)
+       integer(c_int), value :: np
+     end function fgpu_ssh_nn_set_weights
+
+     integer(c_int) function fgpu_ssh_nn_get_weights(ctx, theta, np) bind(C, name="fgpu_ssh_nn_get_weights")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(out) :: theta(*)
+       integer(c_int), value :: np
+     end function fgpu_ssh_nn_get_weights
+
+     integer(c_int) function fgpu_ssh_nn_apply(ctx, r, z) bind(C, name="fgpu_ssh_nn_apply")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: r(*)
+       real(c_double), intent(out) :: z(*)
+     end function fgpu_ssh_nn_apply
+
+     integer(c_int) function fgpu_ssh_nn_loss_grad(ctx, b, x, nsamp, loss, grad) bind(C, name="fgpu_ssh_nn_loss_grad")
+       import :: c_int, c_ptr, c_double, c_float
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: b(*), x(*)
+       integer(c_int), value :: nsamp
+       real(c_double), intent(out) :: loss
+       real(c_float), intent(out) :: grad(*)
+     end function fgpu_ssh_nn_loss_grad
+
+     integer(c_int) function fgpu_ssh_nn_train(ctx, b, x, nsamp, epochs, batch, lr, loss_per_epoch) &
+          bind(C, name="fgpu_ssh_nn_train")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: b(*), x(*)
+       integer(c_int), value :: nsamp, epochs, batch
+       real(c_double), value :: lr
+       real(c_double), intent(out) :: loss_per_epoch(*)
+     end function fgpu_ssh_nn_train
+
+     integer(c_int) function fgpu_ssh_solve_learned(ctx, x, rhs, soltol, maxiter, fallback_maxiter, stats, &
+          fallback_iters) bind(C, name="fgpu_ssh_solve_learned")
+       import :: c_int, c_ptr, c_double, fgpu_cg_stats
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: x(*)
+       real(c_double), intent(in) :: rhs(*)
+       real(c_double), value :: soltol
+       integer(c_int), value :: maxiter, fallback_maxiter
+       type(fgpu_cg_stats), intent(out) :: stats
+       integer(c_int), intent(out) :: fallback_iters
+     end function fgpu_ssh_solve_learned
+
+     integer(c_int) function fgpu_ssh_solve_learned_resident(ctx, soltol, maxiter, fallback_maxiter, nrep, stats, &
+          fallback_iters) bind(C, name="fgpu_ssh_solve_learned_resident")
+       import :: c_int, c_ptr, c_double, fgpu_cg_stats
+       type(c_ptr), value :: ctx
+       real(c_double), value :: soltol
+       integer(c_int), value :: maxiter, fallback_maxiter, nrep
+       type(fgpu_cg_stats), intent(out) :: stats
+       integer(c_int), intent(out) :: fallback_iters
+     end function fgpu_ssh_solve_learned_resident
   end interface
 end module fesom_gpu_c

@@ -34,6 +34,10 @@ module fesom_gpu_host
      real(WP) :: resid_last = 0.0_WP, rtol_last = 0.0_WP
      real(WP) :: ms_last = 0.0_WP
      logical :: pr_ready = .false.
+     ! learned preconditioner: FCG cap before the plain-CG fallback, and the
+     ! fallback's share of the last solve
+     integer :: maxiter_learned = 500
+     integer :: fallback_last = 0
   end type t_gpu_solverinfo
 
   type(c_ptr), save :: ctx = c_null_ptr
@@ -41,7 +45,8 @@ module fesom_gpu_host
   public :: gpu_init, gpu_finalize, gpu_check, gpu_mesh_setup, gpu_set_vertical, gpu_set_state, &
        gpu_tracers_init, gpu_tracer_config, gpu_tracer_put, gpu_tracer_get, gpu_tracer_content, &
        init_tracers_AB, do_oce_adv_tra, diff_tracers_ale, diff_ver_part_impl_ale, solve_tracers_ale, &
-       init_stiff_mat_ale, ssh_solve_preconditioner, ssh_solve_cg, solve_ssh_ale, gpu_synchronize
+       init_stiff_mat_ale, ssh_solve_preconditioner, ssh_solve_cg, solve_ssh_ale, gpu_synchronize, &
+       gpu_autotune_step, gpu_ssh_nn_init, gpu_ssh_nn_train, ssh_solve_cg_learned
 
 contains
 
@@ -248,5 +253,53 @@ contains
     if (.not. solverinfo%pr_ready) call ssh_solve_preconditioner(solverinfo)
     call ssh_solve_cg(d_eta, ssh_rhs, solverinfo)
   end subroutine solve_ssh_ale
+
+  ! fastest launch shape of the batched step for this partition (tracer state kept)
+  subroutine gpu_autotune_step(dt, nsteps, tracers_per_block, ms_per_step)
+    real(WP), intent(in) :: dt
+    integer, intent(in) :: nsteps
+    integer, intent(out) :: tracers_per_block
+    real(WP), intent(out) :: ms_per_step
+    integer(c_int) :: tpb
+    call gpu_check(fgpu_autotune_step(ctx, dt, int(nsteps, c_int), tpb, ms_per_step), 'fgpu_autotune_step')
+    tracers_per_block = tpb
+  end subroutine gpu_autotune_step
+
+  subroutine gpu_ssh_nn_init(seed)
+    integer, intent(in) :: seed
+    call gpu_check(fgpu_ssh_nn_init(ctx, int(seed, c_long_long)), 'fgpu_ssh_nn_init')
+  end subroutine gpu_ssh_nn_init
+
+  ! b(nod2D, nsamp) right-hand sides with solutions x(nod2D, nsamp)
+  subroutine gpu_ssh_nn_train(b, x, epochs, batch, lr, loss_per_epoch)
+    real(WP), intent(in) :: b(:, :), x(:, :)
+    integer, intent(in) :: epochs, batch
+    real(WP), intent(in) :: lr
+    real(WP), intent(out) :: loss_per_epoch(:)
+    call gpu_check(fgpu_ssh_nn_train(ctx, b, x, int(size(b, 2), c_int), int(epochs, c_int), int(batch, c_int), lr, &
+         loss_per_epoch), 'fgpu_ssh_nn_train')
+  end subroutine gpu_ssh_nn_train
+
+  ! ssh_solve_cg with the learned preconditioner in flexible CG; plain CG
+  ! continues from its x when it breaks down or reaches maxiter_learned
+  subroutine ssh_solve_cg_learned(x, rhs, solverinfo)
+    real(WP), intent(inout) :: x(:)
+    real(WP), intent(in) :: rhs(:)
+    type(t_gpu_solverinfo), intent(inout) :: solverinfo
+    type(fgpu_cg_stats) :: st
+    integer(c_int) :: fb
+    call gpu_check(fgpu_ssh_solve_learned(ctx, x, rhs, solverinfo%soltol, int(solverinfo%maxiter_learned, c_int), &
+         int(solverinfo%maxiter, c_int), st, fb), 'ssh_solve_cg_learned')
+    solverinfo%iters_last = st%iters_last
+    solverinfo%iters_max = max(solverinfo%iters_max, st%iters_last)
+    solverinfo%iters_sum = solverinfo%iters_sum + st%iters_last
+    solverinfo%nsolves = solverinfo%nsolves + 1
+    solverinfo%resid_last = st%resid_last
+    solverinfo%rtol_last = st%rtol_last
+    solverinfo%ms_last = st%ms
+    solverinfo%fallback_last = fb
+    solverinfo%nbreakdown = solverinfo%nbreakdown + st%nbreakdown
+    if (st%converged == 0) solverinfo%nonconv = solverinfo%nonconv + 1
+  end subroutine ssh_solve_cg_learned
 
 end module fesom_gpu_host

@@ -82,3 +82,14 @@ def test_fortran_host_c0(c0, c0_oracle, tmp_path):
     orp, oci, ov = c0_oracle.stiff_matrix(state.dt)
     A = sp.csr_matrix((ov.copy(), oci.copy(), orp.copy()), shape=(mesh.nod2D, mesh.nod2D))
     assert np.linalg.norm(rhs - A @ out["d_eta"]) / np.linalg.norm(rhs) < 1e-10 * 1.01
+
+
+def test_fortran_interfaces_cover_the_header():
+    """Every entry point of include/fesom_gpu.h has a bind(C) interface."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "fesom_gpu.h")).read()
+    decl = set(re.findall(r"\b(fgpu_[A-Za-z0-9_]+)\s*\(", hdr))
+    f90 = open(os.path.join(root, "fesom2_amd", "fortran", "fesom_gpu_c.f90")).read()
+    bound = set(re.findall(r'bind\(C, name="(fgpu_[A-Za-z0-9_]+)"\)', f90))
+    assert decl == bound, (sorted(decl - bound), sorted(bound - decl))
