@@ -318,87 +318,7 @@ def build_mesh(*, nx: int = 60, ny: int = 50, dx: float = 20e3,
     tris = [_zip_rows(idx[j], xs[j], idx[j + 1], xs[j + 1], L) for j in range(len(idx) - 1)]
     elem = np.concatenate(tris).astype(np.int64)
 
-    # clockwise orientation (oce_mesh.F90:1721-1742 test_tri)
-    a = coord[elem[:, 0]]
-    b = _trim_cyclic(coord[elem[:, 1], 0] - a[:, 0], L), coord[elem[:, 1], 1] - a[:, 1]
-    c = _trim_cyclic(coord[elem[:, 2], 0] - a[:, 0], L), coord[elem[:, 2], 1] - a[:, 1]
-    r = b[0] * c[1] - b[1] * c[0]
-    sw = r > 0
-    elem[sw, 1], elem[sw, 2] = elem[sw, 2].copy(), elem[sw, 1].copy()
-    F2 = elem.shape[0]
-
-    # --- edges (fvom_init.F90:500-607) -------------------------------------
-    he_a = np.concatenate([elem[:, 0], elem[:, 1], elem[:, 2]])
-    he_b = np.concatenate([elem[:, 1], elem[:, 2], elem[:, 0]])
-    he_e = np.concatenate([np.arange(F2)] * 3)
-    lo_n = np.minimum(he_a, he_b)
-    hi_n = np.maximum(he_a, he_b)
-    key = lo_n * N2 + hi_n
-    order = np.lexsort((he_e, key))
-    key_s = key[order]
-    e_s = he_e[order]
-    uniq, first, counts = np.unique(key_s, return_index=True, return_counts=True)
-    assert counts.max() <= 2
-    el1 = e_s[first]
-    el2 = np.where(counts == 2, e_s[np.minimum(first + 1, len(e_s) - 1)], -1)
-    n1 = (uniq // N2).astype(np.int64)
-    n2 = (uniq % N2).astype(np.int64)
-    inner = counts == 2
-    perm = np.concatenate([np.nonzero(inner)[0], np.nonzero(~inner)[0]])
-    n1, n2, el1, el2 = n1[perm], n2[perm], el1[perm], el2[perm]
-    E2 = len(n1)
-    E2_in = int(inner.sum())
-
-    def elem_center(ei):
-        ax = coord[elem[ei], 0]
-        amin = ax.min(axis=1, keepdims=True)
-        ax = np.where(ax - amin > L / 2.0, ax - L, ax)
-        x = (ax[:, 0] + ax[:, 1] + ax[:, 2]) / 3.0
-        yy = coord[elem[ei], 1]
-        y = (yy[:, 0] + yy[:, 1] + yy[:, 2]) / 3.0
-        return x, y
-
-    # orientation: edge_tri(1) to the left of n1 -> n2 (fvom_init.F90:579-607)
-    cx, cy = elem_center(el1)
-    xc0 = _trim_cyclic(cx - coord[n1, 0], L)
-    xc1 = cy - coord[n1, 1]
-    xe0 = _trim_cyclic(coord[n2, 0] - coord[n1, 0], L)
-    xe1 = coord[n2, 1] - coord[n1, 1]
-    right = xc0 * xe1 - xc1 * xe0 > 0.0
-    swap_el = right & (el2 >= 0)
-    swap_nd = right & (el2 < 0)
-    el1[swap_el], el2[swap_el] = el2[swap_el].copy(), el1[swap_el].copy()
-    n1[swap_nd], n2[swap_nd] = n2[swap_nd].copy(), n1[swap_nd].copy()
-    edges = np.stack([n1, n2], 1).astype(np.int32)
-    edge_tri = np.stack([el1, el2], 1).astype(np.int32)
-
-    # element neighbours across edges
-    elem_nb = -np.ones((F2, 3), np.int64)
-    fill = np.zeros(F2, np.int64)
-    for (ea, eb) in ((el1, el2), (el2, el1)):
-        m = (ea >= 0) & (eb >= 0)
-        ea_m, eb_m = ea[m], eb[m]
-        o = np.argsort(ea_m, kind="stable")
-        ea_m, eb_m = ea_m[o], eb_m[o]
-        # position within each element's list
-        starts = np.searchsorted(ea_m, ea_m, side="left")
-        pos = np.arange(len(ea_m)) - starts + fill[ea_m]
-        elem_nb[ea_m, pos] = eb_m
-        np.add.at(fill, ea_m, 1)
-
-    # nod_in_elem2D: ascending elements per node
-    ne_node = elem.reshape(-1)
-    ne_elem = np.repeat(np.arange(F2), 3)
-    o = np.lexsort((ne_elem, ne_node))
-    ne_node, ne_elem = ne_node[o], ne_elem[o]
-    nnum = np.bincount(ne_node, minlength=N2)
-    kmax = int(nnum.max())
-    starts = np.concatenate([[0], np.cumsum(nnum)[:-1]])
-    pos = np.arange(len(ne_node)) - starts[ne_node]
-    nie = -np.ones((N2, kmax), np.int64)
-    nie[ne_node, pos] = ne_elem
-
-    # --- vertical -----------------------------------------------------------
+    # --- vertical grid and topography ---------------------------------------
     nl = nlayers + 1
     if dz0 is None:
         zbar = _stretched_levels(nlayers, hmax, hmax / nlayers)
@@ -427,26 +347,159 @@ def build_mesh(*, nx: int = 60, ny: int = 50, dx: float = 20e3,
         depth = 30.0 + (hmax - 30.0) * f ** 0.6
     depth = -depth   # negative below sea level (aux3d.out convention)
 
-    dmean = (depth[elem[:, 0]] + depth[elem[:, 1]] + depth[elem[:, 2]]) / 3.0
-    thers = min_layers + 1
-    deeper = Zmid[None, :] < dmean[:, None]
-    has = deeper.any(axis=1)
-    nlev = np.where(has, np.argmax(deeper, axis=1) + 1, nl)
-    nlev = np.where(dmean >= 0, thers, nlev)
-    nlev = np.maximum(nlev, thers)
-    if max_layers is not None:
-        nlev = np.minimum(nlev, max_layers + 1)
-    if open_neighbour_sweep:
-        # fvom_init.F90:888-930, swept in parallel instead of in element order
-        for nz in range(thers + 1, nl + 1):
-            for _ in range(1000):
-                nbl = np.where(elem_nb >= 0, nlev[np.maximum(elem_nb, 0)], 0)
-                open_cnt = ((elem_nb >= 0) & (nbl >= nz)).sum(axis=1)
-                bad = (nlev >= nz) & (open_cnt < 2)
-                if not bad.any():
-                    break
-                nlev = np.where(bad, nz - 1, nlev)
-    nlevels = nlev.astype(np.int32)
+    return assemble_mesh(coord, elem, L=L, cartesian=cartesian, zbar=zbar, depth=depth, min_layers=min_layers,
+                         max_layers=max_layers, open_neighbour_sweep=open_neighbour_sweep)
+
+
+def build_edges(coord, elem, L):
+    """Edges and their two triangles from a triangulation (fvom_init.F90:500-607).
+
+    Returns 0-based (edges, edge_tri, edge2D_in): inner edges first, then
+    boundary edges (edge_tri(2) = -1), each inner/boundary group ordered by its
+    node pair; edge_tri(1) lies to the left of edges(1) -> edges(2), and on a
+    boundary edge the nodes are swapped until it does.
+    """
+    N2 = coord.shape[0]
+    F2 = elem.shape[0]
+    he_a = np.concatenate([elem[:, 0], elem[:, 1], elem[:, 2]])
+    he_b = np.concatenate([elem[:, 1], elem[:, 2], elem[:, 0]])
+    he_e = np.concatenate([np.arange(F2)] * 3)
+    lo_n = np.minimum(he_a, he_b)
+    hi_n = np.maximum(he_a, he_b)
+    key = lo_n * N2 + hi_n
+    order = np.lexsort((he_e, key))
+    key_s = key[order]
+    e_s = he_e[order]
+    uniq, first, counts = np.unique(key_s, return_index=True, return_counts=True)
+    assert counts.max() <= 2
+    el1 = e_s[first]
+    el2 = np.where(counts == 2, e_s[np.minimum(first + 1, len(e_s) - 1)], -1)
+    n1 = (uniq // N2).astype(np.int64)
+    n2 = (uniq % N2).astype(np.int64)
+    inner = counts == 2
+    perm = np.concatenate([np.nonzero(inner)[0], np.nonzero(~inner)[0]])
+    n1, n2, el1, el2 = n1[perm], n2[perm], el1[perm], el2[perm]
+    E2_in = int(inner.sum())
+
+    def elem_center(ei):
+        ax = coord[elem[ei], 0]
+        amin = ax.min(axis=1, keepdims=True)
+        ax = np.where(ax - amin > L / 2.0, ax - L, ax)
+        x = (ax[:, 0] + ax[:, 1] + ax[:, 2]) / 3.0
+        yy = coord[elem[ei], 1]
+        y = (yy[:, 0] + yy[:, 1] + yy[:, 2]) / 3.0
+        return x, y
+
+    # orientation: edge_tri(1) to the left of n1 -> n2 (fvom_init.F90:579-607)
+    cx, cy = elem_center(el1)
+    xc0 = _trim_cyclic(cx - coord[n1, 0], L)
+    xc1 = cy - coord[n1, 1]
+    xe0 = _trim_cyclic(coord[n2, 0] - coord[n1, 0], L)
+    xe1 = coord[n2, 1] - coord[n1, 1]
+    right = xc0 * xe1 - xc1 * xe0 > 0.0
+    swap_el = right & (el2 >= 0)
+    swap_nd = right & (el2 < 0)
+    el1[swap_el], el2[swap_el] = el2[swap_el].copy(), el1[swap_el].copy()
+    n1[swap_nd], n2[swap_nd] = n2[swap_nd].copy(), n1[swap_nd].copy()
+    edges = np.stack([n1, n2], 1).astype(np.int32)
+    edge_tri = np.stack([el1, el2], 1).astype(np.int32)
+
+    return edges, edge_tri, E2_in
+
+
+def assemble_mesh(coord, elem, *, L, cartesian, zbar, depth, min_layers=10, max_layers=None,
+                  open_neighbour_sweep=True, edges=None, edge_tri=None, nlevels=None) -> Mesh:
+    """Connectivity, levels and geometry of a triangulation.
+
+    coord (N2, 2) in radians, elem (F2, 3) 0-based, L the cyclic length in
+    radians, zbar (nl,) the interfaces, depth (N2,) negative below sea level.
+    edges/edge_tri (0-based, -1 = none) and nlevels (1-based per element) may be
+    given, as the reference's mesh files hold them; otherwise they are built.
+    """
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    elem = np.array(elem, dtype=np.int64)
+    N2 = coord.shape[0]
+    # clockwise orientation (oce_mesh.F90:1721-1742 test_tri)
+    a = coord[elem[:, 0]]
+    b = _trim_cyclic(coord[elem[:, 1], 0] - a[:, 0], L), coord[elem[:, 1], 1] - a[:, 1]
+    c = _trim_cyclic(coord[elem[:, 2], 0] - a[:, 0], L), coord[elem[:, 2], 1] - a[:, 1]
+    r = b[0] * c[1] - b[1] * c[0]
+    sw = r > 0
+    elem[sw, 1], elem[sw, 2] = elem[sw, 2].copy(), elem[sw, 1].copy()
+    F2 = elem.shape[0]
+
+    if edges is None:
+        edges, edge_tri, E2_in = build_edges(coord, elem, L)
+    else:
+        edges = np.ascontiguousarray(edges, dtype=np.int32)
+        edge_tri = np.ascontiguousarray(edge_tri, dtype=np.int32)
+        E2_in = int((edge_tri[:, 1] >= 0).sum())
+    n1 = edges[:, 0].astype(np.int64)
+    n2 = edges[:, 1].astype(np.int64)
+    el1 = edge_tri[:, 0].astype(np.int64)
+    el2 = edge_tri[:, 1].astype(np.int64)
+    E2 = len(n1)
+
+    def elem_center(ei):
+        ax = coord[elem[ei], 0]
+        amin = ax.min(axis=1, keepdims=True)
+        ax = np.where(ax - amin > L / 2.0, ax - L, ax)
+        x = (ax[:, 0] + ax[:, 1] + ax[:, 2]) / 3.0
+        yy = coord[elem[ei], 1]
+        y = (yy[:, 0] + yy[:, 1] + yy[:, 2]) / 3.0
+        return x, y
+
+    # element neighbours across edges
+    elem_nb = -np.ones((F2, 3), np.int64)
+    fill = np.zeros(F2, np.int64)
+    for (ea, eb) in ((el1, el2), (el2, el1)):
+        m = (ea >= 0) & (eb >= 0)
+        ea_m, eb_m = ea[m], eb[m]
+        o = np.argsort(ea_m, kind="stable")
+        ea_m, eb_m = ea_m[o], eb_m[o]
+        # position within each element's list
+        starts = np.searchsorted(ea_m, ea_m, side="left")
+        pos = np.arange(len(ea_m)) - starts + fill[ea_m]
+        elem_nb[ea_m, pos] = eb_m
+        np.add.at(fill, ea_m, 1)
+
+    # nod_in_elem2D: ascending elements per node
+    ne_node = elem.reshape(-1)
+    ne_elem = np.repeat(np.arange(F2), 3)
+    o = np.lexsort((ne_elem, ne_node))
+    ne_node, ne_elem = ne_node[o], ne_elem[o]
+    nnum = np.bincount(ne_node, minlength=N2)
+    kmax = int(nnum.max())
+    starts = np.concatenate([[0], np.cumsum(nnum)[:-1]])
+    pos = np.arange(len(ne_node)) - starts[ne_node]
+    nie = -np.ones((N2, kmax), np.int64)
+    nie[ne_node, pos] = ne_elem
+
+    nl = len(zbar)
+    Zmid = 0.5 * (zbar[:-1] + zbar[1:])
+    if nlevels is None:
+        dmean = (depth[elem[:, 0]] + depth[elem[:, 1]] + depth[elem[:, 2]]) / 3.0
+        thers = min_layers + 1
+        deeper = Zmid[None, :] < dmean[:, None]
+        has = deeper.any(axis=1)
+        nlev = np.where(has, np.argmax(deeper, axis=1) + 1, nl)
+        nlev = np.where(dmean >= 0, thers, nlev)
+        nlev = np.maximum(nlev, thers)
+        if max_layers is not None:
+            nlev = np.minimum(nlev, max_layers + 1)
+        if open_neighbour_sweep:
+            # fvom_init.F90:888-930, swept in parallel instead of in element order
+            for nz in range(thers + 1, nl + 1):
+                for _ in range(1000):
+                    nbl = np.where(elem_nb >= 0, nlev[np.maximum(elem_nb, 0)], 0)
+                    open_cnt = ((elem_nb >= 0) & (nbl >= nz)).sum(axis=1)
+                    bad = (nlev >= nz) & (open_cnt < 2)
+                    if not bad.any():
+                        break
+                    nlev = np.where(bad, nz - 1, nlev)
+        nlevels = nlev.astype(np.int32)
+    else:
+        nlevels = np.ascontiguousarray(nlevels, dtype=np.int32)
     ulevels = np.ones(F2, np.int32)
     nln = np.zeros(N2, np.int64)
     np.maximum.at(nln, elem.reshape(-1), np.repeat(nlevels, 3))

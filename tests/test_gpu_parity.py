@@ -268,3 +268,32 @@ def test_c1_100_steps_invariants(c1):
     s, _ = ctx.tracer_download(2)
     assert s[wet].min() >= 3.0 and s[wet].max() <= 45.0
     ctx.close()
+
+
+@pytest.mark.parametrize("name,cyclic,rotate", [("pi", 360.0, (50.0, 15.0, -90.0)), ("soufflet", 4.5, None)])
+def test_reference_mesh_steps(name, cyclic, rotate):
+    """The reference's own test meshes (connectivity and levels from its mesh
+    files): 3 fused steps and the SSH operator, bit for bit against the oracle."""
+    import os
+    from fesom2_amd import meshio
+    from oracle.oracle import Oracle
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"mesh_{name}.npz"))
+    mesh = meshio.mesh_from_arrays({k: f[k] for k in f.files}, cyclic_length_deg=cyclic, rotate=rotate)
+    state = mg.build_state(mesh, seed=7, ntr=2, dt=1800.0)
+    print(f"[gpu] reference mesh {name}: N2={mesh.nod2D} nl={mesh.nl}", flush=True)
+    ctx = _gpu().context_for(mesh, state)
+    ctx.solve_tracers_ale(state.dt, 3)
+    o = Oracle(mesh)
+    v = state.values.copy()
+    vo = state.valuesold.copy()
+    for _ in range(3):
+        o.step(state, v, vo)
+    wet = _wet(mesh)
+    for i in range(state.ntr):
+        gv, _ = ctx.tracer_download(i + 1)
+        np.testing.assert_array_equal(gv[wet], v[i][wet])
+    ctx.init_stiff_mat_ale(state.dt)
+    rp, ci, val, _ = ctx.ssh_get_matrix()
+    orp, oci, ov = o.stiff_matrix(state.dt)
+    np.testing.assert_array_equal(val, ov)
+    ctx.close()
