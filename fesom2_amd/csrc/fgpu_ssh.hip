@@ -100,7 +100,8 @@ struct CGArgs {
     double *x;
     const double *rhs;
     double *r, *z, *p0, *p1, *q;
-    double *part;   // 4 slots x nb: [0] b.b / p.q  [1] r.z  [2] r.r
+    double *w1, *s1;   // second buffers of w and s (single-reduction iteration)
+    double *part;   // 8 slots x nb: [0] b.b / p.q  [1] r.z  [2] r.r; single-reduction: [2p..2p+1], [4+p], [6]
     CGState *st;
     double soltol;
 };
@@ -295,6 +296,148 @@ __global__ void __launch_bounds__(SBLK) k_cg_c(CGArgs A)
     if (blockIdx.x == 0 && threadIdx.x == 0) st->iter += 1;
 }
 
+// ------------------------------------------------------------------------------
+// Diagonally preconditioned CG with one reduction per iteration (Chronopoulos &
+// Gear's arrangement of the same recurrences): with z = D^-1 r and w = A z,
+//   gamma = r.z, delta = w.z, beta = gamma/gamma_prev,
+//   alpha = gamma / (delta - beta gamma / alpha_prev),
+//   s = w + beta s, p = z + beta p, x += alpha p, r -= alpha s.
+// One kernel per iteration: it adds the previous iteration's partial sums,
+// decides convergence on r.r like the reference, updates its rows and forms
+// w = A z of the new residual, recomputing each neighbour's new z from the
+// previous iteration's (double-buffered) r, w and s. Partial sums are double-
+// buffered by iteration parity as well.
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ double cg1_slot(const CGArgs &A, int slot)
+{
+    double x = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int j = threadIdx.x; j < A.nb; j += SBLK) x += A.part[(size_t)slot * A.nb + j];
+    block_sum3(x, d1, d2);
+    return x;
+}
+
+// r0 = b - A x0; partials b.b (slot 6) and r.r (slot 4)
+__global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A)
+{
+    double bb = 0.0, rr = 0.0, dum = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        const double r = A.rhs[row] - row_dot(A.rowptr, A.colind, A.val, A.x, row);
+        A.r[row] = r;
+        bb += A.rhs[row] * A.rhs[row];
+        rr += r * r;
+    }
+    block_sum3(bb, rr, dum);
+    if (threadIdx.x == 0) {
+        A.part[(size_t)6 * A.nb + blockIdx.x] = bb;
+        A.part[(size_t)4 * A.nb + blockIdx.x] = rr;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CGState s{};
+        *A.st = s;
+    }
+}
+
+// w0 = A z0 with z0 = D^-1 r0; partials gamma0, delta0 (slots 0, 1); s_{-1} = p_{-1} = 0
+__global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
+{
+    double g = 0.0, d = 0.0, dum = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        double w = 0.0;
+        for (int p = A.rowptr[row]; p < A.rowptr[row + 1]; ++p) {
+            const int j = A.colind[p];
+            w = w + A.val[p] * (A.dinv[j] * A.r[j]);
+        }
+        const double z = A.dinv[row] * A.r[row];
+        A.q[row] = w;
+        A.p1[row] = 0.0;   // s_{-1}
+        A.p0[row] = 0.0;   // p_{-1}
+        g += A.r[row] * z;
+        d += w * z;
+    }
+    block_sum3(g, d, dum);
+    if (threadIdx.x == 0) {
+        A.part[blockIdx.x] = g;
+        A.part[(size_t)A.nb + blockIdx.x] = d;
+    }
+}
+
+// iteration i (i = st->iter): r_i, w_i in buffers [i&1], s_{i-1} in [(i+1)&1]
+__global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
+{
+    CGState *st = A.st;
+    if (st->done) return;
+    const int it = st->iter, par = it & 1;
+    const double gam = cg1_slot(A, 2 * par), del = cg1_slot(A, 2 * par + 1), rr = cg1_slot(A, 4 + par);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    double rtol;
+    if (it == 0) {
+        rtol = A.soltol * sqrt(cg1_slot(A, 6) / (double)A.n);
+        if (lead) st->rtol = rtol;
+    } else {
+        rtol = st->rtol;
+    }
+    if (lead) st->rr_last = rr;
+    if (sqrt(rr / (double)A.n) < rtol) {
+        if (lead) st->done = 1;
+        return;
+    }
+    if (it >= A.maxiter) {
+        if (lead) st->done = 3;
+        return;
+    }
+    if (gam < 0.0 && lead) st->nneg += 1;
+    double be = 0.0, den = del;
+    if (it > 0) {
+        const double gprev = st->s_old[0], aprev = st->s_old[1];
+        if (gprev == 0.0) {
+            if (lead) { st->nbreak += 1; st->done = 2; }
+            return;
+        }
+        be = gam / gprev;
+        den = del - be * gam / aprev;
+    }
+    if (!(den > 0.0)) {
+        if (lead) { st->nbreak += 1; st->done = 2; }
+        return;
+    }
+    const double al = gam / den;
+    const double *rc = par ? A.z : A.r, *wc = par ? A.w1 : A.q, *sp = par ? A.s1 : A.p1;
+    double *rn = par ? A.r : A.z, *wn = par ? A.q : A.w1, *sc = par ? A.p1 : A.s1;
+    double g = 0.0, d = 0.0, r2 = 0.0;
+    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+        const double s_ = wc[row] + be * sp[row];
+        const double p_ = A.dinv[row] * rc[row] + be * A.p0[row];
+        A.p0[row] = p_;
+        A.x[row] = A.x[row] + al * p_;
+        const double r_ = rc[row] - al * s_;
+        sc[row] = s_;
+        rn[row] = r_;
+        double w = 0.0;
+        for (int p = A.rowptr[row]; p < A.rowptr[row + 1]; ++p) {
+            const int j = A.colind[p];
+            const double zj = A.dinv[j] * (rc[j] - al * (wc[j] + be * sp[j]));
+            w = w + A.val[p] * zj;
+        }
+        wn[row] = w;
+        const double z_ = A.dinv[row] * r_;
+        g += r_ * z_;
+        d += w * z_;
+        r2 += r_ * r_;
+    }
+    block_sum3(g, d, r2);
+    const int np = par ^ 1;
+    if (threadIdx.x == 0) {
+        A.part[(size_t)(2 * np) * A.nb + blockIdx.x] = g;
+        A.part[(size_t)(2 * np + 1) * A.nb + blockIdx.x] = d;
+        A.part[(size_t)(4 + np) * A.nb + blockIdx.x] = r2;
+    }
+    if (lead) {
+        st->s_old[0] = gam;
+        st->s_old[1] = al;
+        st->iter = it + 1;
+    }
+}
+
 __global__ void k_copy(double *dst, const double *src, int n)
 {
     const int i = blockIdx.x * SBLK + threadIdx.x;
@@ -309,13 +452,13 @@ static int ssh_alloc_vectors(fgpu_context *c)
     const size_t n = c->ssh_n;
     int rc;
     double **vs[] = {&c->ssh_x, &c->ssh_x0, &c->ssh_rhs, &c->ssh_r, &c->ssh_z, &c->ssh_p, &c->ssh_p1, &c->ssh_q,
-                     &c->ssh_diag_inv};
+                     &c->ssh_diag_inv, &c->ssh_w1, &c->ssh_s1};
     for (double **v : vs)
         if (!*v && (rc = fgpu_alloc(c, (void **)v, n * sizeof(double)))) return rc;
     int blocks = (int)((n + SBLK - 1) / SBLK);
     if (blocks > 512) blocks = 512;
     c->ssh_blocks = blocks;
-    if (!c->ssh_part && (rc = fgpu_alloc(c, (void **)&c->ssh_part, sizeof(double) * 4 * (size_t)blocks))) return rc;
+    if (!c->ssh_part && (rc = fgpu_alloc(c, (void **)&c->ssh_part, sizeof(double) * 8 * (size_t)blocks))) return rc;
     if (!c->ssh_out && (rc = fgpu_alloc(c, (void **)&c->ssh_out, sizeof(CGState)))) return rc;
     return FGPU_OK;
 }
@@ -458,6 +601,8 @@ static CGArgs cg_args(fgpu_context *c, double soltol, int maxiter)
     A.p0 = c->ssh_p;
     A.p1 = c->ssh_p1;
     A.q = c->ssh_q;
+    A.w1 = c->ssh_w1;
+    A.s1 = c->ssh_s1;
     A.part = c->ssh_part;
     A.st = (CGState *)c->ssh_out;
     A.soltol = soltol;
@@ -469,6 +614,11 @@ static CGArgs cg_args(fgpu_context *c, double soltol, int maxiter)
 static int enqueue_iters(fgpu_context *c, const CGArgs &A, int iters)
 {
     const dim3 g(A.nb), b(SBLK);
+    if (A.variant == 2 && c->ssh_single_reduction) {
+        for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_cg1, g, b, 0, c->stream, A);
+        FGPU_HIP(hipGetLastError());
+        return FGPU_OK;
+    }
     for (int i = 0; i < iters; ++i) {
         hipLaunchKernelGGL(k_cg_a, g, b, 0, c->stream, A);
         if (A.variant == 2) {
@@ -485,8 +635,8 @@ static int enqueue_iters(fgpu_context *c, const CGArgs &A, int iters)
 // graph of CG_GRAPH_ITERS iterations for the current (soltol, maxiter, variant)
 static int cg_graph(fgpu_context *c, const CGArgs &A)
 {
-    if (c->ssh_graph && c->ssh_graph_tol == A.soltol && c->ssh_graph_maxiter == A.maxiter &&
-        c->ssh_graph_variant == A.variant)
+    const int key = A.variant + (c->ssh_single_reduction ? 16 : 0);
+    if (c->ssh_graph && c->ssh_graph_tol == A.soltol && c->ssh_graph_maxiter == A.maxiter && c->ssh_graph_variant == key)
         return FGPU_OK;
     if (c->ssh_graph) { (void)hipGraphExecDestroy(c->ssh_graph); c->ssh_graph = nullptr; }
     hipGraph_t g;
@@ -500,7 +650,7 @@ static int cg_graph(fgpu_context *c, const CGArgs &A)
     if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     c->ssh_graph_tol = A.soltol;
     c->ssh_graph_maxiter = A.maxiter;
-    c->ssh_graph_variant = A.variant;
+    c->ssh_graph_variant = key;
     return FGPU_OK;
 }
 
@@ -514,8 +664,13 @@ static int launch_cg(fgpu_context *c, double soltol, int maxiter, int variant)
     int rc = cg_graph(c, A);
     if (rc) return rc;
     const dim3 g(A.nb), b(SBLK);
-    hipLaunchKernelGGL(k_cg_init, g, b, 0, c->stream, A);
-    if (A.variant != 2) hipLaunchKernelGGL(k_cg_init_z, g, b, 0, c->stream, A);
+    if (A.variant == 2 && c->ssh_single_reduction) {
+        hipLaunchKernelGGL(k_cg1_init_r, g, b, 0, c->stream, A);
+        hipLaunchKernelGGL(k_cg1_init_w, g, b, 0, c->stream, A);
+    } else {
+        hipLaunchKernelGGL(k_cg_init, g, b, 0, c->stream, A);
+        if (A.variant != 2) hipLaunchKernelGGL(k_cg_init_z, g, b, 0, c->stream, A);
+    }
     FGPU_HIP(hipGetLastError());
     const int max_graphs = maxiter / CG_GRAPH_ITERS + 2;
     int expect = c->ssh_last_iters > 0 ? c->ssh_last_iters : 4 * CG_GRAPH_ITERS;

@@ -224,6 +224,32 @@ def test_ssh_cg(c0, c0_oracle, variant):
     ctx.close()
 
 
+def test_ssh_cg_single_reduction_vs_classic(c0, monkeypatch):
+    """The one-reduction-per-iteration arrangement of diagonally preconditioned
+    CG (default for the diagonal preconditioner) and the reference's two-
+    reduction loop (FGPU_SSH_CG=classic) converge to the same tolerance in the
+    same number of iterations within +-2 and agree to cond*tol."""
+    mesh, state, _ = c0
+    rhs = mg.ssh_rhs(mesh, 3)
+    out = {}
+    for mode in ("single", "classic"):
+        if mode == "classic":
+            monkeypatch.setenv("FGPU_SSH_CG", "classic")
+        else:
+            monkeypatch.delenv("FGPU_SSH_CG", raising=False)
+        ctx = _gpu().Context(0)
+        ctx.mesh_init(mesh)
+        ctx.init_stiff_mat_ale(state.dt)
+        ctx.ssh_solve_preconditioner(2)
+        out[mode] = ctx.ssh_solve_cg(np.zeros(mesh.nod2D), rhs, soltol=1e-10, maxiter=5000)
+        ctx.close()
+    (xs, ss), (xc, sc) = out["single"], out["classic"]
+    print(f"[gpu] cg single {ss['iters']} ({ss['ms']:.3f} ms), classic {sc['iters']} ({sc['ms']:.3f} ms)", flush=True)
+    assert ss["converged"] and sc["converged"]
+    assert abs(ss["iters"] - sc["iters"]) <= 2
+    assert np.linalg.norm(xs - xc) / np.linalg.norm(xc) < 1e-7
+
+
 @pytest.fixture(scope="module")
 def c1():
     mesh, state, steps = mg.make_config("c1")
