@@ -30,7 +30,7 @@
 #include "fgpu_internal.h"
 
 #define NXCD 8
-#define TCH 4   // tracers per Thomas chunk in K_D (one lane per tracer)
+#define NG 8    // edges per gather batch in K_B and K_C
 
 __device__ __forceinline__ double dmx(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmn(double a, double b) { return a < b ? a : b; }
@@ -365,12 +365,25 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
         if (lim == LIM_FCT) {
             const double *lo_h = t.lo_h + t.sE * tr;
             double acc = 0.0;
-            for (int j = 0; j < E.deg; ++j) {
-                const int q = rl(E.q, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
-                if (act && k >= a0 && k <= a1) {
-                    const double fl = lo_h[(size_t)(q >> 1) * L + k - 1];
-                    acc = (q & 1) ? acc - fl : acc + fl;
+            // edges in batches of NG: the batch's loads are issued together
+            // (clamped to valid addresses), then added in ascending edge order
+            const int kk = min(k, L) - 1;
+            for (int j0 = 0; j0 < E.deg; j0 += NG) {
+                double fl_[NG];
+                int q_[NG];
+                bool ok_[NG];
+#pragma unroll
+                for (int u = 0; u < NG; ++u) {
+                    const bool real = j0 + u < E.deg;
+                    const int j = real ? j0 + u : 0;
+                    const int q = rl(E.q, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                    ok_[u] = real && act && k >= a0 && k <= a1;
+                    q_[u] = q;
+                    fl_[u] = lo_h[(size_t)(q >> 1) * L + kk];
                 }
+#pragma unroll
+                for (int u = 0; u < NG; ++u)
+                    if (ok_[u]) acc = (q_[u] & 1) ? acc - fl_[u] : acc + fl_[u];
             }
             if (act) {
                 const double lk1 = sLo[tid + 1];
@@ -419,26 +432,41 @@ __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt
             p = p + (dmx(0.0, av[k - 1]) + dmx(0.0, -av[k]));
             q = q + (dmn(0.0, av[k - 1]) + dmn(0.0, -av[k]));
         }
-        for (int j = 0; j < E.deg; ++j) {
-            const int qq = rl(E.q, j), mo = rl(E.other, j);
-            const int a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
-            const int b_nu1 = rl(E.nu1, j), b_nl1 = rl(E.nl1, j), b_nu2 = rl(E.nu2, j), b_nl2 = rl(E.nl2, j);
-            if (!act) continue;
-            const bool has = (k >= b_nu1 && k <= b_nl1) || (b_nu2 > 0 && k >= b_nu2 && k <= b_nl2);
-            if (has) {
-                const size_t mi = (size_t)mo * L + k - 1;
-                const double lm = LO[mi], tm = T[mi];
-                vmax = dmx(vmax, dmx(lm, tm));
-                vmin = dmn(vmin, dmn(lm, tm));
+        const int kk = min(k, L) - 1;
+        for (int j0 = 0; j0 < E.deg; j0 += NG) {
+            double lm_[NG], tm_[NG], fl_[NG];
+            int qq_[NG];
+            bool has_[NG], in_[NG];
+#pragma unroll
+            for (int u = 0; u < NG; ++u) {
+                const bool real = j0 + u < E.deg;
+                const int j = real ? j0 + u : 0;
+                const int qq = rl(E.q, j), mo = rl(E.other, j);
+                const int a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                const int b_nu1 = rl(E.nu1, j), b_nl1 = rl(E.nl1, j), b_nu2 = rl(E.nu2, j), b_nl2 = rl(E.nl2, j);
+                has_[u] = real && act && ((k >= b_nu1 && k <= b_nl1) || (b_nu2 > 0 && k >= b_nu2 && k <= b_nl2));
+                in_[u] = real && act && k >= a0 && k <= a1;
+                qq_[u] = qq;
+                const size_t mi = (size_t)mo * L + kk;
+                lm_[u] = LO[mi];
+                tm_[u] = T[mi];
+                fl_[u] = ah[(size_t)(qq >> 1) * L + kk];
             }
-            if (k >= a0 && k <= a1) {
-                const double fl = ah[(size_t)(qq >> 1) * L + k - 1];
-                if (qq & 1) {
-                    p = p + dmx(0.0, -fl);
-                    q = q + dmn(0.0, -fl);
-                } else {
-                    p = p + dmx(0.0, fl);
-                    q = q + dmn(0.0, fl);
+#pragma unroll
+            for (int u = 0; u < NG; ++u) {
+                if (has_[u]) {
+                    vmax = dmx(vmax, dmx(lm_[u], tm_[u]));
+                    vmin = dmn(vmin, dmn(lm_[u], tm_[u]));
+                }
+                if (in_[u]) {
+                    const double fl = fl_[u];
+                    if (qq_[u] & 1) {
+                        p = p + dmx(0.0, -fl);
+                        q = q + dmn(0.0, -fl);
+                    } else {
+                        p = p + dmx(0.0, fl);
+                        q = q + dmn(0.0, fl);
+                    }
                 }
             }
         }
@@ -475,7 +503,8 @@ __device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &
     }
 }
 
-template <int LB>
+// TCH: tracers per Thomas chunk (one lane per tracer), a launch-shape choice
+template <int LB, int TCH>
 __global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb)
 {
     const int n = xcd_column(m.N2);
@@ -705,6 +734,7 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     // tracers per block: 1 runs every tracer's chain in its own waves
     const int tpb = std::max(1, std::min(c->step_tpb, nt));
     const unsigned ny = (unsigned)((nt + tpb - 1) / tpb);
+    const int tch = c->step_tch == 4 ? 4 : 2;
 #define LAUNCH5(LBV)                                                                                                   \
     {                                                                                                                  \
         {                                                                                                              \
@@ -725,7 +755,10 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
-            hipLaunchKernelGGL(kD_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+            if (tch == 4)                                                                                              \
+                hipLaunchKernelGGL((kD_node<LBV, 4>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+            else                                                                                                       \
+                hipLaunchKernelGGL((kD_node<LBV, 2>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
         }                                                                                                              \
     }
     // one lane per level and per interface: LB >= NL

@@ -898,7 +898,8 @@ extern "C" int fgpu_solve_tracers_ale_timed(fgpu_context *c, double dt, int nste
 }
 
 // Autotuner of the fused step's launch shape: each candidate number of tracers
-// per block (1, 2, 4, ... and all) is timed over `nsteps` replayed steps and the
+// per block (1, 2, 4, ... and all), crossed with 2 or 4 tracers per Thomas
+// chunk in the last kernel, is timed over `nsteps` replayed steps and the
 // fastest is kept. Every candidate computes the same arithmetic per column, so
 // results do not depend on the choice. The tracer state is saved before and
 // restored after the trials.
@@ -922,19 +923,23 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     };
     FGPU_HIP(hipMemcpyAsync(save, c->t.T, sizeof(double) * nT, hipMemcpyDeviceToDevice, c->stream));
     FGPU_HIP(hipMemcpyAsync(save + nT, c->t.Told, sizeof(double) * nO, hipMemcpyDeviceToDevice, c->stream));
-    std::vector<int> cand;
-    for (int v = 1; v < nt; v *= 2) cand.push_back(v);
-    cand.push_back(nt);
+    std::vector<std::pair<int, int>> cand;   // (tracers per block, tracers per Thomas chunk)
+    for (int tch : {2, 4}) {
+        if (tch == 4 && nt < 3) continue;
+        for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch});
+        cand.push_back({nt, tch});
+    }
     const bool was_prof = c->prof;
     c->prof = false;
-    int best = c->step_tpb;
+    int best = c->step_tpb, best_tch = c->step_tch;
     float best_t = -1.f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rc = FGPU_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fgpu_fail(FGPU_ERR_HIP, "hipEventCreate failed");
     for (size_t i = 0; i < cand.size() && !rc; ++i) {
-        c->step_tpb = cand[i];
+        c->step_tpb = cand[i].first;
+        c->step_tch = cand[i].second;
         c->graph_valid = false;
         if ((rc = run_steps(c, dt, 1))) break;   // capture and warm up
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord"); break; }
@@ -945,7 +950,7 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
             rc = fgpu_fail(FGPU_ERR_HIP, "timing the autotuner trial failed");
             break;
         }
-        if (best_t < 0.f || ms < best_t) { best_t = ms; best = cand[i]; }
+        if (best_t < 0.f || ms < best_t) { best_t = ms; best = cand[i].first; best_tch = cand[i].second; }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -953,6 +958,7 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     (void)hipFree(save);
     c->prof = was_prof;
     c->step_tpb = best;
+    c->step_tch = best_tch;
     c->graph_valid = false;
     if (rc) return rc;
     if (rr) return rr;
