@@ -45,7 +45,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // partial-sum slots (double, nbd blocks each)
-enum { SL_BB = 0, SL_RR, SL_RTRT, SL_ZR, SL_ZQ, SL_PQ, SL_XAX, SL_EAE, SL_JR, SL_JQ, NSLOT };
+enum { SL_BB = 0, SL_RR, SL_RTRT, SL_ZR, SL_ZQ, SL_PQ, SL_XAX, SL_EAE, SL_JR, SL_JQ, SL_PQ1, NSLOT };
 
 struct FCGState {
     int iter, done, nbreak, nneg;   // nneg: iterations that fell back to the diagonal direction
@@ -63,6 +63,7 @@ struct NNDev {
     float *dH = nullptr, *dXs = nullptr, *dXm = nullptr;
     float *part = nullptr;                   // NNB x NNP
     double *dpart = nullptr;                 // NSLOT x nbd
+    double *tpart = nullptr;                 // 4 x ntiles
     double *ae = nullptr, *zs = nullptr;     // A e and z of a training sample
     double *loss = nullptr;                  // loss accumulator
     FCGState *st = nullptr;
@@ -86,6 +87,7 @@ struct NNArgs {
     bf16_t *H, *M;
     float *u, *y, *dH, *dXs, *dXm, *part;
     double *dpart, *ae, *loss;
+    double *tpart;  // 4 x ntiles per-tile partials of the last layer (FCG)
     FCGState *st;   // null outside FCG
 };
 
@@ -100,19 +102,34 @@ __device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float(((unsig
 
 __device__ __forceinline__ bool nn_done(const NNArgs &A) { return A.st && A.st->done; }
 
-// block sum of three doubles; valid in every thread
+// block sum of three doubles (butterfly per wave, then the wave sums in order);
+// valid in every thread, identical in every block for identical inputs
 __device__ void nn_bsum3(double &a, double &b, double &c)
 {
-    __shared__ double sh[3][NNT];
-    const int t = threadIdx.x;
-    sh[0][t] = a; sh[1][t] = b; sh[2][t] = c;
-    __syncthreads();
-    for (int w = NNT / 2; w > 0; w >>= 1) {
-        if (t < w) { sh[0][t] += sh[0][t + w]; sh[1][t] += sh[1][t + w]; sh[2][t] += sh[2][t + w]; }
-        __syncthreads();
+    __shared__ double sh[3][NNT / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
     }
-    a = sh[0][0]; b = sh[1][0]; c = sh[2][0];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = c; }
     __syncthreads();
+    a = sh[0][0]; b = sh[1][0]; c = sh[2][0];
+    for (int i = 1; i < NNT / 64; ++i) { a += sh[0][i]; b += sh[1][i]; c += sh[2][i]; }
+    __syncthreads();
+}
+
+// three slots at once
+__device__ void nn_slots3(const double *dpart, int nbd, int s0, int s1, int s2, double &a, double &b, double &c)
+{
+    a = 0.0; b = 0.0; c = 0.0;
+    for (int j = threadIdx.x; j < nbd; j += NNT) {
+        a += dpart[(size_t)s0 * nbd + j];
+        if (s1 >= 0) b += dpart[(size_t)s1 * nbd + j];
+        if (s2 >= 0) c += dpart[(size_t)s2 * nbd + j];
+    }
+    nn_bsum3(a, b, c);
 }
 
 // the nbd partials of a slot, added in block order by every block
@@ -148,16 +165,44 @@ __global__ void __launch_bounds__(NNT) k_nn_rms(NNArgs A, const double *rin)
     nn_put3(A, SL_RTRT, s, -1, 0.0, -1, 0.0);
 }
 
-// input layer; in FCG mode also the convergence test on r.r
+#define LDX 136   // LDS row stride (bf16) of the 128-wide tiles
+#define LDT 72    // LDS row stride (bf16) of the 64-wide tiles
+#define NGB 8     // neighbours per gather batch
+
+// h0 of a node: relu(u W0[0] + W0[1]) on channels c0..c0+7, packed bf16
+__device__ __forceinline__ u32x4 nn_h0_pack(float u, const float *W0, int c0)
+{
+    unsigned int pk[4];
+    for (int c = 0; c < 8; c += 2) {
+        float p0 = u * W0[c0 + c] + W0[NNC + c0 + c];
+        float p1 = u * W0[c0 + c + 1] + W0[NNC + c0 + c + 1];
+        p0 = p0 > 0.f ? p0 : 0.f;
+        p1 = p1 > 0.f ? p1 : 0.f;
+        pk[c >> 1] = (unsigned int)f2bf(p0) | ((unsigned int)f2bf(p1) << 16);
+    }
+    const u32x4 v = {pk[0], pk[1], pk[2], pk[3]};
+    return v;
+}
+
+__device__ __forceinline__ void nn_acc8(float *acc, float wv, const u32x4 a)
+{
+    const unsigned int wd[4] = {a.x, a.y, a.z, a.w};
+    for (int c = 0; c < 4; ++c) {
+        acc[2 * c] = acc[2 * c] + wv * bf2f((bf16_t)(wd[c] & 0xffffu));
+        acc[2 * c + 1] = acc[2 * c + 1] + wv * bf2f((bf16_t)(wd[c] >> 16));
+    }
+}
+
+// input layer over all nodes (grid-stride, nbd blocks): u and H[0]; in FCG mode
+// first the convergence test on r.r, and the scale s for the later layers
 __global__ void __launch_bounds__(NNT) k_nn_in(NNArgs A, const double *rin, double soltol)
 {
     if (nn_done(A)) return;
-    const double rtrt = nn_slot(A, SL_RTRT);
+    double rtrt, bb, rr;
+    nn_slots3(A.dpart, A.nbd, SL_RTRT, A.st ? SL_BB : -1, A.st ? SL_RR : -1, rtrt, bb, rr);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (A.st) {
-        const double bb = nn_slot(A, SL_BB);
-        const double rr = nn_slot(A, SL_RR);
         const double rtol = soltol * sqrt(bb / (double)A.n);
-        const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
         if (lead) { A.st->rtol = rtol; A.st->rr_last = rr; }
         if (sqrt(rr / (double)A.n) < rtol) {
             if (lead) A.st->done = 1;
@@ -166,141 +211,168 @@ __global__ void __launch_bounds__(NNT) k_nn_in(NNArgs A, const double *rin, doub
     }
     double s = sqrt(rtrt / (double)A.n);
     if (!(s > 0.0)) s = 1.0;
-    if (A.st && blockIdx.x == 0 && threadIdx.x == 0) A.st->s = s;
+    if (A.st && lead) A.st->s = s;
     const float *W0 = A.theta + OFF_W0;
-    for (int q = blockIdx.x * NNT + threadIdx.x; q < A.n * 8; q += A.nbd * NNT) {
-        const int i = q >> 3, c0 = (q & 7) * 8;
+    for (int i = blockIdx.x * NNT + threadIdx.x; i < A.n; i += A.nbd * NNT) {
         const float u = (float)(A.isq[i] * rin[i] / s);
-        if ((q & 7) == 0) A.u[i] = u;
-        unsigned int pk[4];
-        for (int c = 0; c < 8; c += 2) {
-            float p0 = u * W0[c0 + c] + W0[NNC + c0 + c];
-            float p1 = u * W0[c0 + c + 1] + W0[NNC + c0 + c + 1];
-            p0 = p0 > 0.f ? p0 : 0.f;
-            p1 = p1 > 0.f ? p1 : 0.f;
-            pk[c >> 1] = (unsigned int)f2bf(p0) | ((unsigned int)f2bf(p1) << 16);
-        }
-        u32x4 v = {pk[0], pk[1], pk[2], pk[3]};
-        *(u32x4 *)(A.H + (size_t)i * NNC + c0) = v;
+        A.u[i] = u;
+        for (int c8 = 0; c8 < NNC; c8 += 8) *(u32x4 *)(A.H + (size_t)i * NNC + c8) = nn_h0_pack(u, W0, c8);
     }
 }
 
-#define LDX 136   // LDS row stride (bf16) of the 128-wide tiles
-#define LDT 72    // LDS row stride (bf16) of the 64-wide tiles
-
-// message-passing layer l (1..3): H[l] = relu([H[l-1], Ahat H[l-1]] W_l + b_l)
-__global__ void __launch_bounds__(NNT) k_nn_layer(NNArgs A, int l)
+// Message-passing layer l (1..3), one 64-node tile per block (grid = ntiles):
+//   H[l] = relu([H[l-1], Ahat H[l-1]] W_l + b_l), M[l-1] = Ahat H[l-1] (bf16)
+// l = 1 also forms the input layer (H[0], u) from rin, and in FCG mode first
+// decides convergence on r.r; l = 3 also forms y = h3 . w_out + b_out,
+// z = D^-1/2 (r~ + s y) and the partial sums z.r, z.q, r.D^-1 r, (D^-1 r).q.
+template <int LL>
+__global__ void __launch_bounds__(NNT) k_nn_layer(NNArgs A, const double *rin, double *zout, const double *q,
+                                                  double soltol)
 {
     if (nn_done(A)) return;
     __shared__ __attribute__((aligned(16))) bf16_t sX[NNTILE * LDX];
     __shared__ __attribute__((aligned(16))) bf16_t sW[NNC * LDX];
+    __shared__ float sY[2][NNTILE];
+    bf16_t *sO = sX;   // output tile, written after the MFMA reads of sX
     const int t = threadIdx.x;
-    const int tile0 = blockIdx.x * NNTILE;
-    const bf16_t *Hin = A.H + (size_t)(l - 1) * A.n * NNC;
-    bf16_t *Hout = A.H + (size_t)l * A.n * NNC;
-    bf16_t *Mout = A.M + (size_t)(l - 1) * A.n * NNC;
-    const bf16_t *Wf = A.Wf + (size_t)(l - 1) * NNC * 128;
-    for (int q = t; q < NNC * 16; q += NNT) {   // W_l^T: 64 rows of 128
-        const int row = q >> 4, c8 = (q & 15) * 8;
+    double s = 1.0;
+    if (LL == 3) {
+        if (A.st) {
+            s = A.st->s;
+        } else {
+            double rtrt, u1, u2;
+            nn_slots3(A.dpart, A.nbd, SL_RTRT, -1, -1, rtrt, u1, u2);
+            s = sqrt(rtrt / (double)A.n);
+            if (!(s > 0.0)) s = 1.0;
+        }
+    }
+    const bf16_t *Hin = A.H + (size_t)(LL - 1) * A.n * NNC;
+    bf16_t *Hout = A.H + (size_t)LL * A.n * NNC;
+    bf16_t *Mout = A.M + (size_t)(LL - 1) * A.n * NNC;
+    const float *W0 = A.theta + OFF_W0;
+    const bf16_t *Wf = A.Wf + (size_t)(LL - 1) * NNC * 128;
+    for (int qq = t; qq < NNC * 16; qq += NNT) {   // W_l^T: 64 rows of 128
+        const int row = qq >> 4, c8 = (qq & 15) * 8;
         *(u32x4 *)&sW[row * LDX + c8] = *(const u32x4 *)(Wf + row * 128 + c8);
     }
-    for (int q = t; q < NNTILE * 8; q += NNT) {   // own features
-        const int nd = q >> 3, c8 = (q & 7) * 8;
-        const int i = tile0 + nd;
-        u32x4 v = {0u, 0u, 0u, 0u};
-        if (i < A.n) v = *(const u32x4 *)(Hin + (size_t)i * NNC + c8);
-        *(u32x4 *)&sX[nd * LDX + c8] = v;
-    }
-    {   // aggregation: node t/4, channels (t%4)*16 .. +16, neighbours in CSR order
-        const int nd = t >> 2, c0 = (t & 3) * 16;
-        const int i = tile0 + nd;
-        float acc[16];
-        for (int c = 0; c < 16; ++c) acc[c] = 0.f;
-        if (i < A.n) {
-            for (int p = A.rowptr[i] + 1; p < A.rowptr[i + 1]; ++p) {
-                const int j = A.colind[p];
-                const float wv = A.w[p];
-                const u32x4 a = *(const u32x4 *)(Hin + (size_t)j * NNC + c0);
-                const u32x4 b = *(const u32x4 *)(Hin + (size_t)j * NNC + c0 + 8);
-                const unsigned int wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-                for (int c = 0; c < 8; ++c) {
-                    acc[2 * c] = acc[2 * c] + wv * bf2f((bf16_t)(wd[c] & 0xffffu));
-                    acc[2 * c + 1] = acc[2 * c + 1] + wv * bf2f((bf16_t)(wd[c] >> 16));
+    const int wv = t >> 6, lane = t & 63;
+    const int rt = wv >> 1, ct = wv & 1;
+    const int arow = rt * 32 + (lane & 31), bcol = ct * 32 + (lane & 31), kh = (lane >> 5) * 8;
+    const float bv = A.theta[OFF_B(LL) + bcol];
+    const float *wo = A.theta + OFF_WOUT;
+    double zr = 0.0, zq = 0.0, jr = 0.0, jq = 0.0;
+    {
+        const int tile0 = blockIdx.x * NNTILE;
+        for (int qq = t; qq < NNTILE * 8; qq += NNT) {   // own features
+            const int nd = qq >> 3, c8 = (qq & 7) * 8;
+            const int i = tile0 + nd;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (i < A.n) v = *(const u32x4 *)(Hin + (size_t)i * NNC + c8);
+            *(u32x4 *)&sX[nd * LDX + c8] = v;
+        }
+        {   // aggregation: node t/4, channels (t%4)*16 .. +16, neighbours in CSR order
+            const int nd = t >> 2, c0 = (t & 3) * 16;
+            const int i = tile0 + nd;
+            float acc[16];
+            for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+            if (i < A.n) {
+                const int p0 = A.rowptr[i] + 1, p1 = A.rowptr[i + 1];
+                for (int pb = p0; pb < p1; pb += NGB) {
+                    int jn[NGB];
+                    float wn[NGB];
+#pragma unroll
+                    for (int u = 0; u < NGB; ++u) {
+                        const int p = pb + u < p1 ? pb + u : p0;
+                        jn[u] = A.colind[p];
+                        wn[u] = A.w[p];
+                    }
+                    {
+                        u32x4 va[NGB], vb[NGB];
+#pragma unroll
+                        for (int u = 0; u < NGB; ++u) {
+                            va[u] = *(const u32x4 *)(Hin + (size_t)jn[u] * NNC + c0);
+                            vb[u] = *(const u32x4 *)(Hin + (size_t)jn[u] * NNC + c0 + 8);
+                        }
+#pragma unroll
+                        for (int u = 0; u < NGB; ++u) {
+                            if (pb + u < p1) {
+                                nn_acc8(acc, wn[u], va[u]);
+                                nn_acc8(acc + 8, wn[u], vb[u]);
+                            }
+                        }
+                    }
+                }
+            }
+            unsigned int pk[8];
+            for (int c = 0; c < 8; ++c)
+                pk[c] = (unsigned int)f2bf(acc[2 * c]) | ((unsigned int)f2bf(acc[2 * c + 1]) << 16);
+            const u32x4 v0 = {pk[0], pk[1], pk[2], pk[3]}, v1 = {pk[4], pk[5], pk[6], pk[7]};
+            *(u32x4 *)&sX[nd * LDX + NNC + c0] = v0;
+            *(u32x4 *)&sX[nd * LDX + NNC + c0 + 8] = v1;
+            if (i < A.n && !A.st) {   // the aggregates are kept for the backward pass only
+                *(u32x4 *)(Mout + (size_t)i * NNC + c0) = v0;
+                *(u32x4 *)(Mout + (size_t)i * NNC + c0 + 8) = v1;
+            }
+        }
+        __syncthreads();
+        f32x16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 8; ++ks) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4 *)&sX[arow * LDX + ks * 16 + kh]);
+            const bf16x8 b = __builtin_bit_cast(bf16x8, *(const u32x4 *)&sW[bcol * LDX + ks * 16 + kh]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+        for (int v = 0; v < 16; ++v) {
+            const int row = rt * 32 + (v & 3) + 4 * (lane >> 5) + 8 * (v >> 2);
+            float h = acc[v] + bv;
+            h = h > 0.f ? h : 0.f;
+            sO[row * LDT + bcol] = f2bf(h);
+        }
+        __syncthreads();
+        for (int qq = t; qq < NNTILE * 8; qq += NNT) {   // coalesced 16-byte stores of the tile
+            const int nd = qq >> 3, c8 = (qq & 7) * 8;
+            const int i = tile0 + nd;
+            if (i < A.n) *(u32x4 *)(Hout + (size_t)i * NNC + c8) = *(const u32x4 *)&sO[nd * LDT + c8];
+        }
+        if (LL == 3) {   // y = h3 . w_out + b_out over channels in order
+            const int nd = t >> 2, part = t & 3;
+            float yv = 0.f;
+            if (part < 2) {
+                for (int c = part * 32; c < part * 32 + 32; ++c) yv = yv + bf2f(sO[nd * LDT + c]) * wo[c];
+                sY[part][nd] = yv;
+            }
+            __syncthreads();
+            if (t < NNTILE) {
+                const int i = tile0 + t;
+                if (i < A.n) {
+                    const float ys = (sY[0][t] + sY[1][t]) + A.theta[OFF_BOUT];
+                    A.y[i] = ys;
+                    const double r_ = rin[i];
+                    const double rt_ = A.isq[i] * r_;
+                    const double z = A.isq[i] * (rt_ + s * (double)ys);
+                    zout[i] = z;
+                    zr += z * r_;
+                    const double jz = A.isq[i] * A.isq[i] * r_;
+                    jr += jz * r_;
+                    if (q) {
+                        zq += z * q[i];
+                        jq += jz * q[i];
+                    }
                 }
             }
         }
-        unsigned int pk[8];
-        for (int c = 0; c < 8; ++c)
-            pk[c] = (unsigned int)f2bf(acc[2 * c]) | ((unsigned int)f2bf(acc[2 * c + 1]) << 16);
-        const u32x4 v0 = {pk[0], pk[1], pk[2], pk[3]}, v1 = {pk[4], pk[5], pk[6], pk[7]};
-        *(u32x4 *)&sX[nd * LDX + NNC + c0] = v0;
-        *(u32x4 *)&sX[nd * LDX + NNC + c0 + 8] = v1;
-        if (i < A.n) {
-            *(u32x4 *)(Mout + (size_t)i * NNC + c0) = v0;
-            *(u32x4 *)(Mout + (size_t)i * NNC + c0 + 8) = v1;
+        __syncthreads();
+    }
+    if (LL == 3 && A.st) {   // per-tile partials (ntiles of them) of z.r, z.q, r.D^-1 r, (D^-1 r).q
+        nn_bsum3(zr, zq, jr);
+        double d1 = 0.0, d2 = 0.0;
+        nn_bsum3(jq, d1, d2);
+        if (t == 0) {
+            A.tpart[blockIdx.x] = zr;
+            A.tpart[(size_t)A.ntiles + blockIdx.x] = zq;
+            A.tpart[(size_t)2 * A.ntiles + blockIdx.x] = jr;
+            A.tpart[(size_t)3 * A.ntiles + blockIdx.x] = jq;
         }
-    }
-    __syncthreads();
-    const int wv = t >> 6, lane = t & 63;
-    const int rt = wv >> 1, ct = wv & 1;
-    f32x16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const int arow = rt * 32 + (lane & 31), bcol = ct * 32 + (lane & 31), kh = (lane >> 5) * 8;
-    for (int ks = 0; ks < 8; ++ks) {
-        const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4 *)&sX[arow * LDX + ks * 16 + kh]);
-        const bf16x8 b = __builtin_bit_cast(bf16x8, *(const u32x4 *)&sW[bcol * LDX + ks * 16 + kh]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-    const float *bias = A.theta + OFF_B(l);
-    const float bv = bias[bcol];
-    for (int v = 0; v < 16; ++v) {
-        const int row = rt * 32 + (v & 3) + 4 * (lane >> 5) + 8 * (v >> 2);
-        const int i = tile0 + row;
-        float h = acc[v] + bv;
-        h = h > 0.f ? h : 0.f;
-        if (i < A.n) Hout[(size_t)i * NNC + bcol] = f2bf(h);
-    }
-}
-
-// output: y = h3 . w_out + b_out, z = D^-1/2 (r~ + s y); partial z.r (and z.q)
-__global__ void __launch_bounds__(NNT) k_nn_out(NNArgs A, const double *rin, double *zout, const double *q)
-{
-    if (nn_done(A)) return;
-    double s;
-    if (A.st) s = A.st->s;
-    else {
-        s = sqrt(nn_slot(A, SL_RTRT) / (double)A.n);
-        if (!(s > 0.0)) s = 1.0;
-    }
-    const bf16_t *H3 = A.H + (size_t)3 * A.n * NNC;
-    const float *wo = A.theta + OFF_WOUT;
-    const float bo = A.theta[OFF_BOUT];
-    double zr = 0.0, zq = 0.0, jr = 0.0, jq = 0.0, d2 = 0.0;
-    for (int i = blockIdx.x * NNT + threadIdx.x; i < A.n; i += A.nbd * NNT) {
-        float yv = 0.f;
-        for (int c8 = 0; c8 < NNC; c8 += 8) {
-            const u32x4 a = *(const u32x4 *)(H3 + (size_t)i * NNC + c8);
-            const unsigned int wd[4] = {a.x, a.y, a.z, a.w};
-            for (int c = 0; c < 4; ++c) {
-                yv = yv + bf2f((bf16_t)(wd[c] & 0xffffu)) * wo[c8 + 2 * c];
-                yv = yv + bf2f((bf16_t)(wd[c] >> 16)) * wo[c8 + 2 * c + 1];
-            }
-        }
-        yv = yv + bo;
-        A.y[i] = yv;
-        const double rt = A.isq[i] * rin[i];
-        const double z = A.isq[i] * (rt + s * (double)yv);
-        zout[i] = z;
-        zr += z * rin[i];
-        const double jz = A.isq[i] * A.isq[i] * rin[i];   // the diagonal preconditioner's z
-        jr += jz * rin[i];
-        if (q) {
-            zq += z * q[i];
-            jq += jz * q[i];
-        }
-    }
-    if (A.st) {
-        nn_put3(A, SL_ZR, zr, SL_ZQ, zq, SL_JR, jr);
-        nn_put3(A, SL_JQ, jq, -1, 0.0, -1, d2);
     }
 }
 
@@ -329,9 +401,10 @@ __global__ void __launch_bounds__(NNT) k_nn_loss(NNArgs A, const double *z, cons
 __global__ void __launch_bounds__(NNT) k_nn_out_bwd(NNArgs A, int batch)
 {
     __shared__ float sg[NNT / 8][NNC + 1];
-    const double xax = nn_slot(A, SL_XAX), eae = nn_slot(A, SL_EAE);
+    double xax, eae, rtrt;
+    nn_slots3(A.dpart, A.nbd, SL_XAX, SL_EAE, SL_RTRT, xax, eae, rtrt);
     if (blockIdx.x == 0 && threadIdx.x == 0) *A.loss += eae / xax / (double)batch;
-    double s = sqrt(nn_slot(A, SL_RTRT) / (double)A.n);
+    double s = sqrt(rtrt / (double)A.n);
     if (!(s > 0.0)) s = 1.0;
     const bf16_t *H3 = A.H + (size_t)3 * A.n * NNC;
     float *dH = A.dH;
@@ -559,10 +632,11 @@ __global__ void __launch_bounds__(NNT) k_nn_adam(float *theta, const float *g, f
 // flexible CG around the network
 // ------------------------------------------------------------------------------
 struct FCGArgs {
-    int n, nbd, maxiter;
+    int n, nbd, maxiter, ntiles;
+    const double *tpart;
     const int *rowptr, *colind;
     const double *val, *isq, *rhs;
-    double *x, *r, *z, *p, *q, *dpart;
+    double *x, *r, *z, *p, *p1, *q, *dpart;
     FCGState *st;
 };
 
@@ -593,6 +667,7 @@ __global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F)
         for (int p = F.rowptr[i]; p < F.rowptr[i + 1]; ++p) ax = ax + F.val[p] * F.x[F.colind[p]];
         const double r = F.rhs[i] - ax;
         F.r[i] = r;
+        F.p1[i] = 0.0;
         bb += F.rhs[i] * F.rhs[i];
         rr += r * r;
         const double rt = F.isq[i] * r;
@@ -605,43 +680,48 @@ __global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F)
     }
 }
 
-// p = z + beta p, beta = -(z.q)/(p.q) of the previous direction (p = z first).
+// Direction and operator product of iteration k = st->iter, one kernel:
+//   p_k = z_k + beta p_{k-1}, beta = -(z_k.q_{k-1}) / (p_{k-1}.q_{k-1}) (p_0 = z_0),
+//   q_k = A p_k, partial p_k.q_k.
 // If the network's z is not a descent direction (z.r <= 0), this iteration
 // takes the diagonal preconditioner's z = D^-1 r instead; flexible CG admits a
-// preconditioner that changes from one iteration to the next.
-__global__ void __launch_bounds__(NNT) k_fcg_p(FCGArgs F)
+// preconditioner that changes from one iteration to the next. p is double-
+// buffered by parity (p_k in P[k&1]) so that neighbours' p_k can be formed from
+// z and p_{k-1} while other blocks write p_k; p.q partials likewise.
+__global__ void __launch_bounds__(NNT) k_fcg_pa(FCGArgs F)
 {
     FCGState *st = F.st;
     if (st->done) return;
-    const bool first = st->iter == 0;
-    double zr = fcg_slot(F, SL_ZR), zq = fcg_slot(F, SL_ZQ);
-    const double jr = fcg_slot(F, SL_JR), jq = fcg_slot(F, SL_JQ);
+    const int it = st->iter, par = it & 1;
+    const bool first = it == 0;
+    double zr, zq, jr, jq, pq, u1, u2;
+    nn_slots3(F.tpart, F.ntiles, 0, 1, 2, zr, zq, jr);
+    nn_slots3(F.tpart, F.ntiles, 3, -1, -1, jq, u1, u2);
+    nn_slots3(F.dpart, F.nbd, first ? SL_BB : (par ? SL_PQ : SL_PQ1), -1, -1, pq, u1, u2);
     const bool jac = !(zr > 0.0);
     if (jac) { zr = jr; zq = jq; }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->zr = zr;
         if (jac) st->nneg += 1;
     }
-    double be = 0.0;
-    if (!first) be = -zq / fcg_slot(F, SL_PQ);
+    const double be = first ? 0.0 : -zq / pq;
+    const double *pold = par ? F.p : F.p1;
+    double *pnew = par ? F.p1 : F.p;
+    double pqn = 0.0;
     for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT) {
+        double sq = 0.0;
+        for (int p = F.rowptr[i]; p < F.rowptr[i + 1]; ++p) {
+            const int j = F.colind[p];
+            const double zj = jac ? F.isq[j] * F.isq[j] * F.r[j] : F.z[j];
+            sq = sq + F.val[p] * (first ? zj : zj + be * pold[j]);
+        }
         const double zi = jac ? F.isq[i] * F.isq[i] * F.r[i] : F.z[i];
-        F.p[i] = first ? zi : zi + be * F.p[i];
+        const double pi = first ? zi : zi + be * pold[i];
+        pnew[i] = pi;
+        F.q[i] = sq;
+        pqn += pi * sq;
     }
-}
-
-// q = A p, partial p.q
-__global__ void __launch_bounds__(NNT) k_fcg_a(FCGArgs F)
-{
-    if (F.st->done) return;
-    double pq = 0.0;
-    for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT) {
-        double s = 0.0;
-        for (int p = F.rowptr[i]; p < F.rowptr[i + 1]; ++p) s = s + F.val[p] * F.p[F.colind[p]];
-        F.q[i] = s;
-        pq += F.p[i] * s;
-    }
-    fcg_put(F, SL_PQ, pq, -1, 0.0, -1, 0.0);
+    fcg_put(F, par ? SL_PQ1 : SL_PQ, pqn, -1, 0.0, -1, 0.0);
 }
 
 // alpha = (z.r)/(p.q); x += alpha p; r -= alpha q; partials r.r, (D^-1/2 r)^2
@@ -649,7 +729,9 @@ __global__ void __launch_bounds__(NNT) k_fcg_b(FCGArgs F)
 {
     FCGState *st = F.st;
     if (st->done) return;
-    const double zr = st->zr, pq = fcg_slot(F, SL_PQ);
+    const int par = st->iter & 1;
+    const double zr = st->zr, pq = fcg_slot(F, par ? SL_PQ1 : SL_PQ);
+    const double *pk = par ? F.p1 : F.p;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (!(pq > 0.0) || !(zr > 0.0)) {   // breakdown: continue with plain CG
         if (lead) {
@@ -661,7 +743,7 @@ __global__ void __launch_bounds__(NNT) k_fcg_b(FCGArgs F)
     const double al = zr / pq;
     double rr = 0.0, tt = 0.0, d2 = 0.0;
     for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT) {
-        F.x[i] = F.x[i] + al * F.p[i];
+        F.x[i] = F.x[i] + al * pk[i];
         const double r = F.r[i] - al * F.q[i];
         F.r[i] = r;
         rr += r * r;
@@ -687,7 +769,7 @@ static NNArgs nn_args(fgpu_context *c, FCGState *st)
     A.w = d.w; A.wt = d.wt; A.isq = d.isq; A.theta = d.theta;
     A.Wf = d.Wf; A.Wb = d.Wb; A.H = d.H; A.M = d.M;
     A.u = d.u; A.y = d.y; A.dH = d.dH; A.dXs = d.dXs; A.dXm = d.dXm; A.part = d.part;
-    A.dpart = d.dpart; A.ae = d.ae; A.loss = d.loss;
+    A.dpart = d.dpart; A.ae = d.ae; A.loss = d.loss; A.tpart = d.tpart;
     A.st = st;
     return A;
 }
@@ -724,6 +806,7 @@ static int nn_setup(fgpu_context *c)
         {(void **)&d.y, sizeof(float) * n}, {(void **)&d.dH, sizeof(float) * (size_t)n * NNC},
         {(void **)&d.dXs, sizeof(float) * (size_t)n * NNC}, {(void **)&d.dXm, sizeof(float) * (size_t)n * NNC},
         {(void **)&d.part, sizeof(float) * (size_t)NNB * NNP}, {(void **)&d.dpart, sizeof(double) * NSLOT * d.nbd},
+        {(void **)&d.tpart, sizeof(double) * 4 * (size_t)d.ntiles},
         {(void **)&d.ae, sizeof(double) * n}, {(void **)&d.zs, sizeof(double) * n},
         {(void **)&d.loss, sizeof(double)}, {(void **)&d.st, sizeof(FCGState)}};
     for (auto &b : bufs)
@@ -781,10 +864,23 @@ static int nn_ready(fgpu_context *c)
 // forward of the network on rin -> zout (training mode: scale from k_nn_rms)
 static int nn_forward(fgpu_context *c, const NNArgs &A, const double *rin, double *zout, const double *q, double soltol)
 {
-    const dim3 gb(A.nbd), bb(NNT), gt(A.ntiles);
-    hipLaunchKernelGGL(k_nn_in, gb, bb, 0, c->stream, A, rin, soltol);
-    for (int l = 1; l <= 3; ++l) hipLaunchKernelGGL(k_nn_layer, gt, bb, 0, c->stream, A, l);
-    hipLaunchKernelGGL(k_nn_out, gb, bb, 0, c->stream, A, rin, zout, q);
+    const dim3 gt(A.ntiles), bb(NNT);
+    {
+        ProfScope ps(c, "k_nn_in");
+        hipLaunchKernelGGL(k_nn_in, dim3(A.nbd), bb, 0, c->stream, A, rin, soltol);
+    }
+    {
+        ProfScope ps(c, "k_nn_layer1");
+        hipLaunchKernelGGL(k_nn_layer<1>, gt, bb, 0, c->stream, A, rin, zout, q, soltol);
+    }
+    {
+        ProfScope ps(c, "k_nn_layer2");
+        hipLaunchKernelGGL(k_nn_layer<2>, gt, bb, 0, c->stream, A, rin, zout, q, soltol);
+    }
+    {
+        ProfScope ps(c, "k_nn_layer3");
+        hipLaunchKernelGGL(k_nn_layer<3>, gt, bb, 0, c->stream, A, rin, zout, q, soltol);
+    }
     FGPU_HIP(hipGetLastError());
     return FGPU_OK;
 }
@@ -957,9 +1053,10 @@ static FCGArgs fcg_args(fgpu_context *c, int maxiter)
 {
     NNDev &d = *c->nn;
     FCGArgs F;
-    F.n = d.n; F.nbd = d.nbd; F.maxiter = maxiter;
+    F.n = d.n; F.nbd = d.nbd; F.maxiter = maxiter; F.ntiles = d.ntiles; F.tpart = d.tpart;
     F.rowptr = c->ssh_rowptr; F.colind = c->ssh_colind; F.val = c->ssh_val; F.isq = d.isq; F.rhs = c->ssh_rhs;
-    F.x = c->ssh_x; F.r = c->ssh_r; F.z = c->ssh_z; F.p = c->ssh_p; F.q = c->ssh_q; F.dpart = d.dpart;
+    F.x = c->ssh_x; F.r = c->ssh_r; F.z = c->ssh_z; F.p = c->ssh_p; F.p1 = c->ssh_p1; F.q = c->ssh_q;
+    F.dpart = d.dpart;
     F.st = d.st;
     return F;
 }
@@ -967,9 +1064,14 @@ static FCGArgs fcg_args(fgpu_context *c, int maxiter)
 static int fcg_enqueue_iter(fgpu_context *c, const FCGArgs &F, const NNArgs &A, double soltol)
 {
     const dim3 gb(F.nbd), bb(NNT);
-    hipLaunchKernelGGL(k_fcg_p, gb, bb, 0, c->stream, F);
-    hipLaunchKernelGGL(k_fcg_a, gb, bb, 0, c->stream, F);
-    hipLaunchKernelGGL(k_fcg_b, gb, bb, 0, c->stream, F);
+    {
+        ProfScope ps(c, "k_fcg_pa");
+        hipLaunchKernelGGL(k_fcg_pa, gb, bb, 0, c->stream, F);
+    }
+    {
+        ProfScope ps(c, "k_fcg_b");
+        hipLaunchKernelGGL(k_fcg_b, gb, bb, 0, c->stream, F);
+    }
     return nn_forward(c, A, F.r, F.z, F.q, soltol);
 }
 
@@ -1004,15 +1106,26 @@ static int fcg_run(fgpu_context *c, double soltol, int maxiter, int fb_maxiter, 
     NNDev &d = *c->nn;
     FCGArgs F = fcg_args(c, maxiter);
     NNArgs A = nn_args(c, d.st);
-    int rc = fcg_graph(c, F, A, soltol);
-    if (rc) return rc;
+    int rc = FGPU_OK;
+    if (!c->prof && (rc = fcg_graph(c, F, A, soltol))) return rc;
     hipLaunchKernelGGL(k_fcg_init, dim3(F.nbd), dim3(NNT), 0, c->stream, F);
     if ((rc = nn_forward(c, A, F.r, F.z, nullptr, soltol))) return rc;
-    const int max_graphs = maxiter / FCG_GRAPH_ITERS + 2;
+    if (c->prof) {   // per-kernel timing: iterations launched one by one
+        FCGState hs;
+        for (int it = 0; it < maxiter + 1; it += FCG_GRAPH_ITERS) {
+            for (int i = 0; i < FCG_GRAPH_ITERS && !rc; ++i) rc = fcg_enqueue_iter(c, F, A, soltol);
+            if (rc) return rc;
+            FGPU_HIP(hipMemcpyAsync(&hs, d.st, sizeof(FCGState), hipMemcpyDeviceToHost, c->stream));
+            FGPU_HIP(hipStreamSynchronize(c->stream));
+            if (hs.done) break;
+        }
+        if ((rc = fgpu_prof_flush(c))) return rc;
+    }
+    const int max_graphs = c->prof ? 0 : maxiter / FCG_GRAPH_ITERS + 2;
     int batch = d.fcg_last_iters > 0 ? (d.fcg_last_iters + FCG_GRAPH_ITERS - 1) / FCG_GRAPH_ITERS : 2;
     int launched = 0;
     FCGState hs;
-    while (true) {
+    while (!c->prof) {
         for (int i = 0; i < batch && launched < max_graphs; ++i, ++launched)
             FGPU_HIP(hipGraphLaunch(d.fcg_graph, c->stream));
         FGPU_HIP(hipMemcpyAsync(&hs, d.st, sizeof(FCGState), hipMemcpyDeviceToHost, c->stream));
@@ -1020,6 +1133,7 @@ static int fcg_run(fgpu_context *c, double soltol, int maxiter, int fb_maxiter, 
         if (hs.done || launched >= max_graphs) break;
         batch = 1;
     }
+    FGPU_HIP(hipMemcpy(&hs, d.st, sizeof(FCGState), hipMemcpyDeviceToHost));
     if (!hs.done) return fgpu_fail(FGPU_ERR_STATE, "FCG iteration graph did not terminate");
     d.fcg_last_iters = hs.iter;
     res->iters = hs.iter;

@@ -113,18 +113,22 @@ __device__ __forceinline__ double row_dot(const int *rowptr, const int *colind, 
     return s;
 }
 
-// block tree reduction of up to three values; result valid in every thread
+// block reduction of up to three values: butterfly within each wave, then the
+// four wave sums in wave order; the result (lane 0's sums) is valid in every
+// thread and identical in every block for identical inputs
 __device__ void block_sum3(double &a, double &b, double &c)
 {
-    __shared__ double sh[3][SBLK];
-    const int t = threadIdx.x;
-    sh[0][t] = a; sh[1][t] = b; sh[2][t] = c;
-    __syncthreads();
-    for (int w = SBLK / 2; w > 0; w >>= 1) {
-        if (t < w) { sh[0][t] += sh[0][t + w]; sh[1][t] += sh[1][t + w]; sh[2][t] += sh[2][t + w]; }
-        __syncthreads();
+    __shared__ double sh[3][SBLK / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
     }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = c; }
+    __syncthreads();
     a = sh[0][0]; b = sh[1][0]; c = sh[2][0];
+    for (int i = 1; i < SBLK / 64; ++i) { a += sh[0][i]; b += sh[1][i]; c += sh[2][i]; }
     __syncthreads();
 }
 
@@ -367,7 +371,13 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
     CGState *st = A.st;
     if (st->done) return;
     const int it = st->iter, par = it & 1;
-    const double gam = cg1_slot(A, 2 * par), del = cg1_slot(A, 2 * par + 1), rr = cg1_slot(A, 4 + par);
+    double gam = 0.0, del = 0.0, rr = 0.0;
+    for (int j = threadIdx.x; j < A.nb; j += SBLK) {
+        gam += A.part[(size_t)(2 * par) * A.nb + j];
+        del += A.part[(size_t)(2 * par + 1) * A.nb + j];
+        rr += A.part[(size_t)(4 + par) * A.nb + j];
+    }
+    block_sum3(gam, del, rr);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     double rtol;
     if (it == 0) {
