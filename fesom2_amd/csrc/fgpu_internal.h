@@ -116,6 +116,7 @@ struct fgpu_context {
     bool fused = true;        // batched step: fused kernels (false: one kernel per reference stage)
     int step_tpb = 4096;      // tracers per block of the fused step kernels (clamped to the tracer count)
     int step_tch = 2;         // tracers per Thomas chunk of the fused step's last kernel (2 or 4)
+    int step_cpb = 1;         // columns per block of the fused step's last kernel (1, 4 or 8; 64-lane columns)
     int stage_slots = 0;      // tr_xy/aux slots allocated for the per-stage kernels (0: none yet)
     // profiling
     bool prof = false;

@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int
 // K_A: edge pass
 // ------------------------------------------------------------------------------
 template <int LB>
-__global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t, int tpb)
+__global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(6, 8))) kA_edge(DevMesh m, DevDyn d, DevTracers t, int tpb)
 {
     const int e = xcd_column(m.E2);
     if (e >= m.E2) return;
@@ -193,9 +193,12 @@ __global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t,
         v2 = d.uv[2 * q2 + 1];
         h2 = m.helem[q2];
     }
-    double vf[5];
-#pragma unroll
-    for (int s = 0; s < 5; ++s) vf[s] = in[s] ? seg_vflux(s, u1, v1, u2, v2, h1, h2, dX1, dY1, dX2, dY2) : 0.0;
+    // at most two segments meet at one level, in this order: A then B, or C, or
+    // D then E (A..E as in oce_adv_tra_hor.F90:166-249)
+    const int s1 = in[0] ? 0 : (in[2] ? 2 : (in[3] ? 3 : -1));
+    const int s2 = in[1] ? 1 : (in[4] ? 4 : -1);
+    const double vf1 = s1 >= 0 ? seg_vflux(s1, u1, v1, u2, v2, h1, h2, dX1, dY1, dX2, dY2) : 0.0;
+    const double vf2 = s2 >= 0 ? seg_vflux(s2, u1, v1, u2, v2, h1, h2, dX1, dY1, dX2, dY2) : 0.0;
     // gradient rule of fill_up_dn_grad at this level (oce_muscl_adv.F90:375-520);
     // writes in the reference's order, a later write wins
     bool shared = false, mean1 = false, mean2 = false;
@@ -224,9 +227,8 @@ __global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t,
         double flo = 0.0;
         const double T1 = T[o1], T2 = T[o2];
         if (lim == LIM_FCT) {
-#pragma unroll
-            for (int s = 0; s < 5; ++s)
-                if (in[s]) flo = upw1_h(T1, T2, vf[s], flo);
+            if (s1 >= 0) flo = upw1_h(T1, T2, vf1, flo);
+            if (s2 >= 0) flo = upw1_h(T1, T2, vf2, flo);
             t.lo_h[t.sE * tr + ei] = flo;
         }
         double A1, A2;
@@ -255,17 +257,15 @@ __global__ void __launch_bounds__(LB) kA_edge(DevMesh m, DevDyn d, DevTracers t,
             }
             const double Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
             const double Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
-#pragma unroll
-            for (int s = 0; s < 5; ++s)
-                if (in[s]) {
-                    const double vflux = vf[s];
-                    const double cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
-                    f = -0.5 * (1.0 - no) * cHO - vflux * no * 0.5 * (Tmean1 + Tmean2) - f;
-                }
+            for (int sg = 0; sg < 2; ++sg) {
+                if ((sg == 0 ? s1 : s2) < 0) continue;
+                const double vflux = sg == 0 ? vf1 : vf2;
+                const double cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
+                f = -0.5 * (1.0 - no) * cHO - vflux * no * 0.5 * (Tmean1 + Tmean2) - f;
+            }
         } else {
-#pragma unroll
-            for (int s = 0; s < 5; ++s)
-                if (in[s]) f = upw1_h(A1, A2, vf[s], f);
+            if (s1 >= 0) f = upw1_h(A1, A2, vf1, f);
+            if (s2 >= 0) f = upw1_h(A1, A2, vf2, f);
         }
         t.a_h[t.sE * tr + ei] = f;
     }
@@ -503,20 +503,31 @@ __device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &
     }
 }
 
-// TCH: tracers per Thomas chunk (one lane per tracer), a launch-shape choice
-template <int LB, int TCH>
-__global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb)
+// TCH: tracers per Thomas chunk; CPB: columns per block (one column per LB
+// lanes). The Thomas sweeps of all CPB x TCH (column, tracer) pairs of a block
+// run in one wave, one lane per pair, while the other waves wait: a serial
+// sweep then costs one wave's issue slots for CPB columns instead of one per
+// column. Both are launch-shape choices; the arithmetic is the same for all.
+template <int LB, int TCH, int CPB>
+__global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer,
+                                                    int tpb)
 {
-    const int n = xcd_column(m.N2);
-    if (n >= m.N2) return;
-    __shared__ double sRp[LB + 2], sRm[LB + 2], sAv[LB + 2];
-    __shared__ double sa[LB + 2], sb[LB + 2], sc[LB + 2], sZ[LB + 2];
-    __shared__ double sTs[TCH][LB + 2], stp[TCH][LB + 2], scp[TCH][LB + 2];
-    const int tid = threadIdx.x, k = tid + 1;
+    const int w = threadIdx.x / LB;
+    const int n0 = xcd_column(m.N2) * CPB + w;
+    const bool valid = n0 < m.N2;
+    const int n = valid ? n0 : m.N2 - 1;
+    __shared__ double sRpA[CPB][LB + 2], sRmA[CPB][LB + 2], sAvA[CPB][LB + 2];
+    __shared__ double saA[CPB][LB + 2], sbA[CPB][LB + 2], scA[CPB][LB + 2], sZA[CPB][LB + 2];
+    __shared__ double sTsA[CPB][TCH][LB + 2], stpA[CPB][TCH][LB + 2], scpA[CPB][TCH][LB + 2];
+    __shared__ int sNz[CPB][2];
+    double *sRp = sRpA[w], *sRm = sRmA[w], *sAv = sAvA[w];
+    double *sa = saA[w], *sb = sbA[w], *sc = scA[w], *sZ = sZA[w];
+    const int tid = threadIdx.x % LB, k = tid + 1;
     const int L = m.L, NL = m.NL;
-    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    const int nzmin = valid ? m.ulevn[n] : 1, nzmax = valid ? m.nlevn[n] : 0;
+    if (tid == 0) { sNz[w][0] = nzmin; sNz[w][1] = nzmax; }
     const bool act = (k >= nzmin && k <= nzmax - 1);
-    const size_t ci = (size_t)n * L + k - 1;
+    const size_t ci = (size_t)n * L + (k <= L ? k - 1 : L - 1);
     const NodeEdges E = load_node_edges(m, n);
     double hn = 1.0, hh = 0.0, av_ = 1.0, arv = 0.0;
     if (act) {
@@ -626,7 +637,7 @@ __global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t,
             if (act) {
                 double del = 0.0 + h + v;
                 del = del + Tk * (hh - hn);
-                sTs[c][tid] = Tk + del / hn;
+                sTsA[w][c][tid] = Tk + del / hn;
                 // valuesold rotation (oce_tracer_mod.F90:94-123)
                 if (abo == 2) {
                     Told[ci] = Tk;
@@ -642,43 +653,49 @@ __global__ void __launch_bounds__(LB) kD_node(DevMesh m, DevDyn d, DevTracers t,
             const int tr = t0 + c;
             if (!t.ivd[tr] || !act) continue;
             const double dz = hn;
+            const double *ts = sTsA[w][c];
             double r;
-            if (k == nzmin) r = -(sb[tid] - dz) * sTs[c][tid] - sc[tid] * sTs[c][tid + 1];
-            else if (k <= nzmax - 2) r = -sa[tid] * sTs[c][tid - 1] - (sb[tid] - dz) * sTs[c][tid] - sc[tid] * sTs[c][tid + 1];
-            else r = -sa[tid] * sTs[c][tid - 1] - (sb[tid] - dz) * sTs[c][tid];
-            if (k == nzmin) r = r + bc_surface_d(t.id[tr], dt, d, n, sTs[c][tid]);
-            stp[c][tid] = r;
+            if (k == nzmin) r = -(sb[tid] - dz) * ts[tid] - sc[tid] * ts[tid + 1];
+            else if (k <= nzmax - 2) r = -sa[tid] * ts[tid - 1] - (sb[tid] - dz) * ts[tid] - sc[tid] * ts[tid + 1];
+            else r = -sa[tid] * ts[tid - 1] - (sb[tid] - dz) * ts[tid];
+            if (k == nzmin) r = r + bc_surface_d(t.id[tr], dt, d, n, ts[tid]);
+            stpA[w][c][tid] = r;
         }
         __syncthreads();
-        // Thomas sweeps (oce_ale_tracer.F90:1406-1430), one lane per tracer; the
-        // pivot chain (cp, m) is evaluated in the same loop as tp so that the two
-        // recurrences overlap
-        if (tid < nch && t.ivd[t0 + tid]) {
-            const int c = tid;
-            double cp = sc[nzmin - 1] / sb[nzmin - 1];
-            double tp = stp[c][nzmin - 1] / sb[nzmin - 1];
-            scp[c][nzmin - 1] = cp;
-            stp[c][nzmin - 1] = tp;
-            for (int nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
-                const double an = sa[nz - 1];
-                const double mm = sb[nz - 1] - cp * an;
-                cp = sc[nz - 1] / mm;
-                tp = (stp[c][nz - 1] - tp * an) / mm;
-                scp[c][nz - 1] = cp;
-                stp[c][nz - 1] = tp;
-            }
-            double x = stp[c][nzmax - 2];
-            sTs[c][nzmax - 2] = sTs[c][nzmax - 2] + x;
-            for (int nz = nzmax - 2; nz >= nzmin; --nz) {
-                x = stp[c][nz - 1] - scp[c][nz - 1] * x;
-                sTs[c][nz - 1] = sTs[c][nz - 1] + x;
+        // Thomas sweeps (oce_ale_tracer.F90:1406-1430): one lane of the first wave
+        // per (column, tracer) pair; the pivot chain (cp, m) is evaluated in the
+        // same loop as tp so that the two recurrences overlap
+        if (threadIdx.x < CPB * nch) {
+            const int wc = threadIdx.x / nch, c = threadIdx.x % nch;
+            const int z0 = sNz[wc][0], z1 = sNz[wc][1];
+            if (z1 - 1 >= z0 && t.ivd[t0 + c]) {
+                const double *a_ = saA[wc], *b_ = sbA[wc], *c_ = scA[wc];
+                double *tp_ = stpA[wc][c], *cp_ = scpA[wc][c], *ts = sTsA[wc][c];
+                double cp = c_[z0 - 1] / b_[z0 - 1];
+                double tp = tp_[z0 - 1] / b_[z0 - 1];
+                cp_[z0 - 1] = cp;
+                tp_[z0 - 1] = tp;
+                for (int nz = z0 + 1; nz <= z1 - 1; ++nz) {
+                    const double an = a_[nz - 1];
+                    const double mm = b_[nz - 1] - cp * an;
+                    cp = c_[nz - 1] / mm;
+                    tp = (tp_[nz - 1] - tp * an) / mm;
+                    cp_[nz - 1] = cp;
+                    tp_[nz - 1] = tp;
+                }
+                double x = tp_[z1 - 2];
+                ts[z1 - 2] = ts[z1 - 2] + x;
+                for (int nz = z1 - 2; nz >= z0; --nz) {
+                    x = tp_[nz - 1] - cp_[nz - 1] * x;
+                    ts[nz - 1] = ts[nz - 1] + x;
+                }
             }
         }
         __syncthreads();
         if (act) {
             for (int c = 0; c < nch; ++c) {
                 const int tr = t0 + c;
-                double v = sTs[c][tid];
+                double v = sTsA[w][c][tid];
                 if (tr == clamp_tracer) {
                     if (v > 45.0) v = 45.0;
                     if (v < 3.0) v = 3.0;
@@ -735,6 +752,7 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     const int tpb = std::max(1, std::min(c->step_tpb, nt));
     const unsigned ny = (unsigned)((nt + tpb - 1) / tpb);
     const int tch = c->step_tch == 4 ? 4 : 2;
+    const int cpb = (c->step_cpb == 4 || c->step_cpb == 8) ? c->step_cpb : 1;   // columns per block of K_D (64-lane columns)
 #define LAUNCH5(LBV)                                                                                                   \
     {                                                                                                                  \
         {                                                                                                              \
@@ -755,10 +773,19 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
-            if (tch == 4)                                                                                              \
-                hipLaunchKernelGGL((kD_node<LBV, 4>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+            if (LBV == 64 && cpb == 8)                                                                                 \
+                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 8 : 1)>), dim3(xcd_grid((m.N2 + 7) / 8), ny),          \
+                                   dim3(LBV * (LBV == 64 ? 8 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
+            else if (LBV == 64 && cpb == 4 && tch == 4)                                                                \
+                hipLaunchKernelGGL((kD_node<LBV, 4, (LBV == 64 ? 4 : 1)>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
+                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
+            else if (LBV == 64 && cpb == 4)                                                                            \
+                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1)>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
+                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
+            else if (tch == 4)                                                                                         \
+                hipLaunchKernelGGL((kD_node<LBV, 4, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else                                                                                                       \
-                hipLaunchKernelGGL((kD_node<LBV, 2>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
         }                                                                                                              \
     }
     // one lane per level and per interface: LB >= NL

@@ -899,8 +899,8 @@ extern "C" int fgpu_solve_tracers_ale_timed(fgpu_context *c, double dt, int nste
 
 // Autotuner of the fused step's launch shape: each candidate number of tracers
 // per block (1, 2, 4, ... and all), crossed with 2 or 4 tracers per Thomas
-// chunk in the last kernel, is timed over `nsteps` replayed steps and the
-// fastest is kept. Every candidate computes the same arithmetic per column, so
+// chunk and 1, 4 or 8 columns per block in the last kernel, is timed over
+// `nsteps` replayed steps and the fastest is kept. Every candidate computes the same arithmetic per column, so
 // results do not depend on the choice. The tracer state is saved before and
 // restored after the trials.
 extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *best_tpb, double *best_ms)
@@ -923,23 +923,29 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     };
     FGPU_HIP(hipMemcpyAsync(save, c->t.T, sizeof(double) * nT, hipMemcpyDeviceToDevice, c->stream));
     FGPU_HIP(hipMemcpyAsync(save + nT, c->t.Told, sizeof(double) * nO, hipMemcpyDeviceToDevice, c->stream));
-    std::vector<std::pair<int, int>> cand;   // (tracers per block, tracers per Thomas chunk)
-    for (int tch : {2, 4}) {
-        if (tch == 4 && nt < 3) continue;
-        for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch});
-        cand.push_back({nt, tch});
+    struct Shape { int tpb, tch, cpb; };
+    std::vector<Shape> cand;
+    const bool narrow = c->m.NL <= 64;   // 64-lane columns: several per block are possible
+    for (int cpb : {1, 4, 8}) {
+        if (cpb > 1 && !narrow) continue;
+        for (int tch : {2, 4}) {
+            if ((tch == 4 && nt < 3) || (tch == 4 && cpb == 8)) continue;
+            for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch, cpb});
+            cand.push_back({nt, tch, cpb});
+        }
     }
     const bool was_prof = c->prof;
     c->prof = false;
-    int best = c->step_tpb, best_tch = c->step_tch;
+    int best = c->step_tpb, best_tch = c->step_tch, best_cpb = c->step_cpb;
     float best_t = -1.f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rc = FGPU_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fgpu_fail(FGPU_ERR_HIP, "hipEventCreate failed");
     for (size_t i = 0; i < cand.size() && !rc; ++i) {
-        c->step_tpb = cand[i].first;
-        c->step_tch = cand[i].second;
+        c->step_tpb = cand[i].tpb;
+        c->step_tch = cand[i].tch;
+        c->step_cpb = cand[i].cpb;
         c->graph_valid = false;
         if ((rc = run_steps(c, dt, 1))) break;   // capture and warm up
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord"); break; }
@@ -950,7 +956,12 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
             rc = fgpu_fail(FGPU_ERR_HIP, "timing the autotuner trial failed");
             break;
         }
-        if (best_t < 0.f || ms < best_t) { best_t = ms; best = cand[i].first; best_tch = cand[i].second; }
+        if (best_t < 0.f || ms < best_t) {
+            best_t = ms;
+            best = cand[i].tpb;
+            best_tch = cand[i].tch;
+            best_cpb = cand[i].cpb;
+        }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -959,6 +970,7 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     c->prof = was_prof;
     c->step_tpb = best;
     c->step_tch = best_tch;
+    c->step_cpb = best_cpb;
     c->graph_valid = false;
     if (rc) return rc;
     if (rr) return rr;

@@ -133,7 +133,8 @@ int fgpu_profile_count(fgpu_context *ctx);
 int fgpu_profile_read(fgpu_context *ctx, int idx, char *name, int name_len, double *total_ms, int *launches);
 /* Autotuner of the fused step (no counterpart in the reference): times the
  * candidate launch shapes (tracers per block 1, 2, 4, ..., all, crossed with 2
- * or 4 tracers per Thomas chunk) over nsteps replayed steps, keeps the fastest
+ * or 4 tracers per Thomas chunk and 1, 4 or 8 columns per block of the last
+ * kernel) over nsteps replayed steps, keeps the fastest
  * for this context and reports its tracers per block and time. The tracer
  * state is restored afterwards; results do not depend on the choice. */
 int fgpu_autotune_step(fgpu_context *ctx, double dt, int nsteps, int *best_tpb, double *best_ms_per_step);
