@@ -127,7 +127,9 @@ def main():
     ctx = gpu.context_for(mesh, state)
     dt = state.dt
     tpb, tune_ms = ctx.autotune_step(dt, 3)
-    print(f"[bench] autotuner: {tpb} tracers per block ({tune_ms:.3f} ms/step)", file=sys.stderr, flush=True)
+    shape = ctx.step_shape()
+    print(f"[bench] autotuner: {shape[0]} tracers per block, {shape[1]} per Thomas chunk, {shape[2]} columns per "
+          f"block ({tune_ms:.3f} ms/step)", file=sys.stderr, flush=True)
     ctx.solve_tracers_ale(dt, max(1, args.warmup))
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -194,7 +196,7 @@ def main():
         "ssh_solve_ms": round(sst["ms"], 3), "ssh_iters": sst["iters"], "ssh_converged": sst["converged"],
         "ssh_precond": "diagonal (Jacobi)",
         "device_ms_per_step": ms_dev / args.steps,
-        "autotuned_tracers_per_block": tpb,
+        "autotuned_shape": {"tracers_per_block": shape[0], "thomas_chunk": shape[1], "columns_per_block": shape[2]},
         "step_alg_GBs": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
         "step_model_bytes": int(step_model_bytes(mesh, state.ntr)),
         "step_model_frac_of_hbm": round(step_model_bytes(mesh, state.ntr) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),

@@ -146,7 +146,7 @@ extern "C" int fgpu_create(fgpu_context **out, int device)
         if (v >= 1 && v <= 4096) c->step_tpb = v;
     }
     if (const char *cv = std::getenv("FGPU_SSH_CG")) c->ssh_single_reduction = std::string(cv) != "classic";
-    if (const char *tv = std::getenv("FGPU_STEP_TCH")) c->step_tch = std::atoi(tv) == 4 ? 4 : 2;
+    if (const char *tv = std::getenv("FGPU_STEP_TCH")) c->step_tch = std::atoi(tv);
     if (const char *pv = std::getenv("FGPU_STEP_CPB")) c->step_cpb = std::atoi(pv);
     // default priority: stream priorities are left alone on purpose
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

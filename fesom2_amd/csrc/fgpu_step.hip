@@ -751,7 +751,7 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     // tracers per block: 1 runs every tracer's chain in its own waves
     const int tpb = std::max(1, std::min(c->step_tpb, nt));
     const unsigned ny = (unsigned)((nt + tpb - 1) / tpb);
-    const int tch = c->step_tch == 4 ? 4 : 2;
+    const int tch = (c->step_tch == 4 || c->step_tch == 8) ? c->step_tch : 2;
     const int cpb = (c->step_cpb == 4 || c->step_cpb == 8) ? c->step_cpb : 1;   // columns per block of K_D (64-lane columns)
 #define LAUNCH5(LBV)                                                                                                   \
     {                                                                                                                  \
@@ -782,6 +782,8 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
             else if (LBV == 64 && cpb == 4)                                                                            \
                 hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1)>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
                                    dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
+            else if (tch == 8)                                                                                         \
+                hipLaunchKernelGGL((kD_node<LBV, 8, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (tch == 4)                                                                                         \
                 hipLaunchKernelGGL((kD_node<LBV, 4, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else                                                                                                       \

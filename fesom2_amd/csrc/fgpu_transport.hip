@@ -898,7 +898,7 @@ extern "C" int fgpu_solve_tracers_ale_timed(fgpu_context *c, double dt, int nste
 }
 
 // Autotuner of the fused step's launch shape: each candidate number of tracers
-// per block (1, 2, 4, ... and all), crossed with 2 or 4 tracers per Thomas
+// per block (1, 2, 4, ... and all), crossed with 2, 4 or 8 tracers per Thomas
 // chunk and 1, 4 or 8 columns per block in the last kernel, is timed over
 // `nsteps` replayed steps and the fastest is kept. Every candidate computes the same arithmetic per column, so
 // results do not depend on the choice. The tracer state is saved before and
@@ -928,8 +928,9 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     const bool narrow = c->m.NL <= 64;   // 64-lane columns: several per block are possible
     for (int cpb : {1, 4, 8}) {
         if (cpb > 1 && !narrow) continue;
-        for (int tch : {2, 4}) {
-            if ((tch == 4 && nt < 3) || (tch == 4 && cpb == 8)) continue;
+        for (int tch : {2, 4, 8}) {
+            if ((tch == 4 && nt < 3) || (tch == 8 && nt < 5) || (tch >= 4 && cpb == 8) || (tch == 8 && cpb > 1))
+                continue;
             for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch, cpb});
             cand.push_back({nt, tch, cpb});
         }
@@ -976,6 +977,16 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     if (rr) return rr;
     if (best_tpb) *best_tpb = best;
     if (best_ms) *best_ms = best_t / nsteps;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_step_shape(fgpu_context *c, int *tracers_per_block, int *thomas_chunk, int *columns_per_block)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    const int nt = c->t.ntr > 0 ? c->t.ntr : 1;
+    if (tracers_per_block) *tracers_per_block = std::max(1, std::min(c->step_tpb, nt));
+    if (thomas_chunk) *thomas_chunk = (c->step_tch == 4 || c->step_tch == 8) ? c->step_tch : 2;
+    if (columns_per_block) *columns_per_block = c->m.NL <= 64 && (c->step_cpb == 4 || c->step_cpb == 8) ? c->step_cpb : 1;
     return FGPU_OK;
 }
 

@@ -31,7 +31,7 @@ module fesom_gpu_c
        fgpu_solve_tracers_ale_timed, fgpu_profile_enable, fgpu_profile_count, fgpu_profile_read, &
        fgpu_tracer_content, fgpu_init_stiff_mat_ale, fgpu_ssh_set_matrix, fgpu_ssh_get_matrix, &
        fgpu_ssh_solve_preconditioner, fgpu_ssh_solve_cg, fgpu_ssh_upload, fgpu_ssh_solve_resident, &
-       fgpu_ssh_download, fgpu_autotune_step, fgpu_ssh_nn_param_count, fgpu_ssh_nn_init, &
+       fgpu_ssh_download, fgpu_autotune_step, fgpu_step_shape, fgpu_ssh_nn_param_count, fgpu_ssh_nn_init, &
        fgpu_ssh_nn_set_weights, fgpu_ssh_nn_get_weights, fgpu_ssh_nn_apply, fgpu_ssh_nn_loss_grad, &
        fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident
 
@@ -299,6 +299,13 @@ module fesom_gpu_c
        integer(c_int), intent(out) :: best_tpb
        real(c_double), intent(out) :: best_ms
      end function fgpu_autotune_step
+
+     integer(c_int) function fgpu_step_shape(ctx, tracers_per_block, thomas_chunk, columns_per_block) &
+          bind(C, name="fgpu_step_shape")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), intent(out) :: tracers_per_block, thomas_chunk, columns_per_block
+     end function fgpu_step_shape
 
      integer(c_int) function fgpu_ssh_nn_param_count() bind(C, name="fgpu_ssh_nn_param_count")
        import :: c_int

@@ -67,6 +67,7 @@ _SIGS = {
     "fgpu_profile_read": [vp, C.c_int, C.c_char_p, C.c_int, dp, ip],
     "fgpu_tracer_content": [vp, C.c_int, dp],
     "fgpu_autotune_step": [vp, C.c_double, C.c_int, ip, dp],
+    "fgpu_step_shape": [vp, ip, ip, ip],
     "fgpu_init_stiff_mat_ale": [vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fgpu_ssh_set_matrix": [vp, C.c_int, C.c_int, ip, ip, dp],
     "fgpu_ssh_get_matrix": [vp, ip, ip, dp, dp, ip],
@@ -275,6 +276,12 @@ class Context:
         ms = C.c_double()
         _check(self.L.fgpu_autotune_step(self.h, float(dt), int(nsteps), C.byref(tpb), C.byref(ms)))
         return tpb.value, ms.value
+
+    def step_shape(self):
+        """(tracers per block, tracers per Thomas chunk, columns per block) of the fused step."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(self.L.fgpu_step_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def tracer_content(self, tr_num):
         v = C.c_double()

@@ -132,12 +132,15 @@ int fgpu_profile_enable(fgpu_context *ctx, int on);
 int fgpu_profile_count(fgpu_context *ctx);
 int fgpu_profile_read(fgpu_context *ctx, int idx, char *name, int name_len, double *total_ms, int *launches);
 /* Autotuner of the fused step (no counterpart in the reference): times the
- * candidate launch shapes (tracers per block 1, 2, 4, ..., all, crossed with 2
- * or 4 tracers per Thomas chunk and 1, 4 or 8 columns per block of the last
+ * candidate launch shapes (tracers per block 1, 2, 4, ..., all, crossed with 2,
+ * 4 or 8 tracers per Thomas chunk and 1, 4 or 8 columns per block of the last
  * kernel) over nsteps replayed steps, keeps the fastest
  * for this context and reports its tracers per block and time. The tracer
  * state is restored afterwards; results do not depend on the choice. */
 int fgpu_autotune_step(fgpu_context *ctx, double dt, int nsteps, int *best_tpb, double *best_ms_per_step);
+/* The fused step's current launch shape (no counterpart in the reference):
+ * tracers per block, tracers per Thomas chunk, columns per block of the last kernel. */
+int fgpu_step_shape(fgpu_context *ctx, int *tracers_per_block, int *thomas_chunk, int *columns_per_block);
 /* Tracer content sum_n sum_k areasvol*hnode_new*T (deterministic device sum). */
 int fgpu_tracer_content(fgpu_context *ctx, int tr_num, double *content);
 
