@@ -14,7 +14,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfesom_gpu.so")
-SOURCES = ["fgpu_runtime.hip", "fgpu_transport.hip", "fgpu_step.hip", "fgpu_ssh.hip", "fgpu_mpnn.hip"]
+SOURCES = ["fgpu_runtime.hip", "fgpu_transport.hip", "fgpu_step.hip", "fgpu_ssh.hip", "fgpu_mpnn.hip",
+           "fgpu_dyn.hip"]
 ARCH = os.environ.get("FGPU_ARCH", "gfx950")
 
 

@@ -144,6 +144,11 @@ struct fgpu_context {
     std::vector<int> h_ssh_rowptr, h_ssh_colind;
     // learned preconditioner (fgpu_mpnn.hip)
     struct NNDev *nn = nullptr;
+    // vert_vel_ale / compute_ssh_rhs_ale on the device (fgpu_dyn.hip)
+    double *vv_hbar = nullptr, *vv_hbar_old = nullptr, *vv_cfl = nullptr;
+    double *vv_uvr = nullptr, *vv_old = nullptr, *vv_ce = nullptr, *vv_rhs = nullptr;
+    void *vv_flags = nullptr;       // max CFL_z bits, NaN column count
+    bool wsplit_active = false;     // W split by fgpu_vert_vel_ale(use_wsplit=1): w_i != 0
 };
 
 // error helpers (fgpu_runtime.hip)

@@ -331,7 +331,8 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
             return fgpu_fail(FGPU_ERR_ARG, "element level numbers out of range");
     for (int n = 0; n < N2; ++n)
         if (ulevels_nod2D[n] < 1 || nlevels_nod2D[n] - ulevels_nod2D[n] < 2 || nlevels_nod2D[n] > nl ||
-            nlevels_nod2D_min[n] > nlevels_nod2D[n] || ulevels_nod2D_max[n] < ulevels_nod2D[n])
+            nlevels_nod2D_min[n] > nlevels_nod2D[n] || nlevels_nod2D_min[n] < ulevels_nod2D[n] + 1 ||
+            ulevels_nod2D_max[n] < ulevels_nod2D[n])
             return fgpu_fail(FGPU_ERR_ARG, "node level numbers out of range (columns need >= 2 layers)");
     // ---- per-edge level numbers
     std::vector<int> einfo(4 * (size_t)E2), erange(2 * (size_t)E2);
@@ -567,6 +568,7 @@ extern "C" int fgpu_set_velocity(fgpu_context *c, const double *uv, const double
     if ((rc = upload_or_refresh(c, &c->d.w_e, w_e, N2 * NL))) return rc;
     if ((rc = upload_or_refresh(c, &c->d.w_i, w_i, N2 * NL))) return rc;
     c->vel_ready = true;
+    c->wsplit_active = false;
     return FGPU_OK;
 }
 

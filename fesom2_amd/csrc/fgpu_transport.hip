@@ -670,6 +670,8 @@ static int check_ready(fgpu_context *c)
     if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
     if (!c->vert_ready) return fgpu_fail(FGPU_ERR_STATE, "vertical geometry not set (fgpu_set_vertical)");
     if (!c->vel_ready) return fgpu_fail(FGPU_ERR_STATE, "velocities not set (fgpu_set_velocity)");
+    if (c->wsplit_active)
+        return fgpu_fail(FGPU_ERR_ARG, "W was split (use_wsplit=.true.): adv_tra_vert_impl is not provided");
     if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
     FGPU_HIP(hipSetDevice(c->device));
     return FGPU_OK;

@@ -11,6 +11,8 @@ module fesom_gpu_c
 
   integer(c_int), parameter, public :: FGPU_OK = 0, FGPU_ERR_ARG = -1, FGPU_ERR_HIP = -2, &
        FGPU_ERR_SCHEME = -3, FGPU_ERR_BC = -4, FGPU_ERR_STATE = -5, FGPU_ERR_NOMEM = -6
+  ! which_ALE (g_config)
+  integer(c_int), parameter, public :: FGPU_ALE_LINFS = 0, FGPU_ALE_ZLEVEL = 1, FGPU_ALE_ZSTAR = 2
 
   ! solverinfo counters of one CG solve (MOD_DYN.F90:13-46)
   type, bind(C), public :: fgpu_cg_stats
@@ -33,7 +35,8 @@ module fesom_gpu_c
        fgpu_ssh_solve_preconditioner, fgpu_ssh_solve_cg, fgpu_ssh_upload, fgpu_ssh_solve_resident, &
        fgpu_ssh_download, fgpu_autotune_step, fgpu_step_shape, fgpu_ssh_nn_param_count, fgpu_ssh_nn_init, &
        fgpu_ssh_nn_set_weights, fgpu_ssh_nn_get_weights, fgpu_ssh_nn_apply, fgpu_ssh_nn_loss_grad, &
-       fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident
+       fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident, fgpu_set_uv, &
+       fgpu_vert_vel_ale, fgpu_vert_vel_download, fgpu_compute_ssh_rhs_ale
 
   interface
      type(c_ptr) function fgpu_last_error() bind(C, name="fgpu_last_error")
@@ -106,6 +109,40 @@ module fesom_gpu_c
             real_salt_flux(*)
        real(c_double), value :: is_nonlinfs
      end function fgpu_set_surface_forcing
+
+     integer(c_int) function fgpu_set_uv(ctx, uv) bind(C, name="fgpu_set_uv")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: uv(*)
+     end function fgpu_set_uv
+
+     integer(c_int) function fgpu_vert_vel_ale(ctx, dt, which_ale, hbar, hbar_old, use_wsplit, wsplit_maxcfl, &
+          cflmax, nan_columns) bind(C, name="fgpu_vert_vel_ale")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dt, wsplit_maxcfl
+       integer(c_int), value :: which_ale, use_wsplit
+       real(c_double), intent(in) :: hbar(*), hbar_old(*)
+       real(c_double), intent(out) :: cflmax
+       integer(c_int), intent(out) :: nan_columns
+     end function fgpu_vert_vel_ale
+
+     integer(c_int) function fgpu_vert_vel_download(ctx, w, w_e, w_i, cfl_z, hnode_new) &
+          bind(C, name="fgpu_vert_vel_download")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: w(*), w_e(*), w_i(*), cfl_z(*), hnode_new(*)
+     end function fgpu_vert_vel_download
+
+     integer(c_int) function fgpu_compute_ssh_rhs_ale(ctx, uv_rhs, ssh_rhs_old, alpha, which_ale, &
+          use_cavity_fw2press, ssh_rhs) bind(C, name="fgpu_compute_ssh_rhs_ale")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: uv_rhs(*), ssh_rhs_old(*)
+       real(c_double), value :: alpha
+       integer(c_int), value :: which_ale, use_cavity_fw2press
+       real(c_double), intent(out) :: ssh_rhs(*)
+     end function fgpu_compute_ssh_rhs_ale
 
      integer(c_int) function fgpu_tracers_init(ctx, num_tracers) bind(C, name="fgpu_tracers_init")
        import :: c_int, c_ptr

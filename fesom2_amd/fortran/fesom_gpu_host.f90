@@ -46,7 +46,8 @@ module fesom_gpu_host
        gpu_tracers_init, gpu_tracer_config, gpu_tracer_put, gpu_tracer_get, gpu_tracer_content, &
        init_tracers_AB, do_oce_adv_tra, diff_tracers_ale, diff_ver_part_impl_ale, solve_tracers_ale, &
        init_stiff_mat_ale, ssh_solve_preconditioner, ssh_solve_cg, solve_ssh_ale, gpu_synchronize, &
-       gpu_autotune_step, gpu_ssh_nn_init, gpu_ssh_nn_train, ssh_solve_cg_learned
+       gpu_autotune_step, gpu_ssh_nn_init, gpu_ssh_nn_train, ssh_solve_cg_learned, vert_vel_ale, &
+       gpu_vert_vel_ale, compute_ssh_rhs_ale
 
 contains
 
@@ -126,6 +127,67 @@ contains
     call gpu_check(fgpu_set_surface_forcing(ctx, heat_flux, water_flux, virtual_salt, relax_salt, &
          real_salt_flux, is_nonlinfs), 'fgpu_set_surface_forcing')
   end subroutine gpu_set_state
+
+  integer(c_int) function ale_code(which_ALE)
+    character(len=*), intent(in) :: which_ALE
+    select case (trim(which_ALE))
+    case ('linfs')
+       ale_code = FGPU_ALE_LINFS
+    case ('zlevel')
+       ale_code = FGPU_ALE_ZLEVEL
+    case ('zstar')
+       ale_code = FGPU_ALE_ZSTAR
+    case default
+       ale_code = -1
+    end select
+  end function ale_code
+
+  ! vert_vel_ale (oce_ale.F90:2107-2668) with compute_CFLz and compute_Wvel_split,
+  ! on the device from uv; the results stay resident for the transport.
+  ! hbar, hbar_old are read under zstar only.
+  subroutine gpu_vert_vel_ale(dt, which_ALE, uv, hbar, hbar_old, use_wsplit, wsplit_maxcfl, cflmax)
+    real(WP), intent(in) :: dt, wsplit_maxcfl
+    character(len=*), intent(in) :: which_ALE
+    real(WP), intent(in) :: uv(:, :, :), hbar(:), hbar_old(:)
+    logical, intent(in) :: use_wsplit
+    real(WP), intent(out) :: cflmax
+    integer(c_int) :: nan_columns, ws
+    ws = 0
+    if (use_wsplit) ws = 1
+    call gpu_check(fgpu_set_uv(ctx, uv), 'fgpu_set_uv')
+    call gpu_check(fgpu_vert_vel_ale(ctx, dt, ale_code(which_ALE), hbar, hbar_old, ws, wsplit_maxcfl, cflmax, &
+         nan_columns), 'vert_vel_ale')
+    if (nan_columns > 0) write(*,*) ' --> subroutine vert_vel_ale --> found Nan in Wvel in', nan_columns, 'columns'
+  end subroutine gpu_vert_vel_ale
+
+  ! the same, with Wvel, Wvel_e, Wvel_i, CFL_z (nl,nod2D) and hnode_new
+  ! (nl-1,nod2D) copied back as the reference's dynamics and mesh hold them
+  subroutine vert_vel_ale(dt, which_ALE, uv, hbar, hbar_old, use_wsplit, wsplit_maxcfl, Wvel, Wvel_e, Wvel_i, &
+       CFL_z, hnode_new)
+    real(WP), intent(in) :: dt, wsplit_maxcfl
+    character(len=*), intent(in) :: which_ALE
+    real(WP), intent(in) :: uv(:, :, :), hbar(:), hbar_old(:)
+    logical, intent(in) :: use_wsplit
+    real(WP), intent(out) :: Wvel(:, :), Wvel_e(:, :), Wvel_i(:, :), CFL_z(:, :), hnode_new(:, :)
+    real(WP) :: cflmax
+    call gpu_vert_vel_ale(dt, which_ALE, uv, hbar, hbar_old, use_wsplit, wsplit_maxcfl, cflmax)
+    call gpu_check(fgpu_vert_vel_download(ctx, Wvel, Wvel_e, Wvel_i, CFL_z, hnode_new), 'fgpu_vert_vel_download')
+  end subroutine vert_vel_ale
+
+  ! compute_ssh_rhs_ale (oce_ale.F90:1803-1938) on the device from uv and
+  ! uv_rhs; ssh_rhs also becomes the SSH solve's right-hand side
+  subroutine compute_ssh_rhs_ale(uv, uv_rhs, ssh_rhs_old, alpha, which_ALE, use_cavity_fw2press, ssh_rhs)
+    real(WP), intent(in) :: uv(:, :, :), uv_rhs(:, :, :), ssh_rhs_old(:), alpha
+    character(len=*), intent(in) :: which_ALE
+    logical, intent(in) :: use_cavity_fw2press
+    real(WP), intent(out) :: ssh_rhs(:)
+    integer(c_int) :: cav
+    cav = 0
+    if (use_cavity_fw2press) cav = 1
+    call gpu_check(fgpu_set_uv(ctx, uv), 'fgpu_set_uv')
+    call gpu_check(fgpu_compute_ssh_rhs_ale(ctx, uv_rhs, ssh_rhs_old, alpha, ale_code(which_ALE), cav, ssh_rhs), &
+         'compute_ssh_rhs_ale')
+  end subroutine compute_ssh_rhs_ale
 
   subroutine gpu_tracers_init(num_tracers)
     integer, intent(in) :: num_tracers

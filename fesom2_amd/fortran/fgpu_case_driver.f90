@@ -4,6 +4,8 @@
 ! the tracers through the reference-shaped per-tracer sequence
 ! (init_tracers_AB, do_oce_adv_tra, diff_tracers_ale; oce_ale_tracer.F90:280-385)
 ! for nsingle steps and through the batched solve_tracers_ale for nbatched more,
+! runs vert_vel_ale (linfs) and compute_ssh_rhs_ale (alpha = 1/2, uv_rhs = uv,
+! ssh_rhs_old = the case's ssh_rhs) on the transported state's velocity,
 ! assembles the SSH operator and solves it with solve_ssh_ale, then writes the
 ! results to a second stream file.
 !
@@ -28,6 +30,7 @@
 !   int32 status(8): iters_last, converged (1/0), nbreakdown, nnegrz, nonconv, 0, 0, 0
 !   real  resid_last, rtol_last, content_before(ntr), content_after(ntr)
 !   values(nl-1,N2,ntr) valuesold(nl-1,N2,ntr) d_eta(N2)
+!   Wvel(nl,N2) Wvel_e(nl,N2) Wvel_i(nl,N2) CFL_z(nl,N2) ssh_rhs_dyn(N2)
 program fgpu_case_driver
   use, intrinsic :: iso_c_binding
   use fesom_gpu_c, only: fgpu_version
@@ -51,6 +54,8 @@ program fgpu_case_driver
   real(WP), allocatable :: uv(:, :, :), w(:, :), w_e(:, :), w_i(:, :), Kv(:, :)
   real(WP), allocatable :: heat_flux(:), water_flux(:), virtual_salt(:), relax_salt(:), real_salt_flux(:)
   real(WP), allocatable :: values(:, :, :), valuesold(:, :, :), ssh_rhs(:), d_eta(:), c0(:), c1(:)
+  real(WP), allocatable :: wv(:, :), wv_e(:, :), wv_i(:, :), cfl_z(:, :), hn_new(:, :), rhs_dyn(:), hbar(:)
+  real(WP) :: cflmax
   type(t_gpu_solverinfo) :: si
 
   call get_command_argument(1, fin)
@@ -144,6 +149,14 @@ program fgpu_case_driver
      c1(tr) = gpu_tracer_content(tr)
   end do
 
+  ! the steps either side of the transport (oce_ale.F90:1803-1938, 2107-3049)
+  allocate(wv(nl, N2), wv_e(nl, N2), wv_i(nl, N2), cfl_z(nl, N2), hn_new(nl - 1, N2), rhs_dyn(N2), hbar(N2))
+  hbar = 0.0_WP
+  call vert_vel_ale(dt, 'linfs', uv, hbar, hbar, .false., 1.0_WP, wv, wv_e, wv_i, cfl_z, hn_new)
+  call compute_ssh_rhs_ale(uv, uv, ssh_rhs, 0.5_WP, 'linfs', .false., rhs_dyn)
+  write(*, '(a,es10.3)') '[driver] vert_vel_ale max |W|=', maxval(abs(wv))
+  flush(6)
+
   ! SSH: operator, lazy preconditioner, CG from d_eta = 0
   call init_stiff_mat_ale(dt, g, 1.0_WP, 1.0_WP)
   d_eta = 0.0_WP
@@ -160,6 +173,6 @@ program fgpu_case_driver
   ostat(4) = si%nnegrz
   ostat(5) = si%nonconv
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
-  write(u) ostat, si%resid_last, si%rtol_last, c0, c1, values, valuesold, d_eta
+  write(u) ostat, si%resid_last, si%rtol_last, c0, c1, values, valuesold, d_eta, wv, wv_e, wv_i, cfl_z, rhs_dyn
   close(u)
 end program fgpu_case_driver

@@ -49,6 +49,10 @@ _SIGS = {
     "fgpu_set_velocity": [vp, dp, dp, dp, dp, C.c_int],
     "fgpu_set_kv": [vp, dp],
     "fgpu_set_surface_forcing": [vp, dp, dp, dp, dp, dp, C.c_double],
+    "fgpu_set_uv": [vp, dp],
+    "fgpu_vert_vel_ale": [vp, C.c_double, C.c_int, dp, dp, C.c_int, C.c_double, dp, ip],
+    "fgpu_vert_vel_download": [vp, dp, dp, dp, dp, dp],
+    "fgpu_compute_ssh_rhs_ale": [vp, dp, dp, C.c_double, C.c_int, C.c_int, dp],
     "fgpu_tracers_init": [vp, C.c_int],
     "fgpu_tracer_config": [vp, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
                            C.c_double, C.c_int],
@@ -201,6 +205,37 @@ class Context:
         self.set_velocity(st.uv, st.w, st.w_e, st.w_i)
         self.set_kv(st.Kv)
         self.set_surface_forcing(st.heat_flux, st.water_flux, st.virtual_salt, st.relax_salt, st.real_salt_flux)
+
+    # ------------------------------------------- dynamics either side of the transport
+    ALE = {"linfs": 0, "zlevel": 1, "zstar": 2}
+
+    def set_uv(self, uv):
+        _check(self.L.fgpu_set_uv(self.h, _d(_f64(uv))))
+
+    def vert_vel_ale(self, dt, which_ale="linfs", hbar=None, hbar_old=None, use_wsplit=False, wsplit_maxcfl=1.0):
+        """W, w_e, w_i, CFL_z on the device from the resident uv; returns (cflmax, nan_columns)."""
+        cmax = C.c_double()
+        nan = C.c_int()
+        hb = None if hbar is None else _f64(hbar)
+        ho = None if hbar_old is None else _f64(hbar_old)
+        _check(self.L.fgpu_vert_vel_ale(self.h, float(dt), self.ALE[which_ale], _d(hb) if hb is not None else None,
+                                        _d(ho) if ho is not None else None, int(bool(use_wsplit)),
+                                        float(wsplit_maxcfl), C.byref(cmax), C.byref(nan)))
+        return cmax.value, nan.value
+
+    def vert_vel_download(self):
+        NL = self.nl
+        out = {k: np.zeros((self.N2, NL)) for k in ("w", "w_e", "w_i", "cfl_z")}
+        out["hnode_new"] = np.zeros((self.N2, NL - 1))
+        _check(self.L.fgpu_vert_vel_download(self.h, _d(out["w"]), _d(out["w_e"]), _d(out["w_i"]),
+                                             _d(out["cfl_z"]), _d(out["hnode_new"])))
+        return out
+
+    def compute_ssh_rhs_ale(self, uv_rhs, ssh_rhs_old, alpha=1.0, which_ale="linfs", use_cavity_fw2press=False):
+        out = np.zeros(self.N2)
+        _check(self.L.fgpu_compute_ssh_rhs_ale(self.h, _d(_f64(uv_rhs)), _d(_f64(ssh_rhs_old)), float(alpha),
+                                               self.ALE[which_ale], int(bool(use_cavity_fw2press)), _d(out)))
+        return out
 
     # ------------------------------------------------------------ tracers
     def tracers_init(self, ntr):

@@ -88,6 +88,42 @@ int fgpu_set_surface_forcing(fgpu_context *ctx, const double *heat_flux, const d
                              const double *virtual_salt, const double *relax_salt,
                              const double *real_salt_flux, double is_nonlinfs);
 
+/* ---- the steps either side of the transport -------------------------------
+ * which_ALE of g_config (gen_modules_config.F90): */
+#define FGPU_ALE_LINFS 0
+#define FGPU_ALE_ZLEVEL 1 /* not provided: fgpu_vert_vel_ale rejects it */
+#define FGPU_ALE_ZSTAR 2
+
+/* dynamics%uv (2,nl-1,elem2D) alone, for fgpu_vert_vel_ale and
+ * fgpu_compute_ssh_rhs_ale (fgpu_set_velocity also sets it) */
+int fgpu_set_uv(fgpu_context *ctx, const double *uv);
+
+/* vert_vel_ale (oce_ale.F90:2107-2668) followed by compute_CFLz
+ * (oce_ale.F90:2910-2997) and compute_Wvel_split (oce_ale.F90:3003-3049), on
+ * the device from the resident uv: dynamics%w, w_e, w_i and cfl_z stay in HBM
+ * for the transport (fgpu_vert_vel_download copies them out). which_ALE linfs
+ * or zstar; zstar also reads hbar, hbar_old (nod2D, host) and the water flux
+ * of fgpu_set_surface_forcing, and updates hnode_new and the implicit
+ * diffusion's Z levels. Levels outside [ulevels_nod2D, nlevels_nod2D] of w_e
+ * and w_i are written as zero. With use_wsplit = 1 the split follows the
+ * reference, but the transport then refuses to step (adv_tra_vert_impl is not
+ * provided). cflmax: max of CFL_z (may be NULL); nan_columns: columns with a
+ * NaN in W, which the reference reports (may be NULL). */
+int fgpu_vert_vel_ale(fgpu_context *ctx, double dt, int which_ale, const double *hbar, const double *hbar_old,
+                      int use_wsplit, double wsplit_maxcfl, double *cflmax, int *nan_columns);
+/* w, w_e, w_i, cfl_z (nl,nod2D) and hnode_new (nl-1,nod2D); NULL skips one */
+int fgpu_vert_vel_download(fgpu_context *ctx, double *w, double *w_e, double *w_i, double *cfl_z,
+                           double *hnode_new);
+
+/* compute_ssh_rhs_ale (oce_ale.F90:1803-1938) from the resident uv and
+ * uv_rhs (2,nl-1,elem2D), ssh_rhs_old (nod2D) and alpha; the full free surface
+ * (which_ale != FGPU_ALE_LINFS) also reads the water flux of
+ * fgpu_set_surface_forcing. The result becomes the SSH solve's right-hand side
+ * when the SSH system is set up (fgpu_ssh_solve_resident then solves it) and is
+ * copied to ssh_rhs (nod2D) unless that is NULL. */
+int fgpu_compute_ssh_rhs_ale(fgpu_context *ctx, const double *uv_rhs, const double *ssh_rhs_old, double alpha,
+                             int which_ale, int use_cavity_fw2press, double *ssh_rhs);
+
 /* ---- tracers: t_tracer (MOD_TRACER.F90:10-109) ---------------------------
  * num_tracers tracers; each configured with its namelist entry
  * (config/namelist.tra:33-38): hor in {"MFCT","UPW1"}, ver in {"QR4C","UPW1"},

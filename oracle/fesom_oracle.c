@@ -26,6 +26,10 @@
  *   16 preconditioner variant 1 off-diagonal sign flipped (ssh_solve_preconditioner)
  *   17 upw1 vertical flux uses T(k) on both sides         (adv_tra_ver_upw1)
  *   18 FCT b1 drops the vertical contribution to P+/P-    (oce_tra_adv_fct)
+ *   19 second-element transport added with the first's sign (vert_vel_ale)
+ *   20 W accumulated from the top instead of the bottom   (vert_vel_ale)
+ *   21 SSH RHS drops helem in the first element's term    (compute_ssh_rhs_ale)
+ *   22 CFL_z below an interface uses W of the interface above (compute_CFLz)
  */
 #include "fesom_oracle.h"
 
@@ -943,4 +947,151 @@ int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *v
     st->rtol_last = rtol;
     free(rr); free(zz); free(pp); free(App);
     return converged ? 0 : 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* vert_vel_ale: oce_ale.F90:2107-2668, which_ALE = 'linfs' (0) or 'zstar'  */
+/* (2); then compute_CFLz (oce_ale.F90:2910-2997) and compute_Wvel_split    */
+/* (oce_ale.F90:3003-3049). 'zlevel' (1) is not restated.                   */
+/* Outputs (N2*NL): w, w_e, w_i, cfl_z. hnode_new (N2*L) holds the current  */
+/* thicknesses on entry and is updated in place under zstar.               */
+/* ------------------------------------------------------------------------ */
+int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale, const double *hbar,
+                    const double *hbar_old, const double *water_flux, int use_wsplit, double wsplit_maxcfl,
+                    double *w, double *w_e, double *w_i, double *cfl_z, double *hnode_new, double *cflmax)
+{
+    const int L = m->nl - 1, NL = m->nl, N2 = m->nod2D;
+    if (which_ale != 0 && which_ale != 2) return -1;
+    if (which_ale == 2 && (!hbar || !hbar_old || !water_flux)) return -2;
+    /* :2164-2171 */
+    memset(w, 0, sizeof(double) * (size_t)N2 * NL);
+    /* thickness divergence, edges ascending (:2175-2265): each element of
+       the edge adds its layer transports to enodes(1) and subtracts them
+       from enodes(2), el(1) before el(2) */
+    for (int ed = 0; ed < m->edge2D; ++ed) {
+        const int n1 = m->edges[2 * ed], n2 = m->edges[2 * ed + 1];
+        for (int s = 0; s < 2; ++s) {
+            const int el = m->edge_tri[2 * ed + s];
+            if (el < 0) continue;
+            const double dX = m->edge_cross_dxdy[4 * ed + 2 * s], dY = m->edge_cross_dxdy[4 * ed + 2 * s + 1];
+            const int nzmin = m->ulevels[el], nzmax = m->nlevels[el] - 1;
+            for (int nz = nzmin; nz <= nzmax; ++nz) {
+                const double *u = uv + ((size_t)el * L + nz - 1) * 2;
+                double c1 = (u[1] * dX - u[0] * dY) * m->helem[LAY(el, nz)];
+                if (s == 1 && g_mut != 19) c1 = -c1;
+                w[INT(n1, nz)] = w[INT(n1, nz)] + c1;
+                w[INT(n2, nz)] = w[INT(n2, nz)] - c1;
+            }
+        }
+    }
+    /* cumulative sum from the bottom (:2277-2299), then divide by the
+       interface area (:2301-2320) */
+    for (int n = 0; n < N2; ++n) {
+        const int nzmin = m->ulevels_nod2D[n], nzmax = m->nlevels_nod2D[n] - 1;
+        if (g_mut == 20) {
+            for (int nz = nzmin + 1; nz <= nzmax; ++nz) w[INT(n, nz)] = w[INT(n, nz)] + w[INT(n, nz - 1)];
+        } else {
+            for (int nz = nzmax; nz >= nzmin; --nz) w[INT(n, nz)] = w[INT(n, nz)] + w[INT(n, nz + 1)];
+        }
+        for (int nz = nzmin; nz <= nzmax; ++nz) w[INT(n, nz)] = w[INT(n, nz)] / m->area[INT(n, nz)];
+    }
+    /* zstar: distribute hbar - hbar_old over the layers above
+       nlevels_nod2D_min (:2539-2600); cavity columns stay linfs */
+    if (which_ale == 2) {
+        for (int n = 0; n < N2; ++n) {
+            const int nzmin = m->ulevels_nod2D[n], nzmax = m->nlevels_nod2D_min[n] - 1;
+            if (nzmin != 1) continue;
+            const double dd1 = m->zbar_3d_n[INT(n, nzmax)];
+            double dd = m->zbar_3d_n[INT(n, nzmin)] - dd1;
+            dd = (hbar[n] - hbar_old[n]) / dd;
+            const double dddt = dd / dt;
+            for (int nz = nzmin; nz <= nzmax - 1; ++nz) {
+                w[INT(n, nz)] = w[INT(n, nz)] - (m->zbar_3d_n[INT(n, nz)] - dd1) * dddt;
+                hnode_new[LAY(n, nz)] = m->hnode[LAY(n, nz)] + (m->zbar_3d_n[INT(n, nz)] - m->zbar_3d_n[INT(n, nz + 1)]) * dd;
+            }
+            w[INT(n, nzmin)] = w[INT(n, nzmin)] - water_flux[n];
+        }
+    }
+    /* compute_CFLz (:2938-2966): interface nz collects |W| dt / h of the
+       layers below (c1) and above (c2) it */
+    memset(cfl_z, 0, sizeof(double) * (size_t)N2 * NL);
+    double cmax = 0.0;
+    for (int n = 0; n < N2; ++n) {
+        const int nzmin = m->ulevels_nod2D[n], nzmax = m->nlevels_nod2D[n] - 1;
+        for (int nz = nzmin; nz <= nzmax; ++nz) {
+            const double c1 = fabs(w[INT(n, nz)] * dt / hnode_new[LAY(n, nz)]);
+            const double c2 = fabs(w[INT(n, g_mut == 22 ? nz : nz + 1)] * dt / hnode_new[LAY(n, nz)]);
+            cfl_z[INT(n, nz)] = cfl_z[INT(n, nz)] + c1;
+            cfl_z[INT(n, nz + 1)] = c2;
+        }
+        for (int k = 1; k <= NL; ++k) cmax = dmax(cmax, cfl_z[INT(n, k)]);
+    }
+    if (cflmax) *cflmax = cmax;
+    /* compute_Wvel_split (:3035-3047) */
+    for (int n = 0; n < N2; ++n) {
+        const int nzmin = m->ulevels_nod2D[n], nzmax = m->nlevels_nod2D[n];
+        for (int k = 1; k <= NL; ++k) { w_e[INT(n, k)] = 0.0; w_i[INT(n, k)] = 0.0; }
+        for (int nz = nzmin; nz <= nzmax; ++nz) {
+            const size_t i = INT(n, nz);
+            w_e[i] = w[i];
+            w_i[i] = 0.0;
+            if (use_wsplit && (cfl_z[i] > wsplit_maxcfl)) {
+                const double dd = dmax((cfl_z[i] - wsplit_maxcfl), 0.0) / dmax(wsplit_maxcfl, 1.e-12);
+                w_e[i] = (1.0 / (1.0 + dd)) * w[i];
+                w_i[i] = (dd / (1.0 + dd)) * w[i];
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_ssh_rhs_ale: oce_ale.F90:1803-1938. Depth-integrated transport of */
+/* u + uv_rhs through the two dual faces of each edge, summed over levels    */
+/* first (:1844-1880) and then into the edge's nodes, edges ascending        */
+/* (:1889-1894); then the free-surface terms (:1913-1934).                   */
+/* which_ale 0 = linfs, otherwise the full free surface branch.              */
+/* ------------------------------------------------------------------------ */
+int fo_compute_ssh_rhs_ale(const fo_mesh *m, const double *uv, const double *uv_rhs, const double *ssh_rhs_old,
+                           const double *water_flux, double alpha, int which_ale, int use_cavity_fw2press,
+                           double *ssh_rhs)
+{
+    const int L = m->nl - 1, NL = m->nl, N2 = m->nod2D;
+    for (int n = 0; n < N2; ++n) ssh_rhs[n] = 0.0;
+    for (int ed = 0; ed < m->edge2D; ++ed) {
+        const int n1 = m->edges[2 * ed], n2 = m->edges[2 * ed + 1];
+        const int el1 = m->edge_tri[2 * ed], el2 = m->edge_tri[2 * ed + 1];
+        double c1 = 0.0, c2 = 0.0;
+        {
+            const double dX = m->edge_cross_dxdy[4 * ed], dY = m->edge_cross_dxdy[4 * ed + 1];
+            for (int nz = m->ulevels[el1]; nz <= m->nlevels[el1] - 1; ++nz) {
+                const size_t i = ((size_t)el1 * L + nz - 1) * 2;
+                const double h = g_mut == 21 ? 1.0 : m->helem[LAY(el1, nz)];
+                c1 = c1 + alpha * ((uv[i + 1] + uv_rhs[i + 1]) * dX - (uv[i] + uv_rhs[i]) * dY) * h;
+            }
+        }
+        if (el2 >= 0) {
+            const double dX = m->edge_cross_dxdy[4 * ed + 2], dY = m->edge_cross_dxdy[4 * ed + 3];
+            for (int nz = m->ulevels[el2]; nz <= m->nlevels[el2] - 1; ++nz) {
+                const size_t i = ((size_t)el2 * L + nz - 1) * 2;
+                c2 = c2 - alpha * ((uv[i + 1] + uv_rhs[i + 1]) * dX - (uv[i] + uv_rhs[i]) * dY) * m->helem[LAY(el2, nz)];
+            }
+        }
+        ssh_rhs[n1] = ssh_rhs[n1] + (c1 + c2);
+        ssh_rhs[n2] = ssh_rhs[n2] - (c1 + c2);
+    }
+    if (which_ale != 0) {
+        for (int n = 0; n < N2; ++n) {
+            const int nzmin = m->ulevels_nod2D[n];
+            if (nzmin > 1) {
+                if (use_cavity_fw2press) ssh_rhs[n] = ssh_rhs[n] - alpha * water_flux[n] * m->areasvol[INT(n, nzmin)];
+            } else {
+                ssh_rhs[n] = ssh_rhs[n] - alpha * water_flux[n] * m->areasvol[INT(n, nzmin)] + (1.0 - alpha) * ssh_rhs_old[n];
+            }
+        }
+    } else {
+        for (int n = 0; n < N2; ++n) ssh_rhs[n] = ssh_rhs[n] + (1.0 - alpha) * ssh_rhs_old[n];
+    }
+    (void)NL;
+    return 0;
 }

@@ -108,6 +108,15 @@ int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *v
                     const double *pr_values, double *x, const double *rhs, double soltol,
                     int maxiter, fo_cg_stats *st);
 
+/* vert_vel_ale for which_ale 0 (linfs) or 2 (zstar), with compute_CFLz and
+ * compute_Wvel_split; hnode_new is updated in place under zstar */
+int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale, const double *hbar,
+                    const double *hbar_old, const double *water_flux, int use_wsplit, double wsplit_maxcfl,
+                    double *w, double *w_e, double *w_i, double *cfl_z, double *hnode_new, double *cflmax);
+int fo_compute_ssh_rhs_ale(const fo_mesh *m, const double *uv, const double *uv_rhs, const double *ssh_rhs_old,
+                           const double *water_flux, double alpha, int which_ale, int use_cavity_fw2press,
+                           double *ssh_rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,9 @@ def lib():
         L.fo_solve_tracers_ale.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
                                            C.POINTER(FoTracer), C.c_int, C.POINTER(FoWork)]
         L.fo_init_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, ip, ip, dp, C.c_int]
+        L.fo_vert_vel_ale.argtypes = [C.POINTER(FoMesh), dp, C.c_double, C.c_int, dp, dp, dp, C.c_int,
+                                      C.c_double, dp, dp, dp, dp, dp, dp]
+        L.fo_compute_ssh_rhs_ale.argtypes = [C.POINTER(FoMesh), dp, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp]
         L.fo_ssh_solve_preconditioner.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp]
         L.fo_ssh_solve_cg.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_int,
                                       C.POINTER(FoCgStats)]
@@ -276,6 +279,40 @@ class Oracle:
         out = np.zeros((self.mesh.nod2D, self.mesh.nl))
         lib().fo_adv_tra_ver_qr4c(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
                                   float(num_ord), _d(out), 1)
+        return out
+
+    # ------------------------------------------------- dynamics either side
+    def vert_vel_ale(self, uv, dt, which_ale=0, hbar=None, hbar_old=None, water_flux=None,
+                     use_wsplit=False, wsplit_maxcfl=1.0, hnode_new=None):
+        """vert_vel_ale + compute_CFLz + compute_Wvel_split; returns a dict of
+        w, w_e, w_i, cfl_z (nod2D x nl), hnode_new (nod2D x nl-1) and cflmax."""
+        m = self.mesh
+        out = {k: np.zeros((m.nod2D, m.nl)) for k in ("w", "w_e", "w_i", "cfl_z")}
+        hn = np.array(m.hnode_new if hnode_new is None else hnode_new, dtype=np.float64, order="C", copy=True)
+        cmax = C.c_double(0.0)
+        opt = lambda a: None if a is None else _d(np.ascontiguousarray(a, dtype=np.float64))
+        keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (hbar, hbar_old, water_flux)]
+        rc = lib().fo_vert_vel_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)), float(dt), int(which_ale),
+                                   opt(keep[0]), opt(keep[1]), opt(keep[2]), int(bool(use_wsplit)),
+                                   float(wsplit_maxcfl), _d(out["w"]), _d(out["w_e"]), _d(out["w_i"]),
+                                   _d(out["cfl_z"]), _d(hn), C.byref(cmax))
+        if rc:
+            raise RuntimeError(f"oracle vert_vel_ale failed rc={rc}")
+        out["hnode_new"] = hn
+        out["cflmax"] = cmax.value
+        return out
+
+    def compute_ssh_rhs_ale(self, uv, uv_rhs, ssh_rhs_old, alpha=1.0, which_ale=0, water_flux=None,
+                            use_cavity_fw2press=False):
+        m = self.mesh
+        out = np.zeros(m.nod2D)
+        wf = np.zeros(m.nod2D) if water_flux is None else np.ascontiguousarray(water_flux, dtype=np.float64)
+        rc = lib().fo_compute_ssh_rhs_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)),
+                                          _d(np.ascontiguousarray(uv_rhs)), _d(np.ascontiguousarray(ssh_rhs_old)),
+                                          _d(wf), float(alpha), int(which_ale), int(bool(use_cavity_fw2press)),
+                                          _d(out))
+        if rc:
+            raise RuntimeError(f"oracle compute_ssh_rhs_ale failed rc={rc}")
         return out
 
     # ------------------------------------------------------------ SSH

@@ -54,6 +54,15 @@ def read_result(path, mesh, ntr):
     valuesold = r[o + n3:o + 2 * n3].reshape(ntr, mesh.nod2D, L)
     d_eta = r[o + 2 * n3:o + 2 * n3 + mesh.nod2D]
     assert d_eta.size == mesh.nod2D, "truncated result file"
+    o = o + 2 * n3 + mesh.nod2D
+    ni = mesh.nod2D * mesh.nl
+    dyn = {}
+    for k in ("w", "w_e", "w_i", "cfl_z"):
+        dyn[k] = r[o:o + ni].reshape(mesh.nod2D, mesh.nl).copy()
+        o += ni
+    dyn["ssh_rhs"] = r[o:o + mesh.nod2D].copy()
+    assert dyn["ssh_rhs"].size == mesh.nod2D, "truncated result file"
     return dict(iters=int(st[0]), converged=bool(st[1]), nbreakdown=int(st[2]), nnegrz=int(st[3]),
                 nonconv=int(st[4]), resid=float(resid), rtol=float(rtol), content_before=c0.copy(),
-                content_after=c1.copy(), values=values.copy(), valuesold=valuesold.copy(), d_eta=d_eta.copy())
+                content_after=c1.copy(), values=values.copy(), valuesold=valuesold.copy(), d_eta=d_eta.copy(),
+                dyn=dyn)

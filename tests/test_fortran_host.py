@@ -62,7 +62,8 @@ def test_fortran_host_c0(c0, c0_oracle, tmp_path):
     rhs = mg.ssh_rhs(mesh)
     case, res = str(tmp_path / "case.bin"), str(tmp_path / "result.bin")
     fc.write_case(case, mesh, state, rhs, nsingle=1, nbatched=2, precond_variant=1)
-    print("[gpu] fortran driver c0: 1 per-tracer + 2 batched steps, solve_ssh_ale", flush=True)
+    print("[gpu] fortran driver c0: 1 per-tracer + 2 batched steps, vert_vel_ale, compute_ssh_rhs_ale, "
+          "solve_ssh_ale", flush=True)
     r = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -79,6 +80,12 @@ def test_fortran_host_c0(c0, c0_oracle, tmp_path):
         np.testing.assert_array_equal(out["valuesold"][i][wet], vo[i][wet])
         assert abs(out["content_after"][i] - out["content_before"][i]) < 1e-13 * abs(out["content_before"][i])
     assert out["converged"] and out["nbreakdown"] == 0 and out["nnegrz"] == 0
+    # vert_vel_ale and compute_ssh_rhs_ale through the Fortran host routines
+    ref = c0_oracle.vert_vel_ale(state.uv, state.dt)
+    for key in ("w", "w_e", "w_i", "cfl_z"):
+        np.testing.assert_array_equal(out["dyn"][key], ref[key], err_msg=key)
+    np.testing.assert_array_equal(out["dyn"]["ssh_rhs"], c0_oracle.compute_ssh_rhs_ale(state.uv, state.uv, rhs,
+                                                                                        alpha=0.5))
     orp, oci, ov = c0_oracle.stiff_matrix(state.dt)
     A = sp.csr_matrix((ov.copy(), oci.copy(), orp.copy()), shape=(mesh.nod2D, mesh.nod2D))
     assert np.linalg.norm(rhs - A @ out["d_eta"]) / np.linalg.norm(rhs) < 1e-10 * 1.01

@@ -1,0 +1,173 @@
+"""The steps either side of the transport on the GPU against the oracle:
+vert_vel_ale + compute_CFLz + compute_Wvel_split (linfs, zstar, W split) and
+compute_ssh_rhs_ale (linfs and full free surface). Node sums gather the edges
+in ascending order like the reference's serial scatter, so every array is
+compared bit for bit. The device W then drives the transport, and the device
+SSH right-hand side is solved in place."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from fesom2_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    from oracle.oracle import Oracle
+    mesh, state, _ = mg.make_config("c1")
+    return mesh, state, Oracle(mesh)
+
+
+def _ctx(mesh, state):
+    from fesom2_amd import gpu
+    return gpu.context_for(mesh, state)
+
+
+def _compare_vv(dev, ref, keys=("w", "w_e", "w_i", "cfl_z")):
+    for k in keys:
+        np.testing.assert_array_equal(dev[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("cfg", ["c0", "c1"])
+def test_vert_vel_linfs_bitwise(cfg, c0, c0_oracle, request):
+    if cfg == "c0":
+        mesh, state, _ = c0
+        o = c0_oracle
+    else:
+        mesh, state, o = request.getfixturevalue("c1")
+    print(f"[gpu] vert_vel_ale linfs {cfg} N2={mesh.nod2D}", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        ctx.set_uv(state.uv)
+        cmax, nan = ctx.vert_vel_ale(state.dt)
+        dev = ctx.vert_vel_download()
+    finally:
+        ctx.close()
+    ref = o.vert_vel_ale(state.uv, state.dt)
+    _compare_vv(dev, ref)
+    np.testing.assert_array_equal(dev["hnode_new"], mesh.hnode_new)
+    assert cmax == ref["cflmax"] and nan == 0
+
+
+def test_vert_vel_wsplit_bitwise_and_transport_refuses(c0, c0_oracle):
+    from fesom2_amd import gpu
+    mesh, state, _ = c0
+    ref0 = c0_oracle.vert_vel_ale(state.uv, state.dt)
+    cap = 0.3 * ref0["cflmax"]
+    print(f"[gpu] vert_vel_ale with W split at {cap:.3e}", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        ctx.set_uv(state.uv)
+        ctx.vert_vel_ale(state.dt, use_wsplit=True, wsplit_maxcfl=cap)
+        dev = ctx.vert_vel_download()
+        with pytest.raises(gpu.FesomGpuError):
+            ctx.solve_tracers_ale(state.dt, 1)
+        ctx.set_velocity(state.uv, state.w, state.w_e, state.w_i)   # explicit W again: the transport runs
+        ctx.solve_tracers_ale(state.dt, 1)
+    finally:
+        ctx.close()
+    ref = c0_oracle.vert_vel_ale(state.uv, state.dt, use_wsplit=True, wsplit_maxcfl=cap)
+    assert np.any(ref["w_i"] != 0.0)
+    _compare_vv(dev, ref)
+
+
+def test_vert_vel_zstar_bitwise(c0, c0_oracle):
+    mesh, state, _ = c0
+    rng = np.random.default_rng(11)
+    hbar_old = rng.normal(0.0, 0.3, mesh.nod2D)
+    hbar = hbar_old + rng.normal(0.0, 0.02, mesh.nod2D)
+    wf = rng.normal(0.0, 1e-6, mesh.nod2D)
+    print("[gpu] vert_vel_ale zstar c0", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        z = np.zeros(mesh.nod2D)
+        ctx.set_surface_forcing(z, wf, z, z, z)
+        ctx.set_uv(state.uv)
+        ctx.vert_vel_ale(state.dt, which_ale="zstar", hbar=hbar, hbar_old=hbar_old)
+        dev = ctx.vert_vel_download()
+    finally:
+        ctx.close()
+    ref = c0_oracle.vert_vel_ale(state.uv, state.dt, which_ale=2, hbar=hbar, hbar_old=hbar_old, water_flux=wf)
+    _compare_vv(dev, ref)
+    np.testing.assert_array_equal(dev["hnode_new"], ref["hnode_new"])
+    assert np.any(ref["hnode_new"] != mesh.hnode_new)
+
+
+@pytest.mark.parametrize("cfg", ["c0", "c1"])
+def test_ssh_rhs_bitwise(cfg, c0, c0_oracle, request):
+    if cfg == "c0":
+        mesh, state, _ = c0
+        o = c0_oracle
+    else:
+        mesh, state, o = request.getfixturevalue("c1")
+    rng = np.random.default_rng(12)
+    uvr = rng.normal(0.0, 0.05, state.uv.shape)
+    old = rng.normal(0.0, 1e4, mesh.nod2D)
+    wf = rng.normal(0.0, 1e-6, mesh.nod2D)
+    print(f"[gpu] compute_ssh_rhs_ale {cfg}", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        z = np.zeros(mesh.nod2D)
+        ctx.set_surface_forcing(z, wf, z, z, z)
+        ctx.set_uv(state.uv)
+        lin = ctx.compute_ssh_rhs_ale(uvr, old, alpha=0.6)
+        full = ctx.compute_ssh_rhs_ale(uvr, old, alpha=0.6, which_ale="zstar")
+    finally:
+        ctx.close()
+    np.testing.assert_array_equal(lin, o.compute_ssh_rhs_ale(state.uv, uvr, old, alpha=0.6))
+    np.testing.assert_array_equal(full, o.compute_ssh_rhs_ale(state.uv, uvr, old, alpha=0.6, which_ale=2,
+                                                              water_flux=wf))
+
+
+def test_device_w_drives_the_transport(c0, c0_oracle):
+    """W from the device's vert_vel_ale feeds the batched step with no host W:
+    the result equals the oracle step given the oracle's vert_vel_ale W."""
+    mesh, state, _ = c0
+    ref = c0_oracle.vert_vel_ale(state.uv, state.dt)
+    st2 = dataclasses.replace(state, w=ref["w"], w_e=ref["w_e"], w_i=ref["w_i"])
+    v, vo = state.values.copy(), state.valuesold.copy()
+    for _ in range(2):
+        c0_oracle.step(st2, v, vo)
+    print("[gpu] device vert_vel_ale -> 2 batched steps", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        zero = np.zeros_like(state.w)
+        ctx.set_velocity(state.uv, zero, zero, zero)   # host W deliberately zero
+        ctx.vert_vel_ale(state.dt)
+        ctx.solve_tracers_ale(state.dt, 2)
+        out = [ctx.tracer_download(i + 1)[0] for i in range(state.ntr)]
+    finally:
+        ctx.close()
+    L = mesh.nl - 1
+    k = np.arange(1, L + 1)
+    wet = (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+    for i in range(state.ntr):
+        np.testing.assert_array_equal(out[i][wet], v[i][wet])
+
+
+def test_device_ssh_rhs_is_solved_in_place(c0, c0_oracle):
+    import scipy.sparse as sp
+    mesh, state, _ = c0
+    rng = np.random.default_rng(13)
+    uvr = rng.normal(0.0, 0.05, state.uv.shape)
+    old = np.zeros(mesh.nod2D)
+    b = c0_oracle.compute_ssh_rhs_ale(state.uv, uvr, old, alpha=1.0)
+    print("[gpu] compute_ssh_rhs_ale -> resident CG", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        ctx.init_stiff_mat_ale(state.dt)
+        ctx.ssh_solve_preconditioner(2)
+        ctx.ssh_upload(np.zeros(mesh.nod2D), np.zeros(mesh.nod2D))
+        ctx.set_uv(state.uv)
+        ctx.compute_ssh_rhs_ale(uvr, old, alpha=1.0)
+        st = ctx.ssh_solve_resident(1e-10, 5000, 1)
+        x = ctx.ssh_download()
+    finally:
+        ctx.close()
+    rp, ci, v = c0_oracle.stiff_matrix(state.dt)
+    A = sp.csr_matrix((v.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    assert st["converged"]
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10 * 1.01
