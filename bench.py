@@ -69,6 +69,13 @@ def kernel_bytes(name, mesh, ntr):
         return ntr * (8 * NI + 16 * N3 + 8 * E3 + 16 * N3 + 8 * N3) + 24 * N3 + lists + 8 * E2
     if name == "k_vdiff":
         return ntr * 16 * N3 + 8 * (N3 + 2 * NI) + 8 * NI + 8 * N2
+    # the steps either side of the transport (tracer independent)
+    if name == "k_vert_vel":                 # uv, helem, area, hnode_new ; W, W_e, W_i, CFL_z | edge geometry
+        return 24 * F3 + 8 * NI + 8 * N3 + 32 * NI + lists + 32 * 2 * E2 + 40 * E2
+    if name == "k_ssh_rhs_edge":             # uv, uv_rhs, helem ; edge sums | edge geometry, element levels
+        return 40 * F3 + 40 * E2 + 8 * F2 + 8 * E2
+    if name == "k_ssh_rhs_node":             # edge sums, ssh_rhs_old ; ssh_rhs | lists
+        return 8 * E2 + lists + 8 * N2 + 8 * N2
     return None
 
 
@@ -169,6 +176,28 @@ def main():
     step_bytes = sum(kernel_bytes(n, mesh, state.ntr) or 0 for n in prof)
     step_ms = sum(ms for ms, c in prof.values()) / nprof
 
+    # the steps either side of the transport: vert_vel_ale and compute_ssh_rhs_ale
+    # (device-resident uv; uv_rhs and ssh_rhs_old are uploaded once before)
+    rng = np.random.default_rng(7)
+    uv_rhs = rng.normal(0.0, 0.01, state.uv.shape)
+    rhs_old = np.zeros(mesh.nod2D)
+    ctx.set_uv(state.uv)
+    ctx.vert_vel_ale(dt)
+    ctx.compute_ssh_rhs_ale(uv_rhs, rhs_old)
+    ctx.profile_enable(True)
+    for _ in range(nprof):
+        ctx.vert_vel_ale(dt)
+        ctx.compute_ssh_rhs_ale(uv_rhs, rhs_old)
+    prof_dyn = ctx.profile()
+    ctx.profile_enable(False)
+    dyn_kernels = {}
+    for name, (ms, cnt) in prof_dyn.items():
+        b = kernel_bytes(name, mesh, state.ntr)
+        avg = ms / max(cnt, 1)
+        dyn_kernels[name] = {"avg_ms": round(avg, 4), "launches": cnt,
+                             "alg_GBs": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+    print(f"[bench] vert_vel_ale / compute_ssh_rhs_ale: {dyn_kernels}", file=sys.stderr, flush=True)
+
     # SSH solve
     ctx.init_stiff_mat_ale(dt)
     ctx.ssh_solve_preconditioner(2)
@@ -201,6 +230,7 @@ def main():
         "step_model_bytes": int(step_model_bytes(mesh, state.ntr)),
         "step_model_frac_of_hbm": round(step_model_bytes(mesh, state.ntr) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernels,
+        "dynamics_kernels": dyn_kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

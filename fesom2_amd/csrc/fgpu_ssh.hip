@@ -19,13 +19,19 @@
 // ------------------------------------------------------------------------------
 // assembly
 // ------------------------------------------------------------------------------
+// One CSR row per thread, its edges in ascending order and each edge's
+// elements in order: init_stiff_mat_ale (oce_ale.F90:1393-1582) with the
+// element depth zbar_e_bot - zbar_e_srf and the areasvol/dt diagonal; with dhe
+// set, update_stiff_mat_ale (oce_ale.F90:1683-1792): -dhe(elem) in place of the
+// depth, added to the current values.
 __global__ void __launch_bounds__(SBLK) k_stiff_rows(DevMesh m, const int *rowptr, const int *colind, double *val,
-                                                     double dt, double factor)
+                                                     double dt, double factor, const double *dhe)
 {
     const int r = blockIdx.x * SBLK + threadIdx.x;
     if (r >= m.N2) return;
     const int p0 = rowptr[r], p1 = rowptr[r + 1];
-    for (int p = p0; p < p1; ++p) val[p] = 0.0;
+    if (!dhe)
+        for (int p = p0; p < p1; ++p) val[p] = 0.0;
     for (int pp = m.ne_ptr[r]; pp < m.ne_ptr[r + 1]; ++pp) {
         const int q = m.ne_idx[pp];
         const int ed = q >> 1, side = q & 1;
@@ -34,7 +40,7 @@ __global__ void __launch_bounds__(SBLK) k_stiff_rows(DevMesh m, const int *rowpt
             if (el < 0) continue;
             const int *en = m.elem_nodes + 3 * (size_t)el;
             const double *gs = m.gsca + 6 * (size_t)el;
-            const double hh = m.zbar_e_bot[el] - m.zbar_e_srf[el];
+            const double hh = dhe ? -dhe[el] : m.zbar_e_bot[el] - m.zbar_e_srf[el];
             const double cx = m.ecd[4 * ed + 2 * i];
             const double cy = m.ecd[4 * ed + 2 * i + 1];
             double fy[3];
@@ -51,7 +57,7 @@ __global__ void __launch_bounds__(SBLK) k_stiff_rows(DevMesh m, const int *rowpt
             }
         }
     }
-    if (m.ulevn[r] <= 1) val[p0] = val[p0] + m.areasvol[(size_t)r * m.NL + m.ulevn[r] - 1] / dt;
+    if (!dhe && m.ulevn[r] <= 1) val[p0] = val[p0] + m.areasvol[(size_t)r * m.NL + m.ulevn[r] - 1] / dt;
 }
 
 __global__ void __launch_bounds__(SBLK) k_precond(int n, const int *rowptr, const int *colind, const double *val,
@@ -520,11 +526,32 @@ extern "C" int fgpu_init_stiff_mat_ale(fgpu_context *c, double dt, double g, dou
     FGPU_HIP(hipMemcpy(c->ssh_colind, c->h_ssh_colind.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
     const double factor = g * dt * alpha * theta;
     hipLaunchKernelGGL(k_stiff_rows, dim3((n + SBLK - 1) / SBLK), dim3(SBLK), 0, c->stream, c->m, c->ssh_rowptr,
-                       c->ssh_colind, c->ssh_val, dt, factor);
+                       c->ssh_colind, c->ssh_val, dt, factor, (const double *)nullptr);
     FGPU_HIP(hipGetLastError());
     FGPU_HIP(hipStreamSynchronize(c->stream));
     c->ssh_ready = true;
     c->ssh_pr_ready = false;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_update_stiff_mat_ale(fgpu_context *c, const double *dhe, double dt, double g, double alpha,
+                                         double theta)
+{
+    if (!c || !dhe) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    if (!c->ssh_ready || c->ssh_n != c->N2) return fgpu_fail(FGPU_ERR_STATE, "SSH matrix not assembled (fgpu_init_stiff_mat_ale)");
+    FGPU_HIP(hipSetDevice(c->device));
+    if (!c->vv_dhe) {   // allocated on the first call, kept by the context
+        int rc = fgpu_alloc(c, (void **)&c->vv_dhe, sizeof(double) * (size_t)c->F2);
+        if (rc) return rc;
+    }
+    double *d_dhe = c->vv_dhe;
+    FGPU_HIP(hipMemcpyAsync(d_dhe, dhe, sizeof(double) * (size_t)c->F2, hipMemcpyHostToDevice, c->stream));
+    const double factor = g * dt * alpha * theta;
+    hipLaunchKernelGGL(k_stiff_rows, dim3((c->N2 + SBLK - 1) / SBLK), dim3(SBLK), 0, c->stream, c->m, c->ssh_rowptr,
+                       c->ssh_colind, c->ssh_val, dt, factor, (const double *)d_dhe);
+    FGPU_HIP(hipGetLastError());
+    FGPU_HIP(hipStreamSynchronize(c->stream));
     return FGPU_OK;
 }
 

@@ -47,7 +47,7 @@ module fesom_gpu_host
        init_tracers_AB, do_oce_adv_tra, diff_tracers_ale, diff_ver_part_impl_ale, solve_tracers_ale, &
        init_stiff_mat_ale, ssh_solve_preconditioner, ssh_solve_cg, solve_ssh_ale, gpu_synchronize, &
        gpu_autotune_step, gpu_ssh_nn_init, gpu_ssh_nn_train, ssh_solve_cg_learned, vert_vel_ale, &
-       gpu_vert_vel_ale, compute_ssh_rhs_ale
+       gpu_vert_vel_ale, compute_ssh_rhs_ale, update_stiff_mat_ale
 
 contains
 
@@ -188,6 +188,13 @@ contains
     call gpu_check(fgpu_compute_ssh_rhs_ale(ctx, uv_rhs, ssh_rhs_old, alpha, ale_code(which_ALE), cav, ssh_rhs), &
          'compute_ssh_rhs_ale')
   end subroutine compute_ssh_rhs_ale
+
+  ! update_stiff_mat_ale (oce_ale.F90:1683-1792): the stiffness term of the
+  ! element ssh change dhe(elem2D) added to the assembled SSH matrix
+  subroutine update_stiff_mat_ale(dhe, dt, g, alpha, theta)
+    real(WP), intent(in) :: dhe(:), dt, g, alpha, theta
+    call gpu_check(fgpu_update_stiff_mat_ale(ctx, dhe, dt, g, alpha, theta), 'update_stiff_mat_ale')
+  end subroutine update_stiff_mat_ale
 
   subroutine gpu_tracers_init(num_tracers)
     integer, intent(in) :: num_tracers

@@ -73,6 +73,7 @@ _SIGS = {
     "fgpu_autotune_step": [vp, C.c_double, C.c_int, ip, dp],
     "fgpu_step_shape": [vp, ip, ip, ip],
     "fgpu_init_stiff_mat_ale": [vp, C.c_double, C.c_double, C.c_double, C.c_double],
+    "fgpu_update_stiff_mat_ale": [vp, dp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fgpu_ssh_set_matrix": [vp, C.c_int, C.c_int, ip, ip, dp],
     "fgpu_ssh_get_matrix": [vp, ip, ip, dp, dp, ip],
     "fgpu_ssh_solve_preconditioner": [vp, C.c_int],
@@ -329,6 +330,9 @@ class Context:
     # ------------------------------------------------------------ SSH
     def init_stiff_mat_ale(self, dt, g=9.81, alpha=1.0, theta=1.0):
         _check(self.L.fgpu_init_stiff_mat_ale(self.h, float(dt), float(g), float(alpha), float(theta)))
+
+    def update_stiff_mat_ale(self, dhe, dt, g=9.81, alpha=1.0, theta=1.0):
+        _check(self.L.fgpu_update_stiff_mat_ale(self.h, _d(_f64(dhe)), float(dt), float(g), float(alpha), float(theta)))
 
     def ssh_set_matrix(self, rowptr1, colind1, values):
         rowptr1, colind1 = _i32(rowptr1), _i32(colind1)

@@ -184,6 +184,14 @@ int fgpu_tracer_content(fgpu_context *ctx, int tr_num, double *content);
 /* init_stiff_mat_ale (oce_ale.F90:1393-1582): assemble on the device from the
  * uploaded mesh (factor g*dt*alpha*theta). */
 int fgpu_init_stiff_mat_ale(fgpu_context *ctx, double dt, double g, double alpha, double theta);
+
+/* update_stiff_mat_ale (oce_ale.F90:1683-1792), full free surface: adds the
+ * stiffness term of the ssh change dhe (elem2D) to the matrix assembled by
+ * fgpu_init_stiff_mat_ale, with factor g*dt*alpha*theta. Like the reference,
+ * the preconditioner is not rebuilt (call fgpu_ssh_solve_preconditioner to
+ * refresh it). */
+int fgpu_update_stiff_mat_ale(fgpu_context *ctx, const double *dhe, double dt, double g, double alpha, double theta);
+
 /* Alternatively hand over an assembled ssh_stiff (MOD_MESH.F90:11-20), 1-based,
  * diagonal first in every row. */
 int fgpu_ssh_set_matrix(fgpu_context *ctx, int dim, int nza, const int *rowptr, const int *colind,

@@ -30,6 +30,7 @@
  *   20 W accumulated from the top instead of the bottom   (vert_vel_ale)
  *   21 SSH RHS drops helem in the first element's term    (compute_ssh_rhs_ale)
  *   22 CFL_z below an interface uses W of the interface above (compute_CFLz)
+ *   23 stiffness update adds +dhe instead of -dhe        (update_stiff_mat_ale)
  */
 #include "fesom_oracle.h"
 
@@ -1093,5 +1094,48 @@ int fo_compute_ssh_rhs_ale(const fo_mesh *m, const double *uv, const double *uv_
         for (int n = 0; n < N2; ++n) ssh_rhs[n] = ssh_rhs[n] + (1.0 - alpha) * ssh_rhs_old[n];
     }
     (void)NL;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* update_stiff_mat_ale: oce_ale.F90:1683-1792 (full free surface). The     */
+/* stiffness term of the actual ssh change: -dhe(elem) in place of the      */
+/* element's column depth, added to the assembled values; per row the edges  */
+/* come in ascending order and each edge's elements in order, as in         */
+/* fo_init_stiff_mat_ale.                                                    */
+/* ------------------------------------------------------------------------ */
+int fo_update_stiff_mat_ale(const fo_mesh *m, double dt, const double *dhe, const int *rowptr, const int *colind,
+                            double *values)
+{
+    const int N2 = m->nod2D;
+    int *pos = (int *)malloc(sizeof(int) * (size_t)N2);
+    if (!pos) return -1;
+    for (int n = 0; n < N2; ++n) pos[n] = -1;
+    const double factor = G_ACC * dt * 1.0 * 1.0;   /* g*dt*alpha*theta */
+    for (int ed = 0; ed < m->edge2D; ++ed) {
+        for (int side = 0; side < 2; ++side) {
+            const int row = m->edges[2 * ed + side];
+            for (int p = rowptr[row]; p < rowptr[row + 1]; ++p) pos[colind[p]] = p;
+            for (int i = 0; i < 2; ++i) {
+                const int el = m->edge_tri[2 * ed + i];
+                if (el < 0) continue;
+                const int *en = m->elem2D_nodes + 3 * (size_t)el;
+                const double *gs = m->gradient_sca + 6 * (size_t)el;
+                const double cx = m->edge_cross_dxdy[4 * ed + 2 * i];
+                const double cy = m->edge_cross_dxdy[4 * ed + 2 * i + 1];
+                const double hh = g_mut == 23 ? dhe[el] : -dhe[el];
+                double fy[3];
+                for (int j = 0; j < 3; ++j) fy[j] = hh * (gs[j] * cy - gs[3 + j] * cx);
+                if (i == 1) for (int j = 0; j < 3; ++j) fy[j] = -fy[j];
+                if (side == 1) for (int j = 0; j < 3; ++j) fy[j] = -fy[j];
+                for (int j = 0; j < 3; ++j) {
+                    const int p = pos[en[j]];
+                    if (p < 0) { free(pos); return -2; }
+                    values[p] = values[p] + fy[j] * factor;
+                }
+            }
+        }
+    }
+    free(pos);
     return 0;
 }

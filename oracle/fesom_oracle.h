@@ -98,6 +98,8 @@ int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, double sval, double
 int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, int ntr, fo_work *w);
 
 int fo_init_stiff_mat_ale(const fo_mesh *m, double dt, int *rowptr, int *colind, double *values, int max_nnz);
+int fo_update_stiff_mat_ale(const fo_mesh *m, double dt, const double *dhe, const int *rowptr, const int *colind,
+                            double *values);
 int fo_ssh_solve_preconditioner(int n, const int *rowptr, const int *colind, const double *values,
                                 int variant, double *pr_values);
 typedef struct {

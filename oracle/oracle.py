@@ -97,6 +97,7 @@ def lib():
         L.fo_init_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, ip, ip, dp, C.c_int]
         L.fo_vert_vel_ale.argtypes = [C.POINTER(FoMesh), dp, C.c_double, C.c_int, dp, dp, dp, C.c_int,
                                       C.c_double, dp, dp, dp, dp, dp, dp]
+        L.fo_update_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, dp, ip, ip, dp]
         L.fo_compute_ssh_rhs_ale.argtypes = [C.POINTER(FoMesh), dp, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp]
         L.fo_ssh_solve_preconditioner.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp]
         L.fo_ssh_solve_cg.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_int,
@@ -326,6 +327,16 @@ class Oracle:
         if nnz < 0:
             raise RuntimeError("stiffness assembly failed")
         return rowptr, colind[:nnz].copy(), vals[:nnz].copy()
+
+    def update_stiff_matrix(self, dt, dhe, rowptr, colind, vals):
+        """update_stiff_mat_ale: the assembled values plus the term of the ssh change dhe (elem2D)."""
+        out = np.array(vals, dtype=np.float64, copy=True)
+        rc = lib().fo_update_stiff_mat_ale(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(dhe, dtype=np.float64)),
+                                           _i(np.ascontiguousarray(rowptr, dtype=np.int32)),
+                                           _i(np.ascontiguousarray(colind, dtype=np.int32)), _d(out))
+        if rc:
+            raise RuntimeError(f"oracle update_stiff_mat_ale failed rc={rc}")
+        return out
 
     @staticmethod
     def preconditioner(rowptr, colind, vals, variant):

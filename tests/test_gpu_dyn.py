@@ -171,3 +171,31 @@ def test_device_ssh_rhs_is_solved_in_place(c0, c0_oracle):
     A = sp.csr_matrix((v.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
     assert st["converged"]
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10 * 1.01
+
+
+def test_update_stiff_mat_bitwise(c0, c0_oracle):
+    """update_stiff_mat_ale on the device adds the ssh-change term to the
+    assembled matrix bit for bit; the updated system still solves to the
+    reference tolerance."""
+    import scipy.sparse as sp
+    mesh, state, _ = c0
+    rng = np.random.default_rng(14)
+    dhe = rng.normal(0.0, 0.5, mesh.elem2D)
+    rp, ci, v = c0_oracle.stiff_matrix(state.dt)
+    ref = c0_oracle.update_stiff_matrix(state.dt, dhe, rp, ci, v)
+    print("[gpu] update_stiff_mat_ale c0", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        ctx.init_stiff_mat_ale(state.dt)
+        ctx.update_stiff_mat_ale(dhe, state.dt)
+        drp, dci, dv, _ = ctx.ssh_get_matrix()
+        ctx.ssh_solve_preconditioner(2)
+        b = mg.ssh_rhs(mesh, 3)
+        x, st = ctx.ssh_solve_cg(np.zeros(mesh.nod2D), b, soltol=1e-10, maxiter=5000)
+    finally:
+        ctx.close()
+    np.testing.assert_array_equal(drp - 1, rp)
+    np.testing.assert_array_equal(dci - 1, ci)
+    np.testing.assert_array_equal(dv, ref)
+    A = sp.csr_matrix((ref.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    assert st["converged"] and np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10 * 1.01

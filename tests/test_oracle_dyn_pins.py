@@ -19,10 +19,16 @@ the oracle's code:
   divergence, which is W·area at the surface;
 * under zstar the column thickness changes by hbar - hbar_old, and with no
   horizontal divergence and a water flux balancing the ssh change the
-  surface W vanishes.
-The last test plants the mistakes 19-22 of fesom_oracle.c and checks that one
+  surface W vanishes;
+* update_stiff_mat_ale adds the stiffness operator of the ssh change: it is
+  symmetric, annihilates constants, and equals the stiffness part of
+  init_stiff_mat_ale (pinned in test_oracle_pins.py) on a mesh whose element
+  depths are -dhe.
+The last test plants the mistakes 19-23 of fesom_oracle.c and checks that one
 of these pins fails.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -203,6 +209,33 @@ def check_zstar(mesh, st, o):
     return bool(ok)
 
 
+def check_update_stiff(mesh, st, o):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(7)
+    dhe = rng.normal(0.0, 0.4, mesh.elem2D)
+    rp, ci, v = o.stiff_matrix(st.dt)
+    upd = o.update_stiff_matrix(st.dt, dhe, rp, ci, np.zeros_like(v))
+    # stiffness part of init_stiff_mat_ale on element depths -dhe
+    m2 = dataclasses.replace(mesh, zbar_e_bot=mesh.zbar_e_srf - dhe)
+    rp2, ci2, v2 = Oracle(m2).stiff_matrix(st.dt)
+    if not (np.array_equal(rp, rp2) and np.array_equal(ci, ci2)):
+        return False
+    diag = rp[:-1]
+    off = np.ones(len(v), bool)
+    off[diag] = False
+    ok = np.array_equal(upd[off], v2[off])
+    av = mesh.areasvol[np.arange(mesh.nod2D), mesh.ulevels_nod2D - 1] / st.dt
+    ok &= np.allclose(upd[diag], v2[diag] - av, rtol=0, atol=1e-12 * np.abs(v2).max())
+    A = sp.csr_matrix((upd, ci, rp), shape=(mesh.nod2D, mesh.nod2D))
+    scale = np.abs(upd).max()
+    ok &= abs(A - A.T).max() < 1e-12 * scale
+    ok &= np.abs(A @ np.ones(mesh.nod2D)).max() < 1e-12 * scale
+    # additive: the update starts from the assembled values
+    full = o.update_stiff_matrix(st.dt, dhe, rp, ci, v)
+    ok &= np.allclose(full, v + upd, rtol=1e-14, atol=1e-12 * scale)
+    return bool(ok)
+
+
 # ------------------------------------------------------------------ tests
 def test_uniform_flow_has_no_divergence(flat):
     assert check_uniform_flow(*flat)
@@ -240,6 +273,10 @@ def test_zstar_thickness_and_surface(flat):
     assert check_zstar(*flat)
 
 
+def test_update_stiff_mat(bumps):
+    assert check_update_stiff(*bumps)
+
+
 def test_zlevel_is_rejected(bumps):
     mesh, st, o = bumps
     with pytest.raises(RuntimeError):
@@ -260,6 +297,7 @@ MUTATIONS = {
     20: lambda b, f: check_divergent_flow(*f),
     21: lambda b, f: check_ssh_rhs_column_divergence(*b),
     22: lambda b, f: check_cfl(*b),
+    23: lambda b, f: check_update_stiff(*b),
 }
 
 
