@@ -17,7 +17,7 @@ struct VVArgs {
     int which_ale, use_wsplit;
     const double *uv, *hbar, *hbar_old, *water_flux;
     double *w, *w_e, *w_i, *cfl, *hnode_new;
-    unsigned long long *cflmax_bits;   // max of CFL_z as the bits of a non-negative double
+    double *colmax;                    // max of CFL_z per column (nod2D)
     int *nan_columns;
 };
 
@@ -29,6 +29,17 @@ struct VVArgs {
 //   3. zstar: the ssh change spread over the layers above nlevels_nod2D_min,
 //      new layer thicknesses, surface water flux (oce_ale.F90:2539-2600)
 //   4. CFL_z (oce_ale.F90:2938-2966) and the W split (oce_ale.F90:3035-3047)
+__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ double rld(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+#define VV_NB 4   // edges whose element loads are issued together
+
 template <int LB>
 __global__ void __launch_bounds__(LB) k_vert_vel(DevMesh m, VVArgs a)
 {
@@ -39,30 +50,81 @@ __global__ void __launch_bounds__(LB) k_vert_vel(DevMesh m, VVArgs a)
     const int tid = threadIdx.x, k = tid + 1;
     if (tid == 0) sBad = 0;
     const int L = m.L, NL = m.NL;
-    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n] - 1;
+    // the node's record, then one edge of its list per lane: slot, element
+    // level ranges, elements and dual-face vectors
+    const int4 nr = *(const int4 *)(m.node_rec + 4 * (size_t)n);
+    const int p0 = nr.x, deg = nr.y, nzmin = nr.z, nzmax = nr.w - 1;
+    const int j = tid & 63;
+    int q = 0, el1 = 0, el2 = -1, nl1 = 0, nu1 = 1, nl2 = 0, nu2 = 1;
+    double dx1 = 0.0, dy1 = 0.0, dx2 = 0.0, dy2 = 0.0;
+    if (j < deg) {
+        const int4 r0 = *(const int4 *)(m.ne_rec + 8 * (size_t)(p0 + j));
+        const int4 r1 = *(const int4 *)(m.ne_rec + 8 * (size_t)(p0 + j) + 4);
+        q = r0.x;
+        nl1 = r1.x; nu1 = r1.y; nl2 = r1.z; nu2 = r1.w;
+        const int e = q >> 1;
+        const int2 et = *(const int2 *)(m.edge_tri + 2 * (size_t)e);
+        el1 = et.x;
+        el2 = et.y;
+        const double2 c01 = *(const double2 *)(m.ecd + 4 * (size_t)e);
+        const double2 c23 = *(const double2 *)(m.ecd + 4 * (size_t)e + 2);
+        dx1 = c01.x; dy1 = c01.y; dx2 = c23.x; dy2 = c23.y;
+    }
+    // thickness divergence: edges in list (ascending) order, el(1) before
+    // el(2); the element loads of VV_NB edges are in flight together
     double acc = 0.0;
-    const int p0 = m.ne_ptr[n], p1 = m.ne_ptr[n + 1];
-    for (int p = p0; p < p1; ++p) {
-        const int *r = m.ne_rec + 8 * (size_t)p;
-        const int q = r[0], e = q >> 1;
-        const int nl1 = r[4], nu1 = r[5], nl2 = r[6], nu2 = r[7];
-        const int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
-        if (k >= nu1 && k <= nl1) {
-            const double *u = a.uv + ((size_t)el1 * L + k - 1) * 2;
-            const double c = (u[1] * m.ecd[4 * e] - u[0] * m.ecd[4 * e + 1]) * m.helem[(size_t)el1 * L + k - 1];
-            acc = (q & 1) ? acc - c : acc + c;
+    for (int j0 = 0; j0 < deg; j0 += VV_NB) {
+        double2 u1[VV_NB], u2[VV_NB];
+        double h1[VV_NB], h2[VV_NB];
+#pragma unroll
+        for (int b = 0; b < VV_NB; ++b) {
+            const int jj = j0 + b < deg ? j0 + b : j0;
+            const int e1 = rl(el1, jj), e2 = rl(el2, jj);
+            const bool a1 = j0 + b < deg && k >= rl(nu1, jj) && k <= rl(nl1, jj);
+            const bool a2 = j0 + b < deg && e2 >= 0 && k >= rl(nu2, jj) && k <= rl(nl2, jj);
+            u1[b] = make_double2(0.0, 0.0);
+            u2[b] = make_double2(0.0, 0.0);
+            h1[b] = 0.0;
+            h2[b] = 0.0;
+            if (a1) {
+                u1[b] = *(const double2 *)(a.uv + ((size_t)e1 * L + k - 1) * 2);
+                h1[b] = m.helem[(size_t)e1 * L + k - 1];
+            }
+            if (a2) {
+                u2[b] = *(const double2 *)(a.uv + ((size_t)e2 * L + k - 1) * 2);
+                h2[b] = m.helem[(size_t)e2 * L + k - 1];
+            }
         }
-        if (el2 >= 0 && k >= nu2 && k <= nl2) {
-            const double *u = a.uv + ((size_t)el2 * L + k - 1) * 2;
-            const double c = -((u[1] * m.ecd[4 * e + 2] - u[0] * m.ecd[4 * e + 3]) * m.helem[(size_t)el2 * L + k - 1]);
-            acc = (q & 1) ? acc - c : acc + c;
+#pragma unroll
+        for (int b = 0; b < VV_NB; ++b) {
+            const int jj = j0 + b;
+            if (jj >= deg) break;
+            const int qq = rl(q, jj), e2 = rl(el2, jj);
+            if (k >= rl(nu1, jj) && k <= rl(nl1, jj)) {
+                const double c = (u1[b].y * rld(dx1, jj) - u1[b].x * rld(dy1, jj)) * h1[b];
+                acc = (qq & 1) ? acc - c : acc + c;
+            }
+            if (e2 >= 0 && k >= rl(nu2, jj) && k <= rl(nl2, jj)) {
+                const double c = -((u2[b].y * rld(dx2, jj) - u2[b].x * rld(dy2, jj)) * h2[b]);
+                acc = (qq & 1) ? acc - c : acc + c;
+            }
         }
     }
     if (tid < LB) sW[tid] = acc;
     if (tid == 0) sW[LB] = 0.0;
     __syncthreads();
-    if (tid == 0)   // serial, bottom up: the reference's association order
-        for (int nz = nzmax; nz >= nzmin; --nz) sW[nz - 1] = sW[nz - 1] + sW[nz];
+    if (tid == 0) {   // serial, bottom up: the reference's association order
+        double run = sW[nzmax];
+        int nz = nzmax;
+        for (; nz - 3 >= nzmin; nz -= 4) {   // four reads ahead of the dependent adds
+            const double d0 = sW[nz - 1], d1 = sW[nz - 2], d2 = sW[nz - 3], d3 = sW[nz - 4];
+            run = d0 + run; sW[nz - 1] = run;
+            run = d1 + run; sW[nz - 2] = run;
+            run = d2 + run; sW[nz - 3] = run;
+            run = d3 + run; sW[nz - 4] = run;
+        }
+        for (; nz >= nzmin; --nz) { run = sW[nz - 1] + run; sW[nz - 1] = run; }
+    }
     __syncthreads();
     double w = sW[tid];
     const bool lay = k >= nzmin && k <= nzmax;
@@ -107,8 +169,7 @@ __global__ void __launch_bounds__(LB) k_vert_vel(DevMesh m, VVArgs a)
         a.w_i[i] = wi;
         a.cfl[i] = cfl;
     }
-    // block maximum of CFL_z, then one atomic max on its bit pattern (the
-    // ordering of non-negative doubles and of their bits agree)
+    // column maximum of CFL_z (reduced over the columns by k_cfl_max)
     double cm = k <= NL ? cfl : 0.0;
     for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_xor(cm, off));
     __syncthreads();
@@ -116,36 +177,64 @@ __global__ void __launch_bounds__(LB) k_vert_vel(DevMesh m, VVArgs a)
     __syncthreads();
     if (tid == 0) {
         for (int i = 1; i < LB / 64; ++i) cm = fmax(cm, sW[i]);
-        atomicMax(a.cflmax_bits, (unsigned long long)__double_as_longlong(cm));
+        a.colmax[n] = cm;
+    }
+}
+
+// maximum over the column maxima: a block maximum per block, then one atomic
+// max per block on the bit pattern (non-negative doubles and their bits order
+// alike); the maximum does not depend on the order
+__global__ void __launch_bounds__(256) k_cfl_max(const double *colmax, int n, unsigned long long *bits)
+{
+    __shared__ double sm[4];
+    double cm = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) cm = fmax(cm, colmax[i]);
+    for (int off = 32; off > 0; off >>= 1) cm = fmax(cm, __shfl_xor(cm, off));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = cm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cm = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+        atomicMax(bits, (unsigned long long)__double_as_longlong(cm));
     }
 }
 
 // Depth-integrated transport of alpha (u + u_rhs) through the two dual faces
-// of edge e, levels in order (oce_ale.F90:1844-1880); one thread per edge.
-__global__ void k_ssh_rhs_edge(DevMesh m, const double *uv, const double *uvr, double alpha, double *ce)
+// of edge e (oce_ale.F90:1844-1880): one block per edge, the layer terms of
+// both elements computed by one lane per level, then each element's levels
+// summed in order (lane 0: el(1), lane 1: el(2)) as the reference does.
+template <int LB>
+__global__ void __launch_bounds__(LB) k_ssh_rhs_edge(DevMesh m, const double *uv, const double *uvr, double alpha,
+                                                     double *ce)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.x;
     if (e >= m.E2) return;
-    const int L = m.L;
-    const int el1 = m.edge_tri[2 * e], el2 = m.edge_tri[2 * e + 1];
-    double c1 = 0.0, c2 = 0.0;
-    {
-        const double dX = m.ecd[4 * e], dY = m.ecd[4 * e + 1];
-        const int nu = m.ulev[el1], nlv = m.nlev[el1] - 1;
-        for (int nz = nu; nz <= nlv; ++nz) {
-            const size_t i = ((size_t)el1 * L + nz - 1) * 2;
-            c1 = c1 + alpha * ((uv[i + 1] + uvr[i + 1]) * dX - (uv[i] + uvr[i]) * dY) * m.helem[(size_t)el1 * L + nz - 1];
-        }
+    __shared__ double s1[LB], s2[LB];
+    const int tid = threadIdx.x, k = tid + 1, L = m.L;
+    const int2 et = *(const int2 *)(m.edge_tri + 2 * (size_t)e);
+    const int el1 = et.x, el2 = et.y;
+    const int nu1 = m.ulev[el1], nl1 = m.nlev[el1] - 1;
+    const int nu2 = el2 >= 0 ? m.ulev[el2] : 1, nl2 = el2 >= 0 ? m.nlev[el2] - 1 : 0;
+    if (k >= nu1 && k <= nl1) {
+        const double2 c = *(const double2 *)(m.ecd + 4 * (size_t)e);
+        const size_t i = ((size_t)el1 * L + k - 1) * 2;
+        const double2 u = *(const double2 *)(uv + i), ur = *(const double2 *)(uvr + i);
+        s1[tid] = alpha * ((u.y + ur.y) * c.x - (u.x + ur.x) * c.y) * m.helem[(size_t)el1 * L + k - 1];
     }
-    if (el2 >= 0) {
-        const double dX = m.ecd[4 * e + 2], dY = m.ecd[4 * e + 3];
-        const int nu = m.ulev[el2], nlv = m.nlev[el2] - 1;
-        for (int nz = nu; nz <= nlv; ++nz) {
-            const size_t i = ((size_t)el2 * L + nz - 1) * 2;
-            c2 = c2 - alpha * ((uv[i + 1] + uvr[i + 1]) * dX - (uv[i] + uvr[i]) * dY) * m.helem[(size_t)el2 * L + nz - 1];
-        }
+    if (k >= nu2 && k <= nl2) {
+        const double2 c = *(const double2 *)(m.ecd + 4 * (size_t)e + 2);
+        const size_t i = ((size_t)el2 * L + k - 1) * 2;
+        const double2 u = *(const double2 *)(uv + i), ur = *(const double2 *)(uvr + i);
+        s2[tid] = alpha * ((u.y + ur.y) * c.x - (u.x + ur.x) * c.y) * m.helem[(size_t)el2 * L + k - 1];
     }
-    ce[e] = c1 + c2;
+    __syncthreads();
+    if (tid < 2) {
+        const double *src = tid == 0 ? s1 : s2;
+        const int lo = tid == 0 ? nu1 : nu2, hi = tid == 0 ? nl1 : nl2;
+        double c = 0.0;
+        for (int nz = lo; nz <= hi; ++nz) c = tid == 0 ? c + src[nz - 1] : c - src[nz - 1];
+        const double c2 = __shfl(c, 1);
+        if (tid == 0) ce[e] = c + c2;
+    }
 }
 
 // Node sums over the edges in ascending order (oce_ale.F90:1889-1894), then
@@ -269,7 +358,8 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
     a.w_i = (double *)c->d.w_i;
     a.cfl = c->vv_cfl;
     a.hnode_new = (double *)c->m.hnode_new;
-    a.cflmax_bits = (unsigned long long *)c->vv_flags;
+    if ((rc = ensure_buf(c, &c->vv_colmax, N2))) return rc;
+    a.colmax = c->vv_colmax;
     a.nan_columns = (int *)((char *)c->vv_flags + 8);
     FGPU_HIP(hipMemsetAsync(c->vv_flags, 0, 16, c->stream));
     {
@@ -278,6 +368,9 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
         else hipLaunchKernelGGL(k_vert_vel<128>, dim3(c->N2), dim3(128), 0, c->stream, c->m, a);
         FGPU_HIP(hipGetLastError());
     }
+    hipLaunchKernelGGL(k_cfl_max, dim3(256), dim3(256), 0, c->stream, (const double *)c->vv_colmax, c->N2,
+                       (unsigned long long *)c->vv_flags);
+    FGPU_HIP(hipGetLastError());
     if (which_ale == FGPU_ALE_ZSTAR && (rc = fgpu_compute_zn(c))) return rc;   // Z levels of the implicit diffusion
     unsigned long long flags[2] = {0ull, 0ull};
     FGPU_HIP(hipMemcpyAsync(flags, c->vv_flags, 16, hipMemcpyDeviceToHost, c->stream));
@@ -326,8 +419,12 @@ extern "C" int fgpu_compute_ssh_rhs_ale(fgpu_context *c, const double *uv_rhs, c
     const double *wf = c->d.water_flux ? c->d.water_flux : c->vv_old;   // unread under linfs
     {
         ProfScope ps(c, "k_ssh_rhs_edge");
-        hipLaunchKernelGGL(k_ssh_rhs_edge, dim3((c->E2 + 255) / 256), dim3(256), 0, c->stream, c->m, c->d.uv,
-                           c->vv_uvr, alpha, c->vv_ce);
+        if (c->nl - 1 <= 64)
+            hipLaunchKernelGGL(k_ssh_rhs_edge<64>, dim3(c->E2), dim3(64), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
+                               alpha, c->vv_ce);
+        else
+            hipLaunchKernelGGL(k_ssh_rhs_edge<128>, dim3(c->E2), dim3(128), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
+                               alpha, c->vv_ce);
         FGPU_HIP(hipGetLastError());
     }
     {

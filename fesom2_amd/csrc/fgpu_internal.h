@@ -148,6 +148,7 @@ struct fgpu_context {
     double *vv_hbar = nullptr, *vv_hbar_old = nullptr, *vv_cfl = nullptr;
     double *vv_uvr = nullptr, *vv_old = nullptr, *vv_ce = nullptr, *vv_rhs = nullptr;
     double *vv_dhe = nullptr;       // dhe of update_stiff_mat_ale (elem2D)
+    double *vv_colmax = nullptr;    // per-column max of CFL_z (nod2D)
     void *vv_flags = nullptr;       // max CFL_z bits, NaN column count
     bool wsplit_active = false;     // W split by fgpu_vert_vel_ale(use_wsplit=1): w_i != 0
 };
