@@ -120,26 +120,40 @@ __device__ void nn_bsum3(double &a, double &b, double &c)
     __syncthreads();
 }
 
+// thread-strided sum of v[0..n): thread t adds v[t], v[t + NNT], ... in that
+// order, with four loads in flight
+__device__ __forceinline__ double nn_strided_sum(const double *v, int n)
+{
+    double a = 0.0;
+    for (int base = threadIdx.x; base < n; base += 4 * NNT) {
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = base + u * NNT < n ? v[base + u * NNT] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (base + u * NNT < n) a += x[u];
+    }
+    return a;
+}
+
 // three slots at once
 __device__ void nn_slots3(const double *dpart, int nbd, int s0, int s1, int s2, double &a, double &b, double &c)
 {
-    a = 0.0; b = 0.0; c = 0.0;
-    for (int j = threadIdx.x; j < nbd; j += NNT) {
-        a += dpart[(size_t)s0 * nbd + j];
-        if (s1 >= 0) b += dpart[(size_t)s1 * nbd + j];
-        if (s2 >= 0) c += dpart[(size_t)s2 * nbd + j];
-    }
+    a = nn_strided_sum(dpart + (size_t)s0 * nbd, nbd);
+    b = s1 >= 0 ? nn_strided_sum(dpart + (size_t)s1 * nbd, nbd) : 0.0;
+    c = s2 >= 0 ? nn_strided_sum(dpart + (size_t)s2 * nbd, nbd) : 0.0;
     nn_bsum3(a, b, c);
 }
 
 // the nbd partials of a slot, added in block order by every block
 __device__ double nn_slot(const NNArgs &A, int slot)
 {
-    double x = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int j = threadIdx.x; j < A.nbd; j += NNT) x += A.dpart[(size_t)slot * A.nbd + j];
+    double x = nn_strided_sum(A.dpart + (size_t)slot * A.nbd, A.nbd), d1 = 0.0, d2 = 0.0;
     nn_bsum3(x, d1, d2);
     return x;
 }
+
+#define NSG 8   // matrix row entries whose loads are in flight together
 
 __device__ __forceinline__ void nn_put3(const NNArgs &A, int s0, double a, int s1, double b, int s2, double c)
 {
@@ -212,7 +226,11 @@ __global__ void __launch_bounds__(NNT) k_nn_in(NNArgs A, const double *rin, doub
     double s = sqrt(rtrt / (double)A.n);
     if (!(s > 0.0)) s = 1.0;
     if (A.st && lead) A.st->s = s;
-    const float *W0 = A.theta + OFF_W0;
+    // the input layer's weights in LDS: read once per block, not once per
+    // channel pair behind every store of the loop below
+    __shared__ float W0[2 * NNC];
+    if (threadIdx.x < 2 * NNC) W0[threadIdx.x] = A.theta[OFF_W0 + threadIdx.x];
+    __syncthreads();
     for (int i = blockIdx.x * NNT + threadIdx.x; i < A.n; i += A.nbd * NNT) {
         const float u = (float)(A.isq[i] * rin[i] / s);
         A.u[i] = u;
@@ -385,10 +403,23 @@ __global__ void __launch_bounds__(NNT) k_nn_loss(NNArgs A, const double *z, cons
     double xax = 0.0, eae = 0.0, d2 = 0.0;
     for (int i = blockIdx.x * NNT + threadIdx.x; i < A.n; i += A.nbd * NNT) {
         double ax = 0.0, ae = 0.0;
-        for (int p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) {
-            const int j = A.colind[p];
-            ax = ax + A.val[p] * x[j];
-            ae = ae + A.val[p] * (z[j] - x[j]);
+        const int p0 = A.rowptr[i], p1 = A.rowptr[i + 1];
+        for (int pb = p0; pb < p1; pb += NSG) {
+            double xv[NSG], zv[NSG], vv[NSG];
+#pragma unroll
+            for (int u = 0; u < NSG; ++u) {
+                const int p = pb + u < p1 ? pb + u : p0;
+                const int j = A.colind[p];
+                vv[u] = A.val[p];
+                xv[u] = x[j];
+                zv[u] = z[j];
+            }
+#pragma unroll
+            for (int u = 0; u < NSG; ++u)
+                if (pb + u < p1) {
+                    ax = ax + vv[u] * xv[u];
+                    ae = ae + vv[u] * (zv[u] - xv[u]);
+                }
         }
         A.ae[i] = ae;
         xax += x[i] * ax;
@@ -642,8 +673,7 @@ struct FCGArgs {
 
 __device__ double fcg_slot(const FCGArgs &F, int slot)
 {
-    double x = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int j = threadIdx.x; j < F.nbd; j += NNT) x += F.dpart[(size_t)slot * F.nbd + j];
+    double x = nn_strided_sum(F.dpart + (size_t)slot * F.nbd, F.nbd), d1 = 0.0, d2 = 0.0;
     nn_bsum3(x, d1, d2);
     return x;
 }
@@ -664,7 +694,19 @@ __global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F)
     double bb = 0.0, rr = 0.0, tt = 0.0;
     for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT) {
         double ax = 0.0;
-        for (int p = F.rowptr[i]; p < F.rowptr[i + 1]; ++p) ax = ax + F.val[p] * F.x[F.colind[p]];
+        const int p0 = F.rowptr[i], p1 = F.rowptr[i + 1];
+        for (int pb = p0; pb < p1; pb += NSG) {
+            double xv[NSG], vv[NSG];
+#pragma unroll
+            for (int u = 0; u < NSG; ++u) {
+                const int p = pb + u < p1 ? pb + u : p0;
+                vv[u] = F.val[p];
+                xv[u] = F.x[F.colind[p]];
+            }
+#pragma unroll
+            for (int u = 0; u < NSG; ++u)
+                if (pb + u < p1) ax = ax + vv[u] * xv[u];
+        }
         const double r = F.rhs[i] - ax;
         F.r[i] = r;
         F.p1[i] = 0.0;
@@ -710,10 +752,32 @@ __global__ void __launch_bounds__(NNT) k_fcg_pa(FCGArgs F)
     double pqn = 0.0;
     for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT) {
         double sq = 0.0;
-        for (int p = F.rowptr[i]; p < F.rowptr[i + 1]; ++p) {
-            const int j = F.colind[p];
-            const double zj = jac ? F.isq[j] * F.isq[j] * F.r[j] : F.z[j];
-            sq = sq + F.val[p] * (first ? zj : zj + be * pold[j]);
+        const int p0 = F.rowptr[i], p1 = F.rowptr[i + 1];
+        for (int pb = p0; pb < p1; pb += NSG) {
+            // column indices first, then each gather in a straight-line batch
+            // (jac and first are uniform; branching per entry would serialise)
+            int jv[NSG];
+            double vv[NSG], zv[NSG], ov[NSG];
+#pragma unroll
+            for (int u = 0; u < NSG; ++u) {
+                const int p = pb + u < p1 ? pb + u : p0;
+                jv[u] = F.colind[p];
+                vv[u] = F.val[p];
+            }
+            if (jac) {
+#pragma unroll
+                for (int u = 0; u < NSG; ++u) zv[u] = F.isq[jv[u]] * F.isq[jv[u]] * F.r[jv[u]];
+            } else {
+#pragma unroll
+                for (int u = 0; u < NSG; ++u) zv[u] = F.z[jv[u]];
+            }
+            if (!first) {
+#pragma unroll
+                for (int u = 0; u < NSG; ++u) ov[u] = pold[jv[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < NSG; ++u)
+                if (pb + u < p1) sq = sq + vv[u] * (first ? zv[u] : zv[u] + be * ov[u]);
         }
         const double zi = jac ? F.isq[i] * F.isq[i] * F.r[i] : F.z[i];
         const double pi = first ? zi : zi + be * pold[i];

@@ -112,11 +112,43 @@ struct CGArgs {
     double soltol;
 };
 
+#define SGB 8   // row entries whose column loads and gathers are in flight together
+
+// sum over row r of v[p] * x[colind[p]] in entry order; the loads of SGB
+// entries are issued before any is added
 __device__ __forceinline__ double row_dot(const int *rowptr, const int *colind, const double *v, const double *x, int r)
 {
     double s = 0.0;
-    for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) s = s + v[p] * x[colind[p]];
+    const int p0 = rowptr[r], p1 = rowptr[r + 1];
+    for (int pb = p0; pb < p1; pb += SGB) {
+        double xv[SGB], vv[SGB];
+#pragma unroll
+        for (int u = 0; u < SGB; ++u) {
+            const int p = pb + u < p1 ? pb + u : p0;
+            vv[u] = v[p];
+            xv[u] = x[colind[p]];
+        }
+#pragma unroll
+        for (int u = 0; u < SGB; ++u)
+            if (pb + u < p1) s = s + vv[u] * xv[u];
+    }
     return s;
+}
+
+// thread-strided sum of v[0..n): thread t adds v[t], v[t + SBLK], ... in that
+// order, with four loads in flight
+__device__ __forceinline__ double strided_sum(const double *v, int n)
+{
+    double a = 0.0;
+    for (int base = threadIdx.x; base < n; base += 4 * SBLK) {
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = base + u * SBLK < n ? v[base + u * SBLK] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (base + u * SBLK < n) a += x[u];
+    }
+    return a;
 }
 
 // block reduction of up to three values: butterfly within each wave, then the
@@ -141,11 +173,8 @@ __device__ void block_sum3(double &a, double &b, double &c)
 // every block adds the nb partials of slot s0 (and s0+1 if two) in one fixed order
 __device__ void sum_partials(const CGArgs &A, int s0, bool two, double &a, double &b)
 {
-    double x = 0.0, y = 0.0, z = 0.0;
-    for (int j = threadIdx.x; j < A.nb; j += SBLK) {
-        x += A.part[(size_t)s0 * A.nb + j];
-        if (two) y += A.part[(size_t)(s0 + 1) * A.nb + j];
-    }
+    double x = strided_sum(A.part + (size_t)s0 * A.nb, A.nb), z = 0.0;
+    double y = two ? strided_sum(A.part + (size_t)(s0 + 1) * A.nb, A.nb) : 0.0;
     block_sum3(x, y, z);
     a = x;
     b = y;
@@ -237,10 +266,19 @@ __global__ void __launch_bounds__(SBLK) k_cg_a(CGArgs A)
     double pq = 0.0, d1 = 0.0, d2 = 0.0;
     for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
         double s = 0.0;
-        for (int p = A.rowptr[row]; p < A.rowptr[row + 1]; ++p) {
-            const int j = A.colind[p];
-            const double pj = first ? A.z[j] : A.z[j] + be * pold[j];
-            s = s + A.val[p] * pj;
+        const int p0 = A.rowptr[row], p1 = A.rowptr[row + 1];
+        for (int pb = p0; pb < p1; pb += SGB) {
+            double pv[SGB], vv[SGB];
+#pragma unroll
+            for (int u = 0; u < SGB; ++u) {
+                const int p = pb + u < p1 ? pb + u : p0;
+                const int j = A.colind[p];
+                vv[u] = A.val[p];
+                pv[u] = first ? A.z[j] : A.z[j] + be * pold[j];
+            }
+#pragma unroll
+            for (int u = 0; u < SGB; ++u)
+                if (pb + u < p1) s = s + vv[u] * pv[u];
         }
         const double pr_ = first ? A.z[row] : A.z[row] + be * pold[row];
         pnew[row] = pr_;
@@ -320,8 +358,7 @@ __global__ void __launch_bounds__(SBLK) k_cg_c(CGArgs A)
 // ------------------------------------------------------------------------------
 __device__ __forceinline__ double cg1_slot(const CGArgs &A, int slot)
 {
-    double x = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int j = threadIdx.x; j < A.nb; j += SBLK) x += A.part[(size_t)slot * A.nb + j];
+    double x = strided_sum(A.part + (size_t)slot * A.nb, A.nb), d1 = 0.0, d2 = 0.0;
     block_sum3(x, d1, d2);
     return x;
 }
@@ -353,9 +390,19 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
     double g = 0.0, d = 0.0, dum = 0.0;
     for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
         double w = 0.0;
-        for (int p = A.rowptr[row]; p < A.rowptr[row + 1]; ++p) {
-            const int j = A.colind[p];
-            w = w + A.val[p] * (A.dinv[j] * A.r[j]);
+        const int p0 = A.rowptr[row], p1 = A.rowptr[row + 1];
+        for (int pb = p0; pb < p1; pb += SGB) {
+            double zv[SGB], vv[SGB];
+#pragma unroll
+            for (int u = 0; u < SGB; ++u) {
+                const int p = pb + u < p1 ? pb + u : p0;
+                const int j = A.colind[p];
+                vv[u] = A.val[p];
+                zv[u] = A.dinv[j] * A.r[j];
+            }
+#pragma unroll
+            for (int u = 0; u < SGB; ++u)
+                if (pb + u < p1) w = w + vv[u] * zv[u];
         }
         const double z = A.dinv[row] * A.r[row];
         A.q[row] = w;
@@ -377,12 +424,9 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
     CGState *st = A.st;
     if (st->done) return;
     const int it = st->iter, par = it & 1;
-    double gam = 0.0, del = 0.0, rr = 0.0;
-    for (int j = threadIdx.x; j < A.nb; j += SBLK) {
-        gam += A.part[(size_t)(2 * par) * A.nb + j];
-        del += A.part[(size_t)(2 * par + 1) * A.nb + j];
-        rr += A.part[(size_t)(4 + par) * A.nb + j];
-    }
+    double gam = strided_sum(A.part + (size_t)(2 * par) * A.nb, A.nb);
+    double del = strided_sum(A.part + (size_t)(2 * par + 1) * A.nb, A.nb);
+    double rr = strided_sum(A.part + (size_t)(4 + par) * A.nb, A.nb);
     block_sum3(gam, del, rr);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     double rtol;
@@ -429,10 +473,19 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
         sc[row] = s_;
         rn[row] = r_;
         double w = 0.0;
-        for (int p = A.rowptr[row]; p < A.rowptr[row + 1]; ++p) {
-            const int j = A.colind[p];
-            const double zj = A.dinv[j] * (rc[j] - al * (wc[j] + be * sp[j]));
-            w = w + A.val[p] * zj;
+        const int p0 = A.rowptr[row], p1 = A.rowptr[row + 1];
+        for (int pb = p0; pb < p1; pb += SGB) {
+            double zv[SGB], vv[SGB];
+#pragma unroll
+            for (int u = 0; u < SGB; ++u) {
+                const int p = pb + u < p1 ? pb + u : p0;
+                const int j = A.colind[p];
+                vv[u] = A.val[p];
+                zv[u] = A.dinv[j] * (rc[j] - al * (wc[j] + be * sp[j]));
+            }
+#pragma unroll
+            for (int u = 0; u < SGB; ++u)
+                if (pb + u < p1) w = w + vv[u] * zv[u];
         }
         wn[row] = w;
         const double z_ = A.dinv[row] * r_;
