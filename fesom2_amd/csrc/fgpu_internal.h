@@ -15,7 +15,7 @@
 
 // tracer scheme codes
 enum { HOR_MFCT = 0, HOR_UPW1 = 1 };
-enum { VER_QR4C = 0, VER_UPW1 = 1 };
+enum { VER_QR4C = 0, VER_UPW1 = 1, VER_CDIFF = 2, VER_PPM = 3 };
 enum { LIM_FCT = 0, LIM_NON = 1 };
 
 // Mesh as seen by the kernels: 0-based connectivity, level numbers 1-based.

@@ -692,13 +692,13 @@ extern "C" int fgpu_tracer_config(fgpu_context *c, int tr_num, int ID, int AB_or
     if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
     if (tr_num < 1 || tr_num > c->t.ntr) return fgpu_fail(FGPU_ERR_ARG, "tr_num out of range");
     static const char *hor_names[] = {"MFCT", "UPW1"};
-    static const char *ver_names[] = {"QR4C", "UPW1"};
+    static const char *ver_names[] = {"QR4C", "UPW1", "CDIFF", "PPM"};
     static const char *lim_names[] = {"FCT", "NON"};
     int h, v, l;
     if (parse_scheme(tra_adv_hor, hor_names, 2, &h))
         return fgpu_fail(FGPU_ERR_SCHEME, std::string("Unknown horizontal advection type ") +
                                               (tra_adv_hor ? tra_adv_hor : "(null)") + "! Check your namelists!");
-    if (parse_scheme(tra_adv_ver, ver_names, 2, &v))
+    if (parse_scheme(tra_adv_ver, ver_names, 4, &v))
         return fgpu_fail(FGPU_ERR_SCHEME, std::string("Unknown vertical advection type ") +
                                               (tra_adv_ver ? tra_adv_ver : "(null)") + "! Check your namelists!");
     if (parse_scheme(tra_adv_lim, lim_names, 2, &l))

@@ -28,6 +28,7 @@
 #include <algorithm>
 
 #include "fgpu_internal.h"
+#include "fgpu_vert.h"
 
 #define NXCD 8
 #define NG 8    // edges per gather batch in K_B and K_C
@@ -317,7 +318,7 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
     const int n = xcd_column(m.N2);
     if (n >= m.N2) return;
     __shared__ double sT[LB + 2], sA[LB + 2], sW[LB + 2], sWe[LB + 2], sAr[LB + 2], sZ[LB + 2], sZb[LB + 2];
-    __shared__ double sLo[LB + 2];
+    __shared__ double sLo[LB + 2], sHn[LB + 2], sHo[LB + 2];
     const int tid = threadIdx.x, k = tid + 1;
     const int L = m.L, NL = m.NL;
     const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
@@ -330,7 +331,11 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
         sAr[tid] = m.area[(size_t)n * NL + tid];
         sZb[tid] = m.zbar3d[(size_t)n * NL + tid];
     }
-    if (k <= L) sZ[tid] = m.Z3d[ci];
+    if (k <= L) {
+        sZ[tid] = m.Z3d[ci];
+        sHn[tid] = m.hnode_new[ci];   // PPM: thicknesses of the neighbouring layers
+        sHo[tid] = m.hnode[ci];
+    }
     double hn = 1.0, hh = 0.0, av_ = 1.0;
     if (act) {
         hn = m.hnode_new[ci];
@@ -358,8 +363,11 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
         }
         __syncthreads();
         if (k >= nzmin && k <= nzmax) {
-            const double ho = (ver == VER_QR4C) ? col_qr4c(k, nzmin, nzmax, sA, W, sAr, sZ, sZb, pv, lk)
-                                                : col_upw1(k, nzmin, nzmax, sA, W, sAr, lk);
+            double ho;
+            if (ver == VER_QR4C) ho = col_qr4c(k, nzmin, nzmax, sA, W, sAr, sZ, sZb, pv, lk);
+            else if (ver == VER_CDIFF) ho = ver_cdiff(k, nzmin, nzmax, sA, W, sAr, lk);
+            else if (ver == VER_PPM) ho = ver_ppm(k, nzmin, nzmax, sA, W, sAr, sHn, sHo, dt, lk);
+            else ho = col_upw1(k, nzmin, nzmax, sA, W, sAr, lk);
             av[k - 1] = ho;
         }
         if (lim == LIM_FCT) {

@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include "fgpu_internal.h"
+#include "fgpu_vert.h"
 
 #define BLK 256
 
@@ -260,9 +261,12 @@ __device__ __forceinline__ double ver_qr4c(int j, int nzmin, int nzmax, const do
 }
 
 __device__ __forceinline__ double ver_ho(int ver, int j, int nzmin, int nzmax, const double *TAB, const double *W,
-                                         const double *ar, const double *Zn, const double *zb, double pv, double f)
+                                         const double *ar, const double *Zn, const double *zb, double pv,
+                                         const double *hn, const double *ho, double dt, double f)
 {
     if (ver == VER_QR4C) return ver_qr4c(j, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, f);
+    if (ver == VER_CDIFF) return ver_cdiff(j, nzmin, nzmax, TAB, W, ar, f);
+    if (ver == VER_PPM) return ver_ppm(j, nzmin, nzmax, TAB, W, ar, hn, ho, dt, f);
     return ver_upw1(j, nzmin, nzmax, TAB, W, ar, f);
 }
 
@@ -289,6 +293,7 @@ __global__ void __launch_bounds__(BLK) k_node_lo(DevMesh m, DevDyn d, DevTracers
     double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
     const int lim = t.lim[tr], ver = t.ver[tr];
     const double pv = t.pv[tr];
+    const double *hn = m.hnode_new + (size_t)n * L, *ho = m.hnode + (size_t)n * L;
     if (lim == LIM_FCT) {
         const double *We = d.w_e + (size_t)n * NL;
         const double *W = d.w + (size_t)n * NL;
@@ -305,12 +310,12 @@ __global__ void __launch_bounds__(BLK) k_node_lo(DevMesh m, DevDyn d, DevTracers
         const double lk1 = ver_upw1(k + 1, nzmin, nzmax, T, We, ar, 0.0);
         const double av_ = m.areasvol[(size_t)n * NL + k - 1];
         t.fct_lo[i + t.sN * tr] = (T[k - 1] * m.hnode[i] + (acc + (lk - lk1)) * dt / av_) / m.hnode_new[i];
-        av[k - 1] = ver_ho(ver, k, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, lk);
-        if (k == nzmax - 1) av[nzmax - 1] = ver_ho(ver, nzmax, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, lk1);
+        av[k - 1] = ver_ho(ver, k, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, hn, ho, dt, lk);
+        if (k == nzmax - 1) av[nzmax - 1] = ver_ho(ver, nzmax, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, hn, ho, dt, lk1);
     } else {
         const double *We = d.w_e + (size_t)n * NL;
-        av[k - 1] = ver_ho(ver, k, nzmin, nzmax, TAB, We, ar, Zn, zb, pv, 0.0);
-        if (k == nzmax - 1) av[nzmax - 1] = ver_ho(ver, nzmax, nzmin, nzmax, TAB, We, ar, Zn, zb, pv, 0.0);
+        av[k - 1] = ver_ho(ver, k, nzmin, nzmax, TAB, We, ar, Zn, zb, pv, hn, ho, dt, 0.0);
+        if (k == nzmax - 1) av[nzmax - 1] = ver_ho(ver, nzmax, nzmin, nzmax, TAB, We, ar, Zn, zb, pv, hn, ho, dt, 0.0);
     }
 }
 

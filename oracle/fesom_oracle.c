@@ -31,6 +31,8 @@
  *   21 SSH RHS drops helem in the first element's term    (compute_ssh_rhs_ale)
  *   22 CFL_z below an interface uses W of the interface above (compute_CFLz)
  *   23 stiffness update adds +dhe instead of -dhe        (update_stiff_mat_ale)
+ *   24 PPM upward flux takes the parabola's lower edge    (adv_tra_vert_ppm)
+ *   25 CDIFF interface value is the lower cell's, not the mean (adv_tra_ver_cdiff)
  */
 #include "fesom_oracle.h"
 
@@ -608,6 +610,112 @@ void fo_oce_tra_adv_flux2dtracer(const fo_mesh *m, double dt, double *dttf_h, do
 }
 
 /* ------------------------------------------------------------------------ */
+/* adv_tra_ver_cdiff: oce_adv_tra_ver.F90:635-695                            */
+/* ------------------------------------------------------------------------ */
+void fo_adv_tra_ver_cdiff(const fo_mesh *m, const double *W, const double *ttf, double *flux, int init_zero)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    double *tvert = (double *)malloc(sizeof(double) * (size_t)(NL + 1));
+    if (init_zero) memset(flux, 0, (size_t)m->nod2D * NL * sizeof(double));
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int nzmax = m->nlevels_nod2D[n] - 1, nzmin = m->ulevels_nod2D[n];
+        tvert[nzmin] = -W[INT(n, nzmin)] * ttf[LAY(n, nzmin)] * m->area[INT(n, nzmin)];
+        tvert[nzmax + 1] = 0.0;
+        for (int nz = nzmin + 1; nz <= nzmax; ++nz) {
+            const double tv = (g_mut == 25) ? ttf[LAY(n, nz)] : 0.5 * (ttf[LAY(n, nz - 1)] + ttf[LAY(n, nz)]);
+            tvert[nz] = -tv * W[INT(n, nz)] * m->area[INT(n, nz)];
+        }
+        for (int nz = nzmin; nz <= nzmax; ++nz) flux[INT(n, nz)] = tvert[nz] - flux[INT(n, nz)];
+    }
+    free(tvert);
+    (void)L;
+}
+
+/* ------------------------------------------------------------------------ */
+/* adv_tra_vert_ppm: oce_adv_tra_ver.F90:438-631 (Colella and Woodward 1984: */
+/* interface values from eqs. 1.6-1.8 on the new thicknesses, limited        */
+/* parabolas, swept-volume fluxes with the old thicknesses)                  */
+/* ------------------------------------------------------------------------ */
+static double fo_sign1(double b) { return signbit(b) ? -1.0 : 1.0; }
+
+void fo_adv_tra_vert_ppm(const fo_mesh *m, double dt, const double *W, const double *ttf, double *flux,
+                         int init_zero)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    double *tvert = (double *)malloc(sizeof(double) * (size_t)(NL + 1));
+    double *tv = (double *)malloc(sizeof(double) * (size_t)(NL + 1));
+    if (init_zero) memset(flux, 0, (size_t)m->nod2D * NL * sizeof(double));
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
+#define TT(k) ttf[LAY(n, (k))]
+#define HN(k) m->hnode_new[LAY(n, (k))]
+        tv[nzmin] = TT(nzmin);
+        tv[nzmin + 1] = 0.5 * (TT(nzmin) + TT(nzmin + 1));
+        tv[nzmax - 1] = 0.5 * (TT(nzmax - 2) + TT(nzmax - 1));
+        tv[nzmax] = TT(nzmax - 1);
+        for (int nz = nzmin + 1; nz <= nzmax - 3; ++nz) {
+            const double dzjm1 = HN(nz - 1), dzj = HN(nz), dzjp1 = HN(nz + 1), dzjp2 = HN(nz + 2);
+            double deltaj = dzj / (dzjm1 + dzj + dzjp1) *
+                            ((2.0 * dzjm1 + dzj) / (dzjp1 + dzj) * (TT(nz + 1) - TT(nz)) +
+                             (dzj + 2.0 * dzjp1) / (dzjm1 + dzj) * (TT(nz) - TT(nz - 1)));
+            double deltajp1 = dzjp1 / (dzj + dzjp1 + dzjp2) *
+                              ((2.0 * dzj + dzjp1) / (dzjp2 + dzjp1) * (TT(nz + 2) - TT(nz + 1)) +
+                               (dzjp1 + 2.0 * dzjp2) / (dzj + dzjp1) * (TT(nz + 1) - TT(nz)));
+            if ((TT(nz + 1) - TT(nz)) * (TT(nz) - TT(nz - 1)) > 0.0) {
+                double a = fabs(deltaj), b = 2.0 * fabs(TT(nz + 1) - TT(nz)), c = 2.0 * fabs(TT(nz) - TT(nz - 1));
+                deltaj = dmin(dmin(a, b), c) * fo_sign1(deltaj);
+            } else {
+                deltaj = 0.0;
+            }
+            if ((TT(nz + 2) - TT(nz + 1)) * (TT(nz + 1) - TT(nz)) > 0.0) {
+                double a = fabs(deltajp1), b = 2.0 * fabs(TT(nz + 2) - TT(nz + 1)), c = 2.0 * fabs(TT(nz + 1) - TT(nz));
+                deltajp1 = dmin(dmin(a, b), c) * fo_sign1(deltajp1);
+            } else {
+                deltajp1 = 0.0;
+            }
+            tv[nz + 1] = TT(nz) + dzj / (dzj + dzjp1) * (TT(nz + 1) - TT(nz)) +
+                         1.0 / (dzjm1 + dzj + dzjp1 + dzjp2) *
+                             (2.0 * dzjp1 * dzj / (dzj + dzjp1) *
+                                  ((dzjm1 + dzj) / (2.0 * dzj + dzjp1) - (dzjp2 + dzjp1) / (2.0 * dzjp1 + dzj)) *
+                                  (TT(nz + 1) - TT(nz)) -
+                              dzj * (dzjm1 + dzj) / (2.0 * dzj + dzjp1) * deltajp1 +
+                              dzjp1 * (dzjp1 + dzjp2) / (dzj + 2.0 * dzjp1) * deltaj);
+        }
+        for (int k = 1; k <= nzmax; ++k) tvert[k] = 0.0;
+        for (int nz = nzmin; nz <= nzmax - 1; ++nz) {
+            const double wt = W[INT(n, nz)], wb = W[INT(n, nz + 1)];
+            if ((wt <= 0.0) && (wb >= 0.0)) continue;
+            double aL = tv[nz], aR = tv[nz + 1];
+            const double t = TT(nz);
+            if ((aR - t) * (t - aL) <= 0.0) { aL = t; aR = t; }
+            if ((aR - aL) * (t - 0.5 * (aL + aR)) > (aR - aL) * (aR - aL) / 6.0) aL = 3.0 * t - 2.0 * aR;
+            if ((aR - aL) * (t - 0.5 * (aR + aL)) < -((aR - aL) * (aR - aL) / 6.0)) aR = 3.0 * t - 2.0 * aL;
+            const double dzj = m->hnode[LAY(n, nz)];
+            const double aj = 6.0 * (t - 0.5 * (aL + aR));
+            if (wt > 0.0) {
+                const double x = dmin(wt * dt / dzj, 1.0);
+                tvert[nz] = (g_mut == 24) ? (-aR - 0.5 * x * (aL - aR + (1.0 - 2.0 / 3.0 * x) * aj))
+                                          : (-aL - 0.5 * x * (aR - aL + (1.0 - 2.0 / 3.0 * x) * aj));
+                tvert[nz] = tvert[nz] * m->area[INT(n, nz)] * wt;
+            }
+            if (wb < 0.0) {
+                const double x = dmin(-wb * dt / dzj, 1.0);
+                tvert[nz + 1] = (-aR + 0.5 * x * (aR - aL - (1.0 - 2.0 / 3.0 * x) * aj));
+                tvert[nz + 1] = tvert[nz + 1] * m->area[INT(n, nz + 1)] * wb;
+            }
+        }
+        tvert[nzmin] = -tv[nzmin] * W[INT(n, nzmin)] * m->area[INT(n, nzmin)];
+        tvert[nzmax] = 0.0;
+        for (int nz = nzmin; nz <= nzmax; ++nz) flux[INT(n, nz)] = tvert[nz] - flux[INT(n, nz)];
+#undef TT
+#undef HN
+    }
+    free(tv);
+    free(tvert);
+    (void)L;
+}
+
+/* ------------------------------------------------------------------------ */
 /* do_oce_adv_tra: oce_adv_tra_driver.F90:46-490 (ltra_diag and DVD off)     */
 /* ------------------------------------------------------------------------ */
 int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w)
@@ -615,7 +723,7 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
     const int L = m->nl - 1, NL = m->nl;
     if (d->use_wsplit) return -2;      /* adv_tra_vert_impl path is not restated */
     if (t->hor != FO_HOR_MFCT && t->hor != FO_HOR_UPW1) return -3;
-    if (t->ver != FO_VER_QR4C && t->ver != FO_VER_UPW1) return -4;
+    if (t->ver != FO_VER_QR4C && t->ver != FO_VER_UPW1 && t->ver != FO_VER_CDIFF && t->ver != FO_VER_PPM) return -4;
     int fct = (t->lim == FO_LIM_FCT);
     if (fct) {
         fo_adv_tra_hor_upw1(m, d->uv, t->values, w->adv_flux_hor, 1);
@@ -645,6 +753,10 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
     const double *pw = fct ? d->w : d->w_e;
     if (t->ver == FO_VER_QR4C)
         fo_adv_tra_ver_qr4c(m, pw, t->valuesAB, t->pv, w->adv_flux_ver, zero);
+    else if (t->ver == FO_VER_CDIFF)
+        fo_adv_tra_ver_cdiff(m, pw, t->valuesAB, w->adv_flux_ver, zero);
+    else if (t->ver == FO_VER_PPM)
+        fo_adv_tra_vert_ppm(m, dt, pw, t->valuesAB, w->adv_flux_ver, zero);
     else
         fo_adv_tra_ver_upw1(m, pw, t->valuesAB, w->adv_flux_ver, zero);
     if (fct) {

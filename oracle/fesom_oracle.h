@@ -50,7 +50,7 @@ typedef struct {
 } fo_dyn;
 
 enum { FO_HOR_MFCT = 0, FO_HOR_UPW1 = 1 };
-enum { FO_VER_QR4C = 0, FO_VER_UPW1 = 1 };
+enum { FO_VER_QR4C = 0, FO_VER_UPW1 = 1, FO_VER_CDIFF = 2, FO_VER_PPM = 3 };
 enum { FO_LIM_FCT = 0, FO_LIM_NON = 1 };
 
 typedef struct {
@@ -85,6 +85,9 @@ void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf,
 void fo_adv_tra_ver_upw1(const fo_mesh *m, const double *w, const double *ttf, double *flux, int init_zero);
 void fo_adv_tra_ver_qr4c(const fo_mesh *m, const double *w, const double *ttf, double num_ord,
                          double *flux, int init_zero);
+void fo_adv_tra_ver_cdiff(const fo_mesh *m, const double *w, const double *ttf, double *flux, int init_zero);
+void fo_adv_tra_vert_ppm(const fo_mesh *m, double dt, const double *w, const double *ttf, double *flux,
+                         int init_zero);
 void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const double *lo,
                         double *adf_h, double *adf_v, double *ttf_min, double *ttf_max,
                         double *fct_plus, double *fct_minus);

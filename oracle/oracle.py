@@ -57,7 +57,7 @@ class FoCgStats(C.Structure):
 
 
 HOR = {"MFCT": 0, "UPW1": 1}
-VER = {"QR4C": 0, "UPW1": 1}
+VER = {"QR4C": 0, "UPW1": 1, "CDIFF": 2, "PPM": 3}
 LIM = {"FCT": 0, "NON": 1}
 
 _lib = None
@@ -85,6 +85,8 @@ def lib():
         L.fo_adv_tra_hor_mfct.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, dp, C.c_int]
         L.fo_adv_tra_ver_upw1.argtypes = [C.POINTER(FoMesh), dp, dp, dp, C.c_int]
         L.fo_adv_tra_ver_qr4c.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, C.c_int]
+        L.fo_adv_tra_ver_cdiff.argtypes = [C.POINTER(FoMesh), dp, dp, dp, C.c_int]
+        L.fo_adv_tra_vert_ppm.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, C.c_int]
         L.fo_oce_tra_adv_fct.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, dp, dp, dp, dp, dp]
         L.fo_oce_tra_adv_flux2dtracer.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, dp, C.c_int, dp, dp]
         L.fo_do_oce_adv_tra.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
@@ -314,6 +316,18 @@ class Oracle:
                                           _d(out))
         if rc:
             raise RuntimeError(f"oracle compute_ssh_rhs_ale failed rc={rc}")
+        return out
+
+    def ver_cdiff(self, W, T):
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl))
+        lib().fo_adv_tra_ver_cdiff(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
+                                   _d(out), 1)
+        return out
+
+    def ver_ppm(self, W, T, dt):
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl))
+        lib().fo_adv_tra_vert_ppm(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(W)),
+                                  _d(np.ascontiguousarray(T)), _d(out), 1)
         return out
 
     # ------------------------------------------------------------ SSH

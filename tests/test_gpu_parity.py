@@ -100,6 +100,33 @@ def test_adv_stage_bitwise(c0, c0_oracle, tr_num):
     ctx.close()
 
 
+@pytest.mark.parametrize("sch", [dict(ver="CDIFF", lim="FCT"), dict(ver="PPM", lim="FCT"), dict(ver="PPM", lim="NON")],
+                         ids=["cdiff-fct", "ppm-fct", "ppm-non"])
+def test_adv_stage_vertical_schemes(c0, c0_oracle, sch):
+    """The per-stage kernels with the CDIFF and PPM vertical schemes: the
+    vertical flux after do_oce_adv_tra and the tracer after diff_tracers_ale are
+    bit-identical to the oracle."""
+    mesh, state, _ = c0
+    full = dict(hor="MFCT", ph=0.0, pv=1.0, **sch)
+    print(f"[gpu] adv stage {full}", flush=True)
+    ctx = _gpu().context_for(mesh, state, schemes=full)
+    ctx.init_tracers_AB(1)
+    ctx.do_oce_adv_tra(state.dt, 1)
+    v = state.values[0].copy()
+    vo = state.valuesold[0].copy()
+    o = c0_oracle.adv_stage(state, v, vo, ID=int(state.ids[0]), **full)
+    g_av = ctx.work(1, "adv_flux_ver", (mesh.nod2D, mesh.nl))
+    ki = np.arange(1, mesh.nl + 1)
+    wi = (ki[None, :] >= mesh.ulevels_nod2D[:, None]) & (ki[None, :] <= mesh.nlevels_nod2D[:, None])
+    np.testing.assert_array_equal(g_av[wi], o["adv_flux_ver"][wi])
+    ctx.diff_tracers_ale(state.dt, 1)
+    c0_oracle.diff_stage(state)
+    gv, _ = ctx.tracer_download(1)
+    wet = _wet(mesh)
+    np.testing.assert_array_equal(gv[wet], v[wet])
+    ctx.close()
+
+
 @pytest.mark.parametrize("nsteps", [1, 10])
 def test_batched_step_c0(c0, c0_oracle, nsteps):
     """Fused batched step (hipGraph replay) vs the oracle's solve_tracers_ale.
@@ -146,11 +173,15 @@ def test_autotuned_step_c0(c0, c0_oracle):
 
 
 def test_scheme_variants_c0(c0, c0_oracle):
-    """UPW1/UPW1/FCT, MFCT(ph=0.5)/QR4C(pv=0)/NON and AB3 against the oracle."""
+    """UPW1/UPW1/FCT, MFCT(ph=0.5)/QR4C(pv=0)/NON, MFCT/UPW1/NON and the CDIFF
+    and PPM vertical schemes in the fused step, against the oracle."""
     mesh, state, _ = c0
     variants = [dict(hor="UPW1", ver="UPW1", lim="FCT"),
                 dict(hor="MFCT", ver="QR4C", lim="NON", ph=0.5, pv=0.0),
-                dict(hor="MFCT", ver="UPW1", lim="NON", ph=1.0, pv=1.0)]
+                dict(hor="MFCT", ver="UPW1", lim="NON", ph=1.0, pv=1.0),
+                dict(hor="MFCT", ver="CDIFF", lim="FCT"),
+                dict(hor="MFCT", ver="PPM", lim="FCT"),
+                dict(hor="UPW1", ver="PPM", lim="NON")]
     for sch in variants:
         print(f"[gpu] scheme {sch}", flush=True)
         ctx = _gpu().context_for(mesh, state, schemes=sch)

@@ -263,6 +263,84 @@ def check_ver_upw1(mesh, st):
     return True
 
 
+def _linear_cells(mesh, a, b):
+    """Cell means of f(z) = a + b z (exact midpoint values) and f itself."""
+    zb = mesh.zbar_3d_n
+    mean = a + b * 0.5 * (zb[:, :-1] + zb[:, 1:])
+    return np.where(_wet(mesh), mean, 0.0), (lambda z: a + b * z)
+
+
+def check_cdiff_linear(mesh, st):
+    """Uniform dz, linear profile: CDIFF's interface value 0.5 (T(k-1) + T(k))
+    is the exact interface value; surface row -W T(nzmin) area; bottom row 0."""
+    o = orc.Oracle(mesh)
+    T, f = _linear_cells(mesh, 10.0, 2e-3)
+    W = st.w
+    fl = o.ver_cdiff(W, T)
+    zb = mesh.zbar_3d_n
+    for n in range(0, mesh.nod2D, 3):
+        nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+        if abs(fl[n, nzmin - 1] - (-W[n, nzmin - 1] * T[n, nzmin - 1] * mesh.area[n, nzmin - 1])) > 1e-13 * abs(fl[n, nzmin - 1]) + 1e-300:
+            return False
+        for kk in range(nzmin + 1, nzmax):
+            exp = -f(zb[n, kk - 1]) * W[n, kk - 1] * mesh.area[n, kk - 1]
+            if abs(fl[n, kk - 1] - exp) > 1e-11 * abs(exp) + 1e-300:
+                return False
+        if fl[n, nzmax - 1] != 0.0:
+            return False
+    return True
+
+
+def check_ppm_linear(mesh, st):
+    """Uniform dz, linear profile: PPM's parabola is the line itself, so the flux
+    through an interface is minus the exact mean of the profile over the volume
+    swept in dt (the layer below for W > 0, above for W < 0), times W area. The
+    surface and bottom layers are excluded: the reference takes their outer
+    interface values as the cell means."""
+    o = orc.Oracle(mesh)
+    T, f = _linear_cells(mesh, 10.0, 2e-3)
+    W = st.w
+    dt = 3.0e5   # Courant numbers of a few per cent, so the swept fraction matters
+    fl = o.ver_ppm(W, T, dt)
+    zb = mesh.zbar_3d_n
+    for n in range(0, mesh.nod2D, 3):
+        nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+        for kk in range(nzmin + 1, nzmax):
+            w = W[n, kk - 1]
+            if w > 0:
+                if kk > nzmax - 2:
+                    continue
+                h = mesh.hnode[n, kk - 1]
+                x = min(w * dt / h, 1.0)
+                zmid = zb[n, kk - 1] - 0.5 * x * h
+            elif w < 0:
+                if kk < nzmin + 2:
+                    continue
+                h = mesh.hnode[n, kk - 2]
+                x = min(-w * dt / h, 1.0)
+                zmid = zb[n, kk - 1] + 0.5 * x * h
+            else:
+                zmid = zb[n, kk - 1]
+            exp = -f(zmid) * w * mesh.area[n, kk - 1]
+            if abs(fl[n, kk - 1] - exp) > 1e-10 * abs(exp) + 1e-300:
+                return False
+        if fl[n, nzmax - 1] != 0.0:
+            return False
+    return True
+
+
+def check_ppm_zigzag(mesh, st):
+    """A profile with an extremum in every layer: the limiter flattens every
+    parabola and PPM becomes the upwind flux (the pinned UPW1 restatement)."""
+    o = orc.Oracle(mesh)
+    k = np.arange(mesh.nl - 1)
+    T = np.where(_wet(mesh), 5.0 + np.where(k % 2 == 0, 1.0, -1.0)[None, :], 0.0)
+    W = st.w
+    a = o.ver_ppm(W, T, 900.0)
+    b = o.ver_upw1(W, T)
+    return bool(np.allclose(a, b, rtol=1e-13, atol=1e-300))
+
+
 def _step(mesh, st, nsteps=1, values=None, Kv=None, schemes=None):
     o = orc.Oracle(mesh)
     st2 = st
@@ -483,6 +561,15 @@ def test_vdiff_finite_volume(small):
     assert check_vdiff_fv(*small)
 
 
+def test_cdiff_linear(small):
+    assert check_cdiff_linear(*small)
+
+
+def test_ppm_linear_and_zigzag(small):
+    assert check_ppm_linear(*small)
+    assert check_ppm_zigzag(*small)
+
+
 def test_stiffness_properties(small, sphere):
     assert check_stiffness(*small)
     assert check_stiffness(*sphere)
@@ -518,6 +605,8 @@ MUTATIONS = {
     16: lambda s, f: check_precond_v1(*s),
     17: lambda s, f: check_ver_upw1(*s),
     18: lambda s, f: check_fct_bounds(*s),
+    24: lambda s, f: check_ppm_linear(*s),
+    25: lambda s, f: check_cdiff_linear(*s),
 }
 
 
