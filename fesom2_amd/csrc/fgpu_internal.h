@@ -14,7 +14,7 @@
 #define FGPU_MAX_NL 128            // columns deeper than this are rejected
 
 // tracer scheme codes
-enum { HOR_MFCT = 0, HOR_UPW1 = 1 };
+enum { HOR_MFCT = 0, HOR_UPW1 = 1, HOR_MUSCL = 2 };
 enum { VER_QR4C = 0, VER_UPW1 = 1, VER_CDIFF = 2, VER_PPM = 3 };
 enum { LIM_FCT = 0, LIM_NON = 1 };
 
@@ -35,6 +35,7 @@ struct DevMesh {
     const int *ne_idx;       // 2*e + side (side 0: node is edges(1,e)), edges ascending
     const int *ne_rec;       // 8 per ne_idx entry: q, other node, nu12, nl12, nl1, nu1, nl2, nu2
     const int *node_rec;     // 4 per node: ne_ptr, degree, ulevels_nod2D, nlevels_nod2D
+    const int *nblay;        // N2: nboundary_lay of MUSCL (oce_muscl_adv.F90:91-157)
     const double *elem_area, *elem_cos, *edge_dxdy, *ecd, *gsca;
     const double *area, *areasvol;        // N2*NL
     const double *hnode, *hnode_new;      // N2*L
@@ -50,7 +51,7 @@ struct DevMesh {
 };
 // erec_i fields
 enum { ER_N1, ER_N2, ER_EL1, ER_EL2, ER_NL1, ER_NU1, ER_NL2, ER_NU2, ER_UA0, ER_UA1, ER_UA2, ER_DA0, ER_DA1,
-       ER_DA2, ER_RULE, ER_NZMIN, ER_NZMAX, ER_U1, ER_U2, ER_NL1N, ER_NL2N, ER_I_PAD1, ER_I_PAD2, ER_I_PAD3, ERI };
+       ER_DA2, ER_RULE, ER_NZMIN, ER_NZMAX, ER_U1, ER_U2, ER_NL1N, ER_NL2N, ER_NBL1, ER_NBL2, ER_I_PAD3, ERI };
 // erec_d fields
 enum { ER_DX1, ER_DY1, ER_DX2, ER_DY2, ER_A, ER_EDX, ER_EDY, ER_GU0, ER_GU1, ER_GU2, ER_GU3, ER_GU4, ER_GU5,
        ER_GD0, ER_GD1, ER_GD2, ER_GD3, ER_GD4, ER_GD5, ER_D_PAD, ERD };

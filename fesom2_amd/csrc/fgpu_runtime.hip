@@ -446,6 +446,22 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
         UP(p_i, nrec.data(), nrec.size()); m.node_rec = p_i;
     }
     UP(p_i, kneed.data(), kneed.size()); c->d_kneed = p_i;
+    // nboundary_lay of MUSCL (muscl_adv_init, oce_muscl_adv.F90:91-157): nl-1,
+    // reset to 0 at both nodes of a boundary edge, else the minimum over the
+    // node's edges of min(nlevels of the two triangles) - 1
+    std::vector<int> nblay(N2, nl - 1);
+    for (int e = 0; e < E2; ++e) {
+        const int n1 = ed[2 * e], n2 = ed[2 * e + 1], el1 = et[2 * e], el2 = et[2 * e + 1];
+        if (el1 < 0 || el2 < 0) {
+            nblay[n1] = 0;
+            nblay[n2] = 0;
+        } else {
+            const int v = std::min(nlevels[el1], nlevels[el2]) - 1;
+            nblay[n1] = std::min(nblay[n1], v);
+            nblay[n2] = std::min(nblay[n2], v);
+        }
+    }
+    UP(p_i, nblay.data(), nblay.size()); m.nblay = p_i;
     {
         // packed edge records (mesh arrays and the fill_up_dn_grad level rule)
         std::vector<int> ri((size_t)E2 * ERI, 0);
@@ -472,6 +488,7 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
             }
             r[ER_U1] = ulevels_nod2D[n1]; r[ER_U2] = ulevels_nod2D[n2];
             r[ER_NL1N] = nlevels_nod2D[n1]; r[ER_NL2N] = nlevels_nod2D[n2];
+            r[ER_NBL1] = nblay[n1]; r[ER_NBL2] = nblay[n2];
             q[ER_DX1] = edge_cross_dxdy[4 * e]; q[ER_DY1] = edge_cross_dxdy[4 * e + 1];
             double a = 6367500.0 * elem_cos[el1];
             if (el2 >= 0) {
@@ -691,11 +708,11 @@ extern "C" int fgpu_tracer_config(fgpu_context *c, int tr_num, int ID, int AB_or
     if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
     if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
     if (tr_num < 1 || tr_num > c->t.ntr) return fgpu_fail(FGPU_ERR_ARG, "tr_num out of range");
-    static const char *hor_names[] = {"MFCT", "UPW1"};
+    static const char *hor_names[] = {"MFCT", "UPW1", "MUSCL"};
     static const char *ver_names[] = {"QR4C", "UPW1", "CDIFF", "PPM"};
     static const char *lim_names[] = {"FCT", "NON"};
     int h, v, l;
-    if (parse_scheme(tra_adv_hor, hor_names, 2, &h))
+    if (parse_scheme(tra_adv_hor, hor_names, 3, &h))
         return fgpu_fail(FGPU_ERR_SCHEME, std::string("Unknown horizontal advection type ") +
                                               (tra_adv_hor ? tra_adv_hor : "(null)") + "! Check your namelists!");
     if (parse_scheme(tra_adv_ver, ver_names, 4, &v))

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int
     const bool act = (k >= k0 && k <= nzmax - 1);
     const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
     for (int tr = trb; tr < tre; ++tr) {
-        if (t.hor[tr] != HOR_MFCT) continue;
+        if (t.hor[tr] == HOR_UPW1) continue;   // node means feed MFCT and MUSCL
         const double *T = t.T + t.sN * tr;
         double tvol = 0.0, tx = 0.0, ty = 0.0;
         for (int i = 0; i < num; ++i) {
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(6, 8)))
             A2 = t_ab(abo, T, Told, o2);
         }
         double f = flo;
-        if (hor == HOR_MFCT) {
+        if (hor != HOR_UPW1) {   // MFCT, MUSCL
             const double no = t.ph[tr];
             double g[4] = {0.0, 0.0, 0.0, 0.0};
             if (shared) {
@@ -256,8 +256,15 @@ __global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(6, 8)))
                 if (mean1) { g[0] = gm[2 * o1]; g[2] = gm[2 * o1 + 1]; }
                 if (mean2) { g[1] = gm[2 * o2]; g[3] = gm[2 * o2 + 1]; }
             }
-            const double Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
-            const double Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
+            double Tmean2, Tmean1;
+            if (hor == HOR_MUSCL) {   // reconstruction only down to nboundary_lay (oce_adv_tra_hor.F90:356-367)
+                const double c1 = k <= ri[ER_NBL1] ? 1.0 : 0.0, c2 = k <= ri[ER_NBL2] ? 1.0 : 0.0;
+                Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0 * c2;
+                Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0 * c1;
+            } else {
+                Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
+                Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
+            }
             for (int sg = 0; sg < 2; ++sg) {
                 if ((sg == 0 ? s1 : s2) < 0) continue;
                 const double vflux = sg == 0 ? vf1 : vf2;

@@ -198,13 +198,20 @@ __global__ void __launch_bounds__(BLK) k_edge_flux(DevMesh m, DevDyn d, DevTrace
     }
     const double A1 = TAB[o1], A2 = TAB[o2];
     double f = flo;   // o_init_zero=.false. with FCT, zero otherwise
-    if (hor == HOR_MFCT) {
+    if (hor != HOR_UPW1) {   // MFCT, MUSCL
         const double no = t.ph[tr];
         double g[4];
         up_dn_grad(m, t.tr_xy + t.sStg * tr, e, n1, n2, k, g);
         const double edx = m.edge_dxdy[2 * e], edy = m.edge_dxdy[2 * e + 1];
-        const double Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
-        const double Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
+        double Tmean2, Tmean1;
+        if (hor == HOR_MUSCL) {   // reconstruction only down to nboundary_lay (oce_adv_tra_hor.F90:356-367)
+            const double c1 = k <= m.nblay[n1] ? 1.0 : 0.0, c2 = k <= m.nblay[n2] ? 1.0 : 0.0;
+            Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0 * c2;
+            Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0 * c1;
+        } else {
+            Tmean2 = A2 - (2.0 * (A2 - A1) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
+            Tmean1 = A1 + (2.0 * (A2 - A1) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
+        }
 #pragma unroll
         for (int s = 0; s < 5; ++s)
             if (in[s]) {

@@ -126,7 +126,7 @@ int fgpu_compute_ssh_rhs_ale(fgpu_context *ctx, const double *uv_rhs, const doub
 
 /* ---- tracers: t_tracer (MOD_TRACER.F90:10-109) ---------------------------
  * num_tracers tracers; each configured with its namelist entry
- * (config/namelist.tra:33-38): hor in {"MFCT","UPW1"}, ver in {"QR4C","UPW1","CDIFF","PPM"},
+ * (config/namelist.tra:33-38): hor in {"MFCT","UPW1","MUSCL"}, ver in {"QR4C","UPW1","CDIFF","PPM"},
  * lim in {"FCT","NON"} (trailing blanks ignored), tra_adv_ph/pv, AB_order 2|3,
  * i_vert_diff. tr_num is 1-based like the reference. */
 int fgpu_tracers_init(fgpu_context *ctx, int num_tracers);

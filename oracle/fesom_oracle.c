@@ -33,6 +33,7 @@
  *   23 stiffness update adds +dhe instead of -dhe        (update_stiff_mat_ale)
  *   24 PPM upward flux takes the parabola's lower edge    (adv_tra_vert_ppm)
  *   25 CDIFF interface value is the lower cell's, not the mean (adv_tra_ver_cdiff)
+ *   26 nboundary_lay not reset at the nodes of boundary edges (muscl_adv_init)
  */
 #include "fesom_oracle.h"
 
@@ -315,8 +316,11 @@ void fo_adv_tra_hor_upw1(const fo_mesh *m, const double *vel, const double *ttf,
 /* ------------------------------------------------------------------------ */
 /* adv_tra_hor_mfct: oce_adv_tra_hor.F90:546-834                             */
 /* ------------------------------------------------------------------------ */
-void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf, double num_ord,
-                         double *flux, const double *grad, int init_zero)
+/* MFCT (nblay NULL) and MUSCL: MUSCL scales each node's reconstruction term by
+   c_lo = 1 at levels down to nboundary_lay(node) and 0 below
+   (oce_adv_tra_hor.F90:261-542, c_lo at :356-357) */
+static void hor_mfct_muscl(const fo_mesh *m, const double *vel, const double *ttf, double num_ord, double *flux,
+                           const double *grad, const int *nblay, int init_zero)
 {
     const int L = m->nl - 1;
     const double w2 = (g_mut == 6) ? 3.0 : 2.0;
@@ -344,8 +348,16 @@ void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf,
             for (int nz = lo; nz <= hi; ++nz) {
                 const double T1 = ttf[LAY(q.n1, nz)], T2 = ttf[LAY(q.n2, nz)];
                 const double *g = grad + LAY(e, nz) * 4;
-                double Tmean2 = T2 - (w2 * (T2 - T1) + edx * a * g[1] + edy * R_EARTH * g[3]) / 6.0;
-                double Tmean1 = T1 + (w2 * (T2 - T1) + edx * a * g[0] + edy * R_EARTH * g[2]) / 6.0;
+                double Tmean2, Tmean1;
+                if (nblay) {
+                    const double c1 = (double)imax((nblay[q.n1] - nz) >= 0 ? 1 : -1, 0);
+                    const double c2 = (double)imax((nblay[q.n2] - nz) >= 0 ? 1 : -1, 0);
+                    Tmean2 = T2 - (w2 * (T2 - T1) + edx * a * g[1] + edy * R_EARTH * g[3]) / 6.0 * c2;
+                    Tmean1 = T1 + (w2 * (T2 - T1) + edx * a * g[0] + edy * R_EARTH * g[2]) / 6.0 * c1;
+                } else {
+                    Tmean2 = T2 - (w2 * (T2 - T1) + edx * a * g[1] + edy * R_EARTH * g[3]) / 6.0;
+                    Tmean1 = T1 + (w2 * (T2 - T1) + edx * a * g[0] + edy * R_EARTH * g[2]) / 6.0;
+                }
                 double vflux;
                 if (pass == 0 || pass == 3)
                     vflux = (-V1(1, nz) * dX1 + V1(0, nz) * dY1) * H1(nz);
@@ -364,6 +376,37 @@ void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf,
 #undef V2
 #undef H1
 #undef H2
+}
+
+void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf, double num_ord,
+                         double *flux, const double *grad, int init_zero)
+{
+    hor_mfct_muscl(m, vel, ttf, num_ord, flux, grad, NULL, init_zero);
+}
+
+void fo_adv_tra_hor_muscl(const fo_mesh *m, const double *vel, const double *ttf, double num_ord,
+                          double *flux, const double *grad, const int *nblay, int init_zero)
+{
+    hor_mfct_muscl(m, vel, ttf, num_ord, flux, grad, nblay, init_zero);
+}
+
+/* nboundary_lay of muscl_adv_init (oce_muscl_adv.F90:91-157): nl-1, reset to 0
+   at both nodes of a boundary edge, else the minimum over the node's edges of
+   min(nlevels of the two triangles) - 1, edges in order */
+void fo_muscl_nboundary_lay(const fo_mesh *m, int *nblay)
+{
+    for (int n = 0; n < m->nod2D; ++n) nblay[n] = m->nl - 1;
+    for (int e = 0; e < m->edge2D; ++e) {
+        const int n1 = m->edges[2 * e], n2 = m->edges[2 * e + 1];
+        const int el1 = m->edge_tri[2 * e], el2 = m->edge_tri[2 * e + 1];
+        if (el1 < 0 || el2 < 0) {
+            if (g_mut != 26) { nblay[n1] = 0; nblay[n2] = 0; }
+        } else {
+            const int v = imin(m->nlevels[el1], m->nlevels[el2]) - 1;
+            nblay[n1] = imin(nblay[n1], v);
+            nblay[n2] = imin(nblay[n2], v);
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -722,7 +765,7 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
 {
     const int L = m->nl - 1, NL = m->nl;
     if (d->use_wsplit) return -2;      /* adv_tra_vert_impl path is not restated */
-    if (t->hor != FO_HOR_MFCT && t->hor != FO_HOR_UPW1) return -3;
+    if (t->hor != FO_HOR_MFCT && t->hor != FO_HOR_UPW1 && t->hor != FO_HOR_MUSCL) return -3;
     if (t->ver != FO_VER_QR4C && t->ver != FO_VER_UPW1 && t->ver != FO_VER_CDIFF && t->ver != FO_VER_PPM) return -4;
     int fct = (t->lim == FO_LIM_FCT);
     if (fct) {
@@ -746,8 +789,15 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
                       * dt / m->areasvol[INT(n, nz)]) / m->hnode_new[LAY(n, nz)];
     }
     int zero = !fct;
-    if (t->hor == FO_HOR_MFCT)
+    if (t->hor == FO_HOR_MFCT) {
         fo_adv_tra_hor_mfct(m, d->uv, t->valuesAB, t->ph, w->adv_flux_hor, w->edge_up_dn_grad, zero);
+    } else if (t->hor == FO_HOR_MUSCL) {
+        int *nblay = (int *)malloc(sizeof(int) * (size_t)m->nod2D);
+        if (!nblay) return -5;
+        fo_muscl_nboundary_lay(m, nblay);
+        fo_adv_tra_hor_muscl(m, d->uv, t->valuesAB, t->ph, w->adv_flux_hor, w->edge_up_dn_grad, nblay, zero);
+        free(nblay);
+    }
     else
         fo_adv_tra_hor_upw1(m, d->uv, t->valuesAB, w->adv_flux_hor, zero);
     const double *pw = fct ? d->w : d->w_e;

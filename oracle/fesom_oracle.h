@@ -49,7 +49,7 @@ typedef struct {
     int use_wsplit;
 } fo_dyn;
 
-enum { FO_HOR_MFCT = 0, FO_HOR_UPW1 = 1 };
+enum { FO_HOR_MFCT = 0, FO_HOR_UPW1 = 1, FO_HOR_MUSCL = 2 };
 enum { FO_VER_QR4C = 0, FO_VER_UPW1 = 1, FO_VER_CDIFF = 2, FO_VER_PPM = 3 };
 enum { FO_LIM_FCT = 0, FO_LIM_NON = 1 };
 
@@ -82,6 +82,9 @@ void fo_fill_up_dn_grad(const fo_mesh *m, const double *tr_xy, double *grad);
 void fo_adv_tra_hor_upw1(const fo_mesh *m, const double *vel, const double *ttf, double *flux, int init_zero);
 void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf, double num_ord,
                          double *flux, const double *grad, int init_zero);
+void fo_adv_tra_hor_muscl(const fo_mesh *m, const double *vel, const double *ttf, double num_ord,
+                          double *flux, const double *grad, const int *nblay, int init_zero);
+void fo_muscl_nboundary_lay(const fo_mesh *m, int *nblay);
 void fo_adv_tra_ver_upw1(const fo_mesh *m, const double *w, const double *ttf, double *flux, int init_zero);
 void fo_adv_tra_ver_qr4c(const fo_mesh *m, const double *w, const double *ttf, double num_ord,
                          double *flux, int init_zero);

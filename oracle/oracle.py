@@ -56,7 +56,7 @@ class FoCgStats(C.Structure):
                 ("nnegrz", C.c_int), ("resid_last", C.c_double), ("rtol_last", C.c_double)]
 
 
-HOR = {"MFCT": 0, "UPW1": 1}
+HOR = {"MFCT": 0, "UPW1": 1, "MUSCL": 2}
 VER = {"QR4C": 0, "UPW1": 1, "CDIFF": 2, "PPM": 3}
 LIM = {"FCT": 0, "NON": 1}
 
@@ -85,6 +85,8 @@ def lib():
         L.fo_adv_tra_hor_mfct.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, dp, C.c_int]
         L.fo_adv_tra_ver_upw1.argtypes = [C.POINTER(FoMesh), dp, dp, dp, C.c_int]
         L.fo_adv_tra_ver_qr4c.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, C.c_int]
+        L.fo_adv_tra_hor_muscl.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, dp, ip, C.c_int]
+        L.fo_muscl_nboundary_lay.argtypes = [C.POINTER(FoMesh), ip]
         L.fo_adv_tra_ver_cdiff.argtypes = [C.POINTER(FoMesh), dp, dp, dp, C.c_int]
         L.fo_adv_tra_vert_ppm.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, C.c_int]
         L.fo_oce_tra_adv_fct.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, dp, dp, dp, dp, dp]
@@ -316,6 +318,18 @@ class Oracle:
                                           _d(out))
         if rc:
             raise RuntimeError(f"oracle compute_ssh_rhs_ale failed rc={rc}")
+        return out
+
+    def nboundary_lay(self):
+        out = np.zeros(self.mesh.nod2D, np.int32)
+        lib().fo_muscl_nboundary_lay(C.byref(self.fm), _i(out))
+        return out
+
+    def hor_muscl(self, uv, T, grad, num_ord=0.0):
+        out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1))
+        nb = self.nboundary_lay()
+        lib().fo_adv_tra_hor_muscl(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
+                                   float(num_ord), _d(out), _d(np.ascontiguousarray(grad)), _i(nb), 1)
         return out
 
     def ver_cdiff(self, W, T):

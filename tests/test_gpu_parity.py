@@ -100,14 +100,15 @@ def test_adv_stage_bitwise(c0, c0_oracle, tr_num):
     ctx.close()
 
 
-@pytest.mark.parametrize("sch", [dict(ver="CDIFF", lim="FCT"), dict(ver="PPM", lim="FCT"), dict(ver="PPM", lim="NON")],
-                         ids=["cdiff-fct", "ppm-fct", "ppm-non"])
-def test_adv_stage_vertical_schemes(c0, c0_oracle, sch):
-    """The per-stage kernels with the CDIFF and PPM vertical schemes: the
-    vertical flux after do_oce_adv_tra and the tracer after diff_tracers_ale are
-    bit-identical to the oracle."""
+@pytest.mark.parametrize("sch", [dict(ver="CDIFF", lim="FCT"), dict(ver="PPM", lim="FCT"), dict(ver="PPM", lim="NON"),
+                                 dict(hor="MUSCL", ver="QR4C", lim="FCT"), dict(hor="MUSCL", ver="UPW1", lim="NON")],
+                         ids=["cdiff-fct", "ppm-fct", "ppm-non", "muscl-fct", "muscl-non"])
+def test_adv_stage_schemes(c0, c0_oracle, sch):
+    """The per-stage kernels with the CDIFF and PPM vertical and the MUSCL
+    horizontal schemes: the fluxes after do_oce_adv_tra and the tracer after
+    diff_tracers_ale are bit-identical to the oracle."""
     mesh, state, _ = c0
-    full = dict(hor="MFCT", ph=0.0, pv=1.0, **sch)
+    full = {**dict(hor="MFCT", ph=0.0, pv=1.0), **sch}
     print(f"[gpu] adv stage {full}", flush=True)
     ctx = _gpu().context_for(mesh, state, schemes=full)
     ctx.init_tracers_AB(1)
@@ -119,6 +120,9 @@ def test_adv_stage_vertical_schemes(c0, c0_oracle, sch):
     ki = np.arange(1, mesh.nl + 1)
     wi = (ki[None, :] >= mesh.ulevels_nod2D[:, None]) & (ki[None, :] <= mesh.nlevels_nod2D[:, None])
     np.testing.assert_array_equal(g_av[wi], o["adv_flux_ver"][wi])
+    em = _edge_mask(mesh)
+    g_ah = ctx.work(1, "adv_flux_hor", (mesh.edge2D, mesh.nl - 1))
+    np.testing.assert_array_equal(g_ah[em], o["adv_flux_hor"][em])
     ctx.diff_tracers_ale(state.dt, 1)
     c0_oracle.diff_stage(state)
     gv, _ = ctx.tracer_download(1)
@@ -173,15 +177,18 @@ def test_autotuned_step_c0(c0, c0_oracle):
 
 
 def test_scheme_variants_c0(c0, c0_oracle):
-    """UPW1/UPW1/FCT, MFCT(ph=0.5)/QR4C(pv=0)/NON, MFCT/UPW1/NON and the CDIFF
-    and PPM vertical schemes in the fused step, against the oracle."""
+    """UPW1/UPW1/FCT, MFCT(ph=0.5)/QR4C(pv=0)/NON, MFCT/UPW1/NON, the CDIFF and
+    PPM vertical and the MUSCL horizontal schemes in the fused step, against
+    the oracle."""
     mesh, state, _ = c0
     variants = [dict(hor="UPW1", ver="UPW1", lim="FCT"),
                 dict(hor="MFCT", ver="QR4C", lim="NON", ph=0.5, pv=0.0),
                 dict(hor="MFCT", ver="UPW1", lim="NON", ph=1.0, pv=1.0),
                 dict(hor="MFCT", ver="CDIFF", lim="FCT"),
                 dict(hor="MFCT", ver="PPM", lim="FCT"),
-                dict(hor="UPW1", ver="PPM", lim="NON")]
+                dict(hor="UPW1", ver="PPM", lim="NON"),
+                dict(hor="MUSCL", ver="QR4C", lim="FCT"),
+                dict(hor="MUSCL", ver="PPM", lim="NON", ph=0.5)]
     for sch in variants:
         print(f"[gpu] scheme {sch}", flush=True)
         ctx = _gpu().context_for(mesh, state, schemes=sch)

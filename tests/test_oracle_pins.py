@@ -263,6 +263,37 @@ def check_ver_upw1(mesh, st):
     return True
 
 
+def check_muscl(mesh, st):
+    """MUSCL keeps MFCT's reconstruction where both nodes are off the walls
+    (flux bit-identical to the pinned MFCT) and drops it at nodes of boundary
+    edges: on a wall edge with ph = 0 the flux is the pinned UPW1 flux. Flat
+    bottom, so every interior node keeps its reconstruction at all levels."""
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(9)
+    T = np.where(_wet(mesh), 10.0 + rng.normal(0, 1, (mesh.nod2D, mesh.nl - 1)), 0.0)
+    grad = o.fill_up_dn_grad(o.gradient_elements(T))
+    mu = o.hor_muscl(st.uv, T, grad, 0.0)
+    mf = o.hor_mfct(st.uv, T, grad, 0.0)
+    up = o.hor_upw1(st.uv, T)
+    wall = mesh.edge_tri[:, 1] < 0
+    on_wall = np.zeros(mesh.nod2D, bool)
+    on_wall[mesh.edges[wall].ravel()] = True
+    inner = ~on_wall[mesh.edges[:, 0]] & ~on_wall[mesh.edges[:, 1]]
+    em = _edge_levels(mesh)
+    if not (inner.any() and wall.any()):
+        return False
+    ok = np.array_equal(mu[inner][em[inner]], mf[inner][em[inner]])
+    ok &= np.allclose(mu[wall][em[wall]], up[wall][em[wall]], rtol=1e-13, atol=1e-300)
+    ok &= not np.allclose(mf[wall][em[wall]], up[wall][em[wall]], rtol=1e-13, atol=1e-300)
+    return bool(ok)
+
+
+def _edge_levels(mesh):
+    nu, nl = mg.edge_level_range(mesh)
+    k = np.arange(1, mesh.nl)
+    return (k[None, :] >= nu[:, None]) & (k[None, :] <= nl[:, None])
+
+
 def _linear_cells(mesh, a, b):
     """Cell means of f(z) = a + b z (exact midpoint values) and f itself."""
     zb = mesh.zbar_3d_n
@@ -561,6 +592,10 @@ def test_vdiff_finite_volume(small):
     assert check_vdiff_fv(*small)
 
 
+def test_muscl_boundary_rule(flat):
+    assert check_muscl(*flat)
+
+
 def test_cdiff_linear(small):
     assert check_cdiff_linear(*small)
 
@@ -607,6 +642,7 @@ MUTATIONS = {
     18: lambda s, f: check_fct_bounds(*s),
     24: lambda s, f: check_ppm_linear(*s),
     25: lambda s, f: check_cdiff_linear(*s),
+    26: lambda s, f: check_muscl(*f),
 }
 
 
