@@ -473,19 +473,10 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
         sc[row] = s_;
         rn[row] = r_;
         double w = 0.0;
-        const int p0 = A.rowptr[row], p1 = A.rowptr[row + 1];
-        for (int pb = p0; pb < p1; pb += SGB) {
-            double zv[SGB], vv[SGB];
-#pragma unroll
-            for (int u = 0; u < SGB; ++u) {
-                const int p = pb + u < p1 ? pb + u : p0;
-                const int j = A.colind[p];
-                vv[u] = A.val[p];
-                zv[u] = A.dinv[j] * (rc[j] - al * (wc[j] + be * sp[j]));
-            }
-#pragma unroll
-            for (int u = 0; u < SGB; ++u)
-                if (pb + u < p1) w = w + vv[u] * zv[u];
+        for (int p = A.rowptr[row]; p < A.rowptr[row + 1]; ++p) {
+            const int j = A.colind[p];
+            const double zj = A.dinv[j] * (rc[j] - al * (wc[j] + be * sp[j]));
+            w = w + A.val[p] * zj;
         }
         wn[row] = w;
         const double z_ = A.dinv[row] * r_;

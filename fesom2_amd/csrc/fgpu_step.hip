@@ -319,13 +319,15 @@ __device__ __forceinline__ double col_qr4c(int j, int nzmin, int nzmax, const do
 // ------------------------------------------------------------------------------
 // K_B: LO solution and antidiffusive vertical flux per node column
 // ------------------------------------------------------------------------------
-template <int LB>
+// XV: the CDIFF and PPM vertical schemes are compiled in (launched only when a
+// tracer uses one; their registers and LDS would slow the QR4C/UPW1 version)
+template <int LB, bool XV>
 __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t, double dt, int tpb)
 {
     const int n = xcd_column(m.N2);
     if (n >= m.N2) return;
     __shared__ double sT[LB + 2], sA[LB + 2], sW[LB + 2], sWe[LB + 2], sAr[LB + 2], sZ[LB + 2], sZb[LB + 2];
-    __shared__ double sLo[LB + 2], sHn[LB + 2], sHo[LB + 2];
+    __shared__ double sLo[LB + 2], sHn[XV ? LB + 2 : 1], sHo[XV ? LB + 2 : 1];
     const int tid = threadIdx.x, k = tid + 1;
     const int L = m.L, NL = m.NL;
     const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
@@ -340,8 +342,10 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
     }
     if (k <= L) {
         sZ[tid] = m.Z3d[ci];
-        sHn[tid] = m.hnode_new[ci];   // PPM: thicknesses of the neighbouring layers
-        sHo[tid] = m.hnode[ci];
+        if (XV) {   // PPM: thicknesses of the neighbouring layers
+            sHn[tid] = m.hnode_new[ci];
+            sHo[tid] = m.hnode[ci];
+        }
     }
     double hn = 1.0, hh = 0.0, av_ = 1.0;
     if (act) {
@@ -372,8 +376,8 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
         if (k >= nzmin && k <= nzmax) {
             double ho;
             if (ver == VER_QR4C) ho = col_qr4c(k, nzmin, nzmax, sA, W, sAr, sZ, sZb, pv, lk);
-            else if (ver == VER_CDIFF) ho = ver_cdiff(k, nzmin, nzmax, sA, W, sAr, lk);
-            else if (ver == VER_PPM) ho = ver_ppm(k, nzmin, nzmax, sA, W, sAr, sHn, sHo, dt, lk);
+            else if (XV && ver == VER_CDIFF) ho = ver_cdiff(k, nzmin, nzmax, sA, W, sAr, lk);
+            else if (XV && ver == VER_PPM) ho = ver_ppm(k, nzmin, nzmax, sA, W, sAr, sHn, sHo, dt, lk);
             else ho = col_upw1(k, nzmin, nzmax, sA, W, sAr, lk);
             av[k - 1] = ho;
         }
@@ -768,6 +772,8 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     const unsigned ny = (unsigned)((nt + tpb - 1) / tpb);
     const int tch = (c->step_tch == 4 || c->step_tch == 8) ? c->step_tch : 2;
     const int cpb = (c->step_cpb == 4 || c->step_cpb == 8) ? c->step_cpb : 1;   // columns per block of K_D (64-lane columns)
+    bool xv = false;   // some tracer uses CDIFF or PPM
+    for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
 #define LAUNCH5(LBV)                                                                                                   \
     {                                                                                                                  \
         {                                                                                                              \
@@ -780,7 +786,10 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kB_node");                                                                                \
-            hipLaunchKernelGGL(kB_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb);        \
+            if (xv)                                                                                                    \
+                hipLaunchKernelGGL((kB_node<LBV, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
+            else                                                                                                       \
+                hipLaunchKernelGGL((kB_node<LBV, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kC_node");                                                                                \
