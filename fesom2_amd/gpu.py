@@ -203,7 +203,7 @@ class Context:
 
     def set_state(self, st):
         """Velocities, Kv and surface forcing of a meshgen.State."""
-        self.set_velocity(st.uv, st.w, st.w_e, st.w_i)
+        self.set_velocity(st.uv, st.w, st.w_e, st.w_i, use_wsplit=getattr(st, "use_wsplit", False))
         self.set_kv(st.Kv)
         self.set_surface_forcing(st.heat_flux, st.water_flux, st.virtual_salt, st.relax_salt, st.real_salt_flux)
 

@@ -126,6 +126,7 @@ class State:
     virtual_salt: np.ndarray         # (N2,)
     relax_salt: np.ndarray           # (N2,)
     real_salt_flux: np.ndarray       # (N2,)
+    use_wsplit: bool = False         # w_i carries an implicit part (dynamics%use_wsplit)
 
     @property
     def ntr(self) -> int:

@@ -34,6 +34,8 @@
  *   24 PPM upward flux takes the parabola's lower edge    (adv_tra_vert_ppm)
  *   25 CDIFF interface value is the lower cell's, not the mean (adv_tra_ver_cdiff)
  *   26 nboundary_lay not reset at the nodes of boundary edges (muscl_adv_init)
+ *   27 implicit advection's lower diagonal takes max(0, w) (adv_tra_vert_impl)
+ *   28 surface w_i term of the implicit diffusion with the wrong sign (diff_ver_part_impl_ale)
  */
 #include "fesom_oracle.h"
 
@@ -759,12 +761,83 @@ void fo_adv_tra_vert_ppm(const fo_mesh *m, double dt, const double *W, const dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* adv_tra_vert_impl: oce_adv_tra_ver.F90:90-240. Implicit first-order      */
+/* upwind vertical advection of ttf (in place) with the implicit part w_i   */
+/* of the split vertical velocity; one tridiagonal solve per column.         */
+/* ------------------------------------------------------------------------ */
+void fo_adv_tra_vert_impl(const fo_mesh *m, double dt, const double *W, double *ttf)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    double *a = (double *)calloc(NL, sizeof(double)), *b = (double *)calloc(NL, sizeof(double));
+    double *c = (double *)calloc(NL, sizeof(double)), *tr = (double *)calloc(NL, sizeof(double));
+    double *cp = (double *)calloc(NL, sizeof(double)), *tp = (double *)calloc(NL, sizeof(double));
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
+#define WI(k) W[INT(n, (k))]
+#define AR(k) m->area[INT(n, (k))]
+#define AV(k) m->areasvol[INT(n, (k))]
+#define HN(k) m->hnode_new[LAY(n, (k))]
+#define TF(k) ttf[LAY(n, (k))]
+#define I(k) ((k) - 1)
+        for (int i = 0; i < NL; ++i) { a[i] = b[i] = c[i] = tr[i] = tp[i] = cp[i] = 0.0; }
+        const double zinv = 1.0 * dt;
+        int nz = nzmin;
+        double v_adv = zinv * AR(nz) / AV(nz);
+        a[I(nz)] = 0.0;
+        b[I(nz)] = HN(nz) + WI(nz) * v_adv;
+        v_adv = zinv * AR(nz + 1) / AV(nz);
+        b[I(nz)] = b[I(nz)] - dmin(0.0, WI(nz + 1)) * v_adv;
+        c[I(nz)] = -dmax(0.0, WI(nz + 1)) * v_adv;
+        for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
+            v_adv = zinv * AR(nz) / AV(nz);
+            a[I(nz)] = (g_mut == 27 ? dmax(0.0, WI(nz)) : dmin(0.0, WI(nz))) * v_adv;
+            b[I(nz)] = HN(nz) + dmax(0.0, WI(nz)) * v_adv;
+            v_adv = zinv * AR(nz + 1) / AV(nz);
+            b[I(nz)] = b[I(nz)] - dmin(0.0, WI(nz + 1)) * v_adv;
+            c[I(nz)] = -dmax(0.0, WI(nz + 1)) * v_adv;
+        }
+        nz = nzmax - 1;
+        v_adv = zinv * AR(nz) / AV(nz);
+        a[I(nz)] = dmin(0.0, WI(nz)) * v_adv;
+        b[I(nz)] = HN(nz) + dmax(0.0, WI(nz)) * v_adv;
+        c[I(nz)] = 0.0;
+        nz = nzmin;
+        double dz = HN(nz);
+        tr[I(nz)] = -(b[I(nz)] - dz) * TF(nz) - c[I(nz)] * TF(nz + 1);
+        for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
+            dz = HN(nz);
+            tr[I(nz)] = -a[I(nz)] * TF(nz - 1) - (b[I(nz)] - dz) * TF(nz) - c[I(nz)] * TF(nz + 1);
+        }
+        nz = nzmax - 1;
+        dz = HN(nz);
+        tr[I(nz)] = -a[I(nz)] * TF(nz - 1) - (b[I(nz)] - dz) * TF(nz);
+        cp[I(nzmin)] = c[I(nzmin)] / b[I(nzmin)];
+        tp[I(nzmin)] = tr[I(nzmin)] / b[I(nzmin)];
+        for (nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+            const double mm = b[I(nz)] - cp[I(nz - 1)] * a[I(nz)];
+            cp[I(nz)] = c[I(nz)] / mm;
+            tp[I(nz)] = (tr[I(nz)] - tp[I(nz - 1)] * a[I(nz)]) / mm;
+        }
+        tr[I(nzmax - 1)] = tp[I(nzmax - 1)];
+        for (nz = nzmax - 2; nz >= nzmin; --nz) tr[I(nz)] = tp[I(nz)] - cp[I(nz)] * tr[I(nz + 1)];
+        for (nz = nzmin; nz <= nzmax - 1; ++nz) TF(nz) = TF(nz) + tr[I(nz)];
+#undef WI
+#undef AR
+#undef AV
+#undef HN
+#undef TF
+#undef I
+    }
+    free(a); free(b); free(c); free(tr); free(cp); free(tp);
+    (void)L;
+}
+
+/* ------------------------------------------------------------------------ */
 /* do_oce_adv_tra: oce_adv_tra_driver.F90:46-490 (ltra_diag and DVD off)     */
 /* ------------------------------------------------------------------------ */
 int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w)
 {
     const int L = m->nl - 1, NL = m->nl;
-    if (d->use_wsplit) return -2;      /* adv_tra_vert_impl path is not restated */
     if (t->hor != FO_HOR_MFCT && t->hor != FO_HOR_UPW1 && t->hor != FO_HOR_MUSCL) return -3;
     if (t->ver != FO_VER_QR4C && t->ver != FO_VER_UPW1 && t->ver != FO_VER_CDIFF && t->ver != FO_VER_PPM) return -4;
     int fct = (t->lim == FO_LIM_FCT);
@@ -787,6 +860,10 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
                 w->fct_LO[LAY(n, nz)] = (t->values[LAY(n, nz)] * m->hnode[LAY(n, nz)]
                     + (w->fct_LO[LAY(n, nz)] + (w->adv_flux_ver[INT(n, nz)] - w->adv_flux_ver[INT(n, nz + 1)]))
                       * dt / m->areasvol[INT(n, nz)]) / m->hnode_new[LAY(n, nz)];
+        if (d->use_wsplit) {   /* oce_adv_tra_driver.F90:323-331 */
+            fo_adv_tra_vert_impl(m, dt, d->w_i, w->fct_LO);
+            fo_adv_tra_ver_upw1(m, d->w, t->values, w->adv_flux_ver, 1);
+        }
     }
     int zero = !fct;
     if (t->hor == FO_HOR_MFCT) {
@@ -888,6 +965,11 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
             Z_n[I(nz - 1)] = zbar_n[I(nz)] + hn[I(nz - 1)] / 2.0;
         }
         zbar_n[I(nzmin)] = zbar_n[I(nzmin + 1)] + hn[I(nzmin)];
+        /* implicit part of the split vertical velocity (oce_ale_tracer.F90:961,
+           1118-1129, 1158-1172, 1194-1202): FCT tracers take it in the LO step */
+        const int wimpl = d->use_wsplit && t->lim != FO_LIM_FCT;
+        const double *wi = d->w_i + INT(n, 1);
+        const double ws = (g_mut == 28) ? -1.0 : 1.0;
         int nz = nzmin;
         double zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
         double zinv = 1.0 * dt;
@@ -895,6 +977,13 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
         a[I(nz)] = 0.0;
         c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
         b[I(nz)] = -c[I(nz)] + hn[I(nz)];
+        if (wimpl) {
+            double v_adv = zinv * (ar[I(nz)] / av[I(nz)]);
+            b[I(nz)] = b[I(nz)] + ws * wi[I(nz)] * v_adv;
+            v_adv = zinv * ar[I(nz + 1)] / av[I(nz)];
+            b[I(nz)] = b[I(nz)] - dmin(0.0, wi[I(nz + 1)]) * v_adv;
+            c[I(nz)] = c[I(nz)] - dmax(0.0, wi[I(nz + 1)]) * v_adv;
+        }
         double zinv1 = zinv2;
         for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
             zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
@@ -903,12 +992,25 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
             c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
             b[I(nz)] = -a[I(nz)] - c[I(nz)] + hn[I(nz)];
             zinv1 = zinv2;
+            if (wimpl) {
+                double v_adv = zinv * (ar[I(nz)] / av[I(nz)]);
+                a[I(nz)] = a[I(nz)] + dmin(0.0, wi[I(nz)]) * v_adv;
+                b[I(nz)] = b[I(nz)] + dmax(0.0, wi[I(nz)]) * v_adv;
+                v_adv = zinv * ar[I(nz + 1)] / av[I(nz)];
+                b[I(nz)] = b[I(nz)] - dmin(0.0, wi[I(nz + 1)]) * v_adv;
+                c[I(nz)] = c[I(nz)] - dmax(0.0, wi[I(nz + 1)]) * v_adv;
+            }
         }
         nz = nzmax - 1;
         zinv = 1.0 * dt;
         a[I(nz)] = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
         c[I(nz)] = 0.0;
         b[I(nz)] = -a[I(nz)] + hn[I(nz)];
+        if (wimpl) {
+            const double v_adv = zinv * (ar[I(nz)] / av[I(nz)]);
+            a[I(nz)] = a[I(nz)] + dmin(0.0, wi[I(nz)]) * v_adv;
+            b[I(nz)] = b[I(nz)] + dmax(0.0, wi[I(nz)]) * v_adv;
+        }
 
         nz = nzmin;
         double dz = hn[I(nz)];

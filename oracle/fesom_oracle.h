@@ -91,6 +91,7 @@ void fo_adv_tra_ver_qr4c(const fo_mesh *m, const double *w, const double *ttf, d
 void fo_adv_tra_ver_cdiff(const fo_mesh *m, const double *w, const double *ttf, double *flux, int init_zero);
 void fo_adv_tra_vert_ppm(const fo_mesh *m, double dt, const double *w, const double *ttf, double *flux,
                          int init_zero);
+void fo_adv_tra_vert_impl(const fo_mesh *m, double dt, const double *w, double *ttf);
 void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const double *lo,
                         double *adf_h, double *adf_v, double *ttf_min, double *ttf_max,
                         double *fct_plus, double *fct_minus);

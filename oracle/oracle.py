@@ -87,6 +87,7 @@ def lib():
         L.fo_adv_tra_ver_qr4c.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, C.c_int]
         L.fo_adv_tra_hor_muscl.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, dp, dp, ip, C.c_int]
         L.fo_muscl_nboundary_lay.argtypes = [C.POINTER(FoMesh), ip]
+        L.fo_adv_tra_vert_impl.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp]
         L.fo_adv_tra_ver_cdiff.argtypes = [C.POINTER(FoMesh), dp, dp, dp, C.c_int]
         L.fo_adv_tra_vert_ppm.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, C.c_int]
         L.fo_oce_tra_adv_fct.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, dp, dp, dp, dp, dp]
@@ -180,7 +181,7 @@ class Oracle:
                     rs=np.ascontiguousarray(st.relax_salt), rsf=np.ascontiguousarray(st.real_salt_flux))
         d = FoDyn(_d(keep["uv"]), _d(keep["w"]), _d(keep["w_e"]), _d(keep["w_i"]), _d(keep["Kv"]),
                   _d(keep["hf"]), _d(keep["wf"]), _d(keep["vs"]), _d(keep["rs"]), _d(keep["rsf"]),
-                  0.0, 0)
+                  0.0, int(bool(getattr(st, "use_wsplit", False))))
         return d, keep
 
     def tracer(self, values, valuesold, ID=1, AB_order=2, hor="MFCT", ver="QR4C", lim="FCT",
@@ -241,12 +242,12 @@ class Oracle:
             if rc:
                 raise RuntimeError("diff_ver_part_impl_ale failed")
 
-    def vdiff(self, st, values, ID=1, dt=None):
+    def vdiff(self, st, values, ID=1, dt=None, lim="FCT"):
         """diff_ver_part_impl_ale alone on ``values`` (in place)."""
         dt = st.dt if dt is None else dt
         d, keep = self.dyn(st)
         vo = np.zeros_like(values)
-        t, vab = self.tracer(values, vo, ID=ID)
+        t, vab = self.tracer(values, vo, ID=ID, lim=lim)
         rc = lib().fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
         if rc:
             raise RuntimeError("diff_ver_part_impl_ale failed")
@@ -330,6 +331,12 @@ class Oracle:
         nb = self.nboundary_lay()
         lib().fo_adv_tra_hor_muscl(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
                                    float(num_ord), _d(out), _d(np.ascontiguousarray(grad)), _i(nb), 1)
+        return out
+
+    def vert_impl(self, W, T, dt):
+        """adv_tra_vert_impl on a copy of T (nod2D x nl-1) with interface velocities W."""
+        out = np.array(T, dtype=np.float64, order="C", copy=True)
+        lib().fo_adv_tra_vert_impl(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(W)), _d(out))
         return out
 
     def ver_cdiff(self, W, T):

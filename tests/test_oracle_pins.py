@@ -484,6 +484,98 @@ def check_vdiff_fv(mesh, st):
     return True
 
 
+def _split_w(mesh, st, frac=0.6):
+    """An implicit part w_i of the vertical velocity: a fraction of W at the
+    interior interfaces, zero at the surface and the bottom of every column."""
+    wi = frac * st.w.copy()
+    n = np.arange(mesh.nod2D)
+    wi[n, mesh.ulevels_nod2D - 1] = 0.0
+    wi[n, mesh.nlevels_nod2D - 1] = 0.0
+    return wi
+
+
+def _upwind_divergence(mesh, n, ks, W, Tn):
+    """dt-free flux divergence of the implicit upwind advection for the layers ks
+    of column n: top-interface flux out minus bottom-interface flux in, with
+    the upwind value (the layer below for W > 0, above for W < 0) and the
+    surface interface taking the surface layer's value."""
+    m = len(ks)
+    div = np.zeros(m)
+    for j, k in enumerate(ks):
+        wt = W[n, k - 1]
+        if j == 0:
+            ft = wt * Tn[j] * mesh.area[n, k - 1]
+        else:
+            ft = wt * (Tn[j] if wt > 0 else Tn[j - 1]) * mesh.area[n, k - 1]
+        fb = 0.0
+        if j + 1 < m:
+            wb = W[n, k]
+            fb = wb * (Tn[j + 1] if wb > 0 else Tn[j]) * mesh.area[n, k]
+        div[j] = ft - fb
+    return div
+
+
+def check_vert_impl(mesh, st):
+    """Implicit upwind vertical advection: the new values satisfy the flux-form
+    balance areasvol h (T' - T) + dt (flux out through the top - flux in through
+    the bottom) = 0, built here from the upwind rule; w_i = 0 leaves T as it is."""
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(10)
+    T = np.where(_wet(mesh), rng.normal(10, 2, (mesh.nod2D, mesh.nl - 1)), 0.0)
+    W = 50.0 * st.w   # Courant numbers near one, so the implicit solve matters
+    dt = st.dt
+    Tn = o.vert_impl(W, T, dt)
+    if not np.array_equal(o.vert_impl(np.zeros_like(W), T, dt), T):
+        return False
+    for n in range(0, mesh.nod2D, 3):
+        nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+        ks = np.arange(nzmin, nzmax)
+        h = mesh.hnode_new[n, ks - 1]
+        av = mesh.areasvol[n, ks - 1]
+        res = av * h * (Tn[n, ks - 1] - T[n, ks - 1]) + dt * _upwind_divergence(mesh, n, ks, W, Tn[n, ks - 1])
+        if np.abs(res).max() > 1e-10 * np.abs(av * h * T[n, ks - 1]).max():
+            return False
+    return True
+
+
+def check_vdiff_wimpl(mesh, st):
+    """Implicit diffusion of a tracer without FCT under the W split: the banded
+    finite-volume system of check_vdiff_fv plus the implicit upwind advection
+    with w_i (flux form as in check_vert_impl)."""
+    import dataclasses
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(11)
+    T = np.where(_wet(mesh), rng.normal(10, 2, (mesh.nod2D, mesh.nl - 1)), 0.0)
+    Kv = 10 ** rng.uniform(-5, -3, st.Kv.shape)
+    wi = rng.normal(0.0, 2e-4, st.w.shape)   # Courant numbers near one, surface included
+    wi[np.arange(mesh.nod2D), mesh.nlevels_nod2D - 1] = 0.0
+    st2 = dataclasses.replace(st, Kv=Kv, w_i=wi, use_wsplit=True)
+    T1 = T.copy()
+    o.vdiff(st2, T1, ID=301, lim="NON")
+    dt = st.dt
+    for n in range(0, mesh.nod2D, 4):
+        nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+        ks = np.arange(nzmin, nzmax)
+        m = len(ks)
+        h = mesh.hnode_new[n, ks - 1]
+        zb = np.zeros(m + 1)
+        zb[m] = mesh.zbar_n_bot[n]
+        for j in range(m - 1, -1, -1):
+            zb[j] = zb[j + 1] + h[j]
+        Zc = zb[1:] + h / 2.0
+        av = mesh.areasvol[n, ks - 1]
+        # residual of the combined implicit system at the computed solution
+        Tn = T1[n, ks - 1]
+        res = av * h * (Tn - T[n, ks - 1]) + dt * _upwind_divergence(mesh, n, ks, wi, Tn)
+        for j in range(m - 1):
+            g = Kv[n, ks[j]] * dt * mesh.area[n, ks[j]] / (Zc[j] - Zc[j + 1])
+            res[j] += g * (Tn[j] - Tn[j + 1])
+            res[j + 1] -= g * (Tn[j] - Tn[j + 1])
+        if np.abs(res).max() > 1e-9 * np.abs(av * h * T[n, ks - 1]).max():
+            return False
+    return True
+
+
 def check_stiffness(mesh, st):
     o = orc.Oracle(mesh)
     rp, ci, v = o.stiff_matrix(st.dt)
@@ -596,6 +688,14 @@ def test_muscl_boundary_rule(flat):
     assert check_muscl(*flat)
 
 
+def test_vert_impl_flux_form(small):
+    assert check_vert_impl(*small)
+
+
+def test_vdiff_with_implicit_advection(small):
+    assert check_vdiff_wimpl(*small)
+
+
 def test_cdiff_linear(small):
     assert check_cdiff_linear(*small)
 
@@ -643,6 +743,8 @@ MUTATIONS = {
     24: lambda s, f: check_ppm_linear(*s),
     25: lambda s, f: check_cdiff_linear(*s),
     26: lambda s, f: check_muscl(*f),
+    27: lambda s, f: check_vert_impl(*s),
+    28: lambda s, f: check_vdiff_wimpl(*s),
 }
 
 
