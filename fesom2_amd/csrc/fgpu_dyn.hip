@@ -378,7 +378,8 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
     if (c->prof && (rc = fgpu_prof_flush(c))) return rc;
     if (cflmax) *cflmax = __builtin_bit_cast(double, flags[0]);
     if (nan_columns) *nan_columns = (int)(flags[1] & 0xffffffffull);
-    c->wsplit_active = use_wsplit != 0;
+    if (c->d.wsplit != (use_wsplit ? 1 : 0)) c->graph_valid = false;
+    c->d.wsplit = use_wsplit ? 1 : 0;
     c->vel_ready = true;
     return FGPU_OK;
 }

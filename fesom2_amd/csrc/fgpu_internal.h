@@ -60,6 +60,7 @@ struct DevDyn {
     const double *uv, *w, *w_e, *w_i, *Kv;
     const double *heat_flux, *water_flux, *virtual_salt, *relax_salt, *real_salt_flux;
     double is_nonlinfs;
+    int wsplit;   // use_wsplit: W = w_e + w_i, w_i taken implicitly (adv_tra_vert_impl, implicit diffusion)
 };
 
 // Per-tracer state, tracer-major: tracer t occupies [t*stride, (t+1)*stride).
@@ -151,7 +152,6 @@ struct fgpu_context {
     double *vv_dhe = nullptr;       // dhe of update_stiff_mat_ale (elem2D)
     double *vv_colmax = nullptr;    // per-column max of CFL_z (nod2D)
     void *vv_flags = nullptr;       // max CFL_z bits, NaN column count
-    bool wsplit_active = false;     // W split by fgpu_vert_vel_ale(use_wsplit=1): w_i != 0
 };
 
 // error helpers (fgpu_runtime.hip)

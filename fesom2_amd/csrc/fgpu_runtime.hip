@@ -576,7 +576,6 @@ extern "C" int fgpu_set_velocity(fgpu_context *c, const double *uv, const double
 {
     if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
     if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
-    if (use_wsplit) return fgpu_fail(FGPU_ERR_ARG, "use_wsplit=.true. (adv_tra_vert_impl) is not supported");
     FGPU_HIP(hipSetDevice(c->device));
     const size_t N2 = c->N2, F2 = c->F2, L = c->nl - 1, NL = c->nl;
     int rc;
@@ -585,7 +584,8 @@ extern "C" int fgpu_set_velocity(fgpu_context *c, const double *uv, const double
     if ((rc = upload_or_refresh(c, &c->d.w_e, w_e, N2 * NL))) return rc;
     if ((rc = upload_or_refresh(c, &c->d.w_i, w_i, N2 * NL))) return rc;
     c->vel_ready = true;
-    c->wsplit_active = false;
+    if (c->d.wsplit != (use_wsplit ? 1 : 0)) c->graph_valid = false;   // the step's kernels depend on it
+    c->d.wsplit = use_wsplit ? 1 : 0;
     return FGPU_OK;
 }
 

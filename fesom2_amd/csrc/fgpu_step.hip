@@ -320,14 +320,19 @@ __device__ __forceinline__ double col_qr4c(int j, int nzmin, int nzmax, const do
 // K_B: LO solution and antidiffusive vertical flux per node column
 // ------------------------------------------------------------------------------
 // XV: the CDIFF and PPM vertical schemes are compiled in (launched only when a
-// tracer uses one; their registers and LDS would slow the QR4C/UPW1 version)
-template <int LB, bool XV>
+// tracer uses one; their registers and LDS would slow the QR4C/UPW1 version).
+// WS: use_wsplit, the LO solution of FCT tracers takes the implicit upwind
+// advection with w_i (adv_tra_vert_impl, oce_adv_tra_ver.F90:90-240) and the
+// HO scheme is handed the upwind flux with the full W.
+template <int LB, bool XV, bool WS>
 __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t, double dt, int tpb)
 {
     const int n = xcd_column(m.N2);
     if (n >= m.N2) return;
     __shared__ double sT[LB + 2], sA[LB + 2], sW[LB + 2], sWe[LB + 2], sAr[LB + 2], sZ[LB + 2], sZb[LB + 2];
     __shared__ double sLo[LB + 2], sHn[XV ? LB + 2 : 1], sHo[XV ? LB + 2 : 1];
+    constexpr int NW = WS ? LB + 2 : 1;
+    __shared__ double sIa[NW], sIb[NW], sIc[NW], sIcp[NW], sIf[NW], sIr[NW];
     const int tid = threadIdx.x, k = tid + 1;
     const int L = m.L, NL = m.NL;
     const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
@@ -353,6 +358,30 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
         hh = m.hnode[ci];
         av_ = m.areasvol[(size_t)n * NL + k - 1];
     }
+    if (WS) __syncthreads();   // sAr
+    if (WS && act) {   // tridiagonal rows of the implicit upwind step (tracer independent)
+        const double *wi = d.w_i + (size_t)n * NL;
+        const double zinv = 1.0 * dt;
+        double v_adv = zinv * sAr[k - 1] / av_;
+        double a, b, c;
+        if (k == nzmin) {
+            a = 0.0;
+            b = hn + wi[k - 1] * v_adv;
+        } else {
+            a = dmn(0.0, wi[k - 1]) * v_adv;
+            b = hn + dmx(0.0, wi[k - 1]) * v_adv;
+        }
+        if (k <= nzmax - 2) {
+            v_adv = zinv * sAr[k] / av_;
+            b = b - dmn(0.0, wi[k]) * v_adv;
+            c = -dmx(0.0, wi[k]) * v_adv;
+        } else {
+            c = 0.0;
+        }
+        sIa[tid] = a;
+        sIb[tid] = b;
+        sIc[tid] = c;
+    }
     const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
     for (int tr = trb; tr < tre; ++tr) {
         const double *T = t.T + t.sN * tr;
@@ -374,11 +403,13 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
         }
         __syncthreads();
         if (k >= nzmin && k <= nzmax) {
+            // LO flux subtracted from the HO one: upwind with the full W under the split
+            const double fk = (WS && lim == LIM_FCT) ? col_upw1(k, nzmin, nzmax, sT, sW, sAr, 0.0) : lk;
             double ho;
-            if (ver == VER_QR4C) ho = col_qr4c(k, nzmin, nzmax, sA, W, sAr, sZ, sZb, pv, lk);
-            else if (XV && ver == VER_CDIFF) ho = ver_cdiff(k, nzmin, nzmax, sA, W, sAr, lk);
-            else if (XV && ver == VER_PPM) ho = ver_ppm(k, nzmin, nzmax, sA, W, sAr, sHn, sHo, dt, lk);
-            else ho = col_upw1(k, nzmin, nzmax, sA, W, sAr, lk);
+            if (ver == VER_QR4C) ho = col_qr4c(k, nzmin, nzmax, sA, W, sAr, sZ, sZb, pv, fk);
+            else if (XV && ver == VER_CDIFF) ho = ver_cdiff(k, nzmin, nzmax, sA, W, sAr, fk);
+            else if (XV && ver == VER_PPM) ho = ver_ppm(k, nzmin, nzmax, sA, W, sAr, sHn, sHo, dt, fk);
+            else ho = col_upw1(k, nzmin, nzmax, sA, W, sAr, fk);
             av[k - 1] = ho;
         }
         if (lim == LIM_FCT) {
@@ -404,9 +435,46 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
                 for (int u = 0; u < NG; ++u)
                     if (ok_[u]) acc = (q_[u] & 1) ? acc - fl_[u] : acc + fl_[u];
             }
-            if (act) {
-                const double lk1 = sLo[tid + 1];
-                t.fct_lo[t.sN * tr + ci] = (sT[tid] * hh + (acc + (lk - lk1)) * dt / av_) / hn;
+            if (!WS) {
+                if (act) {
+                    const double lk1 = sLo[tid + 1];
+                    t.fct_lo[t.sN * tr + ci] = (sT[tid] * hh + (acc + (lk - lk1)) * dt / av_) / hn;
+                }
+            } else {
+                if (act) {
+                    const double lk1 = sLo[tid + 1];
+                    sIf[tid] = (sT[tid] * hh + (acc + (lk - lk1)) * dt / av_) / hn;
+                }
+                __syncthreads();
+                if (act) {   // right-hand side of the implicit step
+                    double r;
+                    if (k == nzmin) r = -(sIb[tid] - hn) * sIf[tid] - sIc[tid] * sIf[tid + 1];
+                    else if (k <= nzmax - 2) r = -sIa[tid] * sIf[tid - 1] - (sIb[tid] - hn) * sIf[tid] - sIc[tid] * sIf[tid + 1];
+                    else r = -sIa[tid] * sIf[tid - 1] - (sIb[tid] - hn) * sIf[tid];
+                    sIr[tid] = r;
+                }
+                __syncthreads();
+                if (tid == 0) {   // Thomas sweeps of the column, one lane
+                    double cp = sIc[nzmin - 1] / sIb[nzmin - 1];
+                    double tp = sIr[nzmin - 1] / sIb[nzmin - 1];
+                    sIcp[nzmin - 1] = cp;
+                    sIr[nzmin - 1] = tp;
+                    for (int nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+                        const double an = sIa[nz - 1];
+                        const double mm = sIb[nz - 1] - cp * an;
+                        cp = sIc[nz - 1] / mm;
+                        tp = (sIr[nz - 1] - tp * an) / mm;
+                        sIcp[nz - 1] = cp;
+                        sIr[nz - 1] = tp;
+                    }
+                    double x = tp;
+                    for (int nz = nzmax - 2; nz >= nzmin; --nz) {
+                        x = sIr[nz - 1] - sIcp[nz - 1] * x;
+                        sIr[nz - 1] = x;
+                    }
+                }
+                __syncthreads();
+                if (act) t.fct_lo[t.sN * tr + ci] = sIf[tid] + sIr[tid];
             }
         }
     }
@@ -527,7 +595,9 @@ __device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &
 // run in one wave, one lane per pair, while the other waves wait: a serial
 // sweep then costs one wave's issue slots for CPB columns instead of one per
 // column. Both are launch-shape choices; the arithmetic is the same for all.
-template <int LB, int TCH, int CPB>
+// WS: use_wsplit, the non-FCT tracers' implicit diffusion also takes the
+// implicit part w_i of the split W (second set of rows).
+template <int LB, int TCH, int CPB, bool WS>
 __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer,
                                                     int tpb)
 {
@@ -539,6 +609,8 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
     __shared__ double saA[CPB][LB + 2], sbA[CPB][LB + 2], scA[CPB][LB + 2], sZA[CPB][LB + 2];
     __shared__ double sTsA[CPB][TCH][LB + 2], stpA[CPB][TCH][LB + 2], scpA[CPB][TCH][LB + 2];
     __shared__ int sNz[CPB][2];
+    constexpr int NW = WS ? LB + 2 : 1;
+    __shared__ double swaA[CPB][NW], swbA[CPB][NW], swcA[CPB][NW];
     double *sRp = sRpA[w], *sRm = sRmA[w], *sAv = sAvA[w];
     double *sa = saA[w], *sb = sbA[w], *sc = scA[w], *sZ = sZA[w];
     const int tid = threadIdx.x % LB, k = tid + 1;
@@ -582,6 +654,24 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
         sa[tid] = a;
         sb[tid] = b;
         sc[tid] = c;
+        if (WS) {   // oce_ale_tracer.F90:1118-1129, 1158-1172, 1194-1202
+            const double *wi = d.w_i + (size_t)n * NL;
+            double v_adv = zinv * (arv / av_);
+            if (k == nzmin) {
+                b = b + wi[k - 1] * v_adv;
+            } else {
+                a = a + dmn(0.0, wi[k - 1]) * v_adv;
+                b = b + dmx(0.0, wi[k - 1]) * v_adv;
+            }
+            if (k <= nzmax - 2) {
+                v_adv = zinv * m.area[(size_t)n * NL + k] / av_;
+                b = b - dmn(0.0, wi[k]) * v_adv;
+                c = c - dmx(0.0, wi[k]) * v_adv;
+            }
+            swaA[w][tid] = a;
+            swbA[w][tid] = b;
+            swcA[w][tid] = c;
+        }
     }
     const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
     for (int t0 = trb; t0 < tre; t0 += TCH) {
@@ -673,10 +763,12 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
             if (!t.ivd[tr] || !act) continue;
             const double dz = hn;
             const double *ts = sTsA[w][c];
+            const bool wim = WS && t.lim[tr] != LIM_FCT;
+            const double *ra = wim ? swaA[w] : sa, *rb = wim ? swbA[w] : sb, *rc = wim ? swcA[w] : sc;
             double r;
-            if (k == nzmin) r = -(sb[tid] - dz) * ts[tid] - sc[tid] * ts[tid + 1];
-            else if (k <= nzmax - 2) r = -sa[tid] * ts[tid - 1] - (sb[tid] - dz) * ts[tid] - sc[tid] * ts[tid + 1];
-            else r = -sa[tid] * ts[tid - 1] - (sb[tid] - dz) * ts[tid];
+            if (k == nzmin) r = -(rb[tid] - dz) * ts[tid] - rc[tid] * ts[tid + 1];
+            else if (k <= nzmax - 2) r = -ra[tid] * ts[tid - 1] - (rb[tid] - dz) * ts[tid] - rc[tid] * ts[tid + 1];
+            else r = -ra[tid] * ts[tid - 1] - (rb[tid] - dz) * ts[tid];
             if (k == nzmin) r = r + bc_surface_d(t.id[tr], dt, d, n, ts[tid]);
             stpA[w][c][tid] = r;
         }
@@ -688,7 +780,8 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
             const int wc = threadIdx.x / nch, c = threadIdx.x % nch;
             const int z0 = sNz[wc][0], z1 = sNz[wc][1];
             if (z1 - 1 >= z0 && t.ivd[t0 + c]) {
-                const double *a_ = saA[wc], *b_ = sbA[wc], *c_ = scA[wc];
+                const bool wim = WS && t.lim[t0 + c] != LIM_FCT;
+                const double *a_ = wim ? swaA[wc] : saA[wc], *b_ = wim ? swbA[wc] : sbA[wc], *c_ = wim ? swcA[wc] : scA[wc];
                 double *tp_ = stpA[wc][c], *cp_ = scpA[wc][c], *ts = sTsA[wc][c];
                 double cp = c_[z0 - 1] / b_[z0 - 1];
                 double tp = tp_[z0 - 1] / b_[z0 - 1];
@@ -774,6 +867,7 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     const int cpb = (c->step_cpb == 4 || c->step_cpb == 8) ? c->step_cpb : 1;   // columns per block of K_D (64-lane columns)
     bool xv = false;   // some tracer uses CDIFF or PPM
     for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
+    const bool ws = c->d.wsplit != 0;   // W split: implicit upwind LO step, w_i rows of the diffusion
 #define LAUNCH5(LBV)                                                                                                   \
     {                                                                                                                  \
         {                                                                                                              \
@@ -786,10 +880,12 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kB_node");                                                                                \
-            if (xv)                                                                                                    \
-                hipLaunchKernelGGL((kB_node<LBV, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
+            if (ws)                                                                                                    \
+                hipLaunchKernelGGL((kB_node<LBV, true, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
+            else if (xv)                                                                                               \
+                hipLaunchKernelGGL((kB_node<LBV, true, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
             else                                                                                                       \
-                hipLaunchKernelGGL((kB_node<LBV, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
+                hipLaunchKernelGGL((kB_node<LBV, false, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kC_node");                                                                                \
@@ -797,21 +893,23 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
-            if (LBV == 64 && cpb == 8)                                                                                 \
-                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 8 : 1)>), dim3(xcd_grid((m.N2 + 7) / 8), ny),          \
+            if (ws)                                                                                                    \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+            else if (LBV == 64 && cpb == 8)                                                                            \
+                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 8 : 1), false>), dim3(xcd_grid((m.N2 + 7) / 8), ny),          \
                                    dim3(LBV * (LBV == 64 ? 8 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
             else if (LBV == 64 && cpb == 4 && tch == 4)                                                                \
-                hipLaunchKernelGGL((kD_node<LBV, 4, (LBV == 64 ? 4 : 1)>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
+                hipLaunchKernelGGL((kD_node<LBV, 4, (LBV == 64 ? 4 : 1), false>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
                                    dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
             else if (LBV == 64 && cpb == 4)                                                                            \
-                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1)>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
+                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1), false>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
                                    dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
             else if (tch == 8)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 8, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 8, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (tch == 4)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 4, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 4, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else                                                                                                       \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
         }                                                                                                              \
     }
     // one lane per level and per interface: LB >= NL

@@ -317,13 +317,83 @@ __global__ void __launch_bounds__(BLK) k_node_lo(DevMesh m, DevDyn d, DevTracers
         const double lk1 = ver_upw1(k + 1, nzmin, nzmax, T, We, ar, 0.0);
         const double av_ = m.areasvol[(size_t)n * NL + k - 1];
         t.fct_lo[i + t.sN * tr] = (T[k - 1] * m.hnode[i] + (acc + (lk - lk1)) * dt / av_) / m.hnode_new[i];
-        av[k - 1] = ver_ho(ver, k, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, hn, ho, dt, lk);
-        if (k == nzmax - 1) av[nzmax - 1] = ver_ho(ver, nzmax, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, hn, ho, dt, lk1);
+        // with the W split the LO flux handed to the HO scheme is upwind with
+        // the full W (oce_adv_tra_driver.F90:323-331); the implicit part enters
+        // the LO solution in k_vert_impl
+        const double fk = d.wsplit ? ver_upw1(k, nzmin, nzmax, T, W, ar, 0.0) : lk;
+        av[k - 1] = ver_ho(ver, k, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, hn, ho, dt, fk);
+        if (k == nzmax - 1) {
+            const double fk1 = d.wsplit ? ver_upw1(nzmax, nzmin, nzmax, T, W, ar, 0.0) : lk1;
+            av[nzmax - 1] = ver_ho(ver, nzmax, nzmin, nzmax, TAB, W, ar, Zn, zb, pv, hn, ho, dt, fk1);
+        }
     } else {
         const double *We = d.w_e + (size_t)n * NL;
         av[k - 1] = ver_ho(ver, k, nzmin, nzmax, TAB, We, ar, Zn, zb, pv, hn, ho, dt, 0.0);
         if (k == nzmax - 1) av[nzmax - 1] = ver_ho(ver, nzmax, nzmin, nzmax, TAB, We, ar, Zn, zb, pv, hn, ho, dt, 0.0);
     }
+}
+
+// ------------------------------------------------------------------------------
+// K4b (use_wsplit only): adv_tra_vert_impl (oce_adv_tra_ver.F90:90-240) on the
+// LO solution of the FCT tracers: implicit first-order upwind advection with
+// w_i, one tridiagonal solve per column, thread per column. Coefficients are
+// formed on the fly in the forward sweep (each depends only on its own level).
+// ------------------------------------------------------------------------------
+template <int MAXL>
+__global__ void __launch_bounds__(64) k_vert_impl(DevMesh m, DevDyn d, DevTracers t, int tr0, double dt)
+{
+    const int tr = tr0 + blockIdx.y;
+    if (t.lim[tr] != LIM_FCT) return;
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n >= m.N2) return;
+    const int L = m.L, NL = m.NL;
+    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    double *TF = t.fct_lo + t.sN * tr + (size_t)n * L;
+    const double *wi = d.w_i + (size_t)n * NL;
+    const double *ar = m.area + (size_t)n * NL;
+    const double *avl = m.areasvol + (size_t)n * NL;
+    const double *hn = m.hnode_new + (size_t)n * L;
+    double cp[MAXL + 1], tp[MAXL + 1];
+#define I(kk) ((kk) - 1)
+    const double zinv = 1.0 * dt;
+    int nz = nzmin;
+    double v_adv = zinv * ar[I(nz)] / avl[I(nz)];
+    double a = 0.0;
+    double b = hn[I(nz)] + wi[I(nz)] * v_adv;
+    v_adv = zinv * ar[I(nz + 1)] / avl[I(nz)];
+    b = b - dmn(0.0, wi[I(nz + 1)]) * v_adv;
+    double c = -dmx(0.0, wi[I(nz + 1)]) * v_adv;
+    double Tm, Tk = TF[I(nz)], Tp = TF[I(nz + 1)];
+    double r = -(b - hn[I(nz)]) * Tk - c * Tp;
+    cp[I(nz)] = c / b;
+    tp[I(nz)] = r / b;
+    for (nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+        Tm = Tk;
+        Tk = Tp;
+        v_adv = zinv * ar[I(nz)] / avl[I(nz)];
+        a = dmn(0.0, wi[I(nz)]) * v_adv;
+        b = hn[I(nz)] + dmx(0.0, wi[I(nz)]) * v_adv;
+        if (nz <= nzmax - 2) {
+            v_adv = zinv * ar[I(nz + 1)] / avl[I(nz)];
+            b = b - dmn(0.0, wi[I(nz + 1)]) * v_adv;
+            c = -dmx(0.0, wi[I(nz + 1)]) * v_adv;
+            Tp = TF[I(nz + 1)];
+            r = -a * Tm - (b - hn[I(nz)]) * Tk - c * Tp;
+        } else {
+            c = 0.0;
+            r = -a * Tm - (b - hn[I(nz)]) * Tk;
+        }
+        const double mm = b - cp[I(nz - 1)] * a;
+        cp[I(nz)] = c / mm;
+        tp[I(nz)] = (r - tp[I(nz - 1)] * a) / mm;
+    }
+    double x = tp[I(nzmax - 1)];
+    TF[I(nzmax - 1)] = TF[I(nzmax - 1)] + x;
+    for (nz = nzmax - 2; nz >= nzmin; --nz) {
+        x = tp[I(nz)] - cp[I(nz)] * x;
+        TF[I(nz)] = TF[I(nz)] + x;
+    }
+#undef I
 }
 
 // ------------------------------------------------------------------------------
@@ -593,6 +663,17 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
         double a = 0.0;
         double c = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)];
         double b = -c + hn[I(nz)];
+        // implicit part of the split W (oce_ale_tracer.F90:1118-1129, 1158-1172,
+        // 1194-1202); FCT tracers took it in their LO solution
+        const bool wimpl = d.wsplit && t.lim[tr] != LIM_FCT;
+        const double *wi = d.w_i + (size_t)n * NL;
+        if (wimpl) {
+            double v_adv = zinv * (ar[I(nz)] / avl[I(nz)]);
+            b = b + wi[I(nz)] * v_adv;
+            v_adv = zinv * ar[I(nz + 1)] / avl[I(nz)];
+            b = b - dmn(0.0, wi[I(nz + 1)]) * v_adv;
+            c = c - dmx(0.0, wi[I(nz + 1)]) * v_adv;
+        }
         double zinv1 = zinv2;
         double Tm = 0.0, Tk = T[I(nz)], Tp = T[I(nz + 1)];
         double tr_ = -(b - hn[I(nz)]) * Tk - c * Tp;
@@ -608,12 +689,25 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
                 c = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)];
                 b = -a - c + hn[I(nz)];
                 zinv1 = zinv2;
+                if (wimpl) {
+                    double v_adv = zinv * (ar[I(nz)] / avl[I(nz)]);
+                    a = a + dmn(0.0, wi[I(nz)]) * v_adv;
+                    b = b + dmx(0.0, wi[I(nz)]) * v_adv;
+                    v_adv = zinv * ar[I(nz + 1)] / avl[I(nz)];
+                    b = b - dmn(0.0, wi[I(nz + 1)]) * v_adv;
+                    c = c - dmx(0.0, wi[I(nz + 1)]) * v_adv;
+                }
                 Tp = T[I(nz + 1)];
                 tr_ = -a * Tm - (b - hn[I(nz)]) * Tk - c * Tp;
             } else {
                 a = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)]);
                 c = 0.0;
                 b = -a + hn[I(nz)];
+                if (wimpl) {
+                    const double v_adv = zinv * (ar[I(nz)] / avl[I(nz)]);
+                    a = a + dmn(0.0, wi[I(nz)]) * v_adv;
+                    b = b + dmx(0.0, wi[I(nz)]) * v_adv;
+                }
                 tr_ = -a * Tm - (b - hn[I(nz)]) * Tk;
             }
             const double mm = b - cp[I(nz - 1)] * a;
@@ -682,8 +776,6 @@ static int check_ready(fgpu_context *c)
     if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
     if (!c->vert_ready) return fgpu_fail(FGPU_ERR_STATE, "vertical geometry not set (fgpu_set_vertical)");
     if (!c->vel_ready) return fgpu_fail(FGPU_ERR_STATE, "velocities not set (fgpu_set_velocity)");
-    if (c->wsplit_active)
-        return fgpu_fail(FGPU_ERR_ARG, "W was split (use_wsplit=.true.): adv_tra_vert_impl is not provided");
     if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
     FGPU_HIP(hipSetDevice(c->device));
     return FGPU_OK;
@@ -744,6 +836,16 @@ static int launch_adv(fgpu_context *c, int tr0, int ntr, double dt, int mode)
         ProfScope ps(c, "k_node_lo");
         hipLaunchKernelGGL(k_node_lo, dim3(nblk(nN), ntr), dim3(BLK), 0, c->stream, m, c->d, c->t, tr0, dt);
         LAUNCH_CHECK("k_node_lo");
+    }
+    if (c->d.wsplit) {
+        ProfScope ps(c, "k_vert_impl");
+        const dim3 g(nblk(c->N2, 64), ntr);
+        const int L = c->nl - 1;
+        if (L <= 32) hipLaunchKernelGGL(k_vert_impl<32>, g, dim3(64), 0, c->stream, m, c->d, c->t, tr0, dt);
+        else if (L <= 48) hipLaunchKernelGGL(k_vert_impl<48>, g, dim3(64), 0, c->stream, m, c->d, c->t, tr0, dt);
+        else if (L <= 72) hipLaunchKernelGGL(k_vert_impl<72>, g, dim3(64), 0, c->stream, m, c->d, c->t, tr0, dt);
+        else hipLaunchKernelGGL(k_vert_impl<FGPU_MAX_NL>, g, dim3(64), 0, c->stream, m, c->d, c->t, tr0, dt);
+        LAUNCH_CHECK("k_vert_impl");
     }
     {
         ProfScope ps(c, "k_elem_aux");

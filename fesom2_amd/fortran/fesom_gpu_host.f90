@@ -116,17 +116,27 @@ contains
          zbar_n_bot, zbar_e_bot, zbar_e_srf), 'fgpu_set_vertical')
   end subroutine gpu_set_vertical
 
-  ! velocities (t_dyn), o_ARRAYS%Kv and the bc_surface forcing arrays
+  ! velocities (t_dyn), o_ARRAYS%Kv and the bc_surface forcing arrays;
+  ! use_wsplit is dynamics%use_wsplit (default .false.)
   subroutine gpu_set_state(uv, w, w_e, w_i, Kv, heat_flux, water_flux, virtual_salt, relax_salt, &
-       real_salt_flux, is_nonlinfs)
+       real_salt_flux, is_nonlinfs, use_wsplit)
     real(WP), intent(in) :: uv(:, :, :), w(:, :), w_e(:, :), w_i(:, :), Kv(:, :)
     real(WP), intent(in) :: heat_flux(:), water_flux(:), virtual_salt(:), relax_salt(:), real_salt_flux(:)
     real(WP), intent(in) :: is_nonlinfs
-    call gpu_check(fgpu_set_velocity(ctx, uv, w, w_e, w_i, 0_c_int), 'fgpu_set_velocity')
+    logical, intent(in), optional :: use_wsplit
+    call gpu_check(fgpu_set_velocity(ctx, uv, w, w_e, w_i, wsplit_code(use_wsplit)), 'fgpu_set_velocity')
     call gpu_check(fgpu_set_kv(ctx, Kv), 'fgpu_set_kv')
     call gpu_check(fgpu_set_surface_forcing(ctx, heat_flux, water_flux, virtual_salt, relax_salt, &
          real_salt_flux, is_nonlinfs), 'fgpu_set_surface_forcing')
   end subroutine gpu_set_state
+
+  integer(c_int) function wsplit_code(use_wsplit)
+    logical, intent(in), optional :: use_wsplit
+    wsplit_code = 0_c_int
+    if (present(use_wsplit)) then
+      if (use_wsplit) wsplit_code = 1_c_int
+    end if
+  end function wsplit_code
 
   integer(c_int) function ale_code(which_ALE)
     character(len=*), intent(in) :: which_ALE
@@ -242,11 +252,14 @@ contains
 
   ! vel(2,nl-1,elem2D); W, WI, WE(nl,nod2D). Leaves del_ttf_advhoriz/advvert on
   ! the device for diff_tracers_ale, like tracers%work in the reference.
-  subroutine do_oce_adv_tra(dt, vel, w, wi, we, tr_num)
+  ! use_wsplit is dynamics%use_wsplit, which the reference reads inside
+  ! (oce_adv_tra_driver.F90:323); default .false.
+  subroutine do_oce_adv_tra(dt, vel, w, wi, we, tr_num, use_wsplit)
     real(WP), intent(in) :: dt
     real(WP), intent(in) :: vel(:, :, :), w(:, :), wi(:, :), we(:, :)
     integer, intent(in) :: tr_num
-    call gpu_check(fgpu_set_velocity(ctx, vel, w, we, wi, 0_c_int), 'do_oce_adv_tra (velocity)')
+    logical, intent(in), optional :: use_wsplit
+    call gpu_check(fgpu_set_velocity(ctx, vel, w, we, wi, wsplit_code(use_wsplit)), 'do_oce_adv_tra (velocity)')
     call gpu_check(fgpu_do_oce_adv_tra(ctx, dt, int(tr_num, c_int)), 'do_oce_adv_tra')
   end subroutine do_oce_adv_tra
 

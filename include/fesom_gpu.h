@@ -78,7 +78,10 @@ int fgpu_set_vertical(fgpu_context *ctx, const double *area, const double *areas
 
 /* ---- dynamics inputs: the vel/w/wi/we arguments of do_oce_adv_tra --------
  * (oce_adv_tra_driver.F90:46-66); uv(2,nl-1,elem2D); w, w_e, w_i (nl,nod2D).
- * use_wsplit must be 0 (adv_tra_vert_impl is not provided). */
+ * use_wsplit = 1 (dynamics%use_wsplit): W = w_e + w_i with w_i taken
+ * implicitly, in the LO solution of FCT tracers (adv_tra_vert_impl,
+ * oce_adv_tra_ver.F90:90-240, called at oce_adv_tra_driver.F90:323-331) and in
+ * the implicit vertical diffusion of the others (oce_ale_tracer.F90:1118-1202). */
 int fgpu_set_velocity(fgpu_context *ctx, const double *uv, const double *w, const double *w_e,
                       const double *w_i, int use_wsplit);
 /* o_ARRAYS%Kv(nl,nod2D), read by diff_ver_part_impl_ale (oce_ale_tracer.F90:914) */
@@ -105,9 +108,9 @@ int fgpu_set_uv(fgpu_context *ctx, const double *uv);
  * or zstar; zstar also reads hbar, hbar_old (nod2D, host) and the water flux
  * of fgpu_set_surface_forcing, and updates hnode_new and the implicit
  * diffusion's Z levels. Levels outside [ulevels_nod2D, nlevels_nod2D] of w_e
- * and w_i are written as zero. With use_wsplit = 1 the split follows the
- * reference, but the transport then refuses to step (adv_tra_vert_impl is not
- * provided). cflmax: max of CFL_z (may be NULL); nan_columns: columns with a
+ * and w_i are written as zero. use_wsplit = 1 splits W where CFL_z exceeds
+ * wsplit_maxcfl and turns on the implicit w_i terms of the transport (as
+ * fgpu_set_velocity with use_wsplit = 1). cflmax: max of CFL_z (may be NULL); nan_columns: columns with a
  * NaN in W, which the reference reports (may be NULL). */
 int fgpu_vert_vel_ale(fgpu_context *ctx, double dt, int which_ale, const double *hbar, const double *hbar_old,
                       int use_wsplit, double wsplit_maxcfl, double *cflmax, int *nan_columns);

@@ -52,26 +52,106 @@ def test_vert_vel_linfs_bitwise(cfg, c0, c0_oracle, request):
     assert cmax == ref["cflmax"] and nan == 0
 
 
-def test_vert_vel_wsplit_bitwise_and_transport_refuses(c0, c0_oracle):
-    from fesom2_amd import gpu
+def _split(o, state, frac=0.3):
+    """The oracle's vert_vel_ale with W split at frac x the unsplit maximum CFL."""
+    cap = frac * o.vert_vel_ale(state.uv, state.dt)["cflmax"]
+    return cap, o.vert_vel_ale(state.uv, state.dt, use_wsplit=True, wsplit_maxcfl=cap)
+
+
+def test_vert_vel_wsplit_bitwise(c0, c0_oracle):
     mesh, state, _ = c0
-    ref0 = c0_oracle.vert_vel_ale(state.uv, state.dt)
-    cap = 0.3 * ref0["cflmax"]
+    cap, ref = _split(c0_oracle, state)
     print(f"[gpu] vert_vel_ale with W split at {cap:.3e}", flush=True)
     ctx = _ctx(mesh, state)
     try:
         ctx.set_uv(state.uv)
         ctx.vert_vel_ale(state.dt, use_wsplit=True, wsplit_maxcfl=cap)
         dev = ctx.vert_vel_download()
-        with pytest.raises(gpu.FesomGpuError):
-            ctx.solve_tracers_ale(state.dt, 1)
-        ctx.set_velocity(state.uv, state.w, state.w_e, state.w_i)   # explicit W again: the transport runs
-        ctx.solve_tracers_ale(state.dt, 1)
     finally:
         ctx.close()
-    ref = c0_oracle.vert_vel_ale(state.uv, state.dt, use_wsplit=True, wsplit_maxcfl=cap)
     assert np.any(ref["w_i"] != 0.0)
     _compare_vv(dev, ref)
+
+
+_WS_SCHEMES = {
+    "qr4c": [dict(lim="FCT"), dict(lim="NON")],
+    "ppm-upw1": [dict(hor="UPW1", ver="PPM", lim="FCT"), dict(hor="MUSCL", ver="UPW1", lim="NON", ph=0.5)],
+}
+
+
+@pytest.mark.parametrize("path", ["fused", "staged"])
+@pytest.mark.parametrize("sch", list(_WS_SCHEMES), ids=list(_WS_SCHEMES))
+def test_wsplit_transport_bitwise(c0, c0_oracle, monkeypatch, path, sch):
+    """use_wsplit: the W split by the device's vert_vel_ale drives two batched
+    steps of an FCT tracer (w_i in the implicit upwind LO step,
+    adv_tra_vert_impl) and a non-FCT tracer (w_i in the implicit diffusion).
+    Every wet value equals the oracle's, given the oracle's split; the split
+    changes the result."""
+    from fesom2_amd import gpu
+    mesh, state, _ = c0
+    schemes = _WS_SCHEMES[sch]
+    cap, ref = _split(c0_oracle, state)
+    st2 = dataclasses.replace(state, w=ref["w"], w_e=ref["w_e"], w_i=ref["w_i"], use_wsplit=True)
+    v, vo = state.values.copy(), state.valuesold.copy()
+    for _ in range(2):
+        c0_oracle.step(st2, v, vo, schemes=schemes)
+    st3 = dataclasses.replace(st2, use_wsplit=False)
+    v3, vo3 = state.values.copy(), state.valuesold.copy()
+    for _ in range(2):
+        c0_oracle.step(st3, v3, vo3, schemes=schemes)
+    if path == "staged":
+        monkeypatch.setenv("FGPU_STEP", "staged")
+    print(f"[gpu] W split -> 2 {path} steps, schemes {sch}", flush=True)
+    ctx = gpu.context_for(mesh, state, schemes=schemes)
+    try:
+        ctx.set_uv(state.uv)
+        ctx.vert_vel_ale(state.dt, use_wsplit=True, wsplit_maxcfl=cap)
+        ctx.solve_tracers_ale(state.dt, 2)
+        out = [ctx.tracer_download(i + 1) for i in range(state.ntr)]
+    finally:
+        ctx.close()
+    L = mesh.nl - 1
+    k = np.arange(1, L + 1)
+    wet = (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+    for i in range(state.ntr):
+        np.testing.assert_array_equal(out[i][0][wet], v[i][wet], err_msg=f"tracer {i + 1}")
+        np.testing.assert_array_equal(out[i][1][wet], vo[i][wet], err_msg=f"tracer {i + 1} valuesold")
+        assert np.any(v3[i][wet] != v[i][wet])
+
+
+@pytest.mark.parametrize("lim", ["FCT", "NON"])
+def test_wsplit_stages_bitwise(c0, c0_oracle, lim):
+    """Per-stage entry points with a host-given split (set_velocity with
+    use_wsplit): the LO solution after adv_tra_vert_impl, the antidiffusive
+    vertical flux built on the full-W upwind flux, and the tracer after the
+    diffusion with the w_i rows."""
+    from fesom2_amd import gpu
+    mesh, state, _ = c0
+    cap, ref = _split(c0_oracle, state, frac=0.2)
+    st2 = dataclasses.replace(state, w=ref["w"], w_e=ref["w_e"], w_i=ref["w_i"], use_wsplit=True)
+    sch = dict(hor="MFCT", ver="QR4C", lim=lim)
+    print(f"[gpu] W split stages lim={lim}", flush=True)
+    ctx = gpu.context_for(mesh, st2, schemes=sch)
+    try:
+        ctx.init_tracers_AB(1)
+        ctx.do_oce_adv_tra(state.dt, 1)
+        v = state.values[0].copy()
+        vo = state.valuesold[0].copy()
+        o = c0_oracle.adv_stage(st2, v, vo, ID=int(state.ids[0]), **sch)
+        L = mesh.nl - 1
+        k = np.arange(1, L + 1)
+        wet = (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+        ki = np.arange(1, mesh.nl + 1)
+        wi = (ki[None, :] >= mesh.ulevels_nod2D[:, None]) & (ki[None, :] <= mesh.nlevels_nod2D[:, None])
+        if lim == "FCT":
+            np.testing.assert_array_equal(ctx.work(1, "fct_LO", (mesh.nod2D, L))[wet], o["fct_LO"][wet])
+        np.testing.assert_array_equal(ctx.work(1, "adv_flux_ver", (mesh.nod2D, mesh.nl))[wi], o["adv_flux_ver"][wi])
+        ctx.diff_tracers_ale(state.dt, 1)
+        c0_oracle.diff_stage(st2)
+        gv, _ = ctx.tracer_download(1)
+    finally:
+        ctx.close()
+    np.testing.assert_array_equal(gv[wet], v[wet])
 
 
 def test_vert_vel_zstar_bitwise(c0, c0_oracle):
