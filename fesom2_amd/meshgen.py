@@ -127,6 +127,9 @@ class State:
     relax_salt: np.ndarray           # (N2,)
     real_salt_flux: np.ndarray       # (N2,)
     use_wsplit: bool = False         # w_i carries an implicit part (dynamics%use_wsplit)
+    Ki: Optional[np.ndarray] = None  # (N2, nl-1) horizontal/isoneutral diffusivity; None: Ki = 0
+    slope_tapered: Optional[np.ndarray] = None  # (N2, nl-1, 3) neutral slopes, read when redi
+    redi: bool = False               # Redi (oce_modules.F90:75)
 
     @property
     def ntr(self) -> int:

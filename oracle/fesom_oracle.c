@@ -36,6 +36,13 @@
  *   26 nboundary_lay not reset at the nodes of boundary edges (muscl_adv_init)
  *   27 implicit advection's lower diagonal takes max(0, w) (adv_tra_vert_impl)
  *   28 surface w_i term of the implicit diffusion with the wrong sign (diff_ver_part_impl_ale)
+ *   29 explicit vertical diffusion divides by the interface spacing above (diff_ver_part_expl_ale)
+ *   30 horizontal diffusion, two-sided levels: second element's dual segment with the wrong sign
+ *      (diff_part_hor_redi)
+ *   31 horizontal diffusion takes Ki of the first node only (diff_part_hor_redi)
+ *   32 Redi vertical flux weights the upper layer by the distance to the interface below
+ *      (diff_ver_part_redi_expl)
+ *   33 Redi K33 below an interior interface scaled by the spacing below (diff_ver_part_impl_ale)
  */
 #include "fesom_oracle.h"
 
@@ -241,8 +248,27 @@ int fo_init_tracers_AB(const fo_mesh *m, fo_tracer *t, fo_work *w)
     }
     /* gradients from values, not valuesAB (oce_tracer_mod.F90:126-131) */
     fo_tracer_gradient_elements(m, t->values, w->tr_xy);
+    if (w->tr_z) fo_tracer_gradient_z(m, t->values, w->tr_z);
     fo_fill_up_dn_grad(m, w->tr_xy, w->edge_up_dn_grad);
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* tracer_gradient_z: oce_tracer_mod.F90:192-232 (old thicknesses hnode)     */
+/* ------------------------------------------------------------------------ */
+void fo_tracer_gradient_z(const fo_mesh *m, const double *ttf, double *tr_z)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
+        for (int nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+            const double dz = 0.5 * (m->hnode[LAY(n, nz - 1)] + m->hnode[LAY(n, nz)]);
+            tr_z[INT(n, nz)] = (ttf[LAY(n, nz - 1)] - ttf[LAY(n, nz)]) / dz;
+        }
+        tr_z[INT(n, nzmin)] = 0.0;
+        tr_z[INT(n, nzmax)] = 0.0;
+    }
+    (void)L;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -901,19 +927,206 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
 /* ------------------------------------------------------------------------ */
 /* T* update: oce_ale_tracer.F90:318-322 and 742-755                         */
 /* ------------------------------------------------------------------------ */
-void fo_diff_tracers_ale_tstar(const fo_mesh *m, fo_tracer *t, fo_work *w)
+void fo_diff_tracers_ale_tstar(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w)
 {
     const int L = m->nl - 1;
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = 1; nz <= L; ++nz)
             w->del_ttf[LAY(n, nz)] = w->del_ttf[LAY(n, nz)] + w->del_ttf_advhoriz[LAY(n, nz)]
                                      + w->del_ttf_advvert[LAY(n, nz)];
+    /* diff_tracers_ale (oce_ale_tracer.F90:595-626): horizontal (and Redi)
+       diffusion, explicit vertical diffusion when i_vert_diff is off, the
+       explicit vertical Redi part; Ki = 0 makes the first a no-op */
+    if (d && d->Ki) fo_diff_part_hor_redi(m, d, dt, w->tr_xy, w->tr_z, w->del_ttf);
+    if (d && !t->i_vert_diff) fo_diff_ver_part_expl_ale(m, d, dt, t, w->del_ttf);
+    if (d && d->redi && d->Ki && d->slope_tapered) fo_diff_ver_part_redi_expl(m, d, dt, w->tr_xy, w->del_ttf);
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
             w->del_ttf[LAY(n, nz)] = w->del_ttf[LAY(n, nz)]
                 + t->values[LAY(n, nz)] * (m->hnode[LAY(n, nz)] - m->hnode_new[LAY(n, nz)]);
             t->values[LAY(n, nz)] = t->values[LAY(n, nz)] + w->del_ttf[LAY(n, nz)] / m->hnode_new[LAY(n, nz)];
         }
+}
+
+/* ------------------------------------------------------------------------ */
+/* diff_part_hor_redi: oce_ale_tracer.F90:1526-1689. Per edge and level the */
+/* flux K (grad T + isredi S T_z) through the edge's dual segments, cases    */
+/* (A)-(E) for levels where one or both elements exist; scattered to both   */
+/* nodes in edge order.                                                      */
+/* ------------------------------------------------------------------------ */
+void fo_diff_part_hor_redi(const fo_mesh *m, const fo_dyn *d, double dt, const double *tr_xy, const double *tr_z,
+                           double *del_ttf)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    const double isredi = d->redi ? 1.0 : 0.0;
+    double *rhs1 = (double *)calloc(L + 1, sizeof(double)), *rhs2 = (double *)calloc(L + 1, sizeof(double));
+    for (int edge = 0; edge < m->edge2D; ++edge) {
+        for (int i = 0; i <= L; ++i) rhs1[i] = rhs2[i] = 0.0;
+        const double dX1 = m->edge_cross_dxdy[4 * edge + 0], dY1 = m->edge_cross_dxdy[4 * edge + 1];
+        const int el1 = m->edge_tri[2 * edge], el2 = m->edge_tri[2 * edge + 1];
+        const int n1 = m->edges[2 * edge], n2 = m->edges[2 * edge + 1];
+        const int nl1 = m->nlevels[el1] - 1, ul1 = m->ulevels[el1];
+        int nl2 = 0, ul2 = 0;
+        double dX2 = 0.0, dY2 = 0.0;
+        if (el2 >= 0) {
+            nl2 = m->nlevels[el2] - 1;
+            ul2 = m->ulevels[el2];
+            dX2 = m->edge_cross_dxdy[4 * edge + 2];
+            dY2 = m->edge_cross_dxdy[4 * edge + 3];
+        }
+        const int nl12 = imin(nl1, nl2), ul12 = imax(ul1, ul2);
+#define TXY(c, nz, el) tr_xy[((size_t)(el) * L + (nz) - 1) * 2 + (c)]
+        /* Kh, Fx, Fy at level nz from the gradient (Tx, Ty) */
+#define FLUX(nz)                                                                                            \
+    const double Kh = (g_mut == 31 ? 2.0 * d->Ki[LAY(n1, nz)] : d->Ki[LAY(n1, nz)] + d->Ki[LAY(n2, nz)]) / 2.0; \
+    double SxTz = 0.0, SyTz = 0.0;                                                                          \
+    if (d->slope_tapered) {                                                                                 \
+        const double Tz1 = 0.5 * (tr_z[INT(n1, nz)] + tr_z[INT(n1, (nz) + 1)]);                             \
+        const double Tz2 = 0.5 * (tr_z[INT(n2, nz)] + tr_z[INT(n2, (nz) + 1)]);                             \
+        SxTz = (Tz1 * d->slope_tapered[LAY(n1, nz) * 3 + 0] + Tz2 * d->slope_tapered[LAY(n2, nz) * 3 + 0]) / 2.0; \
+        SyTz = (Tz1 * d->slope_tapered[LAY(n1, nz) * 3 + 1] + Tz2 * d->slope_tapered[LAY(n2, nz) * 3 + 1]) / 2.0; \
+    }                                                                                                       \
+    const double Fx = Kh * (Tx + SxTz * isredi);                                                            \
+    const double Fy = Kh * (Ty + SyTz * isredi);
+        for (int nz = ul1; nz <= ul12 - 1; ++nz) {   /* (A) */
+            const double dz = m->helem[LAY(el1, nz)];
+            const double Tx = TXY(0, nz, el1), Ty = TXY(1, nz, el1);
+            FLUX(nz)
+            const double c = (-dX1 * Fy + dY1 * Fx) * dz;
+            rhs1[nz] = rhs1[nz] + c;
+            rhs2[nz] = rhs2[nz] - c;
+        }
+        if (ul2 > 0)
+            for (int nz = ul2; nz <= ul12 - 1; ++nz) {   /* (B) */
+                const double dz = m->helem[LAY(el2, nz)];
+                const double Tx = TXY(0, nz, el2), Ty = TXY(1, nz, el2);
+                FLUX(nz)
+                const double c = (dX2 * Fy - dY2 * Fx) * dz;
+                rhs1[nz] = rhs1[nz] + c;
+                rhs2[nz] = rhs2[nz] - c;
+            }
+        for (int nz = ul12; nz <= nl12; ++nz) {   /* (C) */
+            const double dz = (m->helem[LAY(el1, nz)] + m->helem[LAY(el2, nz)]) / 2.0;
+            const double Tx = 0.5 * (TXY(0, nz, el1) + TXY(0, nz, el2));
+            const double Ty = 0.5 * (TXY(1, nz, el1) + TXY(1, nz, el2));
+            FLUX(nz)
+            const double dX2m = (g_mut == 30) ? -dX2 : dX2, dY2m = (g_mut == 30) ? -dY2 : dY2;
+            const double c = ((dX2m - dX1) * Fy - (dY2m - dY1) * Fx) * dz;
+            rhs1[nz] = rhs1[nz] + c;
+            rhs2[nz] = rhs2[nz] - c;
+        }
+        for (int nz = nl12 + 1; nz <= nl1; ++nz) {   /* (D) */
+            const double dz = m->helem[LAY(el1, nz)];
+            const double Tx = TXY(0, nz, el1), Ty = TXY(1, nz, el1);
+            FLUX(nz)
+            const double c = (-dX1 * Fy + dY1 * Fx) * dz;
+            rhs1[nz] = rhs1[nz] + c;
+            rhs2[nz] = rhs2[nz] - c;
+        }
+        for (int nz = nl12 + 1; nz <= nl2; ++nz) {   /* (E) */
+            const double dz = m->helem[LAY(el2, nz)];
+            const double Tx = TXY(0, nz, el2), Ty = TXY(1, nz, el2);
+            FLUX(nz)
+            const double c = (dX2 * Fy - dY2 * Fx) * dz;
+            rhs1[nz] = rhs1[nz] + c;
+            rhs2[nz] = rhs2[nz] - c;
+        }
+#undef FLUX
+#undef TXY
+        const int nls = imax(nl1, nl2);
+        int uls = ul1;
+        if (ul2 > 0) uls = imin(ul1, ul2);
+        for (int nz = uls; nz <= nls; ++nz)
+            del_ttf[LAY(n1, nz)] = del_ttf[LAY(n1, nz)] + rhs1[nz] * dt / m->areasvol[INT(n1, nz)];
+        for (int nz = uls; nz <= nls; ++nz)
+            del_ttf[LAY(n2, nz)] = del_ttf[LAY(n2, nz)] + rhs2[nz] * dt / m->areasvol[INT(n2, nz)];
+    }
+    free(rhs1);
+    free(rhs2);
+    (void)NL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* diff_ver_part_expl_ale: oce_ale_tracer.F90:841-902 (i_vert_diff off)      */
+/* ------------------------------------------------------------------------ */
+void fo_diff_ver_part_expl_ale(const fo_mesh *m, const fo_dyn *d, double dt, const fo_tracer *t, double *del_ttf)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    double *vd = (double *)calloc(NL + 1, sizeof(double));
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int nl1 = m->nlevels_nod2D[n] - 1, ul1 = m->ulevels_nod2D[n];
+        for (int i = 0; i <= NL; ++i) vd[i] = 0.0;
+        double flux;
+        if (t->ID == 1) flux = -d->heat_flux[n] / VCPW;
+        else if (t->ID == 2) flux = d->virtual_salt[n] + d->relax_salt[n] + d->real_salt_flux[n] * d->is_nonlinfs;
+        else flux = 0.0;
+        vd[ul1] = flux;
+        for (int nz = ul1 + 1; nz <= nl1; ++nz) {
+            const double zinv1 = (g_mut == 29) ? 1.0 / (m->zbar_3d_n[INT(n, nz - 1)] - m->zbar_3d_n[INT(n, nz)])
+                                               : 1.0 / (m->Z_3d_n[LAY(n, nz - 1)] - m->Z_3d_n[LAY(n, nz)]);
+            vd[nz] = d->Kv[INT(n, nz)] * (t->values[LAY(n, nz - 1)] - t->values[LAY(n, nz)]) * zinv1
+                     * m->area[INT(n, nz)];
+        }
+        for (int nz = ul1; nz <= nl1 - 1; ++nz)
+            del_ttf[LAY(n, nz)] = del_ttf[LAY(n, nz)] + (vd[nz] - vd[nz + 1])
+                / (m->zbar_3d_n[INT(n, nz)] - m->zbar_3d_n[INT(n, nz + 1)]) * dt / m->areasvol[INT(n, nz)];
+        del_ttf[LAY(n, nl1)] = del_ttf[LAY(n, nl1)]
+            + (vd[nl1] / (m->zbar_3d_n[INT(n, nl1)] - m->zbar_3d_n[INT(n, nl1 + 1)])) * dt / m->areasvol[INT(n, nl1)];
+    }
+    free(vd);
+    (void)L;
+}
+
+/* ------------------------------------------------------------------------ */
+/* diff_ver_part_redi_expl: oce_ale_tracer.F90:1439-1522. Node gradients    */
+/* (element gradients weighted by area / 3 / areasvol), then the vertical    */
+/* flux Ki (S . grad T) on the old thicknesses.                              */
+/* ------------------------------------------------------------------------ */
+void fo_diff_ver_part_redi_expl(const fo_mesh *m, const fo_dyn *d, double dt, const double *tr_xy,
+                                double *del_ttf)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    double *txyn = (double *)calloc((size_t)m->nod2D * L * 2, sizeof(double));
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int nl1 = m->nlevels_nod2D[n] - 1, ul1 = m->ulevels_nod2D[n];
+        for (int nz = ul1; nz <= nl1; ++nz) {
+            double Tx = 0.0, Ty = 0.0;
+            for (int k = 0; k < m->nod_in_elem2D_num[n]; ++k) {
+                const int elem = m->nod_in_elem2D[(size_t)n * m->kmax + k];
+                if (nz <= m->nlevels[elem] - 1 && nz >= m->ulevels[elem]) {
+                    Tx = Tx + tr_xy[((size_t)elem * L + nz - 1) * 2 + 0] * m->elem_area[elem];
+                    Ty = Ty + tr_xy[((size_t)elem * L + nz - 1) * 2 + 1] * m->elem_area[elem];
+                }
+            }
+            txyn[LAY(n, nz) * 2 + 0] = Tx / 3.0 / m->areasvol[INT(n, nz)];
+            txyn[LAY(n, nz) * 2 + 1] = Ty / 3.0 / m->areasvol[INT(n, nz)];
+        }
+    }
+    double *vd = (double *)calloc(NL + 2, sizeof(double)), *zbar_n = (double *)calloc(NL + 2, sizeof(double));
+    double *z_n = (double *)calloc(NL + 2, sizeof(double));
+    const double *S = d->slope_tapered;
+#define SG(nz) (S[LAY(n, nz) * 3 + 0] * txyn[LAY(n, nz) * 2 + 0] + S[LAY(n, nz) * 3 + 1] * txyn[LAY(n, nz) * 2 + 1])
+    for (int n = 0; n < m->nod2D; ++n) {
+        const int nl1 = m->nlevels_nod2D[n] - 1, ul1 = m->ulevels_nod2D[n];
+        for (int i = 0; i <= NL + 1; ++i) vd[i] = zbar_n[i] = z_n[i] = 0.0;
+        zbar_n[nl1 + 1] = m->zbar_n_bot[n];
+        z_n[nl1] = zbar_n[nl1 + 1] + m->hnode[LAY(n, nl1)] / 2.0;
+        for (int nz = nl1; nz >= ul1 + 1; --nz) {
+            zbar_n[nz] = zbar_n[nz + 1] + m->hnode[LAY(n, nz)];
+            z_n[nz - 1] = zbar_n[nz] + m->hnode[LAY(n, nz - 1)] / 2.0;
+        }
+        zbar_n[ul1] = zbar_n[ul1 + 1] + m->hnode[LAY(n, ul1)];
+        for (int nz = ul1 + 1; nz <= nl1; ++nz) {
+            const double wup = (g_mut == 32) ? (z_n[nz - 1] - zbar_n[nz + 1]) : (z_n[nz - 1] - zbar_n[nz]);
+            vd[nz] = wup * SG(nz - 1) * d->Ki[LAY(n, nz - 1)];
+            vd[nz] = vd[nz] + (zbar_n[nz] - z_n[nz]) * SG(nz) * d->Ki[LAY(n, nz)];
+            vd[nz] = vd[nz] / (z_n[nz - 1] - z_n[nz]) * m->area[INT(n, nz)];
+        }
+        for (int nz = ul1; nz <= nl1; ++nz)
+            del_ttf[LAY(n, nz)] = del_ttf[LAY(n, nz)] + (vd[nz] - vd[nz + 1]) * dt / m->areasvol[INT(n, nz)];
+    }
+#undef SG
+    free(txyn); free(vd); free(zbar_n); free(z_n);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -974,8 +1187,19 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
         double zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
         double zinv = 1.0 * dt;
         int ic = (g_mut == 12) ? nz : nz + 1;
+        /* Redi: K33 = Kv + Ki S3^2 interpolated to the interfaces
+           (oce_ale_tracer.F90:1105-1108, 1138-1144, 1181-1184) */
+        const int redi = d->redi && d->Ki && d->slope_tapered;
+#define S3(k) d->slope_tapered[LAY(n, (k)) * 3 + 2]
+#define KI(k) d->Ki[LAY(n, (k))]
         a[I(nz)] = 0.0;
-        c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+        if (redi) {
+            const double Ty1 = (zbar_n[I(nz + 1)] - Z_n[I(nz + 1)]) * zinv2 * (S3(nz) * S3(nz)) * KI(nz)
+                               + (Z_n[I(nz)] - zbar_n[I(nz + 1)]) * zinv2 * (S3(nz + 1) * S3(nz + 1)) * KI(nz + 1);
+            c[I(nz)] = -(kv[I(nz + 1)] + Ty1) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+        } else {
+            c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+        }
         b[I(nz)] = -c[I(nz)] + hn[I(nz)];
         if (wimpl) {
             double v_adv = zinv * (ar[I(nz)] / av[I(nz)]);
@@ -988,8 +1212,18 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
         for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
             zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
             ic = (g_mut == 12) ? nz : nz + 1;
-            a[I(nz)] = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
-            c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+            if (redi) {
+                const double zt = (g_mut == 33) ? zinv2 : zinv1;
+                const double Ty = (zbar_n[I(nz)] - Z_n[I(nz)]) * zt * (S3(nz - 1) * S3(nz - 1)) * KI(nz - 1)
+                                  + (Z_n[I(nz - 1)] - zbar_n[I(nz)]) * zt * (S3(nz) * S3(nz)) * KI(nz);
+                const double Ty1 = (zbar_n[I(nz + 1)] - Z_n[I(nz + 1)]) * zinv2 * (S3(nz) * S3(nz)) * KI(nz)
+                                   + (Z_n[I(nz)] - zbar_n[I(nz + 1)]) * zinv2 * (S3(nz + 1) * S3(nz + 1)) * KI(nz + 1);
+                a[I(nz)] = -(kv[I(nz)] + Ty) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
+                c[I(nz)] = -(kv[I(nz + 1)] + Ty1) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+            } else {
+                a[I(nz)] = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
+                c[I(nz)] = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(ic)] / av[I(nz)];
+            }
             b[I(nz)] = -a[I(nz)] - c[I(nz)] + hn[I(nz)];
             zinv1 = zinv2;
             if (wimpl) {
@@ -1003,7 +1237,15 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
         }
         nz = nzmax - 1;
         zinv = 1.0 * dt;
-        a[I(nz)] = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
+        if (redi) {
+            const double Ty = (zbar_n[I(nz)] - Z_n[I(nz)]) * zinv1 * (S3(nz - 1) * S3(nz - 1)) * KI(nz - 1)
+                              + (Z_n[I(nz - 1)] - zbar_n[I(nz)]) * zinv1 * (S3(nz) * S3(nz)) * KI(nz);
+            a[I(nz)] = -(kv[I(nz)] + Ty) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
+        } else {
+            a[I(nz)] = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / av[I(nz)]);
+        }
+#undef S3
+#undef KI
         c[I(nz)] = 0.0;
         b[I(nz)] = -a[I(nz)] + hn[I(nz)];
         if (wimpl) {
@@ -1058,7 +1300,7 @@ int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer
         if (rc) return rc;
         rc = fo_do_oce_adv_tra(m, d, dt, &t[i], w);
         if (rc) return rc;
-        fo_diff_tracers_ale_tstar(m, &t[i], w);
+        fo_diff_tracers_ale_tstar(m, d, dt, &t[i], w);
         if (t[i].i_vert_diff) {
             rc = fo_diff_ver_part_impl_ale(m, d, dt, &t[i]);
             if (rc) return rc;

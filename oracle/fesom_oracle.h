@@ -47,6 +47,9 @@ typedef struct {
     const double *heat_flux, *water_flux, *virtual_salt, *relax_salt, *real_salt_flux;
     double is_nonlinfs;
     int use_wsplit;
+    const double *Ki;             /* N2*L (o_ARRAYS Ki); NULL: Ki = 0, no horizontal diffusion */
+    const double *slope_tapered;  /* N2*L*3, component fastest; read when redi */
+    int redi;                     /* Redi (oce_modules.F90:75) */
 } fo_dyn;
 
 enum { FO_HOR_MFCT = 0, FO_HOR_UPW1 = 1, FO_HOR_MUSCL = 2 };
@@ -69,6 +72,7 @@ typedef struct {
     double *adv_flux_ver;     /* N2*NL */
     double *fct_ttf_max, *fct_ttf_min, *fct_plus, *fct_minus; /* N2*L */
     double *del_ttf, *del_ttf_advhoriz, *del_ttf_advvert;    /* N2*L */
+    double *tr_z;             /* N2*NL */
 } fo_work;
 
 /* planted-mistake switch for the pinning tests (0 = faithful) */
@@ -99,7 +103,13 @@ void fo_oce_tra_adv_flux2dtracer(const fo_mesh *m, double dt, double *dttf_h, do
                                  const double *flux_h, const double *flux_v,
                                  int use_lo, const double *ttf, const double *lo);
 int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w);
-void fo_diff_tracers_ale_tstar(const fo_mesh *m, fo_tracer *t, fo_work *w);
+void fo_tracer_gradient_z(const fo_mesh *m, const double *ttf, double *tr_z);
+void fo_diff_part_hor_redi(const fo_mesh *m, const fo_dyn *d, double dt, const double *tr_xy, const double *tr_z,
+                           double *del_ttf);
+void fo_diff_ver_part_expl_ale(const fo_mesh *m, const fo_dyn *d, double dt, const fo_tracer *t, double *del_ttf);
+void fo_diff_ver_part_redi_expl(const fo_mesh *m, const fo_dyn *d, double dt, const double *tr_xy,
+                                double *del_ttf);
+void fo_diff_tracers_ale_tstar(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w);
 int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t);
 int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, double sval, double *out);
 int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, int ntr, fo_work *w);

@@ -36,7 +36,8 @@ class FoMesh(C.Structure):
 class FoDyn(C.Structure):
     _fields_ = [("uv", dp), ("w", dp), ("w_e", dp), ("w_i", dp), ("Kv", dp), ("heat_flux", dp),
                 ("water_flux", dp), ("virtual_salt", dp), ("relax_salt", dp), ("real_salt_flux", dp),
-                ("is_nonlinfs", C.c_double), ("use_wsplit", C.c_int)]
+                ("is_nonlinfs", C.c_double), ("use_wsplit", C.c_int), ("Ki", dp), ("slope_tapered", dp),
+                ("redi", C.c_int)]
 
 
 class FoTracer(C.Structure):
@@ -48,7 +49,8 @@ class FoTracer(C.Structure):
 class FoWork(C.Structure):
     _fields_ = [("tr_xy", dp), ("edge_up_dn_grad", dp), ("fct_LO", dp), ("adv_flux_hor", dp),
                 ("adv_flux_ver", dp), ("fct_ttf_max", dp), ("fct_ttf_min", dp), ("fct_plus", dp),
-                ("fct_minus", dp), ("del_ttf", dp), ("del_ttf_advhoriz", dp), ("del_ttf_advvert", dp)]
+                ("fct_minus", dp), ("del_ttf", dp), ("del_ttf_advhoriz", dp), ("del_ttf_advvert", dp),
+                ("tr_z", dp)]
 
 
 class FoCgStats(C.Structure):
@@ -94,9 +96,15 @@ def lib():
         L.fo_oce_tra_adv_flux2dtracer.argtypes = [C.POINTER(FoMesh), C.c_double, dp, dp, dp, dp, C.c_int, dp, dp]
         L.fo_do_oce_adv_tra.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
                                         C.POINTER(FoTracer), C.POINTER(FoWork)]
-        L.fo_diff_tracers_ale_tstar.argtypes = [C.POINTER(FoMesh), C.POINTER(FoTracer), C.POINTER(FoWork)]
+        L.fo_diff_tracers_ale_tstar.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double, C.POINTER(FoTracer),
+                                                C.POINTER(FoWork)]
         L.fo_diff_ver_part_impl_ale.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
                                                 C.POINTER(FoTracer)]
+        L.fo_tracer_gradient_z.argtypes = [C.POINTER(FoMesh), dp, dp]
+        L.fo_diff_part_hor_redi.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double, dp, dp, dp]
+        L.fo_diff_ver_part_expl_ale.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
+                                                C.POINTER(FoTracer), dp]
+        L.fo_diff_ver_part_redi_expl.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double, dp, dp]
         L.fo_solve_tracers_ale.argtypes = [C.POINTER(FoMesh), C.POINTER(FoDyn), C.c_double,
                                            C.POINTER(FoTracer), C.c_int, C.POINTER(FoWork)]
         L.fo_init_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, ip, ip, dp, C.c_int]
@@ -161,7 +169,8 @@ class Oracle:
                                 adv_flux_ver=z(m.nod2D, m.nl), fct_ttf_max=z(m.nod2D, L),
                                 fct_ttf_min=z(m.nod2D, L), fct_plus=z(m.nod2D, L),
                                 fct_minus=z(m.nod2D, L), del_ttf=z(m.nod2D, L),
-                                del_ttf_advhoriz=z(m.nod2D, L), del_ttf_advvert=z(m.nod2D, L))
+                                del_ttf_advhoriz=z(m.nod2D, L), del_ttf_advvert=z(m.nod2D, L),
+                                tr_z=z(m.nod2D, m.nl))
         wa = self.work_arrays
         self.fw = FoWork(*[_d(wa[f[0]]) for f in FoWork._fields_])
 
@@ -179,9 +188,15 @@ class Oracle:
                     Kv=np.ascontiguousarray(st.Kv), hf=np.ascontiguousarray(st.heat_flux),
                     wf=np.ascontiguousarray(st.water_flux), vs=np.ascontiguousarray(st.virtual_salt),
                     rs=np.ascontiguousarray(st.relax_salt), rsf=np.ascontiguousarray(st.real_salt_flux))
+        ki = getattr(st, "Ki", None)
+        sl = getattr(st, "slope_tapered", None)
+        keep["Ki"] = None if ki is None else np.ascontiguousarray(ki, dtype=np.float64)
+        keep["sl"] = None if sl is None else np.ascontiguousarray(sl, dtype=np.float64)
+        opt = lambda a: None if a is None else _d(a)
         d = FoDyn(_d(keep["uv"]), _d(keep["w"]), _d(keep["w_e"]), _d(keep["w_i"]), _d(keep["Kv"]),
                   _d(keep["hf"]), _d(keep["wf"]), _d(keep["vs"]), _d(keep["rs"]), _d(keep["rsf"]),
-                  0.0, int(bool(getattr(st, "use_wsplit", False))))
+                  0.0, int(bool(getattr(st, "use_wsplit", False))), opt(keep["Ki"]), opt(keep["sl"]),
+                  int(bool(getattr(st, "redi", False))))
         return d, keep
 
     def tracer(self, values, valuesold, ID=1, AB_order=2, hor="MFCT", ver="QR4C", lim="FCT",
@@ -236,7 +251,7 @@ class Oracle:
         """T* update + implicit vertical diffusion after adv_stage (same tracer)."""
         dt = st.dt if dt is None else dt
         t, d, keep, vab = self._last
-        lib().fo_diff_tracers_ale_tstar(C.byref(self.fm), C.byref(t), C.byref(self.fw))
+        lib().fo_diff_tracers_ale_tstar(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), C.byref(self.fw))
         if t.i_vert_diff:
             rc = lib().fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
             if rc:
@@ -331,6 +346,42 @@ class Oracle:
         nb = self.nboundary_lay()
         lib().fo_adv_tra_hor_muscl(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
                                    float(num_ord), _d(out), _d(np.ascontiguousarray(grad)), _i(nb), 1)
+        return out
+
+    def gradient_z(self, T):
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl))
+        lib().fo_tracer_gradient_z(C.byref(self.fm), _d(np.ascontiguousarray(T)), _d(out))
+        return out
+
+    def hor_diff(self, st, T, dt=None):
+        """diff_part_hor_redi's increment of del_ttf (from zero) for the tracer
+        values T, with st.Ki, st.slope_tapered and st.redi."""
+        dt = st.dt if dt is None else dt
+        d, keep = self.dyn(st)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        txy, tz = self.gradient_elements(T), self.gradient_z(T)
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl - 1))
+        lib().fo_diff_part_hor_redi(C.byref(self.fm), C.byref(d), float(dt), _d(txy), _d(tz), _d(out))
+        return out
+
+    def expl_vdiff(self, st, T, ID=1, dt=None):
+        """diff_ver_part_expl_ale's increment of del_ttf (from zero)."""
+        dt = st.dt if dt is None else dt
+        d, keep = self.dyn(st)
+        T = np.array(T, dtype=np.float64, order="C", copy=True)
+        vo = np.zeros_like(T)
+        t, vab = self.tracer(T, vo, ID=ID, i_vert_diff=False)
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl - 1))
+        lib().fo_diff_ver_part_expl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), _d(out))
+        return out
+
+    def redi_expl(self, st, T, dt=None):
+        """diff_ver_part_redi_expl's increment of del_ttf (from zero)."""
+        dt = st.dt if dt is None else dt
+        d, keep = self.dyn(st)
+        txy = self.gradient_elements(np.ascontiguousarray(T, dtype=np.float64))
+        out = np.zeros((self.mesh.nod2D, self.mesh.nl - 1))
+        lib().fo_diff_ver_part_redi_expl(C.byref(self.fm), C.byref(d), float(dt), _d(txy), _d(out))
         return out
 
     def vert_impl(self, W, T, dt):
