@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfesom_gpu.so")
 SOURCES = ["fgpu_runtime.hip", "fgpu_transport.hip", "fgpu_step.hip", "fgpu_ssh.hip", "fgpu_mpnn.hip",
-           "fgpu_dyn.hip"]
+           "fgpu_dyn.hip", "fgpu_diff.hip"]
 ARCH = os.environ.get("FGPU_ARCH", "gfx950")
 
 

@@ -48,6 +48,7 @@ struct DevMesh {
     const int *erec_i;       // E2*ERI
     const double *erec_d;    // E2*ERD
     const double *zn_vd;     // N2*L: Z levels rebuilt from hnode_new for the implicit diffusion
+    const double *zbn_vd;    // N2*NL: the interface depths of the same rebuild (Redi K33 weights)
 };
 // erec_i fields
 enum { ER_N1, ER_N2, ER_EL1, ER_EL2, ER_NL1, ER_NU1, ER_NL2, ER_NU2, ER_UA0, ER_UA1, ER_UA2, ER_DA0, ER_DA1,
@@ -61,6 +62,10 @@ struct DevDyn {
     const double *heat_flux, *water_flux, *virtual_salt, *relax_salt, *real_salt_flux;
     double is_nonlinfs;
     int wsplit;   // use_wsplit: W = w_e + w_i, w_i taken implicitly (adv_tra_vert_impl, implicit diffusion)
+    const double *Ki;      // N2*L horizontal/isoneutral diffusivity (o_ARRAYS Ki)
+    const double *slope;   // N2*L*3 slope_tapered, component fastest
+    int hdiff;             // Ki set: diff_part_hor_redi runs
+    int redi;              // Redi: slope terms, diff_ver_part_redi_expl, K33 of the implicit diffusion
 };
 
 // Per-tracer state, tracer-major: tracer t occupies [t*stride, (t+1)*stride).
@@ -82,6 +87,10 @@ struct DevTracers {
     double *dttf_h, *dttf_v;  // N2*L, one tracer's worth
     double *dbg_av;           // N2*NL limited vertical flux of the last per-tracer call
     double *dbg_grad;         // E2*L*4 materialised edge_up_dn_grad of the last per-tracer call
+    // explicit diffusion increments of del_ttf from T^n, per tracer (fgpu_diff.hip), allocated on use
+    double *hd;               // E2*L: edge flux c of diff_part_hor_redi (node 1 adds 0+c, node 2 adds 0-c)
+    double *ve;               // N2*L: diff_ver_part_expl_ale increment (i_vert_diff off)
+    double *vr;               // N2*L: diff_ver_part_redi_expl increment
 };
 
 struct ProfEntry {
@@ -181,6 +190,9 @@ int fgpu_prof_flush(fgpu_context *c);
 int fgpu_enqueue_fused_step(fgpu_context *c, double dt);
 int fgpu_compute_zn(fgpu_context *c);
 int fgpu_ensure_staged(fgpu_context *c, int slots);
+// explicit diffusion terms (fgpu_diff.hip)
+int fgpu_ensure_diff(fgpu_context *c);
+int fgpu_launch_diff(fgpu_context *c, int tr0, int ntr, double dt);
 
 // SSH solver pieces shared with the learned preconditioner
 struct FgpuCgSummary {

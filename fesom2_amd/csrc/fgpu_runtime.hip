@@ -589,6 +589,28 @@ extern "C" int fgpu_set_velocity(fgpu_context *c, const double *uv, const double
     return FGPU_OK;
 }
 
+extern "C" int fgpu_set_ki(fgpu_context *c, const double *Ki, const double *slope_tapered, int redi)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    FGPU_HIP(hipSetDevice(c->device));
+    c->graph_valid = false;
+    if (!Ki) {   // Ki = 0: diff_part_hor_redi adds nothing
+        if (redi) return fgpu_fail(FGPU_ERR_ARG, "Redi needs Ki");
+        c->d.hdiff = 0;
+        c->d.redi = 0;
+        return FGPU_OK;
+    }
+    if (redi && !slope_tapered) return fgpu_fail(FGPU_ERR_ARG, "Redi needs slope_tapered");
+    const size_t n = (size_t)c->N2 * (c->nl - 1);
+    int rc = upload_or_refresh(c, &c->d.Ki, Ki, n);
+    if (rc) return rc;
+    if (slope_tapered && (rc = upload_or_refresh(c, &c->d.slope, slope_tapered, 3 * n))) return rc;
+    c->d.hdiff = 1;
+    c->d.redi = redi ? 1 : 0;
+    return FGPU_OK;
+}
+
 extern "C" int fgpu_set_kv(fgpu_context *c, const double *Kv)
 {
     if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
@@ -643,6 +665,7 @@ extern "C" int fgpu_tracers_init(fgpu_context *c, int num_tracers)
     t.sG = N2 * L * 2;
     t.sStg = 0;
     t.TAB = t.tr_xy = t.aux = t.dttf_h = t.dttf_v = t.dbg_av = t.dbg_grad = nullptr;
+    t.hd = t.ve = t.vr = nullptr;
     c->stage_slots = 0;
     int rc;
     // the fused batched step needs only these; the per-stage buffers come with

@@ -635,19 +635,45 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
         const double *kv = d.Kv + (size_t)n * NL;
         const double zinv = 1.0 * dt;
         double a = 0.0, b, c;
+        // Redi: K33 = Kv + Ki S3^2 at the interfaces (oce_ale_tracer.F90:1105-1108, 1138-1144, 1181-1184)
+        double tya = 0.0, tyc = 0.0;
+        const bool redi = d.redi != 0;
+        if (redi) {
+            const double *zbn = m.zbn_vd + (size_t)n * NL;
+            const double *S = d.slope + (size_t)n * L * 3, *ki = d.Ki + (size_t)n * L;
+            const double s3 = S[(k - 1) * 3 + 2];
+            if (k <= nzmax - 2) {   // interface below layer k
+                const double zinv2 = 1.0 / (sZ[k - 1] - sZ[k]);
+                const double s3b = S[k * 3 + 2];
+                tyc = (zbn[k] - sZ[k]) * zinv2 * (s3 * s3) * ki[k - 1] +
+                      (sZ[k - 1] - zbn[k]) * zinv2 * (s3b * s3b) * ki[k];
+            }
+            if (k > nzmin) {   // interface above layer k
+                const double zinv1 = 1.0 / (sZ[k - 2] - sZ[k - 1]);
+                const double s3a = S[(k - 2) * 3 + 2];
+                tya = (zbn[k - 1] - sZ[k - 1]) * zinv1 * (s3a * s3a) * ki[k - 2] +
+                      (sZ[k - 2] - zbn[k - 1]) * zinv1 * (s3 * s3) * ki[k - 1];
+            }
+        }
         if (k == nzmin) {
             const double zinv2 = 1.0 / (sZ[k - 1] - sZ[k]);
-            c = -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_;
+            c = redi ? -(kv[k] + tyc) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_
+                     : -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_;
             b = -c + hn;
         } else if (k <= nzmax - 2) {
             const double zinv1 = 1.0 / (sZ[k - 2] - sZ[k - 1]);
             const double zinv2 = 1.0 / (sZ[k - 1] - sZ[k]);
-            a = -(kv[k - 1]) * zinv1 * zinv * (arv / av_);
-            c = -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_;
+            if (redi) {
+                a = -(kv[k - 1] + tya) * zinv1 * zinv * (arv / av_);
+                c = -(kv[k] + tyc) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_;
+            } else {
+                a = -(kv[k - 1]) * zinv1 * zinv * (arv / av_);
+                c = -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_;
+            }
             b = -a - c + hn;
         } else {
             const double zinv1 = 1.0 / (sZ[k - 2] - sZ[k - 1]);
-            a = -(kv[k - 1]) * zinv1 * zinv * (arv / av_);
+            a = redi ? -(kv[k - 1] + tya) * zinv1 * zinv * (arv / av_) : -(kv[k - 1]) * zinv1 * zinv * (arv / av_);
             c = 0.0;
             b = -a + hn;
         }
@@ -743,8 +769,22 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
                 if (q & 1) h = h - fl * dt / av_;
                 else h = h + fl * dt / av_;
             }
+            double del = 0.0 + h + v;
+            // explicit diffusion in diff_tracers_ale's order (oce_ale_tracer.F90:595-626):
+            // edge fluxes of diff_part_hor_redi ascending, then the vertical parts
+            if (d.hdiff) {
+                const double *hd = t.hd + t.sE * tr;
+                for (int j = 0; j < E.deg; ++j) {
+                    const int q = rl(E.q, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                    if (!(act && k >= a0 && k <= a1)) continue;
+                    const double c = hd[(size_t)(q >> 1) * L + k - 1];
+                    const double r = (q & 1) ? 0.0 - c : 0.0 + c;
+                    del = del + r * dt / av_;
+                }
+            }
             if (act) {
-                double del = 0.0 + h + v;
+                if (!t.ivd[tr]) del = del + t.ve[t.sN * tr + ci];
+                if (d.redi) del = del + t.vr[t.sN * tr + ci];
                 del = del + Tk * (hh - hn);
                 sTsA[w][c][tid] = Tk + del / hn;
                 // valuesold rotation (oce_tracer_mod.F90:94-123)
@@ -820,17 +860,19 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
 
 // Z levels of diff_ver_part_impl_ale rebuilt bottom-up from hnode_new and
 // zbar_n_bot (oce_ale_tracer.F90:1088-1096), one thread per column
-__global__ void k_zn_vdiff(DevMesh m, double *zn)
+__global__ void k_zn_vdiff(DevMesh m, double *zn, double *zbn)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.N2) return;
     const int L = m.L, nzmin = m.ulevn[n], nzmax = m.nlevn[n];
     const double *hn = m.hnode_new + (size_t)n * L;
-    double *z = zn + (size_t)n * L;
+    double *z = zn + (size_t)n * L, *zi = zbn + (size_t)n * m.NL;
     double zb = m.zbar_n_bot[n];
+    zi[nzmax - 1] = zb;
     z[nzmax - 2] = zb + hn[nzmax - 2] / 2.0;
     for (int nz = nzmax - 1; nz >= nzmin + 1; --nz) {
         zb = zb + hn[nz - 1];
+        zi[nz - 1] = zb;
         z[nz - 2] = zb + hn[nz - 2] / 2.0;
     }
 }
@@ -846,8 +888,12 @@ int fgpu_compute_zn(fgpu_context *c)
         if (rc) return rc;
         FGPU_HIP(hipMemset(p, 0, sizeof(double) * (size_t)c->N2 * (c->nl - 1)));
         c->m.zn_vd = p;
+        if ((rc = fgpu_alloc(c, (void **)&p, sizeof(double) * (size_t)c->N2 * c->nl))) return rc;
+        FGPU_HIP(hipMemset(p, 0, sizeof(double) * (size_t)c->N2 * c->nl));
+        c->m.zbn_vd = p;
     }
-    hipLaunchKernelGGL(k_zn_vdiff, dim3((c->N2 + 255) / 256), dim3(256), 0, c->stream, c->m, (double *)c->m.zn_vd);
+    hipLaunchKernelGGL(k_zn_vdiff, dim3((c->N2 + 255) / 256), dim3(256), 0, c->stream, c->m, (double *)c->m.zn_vd,
+                       (double *)c->m.zbn_vd);
     FGPU_HIP(hipGetLastError());
     FGPU_HIP(hipStreamSynchronize(c->stream));
     return FGPU_OK;
@@ -868,6 +914,10 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     bool xv = false;   // some tracer uses CDIFF or PPM
     for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
     const bool ws = c->d.wsplit != 0;   // W split: implicit upwind LO step, w_i rows of the diffusion
+    {   // explicit diffusion terms from T^n (no-op without Ki, Redi or explicit vertical diffusion)
+        int rc = fgpu_launch_diff(c, 0, nt, dt);
+        if (rc) return rc;
+    }
 #define LAUNCH5(LBV)                                                                                                   \
     {                                                                                                                  \
         {                                                                                                              \

@@ -552,7 +552,30 @@ __device__ __forceinline__ double av_limited(int j, int nzmin, int nzmax, const 
 // update of diff_tracers_ale (oce_ale_tracer.F90:318-322, 742-755).
 // mode 0: fused, T <- T*;  mode 1: write del_ttf_advhoriz/advvert (+ limited A_v).
 // ------------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLK) k_node_update(DevMesh m, DevTracers t, int tr0, double dt, int mode)
+// explicit diffusion parts of del_ttf at (node n, level k) from the
+// precomputed increments (fgpu_diff.hip), in the order of diff_tracers_ale:
+// edges of diff_part_hor_redi ascending, diff_ver_part_expl_ale,
+// diff_ver_part_redi_expl
+__device__ __forceinline__ double add_explicit_diffusion(const DevMesh &m, const DevDyn &d, const DevTracers &t, int tr,
+                                                         int n, int k, size_t i, double avol, double dt, double del)
+{
+    if (d.hdiff) {
+        const double *hd = t.hd + t.sE * tr;
+        for (int p = m.ne_ptr[n]; p < m.ne_ptr[n + 1]; ++p) {
+            const int q = m.ne_idx[p];
+            const int e = q >> 1;
+            if (k < m.erange[2 * e] || k > m.erange[2 * e + 1]) continue;
+            const double c = hd[(size_t)e * m.L + k - 1];
+            const double r = (q & 1) ? 0.0 - c : 0.0 + c;
+            del = del + r * dt / avol;
+        }
+    }
+    if (!t.ivd[tr]) del = del + t.ve[t.sN * tr + i];
+    if (d.redi) del = del + t.vr[t.sN * tr + i];
+    return del;
+}
+
+__global__ void __launch_bounds__(BLK) k_node_update(DevMesh m, DevDyn d, DevTracers t, int tr0, double dt, int mode)
 {
     const int tr = tr0 + blockIdx.y;
     const size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
@@ -592,6 +615,7 @@ __global__ void __launch_bounds__(BLK) k_node_update(DevMesh m, DevTracers t, in
     }
     if (mode == 0) {
         double del = 0.0 + h + v;
+        del = add_explicit_diffusion(m, d, t, tr, n, k, i, avol, dt, del);
         del = del + Tk * (m.hnode[i] - m.hnode_new[i]);
         T[i] = Tk + del / m.hnode_new[i];
     } else {
@@ -604,7 +628,7 @@ __global__ void __launch_bounds__(BLK) k_node_update(DevMesh m, DevTracers t, in
 }
 
 // T* update from stored del_ttf_adv* (per-tracer diff_tracers_ale entry)
-__global__ void __launch_bounds__(BLK) k_tstar(DevMesh m, DevTracers t, int tr)
+__global__ void __launch_bounds__(BLK) k_tstar(DevMesh m, DevDyn d, DevTracers t, int tr, double dt)
 {
     const size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= t.sN) return;
@@ -613,6 +637,7 @@ __global__ void __launch_bounds__(BLK) k_tstar(DevMesh m, DevTracers t, int tr)
     if (k < m.ulevn[n] || k > m.nlevn[n] - 1) return;
     double *T = t.T + t.sN * tr;
     double del = 0.0 + t.dttf_h[i] + t.dttf_v[i];
+    del = add_explicit_diffusion(m, d, t, tr, n, k, i, m.areasvol[(size_t)n * m.NL + k - 1], dt, del);
     del = del + T[i] * (m.hnode[i] - m.hnode_new[i]);
     T[i] = T[i] + del / m.hnode_new[i];
 }
@@ -647,21 +672,34 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
         const double *ar = m.area + (size_t)n * NL;
         const double *avl = m.areasvol + (size_t)n * NL;
         const double *kv = d.Kv + (size_t)n * NL;
-        double Z_n[MAXL + 1], cp[MAXL + 1], tp[MAXL + 1];
+        double Z_n[MAXL + 1], cp[MAXL + 1], tp[MAXL + 1], zbn[MAXL + 2];
 #define I(kk) ((kk) - 1)
         // zbar_n/Z_n rebuilt bottom-up from hnode_new (oce_ale_tracer.F90:1088-1096)
         double zb = m.zbar_n_bot[n];
+        zbn[I(nzmax)] = zb;
         Z_n[I(nzmax - 1)] = zb + hn[I(nzmax - 1)] / 2.0;
         for (int nz = nzmax - 1; nz >= nzmin + 1; --nz) {
             zb = zb + hn[I(nz)];
+            zbn[I(nz)] = zb;
             Z_n[I(nz - 1)] = zb + hn[I(nz - 1)] / 2.0;
         }
+        // Redi: K33 = Kv + Ki S3^2 interpolated to the interfaces
+        // (oce_ale_tracer.F90:1105-1108, 1138-1144, 1181-1184)
+        const bool redi = d.redi != 0;
+        const double *S = d.slope + (size_t)n * L * 3, *ki = d.Ki + (size_t)n * L;
+#define S3(kk) S[((kk) - 1) * 3 + 2]
+#define KI(kk) ki[(kk) - 1]
+#define TY_BELOW(nz, zi) ((zbn[I((nz) + 1)] - Z_n[I((nz) + 1)]) * (zi) * (S3(nz) * S3(nz)) * KI(nz) + \
+                          (Z_n[I(nz)] - zbn[I((nz) + 1)]) * (zi) * (S3((nz) + 1) * S3((nz) + 1)) * KI((nz) + 1))
+#define TY_ABOVE(nz, zi) ((zbn[I(nz)] - Z_n[I(nz)]) * (zi) * (S3((nz) - 1) * S3((nz) - 1)) * KI((nz) - 1) + \
+                          (Z_n[I((nz) - 1)] - zbn[I(nz)]) * (zi) * (S3(nz) * S3(nz)) * KI(nz))
         // forward sweep: coefficients on the fly, cp/tp kept
         const double zinv = 1.0 * dt;
         int nz = nzmin;
         double zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
         double a = 0.0;
-        double c = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)];
+        double c = redi ? -(kv[I(nz + 1)] + TY_BELOW(nz, zinv2)) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)]
+                        : -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)];
         double b = -c + hn[I(nz)];
         // implicit part of the split W (oce_ale_tracer.F90:1118-1129, 1158-1172,
         // 1194-1202); FCT tracers took it in their LO solution
@@ -685,8 +723,13 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
             Tk = Tp;
             if (nz <= nzmax - 2) {
                 zinv2 = 1.0 / (Z_n[I(nz)] - Z_n[I(nz + 1)]);
-                a = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)]);
-                c = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)];
+                if (redi) {
+                    a = -(kv[I(nz)] + TY_ABOVE(nz, zinv1)) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)]);
+                    c = -(kv[I(nz + 1)] + TY_BELOW(nz, zinv2)) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)];
+                } else {
+                    a = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)]);
+                    c = -(kv[I(nz + 1)]) * zinv2 * zinv * ar[I(nz + 1)] / avl[I(nz)];
+                }
                 b = -a - c + hn[I(nz)];
                 zinv1 = zinv2;
                 if (wimpl) {
@@ -700,7 +743,8 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
                 Tp = T[I(nz + 1)];
                 tr_ = -a * Tm - (b - hn[I(nz)]) * Tk - c * Tp;
             } else {
-                a = -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)]);
+                a = redi ? -(kv[I(nz)] + TY_ABOVE(nz, zinv1)) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)])
+                         : -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)]);
                 c = 0.0;
                 b = -a + hn[I(nz)];
                 if (wimpl) {
@@ -714,6 +758,10 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
             cp[I(nz)] = c / mm;
             tp[I(nz)] = (tr_ - tp[I(nz - 1)] * a) / mm;
         }
+#undef TY_BELOW
+#undef TY_ABOVE
+#undef S3
+#undef KI
         // back substitution and update
         double x = tp[I(nzmax - 1)];
         T[I(nzmax - 1)] = T[I(nzmax - 1)] + x;
@@ -862,9 +910,13 @@ static int launch_adv(fgpu_context *c, int tr0, int ntr, double dt, int mode)
         hipLaunchKernelGGL(k_edge_limit, dim3(nblk(nE), ntr), dim3(BLK), 0, c->stream, m, c->t, tr0);
         LAUNCH_CHECK("k_edge_limit");
     }
+    if (mode == 0) {   // the explicit diffusion terms read T^n, which k_node_update overwrites
+        int rc = fgpu_launch_diff(c, tr0, ntr, dt);
+        if (rc) return rc;
+    }
     {
         ProfScope ps(c, "k_node_update");
-        hipLaunchKernelGGL(k_node_update, dim3(nblk(nN), ntr), dim3(BLK), 0, c->stream, m, c->t, tr0, dt, mode);
+        hipLaunchKernelGGL(k_node_update, dim3(nblk(nN), ntr), dim3(BLK), 0, c->stream, m, c->d, c->t, tr0, dt, mode);
         LAUNCH_CHECK("k_node_update");
     }
     return FGPU_OK;
@@ -906,9 +958,12 @@ extern "C" int fgpu_diff_tracers_ale(fgpu_context *c, double dt, int tr_num)
     if ((rc = check_tracer(c, tr_num))) return rc;
     if ((rc = fgpu_ensure_staged(c, 1))) return rc;
     const int tr = tr_num - 1;
-    if (c->h_ivd[tr] && !c->kv_ready) return fgpu_fail(FGPU_ERR_STATE, "Kv not set (fgpu_set_kv)");
-    if (c->h_ivd[tr] && !c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "surface forcing not set");
-    hipLaunchKernelGGL(k_tstar, dim3(nblk(c->t.sN)), dim3(BLK), 0, c->stream, c->m, c->t, tr);
+    // implicit or explicit vertical diffusion: Kv and the surface forcing either way
+    if (!c->kv_ready) return fgpu_fail(FGPU_ERR_STATE, "Kv not set (fgpu_set_kv)");
+    if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "surface forcing not set");
+    if ((rc = fgpu_ensure_diff(c))) return rc;
+    if ((rc = fgpu_launch_diff(c, tr, 1, dt))) return rc;
+    hipLaunchKernelGGL(k_tstar, dim3(nblk(c->t.sN)), dim3(BLK), 0, c->stream, c->m, c->d, c->t, tr, dt);
     LAUNCH_CHECK("k_tstar");
     if (c->h_ivd[tr])
         if ((rc = launch_vdiff(c, tr, 1, dt, 1, -1))) return rc;
@@ -957,6 +1012,7 @@ static int run_steps(fgpu_context *c, double dt, int nsteps)
 {
     int rc;
     if (!c->fused && (rc = fgpu_ensure_staged(c, c->t.ntr))) return rc;
+    if ((rc = fgpu_ensure_diff(c))) return rc;   // before any capture
     if (c->prof) {
         for (int s = 0; s < nsteps; ++s)
             if ((rc = enqueue_step(c, dt))) return rc;

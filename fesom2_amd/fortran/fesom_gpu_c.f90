@@ -26,7 +26,7 @@ module fesom_gpu_c
   end type fgpu_cg_stats
 
   public :: fgpu_last_error, fgpu_version, fgpu_create, fgpu_destroy, fgpu_synchronize, &
-       fgpu_mesh_init, fgpu_set_vertical, fgpu_set_velocity, fgpu_set_kv, fgpu_set_surface_forcing, &
+       fgpu_mesh_init, fgpu_set_vertical, fgpu_set_velocity, fgpu_set_kv, fgpu_set_ki, fgpu_set_surface_forcing, &
        fgpu_tracers_init, fgpu_tracer_config, fgpu_tracer_upload, fgpu_tracer_download, &
        fgpu_tracer_work_download, fgpu_edge_up_dn_tri, fgpu_init_tracers_AB, fgpu_do_oce_adv_tra, &
        fgpu_diff_tracers_ale, fgpu_diff_ver_part_impl_ale, fgpu_solve_tracers_ale, &
@@ -100,6 +100,13 @@ module fesom_gpu_c
        type(c_ptr), value :: ctx
        real(c_double), intent(in) :: Kv(*)
      end function fgpu_set_kv
+
+     ! Ki = c_null_ptr: Ki = 0 (no horizontal diffusion)
+     integer(c_int) function fgpu_set_ki(ctx, Ki, slope_tapered, redi) bind(C, name="fgpu_set_ki")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, Ki, slope_tapered
+       integer(c_int), value :: redi
+     end function fgpu_set_ki
 
      integer(c_int) function fgpu_set_surface_forcing(ctx, heat_flux, water_flux, virtual_salt, &
           relax_salt, real_salt_flux, is_nonlinfs) bind(C, name="fgpu_set_surface_forcing")

@@ -42,6 +42,7 @@ module fesom_gpu_host
 
   type(c_ptr), save :: ctx = c_null_ptr
 
+  public :: gpu_set_ki
   public :: gpu_init, gpu_finalize, gpu_check, gpu_mesh_setup, gpu_set_vertical, gpu_set_state, &
        gpu_tracers_init, gpu_tracer_config, gpu_tracer_put, gpu_tracer_get, gpu_tracer_content, &
        init_tracers_AB, do_oce_adv_tra, diff_tracers_ale, diff_ver_part_impl_ale, solve_tracers_ale, &
@@ -129,6 +130,24 @@ contains
     call gpu_check(fgpu_set_surface_forcing(ctx, heat_flux, water_flux, virtual_salt, relax_salt, &
          real_salt_flux, is_nonlinfs), 'fgpu_set_surface_forcing')
   end subroutine gpu_set_state
+
+  ! o_ARRAYS Ki(nl-1,nod2D), slope_tapered(3,nl-1,nod2D) and Redi (oce_modules.F90:75):
+  ! horizontal/Redi diffusion of diff_tracers_ale. Without Ki the diffusivity is zero.
+  subroutine gpu_set_ki(Ki, slope_tapered, Redi)
+    real(WP), intent(in), target, optional :: Ki(:, :), slope_tapered(:, :, :)
+    logical, intent(in), optional :: Redi
+    type(c_ptr) :: pk, ps
+    integer(c_int) :: r
+    pk = c_null_ptr
+    ps = c_null_ptr
+    r = 0_c_int
+    if (present(Ki)) pk = c_loc(Ki)
+    if (present(slope_tapered)) ps = c_loc(slope_tapered)
+    if (present(Redi)) then
+      if (Redi) r = 1_c_int
+    end if
+    call gpu_check(fgpu_set_ki(ctx, pk, ps, r), 'fgpu_set_ki')
+  end subroutine gpu_set_ki
 
   integer(c_int) function wsplit_code(use_wsplit)
     logical, intent(in), optional :: use_wsplit

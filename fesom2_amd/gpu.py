@@ -48,6 +48,7 @@ _SIGS = {
     "fgpu_set_vertical": [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp],
     "fgpu_set_velocity": [vp, dp, dp, dp, dp, C.c_int],
     "fgpu_set_kv": [vp, dp],
+    "fgpu_set_ki": [vp, dp, dp, C.c_int],
     "fgpu_set_surface_forcing": [vp, dp, dp, dp, dp, dp, C.c_double],
     "fgpu_set_uv": [vp, dp],
     "fgpu_vert_vel_ale": [vp, C.c_double, C.c_int, dp, dp, C.c_int, C.c_double, dp, ip],
@@ -197,6 +198,13 @@ class Context:
     def set_kv(self, Kv):
         _check(self.L.fgpu_set_kv(self.h, _d(_f64(Kv))))
 
+    def set_ki(self, Ki, slope_tapered=None, redi=False):
+        """Ki (N2, nl-1), slope_tapered (N2, nl-1, 3), Redi; Ki None means Ki = 0."""
+        ki = None if Ki is None else _f64(Ki)
+        sl = None if slope_tapered is None else _f64(slope_tapered)
+        _check(self.L.fgpu_set_ki(self.h, None if ki is None else _d(ki), None if sl is None else _d(sl),
+                                  int(bool(redi))))
+
     def set_surface_forcing(self, heat_flux, water_flux, virtual_salt, relax_salt, real_salt_flux, is_nonlinfs=0.0):
         a = [_f64(x) for x in (heat_flux, water_flux, virtual_salt, relax_salt, real_salt_flux)]
         _check(self.L.fgpu_set_surface_forcing(self.h, *[_d(x) for x in a], float(is_nonlinfs)))
@@ -206,6 +214,8 @@ class Context:
         self.set_velocity(st.uv, st.w, st.w_e, st.w_i, use_wsplit=getattr(st, "use_wsplit", False))
         self.set_kv(st.Kv)
         self.set_surface_forcing(st.heat_flux, st.water_flux, st.virtual_salt, st.relax_salt, st.real_salt_flux)
+        if getattr(st, "Ki", None) is not None:
+            self.set_ki(st.Ki, getattr(st, "slope_tapered", None), getattr(st, "redi", False))
 
     # ------------------------------------------- dynamics either side of the transport
     ALE = {"linfs": 0, "zlevel": 1, "zstar": 2}

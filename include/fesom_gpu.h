@@ -84,6 +84,13 @@ int fgpu_set_vertical(fgpu_context *ctx, const double *area, const double *areas
  * the implicit vertical diffusion of the others (oce_ale_tracer.F90:1118-1202). */
 int fgpu_set_velocity(fgpu_context *ctx, const double *uv, const double *w, const double *w_e,
                       const double *w_i, int use_wsplit);
+/* o_ARRAYS Ki(nl-1,nod2D) and slope_tapered(3,nl-1,nod2D), and Redi of
+ * oce_modules.F90:75: the inputs of diff_part_hor_redi
+ * (oce_ale_tracer.F90:1526-1689), diff_ver_part_redi_expl (:1439-1522) and the
+ * K33 = Kv + Ki S3^2 terms of diff_ver_part_impl_ale (:1105-1184).
+ * Ki = NULL means Ki = 0: the horizontal diffusion adds nothing (the default
+ * until this is called). slope_tapered may be NULL when redi = 0. */
+int fgpu_set_ki(fgpu_context *ctx, const double *Ki, const double *slope_tapered, int redi);
 /* o_ARRAYS%Kv(nl,nod2D), read by diff_ver_part_impl_ale (oce_ale_tracer.F90:914) */
 int fgpu_set_kv(fgpu_context *ctx, const double *Kv);
 /* surface fluxes read by bc_surface (oce_ale_tracer.F90:1873-1881), (nod2D) each */
