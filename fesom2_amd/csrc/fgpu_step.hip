@@ -598,21 +598,24 @@ __device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &
 // WS: use_wsplit, the non-FCT tracers' implicit diffusion also takes the
 // implicit part w_i of the split W (second set of rows).
 template <int LB, int TCH, int CPB, bool WS>
-__global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer,
-                                                    int tpb)
+__global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6))) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer,
+                                                    int tpb, int vd_split)
 {
     const int w = threadIdx.x / LB;
     const int n0 = xcd_column(m.N2) * CPB + w;
     const bool valid = n0 < m.N2;
     const int n = valid ? n0 : m.N2 - 1;
+    // LDS reuse: Z_n lives in sAv's rows until the tracer loop starts, and the
+    // Thomas sweep's c' rows of the first two tracers of a chunk live in sRp
+    // and sRm, which are dead between a chunk's flux phase and its sweep
     __shared__ double sRpA[CPB][LB + 2], sRmA[CPB][LB + 2], sAvA[CPB][LB + 2];
-    __shared__ double saA[CPB][LB + 2], sbA[CPB][LB + 2], scA[CPB][LB + 2], sZA[CPB][LB + 2];
-    __shared__ double sTsA[CPB][TCH][LB + 2], stpA[CPB][TCH][LB + 2], scpA[CPB][TCH][LB + 2];
+    __shared__ double saA[CPB][LB + 2], sbA[CPB][LB + 2], scA[CPB][LB + 2];
+    __shared__ double sTsA[CPB][TCH][LB + 2], stpA[CPB][TCH][LB + 2], scpX[CPB][TCH > 2 ? TCH - 2 : 1][LB + 2];
     __shared__ int sNz[CPB][2];
     constexpr int NW = WS ? LB + 2 : 1;
     __shared__ double swaA[CPB][NW], swbA[CPB][NW], swcA[CPB][NW];
     double *sRp = sRpA[w], *sRm = sRmA[w], *sAv = sAvA[w];
-    double *sa = saA[w], *sb = sbA[w], *sc = scA[w], *sZ = sZA[w];
+    double *sa = saA[w], *sb = sbA[w], *sc = scA[w], *sZ = sAvA[w];
     const int tid = threadIdx.x % LB, k = tid + 1;
     const int L = m.L, NL = m.NL;
     const int nzmin = valid ? m.ulevn[n] : 1, nzmax = valid ? m.nlevn[n] : 0;
@@ -631,7 +634,8 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
     if (act) sZ[tid] = m.zn_vd[ci];
     __syncthreads();
     // tridiagonal coefficients (oce_ale_tracer.F90:1098-1175): tracer independent
-    if (act) {
+    // (vd_split: the implicit diffusion runs in k_vdiff after this kernel)
+    if (act && !vd_split) {
         const double *kv = d.Kv + (size_t)n * NL;
         const double zinv = 1.0 * dt;
         double a = 0.0, b, c;
@@ -800,7 +804,7 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
         // right-hand sides (oce_ale_tracer.F90:1206-1229, 1377-1383), all tracers of the chunk
         for (int c = 0; c < nch; ++c) {
             const int tr = t0 + c;
-            if (!t.ivd[tr] || !act) continue;
+            if (vd_split || !t.ivd[tr] || !act) continue;
             const double dz = hn;
             const double *ts = sTsA[w][c];
             const bool wim = WS && t.lim[tr] != LIM_FCT;
@@ -816,13 +820,14 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
         // Thomas sweeps (oce_ale_tracer.F90:1406-1430): one lane of the first wave
         // per (column, tracer) pair; the pivot chain (cp, m) is evaluated in the
         // same loop as tp so that the two recurrences overlap
-        if (threadIdx.x < CPB * nch) {
+        if (!vd_split && threadIdx.x < CPB * nch) {
             const int wc = threadIdx.x / nch, c = threadIdx.x % nch;
             const int z0 = sNz[wc][0], z1 = sNz[wc][1];
             if (z1 - 1 >= z0 && t.ivd[t0 + c]) {
                 const bool wim = WS && t.lim[t0 + c] != LIM_FCT;
                 const double *a_ = wim ? swaA[wc] : saA[wc], *b_ = wim ? swbA[wc] : sbA[wc], *c_ = wim ? swcA[wc] : scA[wc];
-                double *tp_ = stpA[wc][c], *cp_ = scpA[wc][c], *ts = sTsA[wc][c];
+                double *tp_ = stpA[wc][c], *ts = sTsA[wc][c];
+                double *cp_ = c == 0 ? sRpA[wc] : (c == 1 ? sRmA[wc] : scpX[wc][c >= 2 ? c - 2 : 0]);
                 double cp = c_[z0 - 1] / b_[z0 - 1];
                 double tp = tp_[z0 - 1] / b_[z0 - 1];
                 cp_[z0 - 1] = cp;
@@ -848,7 +853,7 @@ __global__ void __launch_bounds__(LB * CPB) kD_node(DevMesh m, DevDyn d, DevTrac
             for (int c = 0; c < nch; ++c) {
                 const int tr = t0 + c;
                 double v = sTsA[w][c][tid];
-                if (tr == clamp_tracer) {
+                if (tr == clamp_tracer && !vd_split) {
                     if (v > 45.0) v = 45.0;
                     if (v < 3.0) v = 3.0;
                 }
@@ -914,6 +919,7 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     bool xv = false;   // some tracer uses CDIFF or PPM
     for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
     const bool ws = c->d.wsplit != 0;   // W split: implicit upwind LO step, w_i rows of the diffusion
+    const int vds = c->step_vd ? 1 : 0;
     {   // explicit diffusion terms from T^n (no-op without Ki, Redi or explicit vertical diffusion)
         int rc = fgpu_launch_diff(c, 0, nt, dt);
         if (rc) return rc;
@@ -944,28 +950,32 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
             if (ws)                                                                                                    \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
             else if (LBV == 64 && cpb == 8)                                                                            \
                 hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 8 : 1), false>), dim3(xcd_grid((m.N2 + 7) / 8), ny),          \
-                                   dim3(LBV * (LBV == 64 ? 8 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
+                                   dim3(LBV * (LBV == 64 ? 8 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
             else if (LBV == 64 && cpb == 4 && tch == 4)                                                                \
                 hipLaunchKernelGGL((kD_node<LBV, 4, (LBV == 64 ? 4 : 1), false>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
-                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
+                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
             else if (LBV == 64 && cpb == 4)                                                                            \
                 hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1), false>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
-                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
+                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
             else if (tch == 8)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 8, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 8, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
             else if (tch == 4)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 4, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 4, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
             else                                                                                                       \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
         }                                                                                                              \
     }
     // one lane per level and per interface: LB >= NL
     if (m.NL <= 64) LAUNCH5(64)
     else LAUNCH5(128)
 #undef LAUNCH5
+    if (vds) {   // implicit vertical diffusion and clamp, one thread per column
+        int rc = fgpu_launch_vdiff(c, 0, nt, dt, 1, clamp);
+        if (rc) return rc;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("fused step launch: ") + hipGetErrorString(e));
     return FGPU_OK;
