@@ -15,7 +15,8 @@
 !
 ! Case file (little-endian, Fortran array order, integers int32, reals FP64):
 !   int32 header(16): magic 1179076949, format 1, nod2D, elem2D, edge2D, nl, kmax,
-!                     ntr, nsingle, nbatched, precond_variant, maxiter, 0, 0, 0, 0
+!                     ntr, nsingle, nbatched, precond_variant, maxiter, has_Ki, Redi,
+!                     use_wsplit, 0
 !   real  dt, cyclic_length, soltol, g
 !   coord_nod2D(2,N2) elem2D_nodes(3,F2) edges(2,E2) edge_tri(2,E2)
 !   nod_in_elem2D(kmax,N2) nod_in_elem2D_num(N2) nlevels(F2) ulevels(F2)
@@ -26,6 +27,7 @@
 !   uv(2,nl-1,F2) w(nl,N2) w_e(nl,N2) w_i(nl,N2) Kv(nl,N2)
 !   heat_flux(N2) water_flux(N2) virtual_salt(N2) relax_salt(N2) real_salt_flux(N2)
 !   ID(ntr) values(nl-1,N2,ntr) valuesold(nl-1,N2,ntr) ssh_rhs(N2)
+!   [Ki(nl-1,N2) if has_Ki] [slope_tapered(3,nl-1,N2) if Redi]
 ! Result file:
 !   int32 status(8): iters_last, converged (1/0), nbreakdown, nnegrz, nonconv, 0, 0, 0
 !   real  resid_last, rtol_last, content_before(ntr), content_after(ntr)
@@ -56,6 +58,8 @@ program fgpu_case_driver
   real(WP), allocatable :: values(:, :, :), valuesold(:, :, :), ssh_rhs(:), d_eta(:), c0(:), c1(:)
   real(WP), allocatable :: wv(:, :), wv_e(:, :), wv_i(:, :), cfl_z(:, :), hn_new(:, :), rhs_dyn(:), hbar(:)
   real(WP) :: cflmax
+  real(WP), allocatable, target :: Ki(:, :), slope_tapered(:, :, :)
+  logical :: use_wsplit, redi
   type(t_gpu_solverinfo) :: si
 
   call get_command_argument(1, fin)
@@ -102,6 +106,16 @@ program fgpu_case_driver
   read(u) heat_flux, water_flux, virtual_salt, relax_salt, real_salt_flux
   allocate(ids(ntr), values(nl - 1, N2, ntr), valuesold(nl - 1, N2, ntr), ssh_rhs(N2), d_eta(N2), c0(ntr), c1(ntr))
   read(u) ids, values, valuesold, ssh_rhs
+  redi = hdr(14) == 1
+  use_wsplit = hdr(15) == 1
+  if (hdr(13) == 1) then
+     allocate(Ki(nl - 1, N2))
+     read(u) Ki
+  end if
+  if (redi) then
+     allocate(slope_tapered(3, nl - 1, N2))
+     read(u) slope_tapered
+  end if
   close(u)
   write(*, '(a,i0,a,i0,a,i0,a,i0,a,i0)') '[driver] nod2D=', N2, ' elem2D=', F2, ' edge2D=', E2, ' nl=', nl, &
        ' ntr=', ntr
@@ -110,6 +124,9 @@ program fgpu_case_driver
      write(*, '(a,i0,1x,i0,1x,i0)') 'sums_int ', sum(int(edges, 8)), sum(int(nod_in_elem2D, 8)), sum(int(ids, 8))
      write(*, '(a,3es25.16)') 'sums_real ', sum(values), sum(uv), sum(ssh_rhs)
      write(*, '(a,2es25.16)') 'last ', gradient_sca(6, F2), Kv(nl, N2)
+     write(*, '(a,i0,1x,i0,1x,i0)') 'flags ', hdr(13), hdr(14), hdr(15)
+     if (allocated(Ki)) write(*, '(a,es25.16)') 'ki ', sum(Ki)
+     if (allocated(slope_tapered)) write(*, '(a,es25.16)') 'slope ', slope_tapered(3, nl - 1, N2)
      stop
   end if
 
@@ -121,7 +138,15 @@ program fgpu_case_driver
   call gpu_set_vertical(area, areasvol, hnode, hnode_new, helem, zbar_3d_n, Z_3d_n, zbar_n_bot, zbar_e_bot, &
        zbar_e_srf)
   call gpu_set_state(uv, w, w_e, w_i, Kv, heat_flux, water_flux, virtual_salt, relax_salt, real_salt_flux, &
-       0.0_WP)
+       0.0_WP, use_wsplit)
+  ! o_ARRAYS Ki and slope_tapered, Redi (horizontal and isoneutral diffusion)
+  if (allocated(Ki)) then
+     if (redi) then
+        call gpu_set_ki(Ki, slope_tapered, .true.)
+     else
+        call gpu_set_ki(Ki)
+     end if
+  end if
   call gpu_tracers_init(ntr)
   do tr = 1, ntr
      call gpu_tracer_config(tr, ids(tr), 2, 'MFCT', 'QR4C', 'FCT', 0.0_WP, 1.0_WP, .true.)
@@ -133,7 +158,7 @@ program fgpu_case_driver
   do istep = 1, nsingle
      do tr = 1, ntr
         call init_tracers_AB(tr)
-        call do_oce_adv_tra(dt, uv, w, w_i, w_e, tr)
+        call do_oce_adv_tra(dt, uv, w, w_i, w_e, tr, use_wsplit)
         call diff_tracers_ale(dt, tr)
      end do
   end do

@@ -18,6 +18,9 @@ def write_case(path, mesh, state, ssh_rhs, nsingle=1, nbatched=2, precond_varian
     hdr = np.zeros(16, dtype="<i4")
     hdr[:12] = [MAGIC, 1, mesh.nod2D, mesh.elem2D, mesh.edge2D, mesh.nl, mesh.kmax, state.ntr,
                 nsingle, nbatched, precond_variant, maxiter]
+    ki = getattr(state, "Ki", None)
+    redi = bool(getattr(state, "redi", False))
+    hdr[12:15] = [ki is not None, redi, bool(getattr(state, "use_wsplit", False))]
     parts = [hdr, f64([state.dt, mesh.cyclic_length, soltol, g]),
              f64(mesh.coord_nod2D), i32(mesh.elem2D_nodes + 1), i32(mesh.edges + 1),
              i32(np.where(mesh.edge_tri >= 0, mesh.edge_tri + 1, -999)),
@@ -33,6 +36,10 @@ def write_case(path, mesh, state, ssh_rhs, nsingle=1, nbatched=2, precond_varian
              f64(state.heat_flux), f64(state.water_flux), f64(state.virtual_salt), f64(state.relax_salt),
              f64(state.real_salt_flux),
              i32(state.ids), f64(state.values), f64(state.valuesold), f64(ssh_rhs)]
+    if ki is not None:
+        parts.append(f64(ki))
+    if redi:
+        parts.append(f64(state.slope_tapered))
     with open(path, "wb") as f:
         for p in parts:
             f.write(p.tobytes())

@@ -54,6 +54,60 @@ def test_case_file_read_by_fortran(c0, tmp_path):
     assert last == [float(mesh.gradient_sca[-1, 5]), float(state.Kv[-1, -1])]
 
 
+def _diffusion_state(c0_oracle, state, mesh):
+    """The c0 state with W split by the oracle's vert_vel_ale, Ki, and Redi slopes."""
+    import dataclasses
+    cap = 0.3 * c0_oracle.vert_vel_ale(state.uv, state.dt)["cflmax"]
+    ref = c0_oracle.vert_vel_ale(state.uv, state.dt, use_wsplit=True, wsplit_maxcfl=cap)
+    rng = np.random.default_rng(41)
+    L = mesh.nl - 1
+    S = rng.normal(0.0, 0.01, (mesh.nod2D, L, 3))
+    return dataclasses.replace(state, w=ref["w"], w_e=ref["w_e"], w_i=ref["w_i"], use_wsplit=True,
+                               Ki=rng.uniform(20.0, 400.0, (mesh.nod2D, L)), slope_tapered=S, redi=True)
+
+
+def test_case_file_with_diffusion_read_by_fortran(c0, c0_oracle, tmp_path):
+    _need_driver()
+    mesh, state, _ = c0
+    st2 = _diffusion_state(c0_oracle, state, mesh)
+    path = str(tmp_path / "case.bin")
+    fc.write_case(path, mesh, st2, mg.ssh_rhs(mesh))
+    r = subprocess.run([DRIVER, "--read", path], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
+    assert [int(x) for x in out["flags"]] == [1, 1, 1]
+    np.testing.assert_allclose(float(out["ki"][0]), st2.Ki.sum(), rtol=1e-12)
+    assert float(out["slope"][0]) == float(st2.slope_tapered[-1, -1, 2])
+
+
+@pytest.mark.gpu
+def test_fortran_host_diffusion_wsplit(c0, c0_oracle, tmp_path):
+    """The driver with Ki, Redi and a split W (gpu_set_ki, use_wsplit in
+    gpu_set_state and do_oce_adv_tra): 1 per-tracer + 2 batched steps equal
+    three oracle steps bit for bit."""
+    _need_driver()
+    mesh, state, _ = c0
+    st2 = _diffusion_state(c0_oracle, state, mesh)
+    rhs = mg.ssh_rhs(mesh)
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "result.bin")
+    fc.write_case(case, mesh, st2, rhs, nsingle=1, nbatched=2, precond_variant=1)
+    print("[gpu] fortran driver c0 with Ki, Redi, W split", flush=True)
+    r = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = fc.read_result(res, mesh, state.ntr)
+    v = state.values.copy()
+    vo = state.valuesold.copy()
+    for _ in range(3):
+        c0_oracle.step(st2, v, vo)
+    L = mesh.nl - 1
+    k = np.arange(1, L + 1)
+    wet = (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+    for i in range(state.ntr):
+        np.testing.assert_array_equal(out["values"][i][wet], v[i][wet])
+        np.testing.assert_array_equal(out["valuesold"][i][wet], vo[i][wet])
+
+
 @pytest.mark.gpu
 def test_fortran_host_c0(c0, c0_oracle, tmp_path):
     _need_driver()
