@@ -230,8 +230,18 @@ __global__ void __launch_bounds__(LB) k_ssh_rhs_edge(DevMesh m, const double *uv
     if (tid < 2) {
         const double *src = tid == 0 ? s1 : s2;
         const int lo = tid == 0 ? nu1 : nu2, hi = tid == 0 ? nl1 : nl2;
+        // levels in order (the reference's association); the LDS reads of
+        // eight levels are issued before their adds
         double c = 0.0;
-        for (int nz = lo; nz <= hi; ++nz) c = tid == 0 ? c + src[nz - 1] : c - src[nz - 1];
+        int nz = lo;
+        for (; nz + 7 <= hi; nz += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = src[nz - 1 + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c = tid == 0 ? c + v[u] : c - v[u];
+        }
+        for (; nz <= hi; ++nz) c = tid == 0 ? c + src[nz - 1] : c - src[nz - 1];
         const double c2 = __shfl(c, 1);
         if (tid == 0) ce[e] = c + c2;
     }

@@ -595,9 +595,11 @@ __device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &
 // run in one wave, one lane per pair, while the other waves wait: a serial
 // sweep then costs one wave's issue slots for CPB columns instead of one per
 // column. Both are launch-shape choices; the arithmetic is the same for all.
-// WS: use_wsplit, the non-FCT tracers' implicit diffusion also takes the
-// implicit part w_i of the split W (second set of rows).
-template <int LB, int TCH, int CPB, bool WS>
+// XF bit 0 (use_wsplit): the non-FCT tracers' implicit diffusion also takes
+// the implicit part w_i of the split W (second set of rows). XF bit 1: the
+// explicit diffusion increments (Ki, Redi, i_vert_diff off) and the Redi K33
+// terms are compiled in. Both are launched only when in use.
+template <int LB, int TCH, int CPB, int XF>
 __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6))) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer,
                                                     int tpb, int vd_split)
 {
@@ -612,6 +614,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
     __shared__ double saA[CPB][LB + 2], sbA[CPB][LB + 2], scA[CPB][LB + 2];
     __shared__ double sTsA[CPB][TCH][LB + 2], stpA[CPB][TCH][LB + 2], scpX[CPB][TCH > 2 ? TCH - 2 : 1][LB + 2];
     __shared__ int sNz[CPB][2];
+    constexpr bool WS = (XF & 1) != 0, DX = (XF & 2) != 0;
     constexpr int NW = WS ? LB + 2 : 1;
     __shared__ double swaA[CPB][NW], swbA[CPB][NW], swcA[CPB][NW];
     double *sRp = sRpA[w], *sRm = sRmA[w], *sAv = sAvA[w];
@@ -641,7 +644,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
         double a = 0.0, b, c;
         // Redi: K33 = Kv + Ki S3^2 at the interfaces (oce_ale_tracer.F90:1105-1108, 1138-1144, 1181-1184)
         double tya = 0.0, tyc = 0.0;
-        const bool redi = d.redi != 0;
+        const bool redi = DX && d.redi != 0;
         if (redi) {
             const double *zbn = m.zbn_vd + (size_t)n * NL;
             const double *S = d.slope + (size_t)n * L * 3, *ki = d.Ki + (size_t)n * L;
@@ -776,7 +779,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
             double del = 0.0 + h + v;
             // explicit diffusion in diff_tracers_ale's order (oce_ale_tracer.F90:595-626):
             // edge fluxes of diff_part_hor_redi ascending, then the vertical parts
-            if (d.hdiff) {
+            if (DX && d.hdiff) {
                 const double *hd = t.hd + t.sE * tr;
                 for (int j = 0; j < E.deg; ++j) {
                     const int q = rl(E.q, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
@@ -787,8 +790,8 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
                 }
             }
             if (act) {
-                if (!t.ivd[tr]) del = del + t.ve[t.sN * tr + ci];
-                if (d.redi) del = del + t.vr[t.sN * tr + ci];
+                if (DX && !t.ivd[tr]) del = del + t.ve[t.sN * tr + ci];
+                if (DX && d.redi) del = del + t.vr[t.sN * tr + ci];
                 del = del + Tk * (hh - hn);
                 sTsA[w][c][tid] = Tk + del / hn;
                 // valuesold rotation (oce_tracer_mod.F90:94-123)
@@ -919,6 +922,9 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     bool xv = false;   // some tracer uses CDIFF or PPM
     for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
     const bool ws = c->d.wsplit != 0;   // W split: implicit upwind LO step, w_i rows of the diffusion
+    bool dx = c->d.hdiff || c->d.redi;   // explicit diffusion increments or Redi K33 in K_D
+    for (int v : c->h_ivd) dx = dx || !v;
+    const int xf = (ws ? 1 : 0) | (dx ? 2 : 0);
     const int vds = c->step_vd ? 1 : 0;
     {   // explicit diffusion terms from T^n (no-op without Ki, Redi or explicit vertical diffusion)
         int rc = fgpu_launch_diff(c, 0, nt, dt);
@@ -949,23 +955,27 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
-            if (ws)                                                                                                    \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+            if (xf == 1)                                                                                               \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+            else if (xf == 2)                                                                                          \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 2>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+            else if (xf == 3)                                                                                          \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 3>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
             else if (LBV == 64 && cpb == 8)                                                                            \
-                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 8 : 1), false>), dim3(xcd_grid((m.N2 + 7) / 8), ny),          \
+                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 8 : 1), 0>), dim3(xcd_grid((m.N2 + 7) / 8), ny),          \
                                    dim3(LBV * (LBV == 64 ? 8 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
             else if (LBV == 64 && cpb == 4 && tch == 4)                                                                \
-                hipLaunchKernelGGL((kD_node<LBV, 4, (LBV == 64 ? 4 : 1), false>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
+                hipLaunchKernelGGL((kD_node<LBV, 4, (LBV == 64 ? 4 : 1), 0>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
                                    dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
             else if (LBV == 64 && cpb == 4)                                                                            \
-                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1), false>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
+                hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1), 0>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
                                    dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
             else if (tch == 8)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 8, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 8, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
             else if (tch == 4)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 4, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 4, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
             else                                                                                                       \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
         }                                                                                                              \
     }
     // one lane per level and per interface: LB >= NL
