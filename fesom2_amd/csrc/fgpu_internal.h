@@ -128,7 +128,6 @@ struct fgpu_context {
     int step_tpb = 4096;      // tracers per block of the fused step kernels (clamped to the tracer count)
     int step_tch = 2;         // tracers per Thomas chunk of the fused step's last kernel (2 or 4)
     int step_cpb = 1;         // columns per block of the fused step's last kernel (1, 4 or 8; 64-lane columns)
-    int step_vd = 0;          // 1: the implicit diffusion runs in a column kernel (k_vdiff) after K_D
     int stage_slots = 0;      // tr_xy/aux slots allocated for the per-stage kernels (0: none yet)
     // profiling
     bool prof = false;
@@ -194,8 +193,6 @@ int fgpu_ensure_staged(fgpu_context *c, int slots);
 // explicit diffusion terms (fgpu_diff.hip)
 int fgpu_ensure_diff(fgpu_context *c);
 int fgpu_launch_diff(fgpu_context *c, int tr0, int ntr, double dt);
-// implicit vertical diffusion, one thread per column (fgpu_transport.hip)
-int fgpu_launch_vdiff(fgpu_context *c, int tr0, int ntr, double dt, int do_diff, int clamp_tracer);
 
 // SSH solver pieces shared with the learned preconditioner
 struct FgpuCgSummary {

@@ -148,7 +148,6 @@ extern "C" int fgpu_create(fgpu_context **out, int device)
     if (const char *cv = std::getenv("FGPU_SSH_CG")) c->ssh_single_reduction = std::string(cv) != "classic";
     if (const char *tv = std::getenv("FGPU_STEP_TCH")) c->step_tch = std::atoi(tv);
     if (const char *pv = std::getenv("FGPU_STEP_CPB")) c->step_cpb = std::atoi(pv);
-    if (const char *vv = std::getenv("FGPU_STEP_VD")) c->step_vd = std::atoi(vv) ? 1 : 0;
     // default priority: stream priorities are left alone on purpose
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {

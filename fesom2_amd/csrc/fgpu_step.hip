@@ -601,7 +601,7 @@ __device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &
 // terms are compiled in. Both are launched only when in use.
 template <int LB, int TCH, int CPB, int XF>
 __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6))) kD_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer,
-                                                    int tpb, int vd_split)
+                                                    int tpb)
 {
     const int w = threadIdx.x / LB;
     const int n0 = xcd_column(m.N2) * CPB + w;
@@ -637,8 +637,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
     if (act) sZ[tid] = m.zn_vd[ci];
     __syncthreads();
     // tridiagonal coefficients (oce_ale_tracer.F90:1098-1175): tracer independent
-    // (vd_split: the implicit diffusion runs in k_vdiff after this kernel)
-    if (act && !vd_split) {
+    if (act) {
         const double *kv = d.Kv + (size_t)n * NL;
         const double zinv = 1.0 * dt;
         double a = 0.0, b, c;
@@ -807,7 +806,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
         // right-hand sides (oce_ale_tracer.F90:1206-1229, 1377-1383), all tracers of the chunk
         for (int c = 0; c < nch; ++c) {
             const int tr = t0 + c;
-            if (vd_split || !t.ivd[tr] || !act) continue;
+            if (!t.ivd[tr] || !act) continue;
             const double dz = hn;
             const double *ts = sTsA[w][c];
             const bool wim = WS && t.lim[tr] != LIM_FCT;
@@ -823,7 +822,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
         // Thomas sweeps (oce_ale_tracer.F90:1406-1430): one lane of the first wave
         // per (column, tracer) pair; the pivot chain (cp, m) is evaluated in the
         // same loop as tp so that the two recurrences overlap
-        if (!vd_split && threadIdx.x < CPB * nch) {
+        if (threadIdx.x < CPB * nch) {
             const int wc = threadIdx.x / nch, c = threadIdx.x % nch;
             const int z0 = sNz[wc][0], z1 = sNz[wc][1];
             if (z1 - 1 >= z0 && t.ivd[t0 + c]) {
@@ -856,7 +855,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
             for (int c = 0; c < nch; ++c) {
                 const int tr = t0 + c;
                 double v = sTsA[w][c][tid];
-                if (tr == clamp_tracer && !vd_split) {
+                if (tr == clamp_tracer) {
                     if (v > 45.0) v = 45.0;
                     if (v < 3.0) v = 3.0;
                 }
@@ -925,7 +924,6 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     bool dx = c->d.hdiff || c->d.redi;   // explicit diffusion increments or Redi K33 in K_D
     for (int v : c->h_ivd) dx = dx || !v;
     const int xf = (ws ? 1 : 0) | (dx ? 2 : 0);
-    const int vds = c->step_vd ? 1 : 0;
     {   // explicit diffusion terms from T^n (no-op without Ki, Redi or explicit vertical diffusion)
         int rc = fgpu_launch_diff(c, 0, nt, dt);
         if (rc) return rc;
@@ -956,36 +954,32 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
             if (xf == 1)                                                                                               \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (xf == 2)                                                                                          \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 2>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 2>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (xf == 3)                                                                                          \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 3>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 3>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (LBV == 64 && cpb == 8)                                                                            \
                 hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 8 : 1), 0>), dim3(xcd_grid((m.N2 + 7) / 8), ny),          \
-                                   dim3(LBV * (LBV == 64 ? 8 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
+                                   dim3(LBV * (LBV == 64 ? 8 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
             else if (LBV == 64 && cpb == 4 && tch == 4)                                                                \
                 hipLaunchKernelGGL((kD_node<LBV, 4, (LBV == 64 ? 4 : 1), 0>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
-                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
+                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
             else if (LBV == 64 && cpb == 4)                                                                            \
                 hipLaunchKernelGGL((kD_node<LBV, 2, (LBV == 64 ? 4 : 1), 0>), dim3(xcd_grid((m.N2 + 3) / 4), ny),          \
-                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds);       \
+                                   dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
             else if (tch == 8)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 8, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 8, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (tch == 4)                                                                                         \
-                hipLaunchKernelGGL((kD_node<LBV, 4, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 4, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else                                                                                                       \
-                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb, vds); \
+                hipLaunchKernelGGL((kD_node<LBV, 2, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
         }                                                                                                              \
     }
     // one lane per level and per interface: LB >= NL
     if (m.NL <= 64) LAUNCH5(64)
     else LAUNCH5(128)
 #undef LAUNCH5
-    if (vds) {   // implicit vertical diffusion and clamp, one thread per column
-        int rc = fgpu_launch_vdiff(c, 0, nt, dt, 1, clamp);
-        if (rc) return rc;
-    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("fused step launch: ") + hipGetErrorString(e));
     return FGPU_OK;

@@ -855,11 +855,6 @@ static int launch_vdiff(fgpu_context *c, int tr0, int ntr, double dt, int do_dif
     return FGPU_OK;
 }
 
-int fgpu_launch_vdiff(fgpu_context *c, int tr0, int ntr, double dt, int do_diff, int clamp_tracer)
-{
-    return launch_vdiff(c, tr0, ntr, dt, do_diff, clamp_tracer);
-}
-
 static int launch_init_ab(fgpu_context *c, int tr0, int ntr)
 {
     const DevMesh &m = c->m;
