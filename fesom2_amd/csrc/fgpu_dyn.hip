@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_cfl_max(const double *colmax, int n, un
 // of edge e (oce_ale.F90:1844-1880): one block per edge, the layer terms of
 // both elements computed by one lane per level, then each element's levels
 // summed in order (lane 0: el(1), lane 1: el(2)) as the reference does.
-template <int LB>
+template <int LB, bool HB>
 __global__ void __launch_bounds__(LB) k_ssh_rhs_edge(DevMesh m, const double *uv, const double *uvr, double alpha,
                                                      double *ce)
 {
@@ -214,17 +214,28 @@ __global__ void __launch_bounds__(LB) k_ssh_rhs_edge(DevMesh m, const double *uv
     const int el1 = et.x, el2 = et.y;
     const int nu1 = m.ulev[el1], nl1 = m.nlev[el1] - 1;
     const int nu2 = el2 >= 0 ? m.ulev[el2] : 1, nl2 = el2 >= 0 ? m.nlev[el2] - 1 : 0;
+    // HB: compute_hbar_ale's transport of u alone (oce_ale.F90:1988-2010)
     if (k >= nu1 && k <= nl1) {
         const double2 c = *(const double2 *)(m.ecd + 4 * (size_t)e);
         const size_t i = ((size_t)el1 * L + k - 1) * 2;
-        const double2 u = *(const double2 *)(uv + i), ur = *(const double2 *)(uvr + i);
-        s1[tid] = alpha * ((u.y + ur.y) * c.x - (u.x + ur.x) * c.y) * m.helem[(size_t)el1 * L + k - 1];
+        const double2 u = *(const double2 *)(uv + i);
+        if (HB) {
+            s1[tid] = (u.y * c.x - u.x * c.y) * m.helem[(size_t)el1 * L + k - 1];
+        } else {
+            const double2 ur = *(const double2 *)(uvr + i);
+            s1[tid] = alpha * ((u.y + ur.y) * c.x - (u.x + ur.x) * c.y) * m.helem[(size_t)el1 * L + k - 1];
+        }
     }
     if (k >= nu2 && k <= nl2) {
         const double2 c = *(const double2 *)(m.ecd + 4 * (size_t)e + 2);
         const size_t i = ((size_t)el2 * L + k - 1) * 2;
-        const double2 u = *(const double2 *)(uv + i), ur = *(const double2 *)(uvr + i);
-        s2[tid] = alpha * ((u.y + ur.y) * c.x - (u.x + ur.x) * c.y) * m.helem[(size_t)el2 * L + k - 1];
+        const double2 u = *(const double2 *)(uv + i);
+        if (HB) {
+            s2[tid] = (u.y * c.x - u.x * c.y) * m.helem[(size_t)el2 * L + k - 1];
+        } else {
+            const double2 ur = *(const double2 *)(uvr + i);
+            s2[tid] = alpha * ((u.y + ur.y) * c.x - (u.x + ur.x) * c.y) * m.helem[(size_t)el2 * L + k - 1];
+        }
     }
     __syncthreads();
     if (tid < 2) {
@@ -271,6 +282,41 @@ __global__ void k_ssh_rhs_node(DevMesh m, const double *ce, const double *old, c
         s = s + (1.0 - alpha) * old[n];
     }
     rhs[n] = s;
+}
+
+// compute_hbar_ale (oce_ale.F90:2013-2083): node gather of the edge
+// transports in edge order, the freshwater flux off linfs, hbar_old = hbar and
+// the new hbar at open-ocean nodes
+__global__ void k_hbar_node(DevMesh m, const double *ce, const double *wf, int which_ale, double dt, double *old,
+                            double *hbar, double *hbar_old)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.N2) return;
+    double s = 0.0;
+    for (int p = m.ne_ptr[n]; p < m.ne_ptr[n + 1]; ++p) {
+        const int q = m.ne_idx[p];
+        s = (q & 1) ? s - ce[q >> 1] : s + ce[q >> 1];
+    }
+    const int nzmin = m.ulevn[n];
+    const double av = m.areasvol[(size_t)n * m.NL + nzmin - 1];
+    if (which_ale != FGPU_ALE_LINFS && nzmin <= 1) s = s - wf[n] * av;
+    old[n] = s;
+    const double h0 = hbar[n];
+    hbar_old[n] = h0;
+    if (nzmin <= 1) hbar[n] = h0 + s * dt / av;   // cavity nodes keep hbar
+}
+
+// element thickness change dhe (oce_ale.F90:2084-2090)
+__global__ void k_hbar_dhe(DevMesh m, const double *hbar, const double *hbar_old, double *dhe)
+{
+    const int el = blockIdx.x * blockDim.x + threadIdx.x;
+    if (el >= m.F2) return;
+    if (m.ulev[el] > 1) {
+        dhe[el] = 0.0;
+        return;
+    }
+    const int a = m.elem_nodes[3 * el], b = m.elem_nodes[3 * el + 1], c = m.elem_nodes[3 * el + 2];
+    dhe[el] = ((hbar[a] - hbar_old[a]) + (hbar[b] - hbar_old[b]) + (hbar[c] - hbar_old[c])) / 3.0;
 }
 
 int ensure_buf(fgpu_context *c, double **p, size_t n)
@@ -346,10 +392,14 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
     a.use_wsplit = use_wsplit ? 1 : 0;
     a.uv = c->d.uv;
     if (which_ale == FGPU_ALE_ZSTAR) {
-        if (!hbar || !hbar_old) return fgpu_fail(FGPU_ERR_ARG, "zstar needs hbar and hbar_old");
         if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "zstar needs the surface water flux (fgpu_set_surface_forcing)");
-        if ((rc = upload_to(c, &c->vv_hbar, hbar, N2))) return rc;
-        if ((rc = upload_to(c, &c->vv_hbar_old, hbar_old, N2))) return rc;
+        if (!hbar && !hbar_old) {   // the device's hbar, hbar_old from fgpu_compute_hbar_ale
+            if (!c->hbar_valid) return fgpu_fail(FGPU_ERR_STATE, "zstar needs hbar and hbar_old (or fgpu_compute_hbar_ale first)");
+        } else {
+            if (!hbar || !hbar_old) return fgpu_fail(FGPU_ERR_ARG, "zstar needs both hbar and hbar_old");
+            if ((rc = upload_to(c, &c->vv_hbar, hbar, N2))) return rc;
+            if ((rc = upload_to(c, &c->vv_hbar_old, hbar_old, N2))) return rc;
+        }
         a.hbar = c->vv_hbar;
         a.hbar_old = c->vv_hbar_old;
         a.water_flux = c->d.water_flux;
@@ -417,24 +467,26 @@ extern "C" int fgpu_compute_ssh_rhs_ale(fgpu_context *c, const double *uv_rhs, c
 {
     int rc = check_dyn(c);
     if (rc) return rc;
-    if (!uv_rhs || !ssh_rhs_old) return fgpu_fail(FGPU_ERR_ARG, "null array");
+    if (!uv_rhs) return fgpu_fail(FGPU_ERR_ARG, "null array");
+    if (!ssh_rhs_old && !c->hbar_valid)
+        return fgpu_fail(FGPU_ERR_STATE, "ssh_rhs_old is NULL and fgpu_compute_hbar_ale has not run");
     if (which_ale != FGPU_ALE_LINFS && which_ale != FGPU_ALE_ZLEVEL && which_ale != FGPU_ALE_ZSTAR)
         return fgpu_fail(FGPU_ERR_ARG, "unknown which_ALE");
     if (which_ale != FGPU_ALE_LINFS && !c->forcing_ready)
         return fgpu_fail(FGPU_ERR_STATE, "the full free surface needs the water flux (fgpu_set_surface_forcing)");
     const size_t N2 = c->N2, E2 = c->E2, F2 = c->F2, L = c->nl - 1;
     if ((rc = upload_to(c, &c->vv_uvr, uv_rhs, F2 * L * 2))) return rc;
-    if ((rc = upload_to(c, &c->vv_old, ssh_rhs_old, N2))) return rc;
+    if (ssh_rhs_old && (rc = upload_to(c, &c->vv_old, ssh_rhs_old, N2))) return rc;   // else the device's
     if ((rc = ensure_buf(c, &c->vv_ce, E2))) return rc;
     if ((rc = ensure_buf(c, &c->vv_rhs, N2))) return rc;
     const double *wf = c->d.water_flux ? c->d.water_flux : c->vv_old;   // unread under linfs
     {
         ProfScope ps(c, "k_ssh_rhs_edge");
         if (c->nl - 1 <= 64)
-            hipLaunchKernelGGL(k_ssh_rhs_edge<64>, dim3(c->E2), dim3(64), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
+            hipLaunchKernelGGL((k_ssh_rhs_edge<64, false>), dim3(c->E2), dim3(64), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
                                alpha, c->vv_ce);
         else
-            hipLaunchKernelGGL(k_ssh_rhs_edge<128>, dim3(c->E2), dim3(128), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
+            hipLaunchKernelGGL((k_ssh_rhs_edge<128, false>), dim3(c->E2), dim3(128), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
                                alpha, c->vv_ce);
         FGPU_HIP(hipGetLastError());
     }
@@ -450,5 +502,49 @@ extern "C" int fgpu_compute_ssh_rhs_ale(fgpu_context *c, const double *uv_rhs, c
     if (ssh_rhs) FGPU_HIP(hipMemcpyAsync(ssh_rhs, c->vv_rhs, sizeof(double) * N2, hipMemcpyDeviceToHost, c->stream));
     FGPU_HIP(hipStreamSynchronize(c->stream));
     if (c->prof && (rc = fgpu_prof_flush(c))) return rc;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_compute_hbar_ale(fgpu_context *c, double dt, int which_ale, const double *hbar, double *hbar_out,
+                                     double *hbar_old_out, double *ssh_rhs_old_out, double *dhe_out)
+{
+    int rc = check_dyn(c);
+    if (rc) return rc;
+    if (which_ale != FGPU_ALE_LINFS && which_ale != FGPU_ALE_ZLEVEL && which_ale != FGPU_ALE_ZSTAR)
+        return fgpu_fail(FGPU_ERR_ARG, "unknown which_ALE");
+    if (which_ale != FGPU_ALE_LINFS && !c->forcing_ready)
+        return fgpu_fail(FGPU_ERR_STATE, "the full free surface needs the water flux (fgpu_set_surface_forcing)");
+    if (!hbar && !c->hbar_valid) return fgpu_fail(FGPU_ERR_ARG, "hbar is NULL and no earlier hbar is on the device");
+    const size_t N2 = c->N2, E2 = c->E2, F2 = c->F2;
+    if (hbar && (rc = upload_to(c, &c->vv_hbar, hbar, N2))) return rc;
+    if ((rc = ensure_buf(c, &c->vv_hbar_old, N2))) return rc;
+    if ((rc = ensure_buf(c, &c->vv_old, N2))) return rc;
+    if ((rc = ensure_buf(c, &c->vv_ce, E2))) return rc;
+    if ((rc = ensure_buf(c, &c->vv_dhe, F2))) return rc;
+    const double *wf = c->d.water_flux ? c->d.water_flux : c->vv_old;   // unread under linfs
+    {
+        ProfScope ps(c, "k_hbar_edge");
+        if (c->nl - 1 <= 64)
+            hipLaunchKernelGGL((k_ssh_rhs_edge<64, true>), dim3(c->E2), dim3(64), 0, c->stream, c->m, c->d.uv, c->d.uv,
+                               1.0, c->vv_ce);
+        else
+            hipLaunchKernelGGL((k_ssh_rhs_edge<128, true>), dim3(c->E2), dim3(128), 0, c->stream, c->m, c->d.uv, c->d.uv,
+                               1.0, c->vv_ce);
+        FGPU_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_hbar_node, dim3((c->N2 + 255) / 256), dim3(256), 0, c->stream, c->m, (const double *)c->vv_ce,
+                       wf, which_ale, dt, c->vv_old, c->vv_hbar, c->vv_hbar_old);
+    FGPU_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hbar_dhe, dim3((c->F2 + 255) / 256), dim3(256), 0, c->stream, c->m, (const double *)c->vv_hbar,
+                       (const double *)c->vv_hbar_old, c->vv_dhe);
+    FGPU_HIP(hipGetLastError());
+    struct { double *dst; const double *src; size_t n; } cp[] = {
+        {hbar_out, c->vv_hbar, N2}, {hbar_old_out, c->vv_hbar_old, N2}, {ssh_rhs_old_out, c->vv_old, N2},
+        {dhe_out, c->vv_dhe, F2}};
+    for (auto &x : cp)
+        if (x.dst) FGPU_HIP(hipMemcpyAsync(x.dst, x.src, sizeof(double) * x.n, hipMemcpyDeviceToHost, c->stream));
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    if (c->prof && (rc = fgpu_prof_flush(c))) return rc;
+    c->hbar_valid = true;
     return FGPU_OK;
 }

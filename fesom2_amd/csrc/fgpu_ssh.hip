@@ -581,7 +581,9 @@ extern "C" int fgpu_init_stiff_mat_ale(fgpu_context *c, double dt, double g, dou
 extern "C" int fgpu_update_stiff_mat_ale(fgpu_context *c, const double *dhe, double dt, double g, double alpha,
                                          double theta)
 {
-    if (!c || !dhe) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!dhe && !c->hbar_valid)
+        return fgpu_fail(FGPU_ERR_ARG, "dhe is NULL and fgpu_compute_hbar_ale has not run");
     if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
     if (!c->ssh_ready || c->ssh_n != c->N2) return fgpu_fail(FGPU_ERR_STATE, "SSH matrix not assembled (fgpu_init_stiff_mat_ale)");
     FGPU_HIP(hipSetDevice(c->device));
@@ -589,8 +591,8 @@ extern "C" int fgpu_update_stiff_mat_ale(fgpu_context *c, const double *dhe, dou
         int rc = fgpu_alloc(c, (void **)&c->vv_dhe, sizeof(double) * (size_t)c->F2);
         if (rc) return rc;
     }
-    double *d_dhe = c->vv_dhe;
-    FGPU_HIP(hipMemcpyAsync(d_dhe, dhe, sizeof(double) * (size_t)c->F2, hipMemcpyHostToDevice, c->stream));
+    double *d_dhe = c->vv_dhe;   // NULL dhe: the device's, from fgpu_compute_hbar_ale
+    if (dhe) FGPU_HIP(hipMemcpyAsync(d_dhe, dhe, sizeof(double) * (size_t)c->F2, hipMemcpyHostToDevice, c->stream));
     const double factor = g * dt * alpha * theta;
     hipLaunchKernelGGL(k_stiff_rows, dim3((c->N2 + SBLK - 1) / SBLK), dim3(SBLK), 0, c->stream, c->m, c->ssh_rowptr,
                        c->ssh_colind, c->ssh_val, dt, factor, (const double *)d_dhe);

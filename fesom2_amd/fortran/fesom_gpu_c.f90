@@ -36,7 +36,8 @@ module fesom_gpu_c
        fgpu_ssh_download, fgpu_autotune_step, fgpu_step_shape, fgpu_ssh_nn_param_count, fgpu_ssh_nn_init, &
        fgpu_ssh_nn_set_weights, fgpu_ssh_nn_get_weights, fgpu_ssh_nn_apply, fgpu_ssh_nn_loss_grad, &
        fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident, fgpu_set_uv, &
-       fgpu_vert_vel_ale, fgpu_vert_vel_download, fgpu_compute_ssh_rhs_ale, fgpu_update_stiff_mat_ale
+       fgpu_vert_vel_ale, fgpu_vert_vel_download, fgpu_compute_ssh_rhs_ale, fgpu_update_stiff_mat_ale, &
+       fgpu_compute_hbar_ale
 
   interface
      type(c_ptr) function fgpu_last_error() bind(C, name="fgpu_last_error")
@@ -158,6 +159,16 @@ module fesom_gpu_c
        real(c_double), intent(in) :: dhe(*)
        real(c_double), value :: dt, g, alpha, theta
      end function fgpu_update_stiff_mat_ale
+
+     integer(c_int) function fgpu_compute_hbar_ale(ctx, dt, which_ale, hbar, hbar_out, hbar_old_out, &
+          ssh_rhs_old_out, dhe_out) bind(C, name="fgpu_compute_hbar_ale")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dt
+       integer(c_int), value :: which_ale
+       real(c_double), intent(in) :: hbar(*)
+       real(c_double), intent(out) :: hbar_out(*), hbar_old_out(*), ssh_rhs_old_out(*), dhe_out(*)
+     end function fgpu_compute_hbar_ale
 
      integer(c_int) function fgpu_tracers_init(ctx, num_tracers) bind(C, name="fgpu_tracers_init")
        import :: c_int, c_ptr

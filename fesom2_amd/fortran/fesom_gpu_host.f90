@@ -48,7 +48,7 @@ module fesom_gpu_host
        init_tracers_AB, do_oce_adv_tra, diff_tracers_ale, diff_ver_part_impl_ale, solve_tracers_ale, &
        init_stiff_mat_ale, ssh_solve_preconditioner, ssh_solve_cg, solve_ssh_ale, gpu_synchronize, &
        gpu_autotune_step, gpu_ssh_nn_init, gpu_ssh_nn_train, ssh_solve_cg_learned, vert_vel_ale, &
-       gpu_vert_vel_ale, compute_ssh_rhs_ale, update_stiff_mat_ale
+       gpu_vert_vel_ale, compute_ssh_rhs_ale, update_stiff_mat_ale, compute_hbar_ale
 
 contains
 
@@ -217,6 +217,21 @@ contains
     call gpu_check(fgpu_compute_ssh_rhs_ale(ctx, uv_rhs, ssh_rhs_old, alpha, ale_code(which_ALE), cav, ssh_rhs), &
          'compute_ssh_rhs_ale')
   end subroutine compute_ssh_rhs_ale
+
+  ! compute_hbar_ale (oce_ale.F90:1956-2091) from uv: hbar is updated in place,
+  ! hbar_old, ssh_rhs_old and dhe are written, like mesh%hbar, hbar_old and
+  ! dynamics%ssh_rhs_old, mesh%dhe in the reference
+  subroutine compute_hbar_ale(dt, which_ALE, uv, hbar, hbar_old, ssh_rhs_old, dhe)
+    real(WP), intent(in) :: dt, uv(:, :, :)
+    character(len=*), intent(in) :: which_ALE
+    real(WP), intent(inout) :: hbar(:)
+    real(WP), intent(out) :: hbar_old(:), ssh_rhs_old(:), dhe(:)
+    real(WP), allocatable :: h0(:)
+    h0 = hbar
+    call gpu_check(fgpu_set_uv(ctx, uv), 'fgpu_set_uv')
+    call gpu_check(fgpu_compute_hbar_ale(ctx, dt, ale_code(which_ALE), h0, hbar, hbar_old, ssh_rhs_old, dhe), &
+         'compute_hbar_ale')
+  end subroutine compute_hbar_ale
 
   ! update_stiff_mat_ale (oce_ale.F90:1683-1792): the stiffness term of the
   ! element ssh change dhe(elem2D) added to the assembled SSH matrix

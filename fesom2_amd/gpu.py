@@ -54,6 +54,7 @@ _SIGS = {
     "fgpu_vert_vel_ale": [vp, C.c_double, C.c_int, dp, dp, C.c_int, C.c_double, dp, ip],
     "fgpu_vert_vel_download": [vp, dp, dp, dp, dp, dp],
     "fgpu_compute_ssh_rhs_ale": [vp, dp, dp, C.c_double, C.c_int, C.c_int, dp],
+    "fgpu_compute_hbar_ale": [vp, C.c_double, C.c_int, dp, dp, dp, dp, dp],
     "fgpu_tracers_init": [vp, C.c_int],
     "fgpu_tracer_config": [vp, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
                            C.c_double, C.c_int],
@@ -155,6 +156,7 @@ class Context:
         self.ntr = 0
         self.nl = 0
         self.N2 = 0
+        self.F2 = 0
 
     def close(self):
         if self.h:
@@ -184,6 +186,7 @@ class Context:
                                      _d(k[13]), _d(k[14]), _d(k[15]), _d(k[16])))
         self.nl = m.nl
         self.N2 = m.nod2D
+        self.F2 = m.elem2D
         self.set_vertical(m)
 
     def set_vertical(self, m):
@@ -243,9 +246,23 @@ class Context:
         return out
 
     def compute_ssh_rhs_ale(self, uv_rhs, ssh_rhs_old, alpha=1.0, which_ale="linfs", use_cavity_fw2press=False):
+        """ssh_rhs_old None: the device's, from compute_hbar_ale."""
         out = np.zeros(self.N2)
-        _check(self.L.fgpu_compute_ssh_rhs_ale(self.h, _d(_f64(uv_rhs)), _d(_f64(ssh_rhs_old)), float(alpha),
-                                               self.ALE[which_ale], int(bool(use_cavity_fw2press)), _d(out)))
+        old = None if ssh_rhs_old is None else _f64(ssh_rhs_old)
+        _check(self.L.fgpu_compute_ssh_rhs_ale(self.h, _d(_f64(uv_rhs)), None if old is None else _d(old),
+                                               float(alpha), self.ALE[which_ale], int(bool(use_cavity_fw2press)),
+                                               _d(out)))
+        return out
+
+    def compute_hbar_ale(self, dt, which_ale="linfs", hbar=None):
+        """compute_hbar_ale; hbar None continues from the device's. Returns
+        dict(hbar, hbar_old, ssh_rhs_old, dhe); all four also stay on the device."""
+        h = None if hbar is None else _f64(hbar)
+        out = dict(hbar=np.zeros(self.N2), hbar_old=np.zeros(self.N2), ssh_rhs_old=np.zeros(self.N2),
+                   dhe=np.zeros(self.F2))
+        _check(self.L.fgpu_compute_hbar_ale(self.h, float(dt), self.ALE[which_ale], None if h is None else _d(h),
+                                            _d(out["hbar"]), _d(out["hbar_old"]), _d(out["ssh_rhs_old"]),
+                                            _d(out["dhe"])))
         return out
 
     # ------------------------------------------------------------ tracers
@@ -342,7 +359,10 @@ class Context:
         _check(self.L.fgpu_init_stiff_mat_ale(self.h, float(dt), float(g), float(alpha), float(theta)))
 
     def update_stiff_mat_ale(self, dhe, dt, g=9.81, alpha=1.0, theta=1.0):
-        _check(self.L.fgpu_update_stiff_mat_ale(self.h, _d(_f64(dhe)), float(dt), float(g), float(alpha), float(theta)))
+        """dhe None: the device's, from compute_hbar_ale."""
+        d = None if dhe is None else _f64(dhe)
+        _check(self.L.fgpu_update_stiff_mat_ale(self.h, None if d is None else _d(d), float(dt), float(g),
+                                                float(alpha), float(theta)))
 
     def ssh_set_matrix(self, rowptr1, colind1, values):
         rowptr1, colind1 = _i32(rowptr1), _i32(colind1)

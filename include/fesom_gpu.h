@@ -112,7 +112,8 @@ int fgpu_set_uv(fgpu_context *ctx, const double *uv);
  * (oce_ale.F90:2910-2997) and compute_Wvel_split (oce_ale.F90:3003-3049), on
  * the device from the resident uv: dynamics%w, w_e, w_i and cfl_z stay in HBM
  * for the transport (fgpu_vert_vel_download copies them out). which_ALE linfs
- * or zstar; zstar also reads hbar, hbar_old (nod2D, host) and the water flux
+ * or zstar; zstar also reads hbar, hbar_old (nod2D, host; both NULL: the
+ * device's from fgpu_compute_hbar_ale) and the water flux
  * of fgpu_set_surface_forcing, and updates hnode_new and the implicit
  * diffusion's Z levels. Levels outside [ulevels_nod2D, nlevels_nod2D] of w_e
  * and w_i are written as zero. use_wsplit = 1 splits W where CFL_z exceeds
@@ -130,9 +131,23 @@ int fgpu_vert_vel_download(fgpu_context *ctx, double *w, double *w_e, double *w_
  * (which_ale != FGPU_ALE_LINFS) also reads the water flux of
  * fgpu_set_surface_forcing. The result becomes the SSH solve's right-hand side
  * when the SSH system is set up (fgpu_ssh_solve_resident then solves it) and is
- * copied to ssh_rhs (nod2D) unless that is NULL. */
+ * copied to ssh_rhs (nod2D) unless that is NULL. ssh_rhs_old = NULL takes the
+ * device's, written by the last fgpu_compute_hbar_ale (the reference's
+ * dynamics%ssh_rhs_old). */
 int fgpu_compute_ssh_rhs_ale(fgpu_context *ctx, const double *uv_rhs, const double *ssh_rhs_old, double alpha,
                              int which_ale, int use_cavity_fw2press, double *ssh_rhs);
+
+/* compute_hbar_ale (oce_ale.F90:1956-2091) from the resident uv:
+ * ssh_rhs_old = the column transport divergence (minus water_flux * areasvol
+ * at open-ocean nodes off linfs), hbar_old = hbar, hbar += ssh_rhs_old * dt /
+ * areasvol at open-ocean nodes, dhe per element (the mean hbar change, 0 under
+ * cavities). hbar (nod2D) may be NULL to continue from the device's hbar of
+ * the previous call. The four results stay on the device for
+ * fgpu_vert_vel_ale (zstar, hbar and hbar_old NULL), fgpu_update_stiff_mat_ale
+ * (dhe NULL) and fgpu_compute_ssh_rhs_ale (ssh_rhs_old NULL); each *_out that
+ * is not NULL receives a copy (hbar, hbar_old, ssh_rhs_old: nod2D; dhe: elem2D). */
+int fgpu_compute_hbar_ale(fgpu_context *ctx, double dt, int which_ale, const double *hbar, double *hbar_out,
+                          double *hbar_old_out, double *ssh_rhs_old_out, double *dhe_out);
 
 /* ---- tracers: t_tracer (MOD_TRACER.F90:10-109) ---------------------------
  * num_tracers tracers; each configured with its namelist entry
@@ -199,7 +214,7 @@ int fgpu_init_stiff_mat_ale(fgpu_context *ctx, double dt, double g, double alpha
  * stiffness term of the ssh change dhe (elem2D) to the matrix assembled by
  * fgpu_init_stiff_mat_ale, with factor g*dt*alpha*theta. Like the reference,
  * the preconditioner is not rebuilt (call fgpu_ssh_solve_preconditioner to
- * refresh it). */
+ * refresh it). dhe = NULL takes the device's, from fgpu_compute_hbar_ale. */
 int fgpu_update_stiff_mat_ale(fgpu_context *ctx, const double *dhe, double dt, double g, double alpha, double theta);
 
 /* Alternatively hand over an assembled ssh_stiff (MOD_MESH.F90:11-20), 1-based,

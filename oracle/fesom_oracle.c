@@ -43,6 +43,8 @@
  *   32 Redi vertical flux weights the upper layer by the distance to the interface below
  *      (diff_ver_part_redi_expl)
  *   33 Redi K33 below an interior interface scaled by the spacing below (diff_ver_part_impl_ale)
+ *   34 hbar updated without dividing by the node's area (compute_hbar_ale)
+ *   35 dhe averages two of the element's three nodes (compute_hbar_ale)
  */
 #include "fesom_oracle.h"
 
@@ -1549,6 +1551,61 @@ int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale
             }
         }
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_hbar_ale: oce_ale.F90:1956-2091. Column-integrated transport     */
+/* divergence per node (edge scatter in edge order), the freshwater flux   */
+/* off linfs, the new hbar and the element thickness change dhe.            */
+/* ------------------------------------------------------------------------ */
+int fo_compute_hbar_ale(const fo_mesh *m, const double *uv, const double *water_flux, double dt, int which_ale,
+                        double *hbar, double *hbar_old, double *ssh_rhs_old, double *dhe)
+{
+    const int L = m->nl - 1, NL = m->nl, N2 = m->nod2D;
+    for (int n = 0; n < N2; ++n) ssh_rhs_old[n] = 0.0;
+    for (int ed = 0; ed < m->edge2D; ++ed) {
+        const int n1 = m->edges[2 * ed], n2 = m->edges[2 * ed + 1];
+        const int el1 = m->edge_tri[2 * ed], el2 = m->edge_tri[2 * ed + 1];
+        double c1 = 0.0, c2 = 0.0;
+        {
+            const double dX = m->edge_cross_dxdy[4 * ed], dY = m->edge_cross_dxdy[4 * ed + 1];
+            for (int nz = m->ulevels[el1]; nz <= m->nlevels[el1] - 1; ++nz) {
+                const size_t i = ((size_t)el1 * L + nz - 1) * 2;
+                c1 = c1 + (uv[i + 1] * dX - uv[i] * dY) * m->helem[LAY(el1, nz)];
+            }
+        }
+        if (el2 >= 0) {
+            const double dX = m->edge_cross_dxdy[4 * ed + 2], dY = m->edge_cross_dxdy[4 * ed + 3];
+            for (int nz = m->ulevels[el2]; nz <= m->nlevels[el2] - 1; ++nz) {
+                const size_t i = ((size_t)el2 * L + nz - 1) * 2;
+                c2 = c2 - (uv[i + 1] * dX - uv[i] * dY) * m->helem[LAY(el2, nz)];
+            }
+        }
+        ssh_rhs_old[n1] = ssh_rhs_old[n1] + (c1 + c2);
+        ssh_rhs_old[n2] = ssh_rhs_old[n2] - (c1 + c2);
+    }
+    if (which_ale != 0)
+        for (int n = 0; n < N2; ++n) {
+            if (m->ulevels_nod2D[n] > 1) continue;
+            ssh_rhs_old[n] = ssh_rhs_old[n] - water_flux[n] * m->areasvol[INT(n, m->ulevels_nod2D[n])];
+        }
+    for (int n = 0; n < N2; ++n) hbar_old[n] = hbar[n];
+    for (int n = 0; n < N2; ++n) {
+        if (m->ulevels_nod2D[n] > 1) continue;   /* cavity node: hbar = hbar_old */
+        const double av = (g_mut == 34) ? 1.0 : m->areasvol[INT(n, m->ulevels_nod2D[n])];
+        hbar[n] = hbar_old[n] + ssh_rhs_old[n] * dt / av;
+    }
+    for (int el = 0; el < m->elem2D; ++el) {
+        const int *en = m->elem2D_nodes + 3 * (size_t)el;
+        if (m->ulevels[el] > 1) {
+            dhe[el] = 0.0;
+        } else {
+            const double d3 = (g_mut == 35) ? 0.0 : (hbar[en[2]] - hbar_old[en[2]]);
+            dhe[el] = ((hbar[en[0]] - hbar_old[en[0]]) + (hbar[en[1]] - hbar_old[en[1]]) + d3) / 3.0;
+        }
+    }
+    (void)NL;
     return 0;
 }
 

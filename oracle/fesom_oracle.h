@@ -132,6 +132,11 @@ int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *v
 int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale, const double *hbar,
                     const double *hbar_old, const double *water_flux, int use_wsplit, double wsplit_maxcfl,
                     double *w, double *w_e, double *w_i, double *cfl_z, double *hnode_new, double *cflmax);
+/* compute_hbar_ale: ssh_rhs_old = column transport divergence (minus the
+ * freshwater flux off linfs), hbar_old = hbar, hbar += ssh_rhs_old dt/areasvol,
+ * dhe per element; hbar is updated in place */
+int fo_compute_hbar_ale(const fo_mesh *m, const double *uv, const double *water_flux, double dt, int which_ale,
+                        double *hbar, double *hbar_old, double *ssh_rhs_old, double *dhe);
 int fo_compute_ssh_rhs_ale(const fo_mesh *m, const double *uv, const double *uv_rhs, const double *ssh_rhs_old,
                            const double *water_flux, double alpha, int which_ale, int use_cavity_fw2press,
                            double *ssh_rhs);

@@ -112,6 +112,7 @@ def lib():
                                       C.c_double, dp, dp, dp, dp, dp, dp]
         L.fo_update_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, dp, ip, ip, dp]
         L.fo_compute_ssh_rhs_ale.argtypes = [C.POINTER(FoMesh), dp, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp]
+        L.fo_compute_hbar_ale.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, C.c_int, dp, dp, dp, dp]
         L.fo_ssh_solve_preconditioner.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp]
         L.fo_ssh_solve_cg.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_int,
                                       C.POINTER(FoCgStats)]
@@ -335,6 +336,18 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle compute_ssh_rhs_ale failed rc={rc}")
         return out
+
+    def compute_hbar_ale(self, uv, dt, hbar, which_ale=0, water_flux=None):
+        """compute_hbar_ale on copies: dict(hbar, hbar_old, ssh_rhs_old, dhe)."""
+        m = self.mesh
+        h = np.array(hbar, dtype=np.float64, order="C", copy=True)
+        ho, old, dhe = np.zeros(m.nod2D), np.zeros(m.nod2D), np.zeros(m.elem2D)
+        wf = np.zeros(m.nod2D) if water_flux is None else np.ascontiguousarray(water_flux, dtype=np.float64)
+        rc = lib().fo_compute_hbar_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(wf), float(dt),
+                                       int(which_ale), _d(h), _d(ho), _d(old), _d(dhe))
+        if rc:
+            raise RuntimeError(f"oracle compute_hbar_ale failed rc={rc}")
+        return dict(hbar=h, hbar_old=ho, ssh_rhs_old=old, dhe=dhe)
 
     def nboundary_lay(self):
         out = np.zeros(self.mesh.nod2D, np.int32)

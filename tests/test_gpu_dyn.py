@@ -279,3 +279,67 @@ def test_update_stiff_mat_bitwise(c0, c0_oracle):
     np.testing.assert_array_equal(dv, ref)
     A = sp.csr_matrix((ref.copy(), ci.copy(), rp.copy()), shape=(mesh.nod2D, mesh.nod2D))
     assert st["converged"] and np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10 * 1.01
+
+
+@pytest.mark.parametrize("which", ["linfs", "zstar"])
+@pytest.mark.parametrize("cfg", ["c0", "c1"])
+def test_compute_hbar_bitwise(cfg, which, c0, c0_oracle, request):
+    """compute_hbar_ale: hbar, hbar_old, ssh_rhs_old and dhe equal the oracle's
+    bit for bit."""
+    if cfg == "c0":
+        mesh, state, _ = c0
+        o = c0_oracle
+    else:
+        mesh, state, o = request.getfixturevalue("c1")
+    rng = np.random.default_rng(15)
+    h0 = rng.normal(0.0, 0.3, mesh.nod2D)
+    wf = rng.normal(0.0, 1e-5, mesh.nod2D)
+    print(f"[gpu] compute_hbar_ale {cfg} {which}", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        z = np.zeros(mesh.nod2D)
+        ctx.set_surface_forcing(z, wf, z, z, z)
+        ctx.set_uv(state.uv)
+        dev = ctx.compute_hbar_ale(state.dt, which_ale=which, hbar=h0)
+    finally:
+        ctx.close()
+    ref = o.compute_hbar_ale(state.uv, state.dt, h0, which_ale={"linfs": 0, "zstar": 2}[which], water_flux=wf)
+    for key in ("hbar", "hbar_old", "ssh_rhs_old", "dhe"):
+        np.testing.assert_array_equal(dev[key], ref[key], err_msg=key)
+    assert np.any(ref["hbar"] != h0)
+
+
+def test_hbar_feeds_the_next_steps_on_the_device(c0, c0_oracle):
+    """compute_hbar_ale's results stay resident: vert_vel_ale (zstar) with no
+    host hbar, compute_ssh_rhs_ale with no host ssh_rhs_old and
+    update_stiff_mat_ale with no host dhe equal the oracle given the oracle's
+    compute_hbar_ale results."""
+    mesh, state, _ = c0
+    rng = np.random.default_rng(16)
+    h0 = rng.normal(0.0, 0.3, mesh.nod2D)
+    wf = rng.normal(0.0, 1e-5, mesh.nod2D)
+    uvr = rng.normal(0.0, 0.05, state.uv.shape)
+    print("[gpu] compute_hbar_ale -> vert_vel_ale, compute_ssh_rhs_ale, update_stiff_mat_ale", flush=True)
+    ctx = _ctx(mesh, state)
+    try:
+        z = np.zeros(mesh.nod2D)
+        ctx.set_surface_forcing(z, wf, z, z, z)
+        ctx.set_uv(state.uv)
+        ctx.init_stiff_mat_ale(state.dt)
+        ctx.compute_hbar_ale(state.dt, which_ale="zstar", hbar=h0)
+        rhs = ctx.compute_ssh_rhs_ale(uvr, None, alpha=0.6, which_ale="zstar")
+        ctx.update_stiff_mat_ale(None, state.dt)
+        _, _, dv, _ = ctx.ssh_get_matrix()
+        ctx.vert_vel_ale(state.dt, which_ale="zstar")
+        dev = ctx.vert_vel_download()
+    finally:
+        ctx.close()
+    hb = c0_oracle.compute_hbar_ale(state.uv, state.dt, h0, which_ale=2, water_flux=wf)
+    np.testing.assert_array_equal(rhs, c0_oracle.compute_ssh_rhs_ale(state.uv, uvr, hb["ssh_rhs_old"], alpha=0.6,
+                                                                      which_ale=2, water_flux=wf))
+    rp, ci, v = c0_oracle.stiff_matrix(state.dt)
+    np.testing.assert_array_equal(dv, c0_oracle.update_stiff_matrix(state.dt, hb["dhe"], rp, ci, v))
+    ref = c0_oracle.vert_vel_ale(state.uv, state.dt, which_ale=2, hbar=hb["hbar"], hbar_old=hb["hbar_old"],
+                                 water_flux=wf)
+    _compare_vv(dev, ref)
+    np.testing.assert_array_equal(dev["hnode_new"], ref["hnode_new"])

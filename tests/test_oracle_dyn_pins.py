@@ -195,6 +195,45 @@ def check_ssh_rhs_alpha_terms(mesh, st, o):
     return bool(ok)
 
 
+def check_hbar_transport(mesh, st, o):
+    """compute_hbar_ale: the surface rises by the column convergence, i.e.
+    (hbar - hbar_old) areasvol / dt equals the pinned surface W area of
+    vert_vel_ale, and ssh_rhs_old equals compute_ssh_rhs_ale with alpha = 1
+    and no uv_rhs."""
+    rng = np.random.default_rng(7)
+    wet = _wet_elem(mesh)
+    uv = np.where(wet[..., None], rng.normal(0, 0.3, st.uv.shape), 0.0)
+    h0 = rng.normal(0, 0.5, mesh.nod2D)
+    r = o.compute_hbar_ale(uv, st.dt, h0)
+    n = np.arange(mesh.nod2D)
+    k = mesh.ulevels_nod2D - 1
+    col = o.vert_vel_ale(uv, st.dt)["w"][n, k] * mesh.area[n, k]
+    scale = _transport_scale(mesh, uv)
+    ok = np.array_equal(r["hbar_old"], h0)
+    ok &= np.abs((r["hbar"] - h0) * mesh.areasvol[n, k] / st.dt - col).max() < 1e-9 * scale
+    rhs = o.compute_ssh_rhs_ale(uv, np.zeros_like(uv), np.zeros(mesh.nod2D), alpha=1.0)
+    ok &= np.allclose(r["ssh_rhs_old"], rhs, rtol=1e-13, atol=1e-12 * scale)
+    return bool(ok)
+
+
+def check_hbar_freshwater(mesh, st, o):
+    """At rest under the full free surface, a freshwater flux moves the
+    surface by -water_flux dt at every node, the element thickness change is
+    the mean of its nodes', and the areasvol-weighted volume change is
+    -dt sum(water_flux areasvol)."""
+    rng = np.random.default_rng(8)
+    wf = rng.normal(0, 1e-5, mesh.nod2D)
+    h0 = rng.normal(0, 0.5, mesh.nod2D)
+    r = o.compute_hbar_ale(np.zeros_like(st.uv), st.dt, h0, which_ale=2, water_flux=wf)
+    dh = r["hbar"] - h0
+    ok = np.allclose(dh, -wf * st.dt, rtol=1e-12, atol=1e-15)
+    ok &= np.allclose(r["dhe"], (-wf * st.dt)[mesh.elem2D_nodes].mean(1), rtol=1e-12, atol=1e-15)
+    n = np.arange(mesh.nod2D)
+    av = mesh.areasvol[n, mesh.ulevels_nod2D - 1]
+    ok &= np.isclose((dh * av).sum(), -st.dt * (wf * av).sum(), rtol=1e-10)
+    return bool(ok)
+
+
 def check_zstar(mesh, st, o):
     rng = np.random.default_rng(6)
     dh = rng.normal(0, 0.05, mesh.nod2D)
@@ -277,6 +316,14 @@ def test_update_stiff_mat(bumps):
     assert check_update_stiff(*bumps)
 
 
+def test_hbar_transport(bumps):
+    assert check_hbar_transport(*bumps)
+
+
+def test_hbar_freshwater(bumps):
+    assert check_hbar_freshwater(*bumps)
+
+
 def test_zlevel_is_rejected(bumps):
     mesh, st, o = bumps
     with pytest.raises(RuntimeError):
@@ -298,6 +345,8 @@ MUTATIONS = {
     21: lambda b, f: check_ssh_rhs_column_divergence(*b),
     22: lambda b, f: check_cfl(*b),
     23: lambda b, f: check_update_stiff(*b),
+    34: lambda b, f: check_hbar_transport(*b),
+    35: lambda b, f: check_hbar_freshwater(*b),
 }
 
 
