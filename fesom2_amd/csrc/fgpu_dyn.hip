@@ -12,10 +12,16 @@
 
 namespace {
 
+__device__ __forceinline__ double dmx(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmn(double a, double b) { return a < b ? a : b; }
+
 struct VVArgs {
     double dt, maxcfl;
     int which_ale, use_wsplit;
     const double *uv, *hbar, *hbar_old, *water_flux;
+    const double *zbar;                // zlevel: reference interface depths (nl)
+    double min_hnode;                  // zlevel: thinnest layer, in units of its nominal thickness
+    int lzstar;                        // zlevel: layers of the local zstar spread
     double *w, *w_e, *w_i, *cfl, *hnode_new;
     double *colmax;                    // max of CFL_z per column (nod2D)
     int *nan_columns;
@@ -27,7 +33,9 @@ struct VVArgs {
 //   2. cumulative sum from the bottom, division by the interface area
 //      (oce_ale.F90:2277-2320)
 //   3. zstar: the ssh change spread over the layers above nlevels_nod2D_min,
-//      new layer thicknesses, surface water flux (oce_ale.F90:2539-2600)
+//      new layer thicknesses, surface water flux (oce_ale.F90:2539-2600);
+//      zlevel: the change in the surface layer, or spread over the top
+//      lzstar_lev layers by one lane (oce_ale.F90:2336-2420)
 //   4. CFL_z (oce_ale.F90:2938-2966) and the W split (oce_ale.F90:3035-3047)
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 __device__ __forceinline__ double rld(double v, int lane)
@@ -46,6 +54,7 @@ __global__ void __launch_bounds__(LB) k_vert_vel(DevMesh m, VVArgs a)
     const int n = blockIdx.x;
     if (n >= m.N2) return;
     __shared__ double sW[LB + 1], sH[LB + 1];
+    __shared__ double sZl[33];   // zlevel: the new thicknesses of layers 1..lzstar_lev
     __shared__ int sBad;
     const int tid = threadIdx.x, k = tid + 1;
     if (tid == 0) sBad = 0;
@@ -144,6 +153,76 @@ __global__ void __launch_bounds__(LB) k_vert_vel(DevMesh m, VVArgs a)
             a.hnode_new[(size_t)n * L + k - 1] = h;
         }
         if (k == nzmin) w = w - a.water_flux[n];
+    }
+    if (a.which_ale == FGPU_ALE_ZLEVEL && nzmin == 1) {   // uniform over the block
+        const int lz = a.lzstar;
+        sW[tid] = w;
+        if (tid < lz) sZl[tid] = h;
+        __syncthreads();
+        if (tid == 0) {
+            const double *ho = m.hnode + (size_t)n * L, *zb = a.zbar, *cprev = a.cfl + (size_t)n * NL;
+            const double dt = a.dt, mh = a.min_hnode;
+            const double dtot = a.hbar[n] - a.hbar_old[n];
+            int zmax = m.nlevn_min[n] - 1;
+            double mx[32], dist[32];
+            if (dtot < 0.0 && ho[0] + dtot <= (zb[0] - zb[1]) * mh) {
+                for (int j = 1; j <= lz; ++j) {
+                    double v = (zb[j - 1] - zb[j]) * mh - ho[j - 1];
+                    if (v >= 0.0) v = 0.0;
+                    if (cprev[j - 1] >= 0.95) v = 0.0;   // the previous step's CFL_z
+                    mx[j - 1] = v;
+                }
+                int nz = lz;   // first level whose pair sum is below the change (reference's cumsum_maxdhbar)
+                for (int j = lz; j >= 1; --j) {
+                    const double cj = j == 1 ? mx[0] : mx[j - 1] + mx[j - 2];
+                    if (cj < dtot) nz = j;
+                }
+                double rest = dtot;
+                zmax = min(nz, zmax - 1);
+                for (int j = 1; j <= zmax; ++j) {
+                    dist[j - 1] = dmx(rest, mx[j - 1]);
+                    rest = rest - dist[j - 1];
+                    rest = dmn(0.0, rest);
+                }
+                double dint = 0.0;
+                for (int j = zmax; j >= 1; --j) {
+                    dint = dint + dist[j - 1];
+                    sW[j - 1] = sW[j - 1] - dint / dt;
+                    sZl[j - 1] = ho[j - 1] + dist[j - 1];
+                }
+            } else {
+                bool thin = false;
+                for (int j = 2; j <= lz; ++j) thin = thin || ho[j - 1] != (zb[j - 1] - zb[j]);
+                if (dtot > 0.0 && thin) {
+                    int nz = 0;
+                    for (int j = 1; j <= lz; ++j) {
+                        mx[j - 1] = (zb[j - 1] - zb[j]) - ho[j - 1];
+                        if (ho[j - 1] != (zb[j - 1] - zb[j])) nz = j;
+                    }
+                    mx[0] = 1000.0;
+                    zmax = min(nz, zmax - 1);
+                    double rest = dtot, dint = 0.0;
+                    for (int j = zmax; j >= 1; --j) {
+                        const double d = dmn(rest, mx[j - 1]);
+                        rest = rest - d;
+                        rest = dmx(0.0, rest);
+                        dint = dint + d;
+                        sW[j - 1] = sW[j - 1] - dint / dt;
+                        sZl[j - 1] = ho[j - 1] + d;
+                    }
+                } else {
+                    sW[0] = sW[0] - dtot / dt;
+                    sZl[0] = ho[0] + dtot;
+                }
+            }
+            sW[0] = sW[0] - a.water_flux[n];
+        }
+        __syncthreads();
+        w = sW[tid];
+        if (tid < lz) {
+            h = sZl[tid];
+            a.hnode_new[(size_t)n * L + tid] = h;
+        }
     }
     if (lay && w != w) sBad = 1;   // NaN check of oce_ale.F90:2288-2320
     sH[tid] = h;
@@ -379,9 +458,10 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
 {
     int rc = check_dyn(c);
     if (rc) return rc;
-    if (which_ale == FGPU_ALE_ZLEVEL)
-        return fgpu_fail(FGPU_ERR_ARG, "which_ALE='zlevel' (local zstar fallback) is not provided");
-    if (which_ale != FGPU_ALE_LINFS && which_ale != FGPU_ALE_ZSTAR) return fgpu_fail(FGPU_ERR_ARG, "unknown which_ALE");
+    if (which_ale != FGPU_ALE_LINFS && which_ale != FGPU_ALE_ZLEVEL && which_ale != FGPU_ALE_ZSTAR)
+        return fgpu_fail(FGPU_ERR_ARG, "unknown which_ALE");
+    if (which_ale == FGPU_ALE_ZLEVEL && !c->ale_zbar)
+        return fgpu_fail(FGPU_ERR_STATE, "zlevel needs zbar, min_hnode and lzstar_lev (fgpu_set_ale_config)");
     if (!(dt > 0.0)) return fgpu_fail(FGPU_ERR_ARG, "dt must be > 0");
     if (c->nl > 128) return fgpu_fail(FGPU_ERR_ARG, "at most 128 interfaces per column");
     const size_t N2 = c->N2, NL = c->nl;
@@ -391,23 +471,29 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
     a.which_ale = which_ale;
     a.use_wsplit = use_wsplit ? 1 : 0;
     a.uv = c->d.uv;
-    if (which_ale == FGPU_ALE_ZSTAR) {
-        if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "zstar needs the surface water flux (fgpu_set_surface_forcing)");
+    if (which_ale != FGPU_ALE_LINFS) {   // zstar and zlevel
+        if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "zstar/zlevel need the surface water flux (fgpu_set_surface_forcing)");
         if (!hbar && !hbar_old) {   // the device's hbar, hbar_old from fgpu_compute_hbar_ale
-            if (!c->hbar_valid) return fgpu_fail(FGPU_ERR_STATE, "zstar needs hbar and hbar_old (or fgpu_compute_hbar_ale first)");
+            if (!c->hbar_valid) return fgpu_fail(FGPU_ERR_STATE, "zstar/zlevel need hbar and hbar_old (or fgpu_compute_hbar_ale first)");
         } else {
-            if (!hbar || !hbar_old) return fgpu_fail(FGPU_ERR_ARG, "zstar needs both hbar and hbar_old");
+            if (!hbar || !hbar_old) return fgpu_fail(FGPU_ERR_ARG, "zstar/zlevel need both hbar and hbar_old");
             if ((rc = upload_to(c, &c->vv_hbar, hbar, N2))) return rc;
             if ((rc = upload_to(c, &c->vv_hbar_old, hbar_old, N2))) return rc;
         }
         a.hbar = c->vv_hbar;
         a.hbar_old = c->vv_hbar_old;
         a.water_flux = c->d.water_flux;
+        a.zbar = c->ale_zbar;
+        a.min_hnode = c->ale_min_hnode;
+        a.lzstar = c->ale_lzstar;
     }
     if ((rc = ensure_const_buf(c, &c->d.w, N2 * NL))) return rc;
     if ((rc = ensure_const_buf(c, &c->d.w_e, N2 * NL))) return rc;
     if ((rc = ensure_const_buf(c, &c->d.w_i, N2 * NL))) return rc;
-    if ((rc = ensure_buf(c, &c->vv_cfl, N2 * NL))) return rc;
+    if (!c->vv_cfl) {   // CFL_z starts at zero; zlevel reads the previous step's
+        if ((rc = ensure_buf(c, &c->vv_cfl, N2 * NL))) return rc;
+        FGPU_HIP(hipMemsetAsync(c->vv_cfl, 0, sizeof(double) * N2 * NL, c->stream));
+    }
     if (!c->vv_flags) {
         void *p = nullptr;
         if ((rc = fgpu_alloc(c, &p, 64))) return rc;
@@ -431,7 +517,7 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
     hipLaunchKernelGGL(k_cfl_max, dim3(256), dim3(256), 0, c->stream, (const double *)c->vv_colmax, c->N2,
                        (unsigned long long *)c->vv_flags);
     FGPU_HIP(hipGetLastError());
-    if (which_ale == FGPU_ALE_ZSTAR && (rc = fgpu_compute_zn(c))) return rc;   // Z levels of the implicit diffusion
+    if (which_ale != FGPU_ALE_LINFS && (rc = fgpu_compute_zn(c))) return rc;   // Z levels of the implicit diffusion
     unsigned long long flags[2] = {0ull, 0ull};
     FGPU_HIP(hipMemcpyAsync(flags, c->vv_flags, 16, hipMemcpyDeviceToHost, c->stream));
     FGPU_HIP(hipStreamSynchronize(c->stream));
@@ -546,5 +632,22 @@ extern "C" int fgpu_compute_hbar_ale(fgpu_context *c, double dt, int which_ale, 
     FGPU_HIP(hipStreamSynchronize(c->stream));
     if (c->prof && (rc = fgpu_prof_flush(c))) return rc;
     c->hbar_valid = true;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_set_ale_config(fgpu_context *c, const double *zbar, double min_hnode, int lzstar_lev)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    if (!zbar) return fgpu_fail(FGPU_ERR_ARG, "null zbar");
+    if (lzstar_lev < 1 || lzstar_lev > 32 || lzstar_lev > c->nl - 1)
+        return fgpu_fail(FGPU_ERR_ARG, "lzstar_lev must be in 1..min(32, nl-1)");
+    if (!(min_hnode >= 0.0)) return fgpu_fail(FGPU_ERR_ARG, "min_hnode must be >= 0");
+    FGPU_HIP(hipSetDevice(c->device));
+    int rc = upload_to(c, &c->ale_zbar, zbar, (size_t)c->nl);
+    if (rc) return rc;
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    c->ale_min_hnode = min_hnode;
+    c->ale_lzstar = lzstar_lev;
     return FGPU_OK;
 }

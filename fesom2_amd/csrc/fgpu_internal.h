@@ -160,6 +160,9 @@ struct fgpu_context {
     double *vv_uvr = nullptr, *vv_old = nullptr, *vv_ce = nullptr, *vv_rhs = nullptr;
     double *vv_dhe = nullptr;       // dhe of update_stiff_mat_ale (elem2D)
     bool hbar_valid = false;        // vv_hbar, vv_hbar_old, vv_old (ssh_rhs_old), vv_dhe set by compute_hbar_ale
+    double *ale_zbar = nullptr;     // zlevel: mesh%zbar (nl), min_hnode and lzstar_lev (fgpu_set_ale_config)
+    double ale_min_hnode = 0.5;
+    int ale_lzstar = 4;
     double *vv_colmax = nullptr;    // per-column max of CFL_z (nod2D)
     void *vv_flags = nullptr;       // max CFL_z bits, NaN column count
 };

@@ -37,7 +37,7 @@ module fesom_gpu_c
        fgpu_ssh_nn_set_weights, fgpu_ssh_nn_get_weights, fgpu_ssh_nn_apply, fgpu_ssh_nn_loss_grad, &
        fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident, fgpu_set_uv, &
        fgpu_vert_vel_ale, fgpu_vert_vel_download, fgpu_compute_ssh_rhs_ale, fgpu_update_stiff_mat_ale, &
-       fgpu_compute_hbar_ale
+       fgpu_compute_hbar_ale, fgpu_set_ale_config
 
   interface
      type(c_ptr) function fgpu_last_error() bind(C, name="fgpu_last_error")
@@ -159,6 +159,15 @@ module fesom_gpu_c
        real(c_double), intent(in) :: dhe(*)
        real(c_double), value :: dt, g, alpha, theta
      end function fgpu_update_stiff_mat_ale
+
+     integer(c_int) function fgpu_set_ale_config(ctx, zbar, min_hnode, lzstar_lev) &
+          bind(C, name="fgpu_set_ale_config")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: zbar(*)
+       real(c_double), value :: min_hnode
+       integer(c_int), value :: lzstar_lev
+     end function fgpu_set_ale_config
 
      integer(c_int) function fgpu_compute_hbar_ale(ctx, dt, which_ale, hbar, hbar_out, hbar_old_out, &
           ssh_rhs_old_out, dhe_out) bind(C, name="fgpu_compute_hbar_ale")

@@ -48,7 +48,8 @@ module fesom_gpu_host
        init_tracers_AB, do_oce_adv_tra, diff_tracers_ale, diff_ver_part_impl_ale, solve_tracers_ale, &
        init_stiff_mat_ale, ssh_solve_preconditioner, ssh_solve_cg, solve_ssh_ale, gpu_synchronize, &
        gpu_autotune_step, gpu_ssh_nn_init, gpu_ssh_nn_train, ssh_solve_cg_learned, vert_vel_ale, &
-       gpu_vert_vel_ale, compute_ssh_rhs_ale, update_stiff_mat_ale, compute_hbar_ale
+       gpu_vert_vel_ale, compute_ssh_rhs_ale, update_stiff_mat_ale, compute_hbar_ale, &
+       gpu_set_ale_config
 
 contains
 
@@ -173,7 +174,7 @@ contains
 
   ! vert_vel_ale (oce_ale.F90:2107-2668) with compute_CFLz and compute_Wvel_split,
   ! on the device from uv; the results stay resident for the transport.
-  ! hbar, hbar_old are read under zstar only.
+  ! hbar, hbar_old are read under zstar and zlevel (zlevel also needs gpu_set_ale_config).
   subroutine gpu_vert_vel_ale(dt, which_ALE, uv, hbar, hbar_old, use_wsplit, wsplit_maxcfl, cflmax)
     real(WP), intent(in) :: dt, wsplit_maxcfl
     character(len=*), intent(in) :: which_ALE
@@ -217,6 +218,13 @@ contains
     call gpu_check(fgpu_compute_ssh_rhs_ale(ctx, uv_rhs, ssh_rhs_old, alpha, ale_code(which_ALE), cav, ssh_rhs), &
          'compute_ssh_rhs_ale')
   end subroutine compute_ssh_rhs_ale
+
+  ! which_ALE = 'zlevel' settings: mesh%zbar, min_hnode, lzstar_lev (namelist ale_def)
+  subroutine gpu_set_ale_config(zbar, min_hnode, lzstar_lev)
+    real(WP), intent(in) :: zbar(:), min_hnode
+    integer, intent(in) :: lzstar_lev
+    call gpu_check(fgpu_set_ale_config(ctx, zbar, min_hnode, int(lzstar_lev, c_int)), 'fgpu_set_ale_config')
+  end subroutine gpu_set_ale_config
 
   ! compute_hbar_ale (oce_ale.F90:1956-2091) from uv: hbar is updated in place,
   ! hbar_old, ssh_rhs_old and dhe are written, like mesh%hbar, hbar_old and

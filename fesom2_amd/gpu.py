@@ -55,6 +55,7 @@ _SIGS = {
     "fgpu_vert_vel_download": [vp, dp, dp, dp, dp, dp],
     "fgpu_compute_ssh_rhs_ale": [vp, dp, dp, C.c_double, C.c_int, C.c_int, dp],
     "fgpu_compute_hbar_ale": [vp, C.c_double, C.c_int, dp, dp, dp, dp, dp],
+    "fgpu_set_ale_config": [vp, dp, C.c_double, C.c_int],
     "fgpu_tracers_init": [vp, C.c_int],
     "fgpu_tracer_config": [vp, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
                            C.c_double, C.c_int],
@@ -253,6 +254,10 @@ class Context:
                                                float(alpha), self.ALE[which_ale], int(bool(use_cavity_fw2press)),
                                                _d(out)))
         return out
+
+    def set_ale_config(self, zbar, min_hnode=0.5, lzstar_lev=4):
+        """zlevel settings: mesh%zbar (nl), min_hnode, lzstar_lev."""
+        _check(self.L.fgpu_set_ale_config(self.h, _d(_f64(zbar)), float(min_hnode), int(lzstar_lev)))
 
     def compute_hbar_ale(self, dt, which_ale="linfs", hbar=None):
         """compute_hbar_ale; hbar None continues from the device's. Returns

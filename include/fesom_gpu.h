@@ -101,8 +101,14 @@ int fgpu_set_surface_forcing(fgpu_context *ctx, const double *heat_flux, const d
 /* ---- the steps either side of the transport -------------------------------
  * which_ALE of g_config (gen_modules_config.F90): */
 #define FGPU_ALE_LINFS 0
-#define FGPU_ALE_ZLEVEL 1 /* not provided: fgpu_vert_vel_ale rejects it */
+#define FGPU_ALE_ZLEVEL 1
 #define FGPU_ALE_ZSTAR 2
+
+/* which_ALE = 'zlevel' settings (gen_modules_config.F90:69-80) and
+ * mesh%zbar (nl), the nominal interface depths: a falling surface that would
+ * thin the surface layer below min_hnode of its nominal thickness is spread
+ * over the top lzstar_lev layers (1..32). */
+int fgpu_set_ale_config(fgpu_context *ctx, const double *zbar, double min_hnode, int lzstar_lev);
 
 /* dynamics%uv (2,nl-1,elem2D) alone, for fgpu_vert_vel_ale and
  * fgpu_compute_ssh_rhs_ale (fgpu_set_velocity also sets it) */
@@ -111,8 +117,10 @@ int fgpu_set_uv(fgpu_context *ctx, const double *uv);
 /* vert_vel_ale (oce_ale.F90:2107-2668) followed by compute_CFLz
  * (oce_ale.F90:2910-2997) and compute_Wvel_split (oce_ale.F90:3003-3049), on
  * the device from the resident uv: dynamics%w, w_e, w_i and cfl_z stay in HBM
- * for the transport (fgpu_vert_vel_download copies them out). which_ALE linfs
- * or zstar; zstar also reads hbar, hbar_old (nod2D, host; both NULL: the
+ * for the transport (fgpu_vert_vel_download copies them out). which_ALE linfs,
+ * zlevel (oce_ale.F90:2336-2420; needs fgpu_set_ale_config and reads the
+ * previous call's CFL_z, zero at first) or zstar; zlevel and zstar also read
+ * hbar, hbar_old (nod2D, host; both NULL: the
  * device's from fgpu_compute_hbar_ale) and the water flux
  * of fgpu_set_surface_forcing, and updates hnode_new and the implicit
  * diffusion's Z levels. Levels outside [ulevels_nod2D, nlevels_nod2D] of w_e

@@ -45,6 +45,8 @@
  *   33 Redi K33 below an interior interface scaled by the spacing below (diff_ver_part_impl_ale)
  *   34 hbar updated without dividing by the node's area (compute_hbar_ale)
  *   35 dhe averages two of the element's three nodes (compute_hbar_ale)
+ *   36 zlevel corrects W at a layer's top by that layer's change alone (vert_vel_ale)
+ *   37 zlevel thins the top layers down to zero instead of min_hnode (vert_vel_ale)
  */
 #include "fesom_oracle.h"
 
@@ -1467,11 +1469,13 @@ int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *v
 /* ------------------------------------------------------------------------ */
 int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale, const double *hbar,
                     const double *hbar_old, const double *water_flux, int use_wsplit, double wsplit_maxcfl,
-                    double *w, double *w_e, double *w_i, double *cfl_z, double *hnode_new, double *cflmax)
+                    double *w, double *w_e, double *w_i, double *cfl_z, double *hnode_new, double *cflmax,
+                    const double *zbar, double min_hnode, int lzstar_lev)
 {
     const int L = m->nl - 1, NL = m->nl, N2 = m->nod2D;
-    if (which_ale != 0 && which_ale != 2) return -1;
-    if (which_ale == 2 && (!hbar || !hbar_old || !water_flux)) return -2;
+    if (which_ale != 0 && which_ale != 1 && which_ale != 2) return -1;
+    if (which_ale != 0 && (!hbar || !hbar_old || !water_flux)) return -2;
+    if (which_ale == 1 && (!zbar || lzstar_lev < 1 || lzstar_lev > L)) return -3;
     /* :2164-2171 */
     memset(w, 0, sizeof(double) * (size_t)N2 * NL);
     /* thickness divergence, edges ascending (:2175-2265): each element of
@@ -1503,6 +1507,74 @@ int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale
             for (int nz = nzmax; nz >= nzmin; --nz) w[INT(n, nz)] = w[INT(n, nz)] + w[INT(n, nz + 1)];
         }
         for (int nz = nzmin; nz <= nzmax; ++nz) w[INT(n, nz)] = w[INT(n, nz)] / m->area[INT(n, nz)];
+    }
+    /* zlevel (:2336-2420): the ssh change goes into the surface layer; when
+       that would thin it below min_hnode of its nominal thickness, it is
+       spread over the top lzstar_lev layers (local zstar), skipping layers
+       with CFL_z >= 0.95 from the previous step; a rising surface first
+       restores thinned layers. Cavity columns stay linfs. */
+    if (which_ale == 1) {
+        const int lz = lzstar_lev;
+        double *mx = (double *)calloc(lz + 1, sizeof(double)), *cum = (double *)calloc(lz + 1, sizeof(double));
+        double *dist = (double *)calloc(lz + 1, sizeof(double));
+        const double mh = (g_mut == 37) ? 0.0 : min_hnode;
+        for (int n = 0; n < N2; ++n) {
+            const int nzmin = m->ulevels_nod2D[n];
+            int nzmax = m->nlevels_nod2D_min[n] - 1;
+            if (nzmin != 1) continue;
+#define HO(k) m->hnode[LAY(n, (k))]
+            const double dtot = hbar[n] - hbar_old[n];
+            if (dtot < 0.0 && HO(nzmin) + dtot <= (zbar[nzmin - 1] - zbar[nzmin]) * mh) {
+                for (int j = 1; j <= lz; ++j) mx[j] = (zbar[nzmin + j - 2] - zbar[nzmin + j - 1]) * mh - HO(nzmin + j - 1);
+                for (int j = 1; j <= lz; ++j) if (mx[j] >= 0.0) mx[j] = 0.0;
+                for (int j = 1; j <= lz; ++j) if (cfl_z[INT(n, nzmin + j - 1)] >= 0.95) mx[j] = 0.0;
+                cum[1] = mx[1];
+                for (int j = 2; j <= lz; ++j) cum[j] = mx[j] + mx[j - 1];
+                int nz = lz;   /* minval of an empty pack is huge(); min with lzstar_lev */
+                for (int j = lz; j >= 1; --j) if (cum[j] < dtot) nz = j;
+                for (int j = 1; j <= lz; ++j) dist[j] = 0.0;
+                double rest = dtot;
+                nzmax = imin(nz, nzmax - 1);
+                for (int j = 1; j <= nzmax; ++j) {
+                    dist[j] = dmax(rest, mx[j]);
+                    rest = rest - dist[j];
+                    rest = dmin(0.0, rest);
+                }
+                double dint = 0.0;
+                for (int j = nzmax; j >= 1; --j) {
+                    dint = dint + dist[j];
+                    w[INT(n, nzmin + j - 1)] = w[INT(n, nzmin + j - 1)] - ((g_mut == 36) ? dist[j] : dint) / dt;
+                    hnode_new[LAY(n, nzmin + j - 1)] = HO(nzmin + j - 1) + dist[j];
+                }
+            } else {
+                int thin = 0;
+                for (int j = 2; j <= lz; ++j) thin |= HO(nzmin + j - 1) != (zbar[nzmin + j - 2] - zbar[nzmin + j - 1]);
+                if (dtot > 0.0 && thin) {
+                    for (int j = 1; j <= lz; ++j) mx[j] = (zbar[nzmin + j - 2] - zbar[nzmin + j - 1]) - HO(nzmin + j - 1);
+                    mx[1] = 1000.0;
+                    int nz = 0;
+                    for (int j = 1; j <= lz; ++j)
+                        if (HO(nzmin + j - 1) != (zbar[nzmin + j - 2] - zbar[nzmin + j - 1])) nz = j;
+                    nzmax = imin(nz, nzmax - 1);
+                    double rest = dtot, dint = 0.0;
+                    for (int j = 1; j <= lz; ++j) dist[j] = 0.0;
+                    for (int j = nzmax; j >= 1; --j) {
+                        dist[j] = dmin(rest, mx[j]);
+                        rest = rest - dist[j];
+                        rest = dmax(0.0, rest);
+                        dint = dint + dist[j];
+                        w[INT(n, nzmin + j - 1)] = w[INT(n, nzmin + j - 1)] - ((g_mut == 36) ? dist[j] : dint) / dt;
+                        hnode_new[LAY(n, nzmin + j - 1)] = HO(nzmin + j - 1) + dist[j];
+                    }
+                } else {
+                    w[INT(n, nzmin)] = w[INT(n, nzmin)] - dtot / dt;
+                    hnode_new[LAY(n, nzmin)] = HO(nzmin) + dtot;
+                }
+            }
+            w[INT(n, nzmin)] = w[INT(n, nzmin)] - water_flux[n];
+#undef HO
+        }
+        free(mx); free(cum); free(dist);
     }
     /* zstar: distribute hbar - hbar_old over the layers above
        nlevels_nod2D_min (:2539-2600); cavity columns stay linfs */

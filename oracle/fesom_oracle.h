@@ -127,11 +127,15 @@ int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *v
                     const double *pr_values, double *x, const double *rhs, double soltol,
                     int maxiter, fo_cg_stats *st);
 
-/* vert_vel_ale for which_ale 0 (linfs) or 2 (zstar), with compute_CFLz and
- * compute_Wvel_split; hnode_new is updated in place under zstar */
+/* vert_vel_ale for which_ale 0 (linfs), 1 (zlevel) or 2 (zstar), with
+ * compute_CFLz and compute_Wvel_split; hnode_new is updated in place under
+ * zlevel and zstar. cfl_z is in/out: zlevel reads the previous step's CFL_z.
+ * zbar (nl) are the reference interface depths (mesh%zbar), used with
+ * min_hnode and lzstar_lev by zlevel only. */
 int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale, const double *hbar,
                     const double *hbar_old, const double *water_flux, int use_wsplit, double wsplit_maxcfl,
-                    double *w, double *w_e, double *w_i, double *cfl_z, double *hnode_new, double *cflmax);
+                    double *w, double *w_e, double *w_i, double *cfl_z, double *hnode_new, double *cflmax,
+                    const double *zbar, double min_hnode, int lzstar_lev);
 /* compute_hbar_ale: ssh_rhs_old = column transport divergence (minus the
  * freshwater flux off linfs), hbar_old = hbar, hbar += ssh_rhs_old dt/areasvol,
  * dhe per element; hbar is updated in place */

@@ -109,7 +109,7 @@ def lib():
                                            C.POINTER(FoTracer), C.c_int, C.POINTER(FoWork)]
         L.fo_init_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, ip, ip, dp, C.c_int]
         L.fo_vert_vel_ale.argtypes = [C.POINTER(FoMesh), dp, C.c_double, C.c_int, dp, dp, dp, C.c_int,
-                                      C.c_double, dp, dp, dp, dp, dp, dp]
+                                      C.c_double, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int]
         L.fo_update_stiff_mat_ale.argtypes = [C.POINTER(FoMesh), C.c_double, dp, ip, ip, dp]
         L.fo_compute_ssh_rhs_ale.argtypes = [C.POINTER(FoMesh), dp, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp]
         L.fo_compute_hbar_ale.argtypes = [C.POINTER(FoMesh), dp, dp, C.c_double, C.c_int, dp, dp, dp, dp]
@@ -305,11 +305,16 @@ class Oracle:
 
     # ------------------------------------------------- dynamics either side
     def vert_vel_ale(self, uv, dt, which_ale=0, hbar=None, hbar_old=None, water_flux=None,
-                     use_wsplit=False, wsplit_maxcfl=1.0, hnode_new=None):
+                     use_wsplit=False, wsplit_maxcfl=1.0, hnode_new=None, cfl_prev=None, min_hnode=0.5,
+                     lzstar_lev=4):
         """vert_vel_ale + compute_CFLz + compute_Wvel_split; returns a dict of
-        w, w_e, w_i, cfl_z (nod2D x nl), hnode_new (nod2D x nl-1) and cflmax."""
+        w, w_e, w_i, cfl_z (nod2D x nl), hnode_new (nod2D x nl-1) and cflmax.
+        zlevel (which_ale 1) reads cfl_prev, the previous CFL_z (default 0)."""
         m = self.mesh
         out = {k: np.zeros((m.nod2D, m.nl)) for k in ("w", "w_e", "w_i", "cfl_z")}
+        if cfl_prev is not None:
+            out["cfl_z"][:] = cfl_prev
+        zb = np.ascontiguousarray(m.zbar, dtype=np.float64)
         hn = np.array(m.hnode_new if hnode_new is None else hnode_new, dtype=np.float64, order="C", copy=True)
         cmax = C.c_double(0.0)
         opt = lambda a: None if a is None else _d(np.ascontiguousarray(a, dtype=np.float64))
@@ -317,7 +322,8 @@ class Oracle:
         rc = lib().fo_vert_vel_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)), float(dt), int(which_ale),
                                    opt(keep[0]), opt(keep[1]), opt(keep[2]), int(bool(use_wsplit)),
                                    float(wsplit_maxcfl), _d(out["w"]), _d(out["w_e"]), _d(out["w_i"]),
-                                   _d(out["cfl_z"]), _d(hn), C.byref(cmax))
+                                   _d(out["cfl_z"]), _d(hn), C.byref(cmax), _d(zb), float(min_hnode),
+                                   int(lzstar_lev))
         if rc:
             raise RuntimeError(f"oracle vert_vel_ale failed rc={rc}")
         out["hnode_new"] = hn

@@ -343,3 +343,49 @@ def test_hbar_feeds_the_next_steps_on_the_device(c0, c0_oracle):
                                  water_flux=wf)
     _compare_vv(dev, ref)
     np.testing.assert_array_equal(dev["hnode_new"], ref["hnode_new"])
+
+
+def test_vert_vel_zlevel_bitwise(c0, c0_oracle):
+    """zlevel vert_vel_ale: small changes in the surface layer, large falls
+    spread over the top layers (with the previous step's CFL_z guard), rises
+    refilling thinned layers; W, its split, CFL_z and hnode_new equal the
+    oracle's bit for bit."""
+    import dataclasses
+    from fesom2_amd import gpu
+    from oracle.oracle import Oracle
+    mesh, state, _ = c0
+    rng = np.random.default_rng(17)
+    hn = mesh.hnode.copy()
+    thin = rng.random(mesh.nod2D) < 0.3
+    hn[thin, 1] *= 0.6
+    hn[thin, 2] *= 0.7
+    m2 = dataclasses.replace(mesh, hnode=hn, hnode_new=hn.copy())
+    o = Oracle(m2)
+    h1 = hn[:, 0]
+    kind = rng.integers(0, 3, mesh.nod2D)
+    dh = np.where(kind == 0, rng.uniform(-0.1, 0.1, mesh.nod2D) * h1,
+                  np.where(kind == 1, -rng.uniform(0.6, 0.9, mesh.nod2D) * h1, rng.uniform(0.05, 0.4, mesh.nod2D) * h1))
+    hbar_old = rng.normal(0.0, 0.3, mesh.nod2D)
+    hbar = hbar_old + dh
+    wf = rng.normal(0.0, 1e-6, mesh.nod2D)
+    fast = rng.normal(0.0, 1.0, state.uv.shape)   # the step before: CFL_z above 0.95 at some interfaces
+    prev = o.vert_vel_ale(fast, state.dt)
+    assert (prev["cfl_z"][:, :4] >= 0.95).any() and (prev["cfl_z"][:, :4] < 0.95).any()
+    ref = o.vert_vel_ale(state.uv, state.dt, which_ale=1, hbar=hbar, hbar_old=hbar_old, water_flux=wf,
+                         cfl_prev=prev["cfl_z"], min_hnode=0.5, lzstar_lev=4)
+    print("[gpu] vert_vel_ale zlevel c0", flush=True)
+    ctx = gpu.context_for(m2, state)
+    try:
+        z = np.zeros(mesh.nod2D)
+        ctx.set_surface_forcing(z, wf, z, z, z)
+        ctx.set_ale_config(mesh.zbar, 0.5, 4)
+        ctx.set_uv(fast)
+        ctx.vert_vel_ale(state.dt)
+        ctx.set_uv(state.uv)
+        ctx.vert_vel_ale(state.dt, which_ale="zlevel", hbar=hbar, hbar_old=hbar_old)
+        dev = ctx.vert_vel_download()
+    finally:
+        ctx.close()
+    _compare_vv(dev, ref)
+    np.testing.assert_array_equal(dev["hnode_new"], ref["hnode_new"])
+    assert np.any(ref["hnode_new"][:, 1] != hn[:, 1])

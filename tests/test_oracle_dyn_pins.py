@@ -324,10 +324,69 @@ def test_hbar_freshwater(bumps):
     assert check_hbar_freshwater(*bumps)
 
 
-def test_zlevel_is_rejected(bumps):
-    mesh, st, o = bumps
-    with pytest.raises(RuntimeError):
-        o.vert_vel_ale(st.uv, st.dt, which_ale=1)
+def _zlevel(mesh, st, dh, hnode=None, cfl_prev=None, min_hnode=0.5, lz=4):
+    """zlevel vert_vel_ale at rest (uv = 0, no water flux) for a surface change dh."""
+    import dataclasses
+    m2 = mesh if hnode is None else dataclasses.replace(mesh, hnode=hnode, hnode_new=hnode.copy())
+    o = Oracle(m2)
+    z = np.zeros(mesh.nod2D)
+    r = o.vert_vel_ale(np.zeros_like(st.uv), st.dt, which_ale=1, hbar=dh, hbar_old=z, water_flux=z,
+                       cfl_prev=cfl_prev, min_hnode=min_hnode, lzstar_lev=lz)
+    return m2, r
+
+
+def _zlevel_laws(m2, st, r, dh):
+    """Continuity (each layer's change = -(W top - W bottom) dt) and
+    conservation (the layer changes add up to dh) at every open-ocean node."""
+    dthk = r["hnode_new"] - m2.hnode
+    w = r["w"]
+    div = -(w[:, :-1] - w[:, 1:]) * st.dt
+    scale = np.abs(dh).max()
+    ok = np.allclose(dthk, div, rtol=0, atol=1e-12 * scale)
+    ok &= np.allclose(dthk.sum(1), dh, rtol=0, atol=1e-12 * scale)
+    return bool(ok)
+
+
+def check_zlevel(mesh, st, o):
+    """zlevel: a small change goes into the surface layer; a fall that would
+    thin it below min_hnode of its nominal thickness is spread over the top
+    layers, none thinner than min_hnode nominal; layers whose previous CFL_z
+    reached 0.95 keep their thickness; a rise first refills thinned layers,
+    none above nominal. Continuity and conservation hold in every case."""
+    nominal = mesh.zbar[:-1] - mesh.zbar[1:]
+    h1 = mesh.hnode[:, 0]
+    ok = True
+    # small change: surface layer only
+    small = np.linspace(-0.1, 0.1, mesh.nod2D) * h1
+    m2, r = _zlevel(mesh, st, small)
+    ok &= _zlevel_laws(m2, st, r, small)
+    ok &= np.allclose(r["hnode_new"][:, 0] - mesh.hnode[:, 0], small, rtol=1e-13, atol=0)
+    # large fall: spread, never below min_hnode of nominal
+    big = -0.8 * h1
+    m2, r = _zlevel(mesh, st, big)
+    ok &= _zlevel_laws(m2, st, r, big)
+    ok &= bool(np.all(r["hnode_new"][:, :4] >= 0.5 * nominal[None, :4] * (1 - 1e-12)))
+    ok &= bool(np.all(r["hnode_new"][:, 1] < mesh.hnode[:, 1]))
+    # previous CFL_z at the surface interface >= 0.95: the surface layer stays
+    cfl = np.zeros((mesh.nod2D, mesh.nl))
+    cfl[:, 0] = 0.97
+    m2, r = _zlevel(mesh, st, big, cfl_prev=cfl)
+    ok &= _zlevel_laws(m2, st, r, big)
+    ok &= bool(np.all(r["hnode_new"][:, 0] == mesh.hnode[:, 0]))
+    # rise over thinned layers 2 and 3: refilled from below, none above nominal
+    hn = mesh.hnode.copy()
+    hn[:, 1] = 0.6 * nominal[1]
+    hn[:, 2] = 0.6 * nominal[2]
+    rise = 0.3 * nominal[0] * np.ones(mesh.nod2D)
+    m2, r = _zlevel(mesh, st, rise, hnode=hn)
+    ok &= _zlevel_laws(m2, st, r, rise)
+    ok &= bool(np.all(r["hnode_new"][:, 1:4] <= nominal[None, 1:4] * (1 + 1e-12)))
+    ok &= bool(np.all(r["hnode_new"][:, 2] > hn[:, 2]))
+    return bool(ok)
+
+
+def test_zlevel_redistribution(flat):
+    assert check_zlevel(*flat)
 
 
 def test_generator_w_agrees(bumps):
@@ -347,6 +406,8 @@ MUTATIONS = {
     23: lambda b, f: check_update_stiff(*b),
     34: lambda b, f: check_hbar_transport(*b),
     35: lambda b, f: check_hbar_freshwater(*b),
+    36: lambda b, f: check_zlevel(*f),
+    37: lambda b, f: check_zlevel(*f),
 }
 
 
