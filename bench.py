@@ -119,12 +119,19 @@ def main():
     ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ssh-reps", type=int, default=5)
+    ap.add_argument("--shape", default=None,
+                    help="TPB,TCH,CPB: fixed launch shape of the fused step instead of the autotuner's "
+                         "(a profiled run then holds only that shape's launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is design-only "
                                    "(halo exchange over RCCL is not built)", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.shape:
+        tpb_s, tch_s, cpb_s = (int(x) for x in args.shape.split(","))
+        os.environ["FGPU_STEP_TPB"], os.environ["FGPU_STEP_TCH"], os.environ["FGPU_STEP_CPB"] = (
+            str(tpb_s), str(tch_s), str(cpb_s))
     from fesom2_amd import gpu, meshgen as mg
     t0 = time.perf_counter()
     mesh, state, _ = mg.make_config(args.config)
@@ -133,10 +140,15 @@ def main():
           f"N3={mesh.n3} ntr={state.ntr} (generated in {t_gen:.1f}s)", file=sys.stderr, flush=True)
     ctx = gpu.context_for(mesh, state)
     dt = state.dt
-    tpb, tune_ms = ctx.autotune_step(dt, 3)
-    shape = ctx.step_shape()
-    print(f"[bench] autotuner: {shape[0]} tracers per block, {shape[1]} per Thomas chunk, {shape[2]} columns per "
-          f"block ({tune_ms:.3f} ms/step)", file=sys.stderr, flush=True)
+    if args.shape:
+        shape = ctx.step_shape()
+        print(f"[bench] fixed shape: {shape[0]} tracers per block, {shape[1]} per Thomas chunk, {shape[2]} "
+              f"columns per block", file=sys.stderr, flush=True)
+    else:
+        tpb, tune_ms = ctx.autotune_step(dt, 3)
+        shape = ctx.step_shape()
+        print(f"[bench] autotuner: {shape[0]} tracers per block, {shape[1]} per Thomas chunk, {shape[2]} columns "
+              f"per block ({tune_ms:.3f} ms/step)", file=sys.stderr, flush=True)
     ctx.solve_tracers_ale(dt, max(1, args.warmup))
     ctx.synchronize()
     t0 = time.perf_counter()

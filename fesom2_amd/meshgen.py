@@ -265,7 +265,7 @@ def build_mesh(*, nx: int = 60, ny: int = 50, dx: float = 20e3,
                nlayers: int = 24, hmax: float = 4000.0, dz0: Optional[float] = None,
                min_layers: int = 10, max_layers: Optional[int] = None,
                topo: str = "bumps", bump_height: float = 1500.0,
-               open_neighbour_sweep: bool = True) -> Mesh:
+               open_neighbour_sweep: bool = True, cavity: Optional[dict] = None) -> Mesh:
     """Jittered periodic band triangulation plus levels and geometry.
 
     ``cartesian=True``: nx x ny rows, spacing dx (metres), walls at the first
@@ -273,6 +273,10 @@ def build_mesh(*, nx: int = 60, ny: int = 50, dx: float = 20e3,
     ``cartesian=False``: latitude band, rows of cos(lat)-scaled length, spacing
     dx at the equator; with ``target_nodes`` the spacing is tuned and the last
     row lengths adjusted so that exactly ``target_nodes`` nodes result.
+    ``cavity=dict(frac=f, draft=d)`` puts an ice shelf over the southern
+    fraction f of the band: its draft falls from d metres at the wall to zero
+    at the front, with a seeded wave along x so that the first wet level
+    varies along the front as well.
     """
     rng = np.random.default_rng(seed)
     if cartesian:
@@ -350,9 +354,18 @@ def build_mesh(*, nx: int = 60, ny: int = 50, dx: float = 20e3,
         f = (f - f.min()) / max(f.max() - f.min(), 1e-30)
         depth = 30.0 + (hmax - 30.0) * f ** 0.6
     depth = -depth   # negative below sea level (aux3d.out convention)
+    cavity_depth = None
+    if cavity is not None:
+        frac = float(cavity.get("frac", 0.3))
+        dmax = float(cavity.get("draft", 600.0))
+        yr = (coord[:, 1] - y0) / max(Ly, 1e-30)
+        ph = rng.uniform(0.0, 2.0 * PI)
+        front = frac * (1.0 + 0.3 * np.sin(2.0 * PI * coord[:, 0] / L * 3.0 + ph))
+        cavity_depth = -dmax * np.clip(1.0 - yr / front, 0.0, 1.0)
 
     return assemble_mesh(coord, elem, L=L, cartesian=cartesian, zbar=zbar, depth=depth, min_layers=min_layers,
-                         max_layers=max_layers, open_neighbour_sweep=open_neighbour_sweep)
+                         max_layers=max_layers, open_neighbour_sweep=open_neighbour_sweep,
+                         cavity_depth=cavity_depth)
 
 
 def build_edges(coord, elem, L):
@@ -412,13 +425,16 @@ def build_edges(coord, elem, L):
 
 
 def assemble_mesh(coord, elem, *, L, cartesian, zbar, depth, min_layers=10, max_layers=None,
-                  open_neighbour_sweep=True, edges=None, edge_tri=None, nlevels=None) -> Mesh:
+                  open_neighbour_sweep=True, edges=None, edge_tri=None, nlevels=None,
+                  cavity_depth=None) -> Mesh:
     """Connectivity, levels and geometry of a triangulation.
 
     coord (N2, 2) in radians, elem (F2, 3) 0-based, L the cyclic length in
     radians, zbar (nl,) the interfaces, depth (N2,) negative below sea level.
     edges/edge_tri (0-based, -1 = none) and nlevels (1-based per element) may be
     given, as the reference's mesh files hold them; otherwise they are built.
+    cavity_depth (N2,), negative under an ice shelf and 0 elsewhere, sets the
+    first wet level of each element (see ``cavity_levels``).
     """
     coord = np.ascontiguousarray(coord, dtype=np.float64)
     elem = np.array(elem, dtype=np.int64)
@@ -504,15 +520,19 @@ def assemble_mesh(coord, elem, *, L, cartesian, zbar, depth, min_layers=10, max_
         nlevels = nlev.astype(np.int32)
     else:
         nlevels = np.ascontiguousarray(nlevels, dtype=np.int32)
-    ulevels = np.ones(F2, np.int32)
     nln = np.zeros(N2, np.int64)
     np.maximum.at(nln, elem.reshape(-1), np.repeat(nlevels, 3))
     nlevels_nod2D = nln.astype(np.int32)
-    ulevels_nod2D = np.ones(N2, np.int32)
     nlmin = np.full(N2, 10 ** 9, np.int64)
     np.minimum.at(nlmin, elem.reshape(-1), np.repeat(nlevels, 3))
     nlevels_nod2D_min = nlmin.astype(np.int32)
-    ulevels_nod2D_max = np.ones(N2, np.int32)
+    if cavity_depth is None:
+        ulevels = np.ones(F2, np.int32)
+        ulevels_nod2D = np.ones(N2, np.int32)
+        ulevels_nod2D_max = np.ones(N2, np.int32)
+    else:
+        ulevels, ulevels_nod2D, ulevels_nod2D_max = cavity_levels(
+            elem, elem_nb, nie, zbar, nlevels, nlevels_nod2D, np.asarray(cavity_depth, np.float64))
 
     # --- horizontal geometry ------------------------------------------------
     # mesh_areas (oce_mesh.F90:2223-2236): areas in rad^2, scaled later
@@ -578,9 +598,11 @@ def assemble_mesh(coord, elem, *, L, cartesian, zbar, depth, min_layers=10, max_
     zbar_3d_n = np.zeros((N2, nl))
     wet_if = lvl[None, :] <= nlevels_nod2D[:, None]
     zbar_3d_n[wet_if] = np.broadcast_to(zbar[None, :], (N2, nl))[wet_if]
+    # oce_ale.F90:340-373: Z and zbar above the cavity too, zero below the bottom
     Z_3d_n = np.zeros((N2, nl - 1))
     wet_l = wet_nodes[:, : nl - 1]
-    Z_3d_n[wet_l] = np.broadcast_to(Zmid[None, :], (N2, nl - 1))[wet_l]
+    to_bot = lvl[None, : nl - 1] <= (nlevels_nod2D - 1)[:, None]
+    Z_3d_n[to_bot] = np.broadcast_to(Zmid[None, :], (N2, nl - 1))[to_bot]
     dzl = zbar[:-1] - zbar[1:]
     hnode = np.where(wet_l, dzl[None, :], 0.0)
     lvl_l = np.arange(1, nl)
@@ -601,6 +623,76 @@ def assemble_mesh(coord, elem, *, L, cartesian, zbar, depth, min_layers=10, max_
         zbar_e_bot=zbar_e_bot, zbar_e_srf=zbar_e_srf, zbar_n_bot=zbar_n_bot,
         zbar_3d_n=zbar_3d_n, Z_3d_n=Z_3d_n, hnode=hnode, hnode_new=hnode.copy(),
         helem=helem)
+
+
+def cavity_levels(elem, elem_nb, nie, zbar, nlevels, nlevels_nod2D, cavity_depth):
+    """First wet level under an ice shelf: ulevels, ulevels_nod2D, ulevels_nod2D_max.
+
+    Per element (fvom_init.F90:1062-1091): the mean cavity depth of the three
+    nodes; ulevels = 2 under any draft, else 1, and the first mid level deeper
+    than the draft or with at most three levels left above the bottom wins.
+    Then, level by level, a wet prism with fewer than two wet neighbours is
+    pushed one level down (fvom_init.F90:1127-1220, swept in parallel instead
+    of in element order; the branch that would leave fewer than three layers
+    is never taken here and raises instead). A node layer between the deepest
+    top and the shallowest bottom of its elements that fewer than two elements
+    cover opens the nearest covering tops (fvom_init.F90:1312-1408).
+    ulevels_nod2D is the minimum over the node's elements (:1230-1240),
+    ulevels_nod2D_max the maximum (oce_mesh.F90:1675-1680); the checks of
+    fvom_init.F90:1243-1274 raise.
+    """
+    F2 = elem.shape[0]
+    N2 = nie.shape[0]
+    nl = len(zbar)
+    Zmid = 0.5 * (zbar[:-1] + zbar[1:])
+    dmean = (cavity_depth[elem[:, 0]] + cavity_depth[elem[:, 1]] + cavity_depth[elem[:, 2]]) / 3.0
+    ul = np.where(dmean < 0.0, 2, 1).astype(np.int64)
+    for e in range(F2):
+        for nz in range(1, int(nlevels[e])):
+            if Zmid[nz - 1] < dmean[e] or nlevels[e] - nz <= 3:
+                ul[e] = nz
+                break
+    nlev = nlevels.astype(np.int64)
+    for _outer in range(10):
+        for nz in range(1, int(ul.max()) + 1):
+            for _ in range(1000):
+                ok = elem_nb >= 0
+                nb = np.maximum(elem_nb, 0)
+                nb_open = ok & (ul[nb] <= nz) & (nlev[nb] > nz)
+                wet = (nz >= ul) & (nz < nlev)
+                iso = wet & (nb_open.sum(axis=1) <= 1)
+                if not iso.any():
+                    break
+                if np.any(iso & (nlev - (nz + 1) < 3)):
+                    raise ValueError("cavity: an isolated prism would keep fewer than three layers")
+                ul = np.where(iso, nz + 1, ul)
+        changed = False
+        for n in range(N2):
+            els = nie[n][nie[n] >= 0]
+            if nlev[els].min() >= ul[els].max():
+                continue
+            cnt = np.zeros(nl + 1, np.int64)
+            for e in els:
+                cnt[ul[e]:nlev[e]] += 1
+            for nz in range(int(ul[els].min()), int(nlev[els].max())):
+                if cnt[nz] <= 1:
+                    above = [(int(ul[e]) - nz, j) for j, e in enumerate(els) if ul[e] > nz]
+                    above.sort()
+                    for _, j in above[: 2 - int(cnt[nz])]:
+                        ul[els[j]] = nz
+                    changed = True
+                    break
+        if not changed:
+            break
+    else:
+        raise ValueError("cavity: levels did not converge")
+    uln = np.full(N2, nl, np.int64)
+    np.minimum.at(uln, elem.reshape(-1), np.repeat(ul, 3))
+    ulx = np.zeros(N2, np.int64)
+    np.maximum.at(ulx, elem.reshape(-1), np.repeat(ul, 3))
+    if np.any(nlev - ul < 3) or np.any(nlevels_nod2D - uln < 3):
+        raise ValueError("cavity: fewer than three wet layers")
+    return ul.astype(np.int32), uln.astype(np.int32), ulx.astype(np.int32)
 
 
 # ----------------------------------------------------------------------------
@@ -745,6 +837,12 @@ CONFIGS: Dict[str, dict] = {
                          nlayers=24, hmax=4000.0, min_layers=10, max_layers=24,
                          topo="bumps", bump_height=1500.0),
                state=dict(ntr=2, dt=900.0, kv=1e-5), steps=10),
+    # c0 under an ice shelf over the southern third: first wet levels up to 5
+    "c0cav": dict(mesh=dict(nx=60, ny=50, dx=20e3, jitter=0.2, seed=0, cartesian=True,
+                            nlayers=24, hmax=4000.0, min_layers=10, max_layers=24,
+                            topo="bumps", bump_height=1500.0,
+                            cavity=dict(frac=0.33, draft=800.0)),
+                  state=dict(ntr=2, dt=900.0, kv=1e-5), steps=10),
     # 126,858-node latitude band, 47 layers (PI-like stretching), T/S, dt=1800 s
     "c1": dict(mesh=dict(cartesian=False, target_nodes=126858, jitter=0.2, seed=1,
                          nlayers=47, hmax=6250.0, dz0=10.0, min_layers=5,

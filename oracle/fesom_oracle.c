@@ -47,6 +47,10 @@
  *   35 dhe averages two of the element's three nodes (compute_hbar_ale)
  *   36 zlevel corrects W at a layer's top by that layer's change alone (vert_vel_ale)
  *   37 zlevel thins the top layers down to zero instead of min_hnode (vert_vel_ale)
+ *   38 SSH diagonal gets the mass term under an ice-shelf cavity too (init_stiff_mat_ale)
+ *   39 zstar stretches the cavity columns as well                (vert_vel_ale)
+ *   40 ssh_rhs_old enters the full free surface RHS at cavity nodes (compute_ssh_rhs_ale)
+ *   41 surface flux added to layer 1 instead of the first wet layer (diff_ver_part_impl_ale)
  */
 #include "fesom_oracle.h"
 
@@ -1271,7 +1275,8 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
 
         double bc;
         if (fo_bc_surface(t->ID, dt, d, n, T[I(nzmin)], &bc) != 0) { rc = -1; break; }
-        tr[I(nzmin)] = tr[I(nzmin)] + bc;
+        if (g_mut == 41) tr[I(1)] = tr[I(1)] + bc;
+        else tr[I(nzmin)] = tr[I(nzmin)] + bc;
 
         cp[I(nzmin)] = c[I(nzmin)] / b[I(nzmin)];
         tp[I(nzmin)] = tr[I(nzmin)] / b[I(nzmin)];
@@ -1370,7 +1375,7 @@ int fo_init_stiff_mat_ale(const fo_mesh *m, double dt, int *rowptr, int *colind,
         }
     }
     for (int row = 0; row < N2; ++row) {
-        if (m->ulevels_nod2D[row] > 1) continue;
+        if (m->ulevels_nod2D[row] > 1 && g_mut != 38) continue;
         int off = rowptr[row];
         double av = m->areasvol[(size_t)row * m->nl + (m->ulevels_nod2D[row] - 1)];
         if (g_mut == 14) values[off] = values[off] + av * dt;
@@ -1461,9 +1466,9 @@ int fo_ssh_solve_cg(int n, const int *rowptr, const int *colind, const double *v
 }
 
 /* ------------------------------------------------------------------------ */
-/* vert_vel_ale: oce_ale.F90:2107-2668, which_ALE = 'linfs' (0) or 'zstar'  */
-/* (2); then compute_CFLz (oce_ale.F90:2910-2997) and compute_Wvel_split    */
-/* (oce_ale.F90:3003-3049). 'zlevel' (1) is not restated.                   */
+/* vert_vel_ale: oce_ale.F90:2107-2668, which_ALE = 'linfs' (0), 'zlevel'  */
+/* (1) or 'zstar' (2); then compute_CFLz (oce_ale.F90:2910-2997) and        */
+/* compute_Wvel_split (oce_ale.F90:3003-3049).                              */
 /* Outputs (N2*NL): w, w_e, w_i, cfl_z. hnode_new (N2*L) holds the current  */
 /* thicknesses on entry and is updated in place under zstar.               */
 /* ------------------------------------------------------------------------ */
@@ -1581,7 +1586,7 @@ int fo_vert_vel_ale(const fo_mesh *m, const double *uv, double dt, int which_ale
     if (which_ale == 2) {
         for (int n = 0; n < N2; ++n) {
             const int nzmin = m->ulevels_nod2D[n], nzmax = m->nlevels_nod2D_min[n] - 1;
-            if (nzmin != 1) continue;
+            if (nzmin != 1 && g_mut != 39) continue;
             const double dd1 = m->zbar_3d_n[INT(n, nzmax)];
             double dd = m->zbar_3d_n[INT(n, nzmin)] - dd1;
             dd = (hbar[n] - hbar_old[n]) / dd;
@@ -1721,6 +1726,7 @@ int fo_compute_ssh_rhs_ale(const fo_mesh *m, const double *uv, const double *uv_
             const int nzmin = m->ulevels_nod2D[n];
             if (nzmin > 1) {
                 if (use_cavity_fw2press) ssh_rhs[n] = ssh_rhs[n] - alpha * water_flux[n] * m->areasvol[INT(n, nzmin)];
+                if (g_mut == 40) ssh_rhs[n] = ssh_rhs[n] + (1.0 - alpha) * ssh_rhs_old[n];
             } else {
                 ssh_rhs[n] = ssh_rhs[n] - alpha * water_flux[n] * m->areasvol[INT(n, nzmin)] + (1.0 - alpha) * ssh_rhs_old[n];
             }

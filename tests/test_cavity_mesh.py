@@ -1,0 +1,173 @@
+"""Ice-shelf cavities (no GPU): the generated cavity levels follow the
+reference's rules, and the oracle's pins hold on a mesh whose first wet level
+varies from column to column.
+
+Under a cavity the surface is a rigid ice-shelf lid (oce_ale.F90): no mass
+term on the SSH diagonal (:1579-1581), no ssh_rhs_old or freshwater term in
+the free-surface RHS unless use_cavity_fw2press (:1913-1926), hbar and dhe
+kept (compute_hbar_ale), and the zstar / zlevel stretching skips the column
+(:2356-2361, :2544-2553). Surface fluxes enter the first wet layer. The pins of
+the open-ocean meshes are reused with those rules; four planted mistakes in
+the cavity branches must each fail one of them.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from fesom2_amd import meshgen as mg
+from oracle import oracle as orc
+from oracle.oracle import Oracle
+from tests import test_oracle_diff_pins as F
+from tests import test_oracle_dyn_pins as D
+from tests import test_oracle_pins as P
+
+
+@pytest.fixture(scope="module")
+def cav():
+    mesh = mg.build_mesh(nx=20, ny=14, dx=20e3, jitter=0.2, seed=5, cartesian=True, nlayers=10,
+                         hmax=2000.0, min_layers=4, max_layers=10, topo="bumps", bump_height=900.0,
+                         cavity=dict(frac=0.4, draft=600.0))
+    st = mg.build_state(mesh, seed=5, ntr=2, dt=900.0, dx=20e3)
+    assert mesh.ulevels.max() >= 3 and np.any(mesh.ulevels_nod2D != mesh.ulevels_nod2D_max)
+    return mesh, st
+
+
+def _level_rules(mesh):
+    """fvom_init.F90:1127-1274: at least three wet layers per element and
+    node, node levels the min / max over their elements, every wet prism at a
+    cavity level with two wet neighbours, every node layer covered, twice
+    where the node's elements leave a gap."""
+    ul, nl_ = mesh.ulevels.astype(int), mesh.nlevels.astype(int)
+    en = mesh.elem2D_nodes
+    ok = bool(np.all(ul >= 1) and np.all(nl_ - ul >= 3))
+    ok &= bool(np.all(mesh.nlevels_nod2D - mesh.ulevels_nod2D >= 3))
+    umin = np.full(mesh.nod2D, 10 ** 6)
+    umax = np.zeros(mesh.nod2D, int)
+    np.minimum.at(umin, en.ravel(), np.repeat(ul, 3))
+    np.maximum.at(umax, en.ravel(), np.repeat(ul, 3))
+    ok &= np.array_equal(umin, mesh.ulevels_nod2D) and np.array_equal(umax, mesh.ulevels_nod2D_max)
+    nb = mesh.elem_neighbors
+    for nz in range(1, int(ul.max()) + 1):
+        wet = (ul <= nz) & (nz < nl_)
+        nbw = (nb >= 0) & wet[np.maximum(nb, 0)]
+        ok &= bool(np.all(nbw.sum(1)[wet] >= 2))
+    for n in range(mesh.nod2D):
+        els = mesh.nod_in_elem2D[n, :mesh.nod_in_elem2D_num[n]]
+        need = 2 if nl_[els].min() < ul[els].max() else 1
+        for nz in range(mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]):
+            cnt = int(np.sum((ul[els] <= nz) & (nz < nl_[els])))
+            ok &= cnt >= need and mesh.area[n, nz - 1] > 0
+    return bool(ok)
+
+
+def test_cavity_levels_follow_the_reference_rules(cav):
+    assert _level_rules(cav[0])
+    mesh, _, _ = mg.make_config("c0cav")
+    assert mesh.ulevels.max() >= 4 and _level_rules(mesh)
+
+
+def test_cavity_thicknesses_and_depths(cav):
+    """Dry layers above the cavity have no thickness; Z_3d_n and zbar_3d_n
+    hold the nominal levels there (oce_ale.F90:340-373)."""
+    mesh = cav[0]
+    k = np.arange(1, mesh.nl)
+    above = k[None, :] < mesh.ulevels_nod2D[:, None]
+    assert above.any()
+    assert not np.any(mesh.hnode[above])
+    Z = np.broadcast_to(mesh.Z[None, :], above.shape)
+    assert np.array_equal(mesh.Z_3d_n[above], Z[above])
+    e_above = k[None, :] < mesh.ulevels[:, None]
+    assert not np.any(mesh.helem[e_above])
+    assert np.array_equal(mesh.zbar_e_srf, mesh.zbar[mesh.ulevels - 1])
+
+
+def check_surface_budget(mesh, st):
+    """At rest (u = W = 0), one step changes each tracer's content by the
+    surface flux through the first wet layer's area: -dt Q / (rho cp) for
+    temperature, dt (virtual_salt + relax_salt) for salinity (linfs)."""
+    rng = np.random.default_rng(29)
+    N2 = mesh.nod2D
+    hf = rng.normal(0.0, 100.0, N2)
+    vs = rng.normal(0.0, 1e-5, N2)
+    rs = rng.normal(0.0, 1e-5, N2)
+    z = np.zeros_like(st.w)
+    Kv = 10.0 ** rng.uniform(-5, -2, st.Kv.shape)
+    st2 = dataclasses.replace(st, uv=np.zeros_like(st.uv), w=z, w_e=z, w_i=z, Kv=Kv, heat_flux=hf,
+                              virtual_salt=vs, relax_salt=rs)
+    v, vo = st.values.copy(), st.valuesold.copy()
+    Oracle(mesh).step(st2, v, vo)
+    vol = mesh.areasvol[:, :mesh.nl - 1] * mesh.hnode
+    top = mesh.areasvol[np.arange(N2), mesh.ulevels_nod2D - 1]
+    exp = [(-st.dt * hf / 4.2e6 * top).sum(), (st.dt * (vs + rs) * top).sum()]
+    ok = True
+    for i in range(2):
+        got = (vol * (v[i] - st.values[i])).sum()
+        ok &= abs(got - exp[i]) <= 1e-13 * np.abs(vol * st.values[i]).sum()
+    return bool(ok)
+
+
+def check_cavity_stiffness(mesh, st):
+    """The pinned matrix properties, and cavity rows with no mass term: their
+    row sums vanish."""
+    o = Oracle(mesh)
+    rp, ci, v = o.stiff_matrix(st.dt)
+    rs = np.add.reduceat(v, rp[:-1])
+    cavn = mesh.ulevels_nod2D > 1
+    return P.check_stiffness(mesh, st) and bool(np.abs(rs[cavn]).max() < 1e-12 * np.abs(v).max())
+
+
+_TRACER = ["check_constant", "check_conservation", "check_fct_bounds", "check_vdiff_fv", "check_vert_impl",
+           "check_vdiff_wimpl", "check_mfct_linear", "check_qr4c_cubic", "check_ppm_linear", "check_cdiff_linear"]
+_DYN = ["check_level_sums_vanish", "check_column_flow_surface", "check_cfl", "check_wsplit",
+        "check_ssh_rhs_column_divergence", "check_ssh_rhs_alpha_terms", "check_hbar_transport",
+        "check_hbar_freshwater", "check_zstar", "check_update_stiff", "check_zlevel"]
+_DIFF = ["check_hor_diff_linear", "check_hor_diff_conserves", "check_hor_redi_shift", "check_expl_vdiff_linear",
+         "check_expl_vdiff_budget", "check_redi_expl_linear", "check_redi_k33"]
+
+
+@pytest.mark.parametrize("name", _TRACER)
+def test_tracer_pins_under_cavity(cav, name):
+    assert getattr(P, name)(*cav)
+
+
+@pytest.mark.parametrize("name", _DYN)
+def test_dynamics_pins_under_cavity(cav, name):
+    mesh, st = cav
+    assert getattr(D, name)(mesh, st, Oracle(mesh))
+
+
+@pytest.mark.parametrize("name", _DIFF)
+def test_diffusion_pins_under_cavity(cav, name):
+    assert getattr(F, name)(*cav)
+
+
+def test_surface_flux_budget_under_cavity(cav):
+    assert check_surface_budget(*cav)
+
+
+def test_ssh_under_cavity(cav):
+    assert check_cavity_stiffness(*cav)
+    assert P.check_cg(*cav)
+
+
+MUTATIONS = {
+    38: lambda c: check_cavity_stiffness(*c),
+    39: lambda c: D.check_zstar(c[0], c[1], Oracle(c[0])),
+    40: lambda c: D.check_ssh_rhs_alpha_terms(c[0], c[1], Oracle(c[0])),
+    41: lambda c: check_surface_budget(*c),
+}
+
+
+@pytest.mark.parametrize("mut", sorted(MUTATIONS))
+def test_planted_cavity_mistake_is_caught(cav, mut):
+    """Each planted mistake in a cavity branch of the oracle fails its check."""
+    assert MUTATIONS[mut](cav), "check must pass on the faithful oracle"
+    orc.set_mutation(mut)
+    try:
+        caught = not MUTATIONS[mut](cav)
+    except Exception:
+        caught = True
+    finally:
+        orc.set_mutation(0)
+    assert caught, f"mutation {mut} not caught"

@@ -150,8 +150,9 @@ def check_hor_redi_shift(mesh, st):
     g = 2e-3
     T = np.zeros((mesh.nod2D, L))
     T[:, 0] = 15.0
+    dz = mesh.zbar[:-1] - mesh.zbar[1:]   # nominal thicknesses: T uniform on a level, cavity or not
     for k in range(1, L):   # tr_z = (T(k-1) - T(k)) / (h(k-1)/2 + h(k)/2) = g
-        T[:, k] = T[:, k - 1] - g * 0.5 * (mesh.hnode[:, k - 1] + mesh.hnode[:, k])
+        T[:, k] = T[:, k - 1] - g * 0.5 * (dz[k - 1] + dz[k])
     S = np.zeros((mesh.nod2D, L, 3))
     S[..., 0], S[..., 1], S[..., 2] = 0.02, -0.035, 0.01
     Ki = np.full((mesh.nod2D, L), 300.0)
@@ -166,8 +167,9 @@ def check_hor_redi_shift(mesh, st):
     checked = 0
     scale = np.abs(exp).max()
     for n in range(mesh.nod2D):
+        kmin = max(mesh.ulevels_nod2D_max[m] for m in nb[n] | {n}) + 1
         kmax = min(mesh.nlevels_nod2D[m] for m in nb[n] | {n}) - 2
-        for k in range(2, kmax + 1):
+        for k in range(kmin, kmax + 1):
             checked += 1
             if abs(got[n, k - 1] - exp[n, k - 1]) > 1e-9 * scale:
                 bad += 1

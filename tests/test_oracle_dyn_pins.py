@@ -68,6 +68,11 @@ def _closed_nodes(mesh):
     return ~open_
 
 
+def _open(mesh):
+    """Nodes with a free surface: not under an ice-shelf cavity."""
+    return mesh.ulevels_nod2D == 1
+
+
 def _elem_centroid_y(mesh):
     return mesh.coord_nod2D[mesh.elem2D_nodes, 1].mean(1) * R
 
@@ -126,6 +131,8 @@ def check_level_sums_vanish(mesh, st, o):
     r = o.vert_vel_ale(uv, st.dt)
     wa = r["w"] * mesh.area
     div = wa[:, :-1] - wa[:, 1:]
+    k = np.arange(1, mesh.nl)
+    div = np.where(k[None, :] >= mesh.ulevels_nod2D[:, None], div, 0.0)   # wet layers only
     scale = _transport_scale(mesh, uv)
     return bool(np.abs(div.sum(0)).max() < 1e-9 * scale)
 
@@ -186,12 +193,20 @@ def check_ssh_rhs_alpha_terms(mesh, st, o):
     ok = np.array_equal(half, base)
     with_old = o.compute_ssh_rhs_ale(uv, uv, old, alpha=0.5)
     ok &= np.allclose(with_old - half, 0.5 * old, rtol=1e-12, atol=1e-9 * np.abs(base).max())
-    # full free surface: the water flux enters through the surface areasvol
+    # full free surface: the water flux enters through the surface areasvol;
+    # under a cavity (rigid ice-shelf lid) neither it nor ssh_rhs_old enters,
+    # unless use_cavity_fw2press adds the freshwater as pressure
     wf = rng.normal(0, 1e-6, mesh.nod2D)
     fs = o.compute_ssh_rhs_ale(uv, uv, old, alpha=0.5, which_ale=2, water_flux=wf)
+    fw = o.compute_ssh_rhs_ale(uv, uv, old, alpha=0.5, which_ale=2, water_flux=wf, use_cavity_fw2press=True)
     k = mesh.ulevels_nod2D - 1
-    ok &= np.allclose(fs - with_old, -0.5 * wf * mesh.areasvol[np.arange(mesh.nod2D), k], rtol=1e-9,
-                      atol=1e-9 * np.abs(base).max())
+    op = _open(mesh)
+    wfa = -0.5 * wf * mesh.areasvol[np.arange(mesh.nod2D), k]
+    tol = 1e-9 * np.abs(base).max()
+    ok &= np.allclose((fs - with_old)[op], wfa[op], rtol=1e-9, atol=tol)
+    ok &= np.array_equal(fw[op], fs[op])
+    ok &= np.array_equal(fs[~op], half[~op])
+    ok &= np.allclose((fw - half)[~op], wfa[~op], rtol=1e-9, atol=tol)
     return bool(ok)
 
 
@@ -209,8 +224,10 @@ def check_hbar_transport(mesh, st, o):
     k = mesh.ulevels_nod2D - 1
     col = o.vert_vel_ale(uv, st.dt)["w"][n, k] * mesh.area[n, k]
     scale = _transport_scale(mesh, uv)
+    op = _open(mesh)
     ok = np.array_equal(r["hbar_old"], h0)
-    ok &= np.abs((r["hbar"] - h0) * mesh.areasvol[n, k] / st.dt - col).max() < 1e-9 * scale
+    ok &= np.abs(((r["hbar"] - h0) * mesh.areasvol[n, k] / st.dt - col)[op]).max() < 1e-9 * scale
+    ok &= np.array_equal(r["hbar"][~op], h0[~op])   # the ice-shelf lid holds the surface
     rhs = o.compute_ssh_rhs_ale(uv, np.zeros_like(uv), np.zeros(mesh.nod2D), alpha=1.0)
     ok &= np.allclose(r["ssh_rhs_old"], rhs, rtol=1e-13, atol=1e-12 * scale)
     return bool(ok)
@@ -226,11 +243,14 @@ def check_hbar_freshwater(mesh, st, o):
     h0 = rng.normal(0, 0.5, mesh.nod2D)
     r = o.compute_hbar_ale(np.zeros_like(st.uv), st.dt, h0, which_ale=2, water_flux=wf)
     dh = r["hbar"] - h0
-    ok = np.allclose(dh, -wf * st.dt, rtol=1e-12, atol=1e-15)
-    ok &= np.allclose(r["dhe"], (-wf * st.dt)[mesh.elem2D_nodes].mean(1), rtol=1e-12, atol=1e-15)
+    op = _open(mesh)
+    expect = np.where(op, -wf * st.dt, 0.0)   # cavity nodes keep hbar
+    ok = np.allclose(dh, expect, rtol=1e-12, atol=1e-15)
+    eo = mesh.ulevels == 1                    # elements under the shelf get no dhe
+    ok &= np.allclose(r["dhe"], np.where(eo, expect[mesh.elem2D_nodes].mean(1), 0.0), rtol=1e-12, atol=1e-15)
     n = np.arange(mesh.nod2D)
     av = mesh.areasvol[n, mesh.ulevels_nod2D - 1]
-    ok &= np.isclose((dh * av).sum(), -st.dt * (wf * av).sum(), rtol=1e-10)
+    ok &= np.isclose((dh * av).sum(), -st.dt * (wf * av)[op].sum(), rtol=1e-10)
     return bool(ok)
 
 
@@ -242,7 +262,9 @@ def check_zstar(mesh, st, o):
     hbar = hbar_old + dh
     uv = np.zeros_like(st.uv)
     r = o.vert_vel_ale(uv, st.dt, which_ale=2, hbar=hbar, hbar_old=hbar_old, water_flux=wf)
-    ok = np.allclose(r["hnode_new"].sum(1) - mesh.hnode.sum(1), dh, rtol=1e-9, atol=1e-12)
+    op = _open(mesh)
+    ok = np.allclose((r["hnode_new"].sum(1) - mesh.hnode.sum(1))[op], dh[op], rtol=1e-9, atol=1e-12)
+    ok &= np.array_equal(r["hnode_new"][~op], mesh.hnode[~op])   # cavity columns stay linfs
     top = r["w"][np.arange(mesh.nod2D), mesh.ulevels_nod2D - 1]
     ok &= np.abs(top).max() < 1e-12 * np.abs(wf).max() + 1e-18
     return bool(ok)
@@ -262,8 +284,9 @@ def check_update_stiff(mesh, st, o):
     diag = rp[:-1]
     off = np.ones(len(v), bool)
     off[diag] = False
-    ok = np.array_equal(upd[off], v2[off])
-    av = mesh.areasvol[np.arange(mesh.nod2D), mesh.ulevels_nod2D - 1] / st.dt
+    # (zbar_e_srf - dhe) - zbar_e_srf rounds where the element lies under a cavity
+    ok = np.allclose(upd[off], v2[off], rtol=0, atol=1e-13 * np.abs(v2).max())
+    av = np.where(_open(mesh), mesh.areasvol[np.arange(mesh.nod2D), mesh.ulevels_nod2D - 1] / st.dt, 0.0)
     ok &= np.allclose(upd[diag], v2[diag] - av, rtol=0, atol=1e-12 * np.abs(v2).max())
     A = sp.csr_matrix((upd, ci, rp), shape=(mesh.nod2D, mesh.nod2D))
     scale = np.abs(upd).max()
@@ -342,8 +365,10 @@ def _zlevel_laws(m2, st, r, dh):
     w = r["w"]
     div = -(w[:, :-1] - w[:, 1:]) * st.dt
     scale = np.abs(dh).max()
-    ok = np.allclose(dthk, div, rtol=0, atol=1e-12 * scale)
-    ok &= np.allclose(dthk.sum(1), dh, rtol=0, atol=1e-12 * scale)
+    op = _open(m2)
+    ok = np.allclose(dthk[op], div[op], rtol=0, atol=1e-12 * scale)
+    ok &= np.allclose(dthk.sum(1)[op], dh[op], rtol=0, atol=1e-12 * scale)
+    ok &= not np.any(dthk[~op])   # cavity columns stay linfs
     return bool(ok)
 
 
@@ -355,24 +380,25 @@ def check_zlevel(mesh, st, o):
     none above nominal. Continuity and conservation hold in every case."""
     nominal = mesh.zbar[:-1] - mesh.zbar[1:]
     h1 = mesh.hnode[:, 0]
+    op = _open(mesh)
     ok = True
     # small change: surface layer only
     small = np.linspace(-0.1, 0.1, mesh.nod2D) * h1
     m2, r = _zlevel(mesh, st, small)
     ok &= _zlevel_laws(m2, st, r, small)
-    ok &= np.allclose(r["hnode_new"][:, 0] - mesh.hnode[:, 0], small, rtol=1e-13, atol=0)
+    ok &= np.allclose((r["hnode_new"][:, 0] - mesh.hnode[:, 0])[op], small[op], rtol=1e-13, atol=0)
     # large fall: spread, never below min_hnode of nominal
     big = -0.8 * h1
     m2, r = _zlevel(mesh, st, big)
     ok &= _zlevel_laws(m2, st, r, big)
-    ok &= bool(np.all(r["hnode_new"][:, :4] >= 0.5 * nominal[None, :4] * (1 - 1e-12)))
-    ok &= bool(np.all(r["hnode_new"][:, 1] < mesh.hnode[:, 1]))
+    ok &= bool(np.all(r["hnode_new"][op, :4] >= 0.5 * nominal[None, :4] * (1 - 1e-12)))
+    ok &= bool(np.all(r["hnode_new"][op, 1] < mesh.hnode[op, 1]))
     # previous CFL_z at the surface interface >= 0.95: the surface layer stays
     cfl = np.zeros((mesh.nod2D, mesh.nl))
     cfl[:, 0] = 0.97
     m2, r = _zlevel(mesh, st, big, cfl_prev=cfl)
     ok &= _zlevel_laws(m2, st, r, big)
-    ok &= bool(np.all(r["hnode_new"][:, 0] == mesh.hnode[:, 0]))
+    ok &= bool(np.all(r["hnode_new"][op, 0] == mesh.hnode[op, 0]))
     # rise over thinned layers 2 and 3: refilled from below, none above nominal
     hn = mesh.hnode.copy()
     hn[:, 1] = 0.6 * nominal[1]
@@ -380,8 +406,8 @@ def check_zlevel(mesh, st, o):
     rise = 0.3 * nominal[0] * np.ones(mesh.nod2D)
     m2, r = _zlevel(mesh, st, rise, hnode=hn)
     ok &= _zlevel_laws(m2, st, r, rise)
-    ok &= bool(np.all(r["hnode_new"][:, 1:4] <= nominal[None, 1:4] * (1 + 1e-12)))
-    ok &= bool(np.all(r["hnode_new"][:, 2] > hn[:, 2]))
+    ok &= bool(np.all(r["hnode_new"][op, 1:4] <= nominal[None, 1:4] * (1 + 1e-12)))
+    ok &= bool(np.all(r["hnode_new"][op, 2] > hn[op, 2]))
     return bool(ok)
 
 
