@@ -646,13 +646,11 @@ def cavity_levels(elem, elem_nb, nie, zbar, nlevels, nlevels_nod2D, cavity_depth
     nl = len(zbar)
     Zmid = 0.5 * (zbar[:-1] + zbar[1:])
     dmean = (cavity_depth[elem[:, 0]] + cavity_depth[elem[:, 1]] + cavity_depth[elem[:, 2]]) / 3.0
-    ul = np.where(dmean < 0.0, 2, 1).astype(np.int64)
-    for e in range(F2):
-        for nz in range(1, int(nlevels[e])):
-            if Zmid[nz - 1] < dmean[e] or nlevels[e] - nz <= 3:
-                ul[e] = nz
-                break
     nlev = nlevels.astype(np.int64)
+    nzs = np.arange(1, nl)
+    hit = (Zmid[None, :] < dmean[:, None]) | (nlev[:, None] - nzs[None, :] <= 3)
+    hit &= nzs[None, :] <= (nlev - 1)[:, None]
+    ul = np.where(hit.any(axis=1), np.argmax(hit, axis=1) + 1, np.where(dmean < 0.0, 2, 1)).astype(np.int64)
     for _outer in range(10):
         for nz in range(1, int(ul.max()) + 1):
             for _ in range(1000):
@@ -667,10 +665,12 @@ def cavity_levels(elem, elem_nb, nie, zbar, nlevels, nlevels_nod2D, cavity_depth
                     raise ValueError("cavity: an isolated prism would keep fewer than three layers")
                 ul = np.where(iso, nz + 1, ul)
         changed = False
-        for n in range(N2):
+        nmin = np.full(N2, nl, np.int64)
+        umax = np.zeros(N2, np.int64)
+        np.minimum.at(nmin, elem.reshape(-1), np.repeat(nlev, 3))
+        np.maximum.at(umax, elem.reshape(-1), np.repeat(ul, 3))
+        for n in np.nonzero(nmin < umax)[0]:   # only where the elements leave a gap
             els = nie[n][nie[n] >= 0]
-            if nlev[els].min() >= ul[els].max():
-                continue
             cnt = np.zeros(nl + 1, np.int64)
             for e in els:
                 cnt[ul[e]:nlev[e]] += 1
