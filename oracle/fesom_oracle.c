@@ -51,6 +51,9 @@
  *   39 zstar stretches the cavity columns as well                (vert_vel_ale)
  *   40 ssh_rhs_old enters the full free surface RHS at cavity nodes (compute_ssh_rhs_ale)
  *   41 surface flux added to layer 1 instead of the first wet layer (diff_ver_part_impl_ale)
+ *   42 UPW1 skips the top segments (A, B) where only one element is wet (adv_tra_hor_upw1)
+ *   43 MFCT skips the top segment (B) where only the second element is wet (adv_tra_hor_mfct)
+ *   44 UPW1 bottom segment (E) takes the first element's velocity and thickness (adv_tra_hor_upw1)
  */
 #include "fesom_oracle.h"
 
@@ -323,11 +326,11 @@ void fo_adv_tra_hor_upw1(const fo_mesh *m, const double *vel, const double *ttf,
 #define V2(c, nz) vel[(LAY(q.el2, nz)) * 2 + (c)]
 #define H1(nz) m->helem[LAY(q.el1, nz)]
 #define H2(nz) m->helem[LAY(q.el2, nz)]
-        for (int nz = q.nu1; nz <= nu12 - 1; ++nz) {               /* (A) */
+        for (int nz = q.nu1; nz <= nu12 - 1 && g_mut != 42; ++nz) {   /* (A) */
             double vflux = (-V1(1, nz) * dX1 + V1(0, nz) * dY1) * H1(nz);
             flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
         }
-        if (q.nu2 > 0) {
+        if (q.nu2 > 0 && g_mut != 42) {
             for (int nz = q.nu2; nz <= nu12 - 1; ++nz) {            /* (B) */
                 double vflux = s2 * (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
                 flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
@@ -343,7 +346,8 @@ void fo_adv_tra_hor_upw1(const fo_mesh *m, const double *vel, const double *ttf,
             flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
         }
         for (int nz = nl12 + 1; nz <= q.nl2; ++nz) {               /* (E) */
-            double vflux = s2 * (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
+            double vflux = (g_mut == 44) ? (V1(1, nz) * dX2 - V1(0, nz) * dY2) * H1(nz)
+                                         : s2 * (V2(1, nz) * dX2 - V2(0, nz) * dY2) * H2(nz);
             flux[LAY(e, nz)] = upw1_flux(ttf[LAY(q.n1, nz)], ttf[LAY(q.n2, nz)], vflux, flux[LAY(e, nz)]);
         }
     }
@@ -377,7 +381,7 @@ static void hor_mfct_muscl(const fo_mesh *m, const double *vel, const double *tt
         for (int pass = 0; pass < 5; ++pass) {
             int lo, hi;
             if (pass == 0) { lo = q.nu1; hi = nu12 - 1; }
-            else if (pass == 1) { if (q.nu2 <= 0) continue; lo = q.nu2; hi = nu12 - 1; }
+            else if (pass == 1) { if (q.nu2 <= 0 || g_mut == 43) continue; lo = q.nu2; hi = nu12 - 1; }
             else if (pass == 2) { lo = nu12; hi = nl12; }
             else if (pass == 3) { lo = nl12 + 1; hi = q.nl1; }
             else { lo = nl12 + 1; hi = q.nl2; }

@@ -7,8 +7,15 @@ term on the SSH diagonal (:1579-1581), no ssh_rhs_old or freshwater term in
 the free-surface RHS unless use_cavity_fw2press (:1913-1926), hbar and dhe
 kept (compute_hbar_ale), and the zstar / zlevel stretching skips the column
 (:2356-2361, :2544-2553). Surface fluxes enter the first wet layer. The pins of
-the open-ocean meshes are reused with those rules; four planted mistakes in
-the cavity branches must each fail one of them.
+the open-ocean meshes are reused with those rules; planted mistakes in the
+cavity branches must each fail one of them.
+
+Edges whose elements differ in first and last wet level use all five level
+segments of the horizontal flux (oce_adv_tra_hor.F90:166-249: one element
+wet above the shared top, both, one below the shared bottom). A constant
+tracer stays constant only if each segment takes exactly its elements'
+transports, with and without the FCT limiter; planted segment mistakes must
+fail that.
 """
 import dataclasses
 
@@ -107,6 +114,43 @@ def check_surface_budget(mesh, st):
     return bool(ok)
 
 
+def check_constant_unlimited(mesh, st):
+    """A constant stays constant without the FCT limiter too (MFCT/QR4C and
+    MUSCL/PPM, lim NON): the horizontal flux of every segment of an edge,
+    top-only, shared and bottom-only levels, balances the W that continuity
+    built from the same element transports."""
+    v0 = np.where(P._wet(mesh)[None], 5.0, 0.0) * np.ones((st.ntr, 1, 1))
+    ok = True
+    for sch in (dict(lim="NON"), dict(hor="MUSCL", ver="PPM", lim="NON")):
+        v = P._step(mesh, st, 2, values=v0, schemes=sch)
+        w = P._wet(mesh)
+        ok &= max(np.max(np.abs(v[i][w] - 5.0)) for i in range(st.ntr)) < 1e-12 * 5.0
+    return bool(ok)
+
+
+def _segments_present(mesh):
+    """Edges whose two elements differ in first (segments A/B) and last
+    (segments D/E) wet level, so that every segment of the flux is in use."""
+    el1, el2 = mesh.edge_tri[:, 0], mesh.edge_tri[:, 1]
+    two = el2 >= 0
+    e2 = np.maximum(el2, 0)
+    top = two & (mesh.ulevels[el1] != mesh.ulevels[e2])
+    bot = two & (mesh.nlevels[el1] != mesh.nlevels[e2])
+    a = top & (mesh.ulevels[el1] < mesh.ulevels[e2])
+    b = top & (mesh.ulevels[el1] > mesh.ulevels[e2])
+    d = bot & (mesh.nlevels[el1] > mesh.nlevels[e2])
+    e = bot & (mesh.nlevels[el1] < mesh.nlevels[e2])
+    return all(x.any() for x in (a, b, d, e))
+
+
+def test_all_flux_segments_in_use(cav):
+    assert _segments_present(cav[0])
+
+
+def test_constant_preserved_without_limiter_under_cavity(cav):
+    assert check_constant_unlimited(*cav)
+
+
 def check_cavity_stiffness(mesh, st):
     """The pinned matrix properties, and cavity rows with no mass term: their
     row sums vanish."""
@@ -156,6 +200,9 @@ MUTATIONS = {
     39: lambda c: D.check_zstar(c[0], c[1], Oracle(c[0])),
     40: lambda c: D.check_ssh_rhs_alpha_terms(c[0], c[1], Oracle(c[0])),
     41: lambda c: check_surface_budget(*c),
+    42: lambda c: P.check_constant(*c),
+    43: lambda c: check_constant_unlimited(*c),
+    44: lambda c: P.check_constant(*c),
 }
 
 
