@@ -12,6 +12,7 @@
 #define FGPU_EPS_AB 0.1            // oce_modules.F90:105
 #define FGPU_VCPW 4.2e6            // oce_modules.F90:31
 #define FGPU_MAX_NL 128            // columns deeper than this are rejected
+#define NXCD 8                     // XCDs of an MI355X; consecutive blocks alternate over them
 
 // tracer scheme codes
 enum { HOR_MFCT = 0, HOR_UPW1 = 1, HOR_MUSCL = 2 };

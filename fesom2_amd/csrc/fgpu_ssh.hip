@@ -418,6 +418,28 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
     }
 }
 
+// Rows of a block. With a multiple of eight blocks, the blocks b with
+// b % 8 == x (one XCD) take the x-th contiguous eighth of the rows, so that the
+// neighbour gathers of nearby rows hit that XCD's L2; otherwise plain
+// grid-stride rows. Every row in [0, n) is visited exactly once either way.
+struct CGRows { int row, end, step; };
+__device__ __forceinline__ CGRows cg_rows(int n, int nb)
+{
+    CGRows q;
+    if (nb % NXCD == 0) {
+        const int per = nb / NXCD, x = (int)(blockIdx.x % NXCD), j = (int)(blockIdx.x / NXCD);
+        const int chunk = (n + NXCD - 1) / NXCD;
+        q.row = x * chunk + j * SBLK + (int)threadIdx.x;
+        q.end = min(n, (x + 1) * chunk);
+        q.step = per * SBLK;
+    } else {
+        q.row = (int)(blockIdx.x * SBLK + threadIdx.x);
+        q.end = n;
+        q.step = nb * SBLK;
+    }
+    return q;
+}
+
 // iteration i (i = st->iter): r_i, w_i in buffers [i&1], s_{i-1} in [(i+1)&1]
 __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
 {
@@ -464,7 +486,8 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
     const double *rc = par ? A.z : A.r, *wc = par ? A.w1 : A.q, *sp = par ? A.s1 : A.p1;
     double *rn = par ? A.r : A.z, *wn = par ? A.q : A.w1, *sc = par ? A.p1 : A.s1;
     double g = 0.0, d = 0.0, r2 = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    const CGRows R = cg_rows(A.n, A.nb);
+    for (int row = R.row; row < R.end; row += R.step) {
         const double s_ = wc[row] + be * sp[row];
         const double p_ = A.dinv[row] * rc[row] + be * A.p0[row];
         A.p0[row] = p_;

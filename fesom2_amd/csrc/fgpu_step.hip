@@ -30,7 +30,6 @@
 #include "fgpu_internal.h"
 #include "fgpu_vert.h"
 
-#define NXCD 8
 #define NG 8    // edges per gather batch in K_B and K_C
 
 __device__ __forceinline__ double dmx(double a, double b) { return a > b ? a : b; }

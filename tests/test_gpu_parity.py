@@ -294,6 +294,39 @@ def c1():
     return mesh, state
 
 
+@pytest.mark.parametrize("mode", ["single", "classic"])
+def test_ssh_cg_c1(c1, monkeypatch, mode):
+    """CG on the c1 operator (496 blocks: rows in XCD-contiguous chunks for the
+    single-reduction kernel): residual below 1e-10, iterations within +-2 of
+    the oracle's, solution within cond*tol of it."""
+    import scipy.sparse as sp
+    from oracle.oracle import Oracle
+    mesh, state = c1
+    if mode == "classic":
+        monkeypatch.setenv("FGPU_SSH_CG", "classic")
+    rhs = mg.ssh_rhs(mesh, 4)
+    print(f"[gpu] cg c1 {mode}", flush=True)
+    ctx = _gpu().Context(0)
+    try:
+        ctx.mesh_init(mesh)
+        ctx.init_stiff_mat_ale(state.dt)
+        ctx.ssh_solve_preconditioner(2)
+        x, st = ctx.ssh_solve_cg(np.zeros(mesh.nod2D), rhs, soltol=1e-10, maxiter=5000)
+    finally:
+        ctx.close()
+    o = Oracle(mesh)
+    orp, oci, ov = o.stiff_matrix(state.dt)
+    ox, ost = o.cg(orp, oci, ov, o.preconditioner(orp, oci, ov, 2), rhs, soltol=1e-10, maxiter=5000)
+    A = sp.csr_matrix((ov.copy(), oci.copy(), orp.copy()), shape=(mesh.nod2D, mesh.nod2D))
+    res = np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs)
+    print(f"[gpu] cg c1 {mode}: iters {st['iters']} (oracle {ost['iters']}), res {res:.3e}, {st['ms']:.3f} ms",
+          flush=True)
+    assert st["converged"] and ost["converged"]
+    assert res < 1e-10 * 1.01
+    assert abs(st["iters"] - ost["iters"]) <= 2
+    assert np.linalg.norm(x - ox) / np.linalg.norm(ox) < 1e-7
+
+
 def test_batched_step_c1_vs_oracle(c1):
     """126,858-node band mesh, 47 levels: 2 fused steps vs the oracle, bit for bit
     (1e-12 relative is the stated bound)."""
