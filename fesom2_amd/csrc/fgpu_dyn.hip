@@ -51,7 +51,7 @@ __device__ __forceinline__ double rld(double v, int lane)
 template <int LB>
 __global__ void __launch_bounds__(LB) k_vert_vel(DevMesh m, VVArgs a)
 {
-    const int n = blockIdx.x;
+    const int n = xcd_unit();
     if (n >= m.N2) return;
     __shared__ double sW[LB + 1], sH[LB + 1];
     __shared__ double sZl[33];   // zlevel: the new thicknesses of layers 1..lzstar_lev
@@ -285,7 +285,7 @@ template <int LB, bool HB>
 __global__ void __launch_bounds__(LB) k_ssh_rhs_edge(DevMesh m, const double *uv, const double *uvr, double alpha,
                                                      double *ce)
 {
-    const int e = blockIdx.x;
+    const int e = xcd_unit();
     if (e >= m.E2) return;
     __shared__ double s1[LB], s2[LB];
     const int tid = threadIdx.x, k = tid + 1, L = m.L;
@@ -510,8 +510,8 @@ extern "C" int fgpu_vert_vel_ale(fgpu_context *c, double dt, int which_ale, cons
     FGPU_HIP(hipMemsetAsync(c->vv_flags, 0, 16, c->stream));
     {
         ProfScope ps(c, "k_vert_vel");
-        if (NL <= 64) hipLaunchKernelGGL(k_vert_vel<64>, dim3(c->N2), dim3(64), 0, c->stream, c->m, a);
-        else hipLaunchKernelGGL(k_vert_vel<128>, dim3(c->N2), dim3(128), 0, c->stream, c->m, a);
+        if (NL <= 64) hipLaunchKernelGGL(k_vert_vel<64>, dim3(xcd_grid(c->N2)), dim3(64), 0, c->stream, c->m, a);
+        else hipLaunchKernelGGL(k_vert_vel<128>, dim3(xcd_grid(c->N2)), dim3(128), 0, c->stream, c->m, a);
         FGPU_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_cfl_max, dim3(256), dim3(256), 0, c->stream, (const double *)c->vv_colmax, c->N2,
@@ -569,10 +569,10 @@ extern "C" int fgpu_compute_ssh_rhs_ale(fgpu_context *c, const double *uv_rhs, c
     {
         ProfScope ps(c, "k_ssh_rhs_edge");
         if (c->nl - 1 <= 64)
-            hipLaunchKernelGGL((k_ssh_rhs_edge<64, false>), dim3(c->E2), dim3(64), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
+            hipLaunchKernelGGL((k_ssh_rhs_edge<64, false>), dim3(xcd_grid(c->E2)), dim3(64), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
                                alpha, c->vv_ce);
         else
-            hipLaunchKernelGGL((k_ssh_rhs_edge<128, false>), dim3(c->E2), dim3(128), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
+            hipLaunchKernelGGL((k_ssh_rhs_edge<128, false>), dim3(xcd_grid(c->E2)), dim3(128), 0, c->stream, c->m, c->d.uv, c->vv_uvr,
                                alpha, c->vv_ce);
         FGPU_HIP(hipGetLastError());
     }
@@ -611,10 +611,10 @@ extern "C" int fgpu_compute_hbar_ale(fgpu_context *c, double dt, int which_ale, 
     {
         ProfScope ps(c, "k_hbar_edge");
         if (c->nl - 1 <= 64)
-            hipLaunchKernelGGL((k_ssh_rhs_edge<64, true>), dim3(c->E2), dim3(64), 0, c->stream, c->m, c->d.uv, c->d.uv,
+            hipLaunchKernelGGL((k_ssh_rhs_edge<64, true>), dim3(xcd_grid(c->E2)), dim3(64), 0, c->stream, c->m, c->d.uv, c->d.uv,
                                1.0, c->vv_ce);
         else
-            hipLaunchKernelGGL((k_ssh_rhs_edge<128, true>), dim3(c->E2), dim3(128), 0, c->stream, c->m, c->d.uv, c->d.uv,
+            hipLaunchKernelGGL((k_ssh_rhs_edge<128, true>), dim3(xcd_grid(c->E2)), dim3(128), 0, c->stream, c->m, c->d.uv, c->d.uv,
                                1.0, c->vv_ce);
         FGPU_HIP(hipGetLastError());
     }

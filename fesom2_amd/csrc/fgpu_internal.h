@@ -14,6 +14,17 @@
 #define FGPU_MAX_NL 128            // columns deeper than this are rejected
 #define NXCD 8                     // XCDs of an MI355X; consecutive blocks alternate over them
 
+// One block per unit (column or edge) over a grid padded to a multiple of NXCD:
+// blocks b, b + NXCD, ... run on one XCD and take one contiguous range of
+// units, so neighbouring units share that XCD's L2. Units at or past the real
+// count are the padding and exit.
+__device__ __forceinline__ int xcd_unit()
+{
+    const int per = (int)(gridDim.x / NXCD);
+    return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+}
+static inline unsigned xcd_grid(int units) { return (unsigned)((units + NXCD - 1) / NXCD * NXCD); }
+
 // tracer scheme codes
 enum { HOR_MFCT = 0, HOR_UPW1 = 1, HOR_MUSCL = 2 };
 enum { VER_QR4C = 0, VER_UPW1 = 1, VER_CDIFF = 2, VER_PPM = 3 };

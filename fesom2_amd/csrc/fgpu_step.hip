@@ -904,8 +904,6 @@ int fgpu_compute_zn(fgpu_context *c)
     FGPU_HIP(hipStreamSynchronize(c->stream));
     return FGPU_OK;
 }
-static inline unsigned xcd_grid(int ncol) { return (unsigned)(((ncol + NXCD - 1) / NXCD) * NXCD); }
-
 int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
 {
     const DevMesh &m = c->m;
