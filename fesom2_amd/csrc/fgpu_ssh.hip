@@ -16,6 +16,28 @@
 
 #define SBLK 256
 
+// Rows of a block. With a multiple of eight blocks, the blocks b with
+// b % 8 == x (one XCD) take the x-th contiguous eighth of the rows, so that the
+// neighbour gathers of nearby rows hit that XCD's L2; otherwise plain
+// grid-stride rows. Every row in [0, n) is visited exactly once either way.
+struct CGRows { int row, end, step; };
+__device__ __forceinline__ CGRows cg_rows(int n, int nb)
+{
+    CGRows q;
+    if (nb % NXCD == 0) {
+        const int per = nb / NXCD, x = (int)(blockIdx.x % NXCD), j = (int)(blockIdx.x / NXCD);
+        const int chunk = (n + NXCD - 1) / NXCD;
+        q.row = x * chunk + j * SBLK + (int)threadIdx.x;
+        q.end = min(n, (x + 1) * chunk);
+        q.step = per * SBLK;
+    } else {
+        q.row = (int)(blockIdx.x * SBLK + threadIdx.x);
+        q.end = n;
+        q.step = nb * SBLK;
+    }
+    return q;
+}
+
 // ------------------------------------------------------------------------------
 // assembly
 // ------------------------------------------------------------------------------
@@ -190,7 +212,8 @@ __device__ __forceinline__ double apply_m(const CGArgs &A, int row)
 __global__ void __launch_bounds__(SBLK) k_cg_init(CGArgs A)
 {
     double bb = 0.0, rz = 0.0, dum = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
+        const int row = R.row;
         const double rr = A.rhs[row] - row_dot(A.rowptr, A.colind, A.val, A.x, row);
         A.r[row] = rr;
         bb += A.rhs[row] * A.rhs[row];
@@ -215,7 +238,8 @@ __global__ void __launch_bounds__(SBLK) k_cg_init(CGArgs A)
 __global__ void __launch_bounds__(SBLK) k_cg_init_z(CGArgs A)
 {
     double rz = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
+        const int row = R.row;
         const double zz = apply_m(A, row);
         A.z[row] = zz;
         rz += A.r[row] * zz;
@@ -264,7 +288,8 @@ __global__ void __launch_bounds__(SBLK) k_cg_a(CGArgs A)
     const double *pold = par ? A.p0 : A.p1;
     double *pnew = par ? A.p1 : A.p0;
     double pq = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
+        const int row = R.row;
         double s = 0.0;
         const int p0 = A.rowptr[row], p1 = A.rowptr[row + 1];
         for (int pb = p0; pb < p1; pb += SGB) {
@@ -303,7 +328,8 @@ __global__ void __launch_bounds__(SBLK) k_cg_b(CGArgs A, int fuse_z)
     const double al = st->s_old[par] / s_aux;
     const double *p = par ? A.p1 : A.p0;
     double rz = 0.0, rr = 0.0, d2 = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
+        const int row = R.row;
         A.x[row] = A.x[row] + al * p[row];
         const double r_ = A.r[row] - al * A.q[row];
         A.r[row] = r_;
@@ -330,7 +356,8 @@ __global__ void __launch_bounds__(SBLK) k_cg_c(CGArgs A)
     CGState *st = A.st;
     if (st->done) return;
     double rz = 0.0, rr = 0.0, d2 = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
+        const int row = R.row;
         const double zz = apply_m(A, row);
         A.z[row] = zz;
         rz += A.r[row] * zz;
@@ -367,7 +394,8 @@ __device__ __forceinline__ double cg1_slot(const CGArgs &A, int slot)
 __global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A)
 {
     double bb = 0.0, rr = 0.0, dum = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
+        const int row = R.row;
         const double r = A.rhs[row] - row_dot(A.rowptr, A.colind, A.val, A.x, row);
         A.r[row] = r;
         bb += A.rhs[row] * A.rhs[row];
@@ -388,7 +416,8 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A)
 __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
 {
     double g = 0.0, d = 0.0, dum = 0.0;
-    for (int row = blockIdx.x * SBLK + threadIdx.x; row < A.n; row += A.nb * SBLK) {
+    for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
+        const int row = R.row;
         double w = 0.0;
         const int p0 = A.rowptr[row], p1 = A.rowptr[row + 1];
         for (int pb = p0; pb < p1; pb += SGB) {
@@ -416,28 +445,6 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
         A.part[blockIdx.x] = g;
         A.part[(size_t)A.nb + blockIdx.x] = d;
     }
-}
-
-// Rows of a block. With a multiple of eight blocks, the blocks b with
-// b % 8 == x (one XCD) take the x-th contiguous eighth of the rows, so that the
-// neighbour gathers of nearby rows hit that XCD's L2; otherwise plain
-// grid-stride rows. Every row in [0, n) is visited exactly once either way.
-struct CGRows { int row, end, step; };
-__device__ __forceinline__ CGRows cg_rows(int n, int nb)
-{
-    CGRows q;
-    if (nb % NXCD == 0) {
-        const int per = nb / NXCD, x = (int)(blockIdx.x % NXCD), j = (int)(blockIdx.x / NXCD);
-        const int chunk = (n + NXCD - 1) / NXCD;
-        q.row = x * chunk + j * SBLK + (int)threadIdx.x;
-        q.end = min(n, (x + 1) * chunk);
-        q.step = per * SBLK;
-    } else {
-        q.row = (int)(blockIdx.x * SBLK + threadIdx.x);
-        q.end = n;
-        q.step = nb * SBLK;
-    }
-    return q;
 }
 
 // iteration i (i = st->iter): r_i, w_i in buffers [i&1], s_{i-1} in [(i+1)&1]

@@ -294,29 +294,31 @@ def c1():
     return mesh, state
 
 
-@pytest.mark.parametrize("mode", ["single", "classic"])
+@pytest.mark.parametrize("mode", ["single", "classic", "variant1"])
 def test_ssh_cg_c1(c1, monkeypatch, mode):
-    """CG on the c1 operator (496 blocks: rows in XCD-contiguous chunks for the
-    single-reduction kernel): residual below 1e-10, iterations within +-2 of
+    """CG on the c1 operator (496 blocks: rows in XCD-contiguous chunks) with
+    Jacobi in the single-reduction and the two-kernel loop, and with the sparse
+    preconditioner variant 1: residual below 1e-10, iterations within +-2 of
     the oracle's, solution within cond*tol of it."""
     import scipy.sparse as sp
     from oracle.oracle import Oracle
     mesh, state = c1
     if mode == "classic":
         monkeypatch.setenv("FGPU_SSH_CG", "classic")
+    var = 1 if mode == "variant1" else 2
     rhs = mg.ssh_rhs(mesh, 4)
     print(f"[gpu] cg c1 {mode}", flush=True)
     ctx = _gpu().Context(0)
     try:
         ctx.mesh_init(mesh)
         ctx.init_stiff_mat_ale(state.dt)
-        ctx.ssh_solve_preconditioner(2)
+        ctx.ssh_solve_preconditioner(var)
         x, st = ctx.ssh_solve_cg(np.zeros(mesh.nod2D), rhs, soltol=1e-10, maxiter=5000)
     finally:
         ctx.close()
     o = Oracle(mesh)
     orp, oci, ov = o.stiff_matrix(state.dt)
-    ox, ost = o.cg(orp, oci, ov, o.preconditioner(orp, oci, ov, 2), rhs, soltol=1e-10, maxiter=5000)
+    ox, ost = o.cg(orp, oci, ov, o.preconditioner(orp, oci, ov, var), rhs, soltol=1e-10, maxiter=5000)
     A = sp.csr_matrix((ov.copy(), oci.copy(), orp.copy()), shape=(mesh.nod2D, mesh.nod2D))
     res = np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs)
     print(f"[gpu] cg c1 {mode}: iters {st['iters']} (oracle {ost['iters']}), res {res:.3e}, {st['ms']:.3f} ms",
