@@ -283,7 +283,15 @@ class Context:
         self._abo[tr_num - 1] = AB_order
 
     def tracer_upload(self, tr_num, values, valuesold):
-        _check(self.L.fgpu_tracer_upload(self.h, int(tr_num), _d(_f64(values)), _d(_f64(valuesold))))
+        """values (nod2D, nl-1); valuesold (nod2D, nl-1) for AB_order 2 or
+        (nod2D, nl-1, 2) for AB_order 3, the newer level first."""
+        v, vo = _f64(values), _f64(valuesold)
+        n3 = self.N2 * (self.nl - 1)
+        abo = self._abo[tr_num - 1]
+        if v.size != n3 or vo.size != (abo - 1) * n3:
+            raise ValueError(f"tracer {tr_num}: values must hold {n3} and valuesold {(abo - 1) * n3} numbers "
+                             f"(AB_order {abo}), got {v.size} and {vo.size}")
+        _check(self.L.fgpu_tracer_upload(self.h, int(tr_num), _d(v), _d(vo)))
 
     def tracer_download(self, tr_num):
         L = self.nl - 1
@@ -465,8 +473,10 @@ class Context:
         return d
 
 
-def context_for(mesh, state=None, device=0, schemes=None):
-    """Context with mesh, state and tracers uploaded (helper for tests/bench)."""
+def context_for(mesh, state=None, device=0, schemes=None, valuesold=None):
+    """Context with mesh, state and tracers uploaded (helper for tests/bench).
+    ``valuesold`` (one array per tracer) replaces the state's old levels, as an
+    AB_order 3 tracer needs two of them."""
     ctx = Context(device)
     ctx.mesh_init(mesh)
     if state is not None:
@@ -477,5 +487,6 @@ def context_for(mesh, state=None, device=0, schemes=None):
             if schemes:
                 sch.update(schemes[i] if isinstance(schemes, (list, tuple)) else schemes)
             ctx.tracer_config(i + 1, int(state.ids[i]), **sch)
-            ctx.tracer_upload(i + 1, state.values[i], state.valuesold[i])
+            vo = state.valuesold[i] if valuesold is None else valuesold[i]
+            ctx.tracer_upload(i + 1, state.values[i], vo)
     return ctx

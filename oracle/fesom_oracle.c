@@ -54,6 +54,7 @@
  *   42 UPW1 skips the top segments (A, B) where only one element is wet (adv_tra_hor_upw1)
  *   43 MFCT skips the top segment (B) where only the second element is wet (adv_tra_hor_mfct)
  *   44 UPW1 bottom segment (E) takes the first element's velocity and thickness (adv_tra_hor_upw1)
+ *   45 AB3 swaps the weights of the two old levels                 (init_tracers_AB)
  */
 #include "fesom_oracle.h"
 
@@ -247,7 +248,8 @@ int fo_init_tracers_AB(const fo_mesh *m, fo_tracer *t, fo_work *w)
         for (size_t i = 0; i < n3; ++i) t->valuesold[i] = t->values[i];
     } else if (t->AB_order == 3) {
         for (size_t i = 0; i < n3; ++i) {
-            double v = 5.0 * t->valuesold[2 * i + 1] - 16.0 * t->valuesold[2 * i] + 23.0 * t->values[i];
+            double v = (g_mut == 45) ? 5.0 * t->valuesold[2 * i] - 16.0 * t->valuesold[2 * i + 1] + 23.0 * t->values[i]
+                                     : 5.0 * t->valuesold[2 * i + 1] - 16.0 * t->valuesold[2 * i] + 23.0 * t->values[i];
             t->valuesAB[i] = v / 12.0;
         }
         for (size_t i = 0; i < n3; ++i) {

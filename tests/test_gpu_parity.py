@@ -204,6 +204,49 @@ def test_scheme_variants_c0(c0, c0_oracle):
         ctx.close()
 
 
+@pytest.mark.parametrize("path", ["fused", "staged"])
+def test_ab3_step_bitwise(c0, c0_oracle, monkeypatch, path):
+    """Tracer 1 with AB_order 3 (valuesAB from two old levels, two-deep
+    rotation of valuesold), tracer 2 with AB2: three steps equal the oracle's
+    bit for bit, both old levels included."""
+    mesh, state, _ = c0
+    wet = _wet(mesh)
+    rng = np.random.default_rng(51)
+    older = np.where(wet, state.valuesold[0] + rng.normal(0.0, 0.05, state.valuesold[0].shape), 0.0)
+    vo3 = np.ascontiguousarray(np.stack([state.valuesold[0], older], axis=-1))
+    schemes = [dict(AB_order=3), dict()]
+    if path == "staged":
+        monkeypatch.setenv("FGPU_STEP", "staged")
+    print(f"[gpu] AB3 tracer, 3 {path} steps", flush=True)
+    ctx = _gpu().context_for(mesh, state, schemes=schemes, valuesold=[vo3, state.valuesold[1]])
+    try:
+        ctx.solve_tracers_ale(state.dt, 3)
+        out = [ctx.tracer_download(i + 1) for i in range(state.ntr)]
+    finally:
+        ctx.close()
+    v = state.values.copy()
+    vos = [vo3.copy(), state.valuesold[1].copy()]
+    for _ in range(3):
+        c0_oracle.step(state, v, vos, schemes=schemes)
+    for i in range(state.ntr):
+        np.testing.assert_array_equal(out[i][0][wet], v[i][wet], err_msg=f"tracer {i + 1}")
+    np.testing.assert_array_equal(out[0][1][wet], vos[0][wet])
+    np.testing.assert_array_equal(out[1][1][wet], vos[1][wet])
+
+
+def test_upload_checks_the_old_levels(c0):
+    """An AB_order 3 tracer given one old level is refused before any copy."""
+    mesh, state, _ = c0
+    ctx = _gpu().context_for(mesh)
+    try:
+        ctx.tracers_init(1)
+        ctx.tracer_config(1, 1, AB_order=3)
+        with pytest.raises(ValueError):
+            ctx.tracer_upload(1, state.values[0], state.valuesold[0])
+    finally:
+        ctx.close()
+
+
 def test_conservation_gpu(c0):
     """sum areasvol*hnode*T changes by <1e-13 relative per 10 steps (W(surface)=0)."""
     mesh, state, _ = c0

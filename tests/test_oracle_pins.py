@@ -110,6 +110,25 @@ def check_ab2(mesh, st):
     return np.max(np.abs(ab - exp)) <= 1e-13 * np.max(np.abs(exp)) and np.array_equal(vo, v)
 
 
+def check_ab3(mesh, st):
+    """AB3 is exact for tracers quadratic in time: with T(t) = a + b t + c t^2
+    at t = n, n-1, n-2, valuesAB is T's mean over [n, n+1],
+    a + b (n + 1/2) + c (n^2 + n + 1/3); then old2 <- old1 and old1 <- T."""
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(3)
+    shp = st.values[0].shape
+    a, b, c = rng.normal(20, 1, shp), rng.normal(0, 0.1, shp), rng.normal(0, 0.01, shp)
+    n = 5.0
+    T = lambda t: a + b * t + c * t * t
+    v = T(n)
+    vo = np.ascontiguousarray(np.stack([T(n - 1), T(n - 2)], axis=-1))
+    old1 = vo[..., 0].copy()
+    out = o.adv_stage(st, v, vo, ID=1, AB_order=3)
+    exp = a + b * (n + 0.5) + c * (n * n + n + 1.0 / 3.0)
+    ok = np.max(np.abs(out["valuesAB"] - exp)) <= 1e-12 * np.max(np.abs(exp))
+    return bool(ok and np.array_equal(vo[..., 0], v) and np.array_equal(vo[..., 1], old1))
+
+
 def check_gradient_plane(mesh):
     o = orc.Oracle(mesh)
     rng = np.random.default_rng(2)
@@ -639,6 +658,10 @@ def test_ab2_closed_form(small):
     assert check_ab2(*small)
 
 
+def test_ab3_exact_for_quadratics(small):
+    assert check_ab3(*small)
+
+
 def test_gradient_plane_fit(small, sphere):
     assert check_gradient_plane(small[0])
     assert check_gradient_plane(sphere[0])
@@ -725,6 +748,7 @@ def test_cg_vs_direct(small):
 
 MUTATIONS = {
     1: lambda s, f: check_ab2(*s),
+    45: lambda s, f: check_ab3(*s),
     2: lambda s, f: check_gradient_plane(s[0]),
     3: lambda s, f: check_updown_geometry(s[0]),
     4: lambda s, f: check_fill_up_dn_grad(s[0]),
