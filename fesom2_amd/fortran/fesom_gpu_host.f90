@@ -42,6 +42,15 @@ module fesom_gpu_host
 
   type(c_ptr), save :: ctx = c_null_ptr
 
+  ! valuesold as the reference holds it, (AB_order-1, nl-1, nod2D), for AB2
+  ! and AB3 tracers; or, for an AB2 tracer, its one old level (nl-1, nod2D)
+  interface gpu_tracer_put
+     module procedure gpu_tracer_put3, gpu_tracer_put2
+  end interface gpu_tracer_put
+  interface gpu_tracer_get
+     module procedure gpu_tracer_get3, gpu_tracer_get2
+  end interface gpu_tracer_get
+
   public :: gpu_set_ki
   public :: gpu_init, gpu_finalize, gpu_check, gpu_mesh_setup, gpu_set_vertical, gpu_set_state, &
        gpu_tracers_init, gpu_tracer_config, gpu_tracer_put, gpu_tracer_get, gpu_tracer_content, &
@@ -268,17 +277,29 @@ contains
   end subroutine gpu_tracer_config
 
   ! tracers%data(tr_num)%values(nl-1,nod2D) and %valuesold(AB_order-1,nl-1,nod2D)
-  subroutine gpu_tracer_put(tr_num, values, valuesold)
+  subroutine gpu_tracer_put3(tr_num, values, valuesold)
+    integer, intent(in) :: tr_num
+    real(WP), intent(in) :: values(:, :), valuesold(:, :, :)
+    call gpu_check(fgpu_tracer_upload(ctx, int(tr_num, c_int), values, valuesold), 'fgpu_tracer_upload')
+  end subroutine gpu_tracer_put3
+
+  subroutine gpu_tracer_put2(tr_num, values, valuesold)
     integer, intent(in) :: tr_num
     real(WP), intent(in) :: values(:, :), valuesold(:, :)
     call gpu_check(fgpu_tracer_upload(ctx, int(tr_num, c_int), values, valuesold), 'fgpu_tracer_upload')
-  end subroutine gpu_tracer_put
+  end subroutine gpu_tracer_put2
 
-  subroutine gpu_tracer_get(tr_num, values, valuesold)
+  subroutine gpu_tracer_get3(tr_num, values, valuesold)
+    integer, intent(in) :: tr_num
+    real(WP), intent(out) :: values(:, :), valuesold(:, :, :)
+    call gpu_check(fgpu_tracer_download(ctx, int(tr_num, c_int), values, valuesold), 'fgpu_tracer_download')
+  end subroutine gpu_tracer_get3
+
+  subroutine gpu_tracer_get2(tr_num, values, valuesold)
     integer, intent(in) :: tr_num
     real(WP), intent(out) :: values(:, :), valuesold(:, :)
     call gpu_check(fgpu_tracer_download(ctx, int(tr_num, c_int), values, valuesold), 'fgpu_tracer_download')
-  end subroutine gpu_tracer_get
+  end subroutine gpu_tracer_get2
 
   real(WP) function gpu_tracer_content(tr_num)
     integer, intent(in) :: tr_num

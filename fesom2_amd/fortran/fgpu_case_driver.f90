@@ -28,11 +28,14 @@
 !   heat_flux(N2) water_flux(N2) virtual_salt(N2) relax_salt(N2) real_salt_flux(N2)
 !   ID(ntr) values(nl-1,N2,ntr) valuesold(nl-1,N2,ntr) ssh_rhs(N2)
 !   [Ki(nl-1,N2) if has_Ki] [slope_tapered(3,nl-1,N2) if Redi]
+!   [older(nl-1,N2) if header word 16 is 3: tracer 1 runs AB3 with old levels
+!    valuesold(:,:,1) and older]
 ! Result file:
 !   int32 status(8): iters_last, converged (1/0), nbreakdown, nnegrz, nonconv, 0, 0, 0
 !   real  resid_last, rtol_last, content_before(ntr), content_after(ntr)
 !   values(nl-1,N2,ntr) valuesold(nl-1,N2,ntr) d_eta(N2)
 !   Wvel(nl,N2) Wvel_e(nl,N2) Wvel_i(nl,N2) CFL_z(nl,N2) ssh_rhs_dyn(N2)
+!   [older(nl-1,N2) if tracer 1 runs AB3]
 program fgpu_case_driver
   use, intrinsic :: iso_c_binding
   use fesom_gpu_c, only: fgpu_version
@@ -59,6 +62,8 @@ program fgpu_case_driver
   real(WP), allocatable :: wv(:, :), wv_e(:, :), wv_i(:, :), cfl_z(:, :), hn_new(:, :), rhs_dyn(:), hbar(:)
   real(WP) :: cflmax
   real(WP), allocatable, target :: Ki(:, :), slope_tapered(:, :, :)
+  real(WP), allocatable :: older(:, :), vo3(:, :, :)
+  integer :: ab1
   logical :: use_wsplit, redi
   type(t_gpu_solverinfo) :: si
 
@@ -116,6 +121,11 @@ program fgpu_case_driver
      allocate(slope_tapered(3, nl - 1, N2))
      read(u) slope_tapered
   end if
+  ab1 = merge(3, 2, hdr(16) == 3)
+  if (ab1 == 3) then
+     allocate(older(nl - 1, N2))
+     read(u) older
+  end if
   close(u)
   write(*, '(a,i0,a,i0,a,i0,a,i0,a,i0)') '[driver] nod2D=', N2, ' elem2D=', F2, ' edge2D=', E2, ' nl=', nl, &
        ' ntr=', ntr
@@ -124,7 +134,7 @@ program fgpu_case_driver
      write(*, '(a,i0,1x,i0,1x,i0)') 'sums_int ', sum(int(edges, 8)), sum(int(nod_in_elem2D, 8)), sum(int(ids, 8))
      write(*, '(a,3es25.16)') 'sums_real ', sum(values), sum(uv), sum(ssh_rhs)
      write(*, '(a,2es25.16)') 'last ', gradient_sca(6, F2), Kv(nl, N2)
-     write(*, '(a,i0,1x,i0,1x,i0)') 'flags ', hdr(13), hdr(14), hdr(15)
+     write(*, '(a,i0,1x,i0,1x,i0,1x,i0)') 'flags ', hdr(13), hdr(14), hdr(15), ab1
      if (allocated(Ki)) write(*, '(a,es25.16)') 'ki ', sum(Ki)
      if (allocated(slope_tapered)) write(*, '(a,es25.16)') 'slope ', slope_tapered(3, nl - 1, N2)
      stop
@@ -149,8 +159,17 @@ program fgpu_case_driver
   end if
   call gpu_tracers_init(ntr)
   do tr = 1, ntr
-     call gpu_tracer_config(tr, ids(tr), 2, 'MFCT', 'QR4C', 'FCT', 0.0_WP, 1.0_WP, .true.)
-     call gpu_tracer_put(tr, values(:, :, tr), valuesold(:, :, tr))
+     if (tr == 1 .and. ab1 == 3) then
+        ! the reference's valuesold(AB_order-1, nl-1, nod2D), newer level first
+        allocate(vo3(2, nl - 1, N2))
+        vo3(1, :, :) = valuesold(:, :, 1)
+        vo3(2, :, :) = older
+        call gpu_tracer_config(tr, ids(tr), 3, 'MFCT', 'QR4C', 'FCT', 0.0_WP, 1.0_WP, .true.)
+        call gpu_tracer_put(tr, values(:, :, tr), vo3)
+     else
+        call gpu_tracer_config(tr, ids(tr), 2, 'MFCT', 'QR4C', 'FCT', 0.0_WP, 1.0_WP, .true.)
+        call gpu_tracer_put(tr, values(:, :, tr), valuesold(:, :, tr))
+     end if
      c0(tr) = gpu_tracer_content(tr)
   end do
 
@@ -170,7 +189,13 @@ program fgpu_case_driver
   write(*, '(a,i0,a)') '[driver] ', nbatched, ' batched steps done'
   flush(6)
   do tr = 1, ntr
-     call gpu_tracer_get(tr, values(:, :, tr), valuesold(:, :, tr))
+     if (tr == 1 .and. ab1 == 3) then
+        call gpu_tracer_get(tr, values(:, :, tr), vo3)
+        valuesold(:, :, 1) = vo3(1, :, :)
+        older = vo3(2, :, :)
+     else
+        call gpu_tracer_get(tr, values(:, :, tr), valuesold(:, :, tr))
+     end if
      c1(tr) = gpu_tracer_content(tr)
   end do
 
@@ -199,5 +224,6 @@ program fgpu_case_driver
   ostat(5) = si%nonconv
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
   write(u) ostat, si%resid_last, si%rtol_last, c0, c1, values, valuesold, d_eta, wv, wv_e, wv_i, cfl_z, rhs_dyn
+  if (ab1 == 3) write(u) older
   close(u)
 end program fgpu_case_driver

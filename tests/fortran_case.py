@@ -12,7 +12,9 @@ MAGIC = 1179076949
 
 
 def write_case(path, mesh, state, ssh_rhs, nsingle=1, nbatched=2, precond_variant=1, maxiter=2000,
-               soltol=1e-10, g=9.81):
+               soltol=1e-10, g=9.81, older=None):
+    """older (nod2D, nl-1): tracer 1 then runs AB3 with old levels
+    state.valuesold[0] (newer) and older."""
     i32 = lambda a: np.ascontiguousarray(a, dtype="<i4")
     f64 = lambda a: np.ascontiguousarray(a, dtype="<f8")
     hdr = np.zeros(16, dtype="<i4")
@@ -20,7 +22,7 @@ def write_case(path, mesh, state, ssh_rhs, nsingle=1, nbatched=2, precond_varian
                 nsingle, nbatched, precond_variant, maxiter]
     ki = getattr(state, "Ki", None)
     redi = bool(getattr(state, "redi", False))
-    hdr[12:15] = [ki is not None, redi, bool(getattr(state, "use_wsplit", False))]
+    hdr[12:16] = [ki is not None, redi, bool(getattr(state, "use_wsplit", False)), 3 if older is not None else 2]
     parts = [hdr, f64([state.dt, mesh.cyclic_length, soltol, g]),
              f64(mesh.coord_nod2D), i32(mesh.elem2D_nodes + 1), i32(mesh.edges + 1),
              i32(np.where(mesh.edge_tri >= 0, mesh.edge_tri + 1, -999)),
@@ -40,13 +42,15 @@ def write_case(path, mesh, state, ssh_rhs, nsingle=1, nbatched=2, precond_varian
         parts.append(f64(ki))
     if redi:
         parts.append(f64(state.slope_tapered))
+    if older is not None:
+        parts.append(f64(older))
     with open(path, "wb") as f:
         for p in parts:
             f.write(p.tobytes())
     return sum(p.nbytes for p in parts)
 
 
-def read_result(path, mesh, ntr):
+def read_result(path, mesh, ntr, ab3=False):
     L = mesh.nl - 1
     raw = open(path, "rb").read()
     st = np.frombuffer(raw, dtype="<i4", count=8)
@@ -69,7 +73,12 @@ def read_result(path, mesh, ntr):
         o += ni
     dyn["ssh_rhs"] = r[o:o + mesh.nod2D].copy()
     assert dyn["ssh_rhs"].size == mesh.nod2D, "truncated result file"
-    return dict(iters=int(st[0]), converged=bool(st[1]), nbreakdown=int(st[2]), nnegrz=int(st[3]),
+    o += mesh.nod2D
+    older = None
+    if ab3:
+        older = r[o:o + mesh.nod2D * L].reshape(mesh.nod2D, L).copy()
+        assert older.size == mesh.nod2D * L, "truncated result file"
+    return dict(older=older, iters=int(st[0]), converged=bool(st[1]), nbreakdown=int(st[2]), nnegrz=int(st[3]),
                 nonconv=int(st[4]), resid=float(resid), rtol=float(rtol), content_before=c0.copy(),
                 content_after=c1.copy(), values=values.copy(), valuesold=valuesold.copy(), d_eta=d_eta.copy(),
                 dyn=dyn)

@@ -75,9 +75,54 @@ def test_case_file_with_diffusion_read_by_fortran(c0, c0_oracle, tmp_path):
     r = subprocess.run([DRIVER, "--read", path], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
-    assert [int(x) for x in out["flags"]] == [1, 1, 1]
+    assert [int(x) for x in out["flags"]] == [1, 1, 1, 2]
     np.testing.assert_allclose(float(out["ki"][0]), st2.Ki.sum(), rtol=1e-12)
     assert float(out["slope"][0]) == float(st2.slope_tapered[-1, -1, 2])
+
+
+def _older(mesh, state):
+    k = np.arange(1, mesh.nl)
+    wet = (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+    rng = np.random.default_rng(52)
+    return wet, np.where(wet, state.valuesold[0] + rng.normal(0.0, 0.05, state.valuesold[0].shape), 0.0)
+
+
+def test_case_file_with_ab3_read_by_fortran(c0, tmp_path):
+    _need_driver()
+    mesh, state, _ = c0
+    _, older = _older(mesh, state)
+    path = str(tmp_path / "case.bin")
+    fc.write_case(path, mesh, state, mg.ssh_rhs(mesh), older=older)
+    r = subprocess.run([DRIVER, "--read", path], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
+    assert [int(x) for x in out["flags"]] == [0, 0, 0, 3]
+
+
+@pytest.mark.gpu
+def test_fortran_host_ab3(c0, c0_oracle, tmp_path):
+    """Tracer 1 runs AB3 through the Fortran host: gpu_tracer_put/get with the
+    reference's valuesold(2, nl-1, nod2D), the per-tracer loop and the batched
+    step; both tracers and both old levels equal three oracle steps."""
+    _need_driver()
+    mesh, state, _ = c0
+    wet, older = _older(mesh, state)
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "result.bin")
+    fc.write_case(case, mesh, state, mg.ssh_rhs(mesh), nsingle=1, nbatched=2, older=older)
+    print("[gpu] fortran driver c0, tracer 1 AB3: 1 per-tracer + 2 batched steps", flush=True)
+    r = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = fc.read_result(res, mesh, state.ntr, ab3=True)
+    v = state.values.copy()
+    vos = [np.ascontiguousarray(np.stack([state.valuesold[0], older], axis=-1)), state.valuesold[1].copy()]
+    for _ in range(3):
+        c0_oracle.step(state, v, vos, schemes=[dict(AB_order=3), dict()])
+    for i in range(state.ntr):
+        np.testing.assert_array_equal(out["values"][i][wet], v[i][wet], err_msg=f"tracer {i + 1}")
+    np.testing.assert_array_equal(out["valuesold"][0][wet], vos[0][..., 0][wet])
+    np.testing.assert_array_equal(out["older"][wet], vos[0][..., 1][wet])
+    np.testing.assert_array_equal(out["valuesold"][1][wet], vos[1][wet])
 
 
 @pytest.mark.gpu
