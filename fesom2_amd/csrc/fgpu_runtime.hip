@@ -747,10 +747,10 @@ extern "C" int fgpu_tracer_config(fgpu_context *c, int tr_num, int ID, int AB_or
     if (AB_order != 2 && AB_order != 3)
         return fgpu_fail(FGPU_ERR_ARG, "Adams-Bashfort tracer order must be 2 or 3");
     switch (ID) {   // bc_surface IDs of a plain build (oce_ale_tracer.F90:1862-2098)
-    case 1: case 2: case 100: case 301: case 302: case 303: break;
+    case 1: case 2: case 100: case 301: case 302: case 303: case 501: break;
     default:
         return fgpu_fail(FGPU_ERR_BC, "invalid ID " + std::to_string(ID) +
-                                          " specified in boundary conditions (supported: 1, 2, 100, 301-303)");
+                                          " specified in boundary conditions (supported: 1, 2, 100, 301-303, 501)");
     }
     const int i = tr_num - 1;
     FGPU_HIP(hipSetDevice(c->device));

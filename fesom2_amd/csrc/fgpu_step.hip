@@ -580,11 +580,13 @@ __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt
 // ------------------------------------------------------------------------------
 // K_D: limited fluxes, flux2dtracer, T*, implicit vertical diffusion
 // ------------------------------------------------------------------------------
-__device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &d, int n, double sval)
+__device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &d, int n, int nzmin, double sval)
 {
     switch (id) {
     case 1: return -dt * (d.heat_flux[n] / FGPU_VCPW + sval * d.water_flux[n] * d.is_nonlinfs);
     case 2: return dt * (d.virtual_salt[n] + d.relax_salt[n] + d.real_salt_flux[n] * d.is_nonlinfs);
+    // ice-shelf water from basal melting, under a cavity only (oce_ale_tracer.F90:2080-2085)
+    case 501: return nzmin == 1 ? 0.0 : dt * d.water_flux[n] * (sval - 1.0);
     default: return 0.0;
     }
 }
@@ -814,7 +816,7 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
             if (k == nzmin) r = -(rb[tid] - dz) * ts[tid] - rc[tid] * ts[tid + 1];
             else if (k <= nzmax - 2) r = -ra[tid] * ts[tid - 1] - (rb[tid] - dz) * ts[tid] - rc[tid] * ts[tid + 1];
             else r = -ra[tid] * ts[tid - 1] - (rb[tid] - dz) * ts[tid];
-            if (k == nzmin) r = r + bc_surface_d(t.id[tr], dt, d, n, ts[tid]);
+            if (k == nzmin) r = r + bc_surface_d(t.id[tr], dt, d, n, nzmin, ts[tid]);
             stpA[w][c][tid] = r;
         }
         __syncthreads();

@@ -648,11 +648,13 @@ __global__ void __launch_bounds__(BLK) k_tstar(DevMesh m, DevDyn d, DevTracers t
 // (:1828-2108), then the salinity clamp of solve_tracers_ale (:502-514) for
 // the second tracer. Redi off (isredi = 0), no w-split implicit advection.
 // ------------------------------------------------------------------------------
-__device__ __forceinline__ double bc_surface(int id, double dt, const DevDyn &d, int n, double sval)
+__device__ __forceinline__ double bc_surface(int id, double dt, const DevDyn &d, int n, int nzmin, double sval)
 {
     switch (id) {
     case 1: return -dt * (d.heat_flux[n] / FGPU_VCPW + sval * d.water_flux[n] * d.is_nonlinfs);
     case 2: return dt * (d.virtual_salt[n] + d.relax_salt[n] + d.real_salt_flux[n] * d.is_nonlinfs);
+    // ice-shelf water from basal melting, under a cavity only (oce_ale_tracer.F90:2080-2085)
+    case 501: return nzmin == 1 ? 0.0 : dt * d.water_flux[n] * (sval - 1.0);
     default: return 0.0;
     }
 }
@@ -715,7 +717,7 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
         double zinv1 = zinv2;
         double Tm = 0.0, Tk = T[I(nz)], Tp = T[I(nz + 1)];
         double tr_ = -(b - hn[I(nz)]) * Tk - c * Tp;
-        tr_ = tr_ + bc_surface(t.id[tr], dt, d, n, Tk);
+        tr_ = tr_ + bc_surface(t.id[tr], dt, d, n, nzmin, Tk);
         cp[I(nz)] = c / b;
         tp[I(nz)] = tr_ / b;
         for (nz = nzmin + 1; nz <= nzmax - 1; ++nz) {

@@ -161,7 +161,10 @@ int fgpu_compute_hbar_ale(fgpu_context *ctx, double dt, int which_ale, const dou
  * num_tracers tracers; each configured with its namelist entry
  * (config/namelist.tra:33-38): hor in {"MFCT","UPW1","MUSCL"}, ver in {"QR4C","UPW1","CDIFF","PPM"},
  * lim in {"FCT","NON"} (trailing blanks ignored), tra_adv_ph/pv, AB_order 2|3,
- * i_vert_diff. tr_num is 1-based like the reference. */
+ * i_vert_diff. tr_num is 1-based like the reference. ID selects bc_surface
+ * (oce_ale_tracer.F90:1862-2098): 1 temperature, 2 salinity, 100 and 301-303
+ * passive (no surface flux), 501 ice-shelf water (dt*water_flux*(T-1) under a
+ * cavity); any other ID fails with FGPU_ERR_BC, as the reference stops. */
 int fgpu_tracers_init(fgpu_context *ctx, int num_tracers);
 int fgpu_tracer_config(fgpu_context *ctx, int tr_num, int ID, int AB_order, const char *tra_adv_hor,
                        const char *tra_adv_ver, const char *tra_adv_lim, double tra_adv_ph,

@@ -55,6 +55,7 @@
  *   43 MFCT skips the top segment (B) where only the second element is wet (adv_tra_hor_mfct)
  *   44 UPW1 bottom segment (E) takes the first element's velocity and thickness (adv_tra_hor_upw1)
  *   45 AB3 swaps the weights of the two old levels                 (init_tracers_AB)
+ *   46 ice-shelf meltwater (ID 501) enters open-ocean columns too   (bc_surface)
  */
 #include "fesom_oracle.h"
 
@@ -1146,7 +1147,7 @@ void fo_diff_ver_part_redi_expl(const fo_mesh *m, const fo_dyn *d, double dt, co
 /* ------------------------------------------------------------------------ */
 /* bc_surface: oce_ale_tracer.F90:1828-2108 (non-RECOM IDs)                  */
 /* ------------------------------------------------------------------------ */
-int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, double sval, double *out)
+int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, int nzmin, double sval, double *out)
 {
     switch (id) {
     case 1:
@@ -1157,6 +1158,9 @@ int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, double sval, double
         return 0;
     case 100: case 301: case 302: case 303:
         *out = 0.0;
+        return 0;
+    case 501:   /* ice-shelf water from basal melting: under a cavity only (:2080-2085) */
+        *out = (nzmin == 1 && g_mut != 46) ? 0.0 : dt * d->water_flux[n] * (sval - 1.0);
         return 0;
     default:
         return -1;   /* par_ex: invalid ID (oce_ale_tracer.F90:2091-2098) */
@@ -1280,7 +1284,7 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
         tr[I(nz)] = -a[I(nz)] * T[I(nz - 1)] - (b[I(nz)] - dz) * T[I(nz)];
 
         double bc;
-        if (fo_bc_surface(t->ID, dt, d, n, T[I(nzmin)], &bc) != 0) { rc = -1; break; }
+        if (fo_bc_surface(t->ID, dt, d, n, nzmin, T[I(nzmin)], &bc) != 0) { rc = -1; break; }
         if (g_mut == 41) tr[I(1)] = tr[I(1)] + bc;
         else tr[I(nzmin)] = tr[I(nzmin)] + bc;
 

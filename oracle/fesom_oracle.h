@@ -111,7 +111,7 @@ void fo_diff_ver_part_redi_expl(const fo_mesh *m, const fo_dyn *d, double dt, co
                                 double *del_ttf);
 void fo_diff_tracers_ale_tstar(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w);
 int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t);
-int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, double sval, double *out);
+int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, int nzmin, double sval, double *out);
 int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, int ntr, fo_work *w);
 
 int fo_init_stiff_mat_ale(const fo_mesh *m, double dt, int *rowptr, int *colind, double *values, int max_nnz);

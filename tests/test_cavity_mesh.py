@@ -114,6 +114,32 @@ def check_surface_budget(mesh, st):
     return bool(ok)
 
 
+def check_meltwater_budget(mesh, st):
+    """Ice-shelf water (ID 501) at rest: one step changes its content by
+    dt water_flux (T - 1) areasvol at the first wet layer of the cavity
+    columns only, T the tracer there before the step."""
+    rng = np.random.default_rng(31)
+    N2 = mesh.nod2D
+    wf = rng.normal(0.0, 1e-5, N2)
+    z = np.zeros_like(st.w)
+    st3 = mg.build_state(mesh, seed=5, ntr=3, dt=st.dt, dx=20e3)
+    st3 = dataclasses.replace(st3, ids=np.array([1, 2, 501], np.int32), uv=np.zeros_like(st3.uv), w=z, w_e=z,
+                              w_i=z, water_flux=wf)
+    v, vo = st3.values.copy(), st3.valuesold.copy()
+    Oracle(mesh).step(st3, v, vo)
+    n = np.arange(N2)
+    top = mesh.ulevels_nod2D - 1
+    cav = mesh.ulevels_nod2D > 1
+    exp = (st3.dt * wf * (st3.values[2][n, top] - 1.0) * mesh.areasvol[n, top])[cav].sum()
+    vol = mesh.areasvol[:, :mesh.nl - 1] * mesh.hnode
+    got = (vol * (v[2] - st3.values[2])).sum()
+    return bool(abs(got - exp) <= 1e-12 * abs(exp) + 1e-13 * np.abs(vol * st3.values[2]).sum())
+
+
+def test_meltwater_budget_under_cavity(cav):
+    assert check_meltwater_budget(*cav)
+
+
 def check_constant_unlimited(mesh, st):
     """A constant stays constant without the FCT limiter too (MFCT/QR4C and
     MUSCL/PPM, lim NON): the horizontal flux of every segment of an edge,
@@ -203,6 +229,7 @@ MUTATIONS = {
     42: lambda c: P.check_constant(*c),
     43: lambda c: check_constant_unlimited(*c),
     44: lambda c: P.check_constant(*c),
+    46: lambda c: check_meltwater_budget(*c),
 }
 
 

@@ -71,6 +71,33 @@ def test_cavity_step_bitwise(cav, monkeypatch, path, sch):
         np.testing.assert_array_equal(out[i][1][wet], vo[i][wet], err_msg=f"tracer {i + 1} valuesold")
 
 
+@pytest.mark.parametrize("path", ["fused", "staged"])
+def test_cavity_meltwater_tracer_bitwise(cav, monkeypatch, path):
+    """A third tracer with ID 501 (ice-shelf water, fed by the water flux under
+    the cavity only): three steps equal the oracle's bit for bit."""
+    from fesom2_amd import gpu
+    mesh, state, o = cav
+    _, st3, _ = mg.make_config("c0cav", ntr=3)
+    st3 = dataclasses.replace(st3, ids=np.array([1, 2, 501], np.int32), heat_flux=state.heat_flux,
+                              water_flux=state.water_flux, virtual_salt=state.virtual_salt,
+                              relax_salt=state.relax_salt, real_salt_flux=state.real_salt_flux)
+    if path == "staged":
+        monkeypatch.setenv("FGPU_STEP", "staged")
+    print(f"[gpu] cavity meltwater tracer -> 3 {path} steps", flush=True)
+    ctx = gpu.context_for(mesh, st3)
+    try:
+        ctx.solve_tracers_ale(st3.dt, 3)
+        out = [ctx.tracer_download(i + 1)[0] for i in range(st3.ntr)]
+    finally:
+        ctx.close()
+    v, vo = st3.values.copy(), st3.valuesold.copy()
+    for _ in range(3):
+        o.step(st3, v, vo)
+    wet = _wet(mesh)
+    for i in range(st3.ntr):
+        np.testing.assert_array_equal(out[i][wet], v[i][wet], err_msg=f"tracer {i + 1}")
+
+
 def test_cavity_adv_stage_bitwise(cav):
     """init_tracers_AB, do_oce_adv_tra and diff_tracers_ale one by one: the
     low-order solution, the horizontal and vertical fluxes and the tracer."""
