@@ -117,6 +117,8 @@ struct CGState {
     int par;        // parity of the iteration in flight (written by k_cg_a)
     int done;       // 0 running, 1 converged, 2 breakdown, 3 maxiter
     int nbreak, nneg, pad;
+    int itp[2];         // k_cg1: iteration number of the launch of each parity
+    double s_oldp[4];   // k_cg1: gamma and alpha of the previous iteration, per parity
     double s_old[2];
     double rtol, rr_last;
 };
@@ -447,12 +449,16 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
     }
 }
 
-// iteration i (i = st->iter): r_i, w_i in buffers [i&1], s_{i-1} in [(i+1)&1]
-__global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
+// iteration i: r_i, w_i in buffers [i&1], s_{i-1} in [(i+1)&1]. par = i & 1
+// comes with the launch (every graph replay runs an even number of iterations
+// from an even one), and i and the previous scalars are read from the slots of
+// that parity, so a block dispatched after others have finished still reads
+// this iteration's values.
+__global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A, int par)
 {
     CGState *st = A.st;
     if (st->done) return;
-    const int it = st->iter, par = it & 1;
+    const int it = st->itp[par];
     double gam = strided_sum(A.part + (size_t)(2 * par) * A.nb, A.nb);
     double del = strided_sum(A.part + (size_t)(2 * par + 1) * A.nb, A.nb);
     double rr = strided_sum(A.part + (size_t)(4 + par) * A.nb, A.nb);
@@ -477,7 +483,7 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
     if (gam < 0.0 && lead) st->nneg += 1;
     double be = 0.0, den = del;
     if (it > 0) {
-        const double gprev = st->s_old[0], aprev = st->s_old[1];
+        const double gprev = st->s_oldp[2 * par], aprev = st->s_oldp[2 * par + 1];
         if (gprev == 0.0) {
             if (lead) { st->nbreak += 1; st->done = 2; }
             return;
@@ -521,9 +527,12 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A)
         A.part[(size_t)(2 * np + 1) * A.nb + blockIdx.x] = d;
         A.part[(size_t)(4 + np) * A.nb + blockIdx.x] = r2;
     }
+    // the next iteration's number and scalars go to the other parity's slots,
+    // which no block of this launch reads
     if (lead) {
-        st->s_old[0] = gam;
-        st->s_old[1] = al;
+        st->s_oldp[2 * np] = gam;
+        st->s_oldp[2 * np + 1] = al;
+        st->itp[np] = it + 1;
         st->iter = it + 1;
     }
 }
@@ -728,7 +737,7 @@ static int enqueue_iters(fgpu_context *c, const CGArgs &A, int iters)
 {
     const dim3 g(A.nb), b(SBLK);
     if (A.variant == 2 && c->ssh_single_reduction) {
-        for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_cg1, g, b, 0, c->stream, A);
+        for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_cg1, g, b, 0, c->stream, A, i & 1);
         FGPU_HIP(hipGetLastError());
         return FGPU_OK;
     }
