@@ -1,6 +1,6 @@
 """Benchmark of the MI355X tracer-transport step and SSH solve.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4]
 
 One "step" is one solve_tracers_ale pass over every tracer of the workload
 (AB2, MFCT/QR4C/FCT advection, implicit vertical diffusion), replayed from a
@@ -10,8 +10,8 @@ solve (diagonally preconditioned CG to 1e-10 relative residual, x0 = 0) is
 timed separately and reported as ssh_solve_ms.
 
 Also reported: the per-kernel profile of the step (hipEvents on the library
-stream), the roofline of the dominant kernel, and the CPU oracle timed on a
-bounded sample of the same workload on this host (one core).
+stream), the roofline of the dominant kernel, and the oracle's OpenMP build
+timed on a bounded sample of the same workload on this host's cores.
 """
 from __future__ import annotations
 
@@ -79,6 +79,36 @@ def kernel_bytes(name, mesh, ntr):
     return None
 
 
+def kernel_source_digest():
+    """Digest of the fused step's kernel source: a committed traffic
+    measurement holds only for the kernels it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("fgpu_step.hip", "fgpu_vert.h", "fgpu_internal.h"):
+        with open(os.path.join(ROOT, "fesom2_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic_<config>.txt
+    (rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when the file was measured on other kernel sources than these."""
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{config}.txt")
+    if not os.path.exists(tfile):
+        return None, "no measurement for this workload"
+    digest, traffic = None, None
+    for line in open(tfile):
+        parts = line.split()
+        if line.startswith("# source digest") and parts:
+            digest = parts[-1]
+        elif len(parts) >= 2 and parts[0] == kernel:
+            traffic = float(parts[1])
+    if digest != kernel_source_digest():
+        return None, f"{os.path.relpath(tfile, ROOT)} was measured on other kernel sources"
+    return traffic, os.path.relpath(tfile, ROOT)
+
+
 def step_model_bytes(mesh, ntr):
     """Whole-step algorithmic bytes of the transport model: per tracer 88 N3 + 16 E3
     (state in/out, LO, A_h, A_v, R+-), shared 24 F3 + 72 N3 + mesh terms."""
@@ -87,13 +117,28 @@ def step_model_bytes(mesh, ntr):
     return ntr * (88 * N3 + 16 * E3) + 24 * F3 + 72 * N3 + 72 * E2 + 80 * F2 + 48 * N2
 
 
-def cpu_baseline(mesh, state, budget_s=12.0):
-    """Oracle (single-threaded C port) on whole steps of the same mesh and state.
-    The sample keeps as many of the tracers as about budget_s of one core allows
-    (at roughly 1e7 tracer-updates/s), at least one, and runs whole steps."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mesh, state, budget_s=15.0):
+    """The oracle's OpenMP build (C port of the reference path, columns, edges
+    and elements in parallel, bit-identical to the serial oracle) on whole
+    steps of the same mesh and state, on OMP_NUM_THREADS host threads (16, the
+    GPU box's CPU share, when unset). The sample keeps as many of the tracers
+    as about budget_s allows at an assumed 2e6 tracer-updates/s per thread (at
+    least T and S) and runs whole steps until budget_s has passed."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)   # read when the OpenMP runtime starts
     from oracle.oracle import Oracle
-    orc = Oracle(mesh)
-    k = max(1, min(state.ntr, int(budget_s * 1.0e7 / max(mesh.n3, 1))))
+    orc = Oracle(mesh, omp=True)
+    k = max(min(2, state.ntr), min(state.ntr, int(budget_s * 2.0e6 * threads / max(mesh.n3, 1))))
     v = state.values[:k].copy()
     vo = state.valuesold[:k].copy()
     t0 = time.perf_counter()
@@ -106,9 +151,11 @@ def cpu_baseline(mesh, state, budget_s=12.0):
             break
     upd = k * mesh.n3 * n / el
     which = f"all {state.ntr} tracers" if k == state.ntr else f"{k} of the {state.ntr} tracers"
-    return {"value": upd, "unit": "tracer-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full steps of the same mesh and state ({which}), "
-                      f"{el:.1f} s, C oracle built with gcc -O2, one thread"}
+    return {"value": upd, "unit": "tracer-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"{n} full steps of the same mesh and state ({which}), {el:.1f} s, the oracle's "
+                      f"OpenMP build (gcc -O3, no FMA contraction, bit-identical to the serial oracle), "
+                      f"{threads} threads"}
 
 
 def main():
@@ -116,7 +163,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--config", default="c4", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ssh-reps", type=int, default=5)
     ap.add_argument("--shape", default=None,
@@ -175,16 +222,10 @@ def main():
     dom_avg_ms = prof[dom][0] / prof[dom][1]
     dom_bytes = kernel_bytes(dom, mesh, state.ntr)
     achieved = dom_bytes / (dom_avg_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.config}.txt")
-    if os.path.exists(tfile):
-        for line in open(tfile):
-            parts = line.split()
-            if len(parts) >= 2 and parts[0] == dom:
-                traffic = float(parts[1])
+    traffic, traffic_src = measured_traffic(args.config, dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "alg_bytes_per_launch": int(dom_bytes)}
+                "traffic_source": traffic_src, "alg_bytes_per_launch": int(dom_bytes)}
     step_bytes = sum(kernel_bytes(n, mesh, state.ntr) or 0 for n in prof)
     step_ms = sum(ms for ms, c in prof.values()) / nprof
 

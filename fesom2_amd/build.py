@@ -29,6 +29,10 @@ def hipcc() -> str:
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_SOURCES = ["fesom_gpu_c.f90", "fesom_gpu_host.f90"]
 FORTRAN_DRIVER = os.path.join(FORTRAN_DIR, "fgpu_case_driver")
+# the drop-in entry points with the reference's argument lists, over stand-ins
+# of the reference's type modules, and the program that runs a case through them
+DROPIN_SOURCES = ["fesom_types.f90", "fesom_gpu_dropin.f90"]
+DROPIN_DRIVER = os.path.join(FORTRAN_DIR, "fgpu_dropin_driver")
 
 
 def flang():
@@ -96,6 +100,27 @@ def build_fortran(verbose: bool = False):
     if r.returncode != 0:
         raise RuntimeError(f"flang link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, FORTRAN_DRIVER)
+    # drop-in: its own module directory, since its type modules carry the
+    # reference's module names
+    ddir = os.path.join(HERE, "_build", "fortran_dropin")
+    os.makedirs(ddir, exist_ok=True)
+    dobjs = [os.path.join(objdir, "fesom_gpu_c.o")]
+    for src in DROPIN_SOURCES + ["fgpu_dropin_driver.f90"]:
+        obj = os.path.join(ddir, src.replace(".f90", ".o"))
+        cmd = [fc, "-O2", "-fPIC", "-module-dir", ddir, "-I", ddir, "-I", objdir, "-c",
+               os.path.join(FORTRAN_DIR, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed on {src}:\n{r.stdout}\n{r.stderr}")
+        dobjs.append(obj)
+    tmp = DROPIN_DRIVER + ".tmp"
+    cmd = [fc, "-O2", "-o", tmp, *dobjs, "-L", HERE, "-lfesom_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang link failed (drop-in driver):\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DROPIN_DRIVER)
     return FORTRAN_DRIVER
 
 

@@ -48,6 +48,8 @@ struct DevMesh {
     const int *ne_rec;       // 8 per ne_idx entry: q, other node, nu12, nl12, nl1, nu1, nl2, nu2
     const int *node_rec;     // 4 per node: ne_ptr, degree, ulevels_nod2D, nlevels_nod2D
     const int *nblay;        // N2: nboundary_lay of MUSCL (oce_muscl_adv.F90:91-157)
+    const int *fct_pad;      // N2*2: levels k < fct_pad[2n] or k >= fct_pad[2n+1] have an adjacent element
+                             // without level k, whose FCT bounds are the +-1e3 padding (oce_adv_tra_fct.F90:148-178)
     const double *elem_area, *elem_cos, *edge_dxdy, *ecd, *gsca;
     const double *area, *areasvol;        // N2*NL
     const double *hnode, *hnode_new;      // N2*L
@@ -78,7 +80,36 @@ struct DevDyn {
     const double *slope;   // N2*L*3 slope_tapered, component fastest
     int hdiff;             // Ki set: diff_part_hor_redi runs
     int redi;              // Redi: slope terms, diff_ver_part_redi_expl, K33 of the implicit diffusion
+    // surface fluxes of the tracer IDs 101-104 and 304 (fgpu_set_surface_forcing_tracers), null until set
+    const double *prec_rain;   // N2, ID 101
+    const double *wiso;        // N2*3, node fastest (wiso_flux_oce(:,1..3)), IDs 102-104
+    const double *hosing;      // N2, ID 304
+    const double *hosing3d;    // N2*L, ID 304 in depth mode
+    int hosing_depth;          // use_hosing .and. hosing_mode == 'depth'
 };
+
+// bc_surface (oce_ale_tracer.F90:1828-2108) for the IDs of a plain build; the
+// ID and the presence of its flux array are checked on the host before any
+// launch (fgpu_check_bc_inputs). nzmin is the column's first wet level.
+__device__ __forceinline__ double bc_surface_dev(int id, double dt, const DevDyn &d, int N2, int L, int n, int nzmin,
+                                                 double sval)
+{
+    switch (id) {
+    case 1: return -dt * (d.heat_flux[n] / FGPU_VCPW + sval * d.water_flux[n] * d.is_nonlinfs);
+    case 2: return dt * (d.virtual_salt[n] + d.relax_salt[n] + d.real_salt_flux[n] * d.is_nonlinfs);
+    case 101: return dt * d.prec_rain[n];                                  // :2059-2060
+    case 102: case 103: case 104: return dt * d.wiso[(size_t)(id - 102) * N2 + n];   // :2062-2067
+    case 304:                                                               // :2074-2079
+        return d.hosing_depth ? dt * d.hosing3d[(size_t)n * L + nzmin - 1] : dt * d.hosing[n];
+    // ice-shelf water from basal melting, under a cavity only (:2080-2085)
+    case 501: return nzmin == 1 ? 0.0 : dt * d.water_flux[n] * (sval - 1.0);
+    default: return 0.0;   // 100, 301-303
+    }
+}
+// the hosing tracer in depth mode takes bc_surface at every wet layer of the
+// column, with that layer as the level being filled (oce_ale_tracer.F90:1374-1380);
+// every other tracer at the first wet layer only
+__device__ __forceinline__ bool bc_every_layer(int id, const DevDyn &d) { return id == 304 && d.hosing_depth; }
 
 // Per-tracer state, tracer-major: tracer t occupies [t*stride, (t+1)*stride).
 struct DevTracers {
@@ -159,8 +190,7 @@ struct fgpu_context {
     double *ssh_w1 = nullptr, *ssh_s1 = nullptr;
     bool ssh_single_reduction = true;   // diagonal preconditioner: one reduction (one kernel) per iteration
     hipGraphExec_t ssh_graph = nullptr;
-    double ssh_graph_tol = 0.0;
-    int ssh_graph_maxiter = 0, ssh_graph_variant = -1, ssh_last_iters = 0;
+    int ssh_graph_variant = -1, ssh_last_iters = 0;
     double *ssh_part = nullptr;   // per-block partial sums
     double *ssh_out = nullptr;    // device stats
     int ssh_blocks = 0;
@@ -189,6 +219,8 @@ int fgpu_fail(int code, const std::string &msg);
     } while (0)
 
 int fgpu_alloc(fgpu_context *c, void **p, size_t bytes, bool tracer_scope = false);
+// FGPU_ERR_STATE unless every flux array that the IDs of tracers tr0..tr0+ntr-1 read is set
+int fgpu_check_bc_inputs(fgpu_context *c, int tr0, int ntr);
 
 // launch bracketing for the per-kernel profile (fgpu_runtime.hip): when the
 // context profiles, the constructor records an event before the launch and the

@@ -463,6 +463,24 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
     }
     UP(p_i, nblay.data(), nblay.size()); m.nblay = p_i;
     {
+        // FCT padding range per node from its elements (oce_adv_tra_fct.F90:148-178,
+        // 195-215): an element without level k enters the node's bounds as
+        // -1e3 / +1e3; some element around n lacks level k iff k < max ulevels
+        // or k >= min nlevels over the elements around n
+        std::vector<int> pad(2 * (size_t)N2);
+        for (int n = 0; n < N2; ++n) {
+            int umax = 1, nmin = nl + 1;
+            for (int k = 0; k < nod_in_elem2D_num[n]; ++k) {
+                const int el = nie[(size_t)n * kmax + k];
+                umax = std::max(umax, ulevels[el]);
+                nmin = std::min(nmin, nlevels[el]);
+            }
+            pad[2 * n] = umax;
+            pad[2 * n + 1] = nmin;
+        }
+        UP(p_i, pad.data(), pad.size()); m.fct_pad = p_i;
+    }
+    {
         // packed edge records (mesh arrays and the fill_up_dn_grad level rule)
         std::vector<int> ri((size_t)E2 * ERI, 0);
         std::vector<double> rd((size_t)E2 * ERD, 0.0);
@@ -535,6 +553,7 @@ template <typename T>
 static int upload_or_refresh(fgpu_context *c, const T **dst, const T *src, size_t n)
 {
     if (!src) return fgpu_fail(FGPU_ERR_ARG, "null array");
+    FGPU_HIP(hipStreamSynchronize(c->stream));   // no launch of the context still reads the old contents
     if (*dst == nullptr) {
         T *p;
         int rc = fgpu_alloc(c, (void **)&p, n * sizeof(T));
@@ -639,6 +658,40 @@ extern "C" int fgpu_set_surface_forcing(fgpu_context *c, const double *heat_flux
     c->d.is_nonlinfs = is_nonlinfs;
     c->forcing_ready = true;
     c->graph_valid = false;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_set_surface_forcing_tracers(fgpu_context *c, const double *prec_rain,
+                                                const double *wiso_flux_oce, const double *hosing_flux,
+                                                const double *hosing_flux3D, int hosing_depth)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    FGPU_HIP(hipSetDevice(c->device));
+    const size_t N2 = c->N2, L = c->nl - 1;
+    int rc;
+    if (prec_rain && (rc = upload_or_refresh(c, &c->d.prec_rain, prec_rain, N2))) return rc;
+    if (wiso_flux_oce && (rc = upload_or_refresh(c, &c->d.wiso, wiso_flux_oce, 3 * N2))) return rc;
+    if (hosing_flux && (rc = upload_or_refresh(c, &c->d.hosing, hosing_flux, N2))) return rc;
+    if (hosing_flux3D && (rc = upload_or_refresh(c, &c->d.hosing3d, hosing_flux3D, N2 * L))) return rc;
+    c->d.hosing_depth = hosing_depth ? 1 : 0;
+    c->graph_valid = false;
+    return FGPU_OK;
+}
+
+int fgpu_check_bc_inputs(fgpu_context *c, int tr0, int ntr)
+{
+    for (int i = tr0; i < tr0 + ntr; ++i) {
+        const int id = c->h_id[i];
+        const char *miss = nullptr;
+        if (id == 101 && !c->d.prec_rain) miss = "prec_rain";
+        if (id >= 102 && id <= 104 && !c->d.wiso) miss = "wiso_flux_oce";
+        if (id == 304 && c->d.hosing_depth && !c->d.hosing3d) miss = "hosing_flux3D";
+        if (id == 304 && !c->d.hosing_depth && !c->d.hosing) miss = "hosing_flux";
+        if (miss)
+            return fgpu_fail(FGPU_ERR_STATE, "tracer " + std::to_string(i + 1) + " (ID " + std::to_string(id) +
+                                                 ") needs " + miss + " (fgpu_set_surface_forcing_tracers)");
+    }
     return FGPU_OK;
 }
 
@@ -747,10 +800,11 @@ extern "C" int fgpu_tracer_config(fgpu_context *c, int tr_num, int ID, int AB_or
     if (AB_order != 2 && AB_order != 3)
         return fgpu_fail(FGPU_ERR_ARG, "Adams-Bashfort tracer order must be 2 or 3");
     switch (ID) {   // bc_surface IDs of a plain build (oce_ale_tracer.F90:1862-2098)
-    case 1: case 2: case 100: case 301: case 302: case 303: case 501: break;
+    case 1: case 2: case 100: case 101: case 102: case 103: case 104: case 301: case 302: case 303: case 304:
+    case 501: break;
     default:
         return fgpu_fail(FGPU_ERR_BC, "invalid ID " + std::to_string(ID) +
-                                          " specified in boundary conditions (supported: 1, 2, 100, 301-303, 501)");
+                                          " specified in boundary conditions (supported: 1, 2, 100-104, 301-304, 501)");
     }
     const int i = tr_num - 1;
     FGPU_HIP(hipSetDevice(c->device));

@@ -11,6 +11,7 @@
 //   Iterates, residual and dot products are FP64.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "fgpu_internal.h"
 
@@ -121,10 +122,12 @@ struct CGState {
     double s_oldp[4];   // k_cg1: gamma and alpha of the previous iteration, per parity
     double s_old[2];
     double rtol, rr_last;
+    double soltol;      // stopping tolerance and iteration cap of the solve in flight, written by the
+    int maxiter, pad2;  // init kernel: one captured iteration graph serves every (soltol, maxiter)
 };
 
 struct CGArgs {
-    int n, variant, maxiter, nb;
+    int n, variant, nb;
     const int *rowptr, *colind;
     const double *val, *pr, *dinv;
     double *x;
@@ -133,7 +136,6 @@ struct CGArgs {
     double *w1, *s1;   // second buffers of w and s (single-reduction iteration)
     double *part;   // 8 slots x nb: [0] b.b / p.q  [1] r.z  [2] r.r; single-reduction: [2p..2p+1], [4+p], [6]
     CGState *st;
-    double soltol;
 };
 
 #define SGB 8   // row entries whose column loads and gathers are in flight together
@@ -211,7 +213,7 @@ __device__ __forceinline__ double apply_m(const CGArgs &A, int row)
 }
 
 // r = b - A x; (Jacobi) z = D^-1 r; partials b.b and r.z; state reset
-__global__ void __launch_bounds__(SBLK) k_cg_init(CGArgs A)
+__global__ void __launch_bounds__(SBLK) k_cg_init(CGArgs A, double soltol, int maxiter)
 {
     double bb = 0.0, rz = 0.0, dum = 0.0;
     for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
@@ -232,6 +234,8 @@ __global__ void __launch_bounds__(SBLK) k_cg_init(CGArgs A)
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         CGState s{};
+        s.soltol = soltol;
+        s.maxiter = maxiter;
         *A.st = s;
     }
 }
@@ -261,7 +265,7 @@ __global__ void __launch_bounds__(SBLK) k_cg_a(CGArgs A)
     if (first) {
         double bb, rz;
         sum_partials(A, 0, true, bb, rz);
-        const double rtol = A.soltol * sqrt(bb / (double)A.n);
+        const double rtol = st->soltol * sqrt(bb / (double)A.n);
         s_new = rz;
         if (blockIdx.x == 0 && threadIdx.x == 0) { st->rtol = rtol; st->rr_last = 0.0; }
     } else {
@@ -282,7 +286,7 @@ __global__ void __launch_bounds__(SBLK) k_cg_a(CGArgs A)
         be = rz / s_prev;
         s_new = rz;
     }
-    if (it >= A.maxiter) {
+    if (it >= st->maxiter) {
         if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
         return;
     }
@@ -392,8 +396,8 @@ __device__ __forceinline__ double cg1_slot(const CGArgs &A, int slot)
     return x;
 }
 
-// r0 = b - A x0; partials b.b (slot 6) and r.r (slot 4)
-__global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A)
+// r0 = b - A x0; partials b.b (slot 6) and r.r (slot 4); state reset
+__global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A, double soltol, int maxiter)
 {
     double bb = 0.0, rr = 0.0, dum = 0.0;
     for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
@@ -410,6 +414,8 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A)
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         CGState s{};
+        s.soltol = soltol;
+        s.maxiter = maxiter;
         *A.st = s;
     }
 }
@@ -466,7 +472,7 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A, int par)
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     double rtol;
     if (it == 0) {
-        rtol = A.soltol * sqrt(cg1_slot(A, 6) / (double)A.n);
+        rtol = st->soltol * sqrt(cg1_slot(A, 6) / (double)A.n);
         if (lead) st->rtol = rtol;
     } else {
         rtol = st->rtol;
@@ -476,7 +482,7 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A, int par)
         if (lead) st->done = 1;
         return;
     }
-    if (it >= A.maxiter) {
+    if (it >= st->maxiter) {
         if (lead) st->done = 3;
         return;
     }
@@ -704,12 +710,30 @@ extern "C" int fgpu_ssh_solve_preconditioner(fgpu_context *c, int variant)
     return FGPU_OK;
 }
 
-static CGArgs cg_args(fgpu_context *c, double soltol, int maxiter)
+// a preconditioner built by the caller (mesh%ssh_stiff%pr_values of
+// ssh_solve_preconditioner) in place of the device's build; the diagonal of a
+// variant-2 (Jacobi) M^-1 is taken from the first entry of each row
+extern "C" int fgpu_ssh_set_preconditioner(fgpu_context *c, int variant, const double *pr_values)
+{
+    if (!c || !pr_values) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c->ssh_ready) return fgpu_fail(FGPU_ERR_STATE, "SSH matrix not set");
+    if (variant < 0 || variant > 2) return fgpu_fail(FGPU_ERR_ARG, "precond_variant must be 0, 1 or 2");
+    FGPU_HIP(hipSetDevice(c->device));
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    FGPU_HIP(hipMemcpy(c->ssh_pr, pr_values, sizeof(double) * c->ssh_nnz, hipMemcpyHostToDevice));
+    std::vector<double> dinv(c->ssh_n);
+    for (int r = 0; r < c->ssh_n; ++r) dinv[r] = pr_values[c->h_ssh_rowptr[r]];
+    FGPU_HIP(hipMemcpy(c->ssh_diag_inv, dinv.data(), sizeof(double) * c->ssh_n, hipMemcpyHostToDevice));
+    c->ssh_variant = variant;
+    c->ssh_pr_ready = true;
+    return FGPU_OK;
+}
+
+static CGArgs cg_args(fgpu_context *c)
 {
     CGArgs A;
     A.n = c->ssh_n;
     A.variant = c->ssh_variant;
-    A.maxiter = maxiter;
     A.nb = c->ssh_blocks;
     A.rowptr = c->ssh_rowptr;
     A.colind = c->ssh_colind;
@@ -727,7 +751,6 @@ static CGArgs cg_args(fgpu_context *c, double soltol, int maxiter)
     A.s1 = c->ssh_s1;
     A.part = c->ssh_part;
     A.st = (CGState *)c->ssh_out;
-    A.soltol = soltol;
     return A;
 }
 
@@ -754,12 +777,14 @@ static int enqueue_iters(fgpu_context *c, const CGArgs &A, int iters)
     return FGPU_OK;
 }
 
-// graph of CG_GRAPH_ITERS iterations for the current (soltol, maxiter, variant)
+// graph of CG_GRAPH_ITERS iterations for the current variant. The tolerance
+// and the iteration cap are read from the device state, which the init kernel
+// of each solve writes, so the graph is captured once per variant and never
+// replaced between solves that differ only in (soltol, maxiter).
 static int cg_graph(fgpu_context *c, const CGArgs &A)
 {
     const int key = A.variant + (c->ssh_single_reduction ? 16 : 0);
-    if (c->ssh_graph && c->ssh_graph_tol == A.soltol && c->ssh_graph_maxiter == A.maxiter && c->ssh_graph_variant == key)
-        return FGPU_OK;
+    if (c->ssh_graph && c->ssh_graph_variant == key) return FGPU_OK;
     if (c->ssh_graph) { (void)hipGraphExecDestroy(c->ssh_graph); c->ssh_graph = nullptr; }
     hipGraph_t g;
     FGPU_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -770,8 +795,6 @@ static int cg_graph(fgpu_context *c, const CGArgs &A)
     e = hipGraphInstantiate(&c->ssh_graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    c->ssh_graph_tol = A.soltol;
-    c->ssh_graph_maxiter = A.maxiter;
     c->ssh_graph_variant = key;
     return FGPU_OK;
 }
@@ -781,16 +804,16 @@ static int cg_graph(fgpu_context *c, const CGArgs &A)
 // flag between batches; kernels after convergence return immediately.
 static int launch_cg(fgpu_context *c, double soltol, int maxiter, int variant)
 {
-    CGArgs A = cg_args(c, soltol, maxiter);
+    CGArgs A = cg_args(c);
     A.variant = variant;
     int rc = cg_graph(c, A);
     if (rc) return rc;
     const dim3 g(A.nb), b(SBLK);
     if (A.variant == 2 && c->ssh_single_reduction) {
-        hipLaunchKernelGGL(k_cg1_init_r, g, b, 0, c->stream, A);
+        hipLaunchKernelGGL(k_cg1_init_r, g, b, 0, c->stream, A, soltol, maxiter);
         hipLaunchKernelGGL(k_cg1_init_w, g, b, 0, c->stream, A);
     } else {
-        hipLaunchKernelGGL(k_cg_init, g, b, 0, c->stream, A);
+        hipLaunchKernelGGL(k_cg_init, g, b, 0, c->stream, A, soltol, maxiter);
         if (A.variant != 2) hipLaunchKernelGGL(k_cg_init_z, g, b, 0, c->stream, A);
     }
     FGPU_HIP(hipGetLastError());

@@ -483,7 +483,9 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
 // K_C: FCT bounds, P+/P-, R+/R- per node column (oce_adv_tra_fct.F90:119-405)
 // The element bound (max over the 3 nodes of max(LO, T^n)), maximised over the
 // elements around the node, equals the maximum over the node and those edge
-// neighbours m for which an element adjacent to edge (n, m) has the level.
+// neighbours m for which an element adjacent to edge (n, m) has the level,
+// together with the padding -1e3 (max) / +1e3 (min) of every element around
+// the node that lacks the level (oce_adv_tra_fct.F90:148-178).
 // ------------------------------------------------------------------------------
 template <int LB>
 __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt, int tpb)
@@ -502,6 +504,8 @@ __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt
         hn = m.hnode_new[ci];
         av_ = m.areasvol[(size_t)n * NL + k - 1];
     }
+    const int2 pr = *(const int2 *)(m.fct_pad + 2 * (size_t)n);
+    const bool padded = k < pr.x || k >= pr.y;   // an element around n lacks level k
     const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
     for (int tr = trb; tr < tre; ++tr) {
         if (t.lim[tr] != LIM_FCT) continue;   // uniform over the block
@@ -514,6 +518,10 @@ __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt
             const double tk = T[ci];
             vmax = dmx(lo, tk);
             vmin = dmn(lo, tk);
+            if (padded) {
+                vmax = dmx(vmax, -1e3);
+                vmin = dmn(vmin, 1e3);
+            }
             const double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
             p = p + (dmx(0.0, av[k - 1]) + dmx(0.0, -av[k]));
             q = q + (dmn(0.0, av[k - 1]) + dmn(0.0, -av[k]));
@@ -580,17 +588,6 @@ __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt
 // ------------------------------------------------------------------------------
 // K_D: limited fluxes, flux2dtracer, T*, implicit vertical diffusion
 // ------------------------------------------------------------------------------
-__device__ __forceinline__ double bc_surface_d(int id, double dt, const DevDyn &d, int n, int nzmin, double sval)
-{
-    switch (id) {
-    case 1: return -dt * (d.heat_flux[n] / FGPU_VCPW + sval * d.water_flux[n] * d.is_nonlinfs);
-    case 2: return dt * (d.virtual_salt[n] + d.relax_salt[n] + d.real_salt_flux[n] * d.is_nonlinfs);
-    // ice-shelf water from basal melting, under a cavity only (oce_ale_tracer.F90:2080-2085)
-    case 501: return nzmin == 1 ? 0.0 : dt * d.water_flux[n] * (sval - 1.0);
-    default: return 0.0;
-    }
-}
-
 // TCH: tracers per Thomas chunk; CPB: columns per block (one column per LB
 // lanes). The Thomas sweeps of all CPB x TCH (column, tracer) pairs of a block
 // run in one wave, one lane per pair, while the other waves wait: a serial
@@ -816,7 +813,8 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
             if (k == nzmin) r = -(rb[tid] - dz) * ts[tid] - rc[tid] * ts[tid + 1];
             else if (k <= nzmax - 2) r = -ra[tid] * ts[tid - 1] - (rb[tid] - dz) * ts[tid] - rc[tid] * ts[tid + 1];
             else r = -ra[tid] * ts[tid - 1] - (rb[tid] - dz) * ts[tid];
-            if (k == nzmin) r = r + bc_surface_d(t.id[tr], dt, d, n, nzmin, ts[tid]);
+            if (bc_every_layer(t.id[tr], d)) r = r + bc_surface_dev(t.id[tr], dt, d, m.N2, L, n, k, ts[tid]);
+            else if (k == nzmin) r = r + bc_surface_dev(t.id[tr], dt, d, m.N2, L, n, nzmin, ts[tid]);
             stpA[w][c][tid] = r;
         }
         __syncthreads();

@@ -648,17 +648,6 @@ __global__ void __launch_bounds__(BLK) k_tstar(DevMesh m, DevDyn d, DevTracers t
 // (:1828-2108), then the salinity clamp of solve_tracers_ale (:502-514) for
 // the second tracer. Redi off (isredi = 0), no w-split implicit advection.
 // ------------------------------------------------------------------------------
-__device__ __forceinline__ double bc_surface(int id, double dt, const DevDyn &d, int n, int nzmin, double sval)
-{
-    switch (id) {
-    case 1: return -dt * (d.heat_flux[n] / FGPU_VCPW + sval * d.water_flux[n] * d.is_nonlinfs);
-    case 2: return dt * (d.virtual_salt[n] + d.relax_salt[n] + d.real_salt_flux[n] * d.is_nonlinfs);
-    // ice-shelf water from basal melting, under a cavity only (oce_ale_tracer.F90:2080-2085)
-    case 501: return nzmin == 1 ? 0.0 : dt * d.water_flux[n] * (sval - 1.0);
-    default: return 0.0;
-    }
-}
-
 template <int MAXL>
 __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t, int tr0, double dt, int do_diff,
                                               int clamp_tracer)
@@ -717,7 +706,8 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
         double zinv1 = zinv2;
         double Tm = 0.0, Tk = T[I(nz)], Tp = T[I(nz + 1)];
         double tr_ = -(b - hn[I(nz)]) * Tk - c * Tp;
-        tr_ = tr_ + bc_surface(t.id[tr], dt, d, n, nzmin, Tk);
+        const bool hall = bc_every_layer(t.id[tr], d);
+        tr_ = tr_ + bc_surface_dev(t.id[tr], dt, d, m.N2, m.L, n, nzmin, Tk);
         cp[I(nz)] = c / b;
         tp[I(nz)] = tr_ / b;
         for (nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
@@ -744,6 +734,7 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
                 }
                 Tp = T[I(nz + 1)];
                 tr_ = -a * Tm - (b - hn[I(nz)]) * Tk - c * Tp;
+                if (hall) tr_ = tr_ + bc_surface_dev(t.id[tr], dt, d, m.N2, m.L, n, nz, Tk);
             } else {
                 a = redi ? -(kv[I(nz)] + TY_ABOVE(nz, zinv1)) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)])
                          : -(kv[I(nz)]) * zinv1 * zinv * (ar[I(nz)] / avl[I(nz)]);
@@ -755,6 +746,7 @@ __global__ void __launch_bounds__(64) k_vdiff(DevMesh m, DevDyn d, DevTracers t,
                     b = b + dmx(0.0, wi[I(nz)]) * v_adv;
                 }
                 tr_ = -a * Tm - (b - hn[I(nz)]) * Tk;
+                if (hall) tr_ = tr_ + bc_surface_dev(t.id[tr], dt, d, m.N2, m.L, n, nz, Tk);
             }
             const double mm = b - cp[I(nz - 1)] * a;
             cp[I(nz)] = c / mm;
@@ -938,6 +930,27 @@ extern "C" int fgpu_init_tracers_AB(fgpu_context *c, int tr_num)
     return FGPU_OK;
 }
 
+// valuesAB of a tracer computed by the caller's init_tracers_AB; the element
+// gradients of values (tr_xy, oce_tracer_mod.F90:126-131), which the same
+// routine leaves for fill_up_dn_grad, are recomputed from the resident values
+extern "C" int fgpu_tracer_set_valuesAB(fgpu_context *c, int tr_num, const double *valuesAB)
+{
+    if (!c || !valuesAB) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c->mesh_ready) return fgpu_fail(FGPU_ERR_STATE, "mesh not initialised");
+    if (!c->tracers_ready) return fgpu_fail(FGPU_ERR_STATE, "tracers not initialised");
+    int rc;
+    if ((rc = check_tracer(c, tr_num))) return rc;
+    FGPU_HIP(hipSetDevice(c->device));
+    if ((rc = fgpu_ensure_staged(c, 1))) return rc;
+    const int tr = tr_num - 1;
+    FGPU_HIP(hipMemcpyAsync(c->t.TAB + c->t.sN * tr, valuesAB, c->t.sN * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    hipLaunchKernelGGL(k_tr_xy, dim3(nblk((size_t)c->m.F2 * c->m.L), 1), dim3(BLK), 0, c->stream, c->m, c->t, tr);
+    LAUNCH_CHECK("k_tr_xy");
+    FGPU_HIP(hipStreamSynchronize(c->stream));
+    return FGPU_OK;
+}
+
 extern "C" int fgpu_do_oce_adv_tra(fgpu_context *c, double dt, int tr_num)
 {
     int rc = check_ready(c);
@@ -963,6 +976,7 @@ extern "C" int fgpu_diff_tracers_ale(fgpu_context *c, double dt, int tr_num)
     // implicit or explicit vertical diffusion: Kv and the surface forcing either way
     if (!c->kv_ready) return fgpu_fail(FGPU_ERR_STATE, "Kv not set (fgpu_set_kv)");
     if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "surface forcing not set");
+    if ((rc = fgpu_check_bc_inputs(c, tr, 1))) return rc;
     if ((rc = fgpu_ensure_diff(c))) return rc;
     if ((rc = fgpu_launch_diff(c, tr, 1, dt))) return rc;
     hipLaunchKernelGGL(k_tstar, dim3(nblk(c->t.sN)), dim3(BLK), 0, c->stream, c->m, c->d, c->t, tr, dt);
@@ -982,6 +996,7 @@ extern "C" int fgpu_diff_ver_part_impl_ale(fgpu_context *c, double dt, int tr_nu
     if (!c->kv_ready) return fgpu_fail(FGPU_ERR_STATE, "Kv not set (fgpu_set_kv)");
     if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "surface forcing not set");
     const int tr = tr_num - 1;
+    if ((rc = fgpu_check_bc_inputs(c, tr, 1))) return rc;
     // diff_ver_part_impl_ale runs regardless of i_vert_diff when called directly
     if ((rc = launch_vdiff(c, tr, 1, dt, 2, -1))) return rc;
     FGPU_HIP(hipStreamSynchronize(c->stream));
@@ -1007,7 +1022,7 @@ static int check_step(fgpu_context *c)
         if (!c->h_configured[i]) return fgpu_fail(FGPU_ERR_STATE, "tracer " + std::to_string(i + 1) + " not configured");
     if (!c->kv_ready) return fgpu_fail(FGPU_ERR_STATE, "Kv not set (fgpu_set_kv)");
     if (!c->forcing_ready) return fgpu_fail(FGPU_ERR_STATE, "surface forcing not set");
-    return FGPU_OK;
+    return fgpu_check_bc_inputs(c, 0, c->t.ntr);
 }
 
 static int run_steps(fgpu_context *c, double dt, int nsteps)

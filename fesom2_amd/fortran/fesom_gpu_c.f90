@@ -37,7 +37,8 @@ module fesom_gpu_c
        fgpu_ssh_nn_set_weights, fgpu_ssh_nn_get_weights, fgpu_ssh_nn_apply, fgpu_ssh_nn_loss_grad, &
        fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident, fgpu_set_uv, &
        fgpu_vert_vel_ale, fgpu_vert_vel_download, fgpu_compute_ssh_rhs_ale, fgpu_update_stiff_mat_ale, &
-       fgpu_compute_hbar_ale, fgpu_set_ale_config
+       fgpu_compute_hbar_ale, fgpu_set_ale_config, fgpu_set_surface_forcing_tracers, &
+       fgpu_tracer_set_valuesAB, fgpu_ssh_set_preconditioner
 
   interface
      type(c_ptr) function fgpu_last_error() bind(C, name="fgpu_last_error")
@@ -117,6 +118,29 @@ module fesom_gpu_c
             real_salt_flux(*)
        real(c_double), value :: is_nonlinfs
      end function fgpu_set_surface_forcing
+
+     ! arrays by address: c_null_ptr leaves one unset
+     integer(c_int) function fgpu_set_surface_forcing_tracers(ctx, prec_rain, wiso_flux_oce, hosing_flux, &
+          hosing_flux3D, hosing_depth) bind(C, name="fgpu_set_surface_forcing_tracers")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, prec_rain, wiso_flux_oce, hosing_flux, hosing_flux3D
+       integer(c_int), value :: hosing_depth
+     end function fgpu_set_surface_forcing_tracers
+
+     integer(c_int) function fgpu_tracer_set_valuesAB(ctx, tr_num, valuesAB) bind(C, name="fgpu_tracer_set_valuesAB")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: tr_num
+       real(c_double), intent(in) :: valuesAB(*)
+     end function fgpu_tracer_set_valuesAB
+
+     integer(c_int) function fgpu_ssh_set_preconditioner(ctx, precond_variant, pr_values) &
+          bind(C, name="fgpu_ssh_set_preconditioner")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: precond_variant
+       real(c_double), intent(in) :: pr_values(*)
+     end function fgpu_ssh_set_preconditioner
 
      integer(c_int) function fgpu_set_uv(ctx, uv) bind(C, name="fgpu_set_uv")
        import :: c_int, c_ptr, c_double

@@ -7,8 +7,11 @@ library is 1-based, exactly as the Fortran derived types hold it.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -50,6 +53,9 @@ _SIGS = {
     "fgpu_set_kv": [vp, dp],
     "fgpu_set_ki": [vp, dp, dp, C.c_int],
     "fgpu_set_surface_forcing": [vp, dp, dp, dp, dp, dp, C.c_double],
+    "fgpu_set_surface_forcing_tracers": [vp, dp, dp, dp, dp, C.c_int],
+    "fgpu_tracer_set_valuesAB": [vp, C.c_int, dp],
+    "fgpu_ssh_set_preconditioner": [vp, C.c_int, dp],
     "fgpu_set_uv": [vp, dp],
     "fgpu_vert_vel_ale": [vp, C.c_double, C.c_int, dp, dp, C.c_int, C.c_double, dp, ip],
     "fgpu_vert_vel_download": [vp, dp, dp, dp, dp, dp],
@@ -97,10 +103,22 @@ _SIGS = {
 EXPORTED = sorted(list(_SIGS) + ["fgpu_last_error", "fgpu_version"])
 
 _lib = None
+_live = weakref.WeakSet()   # open contexts, closed by an exit hook before the HIP runtime tears down
+
+
+def _close_live_contexts():
+    for ctx in list(_live):
+        try:
+            ctx.close()
+        except Exception:
+            pass
 
 
 def load(path: Optional[str] = None):
-    """Load the library (raises if it is not built)."""
+    """Load the library (raises if it is not built). The first load registers
+    an exit hook that closes every context still open, while the HIP runtime
+    is alive; a context collected later, during interpreter shutdown, releases
+    nothing itself."""
     global _lib
     if _lib is not None:
         return _lib
@@ -116,6 +134,7 @@ def load(path: Optional[str] = None):
         f.argtypes = args
         f.restype = C.c_int
     _lib = L
+    atexit.register(_close_live_contexts)
     return L
 
 
@@ -154,6 +173,7 @@ class Context:
         _check(L.fgpu_create(C.byref(h), int(device)))
         self.h = h
         self.L = L
+        _live.add(self)
         self.ntr = 0
         self.nl = 0
         self.N2 = 0
@@ -163,8 +183,11 @@ class Context:
         if self.h:
             self.L.fgpu_destroy(self.h)
             self.h = None
+        _live.discard(self)
 
     def __del__(self):
+        if sys.is_finalizing():   # the exit hook has closed it, or the runtime may be gone
+            return
         try:
             self.close()
         except Exception:
@@ -220,6 +243,16 @@ class Context:
         self.set_surface_forcing(st.heat_flux, st.water_flux, st.virtual_salt, st.relax_salt, st.real_salt_flux)
         if getattr(st, "Ki", None) is not None:
             self.set_ki(st.Ki, getattr(st, "slope_tapered", None), getattr(st, "redi", False))
+        extra = [getattr(st, f, None) for f in ("prec_rain", "wiso_flux_oce", "hosing_flux", "hosing_flux3D")]
+        if any(a is not None for a in extra):
+            self.set_surface_forcing_tracers(*extra, hosing_depth=getattr(st, "hosing_depth", False))
+
+    def set_surface_forcing_tracers(self, prec_rain=None, wiso_flux_oce=None, hosing_flux=None, hosing_flux3D=None,
+                                    hosing_depth=False):
+        """Fluxes of the tracer IDs 101 (prec_rain, N2), 102-104 (wiso_flux_oce, (3, N2): the Fortran
+        (N2, 3) array) and 304 (hosing_flux, N2; hosing_flux3D, (N2, nl-1))."""
+        keep = [None if a is None else _f64(a) for a in (prec_rain, wiso_flux_oce, hosing_flux, hosing_flux3D)]
+        _check(self.L.fgpu_set_surface_forcing_tracers(self.h, *[_d(a) for a in keep], int(bool(hosing_depth))))
 
     # ------------------------------------------- dynamics either side of the transport
     ALE = {"linfs": 0, "zlevel": 1, "zstar": 2}

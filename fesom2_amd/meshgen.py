@@ -130,6 +130,12 @@ class State:
     Ki: Optional[np.ndarray] = None  # (N2, nl-1) horizontal/isoneutral diffusivity; None: Ki = 0
     slope_tapered: Optional[np.ndarray] = None  # (N2, nl-1, 3) neutral slopes, read when redi
     redi: bool = False               # Redi (oce_modules.F90:75)
+    # surface fluxes of the tracer IDs 101-104 and 304 (bc_surface, oce_ale_tracer.F90:2059-2079)
+    prec_rain: Optional[np.ndarray] = None      # (N2,) ID 101
+    wiso_flux_oce: Optional[np.ndarray] = None  # (3, N2): Fortran wiso_flux_oce(N2, 3), IDs 102-104
+    hosing_flux: Optional[np.ndarray] = None    # (N2,) ID 304
+    hosing_flux3D: Optional[np.ndarray] = None  # (N2, nl-1) ID 304 when hosing_depth
+    hosing_depth: bool = False                  # use_hosing .and. hosing_mode == 'depth'
 
     @property
     def ntr(self) -> int:
@@ -843,6 +849,12 @@ CONFIGS: Dict[str, dict] = {
                             topo="bumps", bump_height=1500.0,
                             cavity=dict(frac=0.33, draft=800.0)),
                   state=dict(ntr=2, dt=900.0, kv=1e-5), steps=10),
+    # 40x30 channel with 70 layers (8-70 per column, stretched from 5 m): the
+    # 128-lane kernels of the fused step on a mesh the oracle runs in seconds
+    "c0deep": dict(mesh=dict(nx=40, ny=30, dx=20e3, jitter=0.2, seed=9, cartesian=True,
+                             nlayers=70, hmax=6250.0, dz0=5.0, min_layers=5, max_layers=70,
+                             topo="bumps", bump_height=6200.0),
+                   state=dict(ntr=12, dt=600.0, kv_range=(1e-5, 1e-2)), steps=2),
     # 126,858-node latitude band, 47 layers (PI-like stretching), T/S, dt=1800 s
     "c1": dict(mesh=dict(cartesian=False, target_nodes=126858, jitter=0.2, seed=1,
                          nlayers=47, hmax=6250.0, dz0=10.0, min_layers=5,

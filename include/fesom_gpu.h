@@ -98,6 +98,17 @@ int fgpu_set_surface_forcing(fgpu_context *ctx, const double *heat_flux, const d
                              const double *virtual_salt, const double *relax_salt,
                              const double *real_salt_flux, double is_nonlinfs);
 
+/* Per-node surface fluxes of the tracer IDs 101-104 and 304 in bc_surface
+ * (oce_ale_tracer.F90:2059-2079): prec_rain(nod2D) (g_forcing_arrays, ID 101);
+ * wiso_flux_oce(nod2D,3) (o_ARRAYS, node index fastest; IDs 102, 103, 104 take
+ * columns 1, 2, 3); hosing_flux(nod2D) and hosing_flux3D(nl-1,nod2D) (o_ARRAYS,
+ * ID 304: hosing_depth = use_hosing .and. hosing_mode == 'depth' takes
+ * hosing_flux3D at the column's first wet level, else hosing_flux). A NULL
+ * array is left as it was; a step whose tracers need an array never set fails
+ * with FGPU_ERR_STATE. */
+int fgpu_set_surface_forcing_tracers(fgpu_context *ctx, const double *prec_rain, const double *wiso_flux_oce,
+                                     const double *hosing_flux, const double *hosing_flux3D, int hosing_depth);
+
 /* ---- the steps either side of the transport -------------------------------
  * which_ALE of g_config (gen_modules_config.F90): */
 #define FGPU_ALE_LINFS 0
@@ -163,8 +174,10 @@ int fgpu_compute_hbar_ale(fgpu_context *ctx, double dt, int which_ale, const dou
  * lim in {"FCT","NON"} (trailing blanks ignored), tra_adv_ph/pv, AB_order 2|3,
  * i_vert_diff. tr_num is 1-based like the reference. ID selects bc_surface
  * (oce_ale_tracer.F90:1862-2098): 1 temperature, 2 salinity, 100 and 301-303
- * passive (no surface flux), 501 ice-shelf water (dt*water_flux*(T-1) under a
- * cavity); any other ID fails with FGPU_ERR_BC, as the reference stops. */
+ * passive (no surface flux), 101 (prec_rain), 102-104 (wiso_flux_oce) and 304
+ * (hosing) with the fluxes of fgpu_set_surface_forcing_tracers, 501 ice-shelf
+ * water (dt*water_flux*(T-1) under a cavity); any other ID fails with
+ * FGPU_ERR_BC, as the reference stops. */
 int fgpu_tracers_init(fgpu_context *ctx, int num_tracers);
 int fgpu_tracer_config(fgpu_context *ctx, int tr_num, int ID, int AB_order, const char *tra_adv_hor,
                        const char *tra_adv_ver, const char *tra_adv_lim, double tra_adv_ph,
@@ -179,6 +192,13 @@ int fgpu_tracer_download(fgpu_context *ctx, int tr_num, double *values, double *
 int fgpu_tracer_work_download(fgpu_context *ctx, int tr_num, const char *name, double *out, long n);
 /* edge_up_dn_tri(2,edge2D), 1-based, 0 = none */
 int fgpu_edge_up_dn_tri(fgpu_context *ctx, int *out);
+
+/* valuesAB(nl-1,nod2D) of tracer tr_num as the caller's init_tracers_AB left
+ * it (oce_tracer_mod.F90:44-74); the element gradients of the resident values
+ * (tracer_gradient_elements, oce_tracer_mod.F90:126-131) are recomputed, so
+ * fgpu_do_oce_adv_tra then sees what the reference's do_oce_adv_tra sees
+ * after init_tracers_AB on the host. */
+int fgpu_tracer_set_valuesAB(fgpu_context *ctx, int tr_num, const double *valuesAB);
 
 /* ---- per-tracer entry points (reference call surface) --------------------- */
 /* init_tracers_AB (oce_tracer_mod.F90:13-143) */
@@ -237,6 +257,10 @@ int fgpu_ssh_get_matrix(fgpu_context *ctx, int *rowptr, int *colind, double *val
 /* ssh_solve_preconditioner (solver.F90:31-117): variant 0 and 1 as in the
  * reference, 2 = diagonal (Jacobi) only. */
 int fgpu_ssh_solve_preconditioner(fgpu_context *ctx, int precond_variant);
+/* mesh%ssh_stiff%pr_values(nza) as built by the caller's ssh_solve_preconditioner
+ * (solver.F90:31-117, same sparsity as the matrix) with its precond_variant;
+ * replaces the device build of fgpu_ssh_solve_preconditioner. */
+int fgpu_ssh_set_preconditioner(fgpu_context *ctx, int precond_variant, const double *pr_values);
 /* ssh_solve_cg (solver.F90:120-385): x (nod2D) in/out (warm start), rhs (nod2D). */
 int fgpu_ssh_solve_cg(fgpu_context *ctx, double *x, const double *rhs, double soltol, int maxiter,
                       fgpu_cg_stats *stats);

@@ -56,12 +56,26 @@
  *   44 UPW1 bottom segment (E) takes the first element's velocity and thickness (adv_tra_hor_upw1)
  *   45 AB3 swaps the weights of the two old levels                 (init_tracers_AB)
  *   46 ice-shelf meltwater (ID 501) enters open-ocean columns too   (bc_surface)
+ *   47 the isotope IDs 102 and 104 take each other's flux column   (bc_surface)
+ *   48 depth-mode hosing (ID 304) reads the layer below nzmin     (bc_surface)
+ *   49 the node bounds leave out elements without the level (no +-1e3 padding) (oce_tra_adv_fct)
  */
 #include "fesom_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* The OpenMP build (libfesom_oracle_omp.so, the multi-core CPU baseline) runs
+ * the column, edge and element loops in parallel and replaces each edge->node
+ * scatter by a gather over the node's edges in ascending edge order, which
+ * adds the same terms in the same order: it is bit-identical to the serial
+ * build, which a test checks. The planted mistakes are for the serial build. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
 
 static int g_mut = 0;
 void fo_set_mutation(int m) { g_mut = m; }
@@ -147,6 +161,7 @@ int fo_find_up_downwind_triangles(const fo_mesh *m, int *up_dn)
 void fo_tracer_gradient_elements(const fo_mesh *m, const double *ttf, double *tr_xy)
 {
     const int L = m->nl - 1;
+    OMP_FOR
     for (int el = 0; el < m->elem2D; ++el) {
         const int *en = m->elem2D_nodes + 3 * (size_t)el;
         const double *gs = m->gradient_sca + 6 * (size_t)el;
@@ -183,6 +198,7 @@ static void node_mean_grad(const fo_mesh *m, const double *tr_xy, int node, int 
 void fo_fill_up_dn_grad(const fo_mesh *m, const double *tr_xy, double *grad)
 {
     const int L = m->nl - 1;
+    OMP_FOR
     for (int e = 0; e < m->edge2D; ++e) {
         int n1 = m->edges[2 * e], n2 = m->edges[2 * e + 1];
         int up = m->edge_up_dn_tri[2 * e], dn = m->edge_up_dn_tri[2 * e + 1];
@@ -244,9 +260,11 @@ int fo_init_tracers_AB(const fo_mesh *m, fo_tracer *t, fo_work *w)
     memset(w->del_ttf_advvert, 0, n3 * sizeof(double));
     if (t->AB_order == 2) {
         double c1 = (g_mut == 1) ? 1.5 : (1.5 + EPS_AB);
-        for (size_t i = 0; i < n3; ++i)
+        OMP_FOR
+        for (size_t i = 0; i < n3; ++i) {
             t->valuesAB[i] = -(0.5 + EPS_AB) * t->valuesold[i] + c1 * t->values[i];
-        for (size_t i = 0; i < n3; ++i) t->valuesold[i] = t->values[i];
+            t->valuesold[i] = t->values[i];
+        }
     } else if (t->AB_order == 3) {
         for (size_t i = 0; i < n3; ++i) {
             double v = (g_mut == 45) ? 5.0 * t->valuesold[2 * i] - 16.0 * t->valuesold[2 * i + 1] + 23.0 * t->values[i]
@@ -317,6 +335,7 @@ void fo_adv_tra_hor_upw1(const fo_mesh *m, const double *vel, const double *ttf,
 {
     const int L = m->nl - 1;
     if (init_zero) memset(flux, 0, (size_t)m->edge2D * L * sizeof(double));
+    OMP_FOR
     for (int e = 0; e < m->edge2D; ++e) {
         edge_info q = get_edge(m, e);
         double dX1 = m->edge_cross_dxdy[4 * e + 0], dY1 = m->edge_cross_dxdy[4 * e + 1];
@@ -368,6 +387,7 @@ static void hor_mfct_muscl(const fo_mesh *m, const double *vel, const double *tt
     const int L = m->nl - 1;
     const double w2 = (g_mut == 6) ? 3.0 : 2.0;
     if (init_zero) memset(flux, 0, (size_t)m->edge2D * L * sizeof(double));
+    OMP_FOR
     for (int e = 0; e < m->edge2D; ++e) {
         edge_info q = get_edge(m, e);
         double dX1 = m->edge_cross_dxdy[4 * e + 0], dY1 = m->edge_cross_dxdy[4 * e + 1];
@@ -459,6 +479,7 @@ void fo_adv_tra_ver_upw1(const fo_mesh *m, const double *W, const double *ttf, d
 {
     const int L = m->nl - 1, NL = m->nl;
     if (init_zero) memset(flux, 0, (size_t)m->nod2D * NL * sizeof(double));
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
         int nz = nzmin;
@@ -482,6 +503,7 @@ void fo_adv_tra_ver_qr4c(const fo_mesh *m, const double *W, const double *ttf, d
 {
     const int L = m->nl - 1, NL = m->nl;
     if (init_zero) memset(flux, 0, (size_t)m->nod2D * NL * sizeof(double));
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nzmax = m->nlevels_nod2D[n], nzmin = m->ulevels_nod2D[n];
         const double *T = ttf + LAY(n, 1);
@@ -529,12 +551,14 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
     double *tvmax = (double *)calloc((size_t)m->nod2D * L, sizeof(double));
     double *tvmin = (double *)calloc((size_t)m->nod2D * L, sizeof(double));
     /* a1 */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
             fct_ttf_max[LAY(n, nz)] = dmax(lo[LAY(n, nz)], ttf[LAY(n, nz)]);
             fct_ttf_min[LAY(n, nz)] = dmin(lo[LAY(n, nz)], ttf[LAY(n, nz)]);
         }
     /* a2 */
+    OMP_FOR
     for (int el = 0; el < m->elem2D; ++el) {
         const int *en = m->elem2D_nodes + 3 * (size_t)el;
         int nu1 = m->ulevels[el], nl1 = m->nlevels[el];
@@ -554,6 +578,7 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
         }
     }
     /* a3 */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         const int *ne = m->nod_in_elem2D + (size_t)n * m->kmax;
         for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
@@ -563,10 +588,21 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
                 vmax = dmax(vmax, aux[LAY(ne[k], nz) * 2 + 0]);
                 vmin = dmin(vmin, aux[LAY(ne[k], nz) * 2 + 1]);
             }
+            if (g_mut == 49) {   /* the elements without level nz left out */
+                vmax = -INFINITY;
+                vmin = INFINITY;
+                for (int k = 0; k < m->nod_in_elem2D_num[n]; ++k) {
+                    const int el = ne[k];
+                    if (nz < m->ulevels[el] || nz > m->nlevels[el] - 1) continue;
+                    vmax = dmax(vmax, aux[LAY(el, nz) * 2 + 0]);
+                    vmin = dmin(vmin, aux[LAY(el, nz) * 2 + 1]);
+                }
+            }
             tvmax[LAY(n, nz)] = vmax;
             tvmin[LAY(n, nz)] = vmin;
         }
     }
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         fct_ttf_max[LAY(n, nu1)] = tvmax[LAY(n, nu1)] - lo[LAY(n, nu1)];
@@ -582,12 +618,14 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
         fct_ttf_min[LAY(n, nz)] = tvmin[LAY(n, nz)] - lo[LAY(n, nz)];
     }
     /* b1 */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
             fct_plus[LAY(n, nz)] = 0.0;
             fct_minus[LAY(n, nz)] = 0.0;
         }
     if (g_mut != 18) {
+        OMP_FOR
         for (int n = 0; n < m->nod2D; ++n)
             for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
                 fct_plus[LAY(n, nz)] = fct_plus[LAY(n, nz)]
@@ -596,6 +634,22 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
                     + (dmin(0.0, adf_v[INT(n, nz)]) + dmin(0.0, -adf_v[INT(n, nz + 1)]));
             }
     }
+#ifdef _OPENMP
+    OMP_FOR
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int p = m->nod_edge_ptr[n]; p < m->nod_edge_ptr[n + 1]; ++p) {
+            const int e = m->nod_edge_idx[p] >> 1, side = m->nod_edge_idx[p] & 1;
+            edge_info q = get_edge(m, e);
+            int nl12 = imax(q.nl1, q.nl2);
+            int nu12 = q.nu1;
+            if (q.nu2 > 0) nu12 = imin(q.nu1, q.nu2);
+            for (int nz = nu12; nz <= nl12; ++nz) {
+                const double f = side ? -adf_h[LAY(e, nz)] : adf_h[LAY(e, nz)];
+                fct_plus[LAY(n, nz)] = fct_plus[LAY(n, nz)] + dmax(0.0, f);
+                fct_minus[LAY(n, nz)] = fct_minus[LAY(n, nz)] + dmin(0.0, f);
+            }
+        }
+#else
     for (int e = 0; e < m->edge2D; ++e) {
         edge_info q = get_edge(m, e);
         int nl12 = imax(q.nl1, q.nl2);
@@ -610,7 +664,9 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
             fct_minus[LAY(q.n2, nz)] = fct_minus[LAY(q.n2, nz)] + dmin(0.0, -adf_h[LAY(e, nz)]);
         }
     }
+#endif
     /* b2 */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
             double flux = fct_plus[LAY(n, nz)] * dt / m->areasvol[INT(n, nz)] / m->hnode_new[LAY(n, nz)] + flux_eps;
@@ -619,6 +675,7 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
             fct_minus[LAY(n, nz)] = dmin(1.0, fct_ttf_min[LAY(n, nz)] / flux);
         }
     /* b3 vertical */
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n) {
         int nu1 = m->ulevels_nod2D[n], nl1 = m->nlevels_nod2D[n];
         int nz = nu1;
@@ -641,6 +698,7 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
         }
     }
     /* b3 horizontal */
+    OMP_FOR
     for (int e = 0; e < m->edge2D; ++e) {
         edge_info q = get_edge(m, e);
         int nl12 = imax(q.nl1, q.nl2);
@@ -673,15 +731,32 @@ void fo_oce_tra_adv_flux2dtracer(const fo_mesh *m, double dt, double *dttf_h, do
 {
     const int L = m->nl - 1, NL = m->nl;
     if (use_lo) {
+        OMP_FOR
         for (int n = 0; n < m->nod2D; ++n)
             for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz)
                 dttf_v[LAY(n, nz)] = dttf_v[LAY(n, nz)] - ttf[LAY(n, nz)] * m->hnode[LAY(n, nz)]
                                      + lo[LAY(n, nz)] * m->hnode_new[LAY(n, nz)];
     }
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz)
             dttf_v[LAY(n, nz)] = dttf_v[LAY(n, nz)]
                 + (flux_v[INT(n, nz)] - flux_v[INT(n, nz + 1)]) * dt / m->areasvol[INT(n, nz)];
+#ifdef _OPENMP
+    OMP_FOR
+    for (int n = 0; n < m->nod2D; ++n)
+        for (int p = m->nod_edge_ptr[n]; p < m->nod_edge_ptr[n + 1]; ++p) {
+            const int e = m->nod_edge_idx[p] >> 1, side = m->nod_edge_idx[p] & 1;
+            edge_info q = get_edge(m, e);
+            int nl12 = imax(q.nl1, q.nl2);
+            int nu12 = q.nu1;
+            if (q.nu2 > 0) nu12 = imin(q.nu1, q.nu2);
+            for (int nz = nu12; nz <= nl12; ++nz) {
+                if (side) dttf_h[LAY(n, nz)] = dttf_h[LAY(n, nz)] - flux_h[LAY(e, nz)] * dt / m->areasvol[INT(n, nz)];
+                else dttf_h[LAY(n, nz)] = dttf_h[LAY(n, nz)] + flux_h[LAY(e, nz)] * dt / m->areasvol[INT(n, nz)];
+            }
+        }
+#else
     for (int e = 0; e < m->edge2D; ++e) {
         edge_info q = get_edge(m, e);
         int nl12 = imax(q.nl1, q.nl2);
@@ -693,6 +768,7 @@ void fo_oce_tra_adv_flux2dtracer(const fo_mesh *m, double dt, double *dttf_h, do
         for (int nz = nu12; nz <= nl12; ++nz)
             dttf_h[LAY(q.n2, nz)] = dttf_h[LAY(q.n2, nz)] - flux_h[LAY(e, nz)] * dt / m->areasvol[INT(nb, nz)];
     }
+#endif
 }
 
 /* ------------------------------------------------------------------------ */
@@ -885,6 +961,21 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
     if (fct) {
         fo_adv_tra_hor_upw1(m, d->uv, t->values, w->adv_flux_hor, 1);
         memset(w->fct_LO, 0, (size_t)m->nod2D * L * sizeof(double));
+#ifdef _OPENMP
+        OMP_FOR
+        for (int n = 0; n < m->nod2D; ++n)
+            for (int p = m->nod_edge_ptr[n]; p < m->nod_edge_ptr[n + 1]; ++p) {
+                const int e = m->nod_edge_idx[p] >> 1, side = m->nod_edge_idx[p] & 1;
+                edge_info q = get_edge(m, e);
+                int nl12 = imax(q.nl1, q.nl2);
+                int nu12 = q.nu1;
+                if (q.nu2 > 0) nu12 = imin(q.nu1, q.nu2);
+                for (int nz = nu12; nz <= nl12; ++nz) {
+                    if (side) w->fct_LO[LAY(n, nz)] = w->fct_LO[LAY(n, nz)] - w->adv_flux_hor[LAY(e, nz)];
+                    else w->fct_LO[LAY(n, nz)] = w->fct_LO[LAY(n, nz)] + w->adv_flux_hor[LAY(e, nz)];
+                }
+            }
+#else
         for (int e = 0; e < m->edge2D; ++e) {
             edge_info q = get_edge(m, e);
             int nl12 = imax(q.nl1, q.nl2);
@@ -895,7 +986,9 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
                 w->fct_LO[LAY(q.n2, nz)] = w->fct_LO[LAY(q.n2, nz)] - w->adv_flux_hor[LAY(e, nz)];
             }
         }
+#endif
         fo_adv_tra_ver_upw1(m, d->w_e, t->values, w->adv_flux_ver, 1);
+        OMP_FOR
         for (int n = 0; n < m->nod2D; ++n)
             for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz)
                 w->fct_LO[LAY(n, nz)] = (t->values[LAY(n, nz)] * m->hnode[LAY(n, nz)]
@@ -945,6 +1038,7 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t
 void fo_diff_tracers_ale_tstar(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w)
 {
     const int L = m->nl - 1;
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = 1; nz <= L; ++nz)
             w->del_ttf[LAY(n, nz)] = w->del_ttf[LAY(n, nz)] + w->del_ttf_advhoriz[LAY(n, nz)]
@@ -955,6 +1049,7 @@ void fo_diff_tracers_ale_tstar(const fo_mesh *m, const fo_dyn *d, double dt, fo_
     if (d && d->Ki) fo_diff_part_hor_redi(m, d, dt, w->tr_xy, w->tr_z, w->del_ttf);
     if (d && !t->i_vert_diff) fo_diff_ver_part_expl_ale(m, d, dt, t, w->del_ttf);
     if (d && d->redi && d->Ki && d->slope_tapered) fo_diff_ver_part_redi_expl(m, d, dt, w->tr_xy, w->del_ttf);
+    OMP_FOR
     for (int n = 0; n < m->nod2D; ++n)
         for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
             w->del_ttf[LAY(n, nz)] = w->del_ttf[LAY(n, nz)]
@@ -1147,7 +1242,7 @@ void fo_diff_ver_part_redi_expl(const fo_mesh *m, const fo_dyn *d, double dt, co
 /* ------------------------------------------------------------------------ */
 /* bc_surface: oce_ale_tracer.F90:1828-2108 (non-RECOM IDs)                  */
 /* ------------------------------------------------------------------------ */
-int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, int nzmin, double sval, double *out)
+int fo_bc_surface(const fo_mesh *m, int id, double dt, const fo_dyn *d, int n, int nzmin, double sval, double *out)
 {
     switch (id) {
     case 1:
@@ -1158,6 +1253,24 @@ int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, int nzmin, double s
         return 0;
     case 100: case 301: case 302: case 303:
         *out = 0.0;
+        return 0;
+    case 101:   /* :2059-2060 */
+        if (!d->prec_rain) return -1;
+        *out = dt * d->prec_rain[n];
+        return 0;
+    case 102: case 103: case 104:   /* :2062-2067, wiso_flux_oce(n, ID-101) */
+        if (!d->wiso_flux_oce) return -1;
+        *out = dt * d->wiso_flux_oce[(size_t)(g_mut == 47 ? 104 - id : id - 102) * m->nod2D + n];
+        return 0;
+    case 304:   /* :2074-2079, nzmin is the level being filled */
+        if (d->hosing_depth) {
+            if (!d->hosing_flux3D) return -1;
+            const int kk = (g_mut == 48 && nzmin < m->nl - 1) ? nzmin + 1 : nzmin;
+            *out = dt * d->hosing_flux3D[(size_t)n * (m->nl - 1) + (kk - 1)];
+        } else {
+            if (!d->hosing_flux) return -1;
+            *out = dt * d->hosing_flux[n];
+        }
         return 0;
     case 501:   /* ice-shelf water from basal melting: under a cavity only (:2080-2085) */
         *out = (nzmin == 1 && g_mut != 46) ? 0.0 : dt * d->water_flux[n] * (sval - 1.0);
@@ -1175,12 +1288,22 @@ int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, int nzmin, double s
 int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t)
 {
     const int L = m->nl - 1, NL = m->nl;
+    /* bc_surface fails for every column or for none (an unknown ID, a flux
+       array that is not set): checked once, before any column changes */
+    double bc0;
+    if (m->nod2D > 0 && fo_bc_surface(m, t->ID, dt, d, 0, m->ulevels_nod2D[0], 0.0, &bc0) != 0) return -1;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
     double *a = (double *)calloc(NL, sizeof(double)), *b = (double *)calloc(NL, sizeof(double));
     double *c = (double *)calloc(NL, sizeof(double)), *tr = (double *)calloc(NL, sizeof(double));
     double *cp = (double *)calloc(NL, sizeof(double)), *tp = (double *)calloc(NL, sizeof(double));
     double *zbar_n = (double *)calloc(NL, sizeof(double)), *Z_n = (double *)calloc(NL, sizeof(double));
-    int rc = 0;
-    for (int n = 0; n < m->nod2D && rc == 0; ++n) {
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int n = 0; n < m->nod2D; ++n) {
         const double *hn = m->hnode_new + LAY(n, 1);
         const double *ar = m->area + INT(n, 1);
         const double *av = m->areasvol + INT(n, 1);
@@ -1283,10 +1406,20 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
         dz = hn[I(nz)];
         tr[I(nz)] = -a[I(nz)] * T[I(nz - 1)] - (b[I(nz)] - dz) * T[I(nz)];
 
-        double bc;
-        if (fo_bc_surface(t->ID, dt, d, n, nzmin, T[I(nzmin)], &bc) != 0) { rc = -1; break; }
-        if (g_mut == 41) tr[I(1)] = tr[I(1)] + bc;
-        else tr[I(nzmin)] = tr[I(nzmin)] + bc;
+        double bc = 0.0;
+        if (t->ID == 304 && d->hosing_depth) {
+            /* the hosing tracer in depth mode takes its source at every level
+               nzmin..nzmax (oce_ale_tracer.F90:1374-1380); row nzmax is not
+               part of the system, so the wet layers nzmin..nzmax-1 receive it */
+            for (nz = nzmin; nz <= nzmax - 1; ++nz) {
+                (void)fo_bc_surface(m, t->ID, dt, d, n, nz, T[I(nz)], &bc);
+                tr[I(nz)] = tr[I(nz)] + bc;
+            }
+        } else {
+            (void)fo_bc_surface(m, t->ID, dt, d, n, nzmin, T[I(nzmin)], &bc);
+            if (g_mut == 41) tr[I(1)] = tr[I(1)] + bc;
+            else tr[I(nzmin)] = tr[I(nzmin)] + bc;
+        }
 
         cp[I(nzmin)] = c[I(nzmin)] / b[I(nzmin)];
         tp[I(nzmin)] = tr[I(nzmin)] / b[I(nzmin)];
@@ -1304,7 +1437,8 @@ int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_t
 #undef I
     }
     free(a); free(b); free(c); free(tr); free(cp); free(tp); free(zbar_n); free(Z_n);
-    return rc;
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1327,6 +1461,7 @@ int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer
     }
     if (ntr >= 2) {
         double *s = t[1].values;
+        OMP_FOR
         for (int n = 0; n < m->nod2D; ++n)
             for (int nz = m->ulevels_nod2D[n]; nz <= m->nlevels_nod2D[n] - 1; ++nz) {
                 if (s[LAY(n, nz)] > 45.0) s[LAY(n, nz)] = 45.0;

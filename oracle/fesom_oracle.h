@@ -40,6 +40,11 @@ typedef struct {
     const double *zbar_n_bot;         /* N2 */
     const double *zbar_e_bot, *zbar_e_srf; /* F2 */
     const int *edge_up_dn_tri;        /* E2*2, from fo_find_up_downwind_triangles */
+    /* node -> edge lists (2*e + side, side 1 where the node is edges(2,e)),
+     * edges ascending: the OpenMP build sums the edge->node scatters as
+     * per-node gathers in this order, which is the serial scatter's order */
+    const int *nod_edge_ptr;          /* N2+1 */
+    const int *nod_edge_idx;          /* 2*E2 */
 } fo_mesh;
 
 typedef struct {
@@ -50,6 +55,13 @@ typedef struct {
     const double *Ki;             /* N2*L (o_ARRAYS Ki); NULL: Ki = 0, no horizontal diffusion */
     const double *slope_tapered;  /* N2*L*3, component fastest; read when redi */
     int redi;                     /* Redi (oce_modules.F90:75) */
+    /* per-node surface fluxes of the tracer IDs 101-104 and 304 (bc_surface,
+     * oce_ale_tracer.F90:2059-2079); NULL where no such tracer runs */
+    const double *prec_rain;      /* N2 (g_forcing_arrays prec_rain), ID 101 */
+    const double *wiso_flux_oce;  /* N2*3, node fastest (o_ARRAYS wiso_flux_oce(:,1..3)), IDs 102-104 */
+    const double *hosing_flux;    /* N2 (o_ARRAYS hosing_flux), ID 304 */
+    const double *hosing_flux3D;  /* N2*L, level fastest (o_ARRAYS hosing_flux3D), ID 304 in depth mode */
+    int hosing_depth;             /* use_hosing .and. hosing_mode == 'depth' */
 } fo_dyn;
 
 enum { FO_HOR_MFCT = 0, FO_HOR_UPW1 = 1, FO_HOR_MUSCL = 2 };
@@ -111,7 +123,7 @@ void fo_diff_ver_part_redi_expl(const fo_mesh *m, const fo_dyn *d, double dt, co
                                 double *del_ttf);
 void fo_diff_tracers_ale_tstar(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, fo_work *w);
 int fo_diff_ver_part_impl_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t);
-int fo_bc_surface(int id, double dt, const fo_dyn *d, int n, int nzmin, double sval, double *out);
+int fo_bc_surface(const fo_mesh *m, int id, double dt, const fo_dyn *d, int n, int nzmin, double sval, double *out);
 int fo_solve_tracers_ale(const fo_mesh *m, const fo_dyn *d, double dt, fo_tracer *t, int ntr, fo_work *w);
 
 int fo_init_stiff_mat_ale(const fo_mesh *m, double dt, int *rowptr, int *colind, double *values, int max_nnz);

@@ -16,6 +16,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libfesom_oracle.so")
+_LIB_OMP = os.path.join(_HERE, "libfesom_oracle_omp.so")   # OpenMP build: the multi-core CPU baseline
 
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
@@ -30,14 +31,16 @@ class FoMesh(C.Structure):
                 ("ulevels_nod2D_max", ip), ("elem_area", dp), ("elem_cos", dp), ("edge_dxdy", dp),
                 ("edge_cross_dxdy", dp), ("gradient_sca", dp), ("area", dp), ("areasvol", dp),
                 ("hnode", dp), ("hnode_new", dp), ("helem", dp), ("zbar_3d_n", dp), ("Z_3d_n", dp),
-                ("zbar_n_bot", dp), ("zbar_e_bot", dp), ("zbar_e_srf", dp), ("edge_up_dn_tri", ip)]
+                ("zbar_n_bot", dp), ("zbar_e_bot", dp), ("zbar_e_srf", dp), ("edge_up_dn_tri", ip),
+                ("nod_edge_ptr", ip), ("nod_edge_idx", ip)]
 
 
 class FoDyn(C.Structure):
     _fields_ = [("uv", dp), ("w", dp), ("w_e", dp), ("w_i", dp), ("Kv", dp), ("heat_flux", dp),
                 ("water_flux", dp), ("virtual_salt", dp), ("relax_salt", dp), ("real_salt_flux", dp),
                 ("is_nonlinfs", C.c_double), ("use_wsplit", C.c_int), ("Ki", dp), ("slope_tapered", dp),
-                ("redi", C.c_int)]
+                ("redi", C.c_int), ("prec_rain", dp), ("wiso_flux_oce", dp), ("hosing_flux", dp),
+                ("hosing_flux3D", dp), ("hosing_depth", C.c_int)]
 
 
 class FoTracer(C.Structure):
@@ -66,18 +69,24 @@ _lib = None
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (gcc, serial, no FMA contraction)."""
-    if force or not os.path.exists(_LIB):
+    """Compile the oracle with its Makefile (gcc, no FMA contraction): the
+    serial checker and its OpenMP build."""
+    if force or not os.path.exists(_LIB) or not os.path.exists(_LIB_OMP):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
     return _LIB
 
 
-def lib():
+_libs = {}
+
+
+def lib(omp: bool = False):
+    """The serial oracle, or (omp=True) its OpenMP build, bit-identical to it."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB):
+    if omp not in _libs:
+        path = _LIB_OMP if omp else _LIB
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path)
         L.fo_set_mutation.argtypes = [C.c_int]
         L.fo_find_up_downwind_triangles.argtypes = [C.POINTER(FoMesh), ip]
         L.fo_init_tracers_AB.argtypes = [C.POINTER(FoMesh), C.POINTER(FoTracer), C.POINTER(FoWork)]
@@ -116,8 +125,10 @@ def lib():
         L.fo_ssh_solve_preconditioner.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp]
         L.fo_ssh_solve_cg.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_int,
                                       C.POINTER(FoCgStats)]
-        _lib = L
-    return _lib
+        _libs[omp] = L
+        if not omp:
+            _lib = L
+    return _libs[omp]
 
 
 def set_mutation(m: int) -> None:
@@ -137,8 +148,9 @@ def _i(a):
 class Oracle:
     """Oracle bound to one mesh; keeps the numpy buffers alive."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, omp: bool = False):
         self.mesh = mesh
+        self.L = lib(omp)
         m = mesh
         c = lambda a, t=np.float64: np.ascontiguousarray(a, dtype=t)
         self._keep = {
@@ -154,6 +166,12 @@ class Oracle:
             "zbeb": c(m.zbar_e_bot), "zbes": c(m.zbar_e_srf),
             "updn": np.zeros((m.edge2D, 2), np.int32),
         }
+        # node -> edge lists, 2e + side, edges ascending (the OpenMP build's gathers)
+        q = np.concatenate([2 * np.arange(m.edge2D), 2 * np.arange(m.edge2D) + 1]).astype(np.int64)
+        nodes = np.concatenate([m.edges[:, 0], m.edges[:, 1]]).astype(np.int64)
+        order = np.lexsort((q, nodes))
+        self._keep["ne_idx"] = np.ascontiguousarray(q[order], dtype=np.int32)
+        self._keep["ne_ptr"] = np.concatenate([[0], np.cumsum(np.bincount(nodes, minlength=m.nod2D))]).astype(np.int32)
         k = self._keep
         self.fm = FoMesh(m.nod2D, m.elem2D, m.edge2D, m.nl, m.kmax, int(m.cartesian), m.cyclic_length,
                          _d(k["coord"]), _i(k["elem"]), _i(k["edges"]), _i(k["edge_tri"]), _i(k["nie"]),
@@ -161,8 +179,8 @@ class Oracle:
                          _i(k["nlmin"]), _i(k["ulmax"]), _d(k["earea"]), _d(k["ecos"]), _d(k["edxdy"]),
                          _d(k["ecd"]), _d(k["gsca"]), _d(k["area"]), _d(k["areasvol"]), _d(k["hnode"]),
                          _d(k["hnode_new"]), _d(k["helem"]), _d(k["zbar3d"]), _d(k["Z3d"]), _d(k["zbnb"]),
-                         _d(k["zbeb"]), _d(k["zbes"]), _i(k["updn"]))
-        lib().fo_find_up_downwind_triangles(C.byref(self.fm), _i(k["updn"]))
+                         _d(k["zbeb"]), _d(k["zbes"]), _i(k["updn"]), _i(k["ne_ptr"]), _i(k["ne_idx"]))
+        self.L.fo_find_up_downwind_triangles(C.byref(self.fm), _i(k["updn"]))
         L = m.nl - 1
         z = lambda *s: np.zeros(s)
         self.work_arrays = dict(tr_xy=z(m.elem2D, L, 2), edge_up_dn_grad=z(m.edge2D, L, 4),
@@ -180,7 +198,7 @@ class Oracle:
         return self._keep["updn"]
 
     def refresh_updn(self):
-        lib().fo_find_up_downwind_triangles(C.byref(self.fm), _i(self._keep["updn"]))
+        self.L.fo_find_up_downwind_triangles(C.byref(self.fm), _i(self._keep["updn"]))
 
     # ---------------------------------------------------------------- dynamics
     def dyn(self, st):
@@ -193,11 +211,15 @@ class Oracle:
         sl = getattr(st, "slope_tapered", None)
         keep["Ki"] = None if ki is None else np.ascontiguousarray(ki, dtype=np.float64)
         keep["sl"] = None if sl is None else np.ascontiguousarray(sl, dtype=np.float64)
+        for f in ("prec_rain", "wiso_flux_oce", "hosing_flux", "hosing_flux3D"):
+            a = getattr(st, f, None)
+            keep[f] = None if a is None else np.ascontiguousarray(a, dtype=np.float64)
         opt = lambda a: None if a is None else _d(a)
         d = FoDyn(_d(keep["uv"]), _d(keep["w"]), _d(keep["w_e"]), _d(keep["w_i"]), _d(keep["Kv"]),
                   _d(keep["hf"]), _d(keep["wf"]), _d(keep["vs"]), _d(keep["rs"]), _d(keep["rsf"]),
                   0.0, int(bool(getattr(st, "use_wsplit", False))), opt(keep["Ki"]), opt(keep["sl"]),
-                  int(bool(getattr(st, "redi", False))))
+                  int(bool(getattr(st, "redi", False))), opt(keep["prec_rain"]), opt(keep["wiso_flux_oce"]),
+                  opt(keep["hosing_flux"]), opt(keep["hosing_flux3D"]), int(bool(getattr(st, "hosing_depth", False))))
         return d, keep
 
     def tracer(self, values, valuesold, ID=1, AB_order=2, hor="MFCT", ver="QR4C", lim="FCT",
@@ -222,7 +244,7 @@ class Oracle:
             t, vab = self.tracer(values[i], valuesold[i], ID=int(st.ids[i]), **sch)
             trs[i] = t
             abs_.append(vab)
-        rc = lib().fo_solve_tracers_ale(C.byref(self.fm), C.byref(d), float(dt), trs, ntr, C.byref(self.fw))
+        rc = self.L.fo_solve_tracers_ale(C.byref(self.fm), C.byref(d), float(dt), trs, ntr, C.byref(self.fw))
         if rc != 0:
             raise RuntimeError(f"oracle solve_tracers_ale failed rc={rc}")
         return rc
@@ -234,12 +256,12 @@ class Oracle:
         dt = st.dt if dt is None else dt
         d, keep = self.dyn(st)
         t, vab = self.tracer(values, valuesold, ID=ID, **sch)
-        rc = lib().fo_init_tracers_AB(C.byref(self.fm), C.byref(t), C.byref(self.fw))
+        rc = self.L.fo_init_tracers_AB(C.byref(self.fm), C.byref(t), C.byref(self.fw))
         if rc:
             raise RuntimeError("init_tracers_AB failed")
         out = {"valuesAB": vab.copy(), "tr_xy": self.work_arrays["tr_xy"].copy(),
                "edge_up_dn_grad": self.work_arrays["edge_up_dn_grad"].copy()}
-        rc = lib().fo_do_oce_adv_tra(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), C.byref(self.fw))
+        rc = self.L.fo_do_oce_adv_tra(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), C.byref(self.fw))
         if rc:
             raise RuntimeError(f"do_oce_adv_tra failed rc={rc}")
         for k in ("fct_LO", "adv_flux_hor", "adv_flux_ver", "del_ttf_advhoriz", "del_ttf_advvert",
@@ -252,9 +274,9 @@ class Oracle:
         """T* update + implicit vertical diffusion after adv_stage (same tracer)."""
         dt = st.dt if dt is None else dt
         t, d, keep, vab = self._last
-        lib().fo_diff_tracers_ale_tstar(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), C.byref(self.fw))
+        self.L.fo_diff_tracers_ale_tstar(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), C.byref(self.fw))
         if t.i_vert_diff:
-            rc = lib().fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
+            rc = self.L.fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
             if rc:
                 raise RuntimeError("diff_ver_part_impl_ale failed")
 
@@ -264,42 +286,42 @@ class Oracle:
         d, keep = self.dyn(st)
         vo = np.zeros_like(values)
         t, vab = self.tracer(values, vo, ID=ID, lim=lim)
-        rc = lib().fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
+        rc = self.L.fo_diff_ver_part_impl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t))
         if rc:
             raise RuntimeError("diff_ver_part_impl_ale failed")
 
     # ------------------------------------------------------- single routines
     def gradient_elements(self, T):
         out = np.zeros((self.mesh.elem2D, self.mesh.nl - 1, 2))
-        lib().fo_tracer_gradient_elements(C.byref(self.fm), _d(np.ascontiguousarray(T)), _d(out))
+        self.L.fo_tracer_gradient_elements(C.byref(self.fm), _d(np.ascontiguousarray(T)), _d(out))
         return out
 
     def fill_up_dn_grad(self, tr_xy):
         out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1, 4))
-        lib().fo_fill_up_dn_grad(C.byref(self.fm), _d(np.ascontiguousarray(tr_xy)), _d(out))
+        self.L.fo_fill_up_dn_grad(C.byref(self.fm), _d(np.ascontiguousarray(tr_xy)), _d(out))
         return out
 
     def hor_upw1(self, uv, T):
         out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1))
-        lib().fo_adv_tra_hor_upw1(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
+        self.L.fo_adv_tra_hor_upw1(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
                                   _d(out), 1)
         return out
 
     def hor_mfct(self, uv, T, grad, num_ord=0.0):
         out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1))
-        lib().fo_adv_tra_hor_mfct(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
+        self.L.fo_adv_tra_hor_mfct(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
                                   float(num_ord), _d(out), _d(np.ascontiguousarray(grad)), 1)
         return out
 
     def ver_upw1(self, W, T):
         out = np.zeros((self.mesh.nod2D, self.mesh.nl))
-        lib().fo_adv_tra_ver_upw1(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
+        self.L.fo_adv_tra_ver_upw1(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
                                   _d(out), 1)
         return out
 
     def ver_qr4c(self, W, T, num_ord=1.0):
         out = np.zeros((self.mesh.nod2D, self.mesh.nl))
-        lib().fo_adv_tra_ver_qr4c(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
+        self.L.fo_adv_tra_ver_qr4c(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
                                   float(num_ord), _d(out), 1)
         return out
 
@@ -319,7 +341,7 @@ class Oracle:
         cmax = C.c_double(0.0)
         opt = lambda a: None if a is None else _d(np.ascontiguousarray(a, dtype=np.float64))
         keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (hbar, hbar_old, water_flux)]
-        rc = lib().fo_vert_vel_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)), float(dt), int(which_ale),
+        rc = self.L.fo_vert_vel_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)), float(dt), int(which_ale),
                                    opt(keep[0]), opt(keep[1]), opt(keep[2]), int(bool(use_wsplit)),
                                    float(wsplit_maxcfl), _d(out["w"]), _d(out["w_e"]), _d(out["w_i"]),
                                    _d(out["cfl_z"]), _d(hn), C.byref(cmax), _d(zb), float(min_hnode),
@@ -335,7 +357,7 @@ class Oracle:
         m = self.mesh
         out = np.zeros(m.nod2D)
         wf = np.zeros(m.nod2D) if water_flux is None else np.ascontiguousarray(water_flux, dtype=np.float64)
-        rc = lib().fo_compute_ssh_rhs_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)),
+        rc = self.L.fo_compute_ssh_rhs_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)),
                                           _d(np.ascontiguousarray(uv_rhs)), _d(np.ascontiguousarray(ssh_rhs_old)),
                                           _d(wf), float(alpha), int(which_ale), int(bool(use_cavity_fw2press)),
                                           _d(out))
@@ -349,7 +371,7 @@ class Oracle:
         h = np.array(hbar, dtype=np.float64, order="C", copy=True)
         ho, old, dhe = np.zeros(m.nod2D), np.zeros(m.nod2D), np.zeros(m.elem2D)
         wf = np.zeros(m.nod2D) if water_flux is None else np.ascontiguousarray(water_flux, dtype=np.float64)
-        rc = lib().fo_compute_hbar_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(wf), float(dt),
+        rc = self.L.fo_compute_hbar_ale(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(wf), float(dt),
                                        int(which_ale), _d(h), _d(ho), _d(old), _d(dhe))
         if rc:
             raise RuntimeError(f"oracle compute_hbar_ale failed rc={rc}")
@@ -357,19 +379,19 @@ class Oracle:
 
     def nboundary_lay(self):
         out = np.zeros(self.mesh.nod2D, np.int32)
-        lib().fo_muscl_nboundary_lay(C.byref(self.fm), _i(out))
+        self.L.fo_muscl_nboundary_lay(C.byref(self.fm), _i(out))
         return out
 
     def hor_muscl(self, uv, T, grad, num_ord=0.0):
         out = np.zeros((self.mesh.edge2D, self.mesh.nl - 1))
         nb = self.nboundary_lay()
-        lib().fo_adv_tra_hor_muscl(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
+        self.L.fo_adv_tra_hor_muscl(C.byref(self.fm), _d(np.ascontiguousarray(uv)), _d(np.ascontiguousarray(T)),
                                    float(num_ord), _d(out), _d(np.ascontiguousarray(grad)), _i(nb), 1)
         return out
 
     def gradient_z(self, T):
         out = np.zeros((self.mesh.nod2D, self.mesh.nl))
-        lib().fo_tracer_gradient_z(C.byref(self.fm), _d(np.ascontiguousarray(T)), _d(out))
+        self.L.fo_tracer_gradient_z(C.byref(self.fm), _d(np.ascontiguousarray(T)), _d(out))
         return out
 
     def hor_diff(self, st, T, dt=None):
@@ -380,7 +402,7 @@ class Oracle:
         T = np.ascontiguousarray(T, dtype=np.float64)
         txy, tz = self.gradient_elements(T), self.gradient_z(T)
         out = np.zeros((self.mesh.nod2D, self.mesh.nl - 1))
-        lib().fo_diff_part_hor_redi(C.byref(self.fm), C.byref(d), float(dt), _d(txy), _d(tz), _d(out))
+        self.L.fo_diff_part_hor_redi(C.byref(self.fm), C.byref(d), float(dt), _d(txy), _d(tz), _d(out))
         return out
 
     def expl_vdiff(self, st, T, ID=1, dt=None):
@@ -391,7 +413,7 @@ class Oracle:
         vo = np.zeros_like(T)
         t, vab = self.tracer(T, vo, ID=ID, i_vert_diff=False)
         out = np.zeros((self.mesh.nod2D, self.mesh.nl - 1))
-        lib().fo_diff_ver_part_expl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), _d(out))
+        self.L.fo_diff_ver_part_expl_ale(C.byref(self.fm), C.byref(d), float(dt), C.byref(t), _d(out))
         return out
 
     def redi_expl(self, st, T, dt=None):
@@ -400,24 +422,24 @@ class Oracle:
         d, keep = self.dyn(st)
         txy = self.gradient_elements(np.ascontiguousarray(T, dtype=np.float64))
         out = np.zeros((self.mesh.nod2D, self.mesh.nl - 1))
-        lib().fo_diff_ver_part_redi_expl(C.byref(self.fm), C.byref(d), float(dt), _d(txy), _d(out))
+        self.L.fo_diff_ver_part_redi_expl(C.byref(self.fm), C.byref(d), float(dt), _d(txy), _d(out))
         return out
 
     def vert_impl(self, W, T, dt):
         """adv_tra_vert_impl on a copy of T (nod2D x nl-1) with interface velocities W."""
         out = np.array(T, dtype=np.float64, order="C", copy=True)
-        lib().fo_adv_tra_vert_impl(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(W)), _d(out))
+        self.L.fo_adv_tra_vert_impl(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(W)), _d(out))
         return out
 
     def ver_cdiff(self, W, T):
         out = np.zeros((self.mesh.nod2D, self.mesh.nl))
-        lib().fo_adv_tra_ver_cdiff(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
+        self.L.fo_adv_tra_ver_cdiff(C.byref(self.fm), _d(np.ascontiguousarray(W)), _d(np.ascontiguousarray(T)),
                                    _d(out), 1)
         return out
 
     def ver_ppm(self, W, T, dt):
         out = np.zeros((self.mesh.nod2D, self.mesh.nl))
-        lib().fo_adv_tra_vert_ppm(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(W)),
+        self.L.fo_adv_tra_vert_ppm(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(W)),
                                   _d(np.ascontiguousarray(T)), _d(out), 1)
         return out
 
@@ -428,7 +450,7 @@ class Oracle:
         rowptr = np.zeros(m.nod2D + 1, np.int32)
         colind = np.zeros(max_nnz, np.int32)
         vals = np.zeros(max_nnz)
-        nnz = lib().fo_init_stiff_mat_ale(C.byref(self.fm), float(dt), _i(rowptr), _i(colind), _d(vals), max_nnz)
+        nnz = self.L.fo_init_stiff_mat_ale(C.byref(self.fm), float(dt), _i(rowptr), _i(colind), _d(vals), max_nnz)
         if nnz < 0:
             raise RuntimeError("stiffness assembly failed")
         return rowptr, colind[:nnz].copy(), vals[:nnz].copy()
@@ -436,7 +458,7 @@ class Oracle:
     def update_stiff_matrix(self, dt, dhe, rowptr, colind, vals):
         """update_stiff_mat_ale: the assembled values plus the term of the ssh change dhe (elem2D)."""
         out = np.array(vals, dtype=np.float64, copy=True)
-        rc = lib().fo_update_stiff_mat_ale(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(dhe, dtype=np.float64)),
+        rc = self.L.fo_update_stiff_mat_ale(C.byref(self.fm), float(dt), _d(np.ascontiguousarray(dhe, dtype=np.float64)),
                                            _i(np.ascontiguousarray(rowptr, dtype=np.int32)),
                                            _i(np.ascontiguousarray(colind, dtype=np.int32)), _d(out))
         if rc:

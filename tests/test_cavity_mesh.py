@@ -140,6 +140,65 @@ def test_meltwater_budget_under_cavity(cav):
     assert check_meltwater_budget(*cav)
 
 
+def forcing_tracer_state(mesh, st, hosing_depth, seed=37):
+    """Six tracers with the IDs 101, 102, 103, 104, 304 and 2, at rest, with
+    random per-node fluxes prec_rain, wiso_flux_oce(:,1..3), hosing_flux and a
+    random per-layer hosing_flux3D."""
+    rng = np.random.default_rng(seed)
+    N2, L = mesh.nod2D, mesh.nl - 1
+    z = np.zeros_like(st.w)
+    st6 = mg.build_state(mesh, seed=6, ntr=6, dt=st.dt, dx=20e3)
+    return dataclasses.replace(
+        st6, ids=np.array([101, 102, 103, 104, 304, 2], np.int32), uv=np.zeros_like(st6.uv), w=z, w_e=z, w_i=z,
+        prec_rain=rng.normal(0.0, 1e-5, N2), wiso_flux_oce=rng.normal(0.0, 1e-5, (3, N2)),
+        hosing_flux=rng.normal(0.0, 1e-5, N2), hosing_flux3D=rng.normal(0.0, 1e-5, (N2, L)),
+        hosing_depth=hosing_depth)
+
+
+def check_forcing_tracer_budget(mesh, st):
+    """bc_surface of the IDs 101-104 and 304 (oce_ale_tracer.F90:2059-2079) at
+    rest: one step changes each tracer's content by dt * flux * areasvol at
+    the first wet layer, with flux prec_rain (101), wiso_flux_oce column
+    ID-101 (102-104) or hosing_flux (304); in depth mode the hosing tracer
+    takes dt * hosing_flux3D * areasvol at every wet layer
+    (oce_ale_tracer.F90:1374-1380). The implicit diffusion moves content
+    only within a column."""
+    n = np.arange(mesh.nod2D)
+    top = mesh.ulevels_nod2D - 1
+    L = mesh.nl - 1
+    vol = mesh.areasvol[:, :L] * mesh.hnode
+    k = np.arange(1, L + 1)
+    wet = (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+    ok = True
+    for depth in (False, True):
+        s6 = forcing_tracer_state(mesh, st, depth)
+        v, vo = s6.values.copy(), s6.valuesold.copy()
+        Oracle(mesh).step(s6, v, vo)
+        terms = [s6.dt * f * mesh.areasvol[n, top] for f in
+                 (s6.prec_rain, s6.wiso_flux_oce[0], s6.wiso_flux_oce[1], s6.wiso_flux_oce[2])]
+        if depth:
+            terms.append(np.where(wet, s6.dt * s6.hosing_flux3D * mesh.areasvol[:, :L], 0.0))
+        else:
+            terms.append(s6.dt * s6.hosing_flux * mesh.areasvol[n, top])
+        for i, t in enumerate(terms):
+            got = (vol * (v[i] - s6.values[i])).sum()
+            ok &= abs(got - t.sum()) <= 1e-12 * np.abs(t).sum() + 1e-13 * np.abs(vol * s6.values[i]).sum()
+    return bool(ok)
+
+
+def test_forcing_tracer_budget_under_cavity(cav):
+    assert check_forcing_tracer_budget(*cav)
+
+
+def test_forcing_tracer_needs_its_flux(cav):
+    """A tracer with ID 101 and no prec_rain is refused, like an unknown ID."""
+    mesh, st = cav
+    s6 = dataclasses.replace(forcing_tracer_state(mesh, st, False), prec_rain=None)
+    v, vo = s6.values.copy(), s6.valuesold.copy()
+    with pytest.raises(Exception):
+        Oracle(mesh).step(s6, v, vo)
+
+
 def check_constant_unlimited(mesh, st):
     """A constant stays constant without the FCT limiter too (MFCT/QR4C and
     MUSCL/PPM, lim NON): the horizontal flux of every segment of an edge,
@@ -188,7 +247,8 @@ def check_cavity_stiffness(mesh, st):
 
 
 _TRACER = ["check_constant", "check_conservation", "check_fct_bounds", "check_vdiff_fv", "check_vert_impl",
-           "check_vdiff_wimpl", "check_mfct_linear", "check_qr4c_cubic", "check_ppm_linear", "check_cdiff_linear"]
+           "check_vdiff_wimpl", "check_mfct_linear", "check_qr4c_cubic", "check_ppm_linear", "check_cdiff_linear",
+           "check_fct_padding"]
 _DYN = ["check_level_sums_vanish", "check_column_flow_surface", "check_cfl", "check_wsplit",
         "check_ssh_rhs_column_divergence", "check_ssh_rhs_alpha_terms", "check_hbar_transport",
         "check_hbar_freshwater", "check_zstar", "check_update_stiff", "check_zlevel"]
@@ -230,6 +290,8 @@ MUTATIONS = {
     43: lambda c: check_constant_unlimited(*c),
     44: lambda c: P.check_constant(*c),
     46: lambda c: check_meltwater_budget(*c),
+    47: lambda c: check_forcing_tracer_budget(*c),
+    48: lambda c: check_forcing_tracer_budget(*c),
 }
 
 

@@ -264,3 +264,53 @@ def test_cavity_ssh_solve(cav):
     assert st["converged"] and ost["converged"]
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10 * 1.01
     assert abs(st["iters"] - ost["iters"]) <= 2
+
+
+@pytest.mark.parametrize("path", ["fused", "staged"])
+@pytest.mark.parametrize("depth", [False, True], ids=["hosing_flux", "hosing_flux3D"])
+def test_cavity_forcing_tracers_bitwise(cav, monkeypatch, path, depth):
+    """Tracers with the IDs 101 (prec_rain), 102-104 (wiso_flux_oce) and 304
+    (hosing, per node or at the first wet layer of hosing_flux3D) next to
+    salinity, with the flow on: three steps equal the oracle's bit for bit."""
+    from fesom2_amd import gpu
+    mesh, state, o = cav
+    rng = np.random.default_rng(43)
+    N2, L = mesh.nod2D, mesh.nl - 1
+    _, st6, _ = mg.make_config("c0cav", ntr=6)
+    st6 = dataclasses.replace(
+        st6, ids=np.array([101, 102, 103, 104, 304, 2], np.int32), heat_flux=state.heat_flux,
+        water_flux=state.water_flux, virtual_salt=state.virtual_salt, relax_salt=state.relax_salt,
+        real_salt_flux=state.real_salt_flux, prec_rain=rng.normal(0.0, 1e-5, N2),
+        wiso_flux_oce=rng.normal(0.0, 1e-5, (3, N2)), hosing_flux=rng.normal(0.0, 1e-5, N2),
+        hosing_flux3D=rng.normal(0.0, 1e-5, (N2, L)), hosing_depth=depth)
+    if path == "staged":
+        monkeypatch.setenv("FGPU_STEP", "staged")
+    print(f"[gpu] cavity IDs 101-104/304 depth={depth} -> 3 {path} steps", flush=True)
+    ctx = gpu.context_for(mesh, st6)
+    try:
+        ctx.solve_tracers_ale(st6.dt, 3)
+        out = [ctx.tracer_download(i + 1)[0] for i in range(st6.ntr)]
+    finally:
+        ctx.close()
+    v, vo = st6.values.copy(), st6.valuesold.copy()
+    for _ in range(3):
+        o.step(st6, v, vo)
+    wet = _wet(mesh)
+    for i in range(st6.ntr):
+        np.testing.assert_array_equal(out[i][wet], v[i][wet], err_msg=f"tracer {i + 1}")
+
+
+def test_forcing_tracer_without_flux_is_refused(cav):
+    """A tracer with ID 101 whose prec_rain was never set fails the step with
+    FGPU_ERR_STATE before any launch."""
+    from fesom2_amd import gpu
+    mesh, state, _ = cav
+    _, st1, _ = mg.make_config("c0cav", ntr=2)
+    st1 = dataclasses.replace(st1, ids=np.array([1, 101], np.int32))
+    ctx = gpu.context_for(mesh, st1)
+    try:
+        with pytest.raises(gpu.FesomGpuError) as e:
+            ctx.solve_tracers_ale(st1.dt, 1)
+        assert e.value.code == -5 and "prec_rain" in str(e.value)
+    finally:
+        ctx.close()

@@ -1,0 +1,164 @@
+"""Every launch shape of the fused step, and tracers far from zero, against
+the oracle (bit for bit).
+
+The autotuner picks tracers per block (1, 2, 4, 8 or all), tracers per Thomas
+chunk (2, 4 or 8) and columns per block of the last kernel (1, 4 or 8, only
+for columns of at most 64 levels). Each choice instantiates different kernels,
+so each is run here with twelve tracers, on the 24-layer channel (64-lane
+kernels) and on a 70-layer channel (128-lane kernels), two steps each.
+
+Two of the twelve tracers sit far from zero (about +2100 and -1500, as DIC
+or alkalinity do): the FCT bounds of a node next to an element without the
+level take that element's +-1e3 padding (oce_adv_tra_fct.F90:148-178), which
+only binds for such values. The oracle with the padding left out (planted
+mistake 49) gives a different result on these states, which the last test
+shows on the CPU side of this file.
+"""
+import numpy as np
+import pytest
+
+from fesom2_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+TPB = [1, 2, 4, 8, 12]
+TCH = [2, 4, 8]
+CPB = [1, 4, 8]
+
+
+def _wet(mesh):
+    k = np.arange(1, mesh.nl)
+    return (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+
+
+def offset_state(name, seed=3):
+    """Twelve tracers: T, S, ten passive blobs; the third offset to ~2100 and
+    the fourth to ~-1500, with small noise."""
+    mesh, _, _ = mg.make_config(name)
+    cfgs = dict(mg.CONFIGS[name]["state"])
+    cfgs.update(ntr=12)
+    st = mg.build_state(mesh, seed=seed, dx=20e3, **cfgs)
+    rng = np.random.default_rng(seed + 17)
+    w = _wet(mesh)
+    v = st.values.copy()
+    v[2] = np.where(w, v[2] + 2100.0 + rng.normal(0.0, 2.0, v[2].shape), 0.0)
+    v[3] = np.where(w, v[3] - 1500.0 + rng.normal(0.0, 2.0, v[3].shape), 0.0)
+    import dataclasses
+    st = dataclasses.replace(st, values=v, valuesold=np.where(w[None], v - 0.01 * np.sign(v), 0.0))
+    return mesh, st
+
+
+def _oracle_steps(mesh, st, nsteps, schemes=None):
+    from oracle.oracle import Oracle
+    v, vo = st.values.copy(), st.valuesold.copy()
+    o = Oracle(mesh)
+    for _ in range(nsteps):
+        o.step(st, v, vo, schemes=schemes)
+    return v, vo
+
+
+@pytest.fixture(scope="module")
+def chan():
+    mesh, st = offset_state("c0")
+    return mesh, st, _oracle_steps(mesh, st, 2)
+
+
+@pytest.fixture(scope="module")
+def deep():
+    mesh, st = offset_state("c0deep")
+    assert mesh.nl - 1 > 64 and (mesh.nlevels_nod2D - 1).min() < 20
+    return mesh, st, _oracle_steps(mesh, st, 2)
+
+
+def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nsteps=2, schemes=None):
+    from fesom2_amd import gpu
+    for k, v in (("FGPU_STEP_TPB", tpb), ("FGPU_STEP_TCH", tch), ("FGPU_STEP_CPB", cpb)):
+        if v is None:
+            monkeypatch.delenv(k, raising=False)
+        else:
+            monkeypatch.setenv(k, str(v))
+    if staged:
+        monkeypatch.setenv("FGPU_STEP", "staged")
+    else:
+        monkeypatch.delenv("FGPU_STEP", raising=False)
+    ctx = gpu.context_for(mesh, st, schemes=schemes)
+    try:
+        ctx.solve_tracers_ale(st.dt, nsteps)
+        shape = ctx.step_shape()
+        out = [ctx.tracer_download(i + 1) for i in range(st.ntr)]
+    finally:
+        ctx.close()
+    return out, shape
+
+
+def _compare(mesh, out, ref):
+    w = _wet(mesh)
+    v, vo = ref
+    for i, (gv, gvo) in enumerate(out):
+        np.testing.assert_array_equal(gv[w], v[i][w], err_msg=f"tracer {i + 1}")
+        np.testing.assert_array_equal(gvo[w], vo[i][w], err_msg=f"tracer {i + 1} valuesold")
+
+
+@pytest.mark.parametrize("cpb", CPB)
+@pytest.mark.parametrize("tch", TCH)
+@pytest.mark.parametrize("tpb", TPB)
+def test_fused_shapes_64_lanes(chan, monkeypatch, tpb, tch, cpb):
+    mesh, st, ref = chan
+    print(f"[gpu] c0, 12 tracers, shape tpb={tpb} tch={tch} cpb={cpb}", flush=True)
+    out, shape = _run(mesh, st, monkeypatch, tpb, tch, cpb)
+    assert shape == (tpb, tch, cpb)
+    _compare(mesh, out, ref)
+
+
+@pytest.mark.parametrize("tch", TCH)
+@pytest.mark.parametrize("tpb", TPB)
+def test_fused_shapes_128_lanes(deep, monkeypatch, tpb, tch):
+    mesh, st, ref = deep
+    print(f"[gpu] 70 layers, 12 tracers, shape tpb={tpb} tch={tch}", flush=True)
+    out, shape = _run(mesh, st, monkeypatch, tpb, tch, 4)
+    assert shape == (tpb, tch, 1)   # several columns per block need columns of <= 64 levels
+    _compare(mesh, out, ref)
+
+
+@pytest.mark.parametrize("which", ["chan", "deep"])
+def test_staged_offsets(chan, deep, monkeypatch, which):
+    mesh, st, ref = chan if which == "chan" else deep
+    print(f"[gpu] staged {which}, 12 tracers", flush=True)
+    out, _ = _run(mesh, st, monkeypatch, staged=True)
+    _compare(mesh, out, ref)
+
+
+def test_autotuned_deep(deep, monkeypatch):
+    """The autotuner on the 70-layer mesh restores the state it timed on and
+    its pick stays bit-identical."""
+    from fesom2_amd import gpu
+    mesh, st, ref = deep
+    for k in ("FGPU_STEP_TPB", "FGPU_STEP_TCH", "FGPU_STEP_CPB", "FGPU_STEP"):
+        monkeypatch.delenv(k, raising=False)
+    print("[gpu] autotune 70 layers", flush=True)
+    ctx = gpu.context_for(mesh, st)
+    try:
+        ctx.autotune_step(st.dt, 2)
+        ctx.solve_tracers_ale(st.dt, 2)
+        out = [ctx.tracer_download(i + 1) for i in range(st.ntr)]
+    finally:
+        ctx.close()
+    _compare(mesh, out, ref)
+
+
+MIXED = [dict(), dict(), dict(), dict(), dict(hor="UPW1", ver="UPW1", lim="FCT"),
+         dict(hor="MFCT", ver="QR4C", lim="NON", ph=0.5, pv=0.0), dict(hor="MUSCL", ver="QR4C", lim="FCT"),
+         dict(hor="MFCT", ver="CDIFF", lim="FCT"), dict(hor="MFCT", ver="PPM", lim="FCT"),
+         dict(hor="UPW1", ver="PPM", lim="NON"), dict(ph=0.5), dict(hor="MUSCL", ver="PPM", lim="NON", ph=0.5)]
+
+
+@pytest.mark.parametrize("shape", [(1, 2, 1), (4, 4, 4), (12, 8, 1), (8, 2, 8)], ids=str)
+@pytest.mark.parametrize("which", ["chan", "deep"])
+def test_mixed_schemes_shapes(chan, deep, monkeypatch, which, shape):
+    """Twelve tracers with different schemes in one block (the per-tracer
+    branches of every kernel), on both meshes."""
+    mesh, st, _ = chan if which == "chan" else deep
+    ref = _oracle_steps(mesh, st, 2, schemes=MIXED)
+    print(f"[gpu] mixed schemes {which} shape={shape}", flush=True)
+    out, _ = _run(mesh, st, monkeypatch, *shape, schemes=MIXED)
+    _compare(mesh, out, ref)

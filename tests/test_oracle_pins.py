@@ -463,6 +463,52 @@ def check_fct_bounds(mesh, st):
     return bad == 0
 
 
+def _padded_levels(mesh):
+    """(N2, nl-1) mask: some element around the node lacks the level
+    (above its first or below its last wet layer), taken from the element
+    levels alone."""
+    L = mesh.nl - 1
+    k = np.arange(1, L + 1)
+    lacks = (k[None, :] < mesh.ulevels[:, None]) | (k[None, :] > (mesh.nlevels - 1)[:, None])
+    pad = np.zeros((mesh.nod2D, L), bool)
+    for n in range(mesh.nod2D):
+        pad[n] = lacks[mesh.nod_in_elem2D[n, :mesh.nod_in_elem2D_num[n]]].any(0)
+    return pad
+
+
+def check_fct_padding(mesh, st):
+    """FCT bounds of elements without a level (oce_adv_tra_fct.F90:148-178):
+    with the tracer offset by +5000, the admissible decrease of a layer is
+    exactly 1e3 - LO wherever one of the layers its bound spans (k-1..k+1
+    inside the column, k alone at the top and bottom layer) has an element
+    around the node without that layer, and the true cluster minimum (about
+    5000) elsewhere. With -5000 the admissible increase is -1e3 - LO at the
+    same places. Only the element levels decide which layers these are."""
+    o = orc.Oracle(mesh)
+    pad = _padded_levels(mesh)
+    ok = True
+    for off in (5000.0, -5000.0):
+        w = _wet(mesh)
+        T = np.where(w, st.values[0] + off, 0.0)
+        out = o.adv_stage(st, T.copy(), T.copy(), ID=301)
+        LO = out["fct_LO"]
+        bnd = o.work_arrays["fct_ttf_min" if off > 0 else "fct_ttf_max"]
+        hit = 0
+        for n in range(mesh.nod2D):
+            nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+            for k in range(nzmin, nzmax):
+                span = [k] if (k == nzmin or k == nzmax - 1) else [k - 1, k, k + 1]
+                padded = any(pad[n, j - 1] for j in span)
+                b, lo = bnd[n, k - 1], LO[n, k - 1]
+                if padded:
+                    hit += 1
+                    ok &= b == ((1e3 if off > 0 else -1e3) - lo)
+                else:
+                    ok &= (b + lo > 4000.0) if off > 0 else (b + lo < -4000.0)
+        ok &= hit > 0
+    return bool(ok)
+
+
 def check_vdiff_fv(mesh, st):
     """Implicit diffusion = banded finite-volume solve built from the physics:
     areasvol_k h_k (T'_k - T_k) = F_{k-1/2} - F_{k+1/2},
@@ -703,6 +749,10 @@ def test_fct_bounds(small):
     assert check_fct_bounds(*small)
 
 
+def test_fct_padding(small):
+    assert check_fct_padding(*small)
+
+
 def test_vdiff_finite_volume(small):
     assert check_vdiff_fv(*small)
 
@@ -769,6 +819,7 @@ MUTATIONS = {
     26: lambda s, f: check_muscl(*f),
     27: lambda s, f: check_vert_impl(*s),
     28: lambda s, f: check_vdiff_wimpl(*s),
+    49: lambda s, f: check_fct_padding(*s),
 }
 
 
@@ -784,3 +835,21 @@ def test_planted_mistake_is_caught(small, flat, mut):
     finally:
         orc.set_mutation(0)
     assert caught, f"mutation {mut} not caught"
+
+
+@pytest.mark.parametrize("name", ["c0", "c0deep"])
+def test_offset_states_need_the_padding(name):
+    """The twelve-tracer states of the GPU launch-shape tests (tracers near
+    +2100 and -1500) step differently when the FCT bounds leave out the +-1e3
+    padding (planted mistake 49), so those bitwise tests see the padding."""
+    from tests.test_gpu_shapes import _oracle_steps, _wet, offset_state
+    mesh, st = offset_state(name)
+    good, _ = _oracle_steps(mesh, st, 1)
+    orc.set_mutation(49)
+    try:
+        bad, _ = _oracle_steps(mesh, st, 1)
+    finally:
+        orc.set_mutation(0)
+    w = _wet(mesh)
+    assert np.any(good[2][w] != bad[2][w]) and np.any(good[3][w] != bad[3][w])
+    assert np.array_equal(good[4][w], bad[4][w])   # a tracer within [0, 1] does not see the padding
