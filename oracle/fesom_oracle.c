@@ -59,6 +59,9 @@
  *   47 the isotope IDs 102 and 104 take each other's flux column   (bc_surface)
  *   48 depth-mode hosing (ID 304) reads the layer below nzmin     (bc_surface)
  *   49 the node bounds leave out elements without the level (no +-1e3 padding) (oce_tra_adv_fct)
+ *   50 QR4C's second interface takes the lower layer's value (upwind) instead of the mean (adv_tra_ver_qr4c)
+ *   51 QR4C applies its coinciding second / last-but-one row once in two-layer columns (adv_tra_ver_qr4c)
+ *   52 the node-mean gradient of fill_up_dn_grad is an unweighted mean over the elements (fill_up_dn_grad)
  */
 #include "fesom_oracle.h"
 
@@ -187,9 +190,10 @@ static void node_mean_grad(const fo_mesh *m, const double *tr_xy, int node, int 
     for (int k = 0; k < m->nod_in_elem2D_num[node]; ++k) {
         int elem = m->nod_in_elem2D[(size_t)node * m->kmax + k];
         if (m->nlevels[elem] - 1 < nz || nz < m->ulevels[elem]) continue;
-        tvol = tvol + m->elem_area[elem];
-        tx = tx + tr_xy[LAY(elem, nz) * 2 + 0] * m->elem_area[elem];
-        ty = ty + tr_xy[LAY(elem, nz) * 2 + 1] * m->elem_area[elem];
+        const double wgt = (g_mut == 52) ? 1.0 : m->elem_area[elem];
+        tvol = tvol + wgt;
+        tx = tx + tr_xy[LAY(elem, nz) * 2 + 0] * wgt;
+        ty = ty + tr_xy[LAY(elem, nz) * 2 + 1] * wgt;
     }
     *gx = tx / tvol;
     *gy = ty / tvol;
@@ -517,9 +521,13 @@ void fo_adv_tra_ver_qr4c(const fo_mesh *m, const double *W, const double *ttf, d
         int nz = nzmin;
         f[I(nz)] = -T[I(nz)] * w[I(nz)] * ar[I(nz)] - f[I(nz)];
         nz = nzmin + 1;
-        f[I(nz)] = -0.5 * (T[I(nz - 1)] + T[I(nz)]) * w[I(nz)] * ar[I(nz)] - f[I(nz)];
+        if (g_mut == 50) f[I(nz)] = -T[I(nz)] * w[I(nz)] * ar[I(nz)] - f[I(nz)];
+        else f[I(nz)] = -0.5 * (T[I(nz - 1)] + T[I(nz)]) * w[I(nz)] * ar[I(nz)] - f[I(nz)];
         nz = nzmax - 1;
-        f[I(nz)] = -0.5 * (T[I(nz - 1)] + T[I(nz)]) * w[I(nz)] * ar[I(nz)] - f[I(nz)];
+        /* with two layers nzmax-1 = nzmin+1: the reference applies this row a
+           second time to the flux just written (oce_adv_tra_ver.F90:392-400) */
+        if (!(g_mut == 51 && nzmax - 1 == nzmin + 1))
+            f[I(nz)] = -0.5 * (T[I(nz - 1)] + T[I(nz)]) * w[I(nz)] * ar[I(nz)] - f[I(nz)];
         nz = nzmax;
         f[I(nz)] = 0.0 - f[I(nz)];
         for (nz = nzmin + 2; nz <= nzmax - 2; ++nz) {

@@ -68,11 +68,18 @@ LIM = {"FCT": 0, "NON": 1}
 _lib = None
 
 
+REFERENCE_TREE = "/root/reference"
+
+
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, no FMA contraction): the
-    serial checker and its OpenMP build."""
+    serial checker and its OpenMP build; where the reference tree is present,
+    also oracle/_ref (the reference's own restart I/O, built with flang)."""
     if force or not os.path.exists(_LIB) or not os.path.exists(_LIB_OMP):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    if os.path.isdir(os.path.join(REFERENCE_TREE, "src")):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []) + ["ref", f"REF={REFERENCE_TREE}"],
+                       check=True)
     return _LIB
 
 

@@ -162,3 +162,19 @@ def test_mixed_schemes_shapes(chan, deep, monkeypatch, which, shape):
     print(f"[gpu] mixed schemes {which} shape={shape}", flush=True)
     out, _ = _run(mesh, st, monkeypatch, *shape, schemes=MIXED)
     _compare(mesh, out, ref)
+
+
+@pytest.mark.parametrize("path", ["fused", "staged"])
+def test_two_layer_columns(monkeypatch, path):
+    """A channel whose shallowest columns have two layers: the second and the
+    last-but-one QR4C rows coincide there and the reference applies that row
+    twice (oce_adv_tra_ver.F90:392-400); the Thomas solve has two rows. Three
+    steps, twelve tracers, bit for bit."""
+    mesh = mg.build_mesh(nx=20, ny=14, dx=20e3, jitter=0.2, seed=5, cartesian=True, nlayers=10, hmax=2000.0,
+                         min_layers=2, max_layers=10, topo="bumps", bump_height=1900.0)
+    assert np.any(mesh.nlevels_nod2D - mesh.ulevels_nod2D == 2)
+    st = mg.build_state(mesh, seed=5, ntr=12, dt=900.0, dx=20e3)
+    ref = _oracle_steps(mesh, st, 3)
+    print(f"[gpu] two-layer columns, {path}", flush=True)
+    out, _ = _run(mesh, st, monkeypatch, staged=(path == "staged"), nsteps=3)
+    _compare(mesh, out, ref)

@@ -203,6 +203,113 @@ def check_fill_up_dn_grad(mesh):
     return bad == 0
 
 
+def _green_gauss_mean(mesh, T, node, k):
+    """(1/A) * integral of grad T over the elements around `node` that have
+    level k, for T linear on each element: the sum of the elements' boundary
+    integrals of T n dl (trapezoid rule, exact for linear T) over the sum of
+    their shoelace areas, from the node coordinates alone (cartesian meshes;
+    elements that straddle the periodic seam are unwrapped)."""
+    c = mesh.coord_nod2D
+    num = np.zeros(2)
+    den = 0.0
+    for el in mesh.nod_in_elem2D[node, :mesh.nod_in_elem2D_num[node]]:
+        if not (mesh.ulevels[el] <= k <= mesh.nlevels[el] - 1):
+            continue
+        v = mesh.elem2D_nodes[el]
+        x = mg._trim_cyclic(c[v, 0] - c[v[0], 0], mesh.cyclic_length) * R
+        y = (c[v, 1] - c[v[0], 1]) * R
+        a2 = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0])   # twice the signed area
+        ring = 0.0
+        contour = np.zeros(2)
+        for i in range(3):
+            j = (i + 1) % 3
+            tm = 0.5 * (T[v[i], k - 1] + T[v[j], k - 1])
+            contour += tm * np.array([y[j] - y[i], -(x[j] - x[i])])
+        num += np.sign(a2) * contour
+        den += 0.5 * abs(a2)
+    return num / den
+
+
+def check_fill_up_dn_grad_green_gauss(mesh):
+    """Where an edge takes the mean gradient around a node (the levels not
+    shared by its up- and downwind triangles, and boundary edges), that mean
+    is the area-weighted mean of the element gradients: for a field linear on
+    each element (random nodal values) it equals the Green-Gauss integral
+    (1/A) * boundary integral of T n dl over the node's elements that have the
+    level. An unweighted mean differs on a jittered mesh."""
+    assert mesh.cartesian
+    o = orc.Oracle(mesh)
+    rng = np.random.default_rng(13)
+    T = rng.normal(0, 1, (mesh.nod2D, mesh.nl - 1))
+    g = o.fill_up_dn_grad(o.gradient_elements(T))
+    ud = o.edge_up_dn_tri
+    nu, nl = mg.edge_level_range(mesh)
+    bad = checked = 0
+    for e in range(0, mesh.edge2D, 3):
+        n1, n2 = mesh.edges[e]
+        up, dn = ud[e]
+        for k in range(nu[e], nl[e] + 1):
+            both = up >= 0 and dn >= 0
+            lo_sh = max(mesh.ulevels_nod2D_max[n1], mesh.ulevels_nod2D_max[n2])
+            hi_sh = min(mesh.nlevels_nod2D_min[n1], mesh.nlevels_nod2D_min[n2]) - 1
+            if both and lo_sh <= k <= hi_sh:
+                continue   # shared level: the triangles' own gradients
+            for col, node in ((0, n1), (1, n2)):
+                if not (mesh.ulevels_nod2D[node] <= k <= mesh.nlevels_nod2D[node] - 1):
+                    continue
+                exp = _green_gauss_mean(mesh, T, node, k)
+                got = g[e, k - 1, [col, col + 2]]
+                checked += 1
+                if np.max(np.abs(got - exp)) > 1e-9 * max(np.max(np.abs(exp)), 1e-12):
+                    bad += 1
+    return checked > 0 and bad == 0
+
+
+def check_qr4c_boundary_rows(mesh, st):
+    """The second and the last-but-one interface of QR4C (oce_adv_tra_ver.F90:
+    392-400) take the mean of the two adjacent layers: on uniform layers a
+    linear profile gives the exact interface value there, -T(zbar) W area,
+    for both pv = 1 and pv = 0."""
+    o = orc.Oracle(mesh)
+    zb, Z = mesh.zbar_3d_n, mesh.Z_3d_n
+    T = np.where(_wet(mesh), 7.0 + 2e-3 * Z, 0.0)
+    ok, checked = True, 0
+    for pv in (1.0, 0.0):
+        f = o.ver_qr4c(st.w, T, pv)
+        for n in range(mesh.nod2D):
+            nzmin, nzmax = mesh.ulevels_nod2D[n], mesh.nlevels_nod2D[n]
+            for kk in {nzmin + 1, nzmax - 1}:
+                h_up, h_lo = zb[n, kk - 2] - zb[n, kk - 1], zb[n, kk - 1] - zb[n, kk]
+                if abs(h_up - h_lo) > 1e-9 * h_up or st.w[n, kk - 1] == 0.0:
+                    continue
+                exp = -(7.0 + 2e-3 * zb[n, kk - 1]) * st.w[n, kk - 1] * mesh.area[n, kk - 1]
+                checked += 1
+                ok &= abs(f[n, kk - 1] - exp) <= 1e-12 * abs(exp)
+    return bool(ok and checked > 0)
+
+
+def check_qr4c_two_layers(mesh, st):
+    """In a column of two layers the second and the last-but-one interface
+    coincide, and the reference applies that row twice to the same flux
+    (oce_adv_tra_ver.F90:392-400): starting from zero the flux there is
+    X - (X - 0) = 0 exactly, where a column of three or more layers has the
+    mean-value flux X = -(T1 + T2)/2 W area."""
+    o = orc.Oracle(mesh)
+    T = np.where(_wet(mesh), 4.0 + np.arange(1, mesh.nl)[None, :] * 0.37, 0.0)
+    f = o.ver_qr4c(st.w, T, 1.0)
+    two = np.flatnonzero(mesh.nlevels_nod2D - mesh.ulevels_nod2D == 2)
+    more = np.flatnonzero((mesh.nlevels_nod2D - mesh.ulevels_nod2D > 2) & (st.w[np.arange(mesh.nod2D),
+                                                                              mesh.ulevels_nod2D] != 0.0))
+    ok = two.size > 0 and more.size > 0
+    for n in two:
+        ok &= f[n, mesh.ulevels_nod2D[n]] == 0.0
+    for n in more:
+        k = mesh.ulevels_nod2D[n] + 1
+        exp = -0.5 * (T[n, k - 2] + T[n, k - 1]) * st.w[n, k - 1] * mesh.area[n, k - 1]
+        ok &= abs(f[n, k - 1] - exp) <= 1e-13 * abs(exp) and exp != 0.0
+    return bool(ok)
+
+
 def check_mfct_linear(mesh, st):
     """MFCT with ph=0 on a linear field: Tmean1 = Tmean2 = edge midpoint value, so
     the flux equals T_mid times the upwind flux of T == 1."""
@@ -749,6 +856,29 @@ def test_fct_bounds(small):
     assert check_fct_bounds(*small)
 
 
+def test_fill_up_dn_grad_green_gauss(small, flat):
+    assert check_fill_up_dn_grad_green_gauss(small[0])
+    assert check_fill_up_dn_grad_green_gauss(flat[0])
+
+
+def test_qr4c_boundary_rows(small):
+    assert check_qr4c_boundary_rows(*small)
+
+
+@pytest.fixture(scope="module")
+def shallow():
+    """Channel whose shallowest node columns have two layers."""
+    mesh = mg.build_mesh(nx=20, ny=14, dx=20e3, jitter=0.2, seed=5, cartesian=True, nlayers=10, hmax=2000.0,
+                         min_layers=2, max_layers=10, topo="bumps", bump_height=1900.0)
+    st = mg.build_state(mesh, seed=5, ntr=2, dt=900.0, dx=20e3)
+    assert np.any(mesh.nlevels_nod2D - mesh.ulevels_nod2D == 2)
+    return mesh, st
+
+
+def test_qr4c_two_layer_columns(shallow):
+    assert check_qr4c_two_layers(*shallow)
+
+
 def test_fct_padding(small):
     assert check_fct_padding(*small)
 
@@ -820,6 +950,8 @@ MUTATIONS = {
     27: lambda s, f: check_vert_impl(*s),
     28: lambda s, f: check_vdiff_wimpl(*s),
     49: lambda s, f: check_fct_padding(*s),
+    50: lambda s, f: check_qr4c_boundary_rows(*s),
+    52: lambda s, f: check_fill_up_dn_grad_green_gauss(s[0]),
 }
 
 
@@ -853,3 +985,15 @@ def test_offset_states_need_the_padding(name):
     w = _wet(mesh)
     assert np.any(good[2][w] != bad[2][w]) and np.any(good[3][w] != bad[3][w])
     assert np.array_equal(good[4][w], bad[4][w])   # a tracer within [0, 1] does not see the padding
+
+
+def test_planted_two_layer_mistake_is_caught(shallow):
+    """Planted mistake 51 (the coinciding QR4C row applied once) fails the
+    two-layer check."""
+    assert check_qr4c_two_layers(*shallow)
+    orc.set_mutation(51)
+    try:
+        caught = not check_qr4c_two_layers(*shallow)
+    finally:
+        orc.set_mutation(0)
+    assert caught
