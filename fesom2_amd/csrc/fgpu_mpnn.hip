@@ -50,6 +50,8 @@ enum { SL_BB = 0, SL_RR, SL_RTRT, SL_ZR, SL_ZQ, SL_PQ, SL_XAX, SL_EAE, SL_JR, SL
 struct FCGState {
     int iter, done, nbreak, nneg;   // nneg: iterations that fell back to the diagonal direction
     double rtol, rr_last, s, zr;
+    double soltol;                  // the solve's tolerance and iteration cap, written by k_fcg_init:
+    int maxiter, pad;               // one captured iteration graph serves every (soltol, maxiter)
 };
 
 struct NNDev {
@@ -72,8 +74,7 @@ struct NNDev {
     int adam_t = 0;
     bool ready = false, weights = false;
     hipGraphExec_t fcg_graph = nullptr;
-    double fcg_tol = 0.0;
-    int fcg_maxiter = 0, fcg_last_iters = 0;
+    int fcg_last_iters = 0;
 };
 
 struct NNArgs {
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(NNT) k_nn_in(NNArgs A, const double *rin, doub
     nn_slots3(A.dpart, A.nbd, SL_RTRT, A.st ? SL_BB : -1, A.st ? SL_RR : -1, rtrt, bb, rr);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (A.st) {
-        const double rtol = soltol * sqrt(bb / (double)A.n);
+        const double rtol = A.st->soltol * sqrt(bb / (double)A.n);
         if (lead) { A.st->rtol = rtol; A.st->rr_last = rr; }
         if (sqrt(rr / (double)A.n) < rtol) {
             if (lead) A.st->done = 1;
@@ -689,7 +690,7 @@ __device__ __forceinline__ void fcg_put(const FCGArgs &F, int s0, double a, int 
 }
 
 // r = b - A x; partials b.b, r.r, (D^-1/2 r)^2; state reset
-__global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F)
+__global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F, double soltol, int maxiter)
 {
     double bb = 0.0, rr = 0.0, tt = 0.0;
     for (int i = blockIdx.x * NNT + threadIdx.x; i < F.n; i += F.nbd * NNT) {
@@ -718,6 +719,8 @@ __global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F)
     fcg_put(F, SL_BB, bb, SL_RR, rr, SL_RTRT, tt);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         FCGState s{};
+        s.soltol = soltol;
+        s.maxiter = maxiter;
         *F.st = s;
     }
 }
@@ -729,12 +732,15 @@ __global__ void __launch_bounds__(NNT) k_fcg_init(FCGArgs F)
 // takes the diagonal preconditioner's z = D^-1 r instead; flexible CG admits a
 // preconditioner that changes from one iteration to the next. p is double-
 // buffered by parity (p_k in P[k&1]) so that neighbours' p_k can be formed from
-// z and p_{k-1} while other blocks write p_k; p.q partials likewise.
-__global__ void __launch_bounds__(NNT) k_fcg_pa(FCGArgs F)
+// z and p_{k-1} while other blocks write p_k; p.q partials likewise. The
+// parity k & 1 comes with the launch (every graph replay runs an even number
+// of iterations from an even one), not from the state the previous kernel's
+// lead block advanced.
+__global__ void __launch_bounds__(NNT) k_fcg_pa(FCGArgs F, int par)
 {
     FCGState *st = F.st;
     if (st->done) return;
-    const int it = st->iter, par = it & 1;
+    const int it = st->iter;
     const bool first = it == 0;
     double zr, zq, jr, jq, pq, u1, u2;
     nn_slots3(F.tpart, F.ntiles, 0, 1, 2, zr, zq, jr);
@@ -788,12 +794,13 @@ __global__ void __launch_bounds__(NNT) k_fcg_pa(FCGArgs F)
     fcg_put(F, par ? SL_PQ1 : SL_PQ, pqn, -1, 0.0, -1, 0.0);
 }
 
-// alpha = (z.r)/(p.q); x += alpha p; r -= alpha q; partials r.r, (D^-1/2 r)^2
-__global__ void __launch_bounds__(NNT) k_fcg_b(FCGArgs F)
+// alpha = (z.r)/(p.q); x += alpha p; r -= alpha q; partials r.r, (D^-1/2 r)^2.
+// par = k & 1 comes with the launch: the lead block advances st->iter at the
+// end, which a block dispatched after it must not read as this iteration's.
+__global__ void __launch_bounds__(NNT) k_fcg_b(FCGArgs F, int par)
 {
     FCGState *st = F.st;
     if (st->done) return;
-    const int par = st->iter & 1;
     const double zr = st->zr, pq = fcg_slot(F, par ? SL_PQ1 : SL_PQ);
     const double *pk = par ? F.p1 : F.p;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -817,7 +824,7 @@ __global__ void __launch_bounds__(NNT) k_fcg_b(FCGArgs F)
     fcg_put(F, SL_RR, rr, SL_RTRT, tt, -1, d2);
     if (lead) {
         st->iter += 1;
-        if (st->iter >= F.maxiter) st->done = 3;
+        if (st->iter >= st->maxiter) st->done = 3;
     }
 }
 
@@ -1125,37 +1132,37 @@ static FCGArgs fcg_args(fgpu_context *c, int maxiter)
     return F;
 }
 
-static int fcg_enqueue_iter(fgpu_context *c, const FCGArgs &F, const NNArgs &A, double soltol)
+// iteration i of a group of FCG_GRAPH_ITERS (an even number), parity i & 1
+static int fcg_enqueue_iter(fgpu_context *c, const FCGArgs &F, const NNArgs &A, double soltol, int i)
 {
     const dim3 gb(F.nbd), bb(NNT);
     {
         ProfScope ps(c, "k_fcg_pa");
-        hipLaunchKernelGGL(k_fcg_pa, gb, bb, 0, c->stream, F);
+        hipLaunchKernelGGL(k_fcg_pa, gb, bb, 0, c->stream, F, i & 1);
     }
     {
         ProfScope ps(c, "k_fcg_b");
-        hipLaunchKernelGGL(k_fcg_b, gb, bb, 0, c->stream, F);
+        hipLaunchKernelGGL(k_fcg_b, gb, bb, 0, c->stream, F, i & 1);
     }
     return nn_forward(c, A, F.r, F.z, F.q, soltol);
 }
 
+// the graph of FCG_GRAPH_ITERS iterations, captured once: the tolerance and
+// the iteration cap are read from the device state k_fcg_init writes
 static int fcg_graph(fgpu_context *c, const FCGArgs &F, const NNArgs &A, double soltol)
 {
     NNDev &d = *c->nn;
-    if (d.fcg_graph && d.fcg_tol == soltol && d.fcg_maxiter == F.maxiter) return FGPU_OK;
-    if (d.fcg_graph) { (void)hipGraphExecDestroy(d.fcg_graph); d.fcg_graph = nullptr; }
+    if (d.fcg_graph) return FGPU_OK;
     hipGraph_t g;
     FGPU_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = FGPU_OK;
-    for (int i = 0; i < FCG_GRAPH_ITERS && !rc; ++i) rc = fcg_enqueue_iter(c, F, A, soltol);
+    for (int i = 0; i < FCG_GRAPH_ITERS && !rc; ++i) rc = fcg_enqueue_iter(c, F, A, soltol, i);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(&d.fcg_graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fgpu_fail(FGPU_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    d.fcg_tol = soltol;
-    d.fcg_maxiter = F.maxiter;
     return FGPU_OK;
 }
 
@@ -1172,12 +1179,12 @@ static int fcg_run(fgpu_context *c, double soltol, int maxiter, int fb_maxiter, 
     NNArgs A = nn_args(c, d.st);
     int rc = FGPU_OK;
     if (!c->prof && (rc = fcg_graph(c, F, A, soltol))) return rc;
-    hipLaunchKernelGGL(k_fcg_init, dim3(F.nbd), dim3(NNT), 0, c->stream, F);
+    hipLaunchKernelGGL(k_fcg_init, dim3(F.nbd), dim3(NNT), 0, c->stream, F, soltol, maxiter);
     if ((rc = nn_forward(c, A, F.r, F.z, nullptr, soltol))) return rc;
     if (c->prof) {   // per-kernel timing: iterations launched one by one
         FCGState hs;
         for (int it = 0; it < maxiter + 1; it += FCG_GRAPH_ITERS) {
-            for (int i = 0; i < FCG_GRAPH_ITERS && !rc; ++i) rc = fcg_enqueue_iter(c, F, A, soltol);
+            for (int i = 0; i < FCG_GRAPH_ITERS && !rc; ++i) rc = fcg_enqueue_iter(c, F, A, soltol, i);
             if (rc) return rc;
             FGPU_HIP(hipMemcpyAsync(&hs, d.st, sizeof(FCGState), hipMemcpyDeviceToHost, c->stream));
             FGPU_HIP(hipStreamSynchronize(c->stream));

@@ -35,6 +35,64 @@ def rhs_samples(mesh, n, seed=0):
     return out
 
 
+def _potential_flow(mesh, rng, nfields=3):
+    """Seeded element velocity fields u = grad(phi) of smooth nodal potentials,
+    (nfields, elem2D, 2): the divergent part a momentum tendency carries."""
+    x = mesh.coord_nod2D[:, 0]
+    y = mesh.coord_nod2D[:, 1]
+    L = mesh.cyclic_length if mesh.cyclic_length > 0 else max(np.ptp(x), 1e-12)
+    Ly = max(np.ptp(y), 1e-12)
+    gs = mesh.gradient_sca
+    en = mesh.elem2D_nodes
+    out = np.empty((nfields, mesh.elem2D, 2))
+    for k in range(nfields):
+        phi = mg._smooth_field(x, y - y.min(), L, Ly, rng, nmodes=24, kmax=3 + 2 * k)
+        pe = phi[en]
+        g = np.stack([(gs[:, 0:3] * pe).sum(1), (gs[:, 3:6] * pe).sum(1)], 1)
+        out[k] = g / max(np.abs(g).max(), 1e-300)
+    return out
+
+
+def model_rhs_sequence(ctx, mesh, state, nsteps=2000, every=10, seed=0, umax=0.01, soltol=1e-12,
+                       maxiter=20000):
+    """SSH right-hand sides and solutions from a run of the plain solver.
+
+    Every step sets a slowly turning momentum tendency uv_rhs (three seeded
+    potential-flow fields with phases drifting at their own rates, scaled to
+    umax m/s per step, with the vertical profile of the state's flow), forms
+    the right-hand side on the device with compute_ssh_rhs_ale (linfs,
+    alpha = 1, the resident uv of the state, ssh_rhs_old = the previous
+    step's) and solves it with the plain diagonally preconditioned CG,
+    warm-started from the previous step's solution as the model does. Every
+    `every`-th step's (rhs, solution) is kept.
+    Returns (rhs, sol), each (nsteps // every, nod2D)."""
+    rng = np.random.default_rng(seed + 9100)
+    fields = _potential_flow(mesh, rng)
+    rate = rng.uniform(0.002, 0.02, fields.shape[0])
+    phase = rng.uniform(0.0, 2.0 * np.pi, fields.shape[0])
+    # vertical profile of the generated flow: its column shape, normalised
+    prof = np.abs(state.uv).sum(2)
+    prof = prof / np.maximum(prof.max(1, keepdims=True), 1e-300)
+    ctx.set_uv(state.uv)
+    ctx.ssh_solve_preconditioner(2)
+    x = np.zeros(mesh.nod2D)
+    old = np.zeros(mesh.nod2D)
+    keep_b, keep_x = [], []
+    for s in range(nsteps):
+        c = np.cos(rate * s + phase)
+        tend = np.tensordot(c, fields, 1)                          # (elem2D, 2)
+        uv_rhs = umax * tend[:, None, :] * prof[:, :, None]
+        rhs = ctx.compute_ssh_rhs_ale(uv_rhs, old)
+        x, st = ctx.ssh_solve_cg(x, rhs, soltol=soltol, maxiter=maxiter)
+        if not st["converged"]:
+            raise RuntimeError(f"plain CG did not converge at step {s}: {st}")
+        old = rhs
+        if (s + 1) % every == 0:
+            keep_b.append(rhs.copy())
+            keep_x.append(x.copy())
+    return np.stack(keep_b), np.stack(keep_x)
+
+
 def solve_pairs(ctx, rhs, soltol=1e-12, maxiter=20000):
     """Solutions of A x = b with the plain diagonally preconditioned CG (device)."""
     ctx.ssh_solve_preconditioner(2)

@@ -1,0 +1,144 @@
+"""The benchmark workloads at their full sizes on the GPU.
+
+* c1 (126,858 nodes, 47 layers, 2 tracers), 100 steps at dt = 1800 s: every
+  field within 1e-10 relative of the oracle's 100 steps (its OpenMP build,
+  bit-identical to the serial one) and expected bit for bit; content conserved
+  to 1e-13 relative per tracer.
+* c2 (the c1 mesh, 34 tracers, Kv log-uniform in 1e-5..1e-2), 100 steps, and
+  c4 (1.3 M nodes, 70 layers, 10 tracers, dt = 600 s), 50 steps, at the launch
+  shape the autotuner picks: content conserved to 1e-13 relative per tracer,
+  no NaN, and every passive tracer (blobs in [0, 1], untouched by the salinity
+  clamp) stays within its initial range: the FCT limiter keeps the low-order
+  solution's bounds and the implicit diffusion is an M-matrix solve.
+* c3: the learned preconditioner trained on right-hand sides and solutions of
+  a run of the plain solver on the c1 operator, then flexible CG against plain
+  diagonal CG on held-out right-hand sides: both reach 1e-10, the learned
+  solve in fewer iterations, with solutions within cond * tol.
+"""
+import numpy as np
+import pytest
+
+from fesom2_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _wet(mesh):
+    k = np.arange(1, mesh.nl)
+    return (k[None, :] >= mesh.ulevels_nod2D[:, None]) & (k[None, :] <= (mesh.nlevels_nod2D - 1)[:, None])
+
+
+@pytest.fixture(scope="module")
+def c1():
+    mesh, state, _ = mg.make_config("c1")
+    return mesh, state
+
+
+@pytest.mark.timeout(300)
+def test_c1_100_steps_field_by_field(c1):
+    from fesom2_amd import gpu
+    from oracle.oracle import Oracle
+    mesh, state = c1
+    print(f"[gpu] c1 100 steps vs the oracle, N2={mesh.nod2D}", flush=True)
+    ctx = gpu.context_for(mesh, state)
+    try:
+        c_before = [ctx.tracer_content(i + 1) for i in range(state.ntr)]
+        ctx.solve_tracers_ale(state.dt, 100)
+        out = [ctx.tracer_download(i + 1) for i in range(state.ntr)]
+        c_after = [ctx.tracer_content(i + 1) for i in range(state.ntr)]
+    finally:
+        ctx.close()
+    o = Oracle(mesh, omp=True)
+    v, vo = state.values.copy(), state.valuesold.copy()
+    for _ in range(100):
+        o.step(state, v, vo)
+    w = _wet(mesh)
+    for i in range(state.ntr):
+        ref = v[i][w]
+        rel = np.max(np.abs(out[i][0][w] - ref)) / np.max(np.abs(ref))
+        print(f"[gpu] tracer {i + 1}: max rel diff after 100 steps {rel:.3e}", flush=True)
+        assert rel <= 1e-10
+        np.testing.assert_array_equal(out[i][0][w], ref)
+        np.testing.assert_array_equal(out[i][1][w], vo[i][w])
+        assert abs(c_after[i] - c_before[i]) / abs(c_before[i]) < 1e-13
+
+
+def _properties(mesh, state, nsteps):
+    from fesom2_amd import gpu
+    w = _wet(mesh)
+    lo = [state.values[i][w].min() for i in range(state.ntr)]
+    hi = [state.values[i][w].max() for i in range(state.ntr)]
+    ctx = gpu.context_for(mesh, state)
+    try:
+        tpb, ms = ctx.autotune_step(state.dt, 2)
+        print(f"[gpu] autotuned shape {ctx.step_shape()}, {ms:.2f} ms per step", flush=True)
+        c_before = [ctx.tracer_content(i + 1) for i in range(state.ntr)]
+        ctx.solve_tracers_ale(state.dt, nsteps)
+        c_after = [ctx.tracer_content(i + 1) for i in range(state.ntr)]
+        for i in range(state.ntr):
+            v, _ = ctx.tracer_download(i + 1)
+            vw = v[w]
+            assert np.all(np.isfinite(vw)), f"tracer {i + 1}: NaN"
+            rel = abs(c_after[i] - c_before[i]) / abs(c_before[i])
+            assert rel < 1e-13, f"tracer {i + 1}: content changed by {rel:.2e}"
+            if state.ids[i] >= 300:
+                tol = 1e-12 * max(abs(lo[i]), abs(hi[i]), 1.0)
+                assert vw.min() >= lo[i] - tol and vw.max() <= hi[i] + tol, \
+                    f"tracer {i + 1}: range [{vw.min()}, {vw.max()}] leaves [{lo[i]}, {hi[i]}]"
+            del v, vw
+        s, _ = ctx.tracer_download(2)
+        assert s[w].min() >= 3.0 and s[w].max() <= 45.0
+    finally:
+        ctx.close()
+
+
+@pytest.mark.timeout(300)
+def test_c2_100_steps_properties():
+    mesh, state, steps = mg.make_config("c2")
+    assert state.ntr == 34 and steps == 100
+    print(f"[gpu] c2: 34 tracers, 100 steps", flush=True)
+    _properties(mesh, state, 100)
+
+
+@pytest.mark.timeout(600)
+def test_c4_50_steps_properties():
+    mesh, state, steps = mg.make_config("c4")
+    assert state.ntr == 10 and steps == 50 and mesh.nl - 1 == 70
+    print(f"[gpu] c4: N2={mesh.nod2D}, N3={mesh.n3}, 10 tracers, 50 steps", flush=True)
+    _properties(mesh, state, 50)
+
+
+@pytest.mark.timeout(600)
+def test_c3_learned_solve_on_c1_operator(c1):
+    """Training data from 400 steps of the plain solver on the c1 operator
+    (every 4th step kept, plus the residuals met along plain CG), held-out
+    right-hand sides from another seed's run."""
+    from fesom2_amd import gpu
+    from fesom2_amd import ssh_learned as sl
+    import scipy.sparse as sp
+    mesh, state = c1
+    ctx = gpu.context_for(mesh, state)
+    try:
+        ctx.init_stiff_mat_ale(state.dt)
+        print("[gpu] c3: training data from the plain solver", flush=True)
+        b, x = sl.model_rhs_sequence(ctx, mesh, state, nsteps=400, every=4, seed=1)
+        rb, re = sl.residual_pairs(ctx, b[:24], x[:24], ks=(0, 10, 40, 100))
+        losses = sl.train(ctx, np.concatenate([b, rb]), np.concatenate([x, re]), epochs=20, batch=8, lr=2e-3)
+        print(f"[gpu] c3: loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+        assert losses[-1] < losses[0]
+        hb, hx = sl.model_rhs_sequence(ctx, mesh, state, nsteps=60, every=20, seed=7)
+        rp, ci, val, _ = ctx.ssh_get_matrix()
+        A = sp.csr_matrix((val, ci - 1, rp - 1), shape=(mesh.nod2D, mesh.nod2D))
+        ctx.ssh_solve_preconditioner(2)
+        for s in range(hb.shape[0]):
+            xp, stp = ctx.ssh_solve_cg(np.zeros(mesh.nod2D), hb[s], soltol=1e-10, maxiter=5000)
+            xl, stl = ctx.ssh_solve_learned(np.zeros(mesh.nod2D), hb[s], soltol=1e-10, maxiter=500)
+            print(f"[gpu] c3 held-out {s}: plain {stp['iters']} its {stp['ms']:.2f} ms, learned {stl['iters']} its "
+                  f"({stl['fallback_iters']} fallback) {stl['ms']:.2f} ms", flush=True)
+            assert stp["converged"] and stl["converged"]
+            for xx in (xp, xl):
+                assert np.linalg.norm(hb[s] - A @ xx) / np.linalg.norm(hb[s]) < 1e-10 * 1.01
+            assert stl["iters"] < stp["iters"]
+            assert np.linalg.norm(xl - xp) / np.linalg.norm(xp) < 1e-6
+    finally:
+        ctx.close()
