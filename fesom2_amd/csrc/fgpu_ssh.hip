@@ -9,8 +9,11 @@
 //   captured graph of small kernels; convergence, breakdown and the iteration
 //   cap are decided on the device and later kernels of a replay return at once.
 //   Iterates, residual and dot products are FP64.
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "fgpu_internal.h"
@@ -948,5 +951,101 @@ extern "C" int fgpu_ssh_download(fgpu_context *c, double *x)
     FGPU_HIP(hipSetDevice(c->device));
     FGPU_HIP(hipStreamSynchronize(c->stream));
     FGPU_HIP(hipMemcpy(x, c->ssh_x, sizeof(double) * c->ssh_n, hipMemcpyDeviceToHost));
+    return FGPU_OK;
+}
+
+// ==============================================================================
+// ssh_solve_cg without allocation: the work vectors r, z, p0, p1, q, w1, s1, the
+// per-block partial sums and the CG state live in `scratch`, a static device
+// array of FGPU_NOALLOC_MAX_ROWS rows that every GPU the library runs on holds
+// until the process ends. x and rhs are the caller's device vectors. Kernels
+// are launched directly (no graph instantiation), eight iterations between
+// polls of the state, with the same kernels, launch order and iteration
+// parities as fgpu_ssh_solve_cg, so both give the same x bit for bit.
+// Calls that overlap on one GPU share `scratch` and race.
+// ==============================================================================
+#define NOALLOC_VECS 7
+#define NOALLOC_PART (8 * 512)
+static constexpr size_t NOALLOC_DOUBLES =
+    (size_t)NOALLOC_VECS * FGPU_NOALLOC_MAX_ROWS + NOALLOC_PART + (sizeof(CGState) + 7) / 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+#define NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES];   // device address of `scratch`, per device
+
+// the device address of `scratch` on the current device (hipGetSymbolAddress
+// once per device; the array's host-side name is not a device address)
+static int scratch_on_device(int dev, double **out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fgpu_fail(FGPU_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fgpu_fail(FGPU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_ssh_solve_cg_noalloc(fgpu_context *c, double *x_dev, const double *rhs_dev, double soltol,
+                                         int maxiter, fgpu_cg_stats *stats)
+{
+    int rc = cg_ready(c, soltol, maxiter);
+    if (rc) return rc;
+    if (!x_dev || !rhs_dev) return fgpu_fail(FGPU_ERR_ARG, "null vector");
+    if (c->ssh_n > FGPU_NOALLOC_MAX_ROWS)
+        return fgpu_fail(FGPU_ERR_ARG, "fgpu_ssh_solve_cg_noalloc: " + std::to_string(c->ssh_n) +
+                                           " rows exceed the static work area of " +
+                                           std::to_string(FGPU_NOALLOC_MAX_ROWS) + " rows");
+    if (c->ssh_blocks * 8 > NOALLOC_PART) return fgpu_fail(FGPU_ERR_STATE, "partial-sum area too small");
+    double *s = nullptr;
+    if ((rc = scratch_on_device(c->device, &s))) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    CGArgs A = cg_args(c);
+    const size_t M = FGPU_NOALLOC_MAX_ROWS;
+    A.x = x_dev;
+    A.rhs = rhs_dev;
+    A.r = s;
+    A.z = s + M;
+    A.p0 = s + 2 * M;
+    A.p1 = s + 3 * M;
+    A.q = s + 4 * M;
+    A.w1 = s + 5 * M;
+    A.s1 = s + 6 * M;
+    A.part = s + NOALLOC_VECS * M;
+    A.st = (CGState *)(s + NOALLOC_VECS * M + NOALLOC_PART);
+    const dim3 g(A.nb), b(SBLK);
+    if (A.variant == 2 && c->ssh_single_reduction) {
+        hipLaunchKernelGGL(k_cg1_init_r, g, b, 0, c->stream, A, soltol, maxiter);
+        hipLaunchKernelGGL(k_cg1_init_w, g, b, 0, c->stream, A);
+    } else {
+        hipLaunchKernelGGL(k_cg_init, g, b, 0, c->stream, A, soltol, maxiter);
+        if (A.variant != 2) hipLaunchKernelGGL(k_cg_init_z, g, b, 0, c->stream, A);
+    }
+    FGPU_HIP(hipGetLastError());
+    const int max_batches = maxiter / CG_GRAPH_ITERS + 2;
+    CGState hs;
+    hs.done = 0;
+    for (int batch = 0; batch < max_batches && !hs.done; ++batch) {
+        if ((rc = enqueue_iters(c, A, CG_GRAPH_ITERS))) return rc;
+        FGPU_HIP(hipMemcpyAsync(&hs, A.st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
+        FGPU_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (!hs.done) return fgpu_fail(FGPU_ERR_STATE, "CG did not terminate");
+    c->ssh_last_iters = hs.iter;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) {
+        stats->iters_last = hs.iter < 1 ? 1 : hs.iter;
+        stats->converged = hs.done == 1;
+        stats->nbreakdown = hs.nbreak;
+        stats->nnegrz = hs.nneg;
+        stats->resid_last = std::sqrt(hs.rr_last / (double)c->ssh_n);
+        stats->rtol_last = hs.rtol;
+        stats->ms = ms;
+    }
     return FGPU_OK;
 }

@@ -38,7 +38,7 @@ module fesom_gpu_c
        fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident, fgpu_set_uv, &
        fgpu_vert_vel_ale, fgpu_vert_vel_download, fgpu_compute_ssh_rhs_ale, fgpu_update_stiff_mat_ale, &
        fgpu_compute_hbar_ale, fgpu_set_ale_config, fgpu_set_surface_forcing_tracers, &
-       fgpu_tracer_set_valuesAB, fgpu_ssh_set_preconditioner
+       fgpu_tracer_set_valuesAB, fgpu_ssh_set_preconditioner, fgpu_ssh_solve_cg_noalloc
 
   interface
      type(c_ptr) function fgpu_last_error() bind(C, name="fgpu_last_error")
@@ -385,6 +385,17 @@ module fesom_gpu_c
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: x(*)
      end function fgpu_ssh_download
+
+     ! x_dev, rhs_dev: device addresses (c_ptr) of nod2D doubles; work vectors
+     ! from the library's static device area (no allocation)
+     integer(c_int) function fgpu_ssh_solve_cg_noalloc(ctx, x_dev, rhs_dev, soltol, maxiter, stats) &
+          bind(C, name="fgpu_ssh_solve_cg_noalloc")
+       import :: c_int, c_ptr, c_double, fgpu_cg_stats
+       type(c_ptr), value :: ctx, x_dev, rhs_dev
+       real(c_double), value :: soltol
+       integer(c_int), value :: maxiter
+       type(fgpu_cg_stats), intent(out) :: stats
+     end function fgpu_ssh_solve_cg_noalloc
 
      integer(c_int) function fgpu_autotune_step(ctx, dt, nsteps, best_tpb, best_ms) &
           bind(C, name="fgpu_autotune_step")

@@ -90,6 +90,7 @@ _SIGS = {
     "fgpu_ssh_upload": [vp, dp, dp],
     "fgpu_ssh_solve_resident": [vp, C.c_double, C.c_int, C.c_int, C.POINTER(CgStats)],
     "fgpu_ssh_download": [vp, dp],
+    "fgpu_ssh_solve_cg_noalloc": [vp, vp, vp, C.c_double, C.c_int, C.POINTER(CgStats)],
     "fgpu_ssh_nn_param_count": [],
     "fgpu_ssh_nn_init": [vp, C.c_ulonglong],
     "fgpu_ssh_nn_set_weights": [vp, fp, C.c_int],
@@ -441,6 +442,15 @@ class Context:
     def ssh_solve_resident(self, soltol=1e-10, maxiter=2000, nrep=1):
         st = CgStats()
         _check(self.L.fgpu_ssh_solve_resident(self.h, float(soltol), int(maxiter), int(nrep), C.byref(st)))
+        return st.as_dict()
+
+    def ssh_solve_cg_noalloc(self, x_ptr, rhs_ptr, soltol=1e-10, maxiter=2000):
+        """CG on the caller's device vectors (addresses of nod2D doubles on this
+        context's device; x in/out); the work vectors come from the library's
+        static device area, so the call allocates nothing."""
+        st = CgStats()
+        _check(self.L.fgpu_ssh_solve_cg_noalloc(self.h, C.c_void_p(int(x_ptr)), C.c_void_p(int(rhs_ptr)),
+                                                float(soltol), int(maxiter), C.byref(st)))
         return st.as_dict()
 
     def ssh_download(self):

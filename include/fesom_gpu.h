@@ -270,6 +270,18 @@ int fgpu_ssh_upload(fgpu_context *ctx, const double *x0, const double *rhs);
 int fgpu_ssh_solve_resident(fgpu_context *ctx, double soltol, int maxiter, int nrep,
                             fgpu_cg_stats *stats);
 int fgpu_ssh_download(fgpu_context *ctx, double *x);
+/* No counterpart in the reference: fgpu_ssh_solve_cg for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid allocation). x_dev (in/out) and rhs_dev are the
+ * caller's DEVICE vectors of nod2D entries on the context's device; the work
+ * vectors come from a static device area of FGPU_NOALLOC_MAX_ROWS rows (about
+ * 84 MB on each GPU the library runs on, held until the process ends), so the
+ * call allocates nothing. Larger systems return FGPU_ERR_ARG. Same kernels and
+ * iteration order as fgpu_ssh_solve_cg: the same x bit for bit. Calls that
+ * overlap on one GPU share the static area and race. stats->ms is host time. */
+#define FGPU_NOALLOC_MAX_ROWS 1500000
+int fgpu_ssh_solve_cg_noalloc(fgpu_context *ctx, double *x_dev, const double *rhs_dev, double soltol, int maxiter,
+                              fgpu_cg_stats *stats);
 
 /* ---- learned SSH preconditioner -------------------------------------------
  * No counterpart in the reference: an optional replacement of the diagonal /
