@@ -111,8 +111,8 @@ def test_c4_50_steps_properties():
 @pytest.mark.timeout(600)
 def test_c3_learned_solve_on_c1_operator(c1):
     """Training data from 400 steps of the plain solver on the c1 operator
-    (every 4th step kept, plus the residuals met along plain CG), held-out
-    right-hand sides from another seed's run."""
+    (every 4th step kept, each expanded into the residual/error pairs plain CG
+    meets on it), held-out right-hand sides from another seed's run."""
     from fesom2_amd import gpu
     from fesom2_amd import ssh_learned as sl
     import scipy.sparse as sp
@@ -121,9 +121,8 @@ def test_c3_learned_solve_on_c1_operator(c1):
     try:
         ctx.init_stiff_mat_ale(state.dt)
         print("[gpu] c3: training data from the plain solver", flush=True)
-        b, x = sl.model_rhs_sequence(ctx, mesh, state, nsteps=400, every=4, seed=1)
-        rb, re = sl.residual_pairs(ctx, b[:24], x[:24], ks=(0, 10, 40, 100))
-        losses = sl.train(ctx, np.concatenate([b, rb]), np.concatenate([x, re]), epochs=20, batch=8, lr=2e-3)
+        rb, re = sl.model_training_set(ctx, mesh, state, nsteps=400, every=4, seed=1)
+        losses = sl.train(ctx, rb, re, epochs=40, batch=8, lr=2e-3)
         print(f"[gpu] c3: loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
         assert losses[-1] < losses[0]
         hb, hx = sl.model_rhs_sequence(ctx, mesh, state, nsteps=60, every=20, seed=7)
