@@ -62,6 +62,7 @@
  *   50 QR4C's second interface takes the lower layer's value (upwind) instead of the mean (adv_tra_ver_qr4c)
  *   51 QR4C applies its coinciding second / last-but-one row once in two-layer columns (adv_tra_ver_qr4c)
  *   52 the node-mean gradient of fill_up_dn_grad is an unweighted mean over the elements (fill_up_dn_grad)
+ *   53 the FCT denominators leave out flux_eps                       (oce_tra_adv_fct)
  */
 #include "fesom_oracle.h"
 
@@ -554,7 +555,7 @@ void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const do
                         double *fct_plus, double *fct_minus)
 {
     const int L = m->nl - 1, NL = m->nl;
-    const double flux_eps = 1e-16, bignumber = 1e3;
+    const double flux_eps = (g_mut == 53) ? 0.0 : 1e-16, bignumber = 1e3;
     double *aux = (double *)calloc((size_t)m->elem2D * L * 2, sizeof(double));
     double *tvmax = (double *)calloc((size_t)m->nod2D * L, sizeof(double));
     double *tvmin = (double *)calloc((size_t)m->nod2D * L, sizeof(double));

@@ -952,6 +952,7 @@ MUTATIONS = {
     49: lambda s, f: check_fct_padding(*s),
     50: lambda s, f: check_qr4c_boundary_rows(*s),
     52: lambda s, f: check_fill_up_dn_grad_green_gauss(s[0]),
+    53: lambda s, f: check_constant(*s),
 }
 
 
