@@ -2,12 +2,16 @@
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4]
 
-One "step" is one solve_tracers_ale pass over every tracer of the workload
-(AB2, MFCT/QR4C/FCT advection, implicit vertical diffusion), replayed from a
+The default workload is the 1.3 M-node, 70-level mesh with 10 tracers. One
+"step" is one solve_tracers_ale pass over every tracer of the workload (AB2,
+MFCT/QR4C/FCT advection, implicit vertical diffusion), replayed from a
 captured hipGraph with all fields resident in HBM. `value` is tracer-updates/s
 (wet 3-D nodes x tracers x steps / wall time of the K timed steps). The SSH
 solve (diagonally preconditioned CG to 1e-10 relative residual, x0 = 0) is
-timed separately and reported as ssh_solve_ms.
+timed separately and reported as ssh_solve_ms; beside it, the learned
+preconditioner in flexible CG, trained on this workload's operator from the
+model's own right-hand sides, against plain CG on a held-out right-hand side
+(ssh_learned).
 
 Also reported: the per-kernel profile of the step (hipEvents on the library
 stream), the roofline of the dominant kernel, and the oracle's OpenMP build
@@ -166,9 +170,14 @@ def main():
     ap.add_argument("--config", default="c4", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ssh-reps", type=int, default=5)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner SSH solve")
+    ap.add_argument("--learn-steps", type=int, default=200,
+                    help="plain-solver steps whose right-hand sides (every 4th) train the network")
+    ap.add_argument("--learn-epochs", type=int, default=20)
     ap.add_argument("--shape", default=None,
-                    help="TPB,TCH,CPB: fixed launch shape of the fused step instead of the autotuner's "
-                         "(a profiled run then holds only that shape's launches)")
+                    help="TPB,TCH,CPB[,KD]: fixed launch shape of the fused step (and variant of its last "
+                         "kernel: 0 lock-step, 1 sweep wave, 2 one wave per column) instead of the "
+                         "autotuner's (a profiled run then holds only that shape's launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is design-only "
@@ -176,9 +185,10 @@ def main():
         sys.exit(2)
 
     if args.shape:
-        tpb_s, tch_s, cpb_s = (int(x) for x in args.shape.split(","))
+        sv = [int(x) for x in args.shape.split(",")]
         os.environ["FGPU_STEP_TPB"], os.environ["FGPU_STEP_TCH"], os.environ["FGPU_STEP_CPB"] = (
-            str(tpb_s), str(tch_s), str(cpb_s))
+            str(sv[0]), str(sv[1]), str(sv[2]))
+        os.environ["FGPU_STEP_KD"] = ["lockstep", "sweepwave", "wavelocal"][sv[3] if len(sv) > 3 else 0]
     from fesom2_amd import gpu, meshgen as mg
     t0 = time.perf_counter()
     mesh, state, _ = mg.make_config(args.config)
@@ -195,7 +205,9 @@ def main():
         tpb, tune_ms = ctx.autotune_step(dt, 3)
         shape = ctx.step_shape()
         print(f"[bench] autotuner: {shape[0]} tracers per block, {shape[1]} per Thomas chunk, {shape[2]} columns "
-              f"per block ({tune_ms:.3f} ms/step)", file=sys.stderr, flush=True)
+              f"per block, last kernel variant {ctx.step_variant()} ({tune_ms:.3f} ms/step)", file=sys.stderr,
+              flush=True)
+    ctx_variant = ctx.step_variant()
     ctx.solve_tracers_ale(dt, max(1, args.warmup))
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -260,6 +272,40 @@ def main():
     sst = ctx.ssh_solve_resident(1e-10, 5000, args.ssh_reps)
     print(f"[bench] ssh: {sst}", file=sys.stderr, flush=True)
 
+    # learned preconditioner in flexible CG beside plain diagonal CG, on a
+    # held-out right-hand side of the model's own steps (another seed's run)
+    learned = None
+    if not args.no_learned:
+        from fesom2_amd import ssh_learned as sl
+        t0 = time.perf_counter()
+        rb, re = sl.model_training_set(ctx, mesh, state, nsteps=args.learn_steps, every=4, seed=1)
+        print(f"[bench] learned SSH: {rb.shape[0]} training pairs in {time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+        ctx.nn_init(0)
+        losses = []
+        for e0 in range(0, args.learn_epochs, 5):   # the device's Adam state carries over between calls
+            ne = min(5, args.learn_epochs - e0)
+            losses.extend(ctx.nn_train(rb, re, epochs=ne, batch=8, lr=2e-3))
+            print(f"[bench] learned SSH: epoch {e0 + ne}/{args.learn_epochs}, loss {losses[-1]:.4f}",
+                  file=sys.stderr, flush=True)
+        t_train = time.perf_counter() - t0
+        hb, _ = sl.model_rhs_sequence(ctx, mesh, state, nsteps=20, every=20, seed=7)
+        ctx.ssh_solve_preconditioner(2)
+        ctx.ssh_upload(np.zeros(mesh.nod2D), hb[0])
+        ctx.ssh_solve_resident(1e-10, 5000, 1)
+        pst = ctx.ssh_solve_resident(1e-10, 5000, args.ssh_reps)
+        ctx.ssh_solve_learned_resident(1e-10, 500, 5000, 1)
+        lst = ctx.ssh_solve_learned_resident(1e-10, 500, 5000, args.ssh_reps)
+        learned = {"rhs": "held-out step of a model run (compute_ssh_rhs_ale), x0 = 0",
+                   "plain_cg_ms": round(pst["ms"], 3), "plain_cg_iters": pst["iters"],
+                   "learned_fcg_ms": round(lst["ms"], 3), "learned_fcg_iters": lst["iters"],
+                   "learned_fallback_iters": lst["fallback_iters"], "learned_converged": lst["converged"],
+                   "speedup_vs_plain": round(pst["ms"] / lst["ms"], 3),
+                   "training": f"{rb.shape[0]} residual/error pairs from {args.learn_steps} plain-solver steps, "
+                               f"{args.learn_epochs} epochs, loss {losses[0]:.3f} -> {losses[-1]:.3f}, "
+                               f"{t_train:.1f} s with data"}
+        print(f"[bench] learned SSH: {learned}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, state)
@@ -277,8 +323,10 @@ def main():
                    "n3": mesh.n3, "tracers": state.ntr, "parallelism": "single GPU"},
         "ssh_solve_ms": round(sst["ms"], 3), "ssh_iters": sst["iters"], "ssh_converged": sst["converged"],
         "ssh_precond": "diagonal (Jacobi)",
+        "ssh_learned": learned,
         "device_ms_per_step": ms_dev / args.steps,
-        "autotuned_shape": {"tracers_per_block": shape[0], "thomas_chunk": shape[1], "columns_per_block": shape[2]},
+        "autotuned_shape": {"tracers_per_block": shape[0], "thomas_chunk": shape[1], "columns_per_block": shape[2],
+                            "last_kernel": ["lock-step", "sweep wave", "one wave per column"][ctx_variant]},
         "step_alg_GBs": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
         "step_model_bytes": int(step_model_bytes(mesh, state.ntr)),
         "step_model_frac_of_hbm": round(step_model_bytes(mesh, state.ntr) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),

@@ -63,6 +63,12 @@ struct DevMesh {
     const double *erec_d;    // E2*ERD
     const double *zn_vd;     // N2*L: Z levels rebuilt from hnode_new for the implicit diffusion
     const double *zbn_vd;    // N2*NL: the interface depths of the same rebuild (Redi K33 weights)
+    // processing order of the fused step's column blocks (a permutation of the
+    // nodes / edges; results do not depend on it): along a Hilbert curve over
+    // the coordinates, so that the columns running at once on one XCD are
+    // spatial neighbours and share their neighbour gathers in its L2
+    const int *node_ord;     // N2
+    const int *edge_ord;     // E2
 };
 // erec_i fields
 enum { ER_N1, ER_N2, ER_EL1, ER_EL2, ER_NL1, ER_NU1, ER_NL2, ER_NU2, ER_UA0, ER_UA1, ER_UA2, ER_DA0, ER_DA1,
@@ -142,6 +148,11 @@ struct ProfEntry {
     int launches = 0;
 };
 
+// variants of the fused step's last kernel (same results bit for bit)
+#define KD_LOCKSTEP 0    // kD_node: the block's waves in lock-step, sweeps on the first wave's lanes
+#define KD_SWEEPWAVE 1   // kDp_node<..., true>: one wave per column plus a sweep wave per block
+#define KD_WAVELOCAL 2   // kDp_node<..., false>: one wave per column, which sweeps its own column
+
 struct fgpu_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -171,6 +182,8 @@ struct fgpu_context {
     int step_tpb = 4096;      // tracers per block of the fused step kernels (clamped to the tracer count)
     int step_tch = 2;         // tracers per Thomas chunk of the fused step's last kernel (2 or 4)
     int step_cpb = 1;         // columns per block of the fused step's last kernel (1, 4 or 8; 64-lane columns)
+    int step_order = 1;       // processing order of the fused step's columns: 1 Hilbert, 0 index (FGPU_STEP_ORDER)
+    int step_kdp = 0;         // last kernel of the plain step: KD_LOCKSTEP (kD_node), KD_SWEEPWAVE or KD_WAVELOCAL (kDp_node)
     int stage_slots = 0;      // tr_xy/aux slots allocated for the per-stage kernels (0: none yet)
     // profiling
     bool prof = false;
@@ -236,6 +249,8 @@ int fgpu_prof_flush(fgpu_context *c);
 
 // fused batched step (fgpu_step.hip): four launches per step for all tracers
 int fgpu_enqueue_fused_step(fgpu_context *c, double dt);
+void fgpu_kd_shape(const fgpu_context *c, int *tch, int *cpb);
+int fgpu_kd_extras(const fgpu_context *c);
 int fgpu_compute_zn(fgpu_context *c);
 int fgpu_ensure_staged(fgpu_context *c, int slots);
 // explicit diffusion terms (fgpu_diff.hip)

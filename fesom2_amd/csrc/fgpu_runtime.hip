@@ -16,6 +16,9 @@
 #include <cstring>
 #include <mutex>
 
+#include <algorithm>
+#include <cstdint>
+
 #include "fgpu_internal.h"
 
 static thread_local std::string g_last_error;
@@ -148,6 +151,11 @@ extern "C" int fgpu_create(fgpu_context **out, int device)
     if (const char *cv = std::getenv("FGPU_SSH_CG")) c->ssh_single_reduction = std::string(cv) != "classic";
     if (const char *tv = std::getenv("FGPU_STEP_TCH")) c->step_tch = std::atoi(tv);
     if (const char *pv = std::getenv("FGPU_STEP_CPB")) c->step_cpb = std::atoi(pv);
+    if (const char *ov = std::getenv("FGPU_STEP_ORDER")) c->step_order = std::string(ov) != "index";
+    if (const char *kv = std::getenv("FGPU_STEP_KD")) {
+        const std::string v(kv);
+        c->step_kdp = v == "sweepwave" ? KD_SWEEPWAVE : (v == "wavelocal" ? KD_WAVELOCAL : KD_LOCKSTEP);
+    }
     // default priority: stream priorities are left alone on purpose
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -274,6 +282,52 @@ static int updn_search(const double *coord, const int *elem_nodes, const int *ni
         if ((ab == ax) || (ax == 0.0)) { found = elem; continue; }
     }
     return found;
+}
+
+// index along a Hilbert curve of the cell (x, y) of an n x n grid, n a power
+// of two (the iterative form of Hilbert's quadrant recursion)
+static uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y)
+{
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {   // rotate the quadrant so the curve's pieces join
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// points ordered along a Hilbert curve over their bounding box (65536^2 cells)
+static std::vector<int> hilbert_order(const std::vector<double> &x, const std::vector<double> &y)
+{
+    const size_t n = x.size();
+    std::vector<int> ord(n);
+    if (n == 0) return ord;
+    const double x0 = *std::min_element(x.begin(), x.end()), x1 = *std::max_element(x.begin(), x.end());
+    const double y0 = *std::min_element(y.begin(), y.end()), y1 = *std::max_element(y.begin(), y.end());
+    const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t cx = (uint32_t)std::min(65535.0, std::max(0.0, (x[i] - x0) * sx));
+        const uint32_t cy = (uint32_t)std::min(65535.0, std::max(0.0, (y[i] - y0) * sy));
+        key[i] = {hilbert_index(65536u, cx, cy), (int)i};
+    }
+    std::sort(key.begin(), key.end());
+    for (size_t i = 0; i < n; ++i) ord[i] = key[i].second;
+    return ord;
+}
+
+static std::vector<int> identity_order(int n)
+{
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    return ord;
 }
 
 extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D, int nl, int kmax,
@@ -446,6 +500,26 @@ extern "C" int fgpu_mesh_init(fgpu_context *c, int nod2D, int elem2D, int edge2D
         UP(p_i, nrec.data(), nrec.size()); m.node_rec = p_i;
     }
     UP(p_i, kneed.data(), kneed.size()); c->d_kneed = p_i;
+    {
+        // processing order of the fused step's column blocks: nodes by the
+        // Hilbert index of their coordinates, edges by that of their midpoints
+        std::vector<double> ex(E2), ey(E2);
+        for (int e = 0; e < E2; ++e) {
+            const int n1 = ed[2 * e], n2 = ed[2 * e + 1];
+            ex[e] = 0.5 * (coord_nod2D[2 * n1] + coord_nod2D[2 * n2]);
+            ey[e] = 0.5 * (coord_nod2D[2 * n1 + 1] + coord_nod2D[2 * n2 + 1]);
+        }
+        std::vector<double> nx(N2), ny(N2);
+        for (int n = 0; n < N2; ++n) {
+            nx[n] = coord_nod2D[2 * n];
+            ny[n] = coord_nod2D[2 * n + 1];
+        }
+        const bool hil = c->step_order != 0;
+        const std::vector<int> no = hil ? hilbert_order(nx, ny) : identity_order(N2);
+        const std::vector<int> eo = hil ? hilbert_order(ex, ey) : identity_order(E2);
+        UP(p_i, no.data(), no.size()); m.node_ord = p_i;
+        UP(p_i, eo.data(), eo.size()); m.edge_ord = p_i;
+    }
     // nboundary_lay of MUSCL (muscl_adv_init, oce_muscl_adv.F90:91-157): nl-1,
     // reset to 0 at both nodes of a boundary edge, else the minimum over the
     // node's edges of min(nlevels of the two triangles) - 1

@@ -44,7 +44,9 @@ __device__ __forceinline__ double rld(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// column of this block, XCD-contiguous: blocks b, b+8, ... run on one XCD
+// position of this block's column in the processing order (the node or edge
+// is node_ord / edge_ord of it), XCD-contiguous: blocks b, b+8, ... run on one
+// XCD and take consecutive positions
 __device__ __forceinline__ int xcd_column(int ncol)
 {
     const int per = (int)(gridDim.x / NXCD);
@@ -70,6 +72,25 @@ __device__ __forceinline__ double seg_vflux(int seg, double u1, double v1, doubl
     if (seg == 0 || seg == 3) return (-v1 * dX1 + u1 * dY1) * h1;
     if (seg == 1 || seg == 4) return (v2 * dX2 - u2 * dY2) * h2;
     return (-v1 * dX1 + u1 * dY1) * h1 + (v2 * dX2 - u2 * dY2) * h2;
+}
+
+// limited antidiffusive edge flux as seen from a node (b3 horizontal,
+// oce_adv_tra_fct.F90:483-497): R of this node (rpn, rmn) and of the edge's
+// other node (rpo, rmo); q = 2e + side, side 1 when this node is the edge's
+// second node
+__device__ __forceinline__ double limit_edge_flux(double fl, int q, double rpn, double rmn, double rpo, double rmo)
+{
+    const double rp1 = (q & 1) ? rpo : rpn, rm1 = (q & 1) ? rmo : rmn;
+    const double rp2 = (q & 1) ? rpn : rpo, rm2 = (q & 1) ? rmn : rmo;
+    double ae = 1.0;
+    if (fl >= 0.0) {
+        ae = dmn(ae, rp1);
+        ae = dmn(ae, rm2);
+    } else {
+        ae = dmn(ae, rm1);
+        ae = dmn(ae, rp2);
+    }
+    return ae * fl;
 }
 
 // metadata of a node's edges, one edge per lane (lane < deg)
@@ -111,8 +132,9 @@ __device__ __forceinline__ NodeEdges load_node_edges(const DevMesh &m, int n)
 template <int LB>
 __global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int *kneed, int tpb)
 {
-    const int n = xcd_column(m.N2);
-    if (n >= m.N2) return;
+    const int jn = xcd_column(m.N2);
+    if (jn >= m.N2) return;
+    const int n = m.node_ord[jn];
     const int k0 = kneed[n], nzmax = m.nlevn[n];
     if (k0 > nzmax - 1) return;
     const int L = m.L;
@@ -166,8 +188,9 @@ __global__ void __launch_bounds__(LB) kG_node(DevMesh m, DevTracers t, const int
 template <int LB>
 __global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(6, 8))) kA_edge(DevMesh m, DevDyn d, DevTracers t, int tpb)
 {
-    const int e = xcd_column(m.E2);
-    if (e >= m.E2) return;
+    const int je = xcd_column(m.E2);
+    if (je >= m.E2) return;
+    const int e = m.edge_ord[je];
     const int k = threadIdx.x + 1;
     const int L = m.L;
     const int *ri = m.erec_i + (size_t)e * ERI;
@@ -326,8 +349,9 @@ __device__ __forceinline__ double col_qr4c(int j, int nzmin, int nzmax, const do
 template <int LB, bool XV, bool WS>
 __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t, double dt, int tpb)
 {
-    const int n = xcd_column(m.N2);
-    if (n >= m.N2) return;
+    const int j = xcd_column(m.N2);
+    if (j >= m.N2) return;
+    const int n = m.node_ord[j];
     __shared__ double sT[LB + 2], sA[LB + 2], sW[LB + 2], sWe[LB + 2], sAr[LB + 2], sZ[LB + 2], sZb[LB + 2];
     __shared__ double sLo[LB + 2], sHn[XV ? LB + 2 : 1], sHo[XV ? LB + 2 : 1];
     constexpr int NW = WS ? LB + 2 : 1;
@@ -490,8 +514,9 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
 template <int LB>
 __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt, int tpb)
 {
-    const int n = xcd_column(m.N2);
-    if (n >= m.N2) return;
+    const int j = xcd_column(m.N2);
+    if (j >= m.N2) return;
+    const int n = m.node_ord[j];
     __shared__ double sMx[LB + 2], sMn[LB + 2];
     const int tid = threadIdx.x, k = tid + 1;
     const int L = m.L, NL = m.NL;
@@ -602,9 +627,9 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
                                                     int tpb)
 {
     const int w = threadIdx.x / LB;
-    const int n0 = xcd_column(m.N2) * CPB + w;
-    const bool valid = n0 < m.N2;
-    const int n = valid ? n0 : m.N2 - 1;
+    const int j0 = xcd_column(m.N2) * CPB + w;
+    const bool valid = j0 < m.N2;
+    const int n = m.node_ord[valid ? j0 : m.N2 - 1];
     // LDS reuse: Z_n lives in sAv's rows until the tracer loop starts, and the
     // Thomas sweep's c' rows of the first two tracers of a chunk live in sRp
     // and sRm, which are dead between a chunk's flux phase and its sweep
@@ -750,28 +775,55 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
             }
             const double *ah = t.a_h + t.sE * tr;
             const double rpn = (act && fct) ? sRp[tid] : 1.0, rmn = (act && fct) ? sRm[tid] : 1.0;
-            for (int j = 0; j < E.deg; ++j) {
-                const int q = rl(E.q, j), mo = rl(E.other, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
-                if (!(act && k >= a0 && k <= a1)) continue;
-                double fl = ah[(size_t)(q >> 1) * L + k - 1];
-                if (fct) {
-                    // b3 horizontal (oce_adv_tra_fct.F90:483-497) with R of both nodes
-                    const size_t oi = (size_t)mo * L + k - 1;
-                    const double rpo = t.rp[t.sN * tr + oi], rmo = t.rm[t.sN * tr + oi];
-                    const double rp1 = (q & 1) ? rpo : rpn, rm1 = (q & 1) ? rmo : rmn;
-                    const double rp2 = (q & 1) ? rpn : rpo, rm2 = (q & 1) ? rmn : rmo;
-                    double ae = 1.0;
-                    if (fl >= 0.0) {
-                        ae = dmn(ae, rp1);
-                        ae = dmn(ae, rm2);
-                    } else {
-                        ae = dmn(ae, rm1);
-                        ae = dmn(ae, rp2);
+            // 128 lanes (columns of 65-127 layers): edges in batches of two,
+            // the batch's loads (clamped to the column) issued together, then
+            // added in ascending edge order (at 1.3 M columns of up to 70
+            // layers and 10 tracers: 74 -> 66 ms per launch). At 64 lanes the
+            // extra registers cost more than the overlap gains (0.76 -> 0.79 ms
+            // at 127 k columns), so there the loop takes one edge at a time.
+            if constexpr (LB == 128) {
+                constexpr int KD_NG = 2;
+                const int kk = min(k, L) - 1;
+                for (int j0 = 0; j0 < E.deg; j0 += KD_NG) {
+                    double fl_[KD_NG], rpo_[KD_NG], rmo_[KD_NG];
+                    int q_[KD_NG];
+                    bool ok_[KD_NG];
+#pragma unroll
+                    for (int u = 0; u < KD_NG; ++u) {
+                        const bool real = j0 + u < E.deg;
+                        const int j = real ? j0 + u : 0;
+                        const int q = rl(E.q, j), mo = rl(E.other, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                        ok_[u] = real && act && k >= a0 && k <= a1;
+                        q_[u] = q;
+                        fl_[u] = ah[(size_t)(q >> 1) * L + kk];
+                        if (fct) {
+                            const size_t oi = (size_t)mo * L + kk;
+                            rpo_[u] = t.rp[t.sN * tr + oi];
+                            rmo_[u] = t.rm[t.sN * tr + oi];
+                        }
                     }
-                    fl = ae * fl;
+#pragma unroll
+                    for (int u = 0; u < KD_NG; ++u) {
+                        if (!ok_[u]) continue;
+                        const int q = q_[u];
+                        double fl = fl_[u];
+                        if (fct) fl = limit_edge_flux(fl, q, rpn, rmn, rpo_[u], rmo_[u]);
+                        if (q & 1) h = h - fl * dt / av_;
+                        else h = h + fl * dt / av_;
+                    }
                 }
-                if (q & 1) h = h - fl * dt / av_;
-                else h = h + fl * dt / av_;
+            } else {
+                for (int j = 0; j < E.deg; ++j) {
+                    const int q = rl(E.q, j), mo = rl(E.other, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                    if (!(act && k >= a0 && k <= a1)) continue;
+                    double fl = ah[(size_t)(q >> 1) * L + k - 1];
+                    if (fct) {
+                        const size_t oi = (size_t)mo * L + k - 1;
+                        fl = limit_edge_flux(fl, q, rpn, rmn, t.rp[t.sN * tr + oi], t.rm[t.sN * tr + oi]);
+                    }
+                    if (q & 1) h = h - fl * dt / av_;
+                    else h = h + fl * dt / av_;
+                }
             }
             double del = 0.0 + h + v;
             // explicit diffusion in diff_tracers_ale's order (oce_ale_tracer.F90:595-626):
@@ -864,6 +916,308 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
     }
 }
 
+// ------------------------------------------------------------------------------
+// K_D with one wave per column (plain step: no W split, no explicit diffusion)
+// ------------------------------------------------------------------------------
+// One wave per node column, S levels per lane (level k = lane + 1 + 64 s), so
+// every exchange between levels of a column stays inside one wave and needs
+// no workgroup barrier. The tracers of the block are taken in chunks of TCH.
+// SW: CPB column waves plus one sweep wave per block. In phase p the column
+//     waves compute the limited fluxes, T* and the tridiagonal right-hand
+//     sides of chunk p into one of two LDS buffers, while the sweep wave runs
+//     the Thomas sweeps of chunk p-1 (one lane per (column, tracer) pair) from
+//     the other buffer and stores that chunk's T^{n+1}; the serial sweeps
+//     overlap the level-parallel work of the next chunk.
+// !SW: CPB independent column waves, each of which sweeps and stores its own
+//     column after each chunk.
+// Same arithmetic as kD_node, so the results are bit-identical to it. Which of
+// the three is fastest depends on the workload (the autotuner picks): at
+// 1.3 M columns of up to 70 levels and 10 tracers the sweep wave gives
+// 63.5 ms per launch against 66.8 ms lock-step; at 127 k columns of up to
+// 47 levels and 2 tracers (one chunk, nothing to overlap) the lock-step
+// kernel is fastest (0.76 ms, against 0.84 ms wave-local).
+
+// ordering of LDS accesses between the lanes of one wave: prior LDS writes
+// complete before later accesses, and the compiler keeps the order
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int S, int TCH, int CPB, bool SW>
+__global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0))) kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt,
+                                                                      int clamp_tracer, int tpb)
+{
+    constexpr int RS = 64 * S + 2;   // LDS row: levels 1..64 S, index k - 1, plus guards
+    constexpr int NB = SW ? 2 : 1;   // chunk buffers
+    __shared__ double sRpA[CPB][RS], sRmA[CPB][RS], sAvA[CPB][RS];
+    __shared__ double saA[CPB][RS], sbA[CPB][RS], scA[CPB][RS], scpA[CPB][RS];
+    __shared__ double sTsA[NB][CPB][TCH][RS], stpA[NB][CPB][TCH][RS];
+    __shared__ int sNz[CPB][2];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool sweeper = SW && w == CPB;
+    const int L = m.L, NL = m.NL;
+    const int col0 = xcd_column(m.N2) * CPB;
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    const int nchunk = (tre - trb + TCH - 1) / TCH;
+
+    // ---- column waves: geometry and the tracer-independent tridiagonal rows
+    const int wc_ = sweeper ? 0 : w;
+    const int j0 = col0 + wc_;
+    const bool valid = !sweeper && j0 < m.N2;
+    const int n = m.node_ord[j0 < m.N2 ? j0 : m.N2 - 1];
+    const int nzmin = valid ? m.ulevn[n] : 1, nzmax = valid ? m.nlevn[n] : 0;
+    double *sRp = sRpA[wc_], *sRm = sRmA[wc_], *sAv = sAvA[wc_], *sZ = sAvA[wc_];
+    double *sa = saA[wc_], *sb = sbA[wc_], *sc = scA[wc_];
+    int kk_[S];
+    bool act_[S];
+    size_t ci_[S];
+    double hn_[S], hh_[S], av_[S], arv_[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int k = lane + 1 + 64 * s;
+        kk_[s] = k;
+        act_[s] = k >= nzmin && k <= nzmax - 1;
+        ci_[s] = (size_t)n * L + (k <= L ? k - 1 : L - 1);
+        hn_[s] = 1.0; hh_[s] = 0.0; av_[s] = 1.0; arv_[s] = 0.0;
+        if (act_[s]) {
+            hn_[s] = m.hnode_new[ci_[s]];
+            hh_[s] = m.hnode[ci_[s]];
+            av_[s] = m.areasvol[(size_t)n * NL + k - 1];
+            arv_[s] = m.area[(size_t)n * NL + k - 1];
+        }
+    }
+    NodeEdges E;
+    if (!sweeper) {
+        E = load_node_edges(m, n);
+        if (lane == 0) { sNz[w][0] = nzmin; sNz[w][1] = nzmax; }
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            if (act_[s]) sZ[kk_[s] - 1] = m.zn_vd[ci_[s]];
+        wave_sync();
+        // tridiagonal coefficients (oce_ale_tracer.F90:1098-1175)
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!act_[s]) continue;
+            const int k = kk_[s];
+            const double *kv = d.Kv + (size_t)n * NL;
+            const double zinv = 1.0 * dt;
+            double a = 0.0, b, c;
+            if (k == nzmin) {
+                const double zinv2 = 1.0 / (sZ[k - 1] - sZ[k]);
+                c = -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_[s];
+                b = -c + hn_[s];
+            } else if (k <= nzmax - 2) {
+                const double zinv1 = 1.0 / (sZ[k - 2] - sZ[k - 1]);
+                const double zinv2 = 1.0 / (sZ[k - 1] - sZ[k]);
+                a = -(kv[k - 1]) * zinv1 * zinv * (arv_[s] / av_[s]);
+                c = -(kv[k]) * zinv2 * zinv * m.area[(size_t)n * NL + k] / av_[s];
+                b = -a - c + hn_[s];
+            } else {
+                const double zinv1 = 1.0 / (sZ[k - 2] - sZ[k - 1]);
+                a = -(kv[k - 1]) * zinv1 * zinv * (arv_[s] / av_[s]);
+                c = 0.0;
+                b = -a + hn_[s];
+            }
+            sa[k - 1] = a;
+            sb[k - 1] = b;
+            sc[k - 1] = c;
+        }
+    }
+    if (SW) __syncthreads();
+
+    // Thomas sweeps of chunk q (oce_ale_tracer.F90:1406-1430), one lane per
+    // (column, tracer) of the columns wlo..wlo+nw-1 (every lane of a column
+    // writes the same c' row), then T^{n+1} of those columns with the
+    // salinity clamp
+    auto sweep_store = [&](int q, int wlo, int nw) {
+        const int t0 = trb + q * TCH, nch = min(TCH, tre - t0), bf = SW ? (q & 1) : 0;
+        if (lane < nw * nch) {
+            const int wc = wlo + lane / nch, c = lane % nch;
+            const int z0 = sNz[wc][0], z1 = sNz[wc][1];
+            if (z1 - 1 >= z0 && t.ivd[t0 + c]) {
+                const double *a_ = saA[wc], *b_ = sbA[wc], *c_ = scA[wc];
+                double *tp_ = stpA[bf][wc][c], *ts = sTsA[bf][wc][c], *cp_ = scpA[wc];
+                double cp = c_[z0 - 1] / b_[z0 - 1];
+                double tp = tp_[z0 - 1] / b_[z0 - 1];
+                cp_[z0 - 1] = cp;
+                tp_[z0 - 1] = tp;
+                for (int nz = z0 + 1; nz <= z1 - 1; ++nz) {
+                    const double an = a_[nz - 1];
+                    const double mm = b_[nz - 1] - cp * an;
+                    cp = c_[nz - 1] / mm;
+                    tp = (tp_[nz - 1] - tp * an) / mm;
+                    cp_[nz - 1] = cp;
+                    tp_[nz - 1] = tp;
+                }
+                double x = tp_[z1 - 2];
+                ts[z1 - 2] = ts[z1 - 2] + x;
+                for (int nz = z1 - 2; nz >= z0; --nz) {
+                    x = tp_[nz - 1] - cp_[nz - 1] * x;
+                    ts[nz - 1] = ts[nz - 1] + x;
+                }
+            }
+        }
+        wave_sync();
+        for (int wc = wlo; wc < wlo + nw; ++wc) {
+            if (col0 + wc >= m.N2) break;
+            const int nn = m.node_ord[col0 + wc];
+            const int z0 = sNz[wc][0], z1 = sNz[wc][1];
+            for (int c = 0; c < nch; ++c) {
+                const int tr = t0 + c;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int k = lane + 1 + 64 * s;
+                    if (k < z0 || k > z1 - 1) continue;
+                    double v = sTsA[bf][wc][c][k - 1];
+                    if (tr == clamp_tracer) {
+                        if (v > 45.0) v = 45.0;
+                        if (v < 3.0) v = 3.0;
+                    }
+                    t.T[t.sN * tr + (size_t)nn * L + k - 1] = v;
+                }
+            }
+        }
+    };
+
+    for (int p = 0; p <= nchunk; ++p) {
+        if (!sweeper && p < nchunk) {
+            // ---- chunk p: limited fluxes, flux2dtracer, T*, right-hand sides
+            const int t0 = trb + p * TCH, nch = min(TCH, tre - t0), bf = SW ? (p & 1) : 0;
+            for (int c = 0; c < nch; ++c) {
+                const int tr = t0 + c;
+                double *T = t.T + t.sN * tr;
+                double *Told = t.Told + t.sOld * tr;
+                const int abo = t.abo[tr];
+                const bool fct = t.lim[tr] == LIM_FCT;
+                const double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
+                wave_sync();
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int k = kk_[s];
+                    if (fct && k <= L) {
+                        sRp[k - 1] = t.rp[t.sN * tr + ci_[s]];
+                        sRm[k - 1] = t.rm[t.sN * tr + ci_[s]];
+                    }
+                }
+                wave_sync();
+                // limited vertical antidiffusive flux at interface k (oce_adv_tra_fct.F90:425-456)
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int k = kk_[s];
+                    if (k >= nzmin && k <= nzmax) {
+                        const double flux = av[k - 1];
+                        double v = flux;
+                        if (fct && k <= nzmax - 1) {
+                            double ae = 1.0;
+                            if (k == nzmin) {
+                                if (flux >= 0.0) ae = dmn(ae, sRp[k - 1]);
+                                else ae = dmn(ae, sRm[k - 1]);
+                            } else {
+                                if (flux >= 0.0) {
+                                    ae = dmn(ae, sRm[k - 2]);
+                                    ae = dmn(ae, sRp[k - 1]);
+                                } else {
+                                    ae = dmn(ae, sRp[k - 2]);
+                                    ae = dmn(ae, sRm[k - 1]);
+                                }
+                            }
+                            v = ae * flux;
+                        }
+                        sAv[k - 1] = v;
+                    }
+                }
+                wave_sync();
+                double Tk_[S], v_[S], h_[S], rpn_[S], rmn_[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int k = kk_[s];
+                    Tk_[s] = 0.0; v_[s] = 0.0; h_[s] = 0.0;
+                    if (act_[s]) {
+                        Tk_[s] = T[ci_[s]];
+                        if (fct) v_[s] = v_[s] - Tk_[s] * hh_[s] + t.fct_lo[t.sN * tr + ci_[s]] * hn_[s];
+                        v_[s] = v_[s] + (sAv[k - 1] - sAv[k]) * dt / av_[s];
+                    }
+                    rpn_[s] = (act_[s] && fct) ? sRp[k - 1] : 1.0;
+                    rmn_[s] = (act_[s] && fct) ? sRm[k - 1] : 1.0;
+                }
+                const double *ah = t.a_h + t.sE * tr;
+                for (int j = 0; j < E.deg; ++j) {
+                    const int q = rl(E.q, j), mo = rl(E.other, j), a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                    double fl_[S], rpo_[S], rmo_[S];
+                    bool ok_[S];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {   // the slots' loads together
+                        const int k = kk_[s];
+                        ok_[s] = act_[s] && k >= a0 && k <= a1;
+                        fl_[s] = 0.0; rpo_[s] = 1.0; rmo_[s] = 1.0;
+                        if (ok_[s]) {
+                            fl_[s] = ah[(size_t)(q >> 1) * L + k - 1];
+                            if (fct) {
+                                const size_t oi = (size_t)mo * L + k - 1;
+                                rpo_[s] = t.rp[t.sN * tr + oi];
+                                rmo_[s] = t.rm[t.sN * tr + oi];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        if (!ok_[s]) continue;
+                        double fl = fl_[s];
+                        if (fct) fl = limit_edge_flux(fl, q, rpn_[s], rmn_[s], rpo_[s], rmo_[s]);
+                        if (q & 1) h_[s] = h_[s] - fl * dt / av_[s];
+                        else h_[s] = h_[s] + fl * dt / av_[s];
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (!act_[s]) continue;
+                    const size_t ci = ci_[s];
+                    double del = 0.0 + h_[s] + v_[s];
+                    del = del + Tk_[s] * (hh_[s] - hn_[s]);
+                    sTsA[bf][w][c][kk_[s] - 1] = Tk_[s] + del / hn_[s];
+                    // valuesold rotation (oce_tracer_mod.F90:94-123)
+                    if (abo == 2) {
+                        Told[ci] = Tk_[s];
+                    } else {
+                        Told[2 * ci + 1] = Told[2 * ci];
+                        Told[2 * ci] = Tk_[s];
+                    }
+                }
+            }
+            wave_sync();
+            // right-hand sides (oce_ale_tracer.F90:1206-1229, 1377-1383)
+            for (int c = 0; c < nch; ++c) {
+                const int tr = t0 + c;
+                if (!t.ivd[tr]) continue;
+                const double *ts = sTsA[bf][w][c];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (!act_[s]) continue;
+                    const int k = kk_[s], i = k - 1;
+                    const double dz = hn_[s];
+                    double r;
+                    if (k == nzmin) r = -(sb[i] - dz) * ts[i] - sc[i] * ts[i + 1];
+                    else if (k <= nzmax - 2) r = -sa[i] * ts[i - 1] - (sb[i] - dz) * ts[i] - sc[i] * ts[i + 1];
+                    else r = -sa[i] * ts[i - 1] - (sb[i] - dz) * ts[i];
+                    if (bc_every_layer(t.id[tr], d)) r = r + bc_surface_dev(t.id[tr], dt, d, m.N2, L, n, k, ts[i]);
+                    else if (k == nzmin) r = r + bc_surface_dev(t.id[tr], dt, d, m.N2, L, n, nzmin, ts[i]);
+                    stpA[bf][w][c][i] = r;
+                }
+            }
+            if (!SW) {   // this wave sweeps and stores its own column
+                wave_sync();
+                sweep_store(p, w, 1);
+            }
+        } else if (sweeper && p >= 1) {
+            sweep_store(p - 1, 0, CPB);
+        }
+        if (SW) __syncthreads();
+    }
+}
+
 // Z levels of diff_ver_part_impl_ale rebuilt bottom-up from hnode_new and
 // zbar_n_bot (oce_ale_tracer.F90:1088-1096), one thread per column
 __global__ void k_zn_vdiff(DevMesh m, double *zn, double *zbn)
@@ -904,6 +1258,72 @@ int fgpu_compute_zn(fgpu_context *c)
     FGPU_HIP(hipStreamSynchronize(c->stream));
     return FGPU_OK;
 }
+// Extra terms compiled into K_D: bit 0 the W split (w_i rows of the implicit
+// diffusion for non-FCT tracers), bit 1 explicit diffusion increments (Ki,
+// Redi, i_vert_diff off) and the Redi K33 rows. Only the plain step (0) has
+// the one-wave-per-column variants.
+int fgpu_kd_extras(const fgpu_context *c)
+{
+    bool dx = c->d.hdiff || c->d.redi;
+    for (int v : c->h_ivd) dx = dx || !v;
+    return (c->d.wsplit ? 1 : 0) | (dx ? 2 : 0);
+}
+
+// The Thomas chunk and columns per block K_D runs with, for the requested
+// ones: several columns per block take chunks of 2 (or 4 at four 64-lane
+// columns, or two columns of up to 128 levels); several columns of more than
+// 64 levels need the sweep-wave K_D.
+void fgpu_kd_shape(const fgpu_context *c, int *tch_out, int *cpb_out)
+{
+    int tch = (c->step_tch == 4 || c->step_tch == 8) ? c->step_tch : 2;
+    int cpb = c->step_cpb;
+    if (fgpu_kd_extras(c)) {   // the W split or explicit diffusion terms: one lock-step shape
+        tch = 2;
+        cpb = 1;
+    } else if (c->m.NL <= 64) {
+        if (cpb == 8) tch = 2;
+        else if (cpb == 4) tch = tch == 4 ? 4 : 2;
+        else cpb = 1;
+    } else if (c->step_kdp && cpb >= 2) {
+        cpb = 2;
+        tch = tch == 4 ? 4 : 2;
+    } else {
+        cpb = 1;
+    }
+    *tch_out = tch;
+    *cpb_out = cpb;
+}
+
+// K_D with one wave per column (S levels per lane: 1 up to 64 interfaces, 2 up
+// to 128), with a sweep wave per block (SW) or each column wave sweeping its
+// own column, for the Thomas chunk and columns per block of the launch shape
+template <int S, int TCH, int CPB, bool SW>
+static void launch_kdp(fgpu_context *c, unsigned ny, double dt, int clamp, int tpb)
+{
+    const DevMesh &m = c->m;
+    hipLaunchKernelGGL((kDp_node<S, TCH, CPB, SW>), dim3(xcd_grid((m.N2 + CPB - 1) / CPB), ny),
+                       dim3(64 * (CPB + (SW ? 1 : 0))), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);
+}
+
+template <bool SW>
+static void launch_kdp_shape(fgpu_context *c, unsigned ny, double dt, int clamp, int tpb, int tch, int cpb)
+{
+    if (c->m.NL <= 64) {
+        if (cpb == 8) launch_kdp<1, 2, 8, SW>(c, ny, dt, clamp, tpb);
+        else if (cpb == 4 && tch == 4) launch_kdp<1, 4, 4, SW>(c, ny, dt, clamp, tpb);
+        else if (cpb == 4) launch_kdp<1, 2, 4, SW>(c, ny, dt, clamp, tpb);
+        else if (tch == 8) launch_kdp<1, 8, 1, SW>(c, ny, dt, clamp, tpb);
+        else if (tch == 4) launch_kdp<1, 4, 1, SW>(c, ny, dt, clamp, tpb);
+        else launch_kdp<1, 2, 1, SW>(c, ny, dt, clamp, tpb);
+    } else {
+        if (cpb >= 2 && tch == 4) launch_kdp<2, 4, 2, SW>(c, ny, dt, clamp, tpb);
+        else if (cpb >= 2) launch_kdp<2, 2, 2, SW>(c, ny, dt, clamp, tpb);
+        else if (tch == 8) launch_kdp<2, 8, 1, SW>(c, ny, dt, clamp, tpb);
+        else if (tch == 4) launch_kdp<2, 4, 1, SW>(c, ny, dt, clamp, tpb);
+        else launch_kdp<2, 2, 1, SW>(c, ny, dt, clamp, tpb);
+    }
+}
+
 int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
 {
     const DevMesh &m = c->m;
@@ -913,14 +1333,12 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     // tracers per block: 1 runs every tracer's chain in its own waves
     const int tpb = std::max(1, std::min(c->step_tpb, nt));
     const unsigned ny = (unsigned)((nt + tpb - 1) / tpb);
-    const int tch = (c->step_tch == 4 || c->step_tch == 8) ? c->step_tch : 2;
-    const int cpb = (c->step_cpb == 4 || c->step_cpb == 8) ? c->step_cpb : 1;   // columns per block of K_D (64-lane columns)
+    int tch, cpb;   // Thomas chunk and columns per block of K_D
+    fgpu_kd_shape(c, &tch, &cpb);
     bool xv = false;   // some tracer uses CDIFF or PPM
     for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
     const bool ws = c->d.wsplit != 0;   // W split: implicit upwind LO step, w_i rows of the diffusion
-    bool dx = c->d.hdiff || c->d.redi;   // explicit diffusion increments or Redi K33 in K_D
-    for (int v : c->h_ivd) dx = dx || !v;
-    const int xf = (ws ? 1 : 0) | (dx ? 2 : 0);
+    const int xf = fgpu_kd_extras(c);
     {   // explicit diffusion terms from T^n (no-op without Ki, Redi or explicit vertical diffusion)
         int rc = fgpu_launch_diff(c, 0, nt, dt);
         if (rc) return rc;
@@ -950,7 +1368,11 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \
-            if (xf == 1)                                                                                               \
+            if (xf == 0 && c->step_kdp == KD_SWEEPWAVE)                                                                \
+                launch_kdp_shape<true>(c, ny, dt, clamp, tpb, tch, cpb);                                               \
+            else if (xf == 0 && c->step_kdp == KD_WAVELOCAL)                                                           \
+                launch_kdp_shape<false>(c, ny, dt, clamp, tpb, tch, cpb);                                              \
+            else if (xf == 1)                                                                                          \
                 hipLaunchKernelGGL((kD_node<LBV, 2, 1, 1>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (xf == 2)                                                                                          \
                 hipLaunchKernelGGL((kD_node<LBV, 2, 1, 2>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \

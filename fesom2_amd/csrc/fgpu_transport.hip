@@ -1112,21 +1112,26 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     };
     FGPU_HIP(hipMemcpyAsync(save, c->t.T, sizeof(double) * nT, hipMemcpyDeviceToDevice, c->stream));
     FGPU_HIP(hipMemcpyAsync(save + nT, c->t.Told, sizeof(double) * nO, hipMemcpyDeviceToDevice, c->stream));
-    struct Shape { int tpb, tch, cpb; };
+    struct Shape { int tpb, tch, cpb, kd; };
     std::vector<Shape> cand;
     const bool narrow = c->m.NL <= 64;   // 64-lane columns: several per block are possible
-    for (int cpb : {1, 4, 8}) {
-        if (cpb > 1 && !narrow) continue;
-        for (int tch : {2, 4, 8}) {
-            if ((tch == 4 && nt < 3) || (tch == 8 && nt < 5) || (tch >= 4 && cpb == 8) || (tch == 8 && cpb > 1))
-                continue;
-            for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch, cpb});
-            cand.push_back({nt, tch, cpb});
+    // the variants of the last kernel; with extra terms only the lock-step one exists
+    std::vector<int> kds = {KD_LOCKSTEP};
+    if (!fgpu_kd_extras(c)) kds = {KD_LOCKSTEP, KD_SWEEPWAVE, KD_WAVELOCAL};
+    for (int kd : kds)
+        for (int cpb : {1, 2, 4, 8}) {
+            // two columns per block only for the one-wave-per-column K_D of columns over 64 levels
+            if (narrow ? cpb == 2 : cpb > (kd != KD_LOCKSTEP ? 2 : 1)) continue;
+            for (int tch : {2, 4, 8}) {
+                if ((tch == 4 && nt < 3) || (tch == 8 && nt < 5) || (tch >= 4 && cpb == 8) || (tch == 8 && cpb > 1))
+                    continue;
+                for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch, cpb, kd});
+                cand.push_back({nt, tch, cpb, kd});
+            }
         }
-    }
     const bool was_prof = c->prof;
     c->prof = false;
-    int best = c->step_tpb, best_tch = c->step_tch, best_cpb = c->step_cpb;
+    int best = c->step_tpb, best_tch = c->step_tch, best_cpb = c->step_cpb, best_kd = c->step_kdp;
     float best_t = -1.f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rc = FGPU_OK;
@@ -1136,6 +1141,7 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
         c->step_tpb = cand[i].tpb;
         c->step_tch = cand[i].tch;
         c->step_cpb = cand[i].cpb;
+        c->step_kdp = cand[i].kd;
         c->graph_valid = false;
         if ((rc = run_steps(c, dt, 1))) break;   // capture and warm up
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord"); break; }
@@ -1151,6 +1157,7 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
             best = cand[i].tpb;
             best_tch = cand[i].tch;
             best_cpb = cand[i].cpb;
+            best_kd = cand[i].kd;
         }
     }
     if (e0) (void)hipEventDestroy(e0);
@@ -1161,6 +1168,7 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     c->step_tpb = best;
     c->step_tch = best_tch;
     c->step_cpb = best_cpb;
+    c->step_kdp = best_kd;
     c->graph_valid = false;
     if (rc) return rc;
     if (rr) return rr;
@@ -1169,13 +1177,36 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     return FGPU_OK;
 }
 
+extern "C" int fgpu_set_step_shape(fgpu_context *c, int tracers_per_block, int thomas_chunk, int columns_per_block,
+                                   int kd_variant)
+{
+    if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
+    if (tracers_per_block < 1 || kd_variant < KD_LOCKSTEP || kd_variant > KD_WAVELOCAL)
+        return fgpu_fail(FGPU_ERR_ARG, "tracers_per_block >= 1 and kd_variant in 0..2");
+    c->step_tpb = tracers_per_block;
+    c->step_tch = thomas_chunk;
+    c->step_cpb = columns_per_block;
+    c->step_kdp = kd_variant;
+    c->graph_valid = false;
+    return FGPU_OK;
+}
+
+extern "C" int fgpu_step_variant(fgpu_context *c, int *kd_variant)
+{
+    if (!c || !kd_variant) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    *kd_variant = fgpu_kd_extras(c) ? KD_LOCKSTEP : c->step_kdp;
+    return FGPU_OK;
+}
+
 extern "C" int fgpu_step_shape(fgpu_context *c, int *tracers_per_block, int *thomas_chunk, int *columns_per_block)
 {
     if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
     const int nt = c->t.ntr > 0 ? c->t.ntr : 1;
     if (tracers_per_block) *tracers_per_block = std::max(1, std::min(c->step_tpb, nt));
-    if (thomas_chunk) *thomas_chunk = (c->step_tch == 4 || c->step_tch == 8) ? c->step_tch : 2;
-    if (columns_per_block) *columns_per_block = c->m.NL <= 64 && (c->step_cpb == 4 || c->step_cpb == 8) ? c->step_cpb : 1;
+    int tch, cpb;
+    fgpu_kd_shape(c, &tch, &cpb);
+    if (thomas_chunk) *thomas_chunk = tch;
+    if (columns_per_block) *columns_per_block = cpb;
     return FGPU_OK;
 }
 

@@ -38,7 +38,8 @@ module fesom_gpu_c
        fgpu_ssh_nn_train, fgpu_ssh_solve_learned, fgpu_ssh_solve_learned_resident, fgpu_set_uv, &
        fgpu_vert_vel_ale, fgpu_vert_vel_download, fgpu_compute_ssh_rhs_ale, fgpu_update_stiff_mat_ale, &
        fgpu_compute_hbar_ale, fgpu_set_ale_config, fgpu_set_surface_forcing_tracers, &
-       fgpu_tracer_set_valuesAB, fgpu_ssh_set_preconditioner, fgpu_ssh_solve_cg_noalloc
+       fgpu_tracer_set_valuesAB, fgpu_ssh_set_preconditioner, fgpu_ssh_solve_cg_noalloc, fgpu_step_variant, &
+       fgpu_set_step_shape
 
   interface
      type(c_ptr) function fgpu_last_error() bind(C, name="fgpu_last_error")
@@ -396,6 +397,19 @@ module fesom_gpu_c
        integer(c_int), value :: maxiter
        type(fgpu_cg_stats), intent(out) :: stats
      end function fgpu_ssh_solve_cg_noalloc
+
+     integer(c_int) function fgpu_set_step_shape(ctx, tracers_per_block, thomas_chunk, columns_per_block, &
+          kd_variant) bind(C, name="fgpu_set_step_shape")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: tracers_per_block, thomas_chunk, columns_per_block, kd_variant
+     end function fgpu_set_step_shape
+
+     integer(c_int) function fgpu_step_variant(ctx, kd_variant) bind(C, name="fgpu_step_variant")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), intent(out) :: kd_variant
+     end function fgpu_step_variant
 
      integer(c_int) function fgpu_autotune_step(ctx, dt, nsteps, best_tpb, best_ms) &
           bind(C, name="fgpu_autotune_step")

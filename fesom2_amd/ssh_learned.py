@@ -54,7 +54,7 @@ def _potential_flow(mesh, rng, nfields=3):
 
 
 def model_rhs_sequence(ctx, mesh, state, nsteps=2000, every=10, seed=0, umax=0.01, soltol=1e-12,
-                       maxiter=20000):
+                       maxiter=20000, basis=None):
     """SSH right-hand sides and solutions from a run of the plain solver.
 
     Every step sets a slowly turning momentum tendency uv_rhs (three seeded
@@ -65,6 +65,12 @@ def model_rhs_sequence(ctx, mesh, state, nsteps=2000, every=10, seed=0, umax=0.0
     step's) and solves it with the plain diagonally preconditioned CG,
     warm-started from the previous step's solution as the model does. Every
     `every`-th step's (rhs, solution) is kept.
+    basis (default: meshes over 2e7 element layers): at alpha = 1 the
+    right-hand side is affine in uv_rhs and does not read ssh_rhs_old
+    (oce_ale.F90:1803-1938), so each step's is formed on the host from the
+    device's right-hand sides of the resident flow alone and of each tendency
+    field, instead of uploading a full uv_rhs every step; it agrees with the
+    direct evaluation to rounding.
     Returns (rhs, sol), each (nsteps // every, nod2D)."""
     rng = np.random.default_rng(seed + 9100)
     fields = _potential_flow(mesh, rng)
@@ -78,11 +84,20 @@ def model_rhs_sequence(ctx, mesh, state, nsteps=2000, every=10, seed=0, umax=0.0
     x = np.zeros(mesh.nod2D)
     old = np.zeros(mesh.nod2D)
     keep_b, keep_x = [], []
+    if basis is None:
+        basis = mesh.elem2D * (mesh.nl - 1) > 2e7
+    if basis:
+        r0 = ctx.compute_ssh_rhs_ale(np.zeros_like(state.uv), old)
+        rk = [ctx.compute_ssh_rhs_ale(umax * fields[k][:, None, :] * prof[:, :, None], old) - r0
+              for k in range(fields.shape[0])]
     for s in range(nsteps):
         c = np.cos(rate * s + phase)
-        tend = np.tensordot(c, fields, 1)                          # (elem2D, 2)
-        uv_rhs = umax * tend[:, None, :] * prof[:, :, None]
-        rhs = ctx.compute_ssh_rhs_ale(uv_rhs, old)
+        if basis:
+            rhs = r0 + sum(c[k] * rk[k] for k in range(len(rk)))
+        else:
+            tend = np.tensordot(c, fields, 1)                          # (elem2D, 2)
+            uv_rhs = umax * tend[:, None, :] * prof[:, :, None]
+            rhs = ctx.compute_ssh_rhs_ale(uv_rhs, old)
         x, st = ctx.ssh_solve_cg(x, rhs, soltol=soltol, maxiter=maxiter)
         if not st["converged"]:
             raise RuntimeError(f"plain CG did not converge at step {s}: {st}")

@@ -226,13 +226,28 @@ int fgpu_profile_read(fgpu_context *ctx, int idx, char *name, int name_len, doub
 /* Autotuner of the fused step (no counterpart in the reference): times the
  * candidate launch shapes (tracers per block 1, 2, 4, ..., all, crossed with 2,
  * 4 or 8 tracers per Thomas chunk and 1, 4 or 8 columns per block of the last
- * kernel) over nsteps replayed steps, keeps the fastest
+ * kernel for columns of up to 64 levels, 1 or 2 for deeper ones) over nsteps
+ * replayed steps, keeps the fastest
  * for this context and reports its tracers per block and time. The tracer
  * state is restored afterwards; results do not depend on the choice. */
 int fgpu_autotune_step(fgpu_context *ctx, double dt, int nsteps, int *best_tpb, double *best_ms_per_step);
 /* The fused step's current launch shape (no counterpart in the reference):
- * tracers per block, tracers per Thomas chunk, columns per block of the last kernel. */
+ * tracers per block, tracers per Thomas chunk, columns per block of the last
+ * kernel as launched (a shape that needs another chunk size reports the one used). */
 int fgpu_step_shape(fgpu_context *ctx, int *tracers_per_block, int *thomas_chunk, int *columns_per_block);
+/* Which last kernel the plain step runs (no counterpart in the reference):
+ * 0 the block's waves in lock-step, the sweeps on the first wave's lanes;
+ * 1 one wave per column plus a wave per block that sweeps the previous chunk
+ *   while the column waves compute the next;
+ * 2 one wave per column that sweeps its own column.
+ * The autotuner picks it with the shape; FGPU_STEP_KD=lockstep|sweepwave|
+ * wavelocal sets it when a context is created. Results are the same bit for bit. */
+int fgpu_step_variant(fgpu_context *ctx, int *kd_variant);
+/* Sets the launch shape and last-kernel variant instead of the autotuner (no
+ * counterpart in the reference); values outside the sets above fall back as
+ * fgpu_step_shape reports. */
+int fgpu_set_step_shape(fgpu_context *ctx, int tracers_per_block, int thomas_chunk, int columns_per_block,
+                        int kd_variant);
 /* Tracer content sum_n sum_k areasvol*hnode_new*T (deterministic device sum). */
 int fgpu_tracer_content(fgpu_context *ctx, int tr_num, double *content);
 

@@ -2,10 +2,13 @@
 the oracle (bit for bit).
 
 The autotuner picks tracers per block (1, 2, 4, 8 or all), tracers per Thomas
-chunk (2, 4 or 8) and columns per block of the last kernel (1, 4 or 8, only
-for columns of at most 64 levels). Each choice instantiates different kernels,
-so each is run here with twelve tracers, on the 24-layer channel (64-lane
-kernels) and on a 70-layer channel (128-lane kernels), two steps each.
+chunk (2, 4 or 8) and columns per block of the last kernel (1, 4 or 8 for
+columns of at most 64 levels, 1 or 2 for deeper ones). Each choice
+instantiates different kernels, so each is run here with twelve tracers, on
+the 24-layer channel (64-lane kernels, one level per lane in the last kernel)
+and on a 70-layer channel (128-lane kernels, two levels per lane in the last
+kernel), two steps each, with each variant of the last kernel
+(FGPU_STEP_KD: lock-step, with a sweep wave, one wave per column).
 
 Two of the twelve tracers sit far from zero (about +2100 and -1500, as DIC
 or alkalinity do): the FCT bounds of a node next to an element without the
@@ -24,6 +27,8 @@ pytestmark = pytest.mark.gpu
 TPB = [1, 2, 4, 8, 12]
 TCH = [2, 4, 8]
 CPB = [1, 4, 8]
+KD = ["lockstep", "sweepwave", "wavelocal"]   # variants of the last kernel (FGPU_STEP_KD)
+KD_ID = {"lockstep": 0, "sweepwave": 1, "wavelocal": 2}
 
 
 def _wet(mesh):
@@ -70,9 +75,9 @@ def deep():
     return mesh, st, _oracle_steps(mesh, st, 2)
 
 
-def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nsteps=2, schemes=None):
+def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nsteps=2, schemes=None, kd=None):
     from fesom2_amd import gpu
-    for k, v in (("FGPU_STEP_TPB", tpb), ("FGPU_STEP_TCH", tch), ("FGPU_STEP_CPB", cpb)):
+    for k, v in (("FGPU_STEP_TPB", tpb), ("FGPU_STEP_TCH", tch), ("FGPU_STEP_CPB", cpb), ("FGPU_STEP_KD", kd)):
         if v is None:
             monkeypatch.delenv(k, raising=False)
         else:
@@ -85,6 +90,8 @@ def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nste
     try:
         ctx.solve_tracers_ale(st.dt, nsteps)
         shape = ctx.step_shape()
+        if kd is not None:
+            assert ctx.step_variant() == KD_ID[kd]
         out = [ctx.tracer_download(i + 1) for i in range(st.ntr)]
     finally:
         ctx.close()
@@ -99,24 +106,43 @@ def _compare(mesh, out, ref):
         np.testing.assert_array_equal(gvo[w], vo[i][w], err_msg=f"tracer {i + 1} valuesold")
 
 
+def _launched(tch, cpb, deep, kd):
+    """The (Thomas chunk, columns per block) the last kernel runs with for a
+    requested shape: several columns per block take chunks of 2, or 4 at four
+    64-lane columns or two deep columns; several deep columns need the kernel
+    with the sweep wave."""
+    if not deep:
+        if cpb == 8:
+            return 2, 8
+        if cpb == 4:
+            return (4 if tch == 4 else 2), 4
+        return tch, 1
+    if kd != "lockstep" and cpb >= 2:
+        return (4 if tch == 4 else 2), 2
+    return tch, 1
+
+
+@pytest.mark.parametrize("kd", KD)
 @pytest.mark.parametrize("cpb", CPB)
 @pytest.mark.parametrize("tch", TCH)
 @pytest.mark.parametrize("tpb", TPB)
-def test_fused_shapes_64_lanes(chan, monkeypatch, tpb, tch, cpb):
+def test_fused_shapes_64_lanes(chan, monkeypatch, tpb, tch, cpb, kd):
     mesh, st, ref = chan
-    print(f"[gpu] c0, 12 tracers, shape tpb={tpb} tch={tch} cpb={cpb}", flush=True)
-    out, shape = _run(mesh, st, monkeypatch, tpb, tch, cpb)
-    assert shape == (tpb, tch, cpb)
+    print(f"[gpu] c0, 12 tracers, shape tpb={tpb} tch={tch} cpb={cpb} K_D {kd}", flush=True)
+    out, shape = _run(mesh, st, monkeypatch, tpb, tch, cpb, kd=kd)
+    assert shape == (tpb, *_launched(tch, cpb, False, kd))
     _compare(mesh, out, ref)
 
 
+@pytest.mark.parametrize("kd", KD)
+@pytest.mark.parametrize("cpb", [1, 2])
 @pytest.mark.parametrize("tch", TCH)
 @pytest.mark.parametrize("tpb", TPB)
-def test_fused_shapes_128_lanes(deep, monkeypatch, tpb, tch):
+def test_fused_shapes_128_lanes(deep, monkeypatch, tpb, tch, cpb, kd):
     mesh, st, ref = deep
-    print(f"[gpu] 70 layers, 12 tracers, shape tpb={tpb} tch={tch}", flush=True)
-    out, shape = _run(mesh, st, monkeypatch, tpb, tch, 4)
-    assert shape == (tpb, tch, 1)   # several columns per block need columns of <= 64 levels
+    print(f"[gpu] 70 layers, 12 tracers, shape tpb={tpb} tch={tch} cpb={cpb} K_D {kd}", flush=True)
+    out, shape = _run(mesh, st, monkeypatch, tpb, tch, cpb, kd=kd)
+    assert shape == (tpb, *_launched(tch, cpb, True, kd))
     _compare(mesh, out, ref)
 
 
@@ -133,7 +159,7 @@ def test_autotuned_deep(deep, monkeypatch):
     its pick stays bit-identical."""
     from fesom2_amd import gpu
     mesh, st, ref = deep
-    for k in ("FGPU_STEP_TPB", "FGPU_STEP_TCH", "FGPU_STEP_CPB", "FGPU_STEP"):
+    for k in ("FGPU_STEP_TPB", "FGPU_STEP_TCH", "FGPU_STEP_CPB", "FGPU_STEP", "FGPU_STEP_KD"):
         monkeypatch.delenv(k, raising=False)
     print("[gpu] autotune 70 layers", flush=True)
     ctx = gpu.context_for(mesh, st)
