@@ -175,9 +175,10 @@ def main():
                     help="plain-solver steps whose right-hand sides (every 4th) train the network")
     ap.add_argument("--learn-epochs", type=int, default=20)
     ap.add_argument("--shape", default=None,
-                    help="TPB,TCH,CPB[,KD]: fixed launch shape of the fused step (and variant of its last "
-                         "kernel: 0 lock-step, 1 sweep wave, 2 one wave per column) instead of the "
-                         "autotuner's (a profiled run then holds only that shape's launches)")
+                    help="TPB,TCH,CPB[,KD[,KC]]: fixed launch shape of the fused step (variant of its last "
+                         "kernel: 0 lock-step, 1 sweep wave, 2 one wave per column; tracers per block of the "
+                         "(tracer, level) bound kernel, 0 for the per-column one) instead of the autotuner's "
+                         "(a profiled run then holds only that shape's launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is design-only "
@@ -189,6 +190,7 @@ def main():
         os.environ["FGPU_STEP_TPB"], os.environ["FGPU_STEP_TCH"], os.environ["FGPU_STEP_CPB"] = (
             str(sv[0]), str(sv[1]), str(sv[2]))
         os.environ["FGPU_STEP_KD"] = ["lockstep", "sweepwave", "wavelocal"][sv[3] if len(sv) > 3 else 0]
+        os.environ["FGPU_STEP_KC"] = str(sv[4] if len(sv) > 4 else 0)
     from fesom2_amd import gpu, meshgen as mg
     t0 = time.perf_counter()
     mesh, state, _ = mg.make_config(args.config)
@@ -205,9 +207,9 @@ def main():
         tpb, tune_ms = ctx.autotune_step(dt, 3)
         shape = ctx.step_shape()
         print(f"[bench] autotuner: {shape[0]} tracers per block, {shape[1]} per Thomas chunk, {shape[2]} columns "
-              f"per block, last kernel variant {ctx.step_variant()} ({tune_ms:.3f} ms/step)", file=sys.stderr,
+              f"per block, last kernel variant {ctx.step_variants()} ({tune_ms:.3f} ms/step)", file=sys.stderr,
               flush=True)
-    ctx_variant = ctx.step_variant()
+    ctx_variant, ctx_kc = ctx.step_variants()
     ctx.solve_tracers_ale(dt, max(1, args.warmup))
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -326,7 +328,8 @@ def main():
         "ssh_learned": learned,
         "device_ms_per_step": ms_dev / args.steps,
         "autotuned_shape": {"tracers_per_block": shape[0], "thomas_chunk": shape[1], "columns_per_block": shape[2],
-                            "last_kernel": ["lock-step", "sweep wave", "one wave per column"][ctx_variant]},
+                            "last_kernel": ["lock-step", "sweep wave", "one wave per column"][ctx_variant],
+                            "bound_kernel_tracers_per_block": ctx_kc},
         "step_alg_GBs": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
         "step_model_bytes": int(step_model_bytes(mesh, state.ntr)),
         "step_model_frac_of_hbm": round(step_model_bytes(mesh, state.ntr) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),

@@ -35,6 +35,15 @@
 __device__ __forceinline__ double dmx(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmn(double a, double b) { return a < b ? a : b; }
 
+// ordering of LDS accesses between the lanes of one wave: prior LDS writes
+// complete before later accesses, and the compiler keeps the order
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 __device__ __forceinline__ double rld(double v, int lane)
 {
@@ -611,6 +620,133 @@ __global__ void __launch_bounds__(LB) kC_node(DevMesh m, DevTracers t, double dt
 }
 
 // ------------------------------------------------------------------------------
+// K_C over (tracer, level) items: one wave per node column and TPF tracers;
+// lane j of chunk c takes item i = 64 c + j, tracer trb + i / nw and level
+// nzmin + i % nw of the column's nw wet layers, so a wave instruction covers
+// 64 wet (tracer, level) pairs instead of one tracer's column padded to the
+// lane count. Same arithmetic per (tracer, level) as kC_node, neighbours added
+// in the same order, so the results are bit-identical to it. NCH bounds the
+// chunks: TPF x (levels of the deepest column) / 64, rounded up.
+// ------------------------------------------------------------------------------
+template <int TPF, int NCH>
+__global__ void __launch_bounds__(64) kCf_node(DevMesh m, DevTracers t, double dt)
+{
+    constexpr int RS = 130;   // LDS row per tracer: levels 1..128 (index k - 1) and a guard
+    __shared__ double sMx[TPF][RS], sMn[TPF][RS];
+    const int jn = xcd_column(m.N2);
+    if (jn >= m.N2) return;
+    const int n = m.node_ord[jn];
+    const int lane = threadIdx.x;
+    const int L = m.L, NL = m.NL;
+    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    const int nw = nzmax - nzmin;   // wet layers nzmin..nzmax-1
+    if (nw < 1) return;
+    const NodeEdges E = load_node_edges(m, n);
+    const int2 pr = *(const int2 *)(m.fct_pad + 2 * (size_t)n);
+    const int trb = blockIdx.y * TPF, tre = min(t.ntr, trb + TPF);
+    const int items = (tre - trb) * nw;
+    const float inw = 1.0f / (float)nw;
+    double lo_[NCH], p_[NCH], q_[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        lo_[c] = 0.0; p_[c] = 0.0; q_[c] = 0.0;
+        const int i = 64 * c + lane;
+        if (64 * c >= items) break;   // uniform
+        const bool on = i < items;
+        const int tri = on ? (int)(((float)i + 0.5f) * inw) : 0;
+        const int tr = trb + tri, k = nzmin + (on ? i - tri * nw : 0);
+        const bool act = on && t.lim[tr] == LIM_FCT;
+        const size_t ci = (size_t)n * L + k - 1;
+        const double *T = t.T + t.sN * tr;
+        const double *LO = t.fct_lo + t.sN * tr;
+        const double *ah = t.a_h + t.sE * tr;
+        double vmax = -1e3, vmin = 1e3, lo = 0.0, p = 0.0, q = 0.0;
+        if (act) {
+            lo = LO[ci];
+            const double tk = T[ci];
+            vmax = dmx(lo, tk);
+            vmin = dmn(lo, tk);
+            if (k < pr.x || k >= pr.y) {   // an element around n lacks level k
+                vmax = dmx(vmax, -1e3);
+                vmin = dmn(vmin, 1e3);
+            }
+            const double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
+            p = p + (dmx(0.0, av[k - 1]) + dmx(0.0, -av[k]));
+            q = q + (dmn(0.0, av[k - 1]) + dmn(0.0, -av[k]));
+        }
+        for (int j0 = 0; j0 < E.deg; j0 += NG) {
+            double lm_[NG], tm_[NG], fl_[NG];
+            int qq_[NG];
+            bool has_[NG], in_[NG];
+#pragma unroll
+            for (int u = 0; u < NG; ++u) {
+                const bool real = j0 + u < E.deg;
+                const int j = real ? j0 + u : 0;
+                const int qq = rl(E.q, j), mo = rl(E.other, j);
+                const int a0 = rl(E.nu12, j), a1 = rl(E.nl12, j);
+                const int b_nu1 = rl(E.nu1, j), b_nl1 = rl(E.nl1, j), b_nu2 = rl(E.nu2, j), b_nl2 = rl(E.nl2, j);
+                has_[u] = real && act && ((k >= b_nu1 && k <= b_nl1) || (b_nu2 > 0 && k >= b_nu2 && k <= b_nl2));
+                in_[u] = real && act && k >= a0 && k <= a1;
+                qq_[u] = qq;
+                const size_t mi = (size_t)mo * L + k - 1;
+                lm_[u] = LO[mi];
+                tm_[u] = T[mi];
+                fl_[u] = ah[(size_t)(qq >> 1) * L + k - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < NG; ++u) {
+                if (has_[u]) {
+                    vmax = dmx(vmax, dmx(lm_[u], tm_[u]));
+                    vmin = dmn(vmin, dmn(lm_[u], tm_[u]));
+                }
+                if (in_[u]) {
+                    const double fl = fl_[u];
+                    if (qq_[u] & 1) {
+                        p = p + dmx(0.0, -fl);
+                        q = q + dmn(0.0, -fl);
+                    } else {
+                        p = p + dmx(0.0, fl);
+                        q = q + dmn(0.0, fl);
+                    }
+                }
+            }
+        }
+        if (on) {
+            sMx[tri][k - 1] = vmax;
+            sMn[tri][k - 1] = vmin;
+        }
+        lo_[c] = lo;
+        p_[c] = p;
+        q_[c] = q;
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int i = 64 * c + lane;
+        if (64 * c >= items) break;
+        const bool on = i < items;
+        const int tri = on ? (int)(((float)i + 0.5f) * inw) : 0;
+        const int tr = trb + tri, k = nzmin + (on ? i - tri * nw : 0);
+        if (!(on && t.lim[tr] == LIM_FCT)) continue;
+        const size_t ci = (size_t)n * L + k - 1;
+        const double hn = m.hnode_new[ci], av_ = m.areasvol[(size_t)n * NL + k - 1];
+        const double lo = lo_[c], vmax = sMx[tri][k - 1], vmin = sMn[tri][k - 1];
+        double bmax, bmin;
+        if (k == nzmax - 1 || k == nzmin) {
+            bmax = vmax - lo;
+            bmin = vmin - lo;
+        } else {
+            bmax = dmx(dmx(sMx[tri][k - 2], vmax), sMx[tri][k]) - lo;
+            bmin = dmn(dmn(sMn[tri][k - 2], vmin), sMn[tri][k]) - lo;
+        }
+        double flux = p_[c] * dt / av_ / hn + 1e-16;
+        t.rp[t.sN * tr + ci] = dmn(1.0, bmax / flux);
+        flux = q_[c] * dt / av_ / hn - 1e-16;
+        t.rm[t.sN * tr + ci] = dmn(1.0, bmin / flux);
+    }
+}
+
+// ------------------------------------------------------------------------------
 // K_D: limited fluxes, flux2dtracer, T*, implicit vertical diffusion
 // ------------------------------------------------------------------------------
 // TCH: tracers per Thomas chunk; CPB: columns per block (one column per LB
@@ -937,14 +1073,6 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
 // 47 levels and 2 tracers (one chunk, nothing to overlap) the lock-step
 // kernel is fastest (0.76 ms, against 0.84 ms wave-local).
 
-// ordering of LDS accesses between the lanes of one wave: prior LDS writes
-// complete before later accesses, and the compiler keeps the order
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 template <int S, int TCH, int CPB, bool SW>
 __global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0))) kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt,
@@ -1028,10 +1156,32 @@ __global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0))) kDp_node(DevMesh m,
     }
     if (SW) __syncthreads();
 
+    // The forward-elimination pivots of the columns (oce_ale_tracer.F90:1406-1414):
+    // m = b - c'a and c' = c / m depend on the column only. With the sweep
+    // wave, its first CPB lanes compute the c' rows while the column waves
+    // work on the first chunk; every sweep then forms m again from b, a and
+    // the c' above (the same two operations) and divides once per level.
+    auto pivots = [&]() {
+        if (lane < CPB) {
+            const int wc = lane, z0 = sNz[wc][0], z1 = sNz[wc][1];
+            if (z1 - 1 >= z0) {
+                const double *a_ = saA[wc], *b_ = sbA[wc], *c_ = scA[wc];
+                double *cp_ = scpA[wc];
+                double cp = c_[z0 - 1] / b_[z0 - 1];
+                cp_[z0 - 1] = cp;
+                for (int nz = z0 + 1; nz <= z1 - 1; ++nz) {
+                    const double mm = b_[nz - 1] - cp * a_[nz - 1];
+                    cp = c_[nz - 1] / mm;
+                    cp_[nz - 1] = cp;
+                }
+            }
+        }
+    };
+
     // Thomas sweeps of chunk q (oce_ale_tracer.F90:1406-1430), one lane per
-    // (column, tracer) of the columns wlo..wlo+nw-1 (every lane of a column
-    // writes the same c' row), then T^{n+1} of those columns with the
-    // salinity clamp
+    // (column, tracer) of the columns wlo..wlo+nw-1 (without the sweep wave,
+    // every lane of a column writes the same c' row), then T^{n+1} of those
+    // columns with the salinity clamp
     auto sweep_store = [&](int q, int wlo, int nw) {
         const int t0 = trb + q * TCH, nch = min(TCH, tre - t0), bf = SW ? (q & 1) : 0;
         if (lane < nw * nch) {
@@ -1040,17 +1190,28 @@ __global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0))) kDp_node(DevMesh m,
             if (z1 - 1 >= z0 && t.ivd[t0 + c]) {
                 const double *a_ = saA[wc], *b_ = sbA[wc], *c_ = scA[wc];
                 double *tp_ = stpA[bf][wc][c], *ts = sTsA[bf][wc][c], *cp_ = scpA[wc];
-                double cp = c_[z0 - 1] / b_[z0 - 1];
                 double tp = tp_[z0 - 1] / b_[z0 - 1];
-                cp_[z0 - 1] = cp;
                 tp_[z0 - 1] = tp;
-                for (int nz = z0 + 1; nz <= z1 - 1; ++nz) {
-                    const double an = a_[nz - 1];
-                    const double mm = b_[nz - 1] - cp * an;
-                    cp = c_[nz - 1] / mm;
-                    tp = (tp_[nz - 1] - tp * an) / mm;
-                    cp_[nz - 1] = cp;
-                    tp_[nz - 1] = tp;
+                if constexpr (SW) {
+                    double cpa = cp_[z0 - 1];   // c' of the level above
+                    for (int nz = z0 + 1; nz <= z1 - 1; ++nz) {
+                        const double an = a_[nz - 1];
+                        const double mm = b_[nz - 1] - cpa * an;
+                        tp = (tp_[nz - 1] - tp * an) / mm;
+                        cpa = cp_[nz - 1];
+                        tp_[nz - 1] = tp;
+                    }
+                } else {
+                    double cp = c_[z0 - 1] / b_[z0 - 1];
+                    cp_[z0 - 1] = cp;
+                    for (int nz = z0 + 1; nz <= z1 - 1; ++nz) {
+                        const double an = a_[nz - 1];
+                        const double mm = b_[nz - 1] - cp * an;
+                        cp = c_[nz - 1] / mm;
+                        tp = (tp_[nz - 1] - tp * an) / mm;
+                        cp_[nz - 1] = cp;
+                        tp_[nz - 1] = tp;
+                    }
                 }
                 double x = tp_[z1 - 2];
                 ts[z1 - 2] = ts[z1 - 2] + x;
@@ -1213,6 +1374,8 @@ __global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0))) kDp_node(DevMesh m,
             }
         } else if (sweeper && p >= 1) {
             sweep_store(p - 1, 0, CPB);
+        } else if (sweeper) {
+            pivots();
         }
         if (SW) __syncthreads();
     }
@@ -1324,6 +1487,30 @@ static void launch_kdp_shape(fgpu_context *c, unsigned ny, double dt, int clamp,
     }
 }
 
+// K_C over (tracer, level) items with step_kcf tracers per block (1, 2 or 4)
+template <int TPF, int NCH>
+static void launch_kcf_t(fgpu_context *c, double dt)
+{
+    const DevMesh &m = c->m;
+    const unsigned ny = (unsigned)((c->t.ntr + TPF - 1) / TPF);
+    hipLaunchKernelGGL((kCf_node<TPF, NCH>), dim3(xcd_grid(m.N2), ny), dim3(64), 0, c->stream, m, c->t, dt);
+}
+
+static void launch_kcf(fgpu_context *c, double dt)
+{
+    const bool narrow = c->m.NL <= 64;
+    if (c->step_kcf >= 4) {
+        if (narrow) launch_kcf_t<4, 4>(c, dt);
+        else launch_kcf_t<4, 8>(c, dt);
+    } else if (c->step_kcf >= 2) {
+        if (narrow) launch_kcf_t<2, 2>(c, dt);
+        else launch_kcf_t<2, 4>(c, dt);
+    } else {
+        if (narrow) launch_kcf_t<1, 1>(c, dt);
+        else launch_kcf_t<1, 2>(c, dt);
+    }
+}
+
 int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
 {
     const DevMesh &m = c->m;
@@ -1364,7 +1551,10 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kC_node");                                                                                \
-            hipLaunchKernelGGL(kC_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->t, dt, tpb);              \
+            if (c->step_kcf > 0)                                                                                       \
+                launch_kcf(c, dt);                                                                                     \
+            else                                                                                                       \
+                hipLaunchKernelGGL(kC_node<LBV>, dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->t, dt, tpb); \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kD_node");                                                                                \

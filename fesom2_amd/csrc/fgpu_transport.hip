@@ -1160,6 +1160,34 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
             best_kd = cand[i].kd;
         }
     }
+    // then the bound kernel K_C with the chosen shape: per column over its
+    // levels (0) or over (tracer, level) items with 1, 2 or 4 tracers per block
+    int best_kcf = c->step_kcf;
+    if (!rc && best_t >= 0.f) {
+        c->step_tpb = best;
+        c->step_tch = best_tch;
+        c->step_cpb = best_cpb;
+        c->step_kdp = best_kd;
+        for (int kcf : {0, 1, 2, 4}) {
+            if (kcf == best_kcf) continue;   // timed above
+            c->step_kcf = kcf;
+            c->graph_valid = false;
+            if ((rc = run_steps(c, dt, 1))) break;
+            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord"); break; }
+            if ((rc = run_steps(c, dt, nsteps))) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fgpu_fail(FGPU_ERR_HIP, "timing the autotuner trial failed");
+                break;
+            }
+            if (ms < best_t) {
+                best_t = ms;
+                best_kcf = kcf;
+            }
+        }
+    }
+    c->step_kcf = best_kcf;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     const int rr = restore();
@@ -1178,23 +1206,25 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
 }
 
 extern "C" int fgpu_set_step_shape(fgpu_context *c, int tracers_per_block, int thomas_chunk, int columns_per_block,
-                                   int kd_variant)
+                                   int kd_variant, int kc_tracers)
 {
     if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
-    if (tracers_per_block < 1 || kd_variant < KD_LOCKSTEP || kd_variant > KD_WAVELOCAL)
-        return fgpu_fail(FGPU_ERR_ARG, "tracers_per_block >= 1 and kd_variant in 0..2");
+    if (tracers_per_block < 1 || kd_variant < KD_LOCKSTEP || kd_variant > KD_WAVELOCAL || kc_tracers < 0)
+        return fgpu_fail(FGPU_ERR_ARG, "tracers_per_block >= 1, kd_variant in 0..2 and kc_tracers >= 0");
     c->step_tpb = tracers_per_block;
     c->step_tch = thomas_chunk;
     c->step_cpb = columns_per_block;
     c->step_kdp = kd_variant;
+    c->step_kcf = kc_tracers >= 4 ? 4 : (kc_tracers >= 2 ? 2 : kc_tracers);
     c->graph_valid = false;
     return FGPU_OK;
 }
 
-extern "C" int fgpu_step_variant(fgpu_context *c, int *kd_variant)
+extern "C" int fgpu_step_variant(fgpu_context *c, int *kd_variant, int *kc_tracers)
 {
-    if (!c || !kd_variant) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c || !kd_variant || !kc_tracers) return fgpu_fail(FGPU_ERR_ARG, "null argument");
     *kd_variant = fgpu_kd_extras(c) ? KD_LOCKSTEP : c->step_kdp;
+    *kc_tracers = c->step_kcf >= 4 ? 4 : (c->step_kcf >= 2 ? 2 : (c->step_kcf >= 1 ? 1 : 0));
     return FGPU_OK;
 }
 

@@ -235,19 +235,23 @@ int fgpu_autotune_step(fgpu_context *ctx, double dt, int nsteps, int *best_tpb, 
  * tracers per block, tracers per Thomas chunk, columns per block of the last
  * kernel as launched (a shape that needs another chunk size reports the one used). */
 int fgpu_step_shape(fgpu_context *ctx, int *tracers_per_block, int *thomas_chunk, int *columns_per_block);
-/* Which last kernel the plain step runs (no counterpart in the reference):
+/* Kernel variants of the step (no counterpart in the reference). kd_variant,
+ * the last kernel of the plain step:
  * 0 the block's waves in lock-step, the sweeps on the first wave's lanes;
  * 1 one wave per column plus a wave per block that sweeps the previous chunk
  *   while the column waves compute the next;
  * 2 one wave per column that sweeps its own column.
- * The autotuner picks it with the shape; FGPU_STEP_KD=lockstep|sweepwave|
- * wavelocal sets it when a context is created. Results are the same bit for bit. */
-int fgpu_step_variant(fgpu_context *ctx, int *kd_variant);
-/* Sets the launch shape and last-kernel variant instead of the autotuner (no
+ * kc_tracers, the bound kernel: 0 one column's levels over the lanes, 1, 2 or 4
+ * tracers per block with (tracer, level) items over the lanes.
+ * The autotuner picks both with the shape; FGPU_STEP_KD=lockstep|sweepwave|
+ * wavelocal and FGPU_STEP_KC=0|1|2|4 set them when a context is created.
+ * Results are the same bit for bit. */
+int fgpu_step_variant(fgpu_context *ctx, int *kd_variant, int *kc_tracers);
+/* Sets the launch shape and the kernel variants instead of the autotuner (no
  * counterpart in the reference); values outside the sets above fall back as
  * fgpu_step_shape reports. */
 int fgpu_set_step_shape(fgpu_context *ctx, int tracers_per_block, int thomas_chunk, int columns_per_block,
-                        int kd_variant);
+                        int kd_variant, int kc_tracers);
 /* Tracer content sum_n sum_k areasvol*hnode_new*T (deterministic device sum). */
 int fgpu_tracer_content(fgpu_context *ctx, int tr_num, double *content);
 

@@ -91,7 +91,7 @@ def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nste
         ctx.solve_tracers_ale(st.dt, nsteps)
         shape = ctx.step_shape()
         if kd is not None:
-            assert ctx.step_variant() == KD_ID[kd]
+            assert ctx.step_variants()[0] == KD_ID[kd]
         out = [ctx.tracer_download(i + 1) for i in range(st.ntr)]
     finally:
         ctx.close()
@@ -146,6 +146,20 @@ def test_fused_shapes_128_lanes(deep, monkeypatch, tpb, tch, cpb, kd):
     _compare(mesh, out, ref)
 
 
+@pytest.mark.parametrize("kc", [1, 2, 4])
+@pytest.mark.parametrize("which", ["chan", "deep"])
+def test_flat_bounds_kernel(chan, deep, monkeypatch, which, kc):
+    """K_C over (tracer, level) items (FGPU_STEP_KC: tracers per block), with
+    each K_D variant, bit for bit."""
+    mesh, st, ref = chan if which == "chan" else deep
+    for kd in KD:
+        print(f"[gpu] flat K_C {which} kc={kc} K_D {kd}", flush=True)
+        monkeypatch.setenv("FGPU_STEP_KC", str(kc))
+        out, _ = _run(mesh, st, monkeypatch, 12, 2, 2 if which == "deep" else 4, kd=kd)
+        _compare(mesh, out, ref)
+    monkeypatch.delenv("FGPU_STEP_KC", raising=False)
+
+
 @pytest.mark.parametrize("which", ["chan", "deep"])
 def test_staged_offsets(chan, deep, monkeypatch, which):
     mesh, st, ref = chan if which == "chan" else deep
@@ -188,6 +202,20 @@ def test_mixed_schemes_shapes(chan, deep, monkeypatch, which, shape):
     print(f"[gpu] mixed schemes {which} shape={shape}", flush=True)
     out, _ = _run(mesh, st, monkeypatch, *shape, schemes=MIXED)
     _compare(mesh, out, ref)
+
+
+@pytest.mark.parametrize("kd", ["lockstep", "sweepwave"])
+@pytest.mark.parametrize("which", ["chan", "deep"])
+def test_mixed_schemes_flat_bounds(chan, deep, monkeypatch, which, kd):
+    """The (tracer, level) K_C with tracers of different schemes in one block
+    (FCT and unlimited ones side by side in a chunk), before each K_D."""
+    mesh, st, _ = chan if which == "chan" else deep
+    ref = _oracle_steps(mesh, st, 2, schemes=MIXED)
+    monkeypatch.setenv("FGPU_STEP_KC", "4")
+    print(f"[gpu] mixed schemes {which} (tracer, level) K_C, K_D {kd}", flush=True)
+    out, _ = _run(mesh, st, monkeypatch, 12, 4, 2 if which == "deep" else 4, kd=kd, schemes=MIXED)
+    _compare(mesh, out, ref)
+    monkeypatch.delenv("FGPU_STEP_KC", raising=False)
 
 
 @pytest.mark.parametrize("path", ["fused", "staged"])
