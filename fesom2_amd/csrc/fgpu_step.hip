@@ -310,6 +310,196 @@ __global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(6, 8)))
     }
 }
 
+__device__ __forceinline__ double2 ld2(const double *p) { return *(const double2 *)p; }
+
+// store levels k (x) and k + 1 (y) of a row, those with their flag set: both
+// with one 16-byte store, one of them with an 8-byte store
+__device__ __forceinline__ void st2(double *p, double x, double y, bool a0, bool a1)
+{
+    if (a0 && a1) {
+        *(double2 *)p = make_double2(x, y);
+    } else if (a0) {
+        p[0] = x;
+    } else if (a1) {
+        p[1] = y;
+    }
+}
+
+// ------------------------------------------------------------------------------
+// K_A with two adjacent levels per lane: lane j takes levels 2j+1 and 2j+2, so
+// one wave covers an edge of up to 128 levels, and each per-level row is read
+// and written with 16-byte accesses (an even number of layers keeps every row
+// 16-byte aligned). A 16-byte load per lane costs the vector memory pipeline
+// what an 8-byte one does, so this halves its work against kA_edge<128>. Each
+// level's arithmetic is kA_edge's, in the same order.
+// ------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) kA2_edge(DevMesh m, DevDyn d, DevTracers t, int tpb)
+{
+    const int je = xcd_column(m.E2);
+    if (je >= m.E2) return;
+    const int e = m.edge_ord[je];
+    const int kb = 2 * (int)threadIdx.x + 1;   // this lane's levels: kb, kb + 1
+    const int L = m.L;
+    const int *ri = m.erec_i + (size_t)e * ERI;
+    const double *rd = m.erec_d + (size_t)e * ERD;
+    const int nl1 = ri[ER_NL1], nu1 = ri[ER_NU1], nl2 = ri[ER_NL2], nu2 = ri[ER_NU2];
+    const int nl12 = min(nl1, nl2), nu12 = max(nu1, nu2);
+    bool act[2], need2[2];
+    int s1[2], s2[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int k = kb + s;
+        const bool i0 = (k >= nu1 && k <= nu12 - 1);
+        const bool i1 = (nu2 > 0 && k >= nu2 && k <= nu12 - 1);
+        const bool i2 = (k >= nu12 && k <= nl12);
+        const bool i3 = (k >= nl12 + 1 && k <= nl1);
+        const bool i4 = (k >= nl12 + 1 && k <= nl2);
+        act[s] = i0 || i1 || i2 || i3 || i4;
+        need2[s] = i1 || i2 || i4;
+        // at most two segments meet at one level: A then B, or C, or D then E
+        s1[s] = i0 ? 0 : (i2 ? 2 : (i3 ? 3 : -1));
+        s2[s] = i1 ? 1 : (i4 ? 4 : -1);
+    }
+    if (!act[0] && !act[1]) return;
+    const int n1 = ri[ER_N1], n2 = ri[ER_N2], el1 = ri[ER_EL1], el2 = ri[ER_EL2];
+    const double dX1 = rd[ER_DX1], dY1 = rd[ER_DY1], dX2 = rd[ER_DX2], dY2 = rd[ER_DY2];
+    const double a = rd[ER_A], edx = rd[ER_EDX], edy = rd[ER_EDY];
+    double vf1[2], vf2[2];
+    {
+        const size_t q1 = (size_t)el1 * L + kb - 1;
+        const double2 uva = ld2(d.uv + 2 * q1), uvb = ld2(d.uv + 2 * q1 + 2), hh1 = ld2(m.helem + q1);
+        const double u1[2] = {uva.x, uvb.x}, v1[2] = {uva.y, uvb.y}, h1[2] = {hh1.x, hh1.y};
+        double u2[2] = {0.0, 0.0}, v2[2] = {0.0, 0.0}, h2[2] = {0.0, 0.0};
+        if (el2 >= 0 && ((act[0] && need2[0]) || (act[1] && need2[1]))) {
+            const size_t q2 = (size_t)el2 * L + kb - 1;
+            const double2 wa = ld2(d.uv + 2 * q2), wb = ld2(d.uv + 2 * q2 + 2), hh2 = ld2(m.helem + q2);
+            if (need2[0]) { u2[0] = wa.x; v2[0] = wa.y; h2[0] = hh2.x; }
+            if (need2[1]) { u2[1] = wb.x; v2[1] = wb.y; h2[1] = hh2.y; }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            vf1[s] = s1[s] >= 0 ? seg_vflux(s1[s], u1[s], v1[s], u2[s], v2[s], h1[s], h2[s], dX1, dY1, dX2, dY2) : 0.0;
+            vf2[s] = s2[s] >= 0 ? seg_vflux(s2[s], u1[s], v1[s], u2[s], v2[s], h1[s], h2[s], dX1, dY1, dX2, dY2) : 0.0;
+        }
+    }
+    // gradient rule of fill_up_dn_grad per level (oce_muscl_adv.F90:375-520)
+    bool shared[2], mean1[2], mean2[2];
+    const int u1l = ri[ER_U1], u2l = ri[ER_U2], nl1n = ri[ER_NL1N], nl2n = ri[ER_NL2N];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int k = kb + s;
+        shared[s] = false;
+        if (ri[ER_RULE]) {
+            const int nzmin = ri[ER_NZMIN], nzmax = ri[ER_NZMAX];
+            mean1[s] = (k >= u1l && k <= nzmin - 1);
+            mean2[s] = (k >= u2l && k <= nzmin - 1);
+            if (k >= nzmin && k <= nzmax - 1) { shared[s] = true; mean1[s] = false; mean2[s] = false; }
+            if (k >= nzmax && k <= nl1n - 1) { mean1[s] = true; shared[s] = false; }
+            if (k >= nzmax && k <= nl2n - 1) { mean2[s] = true; shared[s] = false; }
+        } else {
+            mean1[s] = (k >= u1l && k <= nl1n - 1);
+            mean2[s] = (k >= u2l && k <= nl2n - 1);
+        }
+    }
+    const bool any_sh = (act[0] && shared[0]) || (act[1] && shared[1]);
+    const bool any_m1 = (act[0] && !shared[0] && mean1[0]) || (act[1] && !shared[1] && mean1[1]);
+    const bool any_m2 = (act[0] && !shared[0] && mean2[0]) || (act[1] && !shared[1] && mean2[1]);
+    const int ua0 = ri[ER_UA0], ua1 = ri[ER_UA1], ua2 = ri[ER_UA2];
+    const int da0 = ri[ER_DA0], da1 = ri[ER_DA1], da2 = ri[ER_DA2];
+    const size_t o1 = (size_t)n1 * L + kb - 1, o2 = (size_t)n2 * L + kb - 1;
+    const size_t ei = (size_t)e * L + kb - 1;
+    const size_t kk = (size_t)kb - 1;
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    for (int tr = trb; tr < tre; ++tr) {
+        const double *T = t.T + t.sN * tr;
+        const double *Told = t.Told + t.sOld * tr;
+        const int abo = t.abo[tr], lim = t.lim[tr], hor = t.hor[tr];
+        const double2 t1 = ld2(T + o1), t2 = ld2(T + o2);
+        const double T1[2] = {t1.x, t1.y}, T2[2] = {t2.x, t2.y};
+        double flo[2] = {0.0, 0.0};
+        if (lim == LIM_FCT) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s1[s] >= 0) flo[s] = upw1_h(T1[s], T2[s], vf1[s], flo[s]);
+                if (s2[s] >= 0) flo[s] = upw1_h(T1[s], T2[s], vf2[s], flo[s]);
+            }
+            st2(t.lo_h + t.sE * tr + ei, flo[0], flo[1], act[0], act[1]);
+        }
+        double A1[2], A2[2];   // T^AB (oce_tracer_mod.F90:44-59)
+        if (abo == 2) {
+            const double2 p1 = ld2(Told + o1), p2 = ld2(Told + o2);
+            const double O1[2] = {p1.x, p1.y}, O2[2] = {p2.x, p2.y};
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                A1[s] = -(0.5 + FGPU_EPS_AB) * O1[s] + (1.5 + FGPU_EPS_AB) * T1[s];
+                A2[s] = -(0.5 + FGPU_EPS_AB) * O2[s] + (1.5 + FGPU_EPS_AB) * T2[s];
+            }
+        } else {
+            const double2 p1a = ld2(Told + 2 * o1), p1b = ld2(Told + 2 * o1 + 2);
+            const double2 p2a = ld2(Told + 2 * o2), p2b = ld2(Told + 2 * o2 + 2);
+            A1[0] = (5.0 * p1a.y - 16.0 * p1a.x + 23.0 * T1[0]) / 12.0;
+            A1[1] = (5.0 * p1b.y - 16.0 * p1b.x + 23.0 * T1[1]) / 12.0;
+            A2[0] = (5.0 * p2a.y - 16.0 * p2a.x + 23.0 * T2[0]) / 12.0;
+            A2[1] = (5.0 * p2b.y - 16.0 * p2b.x + 23.0 * T2[1]) / 12.0;
+        }
+        double f[2] = {flo[0], flo[1]};
+        if (hor != HOR_UPW1) {   // MFCT, MUSCL
+            const double no = t.ph[tr];
+            double2 ux0, ux1, ux2, wx0, wx1, wx2;
+            if (any_sh) {
+                ux0 = ld2(T + (size_t)ua0 * L + kk);
+                ux1 = ld2(T + (size_t)ua1 * L + kk);
+                ux2 = ld2(T + (size_t)ua2 * L + kk);
+                wx0 = ld2(T + (size_t)da0 * L + kk);
+                wx1 = ld2(T + (size_t)da1 * L + kk);
+                wx2 = ld2(T + (size_t)da2 * L + kk);
+            }
+            const double *gm = t.gmean + t.sG * tr;
+            double2 m1a, m1b, m2a, m2b;
+            if (any_m1) { m1a = ld2(gm + 2 * o1); m1b = ld2(gm + 2 * o1 + 2); }
+            if (any_m2) { m2a = ld2(gm + 2 * o2); m2b = ld2(gm + 2 * o2 + 2); }
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                double g[4] = {0.0, 0.0, 0.0, 0.0};
+                if (shared[s]) {
+                    const double u0 = s ? ux0.y : ux0.x, u1_ = s ? ux1.y : ux1.x, u2_ = s ? ux2.y : ux2.x;
+                    const double w0 = s ? wx0.y : wx0.x, w1 = s ? wx1.y : wx1.x, w2 = s ? wx2.y : wx2.x;
+                    g[0] = rd[ER_GU0] * u0 + rd[ER_GU1] * u1_ + rd[ER_GU2] * u2_;
+                    g[2] = rd[ER_GU3] * u0 + rd[ER_GU4] * u1_ + rd[ER_GU5] * u2_;
+                    g[1] = rd[ER_GD0] * w0 + rd[ER_GD1] * w1 + rd[ER_GD2] * w2;
+                    g[3] = rd[ER_GD3] * w0 + rd[ER_GD4] * w1 + rd[ER_GD5] * w2;
+                } else {
+                    if (mean1[s]) { const double2 q = s ? m1b : m1a; g[0] = q.x; g[2] = q.y; }
+                    if (mean2[s]) { const double2 q = s ? m2b : m2a; g[1] = q.x; g[3] = q.y; }
+                }
+                double Tmean2, Tmean1;
+                if (hor == HOR_MUSCL) {   // reconstruction only down to nboundary_lay (oce_adv_tra_hor.F90:356-367)
+                    const int k = kb + s;
+                    const double c1 = k <= ri[ER_NBL1] ? 1.0 : 0.0, c2 = k <= ri[ER_NBL2] ? 1.0 : 0.0;
+                    Tmean2 = A2[s] - (2.0 * (A2[s] - A1[s]) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0 * c2;
+                    Tmean1 = A1[s] + (2.0 * (A2[s] - A1[s]) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0 * c1;
+                } else {
+                    Tmean2 = A2[s] - (2.0 * (A2[s] - A1[s]) + edx * a * g[1] + edy * FGPU_R_EARTH * g[3]) / 6.0;
+                    Tmean1 = A1[s] + (2.0 * (A2[s] - A1[s]) + edx * a * g[0] + edy * FGPU_R_EARTH * g[2]) / 6.0;
+                }
+                for (int sg = 0; sg < 2; ++sg) {
+                    if ((sg == 0 ? s1[s] : s2[s]) < 0) continue;
+                    const double vflux = sg == 0 ? vf1[s] : vf2[s];
+                    const double cHO = (vflux + fabs(vflux)) * Tmean1 + (vflux - fabs(vflux)) * Tmean2;
+                    f[s] = -0.5 * (1.0 - no) * cHO - vflux * no * 0.5 * (Tmean1 + Tmean2) - f[s];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (s1[s] >= 0) f[s] = upw1_h(A1[s], A2[s], vf1[s], f[s]);
+                if (s2[s] >= 0) f[s] = upw1_h(A1[s], A2[s], vf2[s], f[s]);
+            }
+        }
+        st2(t.a_h + t.sE * tr + ei, f[0], f[1], act[0], act[1]);
+    }
+}
+
 // ------------------------------------------------------------------------------
 // column helpers on LDS copies (index = level - 1)
 // ------------------------------------------------------------------------------
@@ -1526,6 +1716,7 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
     const bool ws = c->d.wsplit != 0;   // W split: implicit upwind LO step, w_i rows of the diffusion
     const int xf = fgpu_kd_extras(c);
+    const bool ka2 = c->step_ka2 && m.L % 2 == 0;   // two levels per lane: 16-byte aligned rows only
     {   // explicit diffusion terms from T^n (no-op without Ki, Redi or explicit vertical diffusion)
         int rc = fgpu_launch_diff(c, 0, nt, dt);
         if (rc) return rc;
@@ -1538,7 +1729,10 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kA_edge");                                                                                \
-            hipLaunchKernelGGL(kA_edge<LBV>, dim3(xcd_grid(m.E2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, tpb);            \
+            if (ka2)                                                                                                   \
+                hipLaunchKernelGGL(kA2_edge, dim3(xcd_grid(m.E2), ny), dim3(64), 0, c->stream, m, c->d, c->t, tpb);    \
+            else                                                                                                       \
+                hipLaunchKernelGGL(kA_edge<LBV>, dim3(xcd_grid(m.E2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, tpb); \
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kB_node");                                                                                \

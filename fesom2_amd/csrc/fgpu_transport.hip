@@ -1188,6 +1188,27 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
         }
     }
     c->step_kcf = best_kcf;
+    // then K_A with two levels per lane, where the rows allow it
+    int best_ka2 = c->step_ka2;
+    if (!rc && best_t >= 0.f && c->m.L % 2 == 0) {
+        const int tried = c->step_ka2;
+        c->step_ka2 = !tried;
+        c->graph_valid = false;
+        if (!(rc = run_steps(c, dt, 1))) {
+            float ms = 0.f;
+            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord");
+            else if (!(rc = run_steps(c, dt, nsteps))) {
+                if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    rc = fgpu_fail(FGPU_ERR_HIP, "timing the autotuner trial failed");
+                else if (ms < best_t) {
+                    best_t = ms;
+                    best_ka2 = !tried;
+                }
+            }
+        }
+    }
+    c->step_ka2 = best_ka2;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     const int rr = restore();
@@ -1206,11 +1227,14 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
 }
 
 extern "C" int fgpu_set_step_shape(fgpu_context *c, int tracers_per_block, int thomas_chunk, int columns_per_block,
-                                   int kd_variant, int kc_tracers)
+                                   int kd_variant, int kc_tracers, int ka_levels_per_lane)
 {
     if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
-    if (tracers_per_block < 1 || kd_variant < KD_LOCKSTEP || kd_variant > KD_WAVELOCAL || kc_tracers < 0)
-        return fgpu_fail(FGPU_ERR_ARG, "tracers_per_block >= 1, kd_variant in 0..2 and kc_tracers >= 0");
+    if (tracers_per_block < 1 || kd_variant < KD_LOCKSTEP || kd_variant > KD_WAVELOCAL || kc_tracers < 0 ||
+        ka_levels_per_lane < 1 || ka_levels_per_lane > 2)
+        return fgpu_fail(FGPU_ERR_ARG,
+                         "tracers_per_block >= 1, kd_variant in 0..2, kc_tracers >= 0 and ka_levels_per_lane 1 or 2");
+    c->step_ka2 = ka_levels_per_lane == 2;
     c->step_tpb = tracers_per_block;
     c->step_tch = thomas_chunk;
     c->step_cpb = columns_per_block;
@@ -1220,9 +1244,10 @@ extern "C" int fgpu_set_step_shape(fgpu_context *c, int tracers_per_block, int t
     return FGPU_OK;
 }
 
-extern "C" int fgpu_step_variant(fgpu_context *c, int *kd_variant, int *kc_tracers)
+extern "C" int fgpu_step_variant(fgpu_context *c, int *kd_variant, int *kc_tracers, int *ka_levels_per_lane)
 {
-    if (!c || !kd_variant || !kc_tracers) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c || !kd_variant || !kc_tracers || !ka_levels_per_lane) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    *ka_levels_per_lane = (c->step_ka2 && c->m.L % 2 == 0) ? 2 : 1;
     *kd_variant = fgpu_kd_extras(c) ? KD_LOCKSTEP : c->step_kdp;
     *kc_tracers = c->step_kcf >= 4 ? 4 : (c->step_kcf >= 2 ? 2 : (c->step_kcf >= 1 ? 1 : 0));
     return FGPU_OK;

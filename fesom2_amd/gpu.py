@@ -81,8 +81,8 @@ _SIGS = {
     "fgpu_tracer_content": [vp, C.c_int, dp],
     "fgpu_autotune_step": [vp, C.c_double, C.c_int, ip, dp],
     "fgpu_step_shape": [vp, ip, ip, ip],
-    "fgpu_step_variant": [vp, ip, ip],
-    "fgpu_set_step_shape": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "fgpu_step_variant": [vp, ip, ip, ip],
+    "fgpu_set_step_shape": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "fgpu_init_stiff_mat_ale": [vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fgpu_update_stiff_mat_ale": [vp, dp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fgpu_ssh_set_matrix": [vp, C.c_int, C.c_int, ip, ip, dp],
@@ -395,20 +395,22 @@ class Context:
         _check(self.L.fgpu_step_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
-    def set_step_shape(self, tpb, tch, cpb, kd=0, kc=0):
+    def set_step_shape(self, tpb, tch, cpb, kd=0, kc=0, ka=1):
         """Fix (tracers per block, Thomas chunk, columns per block, last-kernel variant,
-        tracers per block of the (tracer, level) bound kernel, 0 for the per-column one)."""
-        _check(self.L.fgpu_set_step_shape(self.h, int(tpb), int(tch), int(cpb), int(kd), int(kc)))
+        tracers per block of the (tracer, level) bound kernel, 0 for the per-column one,
+        levels per lane of the edge-flux kernel, 1 or 2)."""
+        _check(self.L.fgpu_set_step_shape(self.h, int(tpb), int(tch), int(cpb), int(kd), int(kc), int(ka)))
 
     def step_variant(self):
         """Last kernel of the plain step: 0 lock-step, 1 with a sweep wave, 2 one wave per column."""
         return self.step_variants()[0]
 
     def step_variants(self):
-        """(last-kernel variant, tracers per block of the (tracer, level) bound kernel or 0)."""
-        v, kc = C.c_int(), C.c_int()
-        _check(self.L.fgpu_step_variant(self.h, C.byref(v), C.byref(kc)))
-        return v.value, kc.value
+        """(last-kernel variant, tracers per block of the (tracer, level) bound kernel or 0,
+        levels per lane of the edge-flux kernel)."""
+        v, kc, ka = C.c_int(), C.c_int(), C.c_int()
+        _check(self.L.fgpu_step_variant(self.h, C.byref(v), C.byref(kc), C.byref(ka)))
+        return v.value, kc.value, ka.value
 
     def tracer_content(self, tr_num):
         v = C.c_double()

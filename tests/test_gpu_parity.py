@@ -204,11 +204,13 @@ def test_scheme_variants_c0(c0, c0_oracle):
         ctx.close()
 
 
-@pytest.mark.parametrize("path", ["fused", "staged"])
+@pytest.mark.parametrize("path", ["fused", "fused_ka2", "staged"])
 def test_ab3_step_bitwise(c0, c0_oracle, monkeypatch, path):
     """Tracer 1 with AB_order 3 (valuesAB from two old levels, two-deep
     rotation of valuesold), tracer 2 with AB2: three steps equal the oracle's
-    bit for bit, both old levels included."""
+    bit for bit, both old levels included (fused_ka2: the edge-flux kernel
+    with two levels per lane, which reads both old levels of a lane's two
+    levels with two 16-byte loads)."""
     mesh, state, _ = c0
     wet = _wet(mesh)
     rng = np.random.default_rng(51)
@@ -217,9 +219,13 @@ def test_ab3_step_bitwise(c0, c0_oracle, monkeypatch, path):
     schemes = [dict(AB_order=3), dict()]
     if path == "staged":
         monkeypatch.setenv("FGPU_STEP", "staged")
+    if path == "fused_ka2":
+        monkeypatch.setenv("FGPU_STEP_KA", "2")
     print(f"[gpu] AB3 tracer, 3 {path} steps", flush=True)
     ctx = _gpu().context_for(mesh, state, schemes=schemes, valuesold=[vo3, state.valuesold[1]])
     try:
+        if path == "fused_ka2":
+            assert ctx.step_variants()[2] == 2
         ctx.solve_tracers_ale(state.dt, 3)
         out = [ctx.tracer_download(i + 1) for i in range(state.ntr)]
     finally:

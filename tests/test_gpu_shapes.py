@@ -75,9 +75,11 @@ def deep():
     return mesh, st, _oracle_steps(mesh, st, 2)
 
 
-def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nsteps=2, schemes=None, kd=None):
+def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nsteps=2, schemes=None, kd=None,
+         ka=None):
     from fesom2_amd import gpu
-    for k, v in (("FGPU_STEP_TPB", tpb), ("FGPU_STEP_TCH", tch), ("FGPU_STEP_CPB", cpb), ("FGPU_STEP_KD", kd)):
+    for k, v in (("FGPU_STEP_TPB", tpb), ("FGPU_STEP_TCH", tch), ("FGPU_STEP_CPB", cpb), ("FGPU_STEP_KD", kd),
+                 ("FGPU_STEP_KA", ka)):
         if v is None:
             monkeypatch.delenv(k, raising=False)
         else:
@@ -92,6 +94,8 @@ def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nste
         shape = ctx.step_shape()
         if kd is not None:
             assert ctx.step_variants()[0] == KD_ID[kd]
+        if ka is not None:
+            assert ctx.step_variants()[2] == ka
         out = [ctx.tracer_download(i + 1) for i in range(st.ntr)]
     finally:
         ctx.close()
@@ -160,6 +164,22 @@ def test_flat_bounds_kernel(chan, deep, monkeypatch, which, kc):
     monkeypatch.delenv("FGPU_STEP_KC", raising=False)
 
 
+@pytest.mark.parametrize("kd", KD)
+@pytest.mark.parametrize("which", ["chan", "deep"])
+def test_edge_kernel_two_levels(chan, deep, monkeypatch, which, kd):
+    """K_A with two adjacent levels per lane (FGPU_STEP_KA=2: 16-byte loads
+    and stores, one wave per edge; both meshes have an even number of
+    layers), before each K_D variant and both K_C variants, bit for bit."""
+    mesh, st, ref = chan if which == "chan" else deep
+    assert (mesh.nl - 1) % 2 == 0
+    for kc in (0, 4):
+        print(f"[gpu] two-level K_A {which} K_D {kd} kc={kc}", flush=True)
+        monkeypatch.setenv("FGPU_STEP_KC", str(kc))
+        out, _ = _run(mesh, st, monkeypatch, 12, 2, 2 if which == "deep" else 4, kd=kd, ka=2)
+        _compare(mesh, out, ref)
+    monkeypatch.delenv("FGPU_STEP_KC", raising=False)
+
+
 @pytest.mark.parametrize("which", ["chan", "deep"])
 def test_staged_offsets(chan, deep, monkeypatch, which):
     mesh, st, ref = chan if which == "chan" else deep
@@ -204,16 +224,18 @@ def test_mixed_schemes_shapes(chan, deep, monkeypatch, which, shape):
     _compare(mesh, out, ref)
 
 
+@pytest.mark.parametrize("ka", [1, 2])
 @pytest.mark.parametrize("kd", ["lockstep", "sweepwave"])
 @pytest.mark.parametrize("which", ["chan", "deep"])
-def test_mixed_schemes_flat_bounds(chan, deep, monkeypatch, which, kd):
+def test_mixed_schemes_flat_bounds(chan, deep, monkeypatch, which, kd, ka):
     """The (tracer, level) K_C with tracers of different schemes in one block
-    (FCT and unlimited ones side by side in a chunk), before each K_D."""
+    (FCT and unlimited ones side by side in a chunk), before each K_D, after
+    both K_A variants (UPW1, MFCT and MUSCL edges, with and without FCT)."""
     mesh, st, _ = chan if which == "chan" else deep
     ref = _oracle_steps(mesh, st, 2, schemes=MIXED)
     monkeypatch.setenv("FGPU_STEP_KC", "4")
-    print(f"[gpu] mixed schemes {which} (tracer, level) K_C, K_D {kd}", flush=True)
-    out, _ = _run(mesh, st, monkeypatch, 12, 4, 2 if which == "deep" else 4, kd=kd, schemes=MIXED)
+    print(f"[gpu] mixed schemes {which} (tracer, level) K_C, K_D {kd}, K_A {ka} levels per lane", flush=True)
+    out, _ = _run(mesh, st, monkeypatch, 12, 4, 2 if which == "deep" else 4, kd=kd, schemes=MIXED, ka=ka)
     _compare(mesh, out, ref)
     monkeypatch.delenv("FGPU_STEP_KC", raising=False)
 
