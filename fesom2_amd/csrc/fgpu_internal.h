@@ -152,6 +152,10 @@ struct ProfEntry {
 #define KD_LOCKSTEP 0    // kD_node: the block's waves in lock-step, sweeps on the first wave's lanes
 #define KD_SWEEPWAVE 1   // kDp_node<..., true>: one wave per column plus a sweep wave per block
 #define KD_WAVELOCAL 2   // kDp_node<..., false>: one wave per column, which sweeps its own column
+// kDp_node's LDS rows for columns of 65..KDP_SHORT_NL interfaces (70-layer
+// meshes): KDP_SHORT_NL + 2 doubles instead of 130, so that more blocks fit per
+// CU, and 3 or 4 columns per block
+#define KDP_SHORT_NL 72
 
 struct fgpu_context {
     int device = 0;

@@ -1264,11 +1264,15 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
 // kernel is fastest (0.76 ms, against 0.84 ms wave-local).
 
 
-template <int S, int TCH, int CPB, bool SW>
-__global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0))) kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt,
-                                                                      int clamp_tracer, int tpb)
+// A one-wave block of short rows is held to 96 VGPRs (5 waves per SIMD; it
+// spills 40 bytes per lane): 54.1 -> 52.8 ms at 1.3 M columns of 70 levels
+template <int S, int TCH, int CPB, bool SW, int RS = 64 * S + 2>
+__global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0)))
+__attribute__((amdgpu_waves_per_eu((!SW && CPB == 1 && S == 2 && RS <= KDP_SHORT_NL + 2) ? 5 : 1, 8)))
+kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb)
 {
-    constexpr int RS = 64 * S + 2;   // LDS row: levels 1..64 S, index k - 1, plus guards
+    // RS: LDS row of at least NL + 2 doubles (index k - 1 of levels and
+    // interfaces 1..NL, plus a guard); every LDS access is within k <= NL
     constexpr int NB = SW ? 2 : 1;   // chunk buffers
     __shared__ double sRpA[CPB][RS], sRmA[CPB][RS], sAvA[CPB][RS];
     __shared__ double saA[CPB][RS], sbA[CPB][RS], scA[CPB][RS], scpA[CPB][RS];
@@ -1637,6 +1641,9 @@ void fgpu_kd_shape(const fgpu_context *c, int *tch_out, int *cpb_out)
         if (cpb == 8) tch = 2;
         else if (cpb == 4) tch = tch == 4 ? 4 : 2;
         else cpb = 1;
+    } else if (c->step_kdp && cpb >= 3 && c->m.NL <= KDP_SHORT_NL) {
+        cpb = cpb >= 4 ? 4 : 3;
+        tch = 2;
     } else if (c->step_kdp && cpb >= 2) {
         cpb = 2;
         tch = tch == 4 ? 4 : 2;
@@ -1650,11 +1657,11 @@ void fgpu_kd_shape(const fgpu_context *c, int *tch_out, int *cpb_out)
 // K_D with one wave per column (S levels per lane: 1 up to 64 interfaces, 2 up
 // to 128), with a sweep wave per block (SW) or each column wave sweeping its
 // own column, for the Thomas chunk and columns per block of the launch shape
-template <int S, int TCH, int CPB, bool SW>
+template <int S, int TCH, int CPB, bool SW, int RS = 64 * S + 2>
 static void launch_kdp(fgpu_context *c, unsigned ny, double dt, int clamp, int tpb)
 {
     const DevMesh &m = c->m;
-    hipLaunchKernelGGL((kDp_node<S, TCH, CPB, SW>), dim3(xcd_grid((m.N2 + CPB - 1) / CPB), ny),
+    hipLaunchKernelGGL((kDp_node<S, TCH, CPB, SW, RS>), dim3(xcd_grid((m.N2 + CPB - 1) / CPB), ny),
                        dim3(64 * (CPB + (SW ? 1 : 0))), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);
 }
 
@@ -1668,6 +1675,15 @@ static void launch_kdp_shape(fgpu_context *c, unsigned ny, double dt, int clamp,
         else if (tch == 8) launch_kdp<1, 8, 1, SW>(c, ny, dt, clamp, tpb);
         else if (tch == 4) launch_kdp<1, 4, 1, SW>(c, ny, dt, clamp, tpb);
         else launch_kdp<1, 2, 1, SW>(c, ny, dt, clamp, tpb);
+    } else if (c->m.NL <= KDP_SHORT_NL) {   // rows of KDP_SHORT_NL + 2
+        constexpr int R = KDP_SHORT_NL + 2;
+        if (cpb >= 4) launch_kdp<2, 2, 4, SW, R>(c, ny, dt, clamp, tpb);
+        else if (cpb == 3) launch_kdp<2, 2, 3, SW, R>(c, ny, dt, clamp, tpb);
+        else if (cpb == 2 && tch == 4) launch_kdp<2, 4, 2, SW, R>(c, ny, dt, clamp, tpb);
+        else if (cpb == 2) launch_kdp<2, 2, 2, SW, R>(c, ny, dt, clamp, tpb);
+        else if (tch == 8) launch_kdp<2, 8, 1, SW, R>(c, ny, dt, clamp, tpb);
+        else if (tch == 4) launch_kdp<2, 4, 1, SW, R>(c, ny, dt, clamp, tpb);
+        else launch_kdp<2, 2, 1, SW, R>(c, ny, dt, clamp, tpb);
     } else {
         if (cpb >= 2 && tch == 4) launch_kdp<2, 4, 2, SW>(c, ny, dt, clamp, tpb);
         else if (cpb >= 2) launch_kdp<2, 2, 2, SW>(c, ny, dt, clamp, tpb);

@@ -1118,12 +1118,15 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     // the variants of the last kernel; with extra terms only the lock-step one exists
     std::vector<int> kds = {KD_LOCKSTEP};
     if (!fgpu_kd_extras(c)) kds = {KD_LOCKSTEP, KD_SWEEPWAVE, KD_WAVELOCAL};
+    // columns over 64 levels: several per block only for the one-wave-per-column
+    // K_D, up to 4 with the short LDS rows, else 2
+    const int deep_cpb = c->m.NL <= KDP_SHORT_NL ? 4 : 2;
     for (int kd : kds)
-        for (int cpb : {1, 2, 4, 8}) {
-            // two columns per block only for the one-wave-per-column K_D of columns over 64 levels
-            if (narrow ? cpb == 2 : cpb > (kd != KD_LOCKSTEP ? 2 : 1)) continue;
+        for (int cpb : {1, 2, 3, 4, 8}) {
+            if (narrow ? (cpb == 2 || cpb == 3) : cpb > (kd != KD_LOCKSTEP ? deep_cpb : 1)) continue;
             for (int tch : {2, 4, 8}) {
-                if ((tch == 4 && nt < 3) || (tch == 8 && nt < 5) || (tch >= 4 && cpb == 8) || (tch == 8 && cpb > 1))
+                if ((tch == 4 && nt < 3) || (tch == 8 && nt < 5) || (tch >= 4 && cpb == 8) || (tch == 8 && cpb > 1) ||
+                    (!narrow && tch >= 4 && cpb >= 3))
                     continue;
                 for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch, cpb, kd});
                 cand.push_back({nt, tch, cpb, kd});

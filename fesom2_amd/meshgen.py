@@ -855,6 +855,12 @@ CONFIGS: Dict[str, dict] = {
                              nlayers=70, hmax=6250.0, dz0=5.0, min_layers=5, max_layers=70,
                              topo="bumps", bump_height=6200.0),
                    state=dict(ntr=12, dt=600.0, kv_range=(1e-5, 1e-2)), steps=2),
+    # 30x24 channel with 90 layers: the last kernel's long LDS rows (columns of
+    # more than 72 interfaces)
+    "c0deep90": dict(mesh=dict(nx=30, ny=24, dx=20e3, jitter=0.2, seed=11, cartesian=True,
+                               nlayers=90, hmax=6250.0, dz0=5.0, min_layers=5, max_layers=90,
+                               topo="bumps", bump_height=6200.0),
+                     state=dict(ntr=12, dt=600.0, kv_range=(1e-5, 1e-2)), steps=2),
     # 126,858-node latitude band, 47 layers (PI-like stretching), T/S, dt=1800 s
     "c1": dict(mesh=dict(cartesian=False, target_nodes=126858, jitter=0.2, seed=1,
                          nlayers=47, hmax=6250.0, dz0=10.0, min_layers=5,

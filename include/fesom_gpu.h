@@ -226,7 +226,8 @@ int fgpu_profile_read(fgpu_context *ctx, int idx, char *name, int name_len, doub
 /* Autotuner of the fused step (no counterpart in the reference): times the
  * candidate launch shapes (tracers per block 1, 2, 4, ..., all, crossed with 2,
  * 4 or 8 tracers per Thomas chunk and 1, 4 or 8 columns per block of the last
- * kernel for columns of up to 64 levels, 1 or 2 for deeper ones) over nsteps
+ * kernel for columns of up to 64 levels, 1 to 4 for columns of up to 72
+ * interfaces, 1 or 2 for deeper ones) over nsteps
  * replayed steps, keeps the fastest
  * for this context and reports its tracers per block and time. The tracer
  * state is restored afterwards; results do not depend on the choice. */

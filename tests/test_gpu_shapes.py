@@ -3,12 +3,14 @@ the oracle (bit for bit).
 
 The autotuner picks tracers per block (1, 2, 4, 8 or all), tracers per Thomas
 chunk (2, 4 or 8) and columns per block of the last kernel (1, 4 or 8 for
-columns of at most 64 levels, 1 or 2 for deeper ones). Each choice
+columns of at most 64 levels; 1 to 4 for deeper ones of at most 72
+interfaces, whose LDS rows are short; 1 or 2 beyond). Each choice
 instantiates different kernels, so each is run here with twelve tracers, on
-the 24-layer channel (64-lane kernels, one level per lane in the last kernel)
-and on a 70-layer channel (128-lane kernels, two levels per lane in the last
-kernel), two steps each, with each variant of the last kernel
-(FGPU_STEP_KD: lock-step, with a sweep wave, one wave per column).
+the 24-layer channel (64-lane kernels, one level per lane in the last kernel),
+on a 70-layer channel (128-lane kernels, two levels per lane in the last
+kernel, short rows) and on a 90-layer one (long rows), two steps each, with
+each variant of the last kernel (FGPU_STEP_KD: lock-step, with a sweep wave,
+one wave per column).
 
 Two of the twelve tracers sit far from zero (about +2100 and -1500, as DIC
 or alkalinity do): the FCT bounds of a node next to an element without the
@@ -29,6 +31,7 @@ TCH = [2, 4, 8]
 CPB = [1, 4, 8]
 KD = ["lockstep", "sweepwave", "wavelocal"]   # variants of the last kernel (FGPU_STEP_KD)
 KD_ID = {"lockstep": 0, "sweepwave": 1, "wavelocal": 2}
+SHORT_NL = 72   # the last kernel's short LDS rows (3 or 4 columns per block) up to this many interfaces
 
 
 def _wet(mesh):
@@ -75,6 +78,13 @@ def deep():
     return mesh, st, _oracle_steps(mesh, st, 2)
 
 
+@pytest.fixture(scope="module")
+def deep90():
+    mesh, st = offset_state("c0deep90")
+    assert mesh.nl > SHORT_NL and (mesh.nlevels_nod2D - 1).min() < 20
+    return mesh, st, _oracle_steps(mesh, st, 2)
+
+
 def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nsteps=2, schemes=None, kd=None,
          ka=None):
     from fesom2_amd import gpu
@@ -110,17 +120,20 @@ def _compare(mesh, out, ref):
         np.testing.assert_array_equal(gvo[w], vo[i][w], err_msg=f"tracer {i + 1} valuesold")
 
 
-def _launched(tch, cpb, deep, kd):
+def _launched(tch, cpb, deep, kd, short=True):
     """The (Thomas chunk, columns per block) the last kernel runs with for a
     requested shape: several columns per block take chunks of 2, or 4 at four
-    64-lane columns or two deep columns; several deep columns need the kernel
-    with the sweep wave."""
+    64-lane columns or two deep columns; several deep columns need a
+    one-wave-per-column kernel, which takes 3 or 4 of them with the short LDS
+    rows (at most SHORT_NL interfaces), else 2."""
     if not deep:
         if cpb == 8:
             return 2, 8
         if cpb == 4:
             return (4 if tch == 4 else 2), 4
         return tch, 1
+    if kd != "lockstep" and cpb >= 3 and short:
+        return 2, min(cpb, 4)
     if kd != "lockstep" and cpb >= 2:
         return (4 if tch == 4 else 2), 2
     return tch, 1
@@ -139,14 +152,29 @@ def test_fused_shapes_64_lanes(chan, monkeypatch, tpb, tch, cpb, kd):
 
 
 @pytest.mark.parametrize("kd", KD)
-@pytest.mark.parametrize("cpb", [1, 2])
+@pytest.mark.parametrize("cpb", [1, 2, 3, 4])
 @pytest.mark.parametrize("tch", TCH)
 @pytest.mark.parametrize("tpb", TPB)
 def test_fused_shapes_128_lanes(deep, monkeypatch, tpb, tch, cpb, kd):
     mesh, st, ref = deep
+    assert mesh.nl <= SHORT_NL
     print(f"[gpu] 70 layers, 12 tracers, shape tpb={tpb} tch={tch} cpb={cpb} K_D {kd}", flush=True)
     out, shape = _run(mesh, st, monkeypatch, tpb, tch, cpb, kd=kd)
     assert shape == (tpb, *_launched(tch, cpb, True, kd))
+    _compare(mesh, out, ref)
+
+
+@pytest.mark.parametrize("kd", KD)
+@pytest.mark.parametrize("cpb", [1, 2, 4])
+@pytest.mark.parametrize("tch", TCH)
+@pytest.mark.parametrize("tpb", [1, 4, 12])
+def test_fused_shapes_long_rows(deep90, monkeypatch, tpb, tch, cpb, kd):
+    """90 layers: the last kernel's LDS rows for up to 128 levels (at most two
+    columns per block), and two-level K_A on an even layer count."""
+    mesh, st, ref = deep90
+    print(f"[gpu] 90 layers, 12 tracers, shape tpb={tpb} tch={tch} cpb={cpb} K_D {kd}", flush=True)
+    out, shape = _run(mesh, st, monkeypatch, tpb, tch, cpb, kd=kd, ka=2 if tpb == 12 else 1)
+    assert shape == (tpb, *_launched(tch, cpb, True, kd, short=False))
     _compare(mesh, out, ref)
 
 
