@@ -1372,9 +1372,13 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
         }
     };
 
+    // Without the sweep wave, the first sweep of a column writes its c' row
+    // (every lane of the column the same values), and the later chunks' sweeps
+    // reuse it as the sweep wave does
+    bool piv_ok = false;
+
     // Thomas sweeps of chunk q (oce_ale_tracer.F90:1406-1430), one lane per
-    // (column, tracer) of the columns wlo..wlo+nw-1 (without the sweep wave,
-    // every lane of a column writes the same c' row), then T^{n+1} of those
+    // (column, tracer) of the columns wlo..wlo+nw-1, then T^{n+1} of those
     // columns with the salinity clamp
     auto sweep_store = [&](int q, int wlo, int nw) {
         const int t0 = trb + q * TCH, nch = min(TCH, tre - t0), bf = SW ? (q & 1) : 0;
@@ -1386,7 +1390,7 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
                 double *tp_ = stpA[bf][wc][c], *ts = sTsA[bf][wc][c], *cp_ = scpA[wc];
                 double tp = tp_[z0 - 1] / b_[z0 - 1];
                 tp_[z0 - 1] = tp;
-                if constexpr (SW) {
+                if (SW || piv_ok) {
                     double cpa = cp_[z0 - 1];   // c' of the level above
                     for (int nz = z0 + 1; nz <= z1 - 1; ++nz) {
                         const double an = a_[nz - 1];
@@ -1452,7 +1456,7 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int k = kk_[s];
-                    if (fct && k <= L) {
+                    if (fct && act_[s]) {   // the limiters read R+- of the wet levels only
                         sRp[k - 1] = t.rp[t.sN * tr + ci_[s]];
                         sRm[k - 1] = t.rm[t.sN * tr + ci_[s]];
                     }
@@ -1565,6 +1569,7 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
             if (!SW) {   // this wave sweeps and stores its own column
                 wave_sync();
                 sweep_store(p, w, 1);
+                for (int c = 0; c < nch && !piv_ok; ++c) piv_ok = t.ivd[t0 + c] != 0;
             }
         } else if (sweeper && p >= 1) {
             sweep_store(p - 1, 0, CPB);
