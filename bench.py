@@ -175,11 +175,11 @@ def main():
                     help="plain-solver steps whose right-hand sides (every 4th) train the network")
     ap.add_argument("--learn-epochs", type=int, default=20)
     ap.add_argument("--shape", default=None,
-                    help="TPB,TCH,CPB[,KD[,KC[,KA]]]: fixed launch shape of the fused step (variant of its last "
-                         "kernel: 0 lock-step, 1 sweep wave, 2 one wave per column; tracers per block of the "
+                    help="TPB,TCH,CPB[,KD[,KC[,KA[,KB]]]]: fixed launch shape of the fused step (variant of its "
+                         "last kernel: 0 lock-step, 1 sweep wave, 2 one wave per column; tracers per block of the "
                          "(tracer, level) bound kernel, 0 for the per-column one; levels per lane of the "
-                         "edge-flux kernel, 1 or 2) instead of the autotuner's (a profiled run then holds only "
-                         "that shape's launches)")
+                         "edge-flux and of the LO kernel, 1 or 2) instead of the autotuner's (a profiled run then "
+                         "holds only that shape's launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path is design-only "
@@ -193,6 +193,7 @@ def main():
         os.environ["FGPU_STEP_KD"] = ["lockstep", "sweepwave", "wavelocal"][sv[3] if len(sv) > 3 else 0]
         os.environ["FGPU_STEP_KC"] = str(sv[4] if len(sv) > 4 else 0)
         os.environ["FGPU_STEP_KA"] = str(sv[5] if len(sv) > 5 else 1)
+        os.environ["FGPU_STEP_KB"] = str(sv[6] if len(sv) > 6 else 1)
     from fesom2_amd import gpu, meshgen as mg
     t0 = time.perf_counter()
     mesh, state, _ = mg.make_config(args.config)
@@ -211,7 +212,7 @@ def main():
         print(f"[bench] autotuner: {shape[0]} tracers per block, {shape[1]} per Thomas chunk, {shape[2]} columns "
               f"per block, last kernel variant {ctx.step_variants()} ({tune_ms:.3f} ms/step)", file=sys.stderr,
               flush=True)
-    ctx_variant, ctx_kc, ctx_ka = ctx.step_variants()
+    ctx_variant, ctx_kc, ctx_ka, ctx_kb = ctx.step_variants()
     ctx.solve_tracers_ale(dt, max(1, args.warmup))
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -332,7 +333,7 @@ def main():
         "autotuned_shape": {"tracers_per_block": shape[0], "thomas_chunk": shape[1], "columns_per_block": shape[2],
                             "last_kernel": ["lock-step", "sweep wave", "one wave per column"][ctx_variant],
                             "bound_kernel_tracers_per_block": ctx_kc,
-                            "edge_kernel_levels_per_lane": ctx_ka},
+                            "edge_kernel_levels_per_lane": ctx_ka, "lo_kernel_levels_per_lane": ctx_kb},
         "step_alg_GBs": round(step_bytes / (step_ms * 1e-3) / 1e9, 1),
         "step_model_bytes": int(step_model_bytes(mesh, state.ntr)),
         "step_model_frac_of_hbm": round(step_model_bytes(mesh, state.ntr) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),

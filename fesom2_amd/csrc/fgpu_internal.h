@@ -189,6 +189,7 @@ struct fgpu_context {
     int step_order = 1;       // processing order of the fused step's columns: 1 Hilbert, 0 index (FGPU_STEP_ORDER)
     int step_kcf = 0;         // K_C over (tracer, level) items with this many tracers per block (1, 2, 4); 0: kC_node
     int step_ka2 = 0;         // 1: K_A with two adjacent levels per lane (kA2_edge), for an even number of layers
+    int step_kb2 = 0;         // 1: K_B likewise (kB2_node), without the W split and CDIFF/PPM tracers
     int step_kdp = 0;         // last kernel of the plain step: KD_LOCKSTEP (kD_node), KD_SWEEPWAVE or KD_WAVELOCAL (kDp_node)
     int stage_slots = 0;      // tr_xy/aux slots allocated for the per-stage kernels (0: none yet)
     // profiling

@@ -154,6 +154,7 @@ extern "C" int fgpu_create(fgpu_context **out, int device)
     if (const char *ov = std::getenv("FGPU_STEP_ORDER")) c->step_order = std::string(ov) != "index";
     if (const char *fv = std::getenv("FGPU_STEP_KC")) c->step_kcf = std::atoi(fv);
     if (const char *av = std::getenv("FGPU_STEP_KA")) c->step_ka2 = std::atoi(av) == 2;
+    if (const char *bv = std::getenv("FGPU_STEP_KB")) c->step_kb2 = std::atoi(bv) == 2;
     if (const char *kv = std::getenv("FGPU_STEP_KD")) {
         const std::string v(kv);
         c->step_kdp = v == "sweepwave" ? KD_SWEEPWAVE : (v == "wavelocal" ? KD_WAVELOCAL : KD_LOCKSTEP);

@@ -703,6 +703,136 @@ __global__ void __launch_bounds__(LB) kB_node(DevMesh m, DevDyn d, DevTracers t,
 }
 
 // ------------------------------------------------------------------------------
+// K_B with two adjacent levels per lane: lane j takes levels and interfaces
+// 2j+1 and 2j+2, so one wave covers a column of up to 128 levels, and the
+// per-level rows (T, valuesold, Z, the thicknesses, the edges' LO fluxes and
+// fct_LO) move with 16-byte accesses (an even number of layers keeps those
+// rows 16-byte aligned). The interface rows (w, area, zbar, A_v) keep 8-byte
+// accesses: with an odd interface count every other column's row starts
+// 8 bytes off. Each level's arithmetic is kB_node's, in the same order. For
+// the step without the W split and without CDIFF or PPM tracers.
+// ------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) kB2_node(DevMesh m, DevDyn d, DevTracers t, double dt, int tpb)
+{
+    const int jc = xcd_column(m.N2);
+    if (jc >= m.N2) return;
+    const int n = m.node_ord[jc];
+    __shared__ double sT[130], sA[130], sW[130], sWe[130], sAr[130], sZ[130], sZb[130], sLo[130];
+    const int kb = 2 * (int)threadIdx.x + 1;   // this lane's levels and interfaces: kb, kb + 1
+    const int L = m.L, NL = m.NL;
+    const int nzmin = m.ulevn[n], nzmax = m.nlevn[n];
+    bool act[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) act[s] = kb + s >= nzmin && kb + s <= nzmax - 1;
+    const size_t ci = (size_t)n * L + kb - 1;   // even
+    const NodeEdges E = load_node_edges(m, n);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int k = kb + s;
+        if (k <= NL) {
+            const size_t ii = (size_t)n * NL + k - 1;
+            sW[k - 1] = d.w[ii];
+            sWe[k - 1] = d.w_e[ii];
+            sAr[k - 1] = m.area[ii];
+            sZb[k - 1] = m.zbar3d[ii];
+        }
+    }
+    if (kb <= L) {
+        const double2 z = ld2(m.Z3d + ci);
+        sZ[kb - 1] = z.x;
+        sZ[kb] = z.y;
+    }
+    double hn[2] = {1.0, 1.0}, hh[2] = {0.0, 0.0}, av_[2] = {1.0, 1.0};
+    if (act[0] || act[1]) {
+        const double2 a = ld2(m.hnode_new + ci), b = ld2(m.hnode + ci);
+        if (act[0]) { hn[0] = a.x; hh[0] = b.x; av_[0] = m.areasvol[(size_t)n * NL + kb - 1]; }
+        if (act[1]) { hn[1] = a.y; hh[1] = b.y; av_[1] = m.areasvol[(size_t)n * NL + kb]; }
+    }
+    const size_t kk = (size_t)min(kb, L - 1) - 1;   // even; the row's last pair for lanes past L
+    const int trb = blockIdx.y * tpb, tre = min(t.ntr, trb + tpb);
+    for (int tr = trb; tr < tre; ++tr) {
+        const double *T = t.T + t.sN * tr;
+        const double *Told = t.Told + t.sOld * tr;
+        const int abo = t.abo[tr], lim = t.lim[tr], ver = t.ver[tr];
+        const double pv = t.pv[tr];
+        __syncthreads();
+        if (kb <= L) {   // T and T^AB (oce_tracer_mod.F90:44-59) of both levels
+            const double2 tv = ld2(T + ci);
+            sT[kb - 1] = tv.x;
+            sT[kb] = tv.y;
+            if (abo == 2) {
+                const double2 o = ld2(Told + ci);
+                sA[kb - 1] = -(0.5 + FGPU_EPS_AB) * o.x + (1.5 + FGPU_EPS_AB) * tv.x;
+                sA[kb] = -(0.5 + FGPU_EPS_AB) * o.y + (1.5 + FGPU_EPS_AB) * tv.y;
+            } else {
+                const double2 oa = ld2(Told + 2 * ci), ob = ld2(Told + 2 * ci + 2);
+                sA[kb - 1] = (5.0 * oa.y - 16.0 * oa.x + 23.0 * tv.x) / 12.0;
+                sA[kb] = (5.0 * ob.y - 16.0 * ob.x + 23.0 * tv.y) / 12.0;
+            }
+        }
+        __syncthreads();
+        double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
+        const double *W = (lim == LIM_FCT) ? sW : sWe;
+        double lk[2] = {0.0, 0.0};
+        if (lim == LIM_FCT) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int k = kb + s;
+                if (k >= nzmin && k <= nzmax) lk[s] = col_upw1(k, nzmin, nzmax, sT, sWe, sAr, 0.0);
+                sLo[k - 1] = lk[s];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int k = kb + s;
+            if (k >= nzmin && k <= nzmax) {
+                double ho;
+                if (ver == VER_QR4C) ho = col_qr4c(k, nzmin, nzmax, sA, W, sAr, sZ, sZb, pv, lk[s]);
+                else ho = col_upw1(k, nzmin, nzmax, sA, W, sAr, lk[s]);
+                av[k - 1] = ho;
+            }
+        }
+        if (lim == LIM_FCT) {
+            const double *lo_h = t.lo_h + t.sE * tr;
+            double acc[2] = {0.0, 0.0};
+            // edges in batches of NG: the batch's loads are issued together,
+            // then added in ascending edge order
+            for (int j0 = 0; j0 < E.deg; j0 += NG) {
+                double2 fl_[NG];
+                int q_[NG], a0_[NG], a1_[NG];
+                bool real_[NG];
+#pragma unroll
+                for (int u = 0; u < NG; ++u) {
+                    real_[u] = j0 + u < E.deg;
+                    const int j = real_[u] ? j0 + u : 0;
+                    q_[u] = rl(E.q, j);
+                    a0_[u] = rl(E.nu12, j);
+                    a1_[u] = rl(E.nl12, j);
+                    fl_[u] = ld2(lo_h + (size_t)(q_[u] >> 1) * L + kk);
+                }
+#pragma unroll
+                for (int u = 0; u < NG; ++u) {
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const int k = kb + s;
+                        if (real_[u] && act[s] && k >= a0_[u] && k <= a1_[u]) {
+                            const double fl = s ? fl_[u].y : fl_[u].x;
+                            acc[s] = (q_[u] & 1) ? acc[s] - fl : acc[s] + fl;
+                        }
+                    }
+                }
+            }
+            double o[2] = {0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (act[s]) o[s] = (sT[kb - 1 + s] * hh[s] + (acc[s] + (lk[s] - sLo[kb + s])) * dt / av_[s]) / hn[s];
+            st2(t.fct_lo + t.sN * tr + ci, o[0], o[1], act[0], act[1]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
 // K_C: FCT bounds, P+/P-, R+/R- per node column (oce_adv_tra_fct.F90:119-405)
 // The element bound (max over the 3 nodes of max(LO, T^n)), maximised over the
 // elements around the node, equals the maximum over the node and those edge
@@ -1738,6 +1868,7 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
     const bool ws = c->d.wsplit != 0;   // W split: implicit upwind LO step, w_i rows of the diffusion
     const int xf = fgpu_kd_extras(c);
     const bool ka2 = c->step_ka2 && m.L % 2 == 0;   // two levels per lane: 16-byte aligned rows only
+    const bool kb2 = c->step_kb2 && m.L % 2 == 0 && !ws && !xv;
     {   // explicit diffusion terms from T^n (no-op without Ki, Redi or explicit vertical diffusion)
         int rc = fgpu_launch_diff(c, 0, nt, dt);
         if (rc) return rc;
@@ -1757,7 +1888,9 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
         }                                                                                                              \
         {                                                                                                              \
             ProfScope ps(c, "kB_node");                                                                                \
-            if (ws)                                                                                                    \
+            if (kb2)                                                                                                   \
+                hipLaunchKernelGGL(kB2_node, dim3(xcd_grid(m.N2), ny), dim3(64), 0, c->stream, m, c->d, c->t, dt, tpb); \
+            else if (ws)                                                                                               \
                 hipLaunchKernelGGL((kB_node<LBV, true, true>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \
             else if (xv)                                                                                               \
                 hipLaunchKernelGGL((kB_node<LBV, true, false>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, tpb); \

@@ -1191,27 +1191,25 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
         }
     }
     c->step_kcf = best_kcf;
-    // then K_A with two levels per lane, where the rows allow it
-    int best_ka2 = c->step_ka2;
-    if (!rc && best_t >= 0.f && c->m.L % 2 == 0) {
-        const int tried = c->step_ka2;
-        c->step_ka2 = !tried;
+    // then K_A and K_B with two levels per lane, where the rows allow it: each
+    // flag flipped once, kept if faster
+    for (int *flag : {&c->step_ka2, &c->step_kb2}) {
+        if (rc || best_t < 0.f || c->m.L % 2 != 0) break;
+        const int tried = *flag;
+        *flag = !tried;
         c->graph_valid = false;
-        if (!(rc = run_steps(c, dt, 1))) {
-            float ms = 0.f;
-            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord");
-            else if (!(rc = run_steps(c, dt, nsteps))) {
-                if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-                    rc = fgpu_fail(FGPU_ERR_HIP, "timing the autotuner trial failed");
-                else if (ms < best_t) {
-                    best_t = ms;
-                    best_ka2 = !tried;
-                }
-            }
+        float ms = 0.f;
+        if ((rc = run_steps(c, dt, 1))) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fgpu_fail(FGPU_ERR_HIP, "hipEventRecord"); break; }
+        if ((rc = run_steps(c, dt, nsteps))) break;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fgpu_fail(FGPU_ERR_HIP, "timing the autotuner trial failed");
+            break;
         }
+        if (ms < best_t) best_t = ms;
+        else *flag = tried;
     }
-    c->step_ka2 = best_ka2;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     const int rr = restore();
@@ -1230,14 +1228,15 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
 }
 
 extern "C" int fgpu_set_step_shape(fgpu_context *c, int tracers_per_block, int thomas_chunk, int columns_per_block,
-                                   int kd_variant, int kc_tracers, int ka_levels_per_lane)
+                                   int kd_variant, int kc_tracers, int ka_levels_per_lane, int kb_levels_per_lane)
 {
     if (!c) return fgpu_fail(FGPU_ERR_ARG, "null context");
     if (tracers_per_block < 1 || kd_variant < KD_LOCKSTEP || kd_variant > KD_WAVELOCAL || kc_tracers < 0 ||
-        ka_levels_per_lane < 1 || ka_levels_per_lane > 2)
-        return fgpu_fail(FGPU_ERR_ARG,
-                         "tracers_per_block >= 1, kd_variant in 0..2, kc_tracers >= 0 and ka_levels_per_lane 1 or 2");
+        ka_levels_per_lane < 1 || ka_levels_per_lane > 2 || kb_levels_per_lane < 1 || kb_levels_per_lane > 2)
+        return fgpu_fail(FGPU_ERR_ARG, "tracers_per_block >= 1, kd_variant in 0..2, kc_tracers >= 0 and "
+                                       "ka_levels_per_lane, kb_levels_per_lane 1 or 2");
     c->step_ka2 = ka_levels_per_lane == 2;
+    c->step_kb2 = kb_levels_per_lane == 2;
     c->step_tpb = tracers_per_block;
     c->step_tch = thomas_chunk;
     c->step_cpb = columns_per_block;
@@ -1247,10 +1246,15 @@ extern "C" int fgpu_set_step_shape(fgpu_context *c, int tracers_per_block, int t
     return FGPU_OK;
 }
 
-extern "C" int fgpu_step_variant(fgpu_context *c, int *kd_variant, int *kc_tracers, int *ka_levels_per_lane)
+extern "C" int fgpu_step_variant(fgpu_context *c, int *kd_variant, int *kc_tracers, int *ka_levels_per_lane,
+                                 int *kb_levels_per_lane)
 {
-    if (!c || !kd_variant || !kc_tracers || !ka_levels_per_lane) return fgpu_fail(FGPU_ERR_ARG, "null argument");
+    if (!c || !kd_variant || !kc_tracers || !ka_levels_per_lane || !kb_levels_per_lane)
+        return fgpu_fail(FGPU_ERR_ARG, "null argument");
     *ka_levels_per_lane = (c->step_ka2 && c->m.L % 2 == 0) ? 2 : 1;
+    bool xv = false;
+    for (int v : c->h_ver) xv = xv || v == VER_CDIFF || v == VER_PPM;
+    *kb_levels_per_lane = (c->step_kb2 && c->m.L % 2 == 0 && !c->d.wsplit && !xv) ? 2 : 1;
     *kd_variant = fgpu_kd_extras(c) ? KD_LOCKSTEP : c->step_kdp;
     *kc_tracers = c->step_kcf >= 4 ? 4 : (c->step_kcf >= 2 ? 2 : (c->step_kcf >= 1 ? 1 : 0));
     return FGPU_OK;

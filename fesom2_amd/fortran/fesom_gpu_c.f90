@@ -399,18 +399,18 @@ module fesom_gpu_c
      end function fgpu_ssh_solve_cg_noalloc
 
      integer(c_int) function fgpu_set_step_shape(ctx, tracers_per_block, thomas_chunk, columns_per_block, &
-          kd_variant, kc_tracers, ka_levels_per_lane) bind(C, name="fgpu_set_step_shape")
+          kd_variant, kc_tracers, ka_levels_per_lane, kb_levels_per_lane) bind(C, name="fgpu_set_step_shape")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
        integer(c_int), value :: tracers_per_block, thomas_chunk, columns_per_block, kd_variant, kc_tracers
-       integer(c_int), value :: ka_levels_per_lane
+       integer(c_int), value :: ka_levels_per_lane, kb_levels_per_lane
      end function fgpu_set_step_shape
 
-     integer(c_int) function fgpu_step_variant(ctx, kd_variant, kc_tracers, ka_levels_per_lane) &
-          bind(C, name="fgpu_step_variant")
+     integer(c_int) function fgpu_step_variant(ctx, kd_variant, kc_tracers, ka_levels_per_lane, &
+          kb_levels_per_lane) bind(C, name="fgpu_step_variant")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
-       integer(c_int), intent(out) :: kd_variant, kc_tracers, ka_levels_per_lane
+       integer(c_int), intent(out) :: kd_variant, kc_tracers, ka_levels_per_lane, kb_levels_per_lane
      end function fgpu_step_variant
 
      integer(c_int) function fgpu_autotune_step(ctx, dt, nsteps, best_tpb, best_ms) &

@@ -81,8 +81,8 @@ _SIGS = {
     "fgpu_tracer_content": [vp, C.c_int, dp],
     "fgpu_autotune_step": [vp, C.c_double, C.c_int, ip, dp],
     "fgpu_step_shape": [vp, ip, ip, ip],
-    "fgpu_step_variant": [vp, ip, ip, ip],
-    "fgpu_set_step_shape": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "fgpu_step_variant": [vp, ip, ip, ip, ip],
+    "fgpu_set_step_shape": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "fgpu_init_stiff_mat_ale": [vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fgpu_update_stiff_mat_ale": [vp, dp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fgpu_ssh_set_matrix": [vp, C.c_int, C.c_int, ip, ip, dp],
@@ -395,11 +395,11 @@ class Context:
         _check(self.L.fgpu_step_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
-    def set_step_shape(self, tpb, tch, cpb, kd=0, kc=0, ka=1):
+    def set_step_shape(self, tpb, tch, cpb, kd=0, kc=0, ka=1, kb=1):
         """Fix (tracers per block, Thomas chunk, columns per block, last-kernel variant,
         tracers per block of the (tracer, level) bound kernel, 0 for the per-column one,
-        levels per lane of the edge-flux kernel, 1 or 2)."""
-        _check(self.L.fgpu_set_step_shape(self.h, int(tpb), int(tch), int(cpb), int(kd), int(kc), int(ka)))
+        levels per lane of the edge-flux and of the LO kernel, 1 or 2)."""
+        _check(self.L.fgpu_set_step_shape(self.h, int(tpb), int(tch), int(cpb), int(kd), int(kc), int(ka), int(kb)))
 
     def step_variant(self):
         """Last kernel of the plain step: 0 lock-step, 1 with a sweep wave, 2 one wave per column."""
@@ -407,10 +407,10 @@ class Context:
 
     def step_variants(self):
         """(last-kernel variant, tracers per block of the (tracer, level) bound kernel or 0,
-        levels per lane of the edge-flux kernel)."""
-        v, kc, ka = C.c_int(), C.c_int(), C.c_int()
-        _check(self.L.fgpu_step_variant(self.h, C.byref(v), C.byref(kc), C.byref(ka)))
-        return v.value, kc.value, ka.value
+        levels per lane of the edge-flux kernel, levels per lane of the LO kernel)."""
+        v, kc, ka, kb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(self.L.fgpu_step_variant(self.h, C.byref(v), C.byref(kc), C.byref(ka), C.byref(kb)))
+        return v.value, kc.value, ka.value, kb.value
 
     def tracer_content(self, tr_num):
         v = C.c_double()

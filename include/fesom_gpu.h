@@ -247,15 +247,18 @@ int fgpu_step_shape(fgpu_context *ctx, int *tracers_per_block, int *thomas_chunk
  * ka_levels_per_lane, the edge-flux kernel: 1 one level per lane (64 or 128
  * lanes per edge), 2 two adjacent levels per lane with 16-byte accesses (one
  * wave per edge; meshes with an even number of layers only).
- * The autotuner picks all three with the shape; FGPU_STEP_KD=lockstep|sweepwave|
- * wavelocal, FGPU_STEP_KC=0|1|2|4 and FGPU_STEP_KA=1|2 set them when a context
- * is created. Results are the same bit for bit. */
-int fgpu_step_variant(fgpu_context *ctx, int *kd_variant, int *kc_tracers, int *ka_levels_per_lane);
+ * kb_levels_per_lane, the LO kernel, likewise (also not with the W split or
+ * CDIFF/PPM tracers).
+ * The autotuner picks all four with the shape; FGPU_STEP_KD=lockstep|sweepwave|
+ * wavelocal, FGPU_STEP_KC=0|1|2|4, FGPU_STEP_KA=1|2 and FGPU_STEP_KB=1|2 set
+ * them when a context is created. Results are the same bit for bit. */
+int fgpu_step_variant(fgpu_context *ctx, int *kd_variant, int *kc_tracers, int *ka_levels_per_lane,
+                      int *kb_levels_per_lane);
 /* Sets the launch shape and the kernel variants instead of the autotuner (no
  * counterpart in the reference); values outside the sets above fall back as
  * fgpu_step_shape reports. */
 int fgpu_set_step_shape(fgpu_context *ctx, int tracers_per_block, int thomas_chunk, int columns_per_block,
-                        int kd_variant, int kc_tracers, int ka_levels_per_lane);
+                        int kd_variant, int kc_tracers, int ka_levels_per_lane, int kb_levels_per_lane);
 /* Tracer content sum_n sum_k areasvol*hnode_new*T (deterministic device sum). */
 int fgpu_tracer_content(fgpu_context *ctx, int tr_num, double *content);
 

@@ -208,9 +208,9 @@ def test_scheme_variants_c0(c0, c0_oracle):
 def test_ab3_step_bitwise(c0, c0_oracle, monkeypatch, path):
     """Tracer 1 with AB_order 3 (valuesAB from two old levels, two-deep
     rotation of valuesold), tracer 2 with AB2: three steps equal the oracle's
-    bit for bit, both old levels included (fused_ka2: the edge-flux kernel
-    with two levels per lane, which reads both old levels of a lane's two
-    levels with two 16-byte loads)."""
+    bit for bit, both old levels included (fused_ka2: the edge-flux and LO
+    kernels with two levels per lane, which read both old levels of a lane's
+    two levels with two 16-byte loads)."""
     mesh, state, _ = c0
     wet = _wet(mesh)
     rng = np.random.default_rng(51)
@@ -221,11 +221,12 @@ def test_ab3_step_bitwise(c0, c0_oracle, monkeypatch, path):
         monkeypatch.setenv("FGPU_STEP", "staged")
     if path == "fused_ka2":
         monkeypatch.setenv("FGPU_STEP_KA", "2")
+        monkeypatch.setenv("FGPU_STEP_KB", "2")
     print(f"[gpu] AB3 tracer, 3 {path} steps", flush=True)
     ctx = _gpu().context_for(mesh, state, schemes=schemes, valuesold=[vo3, state.valuesold[1]])
     try:
         if path == "fused_ka2":
-            assert ctx.step_variants()[2] == 2
+            assert ctx.step_variants()[2:] == (2, 2)
         ctx.solve_tracers_ale(state.dt, 3)
         out = [ctx.tracer_download(i + 1) for i in range(state.ntr)]
     finally:

@@ -86,10 +86,10 @@ def deep90():
 
 
 def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nsteps=2, schemes=None, kd=None,
-         ka=None):
+         ka=None, kb=None):
     from fesom2_amd import gpu
     for k, v in (("FGPU_STEP_TPB", tpb), ("FGPU_STEP_TCH", tch), ("FGPU_STEP_CPB", cpb), ("FGPU_STEP_KD", kd),
-                 ("FGPU_STEP_KA", ka)):
+                 ("FGPU_STEP_KA", ka), ("FGPU_STEP_KB", kb)):
         if v is None:
             monkeypatch.delenv(k, raising=False)
         else:
@@ -106,6 +106,8 @@ def _run(mesh, st, monkeypatch, tpb=None, tch=None, cpb=None, staged=False, nste
             assert ctx.step_variants()[0] == KD_ID[kd]
         if ka is not None:
             assert ctx.step_variants()[2] == ka
+        if kb is not None:
+            assert ctx.step_variants()[3] == kb
         out = [ctx.tracer_download(i + 1) for i in range(st.ntr)]
     finally:
         ctx.close()
@@ -206,6 +208,31 @@ def test_edge_kernel_two_levels(chan, deep, monkeypatch, which, kd):
         out, _ = _run(mesh, st, monkeypatch, 12, 2, 2 if which == "deep" else 4, kd=kd, ka=2)
         _compare(mesh, out, ref)
     monkeypatch.delenv("FGPU_STEP_KC", raising=False)
+
+
+MIXED_QR4C_UPW1 = [dict(), dict(), dict(hor="UPW1", ver="UPW1", lim="FCT"), dict(hor="MFCT", ver="QR4C", lim="NON", ph=0.5, pv=0.0),
+                   dict(hor="MUSCL", ver="QR4C", lim="FCT"), dict(hor="UPW1", ver="UPW1", lim="NON"), dict(ph=0.5),
+                   dict(hor="MUSCL", ver="UPW1", lim="NON", ph=0.5), dict(pv=0.5), dict(), dict(hor="MFCT", ver="UPW1", lim="FCT"),
+                   dict(AB_order=2)]
+
+
+@pytest.mark.parametrize("kd", KD)
+@pytest.mark.parametrize("which", ["chan", "deep", "deep90"])
+def test_bound_kernels_two_levels(chan, deep, deep90, monkeypatch, which, kd):
+    """K_A and K_B with two adjacent levels per lane (FGPU_STEP_KA=2,
+    FGPU_STEP_KB=2: 16-byte accesses, one wave per column or edge), with the
+    default schemes and with tracers of different QR4C/UPW1 schemes side by
+    side, before each K_D variant, bit for bit."""
+    mesh, st, ref = {"chan": chan, "deep": deep, "deep90": deep90}[which]
+    assert (mesh.nl - 1) % 2 == 0
+    cpb = 4 if which == "chan" else 2
+    print(f"[gpu] two-level K_A and K_B {which} K_D {kd}", flush=True)
+    out, _ = _run(mesh, st, monkeypatch, 12, 2, cpb, kd=kd, ka=2, kb=2)
+    _compare(mesh, out, ref)
+    ref2 = _oracle_steps(mesh, st, 2, schemes=MIXED_QR4C_UPW1)
+    print(f"[gpu] two-level K_A and K_B {which} K_D {kd}, mixed QR4C/UPW1 schemes", flush=True)
+    out, _ = _run(mesh, st, monkeypatch, 4, 4, cpb, kd=kd, ka=2, kb=2, schemes=MIXED_QR4C_UPW1)
+    _compare(mesh, out, ref2)
 
 
 @pytest.mark.parametrize("which", ["chan", "deep"])
