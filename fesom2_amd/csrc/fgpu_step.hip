@@ -1767,7 +1767,7 @@ int fgpu_kd_extras(const fgpu_context *c)
 // 64 levels need the sweep-wave K_D.
 void fgpu_kd_shape(const fgpu_context *c, int *tch_out, int *cpb_out)
 {
-    int tch = (c->step_tch == 4 || c->step_tch == 8) ? c->step_tch : 2;
+    int tch = (c->step_tch == 4 || c->step_tch == 5 || c->step_tch == 8) ? c->step_tch : 2;
     int cpb = c->step_cpb;
     if (fgpu_kd_extras(c)) {   // the W split or explicit diffusion terms: one lock-step shape
         tch = 2;
@@ -1808,6 +1808,7 @@ static void launch_kdp_shape(fgpu_context *c, unsigned ny, double dt, int clamp,
         else if (cpb == 4 && tch == 4) launch_kdp<1, 4, 4, SW>(c, ny, dt, clamp, tpb);
         else if (cpb == 4) launch_kdp<1, 2, 4, SW>(c, ny, dt, clamp, tpb);
         else if (tch == 8) launch_kdp<1, 8, 1, SW>(c, ny, dt, clamp, tpb);
+        else if (tch == 5) launch_kdp<1, 5, 1, SW>(c, ny, dt, clamp, tpb);
         else if (tch == 4) launch_kdp<1, 4, 1, SW>(c, ny, dt, clamp, tpb);
         else launch_kdp<1, 2, 1, SW>(c, ny, dt, clamp, tpb);
     } else if (c->m.NL <= KDP_SHORT_NL) {   // rows of KDP_SHORT_NL + 2
@@ -1817,12 +1818,14 @@ static void launch_kdp_shape(fgpu_context *c, unsigned ny, double dt, int clamp,
         else if (cpb == 2 && tch == 4) launch_kdp<2, 4, 2, SW, R>(c, ny, dt, clamp, tpb);
         else if (cpb == 2) launch_kdp<2, 2, 2, SW, R>(c, ny, dt, clamp, tpb);
         else if (tch == 8) launch_kdp<2, 8, 1, SW, R>(c, ny, dt, clamp, tpb);
+        else if (tch == 5) launch_kdp<2, 5, 1, SW, R>(c, ny, dt, clamp, tpb);
         else if (tch == 4) launch_kdp<2, 4, 1, SW, R>(c, ny, dt, clamp, tpb);
         else launch_kdp<2, 2, 1, SW, R>(c, ny, dt, clamp, tpb);
     } else {
         if (cpb >= 2 && tch == 4) launch_kdp<2, 4, 2, SW>(c, ny, dt, clamp, tpb);
         else if (cpb >= 2) launch_kdp<2, 2, 2, SW>(c, ny, dt, clamp, tpb);
         else if (tch == 8) launch_kdp<2, 8, 1, SW>(c, ny, dt, clamp, tpb);
+        else if (tch == 5) launch_kdp<2, 5, 1, SW>(c, ny, dt, clamp, tpb);
         else if (tch == 4) launch_kdp<2, 4, 1, SW>(c, ny, dt, clamp, tpb);
         else launch_kdp<2, 2, 1, SW>(c, ny, dt, clamp, tpb);
     }
@@ -1927,6 +1930,8 @@ int fgpu_enqueue_fused_step(fgpu_context *c, double dt)
                                    dim3(LBV * (LBV == 64 ? 4 : 1)), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);       \
             else if (tch == 8)                                                                                         \
                 hipLaunchKernelGGL((kD_node<LBV, 8, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
+            else if (tch == 5)                                                                                         \
+                hipLaunchKernelGGL((kD_node<LBV, 5, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else if (tch == 4)                                                                                         \
                 hipLaunchKernelGGL((kD_node<LBV, 4, 1, 0>), dim3(xcd_grid(m.N2), ny), dim3(LBV), 0, c->stream, m, c->d, c->t, dt, clamp, tpb); \
             else                                                                                                       \

@@ -1124,8 +1124,10 @@ extern "C" int fgpu_autotune_step(fgpu_context *c, double dt, int nsteps, int *b
     for (int kd : kds)
         for (int cpb : {1, 2, 3, 4, 8}) {
             if (narrow ? (cpb == 2 || cpb == 3) : cpb > (kd != KD_LOCKSTEP ? deep_cpb : 1)) continue;
-            for (int tch : {2, 4, 8}) {
-                if ((tch == 4 && nt < 3) || (tch == 8 && nt < 5) || (tch >= 4 && cpb == 8) || (tch == 8 && cpb > 1) ||
+            for (int tch : {2, 4, 5, 8}) {
+                // chunks of 5 only with one column per block (elsewhere they run as 2)
+                if ((tch == 4 && nt < 3) || (tch == 5 && (nt < 5 || cpb > 1)) || (tch == 8 && nt < 5) ||
+                    (tch >= 4 && cpb == 8) || (tch == 8 && cpb > 1) ||
                     (!narrow && tch >= 4 && cpb >= 3))
                     continue;
                 for (int v = 1; v < nt; v *= 2) cand.push_back({v, tch, cpb, kd});

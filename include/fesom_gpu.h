@@ -225,7 +225,7 @@ int fgpu_profile_count(fgpu_context *ctx);
 int fgpu_profile_read(fgpu_context *ctx, int idx, char *name, int name_len, double *total_ms, int *launches);
 /* Autotuner of the fused step (no counterpart in the reference): times the
  * candidate launch shapes (tracers per block 1, 2, 4, ..., all, crossed with 2,
- * 4 or 8 tracers per Thomas chunk and 1, 4 or 8 columns per block of the last
+ * 4, 5 or 8 tracers per Thomas chunk and 1, 4 or 8 columns per block of the last
  * kernel for columns of up to 64 levels, 1 to 4 for columns of up to 72
  * interfaces, 1 or 2 for deeper ones) over nsteps
  * replayed steps, keeps the fastest

@@ -2,7 +2,7 @@
 the oracle (bit for bit).
 
 The autotuner picks tracers per block (1, 2, 4, 8 or all), tracers per Thomas
-chunk (2, 4 or 8) and columns per block of the last kernel (1, 4 or 8 for
+chunk (2, 4, 5 or 8) and columns per block of the last kernel (1, 4 or 8 for
 columns of at most 64 levels; 1 to 4 for deeper ones of at most 72
 interfaces, whose LDS rows are short; 1 or 2 beyond). Each choice
 instantiates different kernels, so each is run here with twelve tracers, on
@@ -27,7 +27,7 @@ from fesom2_amd import meshgen as mg
 pytestmark = pytest.mark.gpu
 
 TPB = [1, 2, 4, 8, 12]
-TCH = [2, 4, 8]
+TCH = [2, 4, 5, 8]
 CPB = [1, 4, 8]
 KD = ["lockstep", "sweepwave", "wavelocal"]   # variants of the last kernel (FGPU_STEP_KD)
 KD_ID = {"lockstep": 0, "sweepwave": 1, "wavelocal": 2}

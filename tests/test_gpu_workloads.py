@@ -71,7 +71,8 @@ def _properties(mesh, state, nsteps):
     ctx = gpu.context_for(mesh, state)
     try:
         tpb, ms = ctx.autotune_step(state.dt, 2)
-        print(f"[gpu] autotuned shape {ctx.step_shape()}, {ms:.2f} ms per step", flush=True)
+        print(f"[gpu] autotuned shape {ctx.step_shape()}, variants {ctx.step_variants()}, {ms:.2f} ms per step",
+              flush=True)
         c_before = [ctx.tracer_content(i + 1) for i in range(state.ntr)]
         ctx.solve_tracers_ale(state.dt, nsteps)
         c_after = [ctx.tracer_content(i + 1) for i in range(state.ntr)]
