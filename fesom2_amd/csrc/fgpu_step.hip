@@ -1394,9 +1394,34 @@ __global__ void __launch_bounds__(LB * CPB) __attribute__((amdgpu_waves_per_eu(6
 // kernel is fastest (0.76 ms, against 0.84 ms wave-local).
 
 
+// A column's values one level up and one level down the wave: lane j, slot s
+// holds level j + 1 + 64 s (the lane below, or across the slot boundary)
+template <int S>
+__device__ __forceinline__ void level_up(const double (&x)[S], double (&y)[S], int lane)
+{
+    y[0] = __shfl_up(x[0], 1);   // lane 0 gets its own value (level 0 does not exist)
+    if constexpr (S == 2) {
+        const double last0 = __shfl(x[0], 63);
+        const double u1 = __shfl_up(x[1], 1);
+        y[1] = lane == 0 ? last0 : u1;
+    }
+}
+template <int S>
+__device__ __forceinline__ void level_down(const double (&x)[S], double (&y)[S], int lane)
+{
+    const double d0 = __shfl_down(x[0], 1);
+    if constexpr (S == 2) {
+        const double first1 = __shfl(x[1], 0);
+        y[0] = lane == 63 ? first1 : d0;
+        y[1] = __shfl_down(x[1], 1);
+    } else {
+        y[0] = d0;
+    }
+}
+
 // A one-wave block of short rows is held to 96 VGPRs (5 waves per SIMD; it
 // spills 40 bytes per lane): 54.1 -> 52.8 ms at 1.3 M columns of 70 levels
-template <int S, int TCH, int CPB, bool SW, int RS = 64 * S + 2>
+template <int S, int TCH, int CPB, bool SW, int RS = 64 * S + 2, bool SH = false>
 __global__ void __launch_bounds__(64 * (CPB + (SW ? 1 : 0)))
 __attribute__((amdgpu_waves_per_eu((!SW && CPB == 1 && S == 2 && RS <= KDP_SHORT_NL + 2) ? 5 : 1, 8)))
 kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb)
@@ -1404,7 +1429,9 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
     // RS: LDS row of at least NL + 2 doubles (index k - 1 of levels and
     // interfaces 1..NL, plus a guard); every LDS access is within k <= NL
     constexpr int NB = SW ? 2 : 1;   // chunk buffers
-    __shared__ double sRpA[CPB][RS], sRmA[CPB][RS], sAvA[CPB][RS];
+    // SH: R+- and the limited vertical fluxes, read one level away only, move
+    // between lanes by shuffles instead of LDS rows
+    __shared__ double sRpA[SH ? 1 : CPB][SH ? 1 : RS], sRmA[SH ? 1 : CPB][SH ? 1 : RS], sAvA[SH ? 1 : CPB][SH ? 1 : RS];
     __shared__ double saA[CPB][RS], sbA[CPB][RS], scA[CPB][RS], scpA[CPB][RS];
     __shared__ double sTsA[NB][CPB][TCH][RS], stpA[NB][CPB][TCH][RS];
     __shared__ int sNz[CPB][2];
@@ -1421,7 +1448,8 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
     const bool valid = !sweeper && j0 < m.N2;
     const int n = m.node_ord[j0 < m.N2 ? j0 : m.N2 - 1];
     const int nzmin = valid ? m.ulevn[n] : 1, nzmax = valid ? m.nlevn[n] : 0;
-    double *sRp = sRpA[wc_], *sRm = sRmA[wc_], *sAv = sAvA[wc_], *sZ = sAvA[wc_];
+    double *sRp = sRpA[SH ? 0 : wc_], *sRm = sRmA[SH ? 0 : wc_], *sAv = sAvA[SH ? 0 : wc_];
+    double *sZ = SH ? sTsA[0][wc_][0] : sAvA[wc_];   // a row that is dead before the first chunk
     double *sa = saA[wc_], *sb = sbA[wc_], *sc = scA[wc_];
     int kk_[S];
     bool act_[S];
@@ -1582,43 +1610,61 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
                 const int abo = t.abo[tr];
                 const bool fct = t.lim[tr] == LIM_FCT;
                 const double *av = t.a_v + t.sNI * tr + (size_t)n * NL;
-                wave_sync();
+                double rp_[S], rm_[S], rpu_[S], rmu_[S], avk_[S], avd_[S];
+                if constexpr (SH) {
 #pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int k = kk_[s];
-                    if (fct && act_[s]) {   // the limiters read R+- of the wet levels only
-                        sRp[k - 1] = t.rp[t.sN * tr + ci_[s]];
-                        sRm[k - 1] = t.rm[t.sN * tr + ci_[s]];
+                    for (int s = 0; s < S; ++s) {
+                        rp_[s] = 1.0; rm_[s] = 1.0;
+                        if (fct && act_[s]) {
+                            rp_[s] = t.rp[t.sN * tr + ci_[s]];
+                            rm_[s] = t.rm[t.sN * tr + ci_[s]];
+                        }
                     }
+                    level_up<S>(rp_, rpu_, lane);
+                    level_up<S>(rm_, rmu_, lane);
+                } else {
+                    wave_sync();
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int k = kk_[s];
+                        if (fct && act_[s]) {   // the limiters read R+- of the wet levels only
+                            sRp[k - 1] = t.rp[t.sN * tr + ci_[s]];
+                            sRm[k - 1] = t.rm[t.sN * tr + ci_[s]];
+                        }
+                    }
+                    wave_sync();
                 }
-                wave_sync();
                 // limited vertical antidiffusive flux at interface k (oce_adv_tra_fct.F90:425-456)
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int k = kk_[s];
+                    avk_[s] = 0.0;
                     if (k >= nzmin && k <= nzmax) {
                         const double flux = av[k - 1];
                         double v = flux;
                         if (fct && k <= nzmax - 1) {
+                            const double rpk = SH ? rp_[s] : sRp[k - 1], rmk = SH ? rm_[s] : sRm[k - 1];
                             double ae = 1.0;
                             if (k == nzmin) {
-                                if (flux >= 0.0) ae = dmn(ae, sRp[k - 1]);
-                                else ae = dmn(ae, sRm[k - 1]);
+                                if (flux >= 0.0) ae = dmn(ae, rpk);
+                                else ae = dmn(ae, rmk);
                             } else {
                                 if (flux >= 0.0) {
-                                    ae = dmn(ae, sRm[k - 2]);
-                                    ae = dmn(ae, sRp[k - 1]);
+                                    ae = dmn(ae, SH ? rmu_[s] : sRm[k - 2]);
+                                    ae = dmn(ae, rpk);
                                 } else {
-                                    ae = dmn(ae, sRp[k - 2]);
-                                    ae = dmn(ae, sRm[k - 1]);
+                                    ae = dmn(ae, SH ? rpu_[s] : sRp[k - 2]);
+                                    ae = dmn(ae, rmk);
                                 }
                             }
                             v = ae * flux;
                         }
-                        sAv[k - 1] = v;
+                        if (SH) avk_[s] = v;
+                        else sAv[k - 1] = v;
                     }
                 }
-                wave_sync();
+                if constexpr (SH) level_down<S>(avk_, avd_, lane);
+                else wave_sync();
                 double Tk_[S], v_[S], h_[S], rpn_[S], rmn_[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
@@ -1627,10 +1673,10 @@ kDp_node(DevMesh m, DevDyn d, DevTracers t, double dt, int clamp_tracer, int tpb
                     if (act_[s]) {
                         Tk_[s] = T[ci_[s]];
                         if (fct) v_[s] = v_[s] - Tk_[s] * hh_[s] + t.fct_lo[t.sN * tr + ci_[s]] * hn_[s];
-                        v_[s] = v_[s] + (sAv[k - 1] - sAv[k]) * dt / av_[s];
+                        v_[s] = v_[s] + (SH ? avk_[s] - avd_[s] : sAv[k - 1] - sAv[k]) * dt / av_[s];
                     }
-                    rpn_[s] = (act_[s] && fct) ? sRp[k - 1] : 1.0;
-                    rmn_[s] = (act_[s] && fct) ? sRm[k - 1] : 1.0;
+                    rpn_[s] = (act_[s] && fct) ? (SH ? rp_[s] : sRp[k - 1]) : 1.0;
+                    rmn_[s] = (act_[s] && fct) ? (SH ? rm_[s] : sRm[k - 1]) : 1.0;
                 }
                 const double *ah = t.a_h + t.sE * tr;
                 for (int j = 0; j < E.deg; ++j) {
@@ -1792,11 +1838,11 @@ void fgpu_kd_shape(const fgpu_context *c, int *tch_out, int *cpb_out)
 // K_D with one wave per column (S levels per lane: 1 up to 64 interfaces, 2 up
 // to 128), with a sweep wave per block (SW) or each column wave sweeping its
 // own column, for the Thomas chunk and columns per block of the launch shape
-template <int S, int TCH, int CPB, bool SW, int RS = 64 * S + 2>
+template <int S, int TCH, int CPB, bool SW, int RS = 64 * S + 2, bool SH = false>
 static void launch_kdp(fgpu_context *c, unsigned ny, double dt, int clamp, int tpb)
 {
     const DevMesh &m = c->m;
-    hipLaunchKernelGGL((kDp_node<S, TCH, CPB, SW, RS>), dim3(xcd_grid((m.N2 + CPB - 1) / CPB), ny),
+    hipLaunchKernelGGL((kDp_node<S, TCH, CPB, SW, RS, SH>), dim3(xcd_grid((m.N2 + CPB - 1) / CPB), ny),
                        dim3(64 * (CPB + (SW ? 1 : 0))), 0, c->stream, m, c->d, c->t, dt, clamp, tpb);
 }
 
@@ -1812,15 +1858,18 @@ static void launch_kdp_shape(fgpu_context *c, unsigned ny, double dt, int clamp,
         else if (tch == 4) launch_kdp<1, 4, 1, SW>(c, ny, dt, clamp, tpb);
         else launch_kdp<1, 2, 1, SW>(c, ny, dt, clamp, tpb);
     } else if (c->m.NL <= KDP_SHORT_NL) {   // rows of KDP_SHORT_NL + 2
+        // one column per block: R+- and the vertical fluxes by shuffles, which
+        // frees their LDS rows (1.8 KB of a block's 10.1 KB at chunks of 5) for
+        // more waves; 46.4-47.0 -> 45.7-46.0 ms at 1.3 M columns of 70 levels
         constexpr int R = KDP_SHORT_NL + 2;
         if (cpb >= 4) launch_kdp<2, 2, 4, SW, R>(c, ny, dt, clamp, tpb);
         else if (cpb == 3) launch_kdp<2, 2, 3, SW, R>(c, ny, dt, clamp, tpb);
         else if (cpb == 2 && tch == 4) launch_kdp<2, 4, 2, SW, R>(c, ny, dt, clamp, tpb);
         else if (cpb == 2) launch_kdp<2, 2, 2, SW, R>(c, ny, dt, clamp, tpb);
-        else if (tch == 8) launch_kdp<2, 8, 1, SW, R>(c, ny, dt, clamp, tpb);
-        else if (tch == 5) launch_kdp<2, 5, 1, SW, R>(c, ny, dt, clamp, tpb);
-        else if (tch == 4) launch_kdp<2, 4, 1, SW, R>(c, ny, dt, clamp, tpb);
-        else launch_kdp<2, 2, 1, SW, R>(c, ny, dt, clamp, tpb);
+        else if (tch == 8) launch_kdp<2, 8, 1, SW, R, true>(c, ny, dt, clamp, tpb);
+        else if (tch == 5) launch_kdp<2, 5, 1, SW, R, true>(c, ny, dt, clamp, tpb);
+        else if (tch == 4) launch_kdp<2, 4, 1, SW, R, true>(c, ny, dt, clamp, tpb);
+        else launch_kdp<2, 2, 1, SW, R, true>(c, ny, dt, clamp, tpb);
     } else {
         if (cpb >= 2 && tch == 4) launch_kdp<2, 4, 2, SW>(c, ny, dt, clamp, tpb);
         else if (cpb >= 2) launch_kdp<2, 2, 2, SW>(c, ny, dt, clamp, tpb);
