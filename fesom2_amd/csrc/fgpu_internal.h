@@ -207,7 +207,7 @@ struct fgpu_context {
     double *ssh_val = nullptr, *ssh_pr = nullptr, *ssh_diag_inv = nullptr;
     double *ssh_x = nullptr, *ssh_x0 = nullptr, *ssh_rhs = nullptr;
     double *ssh_r = nullptr, *ssh_z = nullptr, *ssh_p = nullptr, *ssh_p1 = nullptr, *ssh_q = nullptr;
-    double *ssh_w1 = nullptr, *ssh_s1 = nullptr;
+    double *ssh_rec0 = nullptr, *ssh_rec1 = nullptr;   // single-reduction CG: 4 doubles per row {r, w, s, D^-1}, per parity
     bool ssh_single_reduction = true;   // diagonal preconditioner: one reduction (one kernel) per iteration
     hipGraphExec_t ssh_graph = nullptr;
     int ssh_graph_variant = -1, ssh_last_iters = 0;

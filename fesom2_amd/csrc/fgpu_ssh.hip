@@ -136,7 +136,9 @@ struct CGArgs {
     double *x;
     const double *rhs;
     double *r, *z, *p0, *p1, *q;
-    double *w1, *s1;   // second buffers of w and s (single-reduction iteration)
+    // single-reduction iteration: per row the record {r, w} {s_prev, D^-1}
+    // (two double2), one array per iteration parity
+    double2 *rec0, *rec1;
     double *part;   // 8 slots x nb: [0] b.b / p.q  [1] r.z  [2] r.r; single-reduction: [2p..2p+1], [4+p], [6]
     CGState *st;
 };
@@ -399,14 +401,16 @@ __device__ __forceinline__ double cg1_slot(const CGArgs &A, int slot)
     return x;
 }
 
-// r0 = b - A x0; partials b.b (slot 6) and r.r (slot 4); state reset
+// r0 = b - A x0 into the parity-0 records {r0, w0 (set next)} {s_-1 = 0, D^-1};
+// partials b.b (slot 6) and r.r (slot 4); state reset
 __global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A, double soltol, int maxiter)
 {
     double bb = 0.0, rr = 0.0, dum = 0.0;
     for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
         const int row = R.row;
         const double r = A.rhs[row] - row_dot(A.rowptr, A.colind, A.val, A.x, row);
-        A.r[row] = r;
+        A.rec0[2 * row] = make_double2(r, 0.0);
+        A.rec0[2 * row + 1] = make_double2(0.0, A.dinv[row]);
         bb += A.rhs[row] * A.rhs[row];
         rr += r * r;
     }
@@ -423,10 +427,12 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_r(CGArgs A, double soltol, in
     }
 }
 
-// w0 = A z0 with z0 = D^-1 r0; partials gamma0, delta0 (slots 0, 1); s_{-1} = p_{-1} = 0
+// w0 = A z0 with z0 = D^-1 r0 into the parity-0 records; partials gamma0,
+// delta0 (slots 0, 1); p_{-1} = 0
 __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
 {
     double g = 0.0, d = 0.0, dum = 0.0;
+    double *rc = (double *)A.rec0;   // row i: rc[4i] = r, rc[4i+1] = w, rc[4i+3] = D^-1
     for (CGRows R = cg_rows(A.n, A.nb); R.row < R.end; R.row += R.step) {
         const int row = R.row;
         double w = 0.0;
@@ -438,17 +444,17 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
                 const int p = pb + u < p1 ? pb + u : p0;
                 const int j = A.colind[p];
                 vv[u] = A.val[p];
-                zv[u] = A.dinv[j] * A.r[j];
+                zv[u] = rc[4 * (size_t)j + 3] * rc[4 * (size_t)j];
             }
 #pragma unroll
             for (int u = 0; u < SGB; ++u)
                 if (pb + u < p1) w = w + vv[u] * zv[u];
         }
-        const double z = A.dinv[row] * A.r[row];
-        A.q[row] = w;
-        A.p1[row] = 0.0;   // s_{-1}
+        const double r = rc[4 * (size_t)row];
+        const double z = rc[4 * (size_t)row + 3] * r;
+        rc[4 * (size_t)row + 1] = w;   // only this row's w word: other blocks read r and D^-1
         A.p0[row] = 0.0;   // p_{-1}
-        g += A.r[row] * z;
+        g += r * z;
         d += w * z;
     }
     block_sum3(g, d, dum);
@@ -458,7 +464,8 @@ __global__ void __launch_bounds__(SBLK) k_cg1_init_w(CGArgs A)
     }
 }
 
-// iteration i: r_i, w_i in buffers [i&1], s_{i-1} in [(i+1)&1]. par = i & 1
+// iteration i reads the records of parity i & 1, {r_i, w_i} {s_{i-1}, D^-1},
+// and writes {r_{i+1}, w_{i+1}} {s_i, D^-1} into the other parity's. par = i & 1
 // comes with the launch (every graph replay runs an even number of iterations
 // from an even one), and i and the previous scalars are read from the slots of
 // that parity, so a block dispatched after others have finished still reads
@@ -505,22 +512,20 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A, int par)
         return;
     }
     const double al = gam / den;
-    const double *rc = par ? A.z : A.r, *wc = par ? A.w1 : A.q, *sp = par ? A.s1 : A.p1;
-    double *rn = par ? A.r : A.z, *wn = par ? A.q : A.w1, *sc = par ? A.p1 : A.s1;
+    const double2 *rc = par ? A.rec1 : A.rec0;
+    double2 *rn = par ? A.rec0 : A.rec1;
     double g = 0.0, d = 0.0, r2 = 0.0;
     const CGRows R = cg_rows(A.n, A.nb);
     for (int row = R.row; row < R.end; row += R.step) {
-        const double s_ = wc[row] + be * sp[row];
-        const double p_ = A.dinv[row] * rc[row] + be * A.p0[row];
+        const double2 o0 = rc[2 * row], o1 = rc[2 * row + 1];   // {r, w}, {s_prev, D^-1}
+        const double s_ = o0.y + be * o1.x;
+        const double p_ = o1.y * o0.x + be * A.p0[row];
         A.p0[row] = p_;
         A.x[row] = A.x[row] + al * p_;
-        const double r_ = rc[row] - al * s_;
-        sc[row] = s_;
-        rn[row] = r_;
-        // w = A z in entry order; the column, value and neighbour loads of SGB
-        // entries are issued before any is added (one at a time, each entry
-        // waited out a full gather: 71 against 63 us per iteration on
-        // 1.3 million rows)
+        const double r_ = o0.x - al * s_;
+        // w = A z in entry order; the column, value and neighbour record loads
+        // of SGB entries are issued before any is added. Each neighbour's z
+        // comes from its record in two 16-byte loads.
         double w = 0.0;
         const int q0 = A.rowptr[row], q1 = A.rowptr[row + 1];
         for (int pb = q0; pb < q1; pb += SGB) {
@@ -530,14 +535,16 @@ __global__ void __launch_bounds__(SBLK) k_cg1(CGArgs A, int par)
                 const int p = pb + u < q1 ? pb + u : q0;
                 const int j = A.colind[p];
                 vv[u] = A.val[p];
-                zv[u] = A.dinv[j] * (rc[j] - al * (wc[j] + be * sp[j]));
+                const double2 a = rc[2 * j], b = rc[2 * j + 1];
+                zv[u] = b.y * (a.x - al * (a.y + be * b.x));
             }
 #pragma unroll
             for (int u = 0; u < SGB; ++u)
                 if (pb + u < q1) w = w + vv[u] * zv[u];
         }
-        wn[row] = w;
-        const double z_ = A.dinv[row] * r_;
+        rn[2 * row] = make_double2(r_, w);
+        rn[2 * row + 1] = make_double2(s_, o1.y);
+        const double z_ = o1.y * r_;
         g += r_ * z_;
         d += w * z_;
         r2 += r_ * r_;
@@ -573,9 +580,11 @@ static int ssh_alloc_vectors(fgpu_context *c)
     const size_t n = c->ssh_n;
     int rc;
     double **vs[] = {&c->ssh_x, &c->ssh_x0, &c->ssh_rhs, &c->ssh_r, &c->ssh_z, &c->ssh_p, &c->ssh_p1, &c->ssh_q,
-                     &c->ssh_diag_inv, &c->ssh_w1, &c->ssh_s1};
+                     &c->ssh_diag_inv};
     for (double **v : vs)
         if (!*v && (rc = fgpu_alloc(c, (void **)v, n * sizeof(double)))) return rc;
+    for (double **v : {&c->ssh_rec0, &c->ssh_rec1})
+        if (!*v && (rc = fgpu_alloc(c, (void **)v, 4 * n * sizeof(double)))) return rc;
     int blocks = (int)((n + SBLK - 1) / SBLK);
     if (blocks > 512) blocks = 512;
     c->ssh_blocks = blocks;
@@ -763,8 +772,8 @@ static CGArgs cg_args(fgpu_context *c)
     A.p0 = c->ssh_p;
     A.p1 = c->ssh_p1;
     A.q = c->ssh_q;
-    A.w1 = c->ssh_w1;
-    A.s1 = c->ssh_s1;
+    A.rec0 = (double2 *)c->ssh_rec0;
+    A.rec1 = (double2 *)c->ssh_rec1;
     A.part = c->ssh_part;
     A.st = (CGState *)c->ssh_out;
     return A;
@@ -968,8 +977,9 @@ extern "C" int fgpu_ssh_download(fgpu_context *c, double *x)
 }
 
 // ==============================================================================
-// ssh_solve_cg without allocation: the work vectors r, z, p0, p1, q, w1, s1, the
-// per-block partial sums and the CG state live in `scratch`, a static device
+// ssh_solve_cg without allocation: the work vectors (the two parities' records
+// {r, w} {s, D^-1} and p of the single-reduction iteration, or r, z, p0, p1
+// and q of the two-kernel one, sharing the area), the per-block partial sums and the CG state live in `scratch`, a static device
 // array of FGPU_NOALLOC_MAX_ROWS rows that every GPU the library runs on holds
 // until the process ends. x and rhs are the caller's device vectors. Kernels
 // are launched directly (no graph instantiation), eight iterations between
@@ -977,7 +987,7 @@ extern "C" int fgpu_ssh_download(fgpu_context *c, double *x)
 // parities as fgpu_ssh_solve_cg, so both give the same x bit for bit.
 // Calls that overlap on one GPU share `scratch` and race.
 // ==============================================================================
-#define NOALLOC_VECS 7
+#define NOALLOC_VECS 9   // doubles per row: two 4-double records and p
 #define NOALLOC_PART (8 * 512)
 static constexpr size_t NOALLOC_DOUBLES =
     (size_t)NOALLOC_VECS * FGPU_NOALLOC_MAX_ROWS + NOALLOC_PART + (sizeof(CGState) + 7) / 8;
@@ -1022,13 +1032,13 @@ extern "C" int fgpu_ssh_solve_cg_noalloc(fgpu_context *c, double *x_dev, const d
     const size_t M = FGPU_NOALLOC_MAX_ROWS;
     A.x = x_dev;
     A.rhs = rhs_dev;
-    A.r = s;
+    A.rec0 = (double2 *)s;
+    A.rec1 = (double2 *)(s + 4 * M);
+    A.r = s;   // the two-kernel iteration's vectors reuse the records' area
     A.z = s + M;
-    A.p0 = s + 2 * M;
-    A.p1 = s + 3 * M;
-    A.q = s + 4 * M;
-    A.w1 = s + 5 * M;
-    A.s1 = s + 6 * M;
+    A.p1 = s + 2 * M;
+    A.q = s + 3 * M;
+    A.p0 = s + 8 * M;
     A.part = s + NOALLOC_VECS * M;
     A.st = (CGState *)(s + NOALLOC_VECS * M + NOALLOC_PART);
     const dim3 g(A.nb), b(SBLK);

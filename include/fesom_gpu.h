@@ -301,7 +301,7 @@ int fgpu_ssh_download(fgpu_context *ctx, double *x);
  * harnesses that forbid allocation). x_dev (in/out) and rhs_dev are the
  * caller's DEVICE vectors of nod2D entries on the context's device; the work
  * vectors come from a static device area of FGPU_NOALLOC_MAX_ROWS rows (about
- * 84 MB on each GPU the library runs on, held until the process ends), so the
+ * 108 MB on each GPU the library runs on, held until the process ends), so the
  * call allocates nothing. Larger systems return FGPU_ERR_ARG. Same kernels and
  * iteration order as fgpu_ssh_solve_cg: the same x bit for bit. Calls that
  * overlap on one GPU share the static area and race. stats->ms is host time. */
